@@ -1,0 +1,676 @@
+"""CPU oracle for the continuum-dislocation-dynamics (CDD) stepper.
+
+TEST INFRASTRUCTURE ONLY.  This module is a NumPy float64 restatement of the
+reference Python path of mattbierbaum/cuda-plasticity, written to check the
+HIP implementation.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import it; the product path never
+does, and never falls back to it.
+
+Every function cites the reference file:line it restates (paths relative to
+the reference repository root).  Fields are NumPy arrays of shape
+``(9, N0, N1[, N2])``; component ``c = 3*i + j`` stands for (i, j) with
+x, y, z = 0, 1, 2 -- the sorted component order of
+``Plasticity/Fields/Fields.py:21,247-249``.
+
+Deliberate deviations (none change the arithmetic of the restated path):
+  * the reference is Python 2; integer ``/`` in shapes is written ``//``;
+  * ``Field`` dictionaries are replaced by the (9, ...) array;
+  * the reference mask forms of MinMod3/Max/Min are kept literally.
+
+Pinning status (tests/test_oracle_*.py): every function below is pinned by
+closed forms, identities or brute force, listed in each docstring.  The
+TVD-RK2 stepper has no reference counterpart (the reference only has RK3)
+and is pinned by its closed-form amplification factor only.
+"""
+import numpy as np
+
+# Plasticity/NumericalMethods.py:4
+ME = np.finfo(float).eps
+
+X, Y, Z = 0, 1, 2
+COMPS = [(i, j) for i in range(3) for j in range(3)]
+
+
+def cidx(i, j):
+    return 3 * i + j
+
+
+# Plasticity/Constants.py:16-22 (Levi-Civita symbol)
+PERM = np.zeros((3, 3, 3))
+PERM[0, 1, 2] = PERM[1, 2, 0] = PERM[2, 0, 1] = 1.0
+PERM[0, 2, 1] = PERM[2, 1, 0] = PERM[1, 0, 2] = -1.0
+
+
+def lame_lambda(mu, nu):
+    """lambda = 2 mu nu / (1 - 2 nu)  (Plasticity/PlasticityStates/PlasticityState.py:123)."""
+    return 2.0 * mu * nu / (1.0 - 2.0 * nu)
+
+
+# ---------------------------------------------------------------------------
+# k-space tools: Plasticity/GridArray/FourierSpaceTools.py:5-61
+# ---------------------------------------------------------------------------
+def _identity(a):
+    # FourierSpaceTools.py:5-7 -- theta - 2 pi [theta > pi]; Nyquist stays +pi
+    greater = (a > np.pi).astype(float)
+    return a - 2 * np.pi * greater
+
+
+class KTools:
+    """k-vector grids on the half spectrum (FourierSpaceTools.py:13-61).
+
+    2D uses ``gridShape[1]`` as the scale of both axes (:29-30); 3D scales
+    each axis by its own size (:47-49).  ``kz`` is zero in 2D (:31).
+    Pinned by: closed form k = 2 pi n_wrapped (tests/test_oracle_fft.py).
+    """
+
+    def __init__(self, shape):
+        shape = tuple(shape)
+        self.shape = shape
+        d = len(shape)
+        if d == 2:
+            n0, n1 = shape
+            ks = (n0, n1 // 2 + 1)
+            ix, iy = np.meshgrid(np.arange(ks[0]), np.arange(ks[1]), indexing="ij")
+            self.kx = n1 * _identity(2.0 * np.pi * ix / float(n0))
+            self.ky = n1 * _identity(2.0 * np.pi * iy / float(n1))
+            self.kz = np.zeros(ks)
+            self.kSq = self.kx * self.kx + self.ky * self.ky
+            self.kmask = 1.0 - ((ix == 0) * (iy == 0)).astype(float)
+        elif d == 3:
+            n0, n1, n2 = shape
+            ks = (n0, n1, n2 // 2 + 1)
+            ix, iy, iz = np.meshgrid(np.arange(ks[0]), np.arange(ks[1]),
+                                     np.arange(ks[2]), indexing="ij")
+            self.kx = n0 * _identity(2.0 * np.pi * ix / float(n0))
+            self.ky = n1 * _identity(2.0 * np.pi * iy / float(n1))
+            self.kz = n2 * _identity(2.0 * np.pi * iz / float(n2))
+            self.kSq = self.kx * self.kx + self.ky * self.ky + self.kz * self.kz
+            self.kmask = 1.0 - ((ix == 0) * (iy == 0) * (iz == 0)).astype(float)
+        else:
+            raise ValueError("only 2D and 3D grids are on the stepper path")
+        self.kSqSq = self.kSq * self.kSq
+        self.k = [self.kx, self.ky, self.kz]
+
+
+_KT = {}
+
+
+def ktools(shape):
+    """Per-shape cache (PlasticityState.py:15-16,35-39)."""
+    shape = tuple(shape)
+    if shape not in _KT:
+        _KT[shape] = KTools(shape)
+    return _KT[shape]
+
+
+# GridArray.py:40-44 (numpy.fft; irfftn with s=gridShape)
+def rfftn(a):
+    return np.fft.rfftn(a)
+
+
+def irfftn(a, shape):
+    return np.fft.irfftn(a, s=shape, axes=tuple(range(len(shape))))
+
+
+def fft_field(beta):
+    """Field.FFT: one rfftn per component (Fields.py:52-62)."""
+    return np.stack([rfftn(beta[c]) for c in range(beta.shape[0])])
+
+
+def ifft_field(kf, shape):
+    """Field.IFFT (Fields.py:64-74)."""
+    return np.stack([irfftn(kf[c], shape) for c in range(kf.shape[0])])
+
+
+# ---------------------------------------------------------------------------
+# Stress: PlasticityState.py:85-230, 357-390
+# ---------------------------------------------------------------------------
+def beta_elastic(beta, mu, nu):
+    """CalculateBetaElastic(type='k') (PlasticityState.py:357-390).
+
+    T_ijmn = lam/(lam+2mu) k_i k_j/K2 d_mn + k_i/K2 (k_m d_jn + k_n d_jm)
+             - 2(lam+mu)/(lam+2mu) k_i k_j k_m k_n / K4,
+    K2 = kSq + ME, K4 = kSqSq + ME; betaE = IFFT(kmask * T.betaP^) - betaP.
+    Pinned by: constant / compatible / antisymmetric / single-mode closed
+    forms and agreement with the 'betaP' (M-tensor) source.
+    """
+    shape = beta.shape[1:]
+    K = ktools(shape)
+    k = K.k
+    lamb = lame_lambda(mu, nu)
+    kb = fft_field(beta)
+    kSq = K.kSq + ME
+    kSqSq = K.kSqSq + ME
+    kbe = np.zeros_like(kb)
+    for i, j in COMPS:
+        acc = np.zeros(kb.shape[1:], dtype=complex)
+        for m, n in COMPS:
+            T = (lamb / (lamb + 2.0 * mu) * k[i] * k[j] / kSq * (m == n)
+                 + k[i] / kSq * ((k[m] * (j == n)) + (k[n] * (j == m)))
+                 - 2.0 * (lamb + mu) / (lamb + 2.0 * mu) * (k[i] * k[j] * k[m] * k[n]) / kSqSq)
+            acc = acc + kb[cidx(m, n)] * T
+        kbe[cidx(i, j)] = acc * K.kmask
+    return ifft_field(kbe, shape) - beta
+
+
+def sigma(beta, mu=1.0, nu=0.3):
+    """CalculateSigma(type='k', source='betaE') (PlasticityState.py:118-127).
+
+    sigma_ij = lam tr(betaE) d_ij + mu (betaE_ij + betaE_ji).
+    """
+    be = beta_elastic(beta, mu, nu)
+    lamb = lame_lambda(mu, nu)
+    tr = be[0] + be[4] + be[8]
+    out = np.zeros_like(beta)
+    for i, j in COMPS:
+        out[cidx(i, j)] = out[cidx(i, j)] + lamb * tr * (i == j) + mu * (be[cidx(i, j)] + be[cidx(j, i)])
+    return out
+
+
+def sigma_betaP_source(beta, mu=1.0, nu=0.3):
+    """CalculateSigma(type='k', source='betaP') (PlasticityState.py:151-182).
+
+    Independent formula (the M_ijmn tensor, also written out in
+    plasticity_kernel.cu:891-908) used to cross-check ``sigma``.
+    """
+    shape = beta.shape[1:]
+    K = ktools(shape)
+    k = K.k
+    kb = fft_field(beta)
+    kSq = K.kSq + ME
+    kSqSq = K.kSqSq + ME
+    out_k = np.zeros_like(kb)
+    for i, j in COMPS:
+        acc = np.zeros(kb.shape[1:], dtype=complex)
+        for m, n in COMPS:
+            M = ((2.0 * mu * nu / (1.0 - nu)) * ((k[m] * k[n] * (i == j) + k[i] * k[j] * (m == n)) / kSq - (i == j) * (m == n))
+                 - mu * ((i == m) * (j == n) + (i == n) * (j == m))
+                 - (2.0 * mu / (1.0 - nu)) * k[i] * k[j] * k[m] * k[n] / kSqSq
+                 + mu * (k[i] * k[n] * (j == m) + k[i] * k[m] * (j == n) + k[j] * k[n] * (i == m) + k[j] * k[m] * (i == n)) / kSq)
+            acc = acc + M * kb[cidx(m, n)]
+        out_k[cidx(i, j)] = acc * K.kmask
+    s = ifft_field(out_k, shape)
+    lamb = lame_lambda(mu, nu)
+    tr = beta[0].mean() + beta[4].mean() + beta[8].mean()
+    for i, j in COMPS:
+        s[cidx(i, j)] -= lamb * tr * (i == j) + mu * (beta[cidx(i, j)].mean() + beta[cidx(j, i)].mean())
+    return s
+
+
+def rho_fourier(beta):
+    """CalculateRhoFourier(type='k') (PlasticityState.py:264-304).
+
+    rho_ij = -eps_ilm d_l beta_mj via i k.  Pinned by rho(grad u) = 0,
+    div rho = 0 and the single-mode closed form.
+    """
+    shape = beta.shape[1:]
+    d = len(shape)
+    K = ktools(shape)
+    kb = fft_field(beta)
+    D = [kb * 1.0j * K.kx, kb * 1.0j * K.ky]
+    if d == 3:
+        D.append(kb * 1.0j * K.kz)
+    kr = np.zeros_like(kb)
+    for i in range(3):
+        if d == 2:
+            kr[cidx(X, i)] = -D[1][cidx(Z, i)]
+            kr[cidx(Y, i)] = D[0][cidx(Z, i)]
+            kr[cidx(Z, i)] = D[1][cidx(X, i)] - D[0][cidx(Y, i)]
+        else:
+            kr[cidx(X, i)] = D[2][cidx(Y, i)] - D[1][cidx(Z, i)]
+            kr[cidx(Y, i)] = D[0][cidx(Z, i)] - D[2][cidx(X, i)]
+            kr[cidx(Z, i)] = D[1][cidx(X, i)] - D[0][cidx(Y, i)]
+    return ifft_field(kr, shape)
+
+
+def elastic_energy_density(sig, mu=1.0, nu=0.3):
+    """CalculateElasticEnergy (PlasticityState.py:317-330).
+
+    e = 1/2 S_ijkm sigma_ij sigma_km,
+    S = (2 d_ik d_jm - d_im d_jk - nu/(1+nu) d_ij d_km) / (2 mu).
+    """
+    e = np.zeros(sig.shape[1:])
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                for m in range(3):
+                    S = (2.0 * (i == k) * (j == m) - (i == m) * (j == k) - nu * (i == j) * (k == m) / (1.0 + nu)) / (2.0 * mu)
+                    if S != 0.0:
+                        e = e + 0.5 * S * sig[cidx(i, j)] * sig[cidx(k, m)]
+    return e
+
+
+def elastic_energy(beta, mu=1.0, nu=0.3):
+    """Total energy = plain sum of the density (Observers/Observer.py:86,99)."""
+    return elastic_energy_density(sigma(beta, mu, nu), mu, nu).sum()
+
+
+# ---------------------------------------------------------------------------
+# Limiter and selections: NumericalMethods.py:352-497
+# ---------------------------------------------------------------------------
+def minmod3(a, b, c):
+    """MinMod3 arithmetic-mask form (NumericalMethods.py:367-378)."""
+    cond_a = (a > 0)
+    cond_b = (b > 0)
+    cond_c = (c > 0)
+    cond_ab = (a < b)
+    cond_bc = (b < c)
+    cond_ac = (a < c)
+    cond_abc = cond_a * cond_b * cond_c
+    cond_not_abc = (1 - cond_a) * (1 - cond_b) * (1 - cond_c)
+    return (cond_abc * (cond_ab * cond_ac * a + (1 - cond_ab) * cond_bc * b + (1 - cond_bc) * (1 - cond_ac) * c)
+            + cond_not_abc * ((1 - cond_ab) * (1 - cond_ac) * a + cond_ab * (1 - cond_bc) * b + cond_bc * cond_ac * c))
+
+
+def max3(a, b, c):
+    """Max3 (NumericalMethods.py:425-430)."""
+    cond_ab = (a > b)
+    cond_bc = (b > c)
+    cond_ac = (a > c)
+    return cond_ab * cond_ac * a + (1 - cond_ab) * cond_bc * b + (1 - cond_bc) * (1 - cond_ac) * c
+
+
+def min3(a, b, c):
+    """Min3 (NumericalMethods.py:462-467)."""
+    cond_ab = (a < b)
+    cond_bc = (b < c)
+    cond_ac = (a < c)
+    return cond_ab * cond_ac * a + (1 - cond_ab) * cond_bc * b + (1 - cond_bc) * (1 - cond_ac) * c
+
+
+def max5(a, b, c, d, e):
+    return max3(a, b, max3(c, d, e))  # NumericalMethods.py:432-433
+
+
+def min5(a, b, c, d, e):
+    return min3(a, b, min3(c, d, e))  # NumericalMethods.py:469-470
+
+
+def max9(a, b, c, d, e, f, g, h, i):
+    return max5(a, b, c, d, max5(e, f, g, h, i))  # :435-436
+
+
+def min9(a, b, c, d, e, f, g, h, i):
+    return min5(a, b, c, d, min5(e, f, g, h, i))  # :472-473
+
+
+THETA = 1.0  # FieldDynamics/CentralUpwindHJ3D.py:9
+
+
+def find_derivatives(field, axis, Dx=None):
+    """FindDerivatives (FieldDynamics/CentralUpwindHJ3D.py:11-34).
+
+    field: (ncomp, *grid); returns (deriv_p, deriv_m) with the same shape.
+    ``rollfield(f, n, dir)`` is ``numpy.roll(f, n, dir)`` per component
+    (NumericalMethods.py:352-356); the grid axis is ``axis + 1`` here.
+    Pinned by: exactness on quadratics/linears, extremum -> 0, and the
+    minmod table.
+    """
+    if Dx is None:
+        Dx = 1.0 / field.shape[-1]
+    ax = axis + 1
+    roll = np.roll
+    diff = roll(field, -1, ax) - field
+    diff2_1 = THETA * (roll(diff, -1, ax) - diff) / Dx / Dx
+    diff2_2 = (roll(diff, -1, ax) - roll(diff, 1, ax)) / Dx / Dx * 0.5
+    diff2_3 = roll(diff2_1, 1, ax)
+    diff2 = minmod3(diff2_1, diff2_2, diff2_3)
+    deriv_p = diff / Dx - 0.5 * Dx * diff2
+    deriv_m = roll(diff, 1, ax) / Dx + 0.5 * Dx * roll(diff2, 1, ax)
+    return deriv_p, deriv_m
+
+
+# ---------------------------------------------------------------------------
+# Velocities and Hamiltonians
+# ---------------------------------------------------------------------------
+def _rhomod(us):
+    """|rho| of one corner: BetaP{2,3}D_RhoMod
+    (FieldDynamics/CentralUpwindHJBetaPDynamics3D.py:34-58)."""
+    d = len(us)
+    rho2 = ME
+    for c in range(9):
+        for a in range(d):
+            rho2 = rho2 + us[a][c] * us[a][c]
+    for j in range(3):
+        rho2 = rho2 - us[0][cidx(X, j)] * us[0][cidx(X, j)]
+        rho2 = rho2 - us[1][cidx(Y, j)] * us[1][cidx(Y, j)]
+        if d == 3:
+            rho2 = rho2 - us[2][cidx(Z, j)] * us[2][cidx(Z, j)]
+        rho2 = rho2 - 2 * us[0][cidx(Y, j)] * us[1][cidx(X, j)]
+        if d == 3:
+            rho2 = rho2 - 2 * us[1][cidx(Z, j)] * us[2][cidx(Y, j)]
+            rho2 = rho2 - 2 * us[2][cidx(X, j)] * us[0][cidx(Z, j)]
+    return np.sqrt(rho2)
+
+
+def velocity_glide_climb(us, sig, Lambda):
+    """BetaP2D_Velocity / BetaP3D_Velocity with climb suppression Lambda
+    (CentralUpwindHJBetaPDynamics3D.py:77-142; 2D loops :83-100).
+
+    us: list over derivative axes of (9, *grid) one-sided derivatives.
+    Returns [v_x, v_y, v_z].  Pinned by the Levi-Civita form
+    v_l = sigma'_st (d_s beta_lt - d_l beta_st)/|rho| with
+    sigma' = sigma - Lambda tr(sigma)/3 I.
+    """
+    d = len(us)
+    v = [np.zeros(sig.shape[1:]) for _ in range(3)]
+    for n in range(3):
+        for l in range(3):
+            for a in range(d):
+                v[l] = v[l] + us[a][cidx(l, n)] * sig[cidx(a, n)]
+            for a in range(d):
+                v[a] = v[a] - us[a][cidx(l, n)] * sig[cidx(l, n)]
+    if Lambda != 0:
+        sigma_tr = (sig[0] + sig[4] + sig[8]) / 3.0 * Lambda
+        for a in range(d):
+            v[a] = v[a] + sigma_tr * (us[a][0] + us[a][4] + us[a][8])
+        for l in range(3):
+            acc = us[0][cidx(l, X)]
+            for a in range(1, d):
+                acc = acc + us[a][cidx(l, a)]
+            v[l] = v[l] - sigma_tr * acc
+    rm = _rhomod(us)
+    return [vl / rm for vl in v]
+
+
+def velocity_lvp(us, sig):
+    """Glide-only ('lvp') velocity (CentralUpwindHJBetaPGlideOnlyDynamics.py:
+    2D :69-128, 3D :131-202).  The code ADDS q (rhov/rhorho), the docstring
+    writes a minus; this restates the code.  Pinned by rhorho - ME = |q|^2
+    and the closed form of v'.q.
+    """
+    d = len(us)
+    ux, uy = us[0], us[1]
+    uz = us[2] if d == 3 else None
+    # climb velocity (no division): :78-91 / :140-155
+    v = [np.zeros(sig.shape[1:]) for _ in range(3)]
+    for n in range(3):
+        for l in range(3):
+            for a in range(d):
+                v[l] = v[l] + us[a][cidx(l, n)] * sig[cidx(a, n)]
+            for a in range(d):
+                v[a] = v[a] - us[a][cidx(l, n)] * sig[cidx(l, n)]
+    if d == 2:
+        rhorho = ME
+        uxTr = ux[0] + ux[4] + ux[8]
+        uyTr = uy[0] + uy[4] + uy[8]
+        rhorho = rhorho + uxTr * uxTr + uyTr * uyTr
+        rhorho = rhorho - 2 * uxTr * (ux[cidx(X, X)] + uy[cidx(X, Y)])
+        rhorho = rhorho - 2 * uyTr * (ux[cidx(Y, X)] + uy[cidx(Y, Y)])
+        for n in range(3):
+            t = ux[cidx(n, X)] + uy[cidx(n, Y)]
+            rhorho = rhorho + t * t
+        rhov = ME
+        rhov = rhov + v[0] * (ux[cidx(Z, Z)] + ux[cidx(Y, Y)] - uy[cidx(X, Y)])
+        rhov = rhov + v[1] * (uy[cidx(X, X)] + uy[cidx(Z, Z)] - ux[cidx(Y, X)])
+        rhov = rhov + -v[2] * (ux[cidx(Z, X)] + uy[cidx(Z, Y)])
+        v[0] = v[0] + (ux[cidx(Y, Y)] + ux[cidx(Z, Z)] - uy[cidx(X, Y)]) * rhov / rhorho
+        v[1] = v[1] + (uy[cidx(X, X)] + uy[cidx(Z, Z)] - ux[cidx(Y, X)]) * rhov / rhorho
+        v[2] = v[2] - (ux[cidx(Z, X)] + uy[cidx(Z, Y)]) * rhov / rhorho
+    else:
+        rhorho = ME
+        uxTr = ux[0] + ux[4] + ux[8]
+        uyTr = uy[0] + uy[4] + uy[8]
+        uzTr = uz[0] + uz[4] + uz[8]
+        rhorho = rhorho + uxTr * uxTr + uyTr * uyTr + uzTr * uzTr
+        rhorho = rhorho - 2 * uxTr * (ux[cidx(X, X)] + uy[cidx(X, Y)] + uz[cidx(X, Z)])
+        rhorho = rhorho - 2 * uyTr * (ux[cidx(Y, X)] + uy[cidx(Y, Y)] + uz[cidx(Y, Z)])
+        rhorho = rhorho - 2 * uzTr * (ux[cidx(Z, X)] + uy[cidx(Z, Y)] + uz[cidx(Z, Z)])
+        for n in range(3):
+            t = ux[cidx(n, X)] + uy[cidx(n, Y)] + uz[cidx(n, Z)]
+            rhorho = rhorho + t * t
+        rhov = ME
+        rhov = rhov + v[0] * (ux[cidx(Z, Z)] + ux[cidx(Y, Y)] - uy[cidx(X, Y)] - uz[cidx(X, Z)])
+        rhov = rhov + v[1] * (uy[cidx(X, X)] + uy[cidx(Z, Z)] - ux[cidx(Y, X)] - uz[cidx(Y, Z)])
+        rhov = rhov + v[2] * (uz[cidx(X, X)] + uz[cidx(Y, Y)] - ux[cidx(Z, X)] - uy[cidx(Z, Y)])
+        factor = rhov / rhorho
+        v[0] = v[0] + (ux[cidx(Y, Y)] + ux[cidx(Z, Z)] - uy[cidx(X, Y)] - uz[cidx(X, Z)]) * factor
+        v[1] = v[1] + (uy[cidx(X, X)] + uy[cidx(Z, Z)] - ux[cidx(Y, X)] - uz[cidx(Y, Z)]) * factor
+        v[2] = v[2] + (uz[cidx(X, X)] + uz[cidx(Y, Y)] - ux[cidx(Z, X)] - uy[cidx(Z, Y)]) * factor
+    rm = _rhomod(us)
+    return [vl / rm for vl in v]
+
+
+def hamiltonian(us, V, Lambda):
+    """H_2D / H_3D (CentralUpwindHJBetaPDynamics3D.py:161-201; glide-only
+    module :216-242 is the Lambda = 0 case).  Pinned by
+    H_ij = eps_ilm v_l rho_mj and tr(H') = (1 - Lambda) tr(H).
+    """
+    d = len(us)
+    H = us[0] * V[0]
+    for a in range(1, d):
+        H = H + us[a] * V[a]
+    H = H.copy()
+    for j in range(3):
+        for l in range(3):
+            for a in range(d):
+                H[cidx(a, j)] = H[cidx(a, j)] - V[l] * us[a][cidx(l, j)]
+    if Lambda != 0:
+        if d == 2:
+            ux, uy = us
+            J = -Lambda / 3.0 * (V[0] * (ux[cidx(Y, Y)] + ux[cidx(Z, Z)] - uy[cidx(X, Y)])
+                                 + V[1] * (uy[cidx(X, X)] + uy[cidx(Z, Z)] - ux[cidx(Y, X)])
+                                 - V[2] * (ux[cidx(Z, X)] + uy[cidx(Z, Y)]))
+        else:
+            ux, uy, uz = us
+            J = -Lambda / 3.0 * (V[0] * (ux[cidx(Y, Y)] + ux[cidx(Z, Z)] - uy[cidx(X, Y)] - uz[cidx(X, Z)])
+                                 + V[1] * (uy[cidx(X, X)] + uy[cidx(Z, Z)] - ux[cidx(Y, X)] - uz[cidx(Y, Z)])
+                                 + V[2] * (uz[cidx(X, X)] + uz[cidx(Y, Y)] - ux[cidx(Z, X)] - uy[cidx(Z, Y)]))
+        for i in range(3):
+            H[cidx(i, i)] = H[cidx(i, i)] + J
+    return H
+
+
+# ---------------------------------------------------------------------------
+# Central-upwind flux: CentralUpwindHJ3D.py:63-271
+# ---------------------------------------------------------------------------
+GLIDE_CLIMB = "glide_climb"
+GLIDE_ONLY = "lvp"
+
+
+def _velocity(dynamics, us, sig, Lambda):
+    if dynamics == GLIDE_ONLY:
+        return velocity_lvp(us, sig)
+    return velocity_glide_climb(us, sig, Lambda)
+
+
+def central_upwind_flux(beta, sig, dynamics=GLIDE_CLIMB, Lambda=0.0, Dx=None):
+    """CentralUpwindHJDynamics.CalculateFlux for the BetaP dynamics
+    (CentralUpwindHJ3D.py:99-188 (2D), :189-271 (3D)).  The glide-only
+    dynamics has no climb-suppression term (Lambda is ignored).
+
+    Returns (rhs, speed) with speed = a_tot + b_tot (+ c_tot).
+    Pinned by: constant field -> rhs = 0 and speed = d ME; translation
+    and 90-degree rotation equivariance; 3D z-invariant == 2D.
+    """
+    d = beta.ndim - 1
+    lam = 0.0 if dynamics == GLIDE_ONLY else Lambda
+    if d == 2:
+        dxp, dxm = find_derivatives(beta, 0, Dx)
+        dyp, dym = find_derivatives(beta, 1, Dx)
+        v_mm = _velocity(dynamics, [dxm, dym], sig, lam)
+        v_mp = _velocity(dynamics, [dxm, dyp], sig, lam)
+        v_pm = _velocity(dynamics, [dxp, dym], sig, lam)
+        v_pp = _velocity(dynamics, [dxp, dyp], sig, lam)
+        a_p = max5(v_mm[0], v_mp[0], v_pm[0], v_pp[0], 0.)
+        a_m = np.fabs(min5(v_mm[0], v_mp[0], v_pm[0], v_pp[0], 0.))
+        b_p = max5(v_mm[1], v_mp[1], v_pm[1], v_pp[1], 0.)
+        b_m = np.fabs(min5(v_mm[1], v_mp[1], v_pm[1], v_pp[1], 0.))
+        a_tot = a_p + a_m + ME
+        b_tot = b_p + b_m + ME
+        H_pp = hamiltonian([dxp, dyp], v_pp, lam)
+        H_pm = hamiltonian([dxp, dym], v_pm, lam)
+        H_mp = hamiltonian([dxm, dyp], v_mp, lam)
+        H_mm = hamiltonian([dxm, dym], v_mm, lam)
+        rhs = (-(H_pp * a_m * b_m + H_pm * a_m * b_p + H_mp * a_p * b_m + H_mm * a_p * b_p) / a_tot / b_tot
+               + ((dxp - dxm) * a_m * a_p / a_tot) + ((dyp - dym) * b_m * b_p / b_tot))
+        return rhs, a_tot + b_tot
+    dxp, dxm = find_derivatives(beta, 0, Dx)
+    dyp, dym = find_derivatives(beta, 1, Dx)
+    dzp, dzm = find_derivatives(beta, 2, Dx)
+    dx = {'+': dxp, '-': dxm}
+    dy = {'+': dyp, '-': dym}
+    dz = {'+': dzp, '-': dzm}
+    opts = ['---', '--+', '-+-', '-++', '+--', '+-+', '++-', '+++']
+    V = {o: _velocity(dynamics, [dx[o[0]], dy[o[1]], dz[o[2]]], sig, lam) for o in opts}
+    a_p = max9(*[V[o][0] for o in opts], 0.)
+    a_m = np.fabs(min9(*[V[o][0] for o in opts], 0.))
+    b_p = max9(*[V[o][1] for o in opts], 0.)
+    b_m = np.fabs(min9(*[V[o][1] for o in opts], 0.))
+    c_p = max9(*[V[o][2] for o in opts], 0.)
+    c_m = np.fabs(min9(*[V[o][2] for o in opts], 0.))
+    a_tot = a_p + a_m + ME
+    b_tot = b_p + b_m + ME
+    c_tot = c_p + c_m + ME
+    wa = {'+': a_m, '-': a_p}
+    wb = {'+': b_m, '-': b_p}
+    wc = {'+': c_m, '-': c_p}
+    # summation order of CentralUpwindHJ3D.py:243-258
+    deriv = None
+    for o in ['+++', '++-', '+-+', '+--', '-++', '-+-', '--+', '---']:
+        H = hamiltonian([dx[o[0]], dy[o[1]], dz[o[2]]], V[o], lam)
+        term = H * wa[o[0]] * wb[o[1]] * wc[o[2]]
+        deriv = -term if deriv is None else deriv - term
+    tempa = (dxp - dxm) * a_m * a_p / a_tot
+    tempb = (dyp - dym) * b_m * b_p / b_tot
+    tempc = (dzp - dzm) * c_m * c_p / c_tot
+    rhs = deriv / a_tot / b_tot / c_tot + tempa + tempb + tempc
+    return rhs, a_tot + b_tot + c_tot
+
+
+# ---------------------------------------------------------------------------
+# External loads (uniform stress shifts)
+# ---------------------------------------------------------------------------
+class Load:
+    """Uniform stress added by the dynamics' GetSigma override.
+
+    kind 'none'; 'strain' -- RunPython.py:21-26,44-49 (ExternalStrain with
+    strain(t) = initial + rate (t - t0)); 'stress' -- RunPython.py:28-32;
+    'uniaxial' -- RunLoadingMPI.py:68-83 (betaE_zz = rate t - (<beta_zz> -
+    init)); 'tensor' -- sigma_ij += s0_ij + rate_ij (t - t0) (the shear of
+    the adjoint workload, no reference counterpart).
+    """
+
+    def __init__(self, kind="none", initial=(0., 0., 0.), rate=(0., 0., 0.), t0=0.0,
+                 init_strain=0.0, uniaxial_rate=0.0, tensor0=None, tensor_rate=None):
+        self.kind = kind
+        self.initial = np.asarray(initial, dtype=float)
+        self.rate = np.asarray(rate, dtype=float)
+        self.t0 = t0
+        self.init_strain = init_strain
+        self.uniaxial_rate = uniaxial_rate
+        self.tensor0 = np.zeros(9) if tensor0 is None else np.asarray(tensor0, float).reshape(9)
+        self.tensor_rate = np.zeros(9) if tensor_rate is None else np.asarray(tensor_rate, float).reshape(9)
+
+    def apply(self, sig, beta, t, mu, nu):
+        lamb = lame_lambda(mu, nu)
+        if self.kind == "none":
+            return sig
+        sig = sig.copy()
+        if self.kind == "strain":
+            e = self.initial + self.rate * (t - self.t0)
+            tr = e[0] + e[1] + e[2]
+            for i in range(3):
+                sig[cidx(i, i)] += lamb * tr + 2.0 * mu * e[i]
+        elif self.kind == "stress":
+            s = self.initial + self.rate * (t - self.t0)
+            for i in range(3):
+                sig[cidx(i, i)] += s[i]
+        elif self.kind == "uniaxial":
+            bzz = self.uniaxial_rate * t - (np.average(beta[8]) - self.init_strain)
+            for i, j in COMPS:
+                sig[cidx(i, j)] += (lamb * (i == j) + 2.0 * mu * (i == Z) * (j == Z)) * bzz
+        elif self.kind == "tensor":
+            s = self.tensor0 + self.tensor_rate * (t - self.t0)
+            for c in range(9):
+                sig[c] += s[c]
+        else:
+            raise ValueError(self.kind)
+        return sig
+
+
+class Dynamics:
+    """BetaPDynamics.CalculateFlux = GetSigma + central-upwind flux
+    (CentralUpwindHJBetaPDynamics3D.py:214-221; glide-only :255-262)."""
+
+    def __init__(self, dynamics=GLIDE_CLIMB, Lambda=0.0, mu=1.0, nu=0.3, load=None):
+        self.dynamics = dynamics
+        self.Lambda = Lambda
+        self.mu = mu
+        self.nu = nu
+        self.load = load if load is not None else Load()
+
+    def get_sigma(self, beta, t):
+        s = sigma(beta, self.mu, self.nu)
+        return self.load.apply(s, beta, t, self.mu, self.nu)
+
+    def flux(self, t, beta):
+        return central_upwind_flux(beta, self.get_sigma(beta, t), self.dynamics, self.Lambda)
+
+
+# ---------------------------------------------------------------------------
+# Time stepping: FieldMovers/FieldMover.py:14-141
+# ---------------------------------------------------------------------------
+def cfl_dt(speed, shape, cfl):
+    """dt = CFL / (max(gridShape) * max speed) (FieldMover.py:94-96)."""
+    return cfl / (max(shape) * speed.max())
+
+
+def tvd_rk3_step(flux, t, u, t_end, cfl, fixed_dt=None):
+    """CalculateOneAdaptiveTimeStep (FieldMover.py:61-141) or, with fixed_dt,
+    CalculateOneFixedTimeStep (:38-58).  flux(t, u) -> (rhs, speed).
+    Every stage is evaluated at the step's start time (:123,133).
+    Returns (u_new, dt).  Pinned by the RK3 amplification factor.
+    """
+    rhs, speed = flux(t, u)
+    dt = fixed_dt if fixed_dt is not None else cfl_dt(speed, u.shape[1:], cfl)
+    if t + dt > t_end:
+        dt = t_end - t
+    L0 = dt * rhs
+    F0 = 1.0 * u + 1.0 * L0
+    F2 = 1.0 / 3.0 * u + 0. * L0 + 0. * F0
+    rhs, _ = flux(t, F0)
+    L1 = dt * rhs
+    F1 = 3.0 / 4.0 * u + 0. * L0 + 1.0 / 4.0 * F0 + 1.0 / 4.0 * L1
+    F2 = F2 + 0. * L1 + 2.0 / 3.0 * F1
+    rhs, _ = flux(t, F1)
+    L2 = dt * rhs
+    F2 = F2 + 2.0 / 3.0 * L2
+    return F2, dt
+
+
+def tvd_rk2_step(flux, t, u, t_end, cfl, fixed_dt=None):
+    """SSP (TVD) RK2 -- no reference counterpart; same CFL rule and stage-time
+    convention as the reference RK3 (FieldMover.py:94-111,123):
+    u1 = u + dt L(u);  u' = u/2 + (u1 + dt L(u1))/2.
+    Pinned by the closed-form amplification 1 + z + z^2/2.
+    """
+    rhs, speed = flux(t, u)
+    dt = fixed_dt if fixed_dt is not None else cfl_dt(speed, u.shape[1:], cfl)
+    if t + dt > t_end:
+        dt = t_end - t
+    u1 = u + dt * rhs
+    rhs, _ = flux(t, u1)
+    return 0.5 * u + 0.5 * (u1 + dt * rhs), dt
+
+
+def run(dyn, beta, t0, nsteps, cfl, order=3, t_end=np.inf, fixed_dt=None, observer=None):
+    """Mover.Run restricted to a fixed number of steps (FieldMover.py:20-36):
+    t += dt after each step, observers called after every step."""
+    step = tvd_rk3_step if order == 3 else tvd_rk2_step
+    t = t0
+    u = beta
+    dts = []
+
+    def f(tt, uu):
+        return dyn.flux(tt, uu)
+
+    for _ in range(nsteps):
+        u, dt = step(f, t, u, t_end, cfl, fixed_dt)
+        t += dt
+        dts.append(dt)
+        if observer is not None:
+            observer(t, u)
+        if t >= t_end:
+            break
+    return u, t, np.array(dts)
