@@ -33,7 +33,7 @@ def SpectralGaussianField(gridShape, lengthscale, rms=0.01, seed=0):
     out = np.empty((9,) + shape, dtype=np.float64)
     for c in range(9):
         z = rng.standard_normal(shape)
-        b = np.fft.irfftn(np.fft.rfftn(z) * filt, s=shape)
+        b = np.fft.irfftn(np.fft.rfftn(z) * filt, s=shape, axes=tuple(range(len(shape))))
         r = np.sqrt(np.mean(b * b))
         out[c] = b * (rms / r)
     return out

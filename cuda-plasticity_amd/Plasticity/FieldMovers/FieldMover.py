@@ -1,0 +1,64 @@
+"""TVD Runge-Kutta field mover on the GPU
+(Plasticity/FieldMovers/FieldMover.py:8-141).
+
+Run(state, observers, dynamics, startTime, endTime, fixedTimeStep) advances
+the state with dt = CFLsafeFactor / (max(gridShape) max speed) from the
+first-stage speeds, clamped to endTime, evaluating every stage at the step's
+start time, and calls every observer after every step -- the reference
+semantics.  The stepping itself (stress solves, fluxes, the CFL reduction
+and the stage combinations) is one device-resident sequence per step in the
+HIP library; when there are no observers, steps are launched in chunks and
+the host only polls the device clock between chunks.
+
+Like the reference, Run does not modify the state object it is given: the
+result reaches the caller through observers (RecallStateObserver) and is
+also kept in ``self.lastState``.
+"""
+from Plasticity.PlasticityStates.PlasticityState import PlasticityState
+from Plasticity.Fields.Fields import TensorField
+
+
+class FieldMover:
+    def Run(self, state, observers, dynamics, startTime, endTime, dT):
+        raise NotImplementedError
+
+
+class TVDRungeKutta_FieldMover(FieldMover):
+    """order=3: Shu-Osher TVD-RK3 (the reference); order=2: SSP-RK2."""
+
+    def __init__(self, CFLsafeFactor=0.1, dtBound=0.01, order=3, chunk=64):
+        self.CFLsafeFactor = CFLsafeFactor
+        self.dtBound = dtBound  # unused by the reference too (FieldMover.py:107-108)
+        self.order = order
+        self.chunk = chunk
+        self.lastState = None
+        self.numberOfSteps = 0
+
+    def _plan(self, state, dynamics):
+        return dynamics._plan(state, cfl=float(self.CFLsafeFactor), rk_order=int(self.order))
+
+    def Run(self, state, observers, dynamics=None, startTime=0., endTime=10., fixedTimeStep=None):
+        p = self._plan(state, dynamics)
+        p.set_state(state.betaP.data)
+        p.set_time(float(startTime), float(endTime), float(fixedTimeStep) if fixedTimeStep else 0.0)
+        steps0 = p.clock()[3]
+        if observers:
+            while True:
+                p.step(1)
+                t, dt, done, ns = p.clock()
+                cur = PlasticityState(state.gridShape, field=TensorField(state.gridShape, data=p.get_state()),
+                                      inherit=state)
+                for observer in observers:
+                    observer.Update(t, cur)
+                if done:
+                    break
+        else:
+            while True:
+                p.step(self.chunk)
+                t, dt, done, ns = p.clock()
+                if done:
+                    break
+        self.numberOfSteps = p.clock()[3] - steps0
+        self.lastState = PlasticityState(state.gridShape, field=TensorField(state.gridShape, data=p.get_state()),
+                                         inherit=state)
+        return self.lastState

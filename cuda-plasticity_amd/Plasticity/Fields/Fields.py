@@ -1,0 +1,104 @@
+"""Tensor fields backed by one device array.
+
+The reference ``Field`` is a dict of per-component NumPy arrays in sorted
+component order (Plasticity/Fields/Fields.py:7-291).  Here a TensorField is
+a view of a single (ncomp, *gridShape) float64 torch tensor on the GPU -- the
+layout the HIP kernels read -- with the same indexing: ``f['x', 'y']`` or
+``f[x, y]`` returns that component's (device) array, and ``.components``
+lists the sorted component keys.
+"""
+import numpy as np
+
+from Plasticity.Constants import x, y, z
+
+TENSOR_COMPONENTS = [(i, j) for i in (x, y, z) for j in (x, y, z)]
+VECTOR_COMPONENTS = [x, y, z]
+
+
+class Field(object):
+    def __init__(self, gridShape, components, data=None, device=None):
+        import torch
+        self.gridShape = tuple(gridShape)
+        self.components = sorted(components)
+        self._index = {c: k for k, c in enumerate(self.components)}
+        if data is None:
+            dev = device if device is not None else torch.device("cuda", 0)
+            data = torch.zeros((len(self.components),) + self.gridShape, dtype=torch.float64, device=dev)
+        self.data = data
+
+    def __getitem__(self, index):
+        return self.data[self._index[index]]
+
+    def __setitem__(self, index, value):
+        self.data[self._index[index]] = value
+
+    def GridShape(self):
+        return self.gridShape
+
+    def GridDimension(self):
+        return len(self.gridShape)
+
+    def numpy(self):
+        """Host copy as an array (ncomp, *gridShape)."""
+        return self.data.detach().cpu().numpy()
+
+    def numpy_arrays(self):
+        a = self.numpy()
+        return {c: a[k] for c, k in self._index.items()}
+
+    def _new(self, data):
+        return self.__class__(self.gridShape, components=self.components, data=data)
+
+    def _other(self, other):
+        return other.data if isinstance(other, Field) else other
+
+    def __add__(self, other):
+        return self._new(self.data + self._other(other))
+
+    __radd__ = __add__
+
+    def __sub__(self, other):
+        return self._new(self.data - self._other(other))
+
+    def __rsub__(self, other):
+        return self._new(self._other(other) - self.data)
+
+    def __mul__(self, other):
+        return self._new(self.data * self._other(other))
+
+    __rmul__ = __mul__
+
+    def __truediv__(self, other):
+        return self._new(self.data / self._other(other))
+
+    def __neg__(self):
+        return self._new(-self.data)
+
+    def max(self):
+        return float(self.data.max())
+
+    def min(self):
+        return float(self.data.min())
+
+    def sum(self):
+        return float(self.data.sum())
+
+    def modulus(self):
+        return (self.data * self.data).sum(dim=0).sqrt()
+
+
+class TensorField(Field):
+    """Second-rank tensor field, nine components xx xy xz yx yy yz zx zy zz
+    (Fields.py:235-251)."""
+
+    def __init__(self, gridShape, components=None, data=None, device=None, kspace=False):
+        if kspace:
+            raise NotImplementedError("k-space fields stay inside the HIP stress solve")
+        super(TensorField, self).__init__(gridShape, components or list(TENSOR_COMPONENTS), data, device)
+
+    @classmethod
+    def from_numpy(cls, arr, device=None):
+        import torch
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        dev = device if device is not None else torch.device("cuda", 0)
+        return cls(arr.shape[1:], data=torch.from_numpy(arr).to(dev))

@@ -1,0 +1,58 @@
+"""Per-step observers (Plasticity/Observers/Observer.py:7-104)."""
+import sys
+
+import numpy as np
+
+
+class Observer:
+    def __init__(self):
+        pass
+
+    def Update(self, time=None, state=None):
+        pass
+
+
+class RecallStateObserver(Observer):
+    """Keeps the latest state (:15-20)."""
+
+    def __init__(self):
+        self.state = None
+
+    def Update(self, time, state):
+        self.state = state
+
+
+class RecordStateObserverRaw(Observer):
+    """Appends [time, 9 components] as raw float64 (:51-69)."""
+
+    def __init__(self, filename=None, mode='wb'):
+        self.file = open(filename, mode) if filename is not None else sys.stdout.buffer
+
+    def Update(self, time, state):
+        self.file.write(np.array([time], dtype=np.float64).tobytes())
+        self.file.write(np.ascontiguousarray(state.GetOrderParameterField().numpy()).tobytes())
+        self.file.flush()
+
+
+class VerboseTimestepObserver(Observer):
+    def Update(self, time, state):
+        print("t = %f" % time)
+
+
+class EnergyDownhillObserver(Observer):
+    """Counts steps where the total elastic energy rose (:89-104)."""
+
+    def __init__(self, verbose=False):
+        self.PreviousEnergy = None
+        self.countTotal = 0
+        self.countBad = 0
+        self.verbose = verbose
+
+    def Update(self, time, state):
+        energy = state.TotalElasticEnergy()
+        self.countTotal += 1
+        if self.PreviousEnergy is not None and self.PreviousEnergy < energy:
+            self.countBad += 1
+        if self.verbose:
+            print("%d/%d" % (self.countBad, self.countTotal))
+        self.PreviousEnergy = energy

@@ -1,0 +1,218 @@
+"""ctypes binding of libcdd.so (include/cdd.h).
+
+The library is built in-tree by ``cuda-plasticity_amd/build_lib.py``; it is
+required -- importing this module without it raises, there is no fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libcdd.so")
+
+GLIDE_CLIMB, GLIDE_ONLY = 0, 1
+LOAD_NONE, LOAD_STRAIN, LOAD_STRESS, LOAD_UNIAXIAL, LOAD_TENSOR = 0, 1, 2, 3, 4
+
+
+class CDDError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int),
+        ("n", ctypes.c_int),
+        ("dynamics", ctypes.c_int),
+        ("rk_order", ctypes.c_int),
+        ("Lambda", ctypes.c_double),
+        ("mu", ctypes.c_double),
+        ("nu", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+        ("load_kind", ctypes.c_int),
+        ("device", ctypes.c_int),
+        ("load_init", ctypes.c_double * 9),
+        ("load_rate", ctypes.c_double * 9),
+        ("load_t0", ctypes.c_double),
+        ("load_init_strain", ctypes.c_double),
+        ("load_uniaxial_rate", ctypes.c_double),
+    ]
+
+
+EXPORTS = [
+    "cdd_plan_create", "cdd_plan_destroy", "cdd_set_stream", "cdd_set_state", "cdd_get_state",
+    "cdd_set_time", "cdd_get_clock", "cdd_sigma", "cdd_rho", "cdd_flux", "cdd_step", "cdd_energy",
+    "cdd_state_ptr", "cdd_last_error", "cdd_timing_enable", "cdd_timing_read",
+]
+KERNEL_KINDS = ["r2c", "cols_fwd", "contract", "cols_inv", "c2r", "flux_stage1", "cfl_reduce",
+                "flux_stage2", "flux_stage3", "advance"]
+
+
+def _load():
+    if not os.path.exists(LIBPATH):
+        raise CDDError("libcdd.so is not built (%s); run __graft_entry__.build()" % LIBPATH)
+    L = ctypes.CDLL(LIBPATH)
+    P = ctypes.c_void_p
+    D = ctypes.c_double
+    I = ctypes.c_int
+    dp = ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "cdd_plan_create": ([ctypes.POINTER(Config), ctypes.POINTER(P)], I),
+        "cdd_plan_destroy": ([P], I),
+        "cdd_set_stream": ([P, P], I),
+        "cdd_set_state": ([P, P, I], I),
+        "cdd_get_state": ([P, P, I], I),
+        "cdd_set_time": ([P, D, D, D], I),
+        "cdd_get_clock": ([P, dp, dp, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_longlong)], I),
+        "cdd_sigma": ([P, P, D, P], I),
+        "cdd_rho": ([P, P, P], I),
+        "cdd_flux": ([P, P, D, P, P], I),
+        "cdd_step": ([P, I], I),
+        "cdd_energy": ([P, P, D, dp], I),
+        "cdd_state_ptr": ([P, ctypes.POINTER(P)], I),
+        "cdd_last_error": ([], ctypes.c_char_p),
+        "cdd_timing_enable": ([P, I], I),
+        "cdd_timing_read": ([P, dp, ctypes.POINTER(I), I], I),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    return L
+
+
+lib = _load()
+
+
+def check(rc):
+    if rc != 0:
+        raise CDDError(lib.cdd_last_error().decode(errors="replace"))
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (must be contiguous float64 on the GPU)."""
+    if not t.is_cuda:
+        raise CDDError("expected a device tensor")
+    if t.dtype.__repr__() != "torch.float64":
+        raise CDDError("expected float64")
+    if not t.is_contiguous():
+        raise CDDError("expected a contiguous tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Plan:
+    """One stepper plan: owns the device state, scratch and clock."""
+
+    def __init__(self, ndim, n, dynamics=GLIDE_CLIMB, rk_order=3, Lambda=0.0, mu=1.0, nu=0.3, cfl=0.25,
+                 load_kind=LOAD_NONE, load_init=None, load_rate=None, load_t0=0.0, load_init_strain=0.0,
+                 load_uniaxial_rate=0.0, device=0):
+        import torch
+        self.torch = torch
+        cfg = Config()
+        cfg.ndim, cfg.n, cfg.dynamics, cfg.rk_order = ndim, n, dynamics, rk_order
+        cfg.Lambda, cfg.mu, cfg.nu, cfg.cfl = Lambda, mu, nu, cfl
+        cfg.load_kind, cfg.device = load_kind, device
+        li = np.zeros(9) if load_init is None else np.asarray(load_init, float)
+        lr = np.zeros(9) if load_rate is None else np.asarray(load_rate, float)
+        if li.size == 3:
+            li = np.concatenate([li, np.zeros(6)])
+        if lr.size == 3:
+            lr = np.concatenate([lr, np.zeros(6)])
+        for c in range(9):
+            cfg.load_init[c] = li[c]
+            cfg.load_rate[c] = lr[c]
+        cfg.load_t0 = load_t0
+        cfg.load_init_strain = load_init_strain
+        cfg.load_uniaxial_rate = load_uniaxial_rate
+        self.cfg = cfg
+        self.ndim, self.n, self.device = ndim, n, device
+        self.shape = (n,) * ndim
+        h = ctypes.c_void_p()
+        check(lib.cdd_plan_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.dev = torch.device("cuda", device)
+        self.bind_stream()
+
+    def bind_stream(self):
+        """Run on torch's current stream so ordering with torch ops is implicit."""
+        s = self.torch.cuda.current_stream(self.dev).cuda_stream
+        check(lib.cdd_set_stream(self.h, ctypes.c_void_p(s)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.cdd_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def empty(self, ncomp=9):
+        return self.torch.empty((ncomp,) + self.shape, dtype=self.torch.float64, device=self.dev)
+
+    def set_state(self, beta):
+        t = self._as_dev(beta)
+        check(lib.cdd_set_state(self.h, _ptr(t), 1))
+
+    def get_state(self):
+        out = self.empty()
+        check(lib.cdd_get_state(self.h, _ptr(out), 1))
+        return out
+
+    def set_time(self, t, t_end=float("inf"), fixed_dt=0.0):
+        check(lib.cdd_set_time(self.h, t, t_end, fixed_dt))
+
+    def clock(self):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        done, ns = ctypes.c_int(), ctypes.c_longlong()
+        check(lib.cdd_get_clock(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(done), ctypes.byref(ns)))
+        return t.value, dt.value, bool(done.value), ns.value
+
+    def step(self, nsteps=1):
+        check(lib.cdd_step(self.h, int(nsteps)))
+
+    def sigma(self, beta, t=0.0):
+        b = self._as_dev(beta)
+        out = self.empty()
+        check(lib.cdd_sigma(self.h, _ptr(b), t, _ptr(out)))
+        return out
+
+    def rho(self, beta):
+        b = self._as_dev(beta)
+        out = self.empty()
+        check(lib.cdd_rho(self.h, _ptr(b), _ptr(out)))
+        return out
+
+    def flux(self, beta, t=0.0):
+        b = self._as_dev(beta)
+        rhs = self.empty()
+        speed = self.torch.empty(self.shape, dtype=self.torch.float64, device=self.dev)
+        check(lib.cdd_flux(self.h, _ptr(b), t, _ptr(rhs), _ptr(speed)))
+        return rhs, speed
+
+    def energy(self, beta=None, t=0.0):
+        e = ctypes.c_double()
+        p = _ptr(self._as_dev(beta)) if beta is not None else None
+        check(lib.cdd_energy(self.h, p, t, ctypes.byref(e)))
+        return e.value
+
+    def timing_enable(self, max_events):
+        check(lib.cdd_timing_enable(self.h, int(max_events)))
+
+    def timing_read(self):
+        k = len(KERNEL_KINDS)
+        ms = (ctypes.c_double * k)()
+        cnt = (ctypes.c_int * k)()
+        check(lib.cdd_timing_read(self.h, ms, cnt, k))
+        return {KERNEL_KINDS[i]: (ms[i], cnt[i]) for i in range(k)}
+
+    def _as_dev(self, beta):
+        torch = self.torch
+        if isinstance(beta, np.ndarray):
+            beta = torch.from_numpy(np.ascontiguousarray(beta, dtype=np.float64))
+        t = beta.to(device=self.dev, dtype=torch.float64).contiguous()
+        if tuple(t.shape) != (9,) + self.shape:
+            raise CDDError("expected shape %s, got %s" % ((9,) + self.shape, tuple(t.shape)))
+        return t

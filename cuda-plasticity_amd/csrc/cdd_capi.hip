@@ -1,0 +1,476 @@
+// C ABI of the CDD stepper: plan ownership, state transfer and the RK
+// stage sequence (declared in include/cdd.h).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/cdd.h"
+#include "cdd_kernels.h"
+
+using namespace cdd;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      char b_[256];                                                                \
+      snprintf(b_, sizeof b_, "%s failed: %s", #expr, hipGetErrorString(e_));      \
+      return fail(CDD_ERR_HIP, b_);                                                \
+    }                                                                              \
+  } while (0)
+
+struct cdd_plan {
+  cdd_config cfg;
+  int d, n;
+  long long cells;
+  long long spec_elems;  // complex elements for 9 components
+  double* state[2];
+  int cur;               // buffer holding the state at steps_base
+  long long steps_base;
+  long long steps_launched;
+  double* L0;
+  double* buf;
+  double* G;
+  double* work9;
+  double* partial;
+  double* scalar;
+  void* spec;
+  void* tw;
+  Clock* clk;
+  hipStream_t own;
+  hipStream_t stream;
+  int nblocks;
+  int npartial;
+  GreenParams gp;
+  // HIP-event timing of every kernel launched by cdd_step (optional)
+  bool timing;
+  std::vector<hipEvent_t>* ev;
+  std::vector<int>* evkind;
+  int nev;
+};
+
+static void timing_hook(void* ctx, int kind) {
+  cdd_plan* p = (cdd_plan*)ctx;
+  if (!p->timing || p->nev >= (int)p->ev->size()) return;
+  if (hipEventRecord((*p->ev)[p->nev], p->stream) == hipSuccess) {
+    (*p->evkind)[p->nev] = kind;
+    p->nev++;
+  }
+}
+
+static bool valid_n(int d, int n) {
+  const int ok3[] = {8, 16, 32, 64, 128, 256, 512};
+  const int ok2[] = {8, 16, 32, 64, 128, 256, 384, 512, 768, 1024};
+  if (d == 3) {
+    for (int v : ok3)
+      if (v == n) return true;
+  } else if (d == 2) {
+    for (int v : ok2)
+      if (v == n) return true;
+  }
+  return false;
+}
+
+static int cur_state(cdd_plan* p) {
+  // RK2 ping-pongs between state[0] and state[1]; RK3 updates in place
+  if (p->cfg.rk_order == 3) return p->cur;
+  return p->cur ^ (int)((p->steps_launched - p->steps_base) & 1);
+}
+
+// Re-synchronise the host's view of the ping-pong buffer with the steps the
+// device actually executed (steps after `done` are skipped on the device).
+static int sync_clock(cdd_plan* p, Clock* host) {
+  HIPCHK(hipStreamSynchronize(p->stream));
+  Clock c;
+  HIPCHK(hipMemcpy(&c, p->clk, sizeof(Clock), hipMemcpyDeviceToHost));
+  if (p->cfg.rk_order != 3) {
+    long long executed = c.nsteps;
+    p->cur = p->cur ^ (int)((executed - p->steps_base) & 1);
+    p->steps_base = executed;
+    p->steps_launched = executed;
+  }
+  if (host) *host = c;
+  return 0;
+}
+
+extern "C" {
+
+const char* cdd_last_error(void) { return g_err.c_str(); }
+
+int cdd_plan_create(const cdd_config* cfg, cdd_plan** out) {
+  if (!cfg || !out) return fail(CDD_ERR_ARG, "null argument");
+  if (!valid_n(cfg->ndim, cfg->n)) return fail(CDD_ERR_UNSUPPORTED, "unsupported grid dimension/size");
+  if (cfg->rk_order != 2 && cfg->rk_order != 3) return fail(CDD_ERR_ARG, "rk_order must be 2 or 3");
+  if (cfg->dynamics != CDD_GLIDE_CLIMB && cfg->dynamics != CDD_GLIDE_ONLY) return fail(CDD_ERR_ARG, "bad dynamics");
+  if (cfg->load_kind < 0 || cfg->load_kind > 4) return fail(CDD_ERR_ARG, "bad load_kind");
+  if (!(cfg->mu > 0) || !(cfg->nu < 0.5) || !(cfg->cfl > 0)) return fail(CDD_ERR_ARG, "bad mu/nu/cfl");
+  HIPCHK(hipSetDevice(cfg->device));
+  cdd_plan* p = new cdd_plan();
+  memset(p, 0, sizeof(*p));
+  p->cfg = *cfg;
+  p->d = cfg->ndim;
+  p->n = cfg->n;
+  const int n = cfg->n;
+  p->cells = (long long)n * n * (p->d == 3 ? n : 1);
+  const long long H = n / 2 + 1;
+  p->spec_elems = 9LL * H * (p->d == 3 ? (long long)n * n : (long long)n);
+  p->nblocks = flux_nblocks(p->d, n);
+  p->npartial = p->nblocks > (int)((p->cells + 255) / 256) ? p->nblocks : (int)((p->cells + 255) / 256);
+  const size_t fb = sizeof(double) * 9 * (size_t)p->cells;
+#define ALLOC(ptr, bytes)                                 \
+  do {                                                    \
+    hipError_t e_ = hipMalloc((void**)&(ptr), (bytes));   \
+    if (e_ != hipSuccess) {                               \
+      cdd_plan_destroy(p);                                \
+      return fail(CDD_ERR_HIP, "hipMalloc failed");       \
+    }                                                     \
+  } while (0)
+  ALLOC(p->state[0], fb);
+  if (cfg->rk_order == 2) ALLOC(p->state[1], fb);
+  ALLOC(p->L0, fb);
+  ALLOC(p->buf, fb);
+  ALLOC(p->G, sizeof(double) * 6 * (size_t)p->cells);
+  ALLOC(p->work9, fb);
+  ALLOC(p->partial, sizeof(double) * (size_t)p->npartial);
+  ALLOC(p->scalar, sizeof(double) * 8);
+  ALLOC(p->spec, sizeof(double) * 2 * (size_t)p->spec_elems);
+  ALLOC(p->tw, sizeof(double) * 2 * (size_t)n);
+  ALLOC(p->clk, sizeof(Clock));
+#undef ALLOC
+  if (cfg->rk_order == 3) p->state[1] = p->buf;
+  // twiddle table W_n^e = exp(-2 pi i e / n)
+  {
+    double* h = new double[2 * n];
+    for (int e = 0; e < n; ++e) {
+      long double th = 2.0L * 3.14159265358979323846264338327950288L * (long double)e / (long double)n;
+      h[2 * e] = (double)cosl(th);
+      h[2 * e + 1] = (double)(-sinl(th));
+    }
+    hipError_t e_ = hipMemcpy(p->tw, h, sizeof(double) * 2 * n, hipMemcpyHostToDevice);
+    delete[] h;
+    if (e_ != hipSuccess) {
+      cdd_plan_destroy(p);
+      return fail(CDD_ERR_HIP, "twiddle upload failed");
+    }
+  }
+  if (hipStreamCreateWithFlags(&p->own, hipStreamNonBlocking) != hipSuccess) {
+    cdd_plan_destroy(p);
+    return fail(CDD_ERR_HIP, "stream create failed");
+  }
+  p->stream = p->own;
+  HIPCHK(hipMemsetAsync(p->state[0], 0, fb, p->stream));
+  // elastic / Green's constants
+  const double mu = cfg->mu, nu = cfg->nu;
+  const double lam = 2.0 * mu * nu / (1.0 - 2.0 * nu);
+  p->gp.A = lam / (lam + 2.0 * mu);
+  p->gp.C = 2.0 * (lam + mu) / (lam + 2.0 * mu);
+  p->gp.mu = mu;
+  p->gp.lam = lam;
+  p->gp.norm = 1.0 / (double)p->cells;
+  p->gp.el = Elastic{mu, nu, lam};
+  p->gp.load.kind = cfg->load_kind;
+  for (int c = 0; c < 9; ++c) {
+    p->gp.load.init[c] = cfg->load_init[c];
+    p->gp.load.rate[c] = cfg->load_rate[c];
+  }
+  p->gp.load.t0 = cfg->load_t0;
+  p->gp.load.init_strain = cfg->load_init_strain;
+  p->gp.load.uniaxial_rate = cfg->load_uniaxial_rate;
+  Clock c0;
+  memset(&c0, 0, sizeof c0);
+  c0.t = 0.0;
+  c0.t_end = INFINITY;
+  c0.cfl = cfg->cfl;
+  HIPCHK(hipMemcpy(p->clk, &c0, sizeof c0, hipMemcpyHostToDevice));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  *out = p;
+  return CDD_OK;
+}
+
+int cdd_plan_destroy(cdd_plan* p) {
+  if (!p) return CDD_OK;
+  (void)hipSetDevice(p->cfg.device);
+  if (p->own) (void)hipStreamSynchronize(p->own);
+  (void)hipFree(p->state[0]);
+  if (p->cfg.rk_order == 2) (void)hipFree(p->state[1]);
+  (void)hipFree(p->L0);
+  (void)hipFree(p->buf);
+  (void)hipFree(p->G);
+  (void)hipFree(p->work9);
+  (void)hipFree(p->partial);
+  (void)hipFree(p->scalar);
+  (void)hipFree(p->spec);
+  (void)hipFree(p->tw);
+  (void)hipFree(p->clk);
+  if (p->own) (void)hipStreamDestroy(p->own);
+  if (p->ev) {
+    for (hipEvent_t e : *p->ev) (void)hipEventDestroy(e);
+    delete p->ev;
+    delete p->evkind;
+  }
+  delete p;
+  return CDD_OK;
+}
+
+int cdd_set_stream(cdd_plan* p, void* stream) {
+  if (!p) return fail(CDD_ERR_ARG, "null plan");
+  // NULL selects the device's default (null) stream, as torch uses it
+  p->stream = (hipStream_t)stream;
+  return CDD_OK;
+}
+
+int cdd_set_state(cdd_plan* p, const double* beta, int on_device) {
+  if (!p || !beta) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  Clock c;
+  if (sync_clock(p, &c)) return CDD_ERR_HIP;
+  const size_t fb = sizeof(double) * 9 * (size_t)p->cells;
+  HIPCHK(hipMemcpyAsync(p->state[cur_state(p)], beta, fb, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                        p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return CDD_OK;
+}
+
+int cdd_get_state(cdd_plan* p, double* beta, int on_device) {
+  if (!p || !beta) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  if (sync_clock(p, nullptr)) return CDD_ERR_HIP;
+  const size_t fb = sizeof(double) * 9 * (size_t)p->cells;
+  HIPCHK(hipMemcpyAsync(beta, p->state[cur_state(p)], fb, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                        p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return CDD_OK;
+}
+
+int cdd_state_ptr(cdd_plan* p, double** beta_dev) {
+  if (!p || !beta_dev) return fail(CDD_ERR_ARG, "null argument");
+  if (sync_clock(p, nullptr)) return CDD_ERR_HIP;
+  *beta_dev = p->state[cur_state(p)];
+  return CDD_OK;
+}
+
+int cdd_set_time(cdd_plan* p, double t, double t_end, double fixed_dt) {
+  if (!p) return fail(CDD_ERR_ARG, "null plan");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  Clock c;
+  if (sync_clock(p, &c)) return CDD_ERR_HIP;
+  c.t = t;
+  c.t_end = t_end;
+  c.fixed_dt = fixed_dt;
+  c.done = (t >= t_end) ? 1 : 0;
+  HIPCHK(hipMemcpy(p->clk, &c, sizeof c, hipMemcpyHostToDevice));
+  return CDD_OK;
+}
+
+int cdd_get_clock(cdd_plan* p, double* t, double* last_dt, int* done, long long* nsteps) {
+  if (!p) return fail(CDD_ERR_ARG, "null plan");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  Clock c;
+  if (sync_clock(p, &c)) return CDD_ERR_HIP;
+  if (t) *t = c.t;
+  if (last_dt) *last_dt = c.dt;
+  if (done) *done = c.done;
+  if (nsteps) *nsteps = c.nsteps;
+  return CDD_OK;
+}
+
+static SpectralJob spectral_job(cdd_plan* p, int nc, const double* A, const double* B, const Clock* clk, double t,
+                                double* out) {
+  SpectralJob j;
+  j.d = p->d;
+  j.n = p->n;
+  j.nc = nc;
+  j.cells = p->cells;
+  j.src = SrcField<double>{A, B, clk, p->cells};
+  j.spec = p->spec;
+  j.tw = p->tw;
+  j.out = out;
+  j.gp = p->gp;
+  j.gp.clk = clk;
+  j.gp.t = t;
+  j.stream = p->stream;
+  j.hook = p->timing ? timing_hook : nullptr;
+  j.hook_ctx = p;
+  return j;
+}
+
+static FluxJob flux_job(cdd_plan* p) {
+  FluxJob f;
+  memset(&f, 0, sizeof f);
+  f.d = p->d;
+  f.n = p->n;
+  f.dyn = p->cfg.dynamics;
+  f.cells = p->cells;
+  f.G = p->G;
+  f.Lambda = p->cfg.dynamics == CDD_GLIDE_ONLY ? 0.0 : p->cfg.Lambda;
+  f.mu = p->gp.mu;
+  f.lam = p->gp.lam;
+  f.stream = p->stream;
+  return f;
+}
+
+int cdd_sigma(cdd_plan* p, const double* beta, double t, double* sigma) {
+  if (!p || !beta || !sigma) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  SpectralJob j = spectral_job(p, 6, beta, nullptr, nullptr, t, p->G);
+  HIPCHK(launch_spectral(j));
+  HIPCHK(launch_sigma_finalize(p->G, beta, sigma, p->cells, p->gp.mu, p->gp.lam, p->stream));
+  return CDD_OK;
+}
+
+int cdd_rho(cdd_plan* p, const double* beta, double* rho) {
+  if (!p || !beta || !rho) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  SpectralJob j = spectral_job(p, 9, beta, nullptr, nullptr, 0.0, rho);
+  HIPCHK(launch_spectral(j));
+  return CDD_OK;
+}
+
+int cdd_flux(cdd_plan* p, const double* beta, double t, double* rhs, double* speed) {
+  if (!p || !beta || !rhs) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  SpectralJob j = spectral_job(p, 6, beta, nullptr, nullptr, t, p->G);
+  HIPCHK(launch_spectral(j));
+  FluxJob f = flux_job(p);
+  f.A = beta;
+  f.out = rhs;
+  f.speed = speed;
+  f.mode = 0;
+  HIPCHK(launch_flux(f, nullptr));
+  return CDD_OK;
+}
+
+int cdd_step(cdd_plan* p, int nsteps) {
+  if (!p || nsteps < 0) return fail(CDD_ERR_ARG, "bad argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  const double nmax = (double)p->n;
+  for (int s = 0; s < nsteps; ++s) {
+    const int ci = cur_state(p);
+    double* u = p->state[ci];
+    // stage 1: L0 = L(u); dt from the stage-1 speeds
+    if (p->timing) timing_hook(p, -1);
+    HIPCHK(launch_spectral(spectral_job(p, 6, u, nullptr, p->clk, 0.0, p->G)));
+    FluxJob f = flux_job(p);
+    f.clk = p->clk;
+    f.A = u;
+    f.out = p->L0;
+    f.mode = 0;
+    f.partial = p->clk ? p->partial : nullptr;
+    int nb = 0;
+    HIPCHK(launch_flux(f, &nb));
+    if (p->timing) timing_hook(p, KK_FLUX1);
+    HIPCHK(launch_finalize_dt(p->partial, nb, p->clk, nmax, p->stream));
+    if (p->timing) timing_hook(p, KK_FINALIZE);
+    // stage 2 on F0 = u + dt L0 (formed on the fly)
+    HIPCHK(launch_spectral(spectral_job(p, 6, u, p->L0, p->clk, 0.0, p->G)));
+    if (p->cfg.rk_order == 3) {
+      FluxJob f2 = flux_job(p);
+      f2.clk = p->clk;
+      f2.A = u;
+      f2.B = p->L0;
+      f2.U = u;
+      f2.out = p->buf;
+      f2.mode = 1;
+      f2.cu = 0.75;
+      f2.cf = 0.25;
+      HIPCHK(launch_flux(f2, nullptr));
+      if (p->timing) timing_hook(p, KK_FLUX2);
+      // stage 3 on F1
+      HIPCHK(launch_spectral(spectral_job(p, 6, p->buf, nullptr, p->clk, 0.0, p->G)));
+      FluxJob f3 = flux_job(p);
+      f3.clk = p->clk;
+      f3.A = p->buf;
+      f3.U = u;
+      f3.out = u;  // in place: u is only read at the thread's own cell
+      f3.mode = 1;
+      f3.cu = 1.0 / 3.0;
+      f3.cf = 2.0 / 3.0;
+      HIPCHK(launch_flux(f3, nullptr));
+      if (p->timing) timing_hook(p, KK_FLUX3);
+    } else {
+      FluxJob f2 = flux_job(p);
+      f2.clk = p->clk;
+      f2.A = u;
+      f2.B = p->L0;
+      f2.U = u;
+      f2.out = p->state[ci ^ 1];
+      f2.mode = 1;
+      f2.cu = 0.5;
+      f2.cf = 0.5;
+      HIPCHK(launch_flux(f2, nullptr));
+      if (p->timing) timing_hook(p, KK_FLUX2);
+    }
+    HIPCHK(launch_advance(p->clk, p->stream));
+    if (p->timing) timing_hook(p, KK_ADVANCE);
+    p->steps_launched += 1;
+  }
+  return CDD_OK;
+}
+
+int cdd_energy(cdd_plan* p, const double* beta, double t, double* energy) {
+  if (!p || !energy) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  if (!beta) {
+    if (sync_clock(p, nullptr)) return CDD_ERR_HIP;
+    beta = p->state[cur_state(p)];
+  }
+  HIPCHK(launch_spectral(spectral_job(p, 6, beta, nullptr, nullptr, t, p->G)));
+  int nb = 0;
+  HIPCHK(launch_energy(p->G, beta, p->cells, p->gp.mu, p->gp.el.nu, p->gp.lam, p->partial, &nb, p->stream));
+  HIPCHK(launch_reduce_sum(p->partial, nb, p->scalar, p->stream));
+  HIPCHK(hipMemcpyAsync(energy, p->scalar, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return CDD_OK;
+}
+
+int cdd_timing_enable(cdd_plan* p, int max_events) {
+  if (!p || max_events < 0) return fail(CDD_ERR_ARG, "bad argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  if (!p->ev) {
+    p->ev = new std::vector<hipEvent_t>();
+    p->evkind = new std::vector<int>();
+  }
+  for (hipEvent_t e : *p->ev) (void)hipEventDestroy(e);
+  p->ev->clear();
+  p->evkind->assign(max_events, -1);
+  for (int i = 0; i < max_events; ++i) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    p->ev->push_back(e);
+  }
+  p->nev = 0;
+  p->timing = max_events > 0;
+  return CDD_OK;
+}
+
+int cdd_timing_read(cdd_plan* p, double* ms, int* counts, int nkinds) {
+  if (!p || !ms || !counts) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  for (int k = 0; k < nkinds; ++k) {
+    ms[k] = 0.0;
+    counts[k] = 0;
+  }
+  for (int i = 1; i < p->nev; ++i) {
+    const int k = (*p->evkind)[i];
+    if (k < 0 || k >= nkinds) continue;
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, (*p->ev)[i - 1], (*p->ev)[i]));
+    ms[k] += t;
+    counts[k] += 1;
+  }
+  return CDD_OK;
+}
+
+}  // extern "C"

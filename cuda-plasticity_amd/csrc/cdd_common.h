@@ -1,0 +1,86 @@
+// Shared device types and helpers for the CDD stepper (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cdd {
+
+// Machine epsilon of float64: the ME of Plasticity/NumericalMethods.py:4.
+constexpr double kME = 2.220446049250313e-16;
+
+template <typename T>
+struct cx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator*(cx<T> a, T s) { return {a.x * s, a.y * s}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> conj(cx<T> a) { return {a.x, -a.y}; }
+// -i * a
+template <typename T>
+__device__ __forceinline__ cx<T> mul_ni(cx<T> a) { return {a.y, -a.x}; }
+// +i * a
+template <typename T>
+__device__ __forceinline__ cx<T> mul_pi(cx<T> a) { return {-a.y, a.x}; }
+
+// Run-time clock of one stepper instance, resident in device memory so the
+// CFL time step never round-trips through the host.
+struct Clock {
+  double t;         // time at the start of the current step
+  double t_end;     // integration end time (Run's endTime)
+  double dt;        // time step of the current step
+  double fixed_dt;  // > 0: fixed step (CalculateOneFixedTimeStep)
+  double cfl;       // CFLsafeFactor
+  double last_max;  // max speed seen by the last CFL reduction
+  int done;         // 1 once t >= t_end: every step kernel returns at once
+  int pad;
+  long long nsteps;  // completed steps
+};
+
+// Uniform load added to sigma (it enters the stress solve as the k = 0 mode).
+struct LoadParams {
+  int kind;  // 0 none, 1 strain, 2 stress, 3 uniaxial strain-rate, 4 full tensor
+  int pad;
+  double init[9];  // strain/stress: diag in [0..2]; tensor: 9 components
+  double rate[9];
+  double t0;
+  double init_strain;    // uniaxial: initial <beta_zz>
+  double uniaxial_rate;  // uniaxial: strain rate
+};
+
+struct Elastic {
+  double mu, nu, lam;  // lam = 2 mu nu / (1 - 2 nu)
+};
+
+// Uniform stress shift of the load at time t; mean_bzz = <beta_zz> of the
+// stage state (used by the uniaxial strain-rate control only).
+__device__ __forceinline__ void load_shift(const LoadParams& L, const Elastic& E, double t,
+                                           double mean_bzz, double s[9]) {
+#pragma unroll
+  for (int c = 0; c < 9; ++c) s[c] = 0.0;
+  if (L.kind == 1) {  // RunPython.py:21-26, 44-49
+    double e[3];
+    for (int i = 0; i < 3; ++i) e[i] = L.init[i] + L.rate[i] * (t - L.t0);
+    double tr = e[0] + e[1] + e[2];
+    for (int i = 0; i < 3; ++i) s[4 * i] = E.lam * tr + 2.0 * E.mu * e[i];
+  } else if (L.kind == 2) {  // RunPython.py:28-32
+    for (int i = 0; i < 3; ++i) s[4 * i] = L.init[i] + L.rate[i] * (t - L.t0);
+  } else if (L.kind == 3) {  // MattTools/Run/RunLoadingMPI.py:68-83
+    double bzz = L.uniaxial_rate * t - (mean_bzz - L.init_strain);
+    for (int i = 0; i < 3; ++i) s[4 * i] = E.lam * bzz;
+    s[8] += 2.0 * E.mu * bzz;
+  } else if (L.kind == 4) {
+    for (int c = 0; c < 9; ++c) s[c] = L.init[c] + L.rate[c] * (t - L.t0);
+  }
+}
+
+}  // namespace cdd

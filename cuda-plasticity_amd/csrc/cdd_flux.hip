@@ -1,0 +1,478 @@
+// Fused central-upwind Hamilton-Jacobi flux + TVD Runge-Kutta stage kernels.
+//
+// Replaces, per RK stage, CentralUpwindHJDynamics.CalculateFlux
+// (Plasticity/FieldDynamics/CentralUpwindHJ3D.py:63-271) with the BetaP
+// velocities/Hamiltonians of CentralUpwindHJBetaPDynamics3D.py:34-221
+// (glide + climb, climb suppression Lambda) and
+// CentralUpwindHJBetaPGlideOnlyDynamics.py:24-262 ('lvp' glide only), plus
+// the stage combination of TVDRungeKutta_FieldMover
+// (Plasticity/FieldMovers/FieldMover.py:115-135).
+//
+// One thread owns one cell.  It reads the stage field at the 4d+1 cells of
+// the radius-2 cross stencil (optionally forming F0 = u + dt L0 on the fly,
+// so the first RK stage update never makes its own HBM pass), the six
+// non-local stress planes G and, for the combination, the step-start state
+// at the cell, and writes nine output planes.
+//
+// The reference evaluates the corner velocity 2^d + d 2^d times per stage and
+// the Hamiltonian once per corner with full 9x3 contractions.  Both are
+// linear in per-axis quantities, which this kernel exploits:
+//   v_l(corner)  = sum_a w_a^{s_a}[l] - [l<d] z_l^{s_l}  (+ Lambda / lvp terms)
+//   w_a^s[l] = sum_n u_a^s[l,n] sigma_an,  z_a^s = sum_mn u_a^s[m,n] sigma_mn
+//   |rho|^2 = ME + sum_a P_a^{s_a} - 2 sum_pairs C_ab^{s_a s_b}
+//   sum_c w_c H_ij(c) = sum_a sum_s u_a^s[i,j] Y_a^s[a]
+//                       - [i<d] sum_s sum_l u_i^s[l,j] Y_i^s[l] + d_ij sum_c w_c J_c
+//   with Y_a^s[l] = sum over corners with s_a = s of w_c v_l(c).
+// The rounding differs from the reference at the 1e-16 level only.
+#include "cdd_kernels.h"
+
+namespace cdd {
+
+template <typename T>
+__device__ __forceinline__ T minmod3(T a, T b, T c) {
+  // NumericalMethods.py:367-378: min if all > 0, max if all <= 0, else 0
+  const T mn = fmin(a, fmin(b, c));
+  const T mx = fmax(a, fmax(b, c));
+  return mn > T(0) ? mn : (mx <= T(0) ? mx : T(0));
+}
+
+// One-sided limited derivatives of f at i from f[i-2..i+2]
+// (FindDerivatives, CentralUpwindHJ3D.py:11-34, theta = 1, Dx = 1/N_last).
+template <typename T>
+__device__ __forceinline__ void limited_derivs(const T f[5], T invDx, T halfDx, T& up, T& um) {
+  const T Dm2 = f[1] - f[0], Dm1 = f[2] - f[1], D0 = f[3] - f[2], D1 = f[4] - f[3];
+  const T s1_i = (D1 - D0) * invDx * invDx;
+  const T s1_im1 = (D0 - Dm1) * invDx * invDx;
+  const T s1_im2 = (Dm1 - Dm2) * invDx * invDx;
+  const T s2_i = (D1 - Dm1) * invDx * invDx * T(0.5);
+  const T s2_im1 = (D0 - Dm2) * invDx * invDx * T(0.5);
+  const T S_i = minmod3(s1_i, s2_i, s1_im1);
+  const T S_im1 = minmod3(s1_im1, s2_im1, s1_im2);
+  up = D0 * invDx - halfDx * S_i;
+  um = Dm1 * invDx + halfDx * S_im1;
+}
+
+// Per-cell flux.  u[a][s][c]: derivative along axis a, s = 0 ('+') or 1
+// ('-'), component c = 3 i + j.  sg: full (symmetric) stress.
+template <typename T, int D, int DYN>
+__device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9], T Lambda, T (&rhs)[9],
+                                          T& speed) {
+  constexpr int NCOR = 1 << D;
+  const T ME = T(kME);
+  // per axis / sign quantities
+  T w[D][2][3], zz[D][2], tr[D][2], P[D][2];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const T* ua = u[a][s];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) w[a][s][l] = ua[3 * l] * sg[3 * a] + ua[3 * l + 1] * sg[3 * a + 1] + ua[3 * l + 2] * sg[3 * a + 2];
+      T z = T(0);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) z += ua[c] * sg[c];
+      zz[a][s] = z;
+      tr[a][s] = ua[0] + ua[4] + ua[8];
+      T p = T(0);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        if (m == a) continue;
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) p += ua[3 * m + jj] * ua[3 * m + jj];
+      }
+      P[a][s] = p;
+    }
+  }
+  // |rho|^2 cross terms (BetaP3D_RhoMod, CentralUpwindHJBetaPDynamics3D.py:45-58)
+  T Cxy[2][2], Cyz[2][2], Czx[2][2];
+#pragma unroll
+  for (int s0 = 0; s0 < 2; ++s0)
+#pragma unroll
+    for (int s1 = 0; s1 < 2; ++s1) {
+      Cxy[s0][s1] = u[0][s0][3] * u[1][s1][0] + u[0][s0][4] * u[1][s1][1] + u[0][s0][5] * u[1][s1][2];
+      if constexpr (D == 3) {
+        Cyz[s0][s1] = u[1][s0][6] * u[2][s1][3] + u[1][s0][7] * u[2][s1][4] + u[1][s0][8] * u[2][s1][5];
+        Czx[s0][s1] = u[2][s0][0] * u[0][s1][6] + u[2][s0][1] * u[0][s1][7] + u[2][s0][2] * u[0][s1][8];
+      } else {
+        Cyz[s0][s1] = T(0);
+        Czx[s0][s1] = T(0);
+      }
+    }
+  const T st = (DYN == DYN_GLIDE_CLIMB) ? Lambda * (sg[0] + sg[4] + sg[8]) / T(3) : T(0);
+  const T jfac = -Lambda / T(3);
+  const bool climb_sup = (DYN == DYN_GLIDE_CLIMB) && (Lambda != T(0));
+
+  T vc[NCOR][3], Jc[NCOR];
+  T amax[3], amin[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    amax[a] = -INFINITY;
+    amin[a] = INFINITY;
+  }
+#pragma unroll
+  for (int c = 0; c < NCOR; ++c) {
+    int sgn[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < D; ++a) sgn[a] = (c >> (D - 1 - a)) & 1;
+    T R[3], v0[3], q[3];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      T r = T(0), vv = T(0);
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        r += u[a][sgn[a]][3 * l + a];
+        vv += w[a][sgn[a]][l];
+      }
+      R[l] = r;
+      v0[l] = vv;
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) v0[a] -= zz[a][sgn[a]];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      // q_a = tr(u_a) - sum_b u_b[a, b]; tr(u_a) = 0 for a >= d
+      const T tra = (a < D) ? tr[a % D][sgn[a]] : T(0);
+      q[a] = tra - R[a];
+    }
+    if (DYN == DYN_GLIDE_CLIMB) {
+      if (climb_sup) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) v0[a] += st * tr[a][sgn[a]];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) v0[l] -= st * R[l];
+      }
+    } else {
+      // lvp projection (CentralUpwindHJBetaPGlideOnlyDynamics.py:170-195)
+      T rr = ME;
+#pragma unroll
+      for (int a = 0; a < D; ++a) rr += tr[a][sgn[a]] * tr[a][sgn[a]] - T(2) * tr[a][sgn[a]] * R[a];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) rr += R[l] * R[l];
+      const T rv = ME + v0[0] * q[0] + v0[1] * q[1] + v0[2] * q[2];
+      const T f = rv / rr;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) v0[a] += q[a] * f;
+    }
+    T r2 = ME;
+#pragma unroll
+    for (int a = 0; a < D; ++a) r2 += P[a][sgn[a]];
+    T cross = Cxy[sgn[0]][sgn[1]];
+    if constexpr (D == 3) cross += Cyz[sgn[1]][sgn[2]] + Czx[sgn[2]][sgn[0]];
+    r2 -= T(2) * cross;
+    const T rinv = T(1) / sqrt(r2);
+#pragma unroll
+    for (int l = 0; l < 3; ++l) vc[c][l] = v0[l] * rinv;
+    Jc[c] = climb_sup ? jfac * (vc[c][0] * q[0] + vc[c][1] * q[1] + vc[c][2] * q[2]) : T(0);
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      amax[a] = fmax(amax[a], vc[c][a]);
+      amin[a] = fmin(amin[a], vc[c][a]);
+    }
+  }
+  // local one-sided speeds (Max5/Max9 with 0, CentralUpwindHJ3D.py:114-241)
+  T ap[3], am[3], at[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    ap[a] = fmax(amax[a], T(0));
+    am[a] = -fmin(amin[a], T(0));
+    at[a] = ap[a] + am[a] + ME;
+  }
+  T Y[D][2][3], Jsum = T(0);
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) Y[a][s][l] = T(0);
+#pragma unroll
+  for (int c = 0; c < NCOR; ++c) {
+    T wc = T(1);
+    int sgn[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      sgn[a] = (c >> (D - 1 - a)) & 1;
+      wc *= sgn[a] == 0 ? am[a] : ap[a];
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) Y[a][sgn[a]][l] += wc * vc[c][l];
+    Jsum += wc * Jc[c];
+  }
+  T inv = T(1);
+#pragma unroll
+  for (int a = 0; a < D; ++a) inv *= at[a];
+  inv = T(1) / inv;
+  speed = T(0);
+#pragma unroll
+  for (int a = 0; a < D; ++a) speed += at[a];
+  T tw[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a) tw[a] = am[a] * ap[a] / at[a];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+      const int cc = 3 * i + jj;
+      T h = (i == jj) ? Jsum : T(0);
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) h += u[a][s][cc] * Y[a][s][a];
+      if (i < D) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+          for (int l = 0; l < 3; ++l) h -= u[i < D ? i : 0][s][3 * l + jj] * Y[i < D ? i : 0][s][l];
+      }
+      T r = -h * inv;
+#pragma unroll
+      for (int a = 0; a < D; ++a) r += (u[a][0][cc] - u[a][1][cc]) * tw[a];
+      rhs[cc] = r;
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T* red) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const int nt = blockDim.x * blockDim.y;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const int wid = tid >> 6, lane = tid & 63;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  T m = -INFINITY;
+  if (tid == 0) {
+    for (int k = 0; k < (nt + 63) / 64; ++k) m = fmax(m, red[k]);
+  }
+  return m;
+}
+
+template <typename T, int D, int DYN>
+__global__ void __launch_bounds__(256) k_flux(FluxJob job) {
+  if (job.clk && job.clk->done) return;
+  __shared__ T red[4];
+  const int n = job.n;
+  const long long cells = job.cells;
+  int cx_, cy_, cz_ = 0;
+  if constexpr (D == 3) {
+    cz_ = blockIdx.x * blockDim.x + threadIdx.x;
+    cy_ = blockIdx.y * blockDim.y + threadIdx.y;
+    cx_ = blockIdx.z;
+  } else {
+    cy_ = blockIdx.x * blockDim.x + threadIdx.x;
+    cx_ = blockIdx.y * blockDim.y + threadIdx.y;
+  }
+  const T invDx = T(n);
+  const T halfDx = T(0.5) * (T(1) / T(n));
+  const T dt = job.clk ? T(job.clk->dt) : T(0);
+  const bool useB = job.B != nullptr;
+
+  int coord[3] = {cx_, cy_, cz_};
+  long long stride[3];
+  if constexpr (D == 3) {
+    stride[0] = (long long)n * n;
+    stride[1] = n;
+    stride[2] = 1;
+  } else {
+    stride[0] = n;
+    stride[1] = 1;
+    stride[2] = 0;
+  }
+  const long long idx = (long long)coord[0] * stride[0] + (long long)coord[1] * stride[1] + (D == 3 ? coord[2] : 0);
+
+  T u[D][2][9];
+  T fc[9];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    long long off[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      int p = coord[a] + k - 2;
+      p = p < 0 ? p + n : (p >= n ? p - n : p);
+      off[k] = idx + (long long)(p - coord[a]) * stride[a];
+    }
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      T f[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        T vv = job.A[c * cells + off[k]];
+        if (useB) vv += dt * job.B[c * cells + off[k]];
+        f[k] = vv;
+      }
+      if (a == 0) fc[c] = f[2];
+      limited_derivs<T>(f, invDx, halfDx, u[a][0][c], u[a][1][c]);
+    }
+  }
+  // sigma = G - lam tr(beta) d - mu (beta + beta^T) at the cell
+  T sg[9];
+  {
+    const T tr = fc[0] + fc[4] + fc[8];
+    const T g0 = job.G[0 * cells + idx], g1 = job.G[1 * cells + idx], g2 = job.G[2 * cells + idx];
+    const T g3 = job.G[3 * cells + idx], g4 = job.G[4 * cells + idx], g5 = job.G[5 * cells + idx];
+    const T mu = T(job.mu), lam = T(job.lam);
+    sg[0] = g0 - lam * tr - mu * (fc[0] + fc[0]);
+    sg[4] = g1 - lam * tr - mu * (fc[4] + fc[4]);
+    sg[8] = g2 - lam * tr - mu * (fc[8] + fc[8]);
+    sg[1] = sg[3] = g3 - mu * (fc[1] + fc[3]);
+    sg[2] = sg[6] = g4 - mu * (fc[2] + fc[6]);
+    sg[5] = sg[7] = g5 - mu * (fc[5] + fc[7]);
+  }
+  T rhs[9], speed;
+  cell_flux<T, D, DYN>(u, sg, T(job.Lambda), rhs, speed);
+  if (job.mode == 0) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) job.out[c * cells + idx] = rhs[c];
+  } else {
+    const T cu = T(job.cu), cf = T(job.cf);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) job.out[c * cells + idx] = cu * job.U[c * cells + idx] + cf * (fc[c] + dt * rhs[c]);
+  }
+  if (job.speed) job.speed[idx] = speed;
+  if (job.partial) {
+    T m = block_max<T>(speed, red);
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    if (tid == 0) job.partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = m;
+  }
+}
+
+static void flux_geometry(int d, int n, dim3& g, dim3& b) {
+  const int bx = n < 64 ? n : 64;
+  int by = 256 / bx;
+  if (by > n) by = n;
+  b = dim3(bx, by, 1);
+  if (d == 3) g = dim3(n / bx, n / by, n);
+  else g = dim3(n / bx, n / by, 1);
+}
+
+int flux_nblocks(int d, int n) {
+  dim3 g, b;
+  flux_geometry(d, n, g, b);
+  return (int)(g.x * g.y * g.z);
+}
+
+hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
+  dim3 g, b;
+  flux_geometry(job.d, job.n, g, b);
+  if (nblocks_out) *nblocks_out = (int)(g.x * g.y * g.z);
+  if (job.d == 3) {
+    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_ONLY>), g, b, 0, job.stream, job);
+    else hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_CLIMB>), g, b, 0, job.stream, job);
+  } else {
+    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_ONLY>), g, b, 0, job.stream, job);
+    else hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_CLIMB>), g, b, 0, job.stream, job);
+  }
+  return hipGetLastError();
+}
+
+// dt = CFL / (max(gridShape) max speed), clamped to the end time
+// (FieldMover.py:94-111).  One block.
+__global__ void __launch_bounds__(1024) k_finalize_dt(const double* partial, int nb, Clock* clk, double nmax) {
+  if (clk->done) return;
+  __shared__ double red[16];
+  double m = -INFINITY;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) m = fmax(m, partial[k]);
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = -INFINITY;
+    for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) mm = fmax(mm, red[k]);
+    double dt = clk->fixed_dt > 0.0 ? clk->fixed_dt : clk->cfl / (nmax * mm);
+    if (clk->t + dt > clk->t_end) dt = clk->t_end - clk->t;
+    clk->dt = dt;
+    clk->last_max = mm;
+  }
+}
+
+__global__ void k_fixed_dt(Clock* clk) {
+  if (clk->done) return;
+  double dt = clk->fixed_dt;
+  if (clk->t + dt > clk->t_end) dt = clk->t_end - clk->t;
+  clk->dt = dt;
+}
+
+// t += dt after the step; stop once t >= endTime (FieldMover.py:30-36)
+__global__ void k_advance(Clock* clk) {
+  if (clk->done) return;
+  clk->t += clk->dt;
+  clk->nsteps += 1;
+  if (clk->t >= clk->t_end) clk->done = 1;
+}
+
+hipError_t launch_finalize_dt(const double* partial, int nb, Clock* clk, double nmax, hipStream_t st) {
+  if (partial) hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(1024), 0, st, partial, nb, clk, nmax);
+  else hipLaunchKernelGGL(k_fixed_dt, dim3(1), dim3(1), 0, st, clk);
+  return hipGetLastError();
+}
+
+hipError_t launch_advance(Clock* clk, hipStream_t st) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, clk);
+  return hipGetLastError();
+}
+
+// Elastic energy density e = 1/2 S_ijkm sigma_ij sigma_km
+// (PlasticityState.py:317-330), summed per block.
+__global__ void __launch_bounds__(256) k_energy(const double* G, const double* beta, long long cells, double mu,
+                                               double nu, double lam, double* partial) {
+  __shared__ double red[4];
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0;
+  if (idx < cells) {
+    double b[9];
+    for (int c = 0; c < 9; ++c) b[c] = beta[c * cells + idx];
+    const double tr = b[0] + b[4] + b[8];
+    double s[9];
+    const int SC[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+    for (int c = 0; c < 9; ++c) {
+      const int i = c / 3, j = c % 3;
+      s[c] = G[SC[c] * cells + idx] - mu * (b[c] + b[3 * j + i]) - (i == j ? lam * tr : 0.0);
+    }
+    double ss = 0.0, sst = 0.0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        ss += s[3 * i + j] * s[3 * i + j];
+        sst += s[3 * i + j] * s[3 * j + i];
+      }
+    const double trs = s[0] + s[4] + s[8];
+    e = 0.5 * (2.0 * ss - sst - nu / (1.0 + nu) * trs * trs) / (2.0 * mu);
+  }
+  for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) t += red[k];
+    partial[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_reduce_sum(const double* partial, int nb, double* out) {
+  __shared__ double red[16];
+  double m = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) m += partial[k];
+  for (int off = 32; off > 0; off >>= 1) m += __shfl_xor(m, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) t += red[k];
+    *out = t;
+  }
+}
+
+hipError_t launch_energy(const double* G, const double* beta, long long cells, double mu, double nu, double lam,
+                         double* partial, int* nblocks, hipStream_t st) {
+  const int nb = (int)((cells + 255) / 256);
+  *nblocks = nb;
+  hipLaunchKernelGGL(k_energy, dim3(nb), dim3(256), 0, st, G, beta, cells, mu, nu, lam, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_sum(const double* partial, int n, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(1024), 0, st, partial, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace cdd

@@ -1,0 +1,78 @@
+// Internal launch interface between the C-ABI layer and the kernel files.
+#pragma once
+#include "cdd_common.h"
+#include "cdd_fft.h"
+
+namespace cdd {
+
+// A stencil/source field: value = A + dt * B (B optional, dt from clk).
+template <typename T>
+struct SrcField {
+  const T* A;
+  const T* B;
+  const Clock* clk;
+  long long cs;  // component stride (cells)
+};
+
+struct GreenParams {
+  double A, C;      // lam/(lam+2mu), 2(lam+mu)/(lam+2mu)
+  double mu, lam;
+  double norm;      // 1 / N^d
+  double t;         // stage time when clk == nullptr
+  const Clock* clk;
+  LoadParams load;
+  Elastic el;
+};
+
+// Called after each kernel launch with the kernel kind (HIP-event timing).
+enum KernelKind {
+  KK_R2C = 0, KK_COLS_FWD = 1, KK_CONTRACT = 2, KK_COLS_INV = 3, KK_C2R = 4,
+  KK_FLUX1 = 5, KK_FINALIZE = 6, KK_FLUX2 = 7, KK_FLUX3 = 8, KK_ADVANCE = 9, KK_OTHER = 10, KK_COUNT = 11
+};
+typedef void (*LaunchHook)(void* ctx, int kind);
+
+struct SpectralJob {
+  LaunchHook hook;
+  void* hook_ctx;
+  int d, n, nc;          // nc = 6 (stress) or 9 (Nye tensor)
+  long long cells;
+  SrcField<double> src;
+  void* spec;            // complex workspace, nc * (n/2+1) * n^(d-1)
+  const void* tw;        // twiddle table for n
+  double* out;           // nc real planes
+  GreenParams gp;
+  hipStream_t stream;
+};
+
+hipError_t launch_spectral(const SpectralJob& job);
+hipError_t launch_sigma_finalize(const double* G, const double* beta, double* sig, long long cells, double mu,
+                                 double lam, hipStream_t st);
+
+enum Dyn { DYN_GLIDE_CLIMB = 0, DYN_GLIDE_ONLY = 1 };
+
+struct FluxJob {
+  int d, n, dyn;
+  long long cells;
+  const double* A;     // stencil source
+  const double* B;     // optional increment: f = A + dt * B
+  const double* G;     // 6 planes of the non-local stress
+  const double* U;     // optional state for the RK combination
+  double* out;         // 9 planes
+  double* speed;       // optional per-cell speed
+  double* partial;     // optional per-block max of the speed
+  Clock* clk;          // dt (for B and the combination)
+  int mode;            // 0: out = rhs; 1: out = cu U + cf (f + dt rhs)
+  double cu, cf;
+  double Lambda, mu, lam;
+  hipStream_t stream;
+};
+
+hipError_t launch_flux(const FluxJob& job, int* nblocks_out);
+int flux_nblocks(int d, int n);
+hipError_t launch_finalize_dt(const double* partial, int nblocks, Clock* clk, double nmax, hipStream_t st);
+hipError_t launch_advance(Clock* clk, hipStream_t st);
+hipError_t launch_energy(const double* G, const double* beta, long long cells, double mu, double nu, double lam,
+                         double* partial, int* nblocks, hipStream_t st);
+hipError_t launch_reduce_sum(const double* partial, int n, double* out, hipStream_t st);
+
+}  // namespace cdd

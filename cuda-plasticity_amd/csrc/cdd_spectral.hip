@@ -1,0 +1,415 @@
+// Spectral passes of the stress solve (and the Nye tensor) on gfx950.
+//
+// Replaces PlasticityState.CalculateSigma(type='k', source='betaE')
+// (Plasticity/PlasticityStates/PlasticityState.py:85-127, 357-390) and
+// CalculateRhoFourier (:264-304), i.e. 9 numpy rfftn + 81 array expressions
+// + 9 irfftn per call.  Here one stress solve is a chain of HBM passes over
+// a half spectrum that only ever holds the six symmetric components
+// e_mn = (beta_mn + beta_nm) / 2 (the Green's tensor T_ijmn is symmetric in
+// m, n, so the antisymmetric part never contributes):
+//
+//   3D (N^3, spectrum [c][kz][x][y], kz in 0..N/2):
+//     r2c_last   : R2C along z of two real components at once, store
+//                  transposed so every later pass is unit-stride along y
+//     fft_cols   : complex FFT along x (columns of 16 consecutive y)
+//     contract   : FFT along y, Green's contraction (+ k = 0 load mode),
+//                  inverse FFT along y -- one pass, data stays in LDS/VGPRs
+//     fft_cols   : inverse FFT along x
+//     c2r_last   : C2R along z, transposed load, writes 6 real planes G
+//   2D (N^2, spectrum [c][ky][x]): r2c_last, contract (along x), c2r_last.
+//
+// G_ij = IFFT(lam d_ij T_ll(k) + mu (T_ij + T_ji)) is the non-local part of
+// sigma; the local part -lam tr(beta) d_ij - mu (beta_ij + beta_ji) is added
+// by whoever reads beta at the cell anyway (the flux kernel, or
+// k_sigma_finalize for the standalone CalculateSigma).
+#include "cdd_kernels.h"
+
+namespace cdd {
+
+// ------------------------------------------------------------- tile sizes
+template <int N>
+struct Tiles {
+  static constexpr int TT = FFTPlan<N>::T;
+  static constexpr int P = LdsRow<N>::P;
+  // rows per r2c/c2r tile (transposed store: TY consecutive complex values)
+  // largest power of two <= min(16, 4096 / N, N), so TY divides N
+  static constexpr int TY0 = (4096 / N) < 16 ? (4096 / N) : 16;
+  static constexpr int TY1 = TY0 >= 16 ? 16 : (TY0 >= 8 ? 8 : (TY0 >= 4 ? 4 : (TY0 >= 2 ? 2 : 1)));
+  static constexpr int TY = TY1 < N ? TY1 : N;
+  static_assert(N % TY == 0, "row tile must divide N");
+  // columns per strided tile
+  static constexpr int TW = TY;
+};
+
+template <int N, int NC>
+struct ContractTiles {
+  static constexpr int TT = FFTPlan<N>::T;
+  static constexpr int TR0 = 256 / (NC * TT);
+  static constexpr int TR = TR0 >= 16 ? 16 : (TR0 >= 8 ? 8 : (TR0 >= 4 ? 4 : (TR0 >= 2 ? 2 : 1)));
+};
+
+// value of source component c (SYM: symmetric-part index) at cell idx
+template <typename T, bool SYM>
+__device__ __forceinline__ T src_comp(const SrcField<T>& s, int c, long long idx, T dt) {
+  auto b = [&](int k) -> T {
+    T v = s.A[k * s.cs + idx];
+    if (s.B) v += dt * s.B[k * s.cs + idx];
+    return v;
+  };
+  if constexpr (SYM) {
+    switch (c) {
+      case 0: return b(0);
+      case 1: return b(4);
+      case 2: return b(8);
+      case 3: return T(0.5) * (b(1) + b(3));
+      case 4: return T(0.5) * (b(2) + b(6));
+      default: return T(0.5) * (b(5) + b(7));
+    }
+  } else {
+    return b(c);
+  }
+}
+
+// --------------------------------------------------------------- r2c_last
+// grid: (N / TY, NO, npairs); block: TY * TT.  NO = N in 3D (outer x), 1 in 2D.
+template <typename T, int N, bool SYM>
+__global__ void __launch_bounds__(Tiles<N>::TY * Tiles<N>::TT)
+k_r2c_last(SrcField<T> src, cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, int NO, int NC) {
+  using TL = Tiles<N>;
+  constexpr int TT = TL::TT, TY = TL::TY, P = TL::P, V = FFTPlan<N>::V, H = N / 2 + 1;
+  if (src.clk && src.clk->done) return;
+  __shared__ cx<T> lds[TY * P];
+  const int tid = threadIdx.x;
+  const int r = tid / TT, j = tid % TT;
+  const int r0 = blockIdx.x * TY;
+  const int o = blockIdx.y;
+  const int ca = 2 * blockIdx.z, cb = 2 * blockIdx.z + 1;
+  const bool hasb = cb < NC;
+  const T dt = (src.B && src.clk) ? T(src.clk->dt) : T(0);
+
+  cx<T> v[V];
+  const long long rowbase = ((long long)o * N + (r0 + r)) * N;
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const long long idx = rowbase + fft_in_index<N>(j, i);
+    v[i].x = src_comp<T, SYM>(src, ca, idx, dt);
+    v[i].y = hasb ? src_comp<T, SYM>(src, cb, idx, dt) : T(0);
+  }
+  RowExchange<T> ex{lds + r * P};
+  fft_forward<T, N>(v, j, tw, ex);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < V; ++i) lds[r * P + lds_pad(fft_out_index<N>(j, i))] = v[i];
+  __syncthreads();
+  // separate the two real transforms and store [c][k][o][r] (r contiguous)
+  const int rr = tid % TY, g = tid / TY;
+  for (int k = g; k < H; k += TT) {
+    cx<T> z1 = lds[rr * P + lds_pad(k)];
+    cx<T> z2 = conj(lds[rr * P + lds_pad((N - k) % N)]);
+    const long long oa = (((long long)ca * H + k) * NO + o) * N + r0 + rr;
+    spec[oa] = (z1 + z2) * T(0.5);
+    if (hasb) {
+      const long long ob = (((long long)cb * H + k) * NO + o) * N + r0 + rr;
+      spec[ob] = mul_ni(z1 - z2) * T(0.5);
+    }
+  }
+}
+
+// --------------------------------------------------------------- c2r_last
+// Inverse of k_r2c_last: reads [c][k][o][r], writes real out[c][o][r][e].
+template <typename T, int N>
+__global__ void __launch_bounds__(Tiles<N>::TY * Tiles<N>::TT)
+k_c2r_last(const cx<T>* __restrict__ spec, T* __restrict__ out, long long cs, const cx<T>* __restrict__ tw,
+           int NO, int NC, const Clock* clk) {
+  using TL = Tiles<N>;
+  constexpr int TT = TL::TT, TY = TL::TY, P = TL::P, V = FFTPlan<N>::V, H = N / 2 + 1;
+  if (clk && clk->done) return;
+  __shared__ cx<T> lds[TY * P];
+  const int tid = threadIdx.x;
+  const int r0 = blockIdx.x * TY;
+  const int o = blockIdx.y;
+  const int ca = 2 * blockIdx.z, cb = 2 * blockIdx.z + 1;
+  const bool hasb = cb < NC;
+  {
+    const int rr = tid % TY, g = tid / TY;
+    for (int k = g; k < H; k += TT) {
+      cx<T> A = spec[(((long long)ca * H + k) * NO + o) * N + r0 + rr];
+      cx<T> B = hasb ? spec[(((long long)cb * H + k) * NO + o) * N + r0 + rr] : cx<T>{T(0), T(0)};
+      if (k == 0 || k == N / 2) {  // irfft ignores the imaginary part here
+        A.y = T(0);
+        B.y = T(0);
+      }
+      lds[rr * P + lds_pad(k)] = cx<T>{A.x - B.y, A.y + B.x};
+      if (k > 0 && k < N / 2) lds[rr * P + lds_pad(N - k)] = cx<T>{A.x + B.y, B.x - A.y};
+    }
+  }
+  __syncthreads();
+  const int r = tid / TT, j = tid % TT;
+  cx<T> v[V];
+#pragma unroll
+  for (int i = 0; i < V; ++i) v[i] = conj(lds[r * P + lds_pad(fft_in_index<N>(j, i))]);
+  RowExchange<T> ex{lds + r * P};
+  fft_forward<T, N>(v, j, tw, ex);
+  const long long rowbase = ((long long)o * N + (r0 + r)) * N;
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const long long idx = rowbase + fft_out_index<N>(j, i);
+    out[ca * cs + idx] = v[i].x;   // conj(FFT(conj z)).x
+    if (hasb) out[cb * cs + idx] = -v[i].y;
+  }
+}
+
+// --------------------------------------------------------------- fft_cols
+// In-place complex FFT along x of spec[c][kz][x][y]; grid (N / TW, H, NC).
+template <typename T, int N, bool INV>
+__global__ void __launch_bounds__(Tiles<N>::TW * Tiles<N>::TT)
+k_fft_cols(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, const Clock* clk) {
+  using TL = Tiles<N>;
+  constexpr int TT = TL::TT, TW = TL::TW, V = FFTPlan<N>::V, H = N / 2 + 1;
+  if (clk && clk->done) return;
+  __shared__ cx<T> lds[TW * N];
+  const int tid = threadIdx.x;
+  const int t = tid % TW, j = tid / TW;
+  const int y0 = blockIdx.x * TW;
+  const long long base = ((long long)blockIdx.z * H + blockIdx.y) * N * N + y0 + t;
+  cx<T> v[V];
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    cx<T> a = spec[base + (long long)fft_in_index<N>(j, i) * N];
+    v[i] = INV ? conj(a) : a;
+  }
+  ColExchange<T, TW> ex{lds + t};
+  fft_forward<T, N>(v, j, tw, ex);
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    cx<T> a = v[i];
+    spec[base + (long long)fft_out_index<N>(j, i) * N] = INV ? conj(a) : a;
+  }
+}
+
+// --------------------------------------------------------------- contract
+__device__ __forceinline__ double kwrap(int i, int n) {
+  // 2 pi n_wrapped with the Nyquist index kept positive
+  // (Plasticity/GridArray/FourierSpaceTools.py:5-7, 29-30, 47-49)
+  return 6.283185307179586 * (double)(i <= n / 2 ? i : i - n);
+}
+
+// Green's contraction for output symmetric component c at one k.
+// e[6] = FFT of (xx, yy, zz, xy, xz, yz) of the symmetric part of beta.
+template <typename T>
+__device__ __forceinline__ cx<T> green_sigma(const cx<T>* e, int c, T kx, T ky, T kz, const GreenParams& gp) {
+  const T kSq = kx * kx + ky * ky + kz * kz;
+  const T K2 = kSq + T(kME);
+  const T K4 = kSq * kSq + T(kME);
+  const cx<T> t = e[0] + e[1] + e[2];
+  cx<T> p[3];
+  p[0] = e[0] * kx + e[3] * ky + e[4] * kz;
+  p[1] = e[3] * kx + e[1] * ky + e[5] * kz;
+  p[2] = e[4] * kx + e[5] * ky + e[2] * kz;
+  const cx<T> q = p[0] * kx + p[1] * ky + p[2] * kz;
+  const T iK2 = T(1) / K2, iK4 = T(1) / K4;
+  const cx<T> a1 = t * (T(gp.A) * iK2) - q * (T(gp.C) * iK4);
+  // symmetric component c -> (i, j): xx yy zz xy xz yz
+  const int i = c < 3 ? c : (c == 5 ? 1 : 0);
+  const int j = c < 3 ? c : (c == 3 ? 1 : 2);
+  const T ki = i == 0 ? kx : (i == 1 ? ky : kz);
+  const T kj = j == 0 ? kx : (j == 1 ? ky : kz);
+  const cx<T> pi_ = i == 0 ? p[0] : (i == 1 ? p[1] : p[2]);
+  const cx<T> pj_ = j == 0 ? p[0] : (j == 1 ? p[1] : p[2]);
+  // S_ij = T_ij + T_ji
+  cx<T> S = a1 * (T(2) * ki * kj) + (pj_ * ki + pi_ * kj) * (T(2) * iK2);
+  cx<T> out = S * T(gp.mu);
+  if (i == j) {
+    cx<T> Tll = a1 * kSq + q * (T(2) * iK2);
+    out = out + Tll * T(gp.lam);
+  }
+  return out * T(gp.norm);
+}
+
+// Nye tensor rho_ij = -eps_ilm d_l beta_mj in k space (times 1/N^d).
+template <typename T>
+__device__ __forceinline__ cx<T> green_rho(const cx<T>* b, int c, T kx, T ky, T kz, T norm) {
+  const int i = c / 3, j = c % 3;
+  cx<T> r;
+  if (i == 0) r = b[3 + j] * kz - b[6 + j] * ky;
+  else if (i == 1) r = b[6 + j] * kx - b[j] * kz;
+  else r = b[j] * ky - b[3 + j] * kx;
+  return mul_pi(r) * norm;
+}
+
+// Rows of spec laid out as (c * NR + row) * N + e.  grid.x = ceil(NR / TR).
+// 3D: row = kz * N + x, e = y;  2D: row = ky, e = x.
+template <typename T, int N, int NC, int D>
+__global__ void __launch_bounds__((NC * ContractTiles<N, NC>::TR * ContractTiles<N, NC>::TT))
+k_contract(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, GreenParams gp) {
+  using CT = ContractTiles<N, NC>;
+  constexpr int TT = CT::TT, TR = CT::TR, P = LdsRow<N>::P, V = FFTPlan<N>::V, H = N / 2 + 1;
+  constexpr long long NR = (D == 3) ? (long long)H * N : (long long)H;
+  if (gp.clk && gp.clk->done) return;
+  __shared__ cx<T> lds[NC * TR * P];
+  const int tid = threadIdx.x;
+  const int j = tid % TT;
+  const int r = (tid / TT) % TR;
+  const int c = tid / (TT * TR);
+  const long long row = (long long)blockIdx.x * TR + r;
+  const bool valid = row < NR;
+  const long long rbase = ((long long)c * NR + row) * N;
+
+  cx<T> v[V];
+#pragma unroll
+  for (int i = 0; i < V; ++i) v[i] = valid ? spec[rbase + fft_in_index<N>(j, i)] : cx<T>{T(0), T(0)};
+  cx<T>* myrow = lds + (c * TR + r) * P;
+  RowExchange<T> ex{myrow};
+  fft_forward<T, N>(v, j, tw, ex);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < V; ++i) myrow[lds_pad(fft_out_index<N>(j, i))] = v[i];
+  __syncthreads();
+
+  // k vector of this row
+  T kx, ky, kz;
+  int kzi = 0, rowi = 0;
+  if constexpr (D == 3) {
+    kzi = (int)(row / N);
+    rowi = (int)(row % N);  // x index
+    kz = T(6.283185307179586 * kzi);
+    kx = T(kwrap(rowi, N));
+  } else {
+    ky = T(6.283185307179586 * (double)row);
+    kz = T(0);
+  }
+  double tnow = gp.clk ? gp.clk->t : gp.t;
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const int e = fft_in_index<N>(j, i);
+    cx<T> in[NC];
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) in[cc] = lds[(cc * TR + r) * P + lds_pad(e)];
+    if constexpr (D == 3) ky = T(kwrap(e, N));
+    else kx = T(kwrap(e, N));
+    cx<T> o;
+    if constexpr (NC == 6) {
+      const bool k0 = (row == 0) && (e == 0);
+      if (!k0) {
+        o = green_sigma<T>(in, c, kx, ky, kz, gp);
+      } else {
+        // k = 0: kmask removes the Green's term; the uniform load enters here
+        double s[9];
+        const double mean_bzz = (double)in[2].x * gp.norm;
+        load_shift(gp.load, gp.el, tnow, mean_bzz, s);
+        const int cs9 = c == 0 ? 0 : c == 1 ? 4 : c == 2 ? 8 : c == 3 ? 1 : c == 4 ? 2 : 5;
+        T sv = T(0);
+#pragma unroll
+        for (int q9 = 0; q9 < 9; ++q9)
+          if (q9 == cs9) sv = T(s[q9]);
+        o = cx<T>{sv, T(0)};
+      }
+    } else {
+      o = green_rho<T>(in, c, kx, ky, kz, T(gp.norm));
+    }
+    v[i] = conj(o);
+  }
+  fft_forward<T, N>(v, j, tw, ex);
+  if (valid) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) spec[rbase + fft_out_index<N>(j, i)] = conj(v[i]);
+  }
+}
+
+// sigma_ij = G_(ij) - lam tr(beta) d_ij - mu (beta_ij + beta_ji)
+template <typename T>
+__global__ void k_sigma_finalize(const T* __restrict__ G, const T* __restrict__ beta, T* __restrict__ sig,
+                                 long long cells, T mu, T lam) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= cells) return;
+  T b[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) b[c] = beta[c * cells + idx];
+  const T tr = b[0] + b[4] + b[8];
+  constexpr int SC[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    const int i = c / 3, jj = c % 3;
+    T s = G[SC[c] * cells + idx] - mu * (b[c] + b[3 * jj + i]);
+    if (i == jj) s -= lam * tr;
+    sig[c * cells + idx] = s;
+  }
+}
+
+// ------------------------------------------------------------- launchers
+template <int N>
+static hipError_t launch_spectral_n(const SpectralJob& job) {
+  using T = double;
+  using TL = Tiles<N>;
+  const int d = job.d;
+  const int NO = d == 3 ? N : 1;
+  const int NC = job.nc;
+  const int npairs = (NC + 1) / 2;
+  const long long cs = job.cells;
+  const int H = N / 2 + 1;
+  hipStream_t st = job.stream;
+  const cx<T>* tw = reinterpret_cast<const cx<T>*>(job.tw);
+  cx<T>* spec = reinterpret_cast<cx<T>*>(job.spec);
+
+  dim3 g1(N / TL::TY, NO, npairs), b1(TL::TY * TL::TT);
+  auto hook = [&](int kind) {
+    if (job.hook) job.hook(job.hook_ctx, kind);
+  };
+  if (NC == 6)
+    hipLaunchKernelGGL((k_r2c_last<T, N, true>), g1, b1, 0, st, job.src, spec, tw, NO, NC);
+  else
+    hipLaunchKernelGGL((k_r2c_last<T, N, false>), g1, b1, 0, st, job.src, spec, tw, NO, NC);
+  hook(KK_R2C);
+  if (d == 3) {
+    dim3 g2(N / TL::TW, H, NC), b2(TL::TW * TL::TT);
+    hipLaunchKernelGGL((k_fft_cols<T, N, false>), g2, b2, 0, st, spec, tw, job.gp.clk);
+    hook(KK_COLS_FWD);
+  }
+  const long long NR = d == 3 ? (long long)H * N : H;
+  if (NC == 6) {
+    using CT = ContractTiles<N, 6>;
+    dim3 g3((unsigned)((NR + CT::TR - 1) / CT::TR)), b3(6 * CT::TR * CT::TT);
+    if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 6, 3>), g3, b3, 0, st, spec, tw, job.gp);
+    else hipLaunchKernelGGL((k_contract<T, N, 6, 2>), g3, b3, 0, st, spec, tw, job.gp);
+  } else {
+    using CT = ContractTiles<N, 9>;
+    dim3 g3((unsigned)((NR + CT::TR - 1) / CT::TR)), b3(9 * CT::TR * CT::TT);
+    if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 9, 3>), g3, b3, 0, st, spec, tw, job.gp);
+    else hipLaunchKernelGGL((k_contract<T, N, 9, 2>), g3, b3, 0, st, spec, tw, job.gp);
+  }
+  hook(KK_CONTRACT);
+  if (d == 3) {
+    dim3 g2(N / TL::TW, H, NC), b2(TL::TW * TL::TT);
+    hipLaunchKernelGGL((k_fft_cols<T, N, true>), g2, b2, 0, st, spec, tw, job.gp.clk);
+    hook(KK_COLS_INV);
+  }
+  hipLaunchKernelGGL((k_c2r_last<T, N>), g1, b1, 0, st, spec, job.out, cs, tw, NO, NC, job.gp.clk);
+  hook(KK_C2R);
+  return hipGetLastError();
+}
+
+hipError_t launch_spectral(const SpectralJob& job) {
+  switch (job.n) {
+    case 8: return launch_spectral_n<8>(job);
+    case 16: return launch_spectral_n<16>(job);
+    case 32: return launch_spectral_n<32>(job);
+    case 64: return launch_spectral_n<64>(job);
+    case 128: return launch_spectral_n<128>(job);
+    case 256: return launch_spectral_n<256>(job);
+    case 384: return job.d == 2 ? launch_spectral_n<384>(job) : hipErrorInvalidValue;
+    case 512: return launch_spectral_n<512>(job);
+    case 768: return job.d == 2 ? launch_spectral_n<768>(job) : hipErrorInvalidValue;
+    case 1024: return job.d == 2 ? launch_spectral_n<1024>(job) : hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_sigma_finalize(const double* G, const double* beta, double* sig, long long cells, double mu,
+                                 double lam, hipStream_t st) {
+  const int bs = 256;
+  hipLaunchKernelGGL((k_sigma_finalize<double>), dim3((unsigned)((cells + bs - 1) / bs)), dim3(bs), 0, st, G, beta,
+                     sig, cells, mu, lam);
+  return hipGetLastError();
+}
+
+}  // namespace cdd

@@ -1,0 +1,141 @@
+/* C ABI of the MI355X-native continuum-dislocation-dynamics (CDD) stepper.
+ *
+ * The reference crosses its native boundary as a process + file exchange:
+ * RunCUDA.simulation (Plasticity/RunCUDA.py:120-254) writes a transposed
+ * .ics state (:9-22) and a #define header (:26-29, 54-95), runs
+ * `make HEADER=...` (:214) and `./exe --device=N` (:225), and reads back
+ * .plas records [t, 9 N^d] (assist.h:164-192).  Inside the process the hot
+ * path is plasticity.cu runTest (:40-376) driving calculateFlux /
+ * calculateSigma / centralHJ / simpleTVD of plasticity_kernel.cu
+ * (:951-1206).  On the Python side the same path is the
+ * PlasticitySystem -> FieldMover -> FieldDynamics -> PlasticityState call
+ * chain.  This library replaces all of it with in-process calls on a plan
+ * that owns its device buffers; the Python host layer
+ * (cuda-plasticity_amd/Plasticity) binds these symbols through ctypes.
+ *
+ * Conventions
+ *   - Fields are float64 arrays of shape (9, N0, N1[, N2]), C-contiguous,
+ *     component c = 3 i + j in the sorted order xx xy xz yx yy yz zx zy zz
+ *     (Plasticity/Fields/Fields.py:21, 247-249): NumPy layout, last axis
+ *     contiguous.  Grids are square (side n) and periodic.
+ *   - Pointers marked `dev` are device pointers on the plan's device
+ *     (e.g. torch HIP tensors); they are borrowed for the call only and used
+ *     in stream order on the plan's stream.
+ *   - Every function returns 0 on success and a negative code on failure;
+ *     cdd_last_error() returns the message of the calling thread's last
+ *     failure.  The Python layer raises RuntimeError, as RunCUDA does on a
+ *     non-zero exit status (Plasticity/RunCUDA.py:221-228).
+ *   - Calls on different plans may run concurrently from different host
+ *     threads (the exec_batch.py pattern); one plan is not thread-safe.
+ *   - cdd_step never synchronises with the host: the CFL time step is
+ *     reduced and kept on the device.  Functions that return host values
+ *     (cdd_get_clock, cdd_energy) synchronise the plan's stream.
+ */
+#ifndef CDD_H
+#define CDD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CDD_OK = 0,
+  CDD_ERR_ARG = -1,
+  CDD_ERR_HIP = -2,
+  CDD_ERR_UNSUPPORTED = -3
+};
+
+enum { CDD_GLIDE_CLIMB = 0, CDD_GLIDE_ONLY = 1 };
+
+enum {
+  CDD_LOAD_NONE = 0,
+  CDD_LOAD_STRAIN = 1,   /* RunPython.py:21-26,44-49 ExternalStrain, strain(t) = init + rate (t - t0) */
+  CDD_LOAD_STRESS = 2,   /* RunPython.py:28-32 ExternalStress */
+  CDD_LOAD_UNIAXIAL = 3, /* RunLoadingMPI.py:68-83 strain-rate control along z */
+  CDD_LOAD_TENSOR = 4    /* sigma_ij += init_ij + rate_ij (t - t0); e.g. uniform shear */
+};
+
+typedef struct cdd_config {
+  int ndim;          /* 2 or 3 */
+  int n;             /* grid side: 8..512 powers of two in 3D; 8..1024 and 384, 768 in 2D */
+  int dynamics;      /* CDD_GLIDE_CLIMB ('mdp' Lambda=1, 'gcd' Lambda=0, partial climb) or CDD_GLIDE_ONLY ('lvp') */
+  int rk_order;      /* 3: Shu-Osher TVD-RK3 (FieldMover.py:61-141); 2: SSP-RK2 */
+  double Lambda;     /* climb suppression (CentralUpwindHJBetaPDynamics3D.py:126-133) */
+  double mu, nu;     /* isotropic elasticity (PlasticityState.py:23) */
+  double cfl;        /* CFLsafeFactor (FieldMover.py:16,96) */
+  int load_kind;     /* CDD_LOAD_* */
+  int device;        /* HIP device ordinal */
+  double load_init[9];
+  double load_rate[9];
+  double load_t0;
+  double load_init_strain;
+  double load_uniaxial_rate;
+} cdd_config;
+
+typedef struct cdd_plan cdd_plan;
+
+/* Create / destroy a stepper plan (replaces plasticity.cu:40-240 set-up and
+ * plasticity_kernel.cu setupSystem :931-949). */
+int cdd_plan_create(const cdd_config* cfg, cdd_plan** out);
+int cdd_plan_destroy(cdd_plan* plan);
+
+/* Run the plan's work on an existing hipStream_t, e.g. torch's current
+ * stream; NULL is the default (null) stream.  A new plan uses a private
+ * non-blocking stream. */
+int cdd_set_stream(cdd_plan* plan, void* stream);
+
+/* State transfer (replaces the .ics write / .plas read, RunCUDA.py:9-22 and
+ * plasticity.cu:139-206,268-270).  on_device: 1 if the pointer is a device
+ * pointer, 0 for host memory. */
+int cdd_set_state(cdd_plan* plan, const double* beta, int on_device);
+int cdd_get_state(cdd_plan* plan, double* beta, int on_device);
+
+/* Clock: time at the start of the next step and Run's endTime
+ * (FieldMover.py:20-36).  fixed_dt > 0 selects CalculateOneFixedTimeStep. */
+int cdd_set_time(cdd_plan* plan, double t, double t_end, double fixed_dt);
+int cdd_get_clock(cdd_plan* plan, double* t, double* last_dt, int* done, long long* nsteps);
+
+/* sigma = PlasticityState.CalculateSigma(type='k', source='betaE') of beta
+ * (PlasticityState.py:85-127,357-390) plus the plan's load at time t (the
+ * GetSigma override, RunPython.py:44-49).  beta_dev, sigma_dev: (9, N^d). */
+int cdd_sigma(cdd_plan* plan, const double* beta_dev, double t, double* sigma_dev);
+
+/* rho = PlasticityState.CalculateRhoFourier(type='k') (PlasticityState.py:264-304). */
+int cdd_rho(cdd_plan* plan, const double* beta_dev, double* rho_dev);
+
+/* rhs (9, N^d) and the CFL speed field a_tot + b_tot (+ c_tot) (N^d,
+ * nullable) of BetaPDynamics.CalculateFlux(t, state, CFLCondition=True)
+ * (CentralUpwindHJBetaPDynamics3D.py:214-221 / GlideOnly :255-262 ->
+ * CentralUpwindHJ3D.py:63-271). */
+int cdd_flux(cdd_plan* plan, const double* beta_dev, double t, double* rhs_dev, double* speed_dev);
+
+/* Advance the plan's state by nsteps TVD-RK steps
+ * (TVDRungeKutta_FieldMover.CalculateOneAdaptiveTimeStep, FieldMover.py:61-141;
+ * plasticity.cu:242-271 loop).  Stops on the device once t >= t_end; no host
+ * synchronisation. */
+int cdd_step(cdd_plan* plan, int nsteps);
+
+/* Total elastic energy sum_cells 1/2 S sigma sigma of beta (nullable: the
+ * plan's state) at time t (PlasticityState.py:317-330, Observer.py:86). */
+int cdd_energy(cdd_plan* plan, const double* beta_dev, double t, double* energy);
+
+/* Device pointer of the plan's current state (9, N^d) after the last
+ * synchronising call. */
+int cdd_state_ptr(cdd_plan* plan, double** beta_dev);
+
+/* HIP-event timing of the kernels cdd_step launches (the tracing the
+ * reference does with cutil timers, plasticity.cu:169-170,227-228,320-350):
+ * enable with room for max_events records (0 disables), run steps, then
+ * read per-kind totals in ms and launch counts.  Kinds: 0 r2c, 1 column FFT
+ * forward, 2 Green's contraction pass, 3 column FFT inverse, 4 c2r,
+ * 5 flux stage 1, 6 CFL reduction, 7 flux stage 2, 8 flux stage 3,
+ * 9 clock advance. */
+int cdd_timing_enable(cdd_plan* plan, int max_events);
+int cdd_timing_read(cdd_plan* plan, double* ms, int* counts, int nkinds);
+
+const char* cdd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,149 @@
+"""GPU parity: the HIP path through the C ABI against the oracle on the
+same seeded fields (tolerances are relative L2 / relative max as noted)."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import cdd_oracle as O
+from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
+
+pytestmark = pytest.mark.gpu
+
+MU, NU = 1.0, 0.3
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel())
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from Plasticity import amd as A
+    return A
+
+
+def _plan(amd, d, n, dyn=0, Lambda=0.0, rk=3, **kw):
+    return amd.Plan(d, n, dynamics=dyn, rk_order=rk, Lambda=Lambda, mu=MU, nu=NU, cfl=kw.pop("cfl", 0.25), **kw)
+
+
+@pytest.mark.parametrize("d,n", [(2, 8), (2, 16), (2, 64), (2, 384), (3, 8), (3, 16), (3, 32)])
+def test_sigma_matches_oracle(amd, d, n):
+    say("sigma", d, n)
+    beta = SpectralGaussianField((n,) * d, 2.0, 0.01, seed=11)
+    p = _plan(amd, d, n)
+    got = p.sigma(beta).cpu().numpy()
+    ref = O.sigma(beta, MU, NU)
+    assert rel_l2(got, ref) < 1e-13
+
+
+@pytest.mark.parametrize("d,n", [(2, 16), (2, 128), (3, 16)])
+def test_rho_matches_oracle(amd, d, n):
+    say("rho", d, n)
+    beta = SpectralGaussianField((n,) * d, 2.0, 0.01, seed=12)
+    p = _plan(amd, d, n)
+    got = p.rho(beta).cpu().numpy()
+    ref = O.rho_fourier(beta)
+    assert rel_l2(got, ref) < 1e-13
+
+
+@pytest.mark.parametrize("dyn,Lambda", [(0, 0.0), (0, 1.0), (0, 0.9), (1, 0.0)])
+@pytest.mark.parametrize("d,n", [(2, 16), (2, 64), (3, 8), (3, 16)])
+def test_flux_matches_oracle(amd, d, n, dyn, Lambda):
+    say("flux", d, n, dyn, Lambda)
+    beta = SpectralGaussianField((n,) * d, 2.0, 0.01, seed=13)
+    p = _plan(amd, d, n, dyn, Lambda)
+    rhs, speed = p.flux(beta)
+    od = O.Dynamics(O.GLIDE_ONLY if dyn == 1 else O.GLIDE_CLIMB, Lambda, MU, NU)
+    ref_rhs, ref_speed = od.flux(0.0, beta)
+    assert rel_l2(rhs.cpu().numpy(), ref_rhs) < 1e-12
+    assert rel_l2(speed.cpu().numpy(), ref_speed) < 1e-12
+
+
+@pytest.mark.parametrize("rk", [2, 3])
+@pytest.mark.parametrize("d,n,dyn,Lambda", [(2, 32, 1, 0.0), (2, 32, 0, 0.9), (3, 16, 0, 0.9), (3, 16, 1, 0.0)])
+def test_steps_match_oracle(amd, d, n, dyn, Lambda, rk):
+    nsteps = 10
+    say("steps", d, n, dyn, Lambda, rk)
+    beta = SpectralGaussianField((n,) * d, 2.0, 0.01, seed=14)
+    p = _plan(amd, d, n, dyn, Lambda, rk=rk)
+    p.set_state(beta)
+    p.set_time(0.0)
+    p.step(nsteps)
+    got = p.get_state().cpu().numpy()
+    t, dt, done, ns = p.clock()
+    od = O.Dynamics(O.GLIDE_ONLY if dyn == 1 else O.GLIDE_CLIMB, Lambda, MU, NU)
+    ref, tref, dts = O.run(od, beta, 0.0, nsteps, 0.25, order=rk)
+    assert ns == nsteps
+    assert abs(t - tref) <= 1e-12 * tref
+    assert rel_l2(got, ref) < 1e-11
+
+
+def test_energy_matches_oracle(amd):
+    say("energy")
+    for d, n in [(2, 32), (3, 16)]:
+        beta = SpectralGaussianField((n,) * d, 2.0, 0.01, seed=15)
+        p = _plan(amd, d, n)
+        e = p.energy(beta)
+        ref = O.elastic_energy(beta, MU, NU)
+        assert abs(e - ref) < 1e-12 * abs(ref)
+
+
+def test_end_time_stops_on_device(amd):
+    say("end time")
+    n = 16
+    beta = SpectralGaussianField((n, n), 2.0, 0.01, seed=16)
+    p = _plan(amd, 2, n, 1, rk=3)
+    p.set_state(beta)
+    od = O.Dynamics(O.GLIDE_ONLY, 0.0, MU, NU)
+    _, t1, dts = O.run(od, beta, 0.0, 1, 0.25)
+    t_end = 2.5 * dts[0]
+    p.set_time(0.0, t_end)
+    p.step(20)  # only the steps up to t_end execute
+    t, dt, done, ns = p.clock()
+    ref, tref, dts = O.run(od, beta, 0.0, 20, 0.25, t_end=t_end)
+    assert done and ns == len(dts) and abs(t - t_end) < 1e-15
+    assert rel_l2(p.get_state().cpu().numpy(), ref) < 1e-12
+
+
+def test_reference_api_drop_in(amd):
+    """PlasticitySystem -> TVDRungeKutta_FieldMover -> BetaPDynamics with a
+    RecallStateObserver, the call chain of RunPython.simulation
+    (RunPython.py:224-257), against the oracle."""
+    say("api drop-in")
+    from Plasticity import PlasticitySystem
+    from Plasticity.FieldMovers import FieldMover
+    from Plasticity.FieldDynamics import CentralUpwindHJBetaPGlideOnlyDynamics
+    from Plasticity.Observers import Observer
+    from Plasticity.PlasticityStates.PlasticityState import PlasticityState
+    n = 32
+    beta = SpectralGaussianField((n, n), 2.0, 0.01, seed=17)
+    state = PlasticityState((n, n), field=beta, mu=MU, nu=NU)
+    dyn = CentralUpwindHJBetaPGlideOnlyDynamics.BetaPDynamics()
+    mover = FieldMover.TVDRungeKutta_FieldMover(CFLsafeFactor=0.25)
+    obs = Observer.RecallStateObserver()
+    system = PlasticitySystem.PlasticitySystem((n, n), state, mover, dyn, [obs])
+    od = O.Dynamics(O.GLIDE_ONLY, 0.0, MU, NU)
+    _, _, dts = O.run(od, beta, 0.0, 6, 0.25)
+    t_end = float(np.sum(dts[:5]) + 0.5 * dts[5])
+    system.Run(startTime=0.0, endTime=t_end)
+    ref, tref, dref = O.run(od, beta, 0.0, 100, 0.25, t_end=t_end)
+    assert mover.numberOfSteps == len(dref)
+    assert rel_l2(obs.state.betaP.numpy(), ref) < 1e-12
+    # the observer-free path (chunked launches) gives the same state
+    mover2 = FieldMover.TVDRungeKutta_FieldMover(CFLsafeFactor=0.25)
+    mover2.Run(state, [], dyn, 0.0, t_end)
+    assert rel_l2(mover2.lastState.betaP.numpy(), ref) < 1e-12
+    # stress through the state API
+    s = state.CalculateSigma().numpy()
+    assert rel_l2(s, O.sigma(beta, MU, NU)) < 1e-13
