@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X CDD stepper: 3D 256^3 fp64 glide + climb
+(Lambda = 0.9, i.e. climb mobility 0.1) under uniaxial strain-rate loading,
+TVD-RK3, CFL 0.25, mu = 1, nu = 0.3 -- the workload the headline metric
+(cell-updates/s + HBM roofline %) is quoted on.
+
+A step is one full TVD-RK3 step (three stress solves + three fused
+flux/stage kernels + the device-side CFL reduction) over the whole grid;
+inputs are resident in HBM before the timed region starts.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "cuda-plasticity_amd")]
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E datasheet peak
+
+
+def spectral_bytes(n, d=3):
+    """Algorithmic HBM bytes per cell of each kernel kind (fp64, complex = 16 B,
+    half spectrum h = (n/2+1)/n; 6 symmetric stress components)."""
+    h = (n // 2 + 1) / n
+    return {
+        # 9 real beta planes in, 6 half-spectrum planes out; the stage-2 launch
+        # also reads L0 and writes F0 = u + dt L0 (one launch in three)
+        "r2c": 72 + 96 * h + (72 + 72) / 3.0,
+        "cols_fwd": 192 * h,          # 6 complex planes in + out
+        "contract": 192 * h,
+        "cols_inv": 192 * h,
+        "c2r": 96 * h + 48,           # 6 half-spectrum planes in, 6 real G planes out
+        "flux_stage1": 72 + 48 + 72,  # beta, G in; L0 out
+        "flux_stage2": 72 + 48 + 72 + 72,  # F0, G, u in; F1 out
+        "flux_stage3": 72 + 48 + 72 + 72,  # F1, G, u in; u' out
+    }
+
+
+def canonical_bytes_per_cell_update(n, d=3, order=3):
+    """B/cell-update of the canonical fused pipeline: per stage
+    B_fft = 96 + 384 h (d - 1); flux + update 408 B (stage 1) and 264 B
+    (later stages)."""
+    h = (n // 2 + 1) / n
+    bfft = 96 + 384 * h * (d - 1)
+    return order * bfft + 408 + 264 * (order - 1)
+
+
+def cpu_baseline(seconds_hint=15.0):
+    """The NumPy oracle (the reference Python path restated) on a bounded
+    sample of the same dynamics: 3D 64^3, 2 RK3 steps, one core."""
+    import numpy as np
+    from oracle import cdd_oracle as O
+    from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
+    n, steps = 64, 2
+    b = SpectralGaussianField((n,) * 3, 6.0, 0.01, seed=0)
+    dyn = O.Dynamics(O.GLIDE_CLIMB, 0.9, 1.0, 0.3,
+                     O.Load("uniaxial", init_strain=float(b[8].mean()), uniaxial_rate=0.01))
+    t0 = time.perf_counter()
+    O.run(dyn, b, 0.0, steps, 0.25)
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "NumPy oracle, 3D 64^3 glide+climb Lambda=0.9 uniaxial load, 2 TVD-RK3 steps (%.1f s), "
+                      "single-threaded" % dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (independent realisations "
+              "would shard one per GPU with no communication); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from Plasticity.amd import lib as L
+    from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
+
+    n = args.n
+    cells = n ** 3
+    beta = SpectralGaussianField((n,) * 3, 6.0, 0.01, seed=0)
+    init_strain = float(beta[8].mean())
+    dev = torch.device("cuda", 0)
+    beta_t = torch.from_numpy(beta).to(dev)
+    del beta
+    # load ramp: strain 0.01 reached after ~1000 steps of the initial time step
+    probe = L.Plan(3, n, dynamics=L.GLIDE_CLIMB, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25)
+    _, speed = probe.flux(beta_t)
+    dt0 = 0.25 / (n * float(speed.max()))
+    probe.close()
+    del speed
+    rate = 0.01 / (1000.0 * dt0)
+    plan = L.Plan(3, n, dynamics=L.GLIDE_CLIMB, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25,
+                  load_kind=L.LOAD_UNIAXIAL, load_init_strain=init_strain, load_uniaxial_rate=rate)
+    plan.set_state(beta_t)
+    plan.set_time(0.0)
+    plan.step(args.warmup)
+    torch.cuda.synchronize()
+    plan.timing_enable(args.steps * 24 + 8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.step(args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kt = plan.timing_read()
+    plan.timing_enable(0)
+    t, dt, done, nsteps = plan.clock()
+    state = plan.get_state()
+    finite = bool(torch.isfinite(state).all())
+    elapsed = t1 - t0
+    ms_step = 1e3 * elapsed / args.steps
+    value = cells * args.steps / elapsed
+
+    per_cell = spectral_bytes(n)
+    kernels = {}
+    for k, (ms, cnt) in kt.items():
+        if cnt == 0:
+            continue
+        entry = {"total_ms": round(ms, 4), "launches": cnt, "avg_ms": round(ms / cnt, 5)}
+        if k in per_cell:
+            b = per_cell[k] * cells
+            entry["gbs"] = round(b / (ms / cnt * 1e-3) / 1e9, 1)
+        kernels[k] = entry
+    dom = max((k for k in kernels if k in per_cell), key=lambda k: kernels[k]["total_ms"])
+    dom_bytes = per_cell[dom] * cells
+    achieved = dom_bytes / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_3d256.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get(dom)
+        except Exception:
+            traffic = None
+    bpc = canonical_bytes_per_cell_update(n)
+    out = {
+        "metric": "cell-updates/s (3D 256^3 fp64 glide+climb TVD-RK3)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded Gaussian random beta^P (k-filter exp(-k^2 L^2/2), L=6 cells, RMS 0.01)",
+        "config": {"workload": "cdd-3d-256-glide-climb-rk3", "grid": [n, n, n], "dynamics": "glide+climb",
+                   "Lambda": 0.9, "rk_order": 3, "cfl": 0.25, "mu": 1.0, "nu": 0.3,
+                   "load": "uniaxial strain rate %.4g (0.01 after ~1000 steps)" % rate},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": dom_bytes},
+        "step_roofline": {"bytes_per_cell_update": bpc, "frac": round(bpc * value / (PEAK_HBM_GBS * 1e9), 4)},
+        "kernels": kernels,
+        "sim_time": t,
+        "last_dt": dt,
+        "state_finite": finite,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    if "OMP_NUM_THREADS" not in os.environ:
+        os.environ["OMP_NUM_THREADS"] = "1"
+    main()

@@ -291,7 +291,7 @@ static SpectralJob spectral_job(cdd_plan* p, int nc, const double* A, const doub
   j.n = p->n;
   j.nc = nc;
   j.cells = p->cells;
-  j.src = SrcField<double>{A, B, clk, p->cells};
+  j.src = SrcField<double>{A, B, clk, p->cells, nullptr};
   j.spec = p->spec;
   j.tw = p->tw;
   j.out = out;
@@ -371,13 +371,17 @@ int cdd_step(cdd_plan* p, int nsteps) {
     if (p->timing) timing_hook(p, KK_FLUX1);
     HIPCHK(launch_finalize_dt(p->partial, nb, p->clk, nmax, p->stream));
     if (p->timing) timing_hook(p, KK_FINALIZE);
-    // stage 2 on F0 = u + dt L0 (formed on the fly)
-    HIPCHK(launch_spectral(spectral_job(p, 6, u, p->L0, p->clk, 0.0, p->G)));
+    // stage 2 on F0 = u + dt L0: the R2C pass forms F0 while reading u and
+    // L0 for the stress solve and stores it over L0
+    {
+      SpectralJob j2 = spectral_job(p, 6, u, p->L0, p->clk, 0.0, p->G);
+      j2.src.W = p->L0;
+      HIPCHK(launch_spectral(j2));
+    }
     if (p->cfg.rk_order == 3) {
       FluxJob f2 = flux_job(p);
       f2.clk = p->clk;
-      f2.A = u;
-      f2.B = p->L0;
+      f2.A = p->L0;
       f2.U = u;
       f2.out = p->buf;
       f2.mode = 1;
@@ -400,8 +404,7 @@ int cdd_step(cdd_plan* p, int nsteps) {
     } else {
       FluxJob f2 = flux_job(p);
       f2.clk = p->clk;
-      f2.A = u;
-      f2.B = p->L0;
+      f2.A = p->L0;
       f2.U = u;
       f2.out = p->state[ci ^ 1];
       f2.mode = 1;

@@ -54,9 +54,13 @@ __device__ __forceinline__ void limited_derivs(const T f[5], T invDx, T halfDx, 
 
 // Per-cell flux.  u[a][s][c]: derivative along axis a, s = 0 ('+') or 1
 // ('-'), component c = 3 i + j.  sg: full (symmetric) stress.
+// Per-cell flux.  u[a][s][c]: derivative along axis a, s = 0 ('+') or 1
+// ('-'), component c = 3 i + j.  sg: full (symmetric) stress.  vbuf: this
+// thread's slot of an LDS array [4 * 2^D][blockDim] that parks the corner
+// velocities between the speed pass and the Hamiltonian pass.
 template <typename T, int D, int DYN>
 __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9], T Lambda, T (&rhs)[9],
-                                          T& speed) {
+                                          T& speed, T* __restrict__ vbuf, int vstride) {
   constexpr int NCOR = 1 << D;
   const T ME = T(kME);
   // per axis / sign quantities
@@ -67,7 +71,8 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
     for (int s = 0; s < 2; ++s) {
       const T* ua = u[a][s];
 #pragma unroll
-      for (int l = 0; l < 3; ++l) w[a][s][l] = ua[3 * l] * sg[3 * a] + ua[3 * l + 1] * sg[3 * a + 1] + ua[3 * l + 2] * sg[3 * a + 2];
+      for (int l = 0; l < 3; ++l)
+        w[a][s][l] = ua[3 * l] * sg[3 * a] + ua[3 * l + 1] * sg[3 * a + 1] + ua[3 * l + 2] * sg[3 * a + 2];
       T z = T(0);
 #pragma unroll
       for (int c = 0; c < 9; ++c) z += ua[c] * sg[c];
@@ -83,26 +88,10 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
       P[a][s] = p;
     }
   }
-  // |rho|^2 cross terms (BetaP3D_RhoMod, CentralUpwindHJBetaPDynamics3D.py:45-58)
-  T Cxy[2][2], Cyz[2][2], Czx[2][2];
-#pragma unroll
-  for (int s0 = 0; s0 < 2; ++s0)
-#pragma unroll
-    for (int s1 = 0; s1 < 2; ++s1) {
-      Cxy[s0][s1] = u[0][s0][3] * u[1][s1][0] + u[0][s0][4] * u[1][s1][1] + u[0][s0][5] * u[1][s1][2];
-      if constexpr (D == 3) {
-        Cyz[s0][s1] = u[1][s0][6] * u[2][s1][3] + u[1][s0][7] * u[2][s1][4] + u[1][s0][8] * u[2][s1][5];
-        Czx[s0][s1] = u[2][s0][0] * u[0][s1][6] + u[2][s0][1] * u[0][s1][7] + u[2][s0][2] * u[0][s1][8];
-      } else {
-        Cyz[s0][s1] = T(0);
-        Czx[s0][s1] = T(0);
-      }
-    }
   const T st = (DYN == DYN_GLIDE_CLIMB) ? Lambda * (sg[0] + sg[4] + sg[8]) / T(3) : T(0);
   const T jfac = -Lambda / T(3);
   const bool climb_sup = (DYN == DYN_GLIDE_CLIMB) && (Lambda != T(0));
 
-  T vc[NCOR][3], Jc[NCOR];
   T amax[3], amin[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -153,20 +142,32 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
 #pragma unroll
       for (int a = 0; a < 3; ++a) v0[a] += q[a] * f;
     }
+    // |rho|^2 (BetaP3D_RhoMod, CentralUpwindHJBetaPDynamics3D.py:45-58)
     T r2 = ME;
 #pragma unroll
     for (int a = 0; a < D; ++a) r2 += P[a][sgn[a]];
-    T cross = Cxy[sgn[0]][sgn[1]];
-    if constexpr (D == 3) cross += Cyz[sgn[1]][sgn[2]] + Czx[sgn[2]][sgn[0]];
+    T cross = T(0);
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+      cross += u[0][sgn[0]][3 + jj] * u[1][sgn[1]][jj];
+      if constexpr (D == 3) {
+        cross += u[1][sgn[1]][6 + jj] * u[2][sgn[2]][3 + jj];
+        cross += u[2][sgn[2]][jj] * u[0][sgn[0]][6 + jj];
+      }
+    }
     r2 -= T(2) * cross;
     const T rinv = T(1) / sqrt(r2);
+    T vv[3];
 #pragma unroll
-    for (int l = 0; l < 3; ++l) vc[c][l] = v0[l] * rinv;
-    Jc[c] = climb_sup ? jfac * (vc[c][0] * q[0] + vc[c][1] * q[1] + vc[c][2] * q[2]) : T(0);
+    for (int l = 0; l < 3; ++l) vv[l] = v0[l] * rinv;
+    const T J = climb_sup ? jfac * (vv[0] * q[0] + vv[1] * q[1] + vv[2] * q[2]) : T(0);
+#pragma unroll
+    for (int l = 0; l < 3; ++l) vbuf[(4 * c + l) * vstride] = vv[l];
+    vbuf[(4 * c + 3) * vstride] = J;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      amax[a] = fmax(amax[a], vc[c][a]);
-      amin[a] = fmin(amin[a], vc[c][a]);
+      amax[a] = fmax(amax[a], vv[a]);
+      amin[a] = fmin(amin[a], vv[a]);
     }
   }
   // local one-sided speeds (Max5/Max9 with 0, CentralUpwindHJ3D.py:114-241)
@@ -193,11 +194,14 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
       sgn[a] = (c >> (D - 1 - a)) & 1;
       wc *= sgn[a] == 0 ? am[a] : ap[a];
     }
+    T vv[3];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) vv[l] = vbuf[(4 * c + l) * vstride];
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
-      for (int l = 0; l < 3; ++l) Y[a][sgn[a]][l] += wc * vc[c][l];
-    Jsum += wc * Jc[c];
+      for (int l = 0; l < 3; ++l) Y[a][sgn[a]][l] += wc * vv[l];
+    if (climb_sup) Jsum += wc * vbuf[(4 * c + 3) * vstride];
   }
   T inv = T(1);
 #pragma unroll
@@ -223,7 +227,7 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
 #pragma unroll
         for (int s = 0; s < 2; ++s)
 #pragma unroll
-          for (int l = 0; l < 3; ++l) h -= u[i < D ? i : 0][s][3 * l + jj] * Y[i < D ? i : 0][s][l];
+          for (int l = 0; l < 3; ++l) h -= u[i % D][s][3 * l + jj] * Y[i % D][s][l];
       }
       T r = -h * inv;
 #pragma unroll
@@ -248,61 +252,103 @@ __device__ __forceinline__ T block_max(T v, T* red) {
   return m;
 }
 
-template <typename T, int D, int DYN>
-__global__ void __launch_bounds__(256) k_flux(FluxJob job) {
+// Block = TC x TR cells of the two fastest axes (TC contiguous); 3D blocks
+// sit at one x plane.  For each component the block stages the plane tile
+// with a 2-cell halo in LDS (double-buffered, prefetched into registers one
+// component ahead); the slow-axis (x) neighbours of 3D come straight from
+// global memory, coalesced along the contiguous axis.
+template <int TC, int TR>
+struct FluxTile {
+  static constexpr int PW = TC + 4, PH = TR + 4;  // padded tile
+  static constexpr int TILE = PW * PH;
+  static constexpr int NT = TC * TR;               // threads
+  static constexpr int FILL = (TILE + NT - 1) / NT; // fill loads per thread
+};
+
+template <typename T, int D, int DYN, int FTC, int FTR>
+__global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
+  using FT = FluxTile<FTC, FTR>;
+  constexpr int FPW = FT::PW, FTILE = FT::TILE, FNT = FT::NT, FFILL = FT::FILL;
   if (job.clk && job.clk->done) return;
-  __shared__ T red[4];
+  __shared__ T tile[2][FTILE];
+  __shared__ T vsh[4 * (1 << D)][FNT];
+  __shared__ T red[(FNT + 63) / 64];
   const int n = job.n;
   const long long cells = job.cells;
-  int cx_, cy_, cz_ = 0;
-  if constexpr (D == 3) {
-    cz_ = blockIdx.x * blockDim.x + threadIdx.x;
-    cy_ = blockIdx.y * blockDim.y + threadIdx.y;
-    cx_ = blockIdx.z;
-  } else {
-    cy_ = blockIdx.x * blockDim.x + threadIdx.x;
-    cx_ = blockIdx.y * blockDim.y + threadIdx.y;
-  }
+  const int tc = threadIdx.x, trr = threadIdx.y;
+  const int tid = trr * FTC + tc;
+  const int c0 = blockIdx.x * FTC, r0 = blockIdx.y * FTR;
+  const int px = (D == 3) ? blockIdx.z : 0;
+  // strides: col (contiguous) axis = last, row axis = second last
+  const long long scol = 1, srow = n, splane = (long long)n * n;
+  const long long pbase = (D == 3) ? (long long)px * splane : 0;
+  const int mycol = c0 + tc, myrow = r0 + trr;
+  const long long idx = pbase + (long long)myrow * srow + mycol;
   const T invDx = T(n);
   const T halfDx = T(0.5) * (T(1) / T(n));
   const T dt = job.clk ? T(job.clk->dt) : T(0);
-  const bool useB = job.B != nullptr;
+  const T* __restrict__ A = job.A;
 
-  int coord[3] = {cx_, cy_, cz_};
-  long long stride[3];
-  if constexpr (D == 3) {
-    stride[0] = (long long)n * n;
-    stride[1] = n;
-    stride[2] = 1;
-  } else {
-    stride[0] = n;
-    stride[1] = 1;
-    stride[2] = 0;
+  // tile fill coordinates (periodic wrap)
+  long long foff[FFILL];
+  bool fval[FFILL];
+#pragma unroll
+  for (int k = 0; k < FFILL; ++k) {
+    const int e = tid + k * FNT;
+    fval[k] = e < FTILE;
+    const int er = e / FPW, ec = e % FPW;
+    int gr = r0 + er - 2, gc = c0 + ec - 2;
+    gr = gr < 0 ? gr + n : (gr >= n ? gr - n : gr);
+    gc = gc < 0 ? gc + n : (gc >= n ? gc - n : gc);
+    foff[k] = pbase + (long long)gr * srow + gc;
   }
-  const long long idx = (long long)coord[0] * stride[0] + (long long)coord[1] * stride[1] + (D == 3 ? coord[2] : 0);
-
+  long long xoff[4] = {0, 0, 0, 0};
+  if constexpr (D == 3) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      int p = px + (k < 2 ? k - 2 : k - 1);
+      p = p < 0 ? p + n : (p >= n ? p - n : p);
+      xoff[k] = (long long)p * splane + (long long)myrow * srow + mycol;
+    }
+  }
+  T pf[FFILL], px4[4];
+  auto prefetch = [&](int c) {
+#pragma unroll
+    for (int k = 0; k < FFILL; ++k) pf[k] = fval[k] ? A[c * cells + foff[k]] : T(0);
+    if constexpr (D == 3) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) px4[k] = A[c * cells + xoff[k]];
+    }
+  };
   T u[D][2][9];
   T fc[9];
+  prefetch(0);
 #pragma unroll
-  for (int a = 0; a < D; ++a) {
-    long long off[5];
+  for (int c = 0; c < 9; ++c) {
+    T* tb = tile[c & 1];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      int p = coord[a] + k - 2;
-      p = p < 0 ? p + n : (p >= n ? p - n : p);
-      off[k] = idx + (long long)(p - coord[a]) * stride[a];
+    for (int k = 0; k < FFILL; ++k)
+      if (fval[k]) tb[tid + k * FNT] = pf[k];
+    T xs[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) xs[k] = px4[k];
+    if (c + 1 < 9) prefetch(c + 1);
+    __syncthreads();
+    const int lr = trr + 2, lc = tc + 2;
+    const T ctr = tb[lr * FPW + lc];
+    fc[c] = ctr;
+    {  // contiguous axis (last)
+      T f[5] = {tb[lr * FPW + lc - 2], tb[lr * FPW + lc - 1], ctr, tb[lr * FPW + lc + 1], tb[lr * FPW + lc + 2]};
+      limited_derivs<T>(f, invDx, halfDx, u[D - 1][0][c], u[D - 1][1][c]);
     }
-#pragma unroll
-    for (int c = 0; c < 9; ++c) {
-      T f[5];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        T vv = job.A[c * cells + off[k]];
-        if (useB) vv += dt * job.B[c * cells + off[k]];
-        f[k] = vv;
-      }
-      if (a == 0) fc[c] = f[2];
-      limited_derivs<T>(f, invDx, halfDx, u[a][0][c], u[a][1][c]);
+    {  // row axis (second last)
+      T f[5] = {tb[(lr - 2) * FPW + lc], tb[(lr - 1) * FPW + lc], ctr, tb[(lr + 1) * FPW + lc],
+                tb[(lr + 2) * FPW + lc]};
+      limited_derivs<T>(f, invDx, halfDx, u[D - 2][0][c], u[D - 2][1][c]);
+    }
+    if constexpr (D == 3) {
+      T f[5] = {xs[0], xs[1], ctr, xs[2], xs[3]};
+      limited_derivs<T>(f, invDx, halfDx, u[0][0][c], u[0][1][c]);
     }
   }
   // sigma = G - lam tr(beta) d - mu (beta + beta^T) at the cell
@@ -320,7 +366,7 @@ __global__ void __launch_bounds__(256) k_flux(FluxJob job) {
     sg[5] = sg[7] = g5 - mu * (fc[5] + fc[7]);
   }
   T rhs[9], speed;
-  cell_flux<T, D, DYN>(u, sg, T(job.Lambda), rhs, speed);
+  cell_flux<T, D, DYN>(u, sg, T(job.Lambda), rhs, speed, &vsh[0][tid], FNT);
   if (job.mode == 0) {
 #pragma unroll
     for (int c = 0; c < 9; ++c) job.out[c * cells + idx] = rhs[c];
@@ -332,18 +378,27 @@ __global__ void __launch_bounds__(256) k_flux(FluxJob job) {
   if (job.speed) job.speed[idx] = speed;
   if (job.partial) {
     T m = block_max<T>(speed, red);
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     if (tid == 0) job.partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = m;
   }
 }
 
+// Tile shapes: 64 x 4 cells for n >= 64; whole rows below that.
+static void flux_tile(int n, int& tc, int& tr) {
+  if (n >= 64) {
+    tc = 64;
+    tr = 4;
+  } else {
+    tc = n;
+    tr = 256 / n < n ? 256 / n : n;
+  }
+}
+
 static void flux_geometry(int d, int n, dim3& g, dim3& b) {
-  const int bx = n < 64 ? n : 64;
-  int by = 256 / bx;
-  if (by > n) by = n;
-  b = dim3(bx, by, 1);
-  if (d == 3) g = dim3(n / bx, n / by, n);
-  else g = dim3(n / bx, n / by, 1);
+  int tc, tr;
+  flux_tile(n, tc, tr);
+  b = dim3(tc, tr, 1);
+  if (d == 3) g = dim3(n / tc, n / tr, n);
+  else g = dim3(n / tc, n / tr, 1);
 }
 
 int flux_nblocks(int d, int n) {
@@ -352,16 +407,28 @@ int flux_nblocks(int d, int n) {
   return (int)(g.x * g.y * g.z);
 }
 
+template <int TC, int TR>
+static void launch_flux_t(const FluxJob& job, dim3 g, dim3 b) {
+  if (job.d == 3) {
+    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_ONLY, TC, TR>), g, b, 0, job.stream, job);
+    else hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_CLIMB, TC, TR>), g, b, 0, job.stream, job);
+  } else {
+    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_ONLY, TC, TR>), g, b, 0, job.stream, job);
+    else hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_CLIMB, TC, TR>), g, b, 0, job.stream, job);
+  }
+}
+
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
+  if (job.B) return hipErrorInvalidValue;  // stage inputs are materialised before the flux pass
   dim3 g, b;
   flux_geometry(job.d, job.n, g, b);
   if (nblocks_out) *nblocks_out = (int)(g.x * g.y * g.z);
-  if (job.d == 3) {
-    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_ONLY>), g, b, 0, job.stream, job);
-    else hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_CLIMB>), g, b, 0, job.stream, job);
-  } else {
-    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_ONLY>), g, b, 0, job.stream, job);
-    else hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_CLIMB>), g, b, 0, job.stream, job);
+  switch (b.x) {
+    case 64: launch_flux_t<64, 4>(job, g, b); break;
+    case 32: launch_flux_t<32, 8>(job, g, b); break;
+    case 16: launch_flux_t<16, 16>(job, g, b); break;
+    case 8: launch_flux_t<8, 8>(job, g, b); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

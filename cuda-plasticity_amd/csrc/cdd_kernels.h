@@ -12,6 +12,7 @@ struct SrcField {
   const T* B;
   const Clock* clk;
   long long cs;  // component stride (cells)
+  T* W;          // optional: the formed A + dt B is also stored here (may alias B)
 };
 
 struct GreenParams {

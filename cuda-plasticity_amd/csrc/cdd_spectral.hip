@@ -54,6 +54,7 @@ __device__ __forceinline__ T src_comp(const SrcField<T>& s, int c, long long idx
   auto b = [&](int k) -> T {
     T v = s.A[k * s.cs + idx];
     if (s.B) v += dt * s.B[k * s.cs + idx];
+    if (s.W) s.W[k * s.cs + idx] = v;  // every component is read exactly once per cell
     return v;
   };
   if constexpr (SYM) {
