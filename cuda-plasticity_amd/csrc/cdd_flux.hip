@@ -24,32 +24,42 @@
 //                       - [i<d] sum_s sum_l u_i^s[l,j] Y_i^s[l] + d_ij sum_c w_c J_c
 //   with Y_a^s[l] = sum over corners with s_a = s of w_c v_l(c).
 // The rounding differs from the reference at the 1e-16 level only.
+#include <stdlib.h>
+
 #include "cdd_kernels.h"
 
 namespace cdd {
 
 template <typename T>
 __device__ __forceinline__ T minmod3(T a, T b, T c) {
-  // NumericalMethods.py:367-378: min if all > 0, max if all <= 0, else 0
+  // NumericalMethods.py:367-378: min if all > 0, max if all <= 0, else 0;
+  // written as max(min, 0) + min(max, 0), which is exactly that (the other
+  // term is +0 in every case) without compares or selects.
   const T mn = fmin(a, fmin(b, c));
   const T mx = fmax(a, fmax(b, c));
-  return mn > T(0) ? mn : (mx <= T(0) ? mx : T(0));
+  return fmax(mn, T(0)) + fmin(mx, T(0));
 }
 
 // One-sided limited derivatives of f at i from f[i-2..i+2]
 // (FindDerivatives, CentralUpwindHJ3D.py:11-34, theta = 1, Dx = 1/N_last).
+// The reference scales the minmod arguments by 1/Dx^2 and the result by
+// Dx/2; with N a power of two these scalings are exact, commute with the
+// minmod, and the result is bit-identical to
+//   u+ = N (D_i - m_i / 2),  u- = N (D_{i-1} + m_{i-1} / 2),
+//   m_i = minmod(D_{i+1} - D_i, (D_{i+1} - D_{i-1}) / 2, D_i - D_{i-1}).
 template <typename T>
 __device__ __forceinline__ void limited_derivs(const T f[5], T invDx, T halfDx, T& up, T& um) {
+  (void)halfDx;
   const T Dm2 = f[1] - f[0], Dm1 = f[2] - f[1], D0 = f[3] - f[2], D1 = f[4] - f[3];
-  const T s1_i = (D1 - D0) * invDx * invDx;
-  const T s1_im1 = (D0 - Dm1) * invDx * invDx;
-  const T s1_im2 = (Dm1 - Dm2) * invDx * invDx;
-  const T s2_i = (D1 - Dm1) * invDx * invDx * T(0.5);
-  const T s2_im1 = (D0 - Dm2) * invDx * invDx * T(0.5);
-  const T S_i = minmod3(s1_i, s2_i, s1_im1);
-  const T S_im1 = minmod3(s1_im1, s2_im1, s1_im2);
-  up = D0 * invDx - halfDx * S_i;
-  um = Dm1 * invDx + halfDx * S_im1;
+  const T s1_i = D1 - D0;
+  const T s1_im1 = D0 - Dm1;
+  const T s1_im2 = Dm1 - Dm2;
+  const T s2_i = (D1 - Dm1) * T(0.5);
+  const T s2_im1 = (D0 - Dm2) * T(0.5);
+  const T m_i = minmod3(s1_i, s2_i, s1_im1);
+  const T m_im1 = minmod3(s1_im1, s2_im1, s1_im2);
+  up = invDx * fma(T(-0.5), m_i, D0);
+  um = invDx * fma(T(0.5), m_im1, Dm1);
 }
 
 // Per-cell flux.  u[a][s][c]: derivative along axis a, s = 0 ('+') or 1
@@ -58,7 +68,9 @@ __device__ __forceinline__ void limited_derivs(const T f[5], T invDx, T halfDx, 
 // ('-'), component c = 3 i + j.  sg: full (symmetric) stress.  vbuf: this
 // thread's slot of an LDS array [4 * 2^D][blockDim] that parks the corner
 // velocities between the speed pass and the Hamiltonian pass.
-template <typename T, int D, int DYN>
+// PRE: keep the per-axis velocity numerators w, z in registers (fewer
+// flops) or recompute them at every corner (fewer live registers).
+template <typename T, int D, int DYN, bool PRE>
 __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9], T Lambda, T (&rhs)[9],
                                           T& speed, T* __restrict__ vbuf, int vstride) {
   constexpr int NCOR = 1 << D;
@@ -70,13 +82,15 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
       const T* ua = u[a][s];
+      if constexpr (PRE) {
 #pragma unroll
-      for (int l = 0; l < 3; ++l)
-        w[a][s][l] = ua[3 * l] * sg[3 * a] + ua[3 * l + 1] * sg[3 * a + 1] + ua[3 * l + 2] * sg[3 * a + 2];
-      T z = T(0);
+        for (int l = 0; l < 3; ++l)
+          w[a][s][l] = ua[3 * l] * sg[3 * a] + ua[3 * l + 1] * sg[3 * a + 1] + ua[3 * l + 2] * sg[3 * a + 2];
+        T z = T(0);
 #pragma unroll
-      for (int c = 0; c < 9; ++c) z += ua[c] * sg[c];
-      zz[a][s] = z;
+        for (int c = 0; c < 9; ++c) z += ua[c] * sg[c];
+        zz[a][s] = z;
+      }
       tr[a][s] = ua[0] + ua[4] + ua[8];
       T p = T(0);
 #pragma unroll
@@ -110,13 +124,27 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
 #pragma unroll
       for (int a = 0; a < D; ++a) {
         r += u[a][sgn[a]][3 * l + a];
-        vv += w[a][sgn[a]][l];
+        if constexpr (PRE) {
+          vv += w[a][sgn[a]][l];
+        } else {
+#pragma unroll
+          for (int nn = 0; nn < 3; ++nn) vv += u[a][sgn[a]][3 * l + nn] * sg[3 * a + nn];
+        }
       }
       R[l] = r;
       v0[l] = vv;
     }
 #pragma unroll
-    for (int a = 0; a < D; ++a) v0[a] -= zz[a][sgn[a]];
+    for (int a = 0; a < D; ++a) {
+      if constexpr (PRE) {
+        v0[a] -= zz[a][sgn[a]];
+      } else {
+        T z = T(0);
+#pragma unroll
+        for (int cc = 0; cc < 9; ++cc) z += u[a][sgn[a]][cc] * sg[cc];
+        v0[a] -= z;
+      }
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       // q_a = tr(u_a) - sum_b u_b[a, b]; tr(u_a) = 0 for a >= d
@@ -253,10 +281,10 @@ __device__ __forceinline__ T block_max(T v, T* red) {
 }
 
 // Block = TC x TR cells of the two fastest axes (TC contiguous); 3D blocks
-// sit at one x plane.  For each component the block stages the plane tile
-// with a 2-cell halo in LDS (double-buffered, prefetched into registers one
-// component ahead); the slow-axis (x) neighbours of 3D come straight from
-// global memory, coalesced along the contiguous axis.
+// sit at one x plane.  The block loads, in one batch, the plane tile of all
+// nine components with a 2-cell halo (into LDS) and, in 3D, each thread's
+// four slow-axis (x) neighbours (into registers, coalesced along the
+// contiguous axis) -- one global-memory round trip per block.
 template <int TC, int TR>
 struct FluxTile {
   static constexpr int PW = TC + 4, PH = TR + 4;  // padded tile
@@ -265,14 +293,17 @@ struct FluxTile {
   static constexpr int FILL = (TILE + NT - 1) / NT; // fill loads per thread
 };
 
-template <typename T, int D, int DYN, int FTC, int FTR>
+template <typename T, int D, int DYN, int FTC, int FTR, bool PRE>
 __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   using FT = FluxTile<FTC, FTR>;
   constexpr int FPW = FT::PW, FTILE = FT::TILE, FNT = FT::NT, FFILL = FT::FILL;
   if (job.clk && job.clk->done) return;
-  __shared__ T tile[2][FTILE];
+  __shared__ T tile[9][FTILE];
   __shared__ T vsh[4 * (1 << D)][FNT];
   __shared__ T red[(FNT + 63) / 64];
+  // 3D: per-wave LDS landing zone for G (planes 0-5) and U (planes 6-15),
+  // filled by asynchronous global->LDS loads issued at kernel start
+  __shared__ T ug[(D == 3) ? FNT / 64 : 1][16][64];
   const int n = job.n;
   const long long cells = job.cells;
   const int tc = threadIdx.x, trr = threadIdx.y;
@@ -280,7 +311,7 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   const int c0 = blockIdx.x * FTC, r0 = blockIdx.y * FTR;
   const int px = (D == 3) ? blockIdx.z : 0;
   // strides: col (contiguous) axis = last, row axis = second last
-  const long long scol = 1, srow = n, splane = (long long)n * n;
+  const long long srow = n, splane = (long long)n * n;
   const long long pbase = (D == 3) ? (long long)px * splane : 0;
   const int mycol = c0 + tc, myrow = r0 + trr;
   const long long idx = pbase + (long long)myrow * srow + mycol;
@@ -289,7 +320,9 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   const T dt = job.clk ? T(job.clk->dt) : T(0);
   const T* __restrict__ A = job.A;
 
-  // tile fill coordinates (periodic wrap)
+  // One batch of loads for every component (a single memory round trip):
+  // the plane tile with its 2-cell halo (periodic wrap) and, in 3D, the
+  // four slow-axis neighbours of this thread's cell.
   long long foff[FFILL];
   bool fval[FFILL];
 #pragma unroll
@@ -302,39 +335,61 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
     gc = gc < 0 ? gc + n : (gc >= n ? gc - n : gc);
     foff[k] = pbase + (long long)gr * srow + gc;
   }
-  long long xoff[4] = {0, 0, 0, 0};
-  if constexpr (D == 3) {
+  constexpr int NX = (D == 3) ? 4 : 1;
+  long long xoff[NX];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      int p = px + (k < 2 ? k - 2 : k - 1);
-      p = p < 0 ? p + n : (p >= n ? p - n : p);
-      xoff[k] = (long long)p * splane + (long long)myrow * srow + mycol;
+  for (int k = 0; k < NX; ++k) {
+    int p = px + (k < 2 ? k - 2 : k - 1);
+    p = p < 0 ? p + n : (p >= n ? p - n : p);
+    xoff[k] = (long long)p * splane + (long long)myrow * srow + mycol;
+  }
+  if constexpr (D == 3) {
+    // Each wave's 64 cells are contiguous in memory (TC = 64, or TC = n);
+    // every lane moves 16 B (two cells) straight into LDS, no VGPRs used.
+    const int wv = tid >> 6, lane = tid & 63;
+    const long long cell0 = pbase + (long long)(r0 + (wv * 64) / FTC) * srow + c0 + (wv * 64) % FTC;
+    const int half = lane >> 5, pair = 2 * (lane & 31);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const T* src = job.G + (2 * k + half) * cells + cell0 + pair;
+      __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)&ug[wv][2 * k][0], 16,
+                                       0, 0);
+    }
+    if (job.mode != 0) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        int c = 2 * k + half;
+        c = c < 9 ? c : 8;  // the spare 16th plane receives a copy of plane 8
+        const T* src = job.U + c * cells + cell0 + pair;
+        __builtin_amdgcn_global_load_lds((const void*)src,
+                                         (__attribute__((address_space(3))) void*)&ug[wv][6 + 2 * k][0], 16, 0, 0);
+      }
     }
   }
-  T pf[FFILL], px4[4];
-  auto prefetch = [&](int c) {
-#pragma unroll
-    for (int k = 0; k < FFILL; ++k) pf[k] = fval[k] ? A[c * cells + foff[k]] : T(0);
-    if constexpr (D == 3) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) px4[k] = A[c * cells + xoff[k]];
-    }
-  };
-  T u[D][2][9];
-  T fc[9];
-  prefetch(0);
+  T tf[9][FFILL];
+  T xn[9][NX];
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
-    T* tb = tile[c & 1];
+#pragma unroll
+    for (int k = 0; k < FFILL; ++k) tf[c][k] = fval[k] ? A[c * cells + foff[k]] : T(0);
+    if constexpr (D == 3) {
+#pragma unroll
+      for (int k = 0; k < NX; ++k) xn[c][k] = A[c * cells + xoff[k]];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 9; ++c)
 #pragma unroll
     for (int k = 0; k < FFILL; ++k)
-      if (fval[k]) tb[tid + k * FNT] = pf[k];
-    T xs[4];
+      if (fval[k]) tile[c][tid + k * FNT] = tf[c][k];
+  __syncthreads();
+
+  T u[D][2][9];
+  T fc[9];
+  const int lr = trr + 2, lc = tc + 2;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) xs[k] = px4[k];
-    if (c + 1 < 9) prefetch(c + 1);
-    __syncthreads();
-    const int lr = trr + 2, lc = tc + 2;
+  for (int c = 0; c < 9; ++c) {
+    const T* tb = tile[c];
     const T ctr = tb[lr * FPW + lc];
     fc[c] = ctr;
     {  // contiguous axis (last)
@@ -347,16 +402,26 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
       limited_derivs<T>(f, invDx, halfDx, u[D - 2][0][c], u[D - 2][1][c]);
     }
     if constexpr (D == 3) {
-      T f[5] = {xs[0], xs[1], ctr, xs[2], xs[3]};
+      T f[5] = {xn[c][0], xn[c][1], ctr, xn[c][2], xn[c][3 % NX]};
       limited_derivs<T>(f, invDx, halfDx, u[0][0][c], u[0][1][c]);
     }
   }
+  // the asynchronous G/U loads must have landed before LDS is read
+  if constexpr (D == 3) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  auto Gv = [&](int c) -> T {
+    if constexpr (D == 3) return ug[tid >> 6][c][tid & 63];
+    else return job.G[c * cells + idx];
+  };
+  auto Uv = [&](int c) -> T {
+    if constexpr (D == 3) return ug[tid >> 6][6 + c][tid & 63];
+    else return job.U[c * cells + idx];
+  };
   // sigma = G - lam tr(beta) d - mu (beta + beta^T) at the cell
   T sg[9];
   {
     const T tr = fc[0] + fc[4] + fc[8];
-    const T g0 = job.G[0 * cells + idx], g1 = job.G[1 * cells + idx], g2 = job.G[2 * cells + idx];
-    const T g3 = job.G[3 * cells + idx], g4 = job.G[4 * cells + idx], g5 = job.G[5 * cells + idx];
+    const T g0 = Gv(0), g1 = Gv(1), g2 = Gv(2);
+    const T g3 = Gv(3), g4 = Gv(4), g5 = Gv(5);
     const T mu = T(job.mu), lam = T(job.lam);
     sg[0] = g0 - lam * tr - mu * (fc[0] + fc[0]);
     sg[4] = g1 - lam * tr - mu * (fc[4] + fc[4]);
@@ -366,14 +431,14 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
     sg[5] = sg[7] = g5 - mu * (fc[5] + fc[7]);
   }
   T rhs[9], speed;
-  cell_flux<T, D, DYN>(u, sg, T(job.Lambda), rhs, speed, &vsh[0][tid], FNT);
+  cell_flux<T, D, DYN, PRE>(u, sg, T(job.Lambda), rhs, speed, &vsh[0][tid], FNT);
   if (job.mode == 0) {
 #pragma unroll
     for (int c = 0; c < 9; ++c) job.out[c * cells + idx] = rhs[c];
   } else {
     const T cu = T(job.cu), cf = T(job.cf);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) job.out[c * cells + idx] = cu * job.U[c * cells + idx] + cf * (fc[c] + dt * rhs[c]);
+    for (int c = 0; c < 9; ++c) job.out[c * cells + idx] = cu * Uv(c) + cf * (fc[c] + dt * rhs[c]);
   }
   if (job.speed) job.speed[idx] = speed;
   if (job.partial) {
@@ -407,15 +472,24 @@ int flux_nblocks(int d, int n) {
   return (int)(g.x * g.y * g.z);
 }
 
-template <int TC, int TR>
+template <int TC, int TR, bool PRE>
 static void launch_flux_t(const FluxJob& job, dim3 g, dim3 b) {
   if (job.d == 3) {
-    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_ONLY, TC, TR>), g, b, 0, job.stream, job);
-    else hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_CLIMB, TC, TR>), g, b, 0, job.stream, job);
+    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_ONLY, TC, TR, PRE>), g, b, 0, job.stream, job);
+    else hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_CLIMB, TC, TR, PRE>), g, b, 0, job.stream, job);
   } else {
-    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_ONLY, TC, TR>), g, b, 0, job.stream, job);
-    else hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_CLIMB, TC, TR>), g, b, 0, job.stream, job);
+    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_ONLY, TC, TR, PRE>), g, b, 0, job.stream, job);
+    else hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_CLIMB, TC, TR, PRE>), g, b, 0, job.stream, job);
   }
+}
+
+static int flux_variant() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("CDD_FLUX_VARIANT");
+    v = e ? atoi(e) : 0;
+  }
+  return v;
 }
 
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
@@ -423,11 +497,15 @@ hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
   dim3 g, b;
   flux_geometry(job.d, job.n, g, b);
   if (nblocks_out) *nblocks_out = (int)(g.x * g.y * g.z);
+  const bool pre = flux_variant() == 0;
   switch (b.x) {
-    case 64: launch_flux_t<64, 4>(job, g, b); break;
-    case 32: launch_flux_t<32, 8>(job, g, b); break;
-    case 16: launch_flux_t<16, 16>(job, g, b); break;
-    case 8: launch_flux_t<8, 8>(job, g, b); break;
+    case 64:
+      if (pre) launch_flux_t<64, 4, true>(job, g, b);
+      else launch_flux_t<64, 4, false>(job, g, b);
+      break;
+    case 32: launch_flux_t<32, 8, true>(job, g, b); break;
+    case 16: launch_flux_t<16, 16, true>(job, g, b); break;
+    case 8: launch_flux_t<8, 8, true>(job, g, b); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
