@@ -75,7 +75,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(args.copy_calibration * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 8, "launches": 3}))
+        return
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (independent realisations "
               "would shard one per GPU with no communication); run with --gpus 1", file=sys.stderr)

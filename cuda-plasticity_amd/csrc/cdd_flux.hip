@@ -24,8 +24,6 @@
 //                       - [i<d] sum_s sum_l u_i^s[l,j] Y_i^s[l] + d_ij sum_c w_c J_c
 //   with Y_a^s[l] = sum over corners with s_a = s of w_c v_l(c).
 // The rounding differs from the reference at the 1e-16 level only.
-#include <stdlib.h>
-
 #include "cdd_kernels.h"
 
 namespace cdd {
@@ -308,8 +306,25 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   const long long cells = job.cells;
   const int tc = threadIdx.x, trr = threadIdx.y;
   const int tid = trr * FTC + tc;
-  const int c0 = blockIdx.x * FTC, r0 = blockIdx.y * FTR;
-  const int px = (D == 3) ? blockIdx.z : 0;
+  // XCD-aware tile order.  Workgroups are dispatched round-robin over the
+  // 8 XCDs (hardware block b runs on XCD b % 8); logical tile
+  // L = (b % 8) * (nb / 8) + b / 8 gives every XCD one contiguous range of
+  // L, and L runs x-fastest, so an XCD's CUs work on consecutive x planes
+  // of the same (y, z) column at once and the slow-axis neighbour planes
+  // are served from that XCD's L2.
+  const int nct = n / FTC, nrt = n / FTR;
+  const int nb = (D == 3 ? n : 1) * nct * nrt;
+  const int bl = blockIdx.x;
+  const int L = (nb % 8 == 0) ? (bl % 8) * (nb / 8) + bl / 8 : bl;
+  int px, tyz;
+  if constexpr (D == 3) {
+    px = L % n;
+    tyz = L / n;
+  } else {
+    px = 0;
+    tyz = L;
+  }
+  const int c0 = (tyz % nct) * FTC, r0 = (tyz / nct) * FTR;
   // strides: col (contiguous) axis = last, row axis = second last
   const long long srow = n, splane = (long long)n * n;
   const long long pbase = (D == 3) ? (long long)px * splane : 0;
@@ -346,12 +361,14 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   if constexpr (D == 3) {
     // Each wave's 64 cells are contiguous in memory (TC = 64, or TC = n);
     // every lane moves 16 B (two cells) straight into LDS, no VGPRs used.
+    // lane l moves cells (2 l', 2 l' + 1), l' = l mod 32, of the wave's 64
+    // cells (tid order) for plane 2k + l / 32; rows of FTC cells, FTC even
     const int wv = tid >> 6, lane = tid & 63;
-    const long long cell0 = pbase + (long long)(r0 + (wv * 64) / FTC) * srow + c0 + (wv * 64) % FTC;
-    const int half = lane >> 5, pair = 2 * (lane & 31);
+    const int half = lane >> 5, e = 64 * wv + 2 * (lane & 31);
+    const long long cellp = pbase + (long long)(r0 + e / FTC) * srow + c0 + e % FTC;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const T* src = job.G + (2 * k + half) * cells + cell0 + pair;
+      const T* src = job.G + (2 * k + half) * cells + cellp;
       __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)&ug[wv][2 * k][0], 16,
                                        0, 0);
     }
@@ -360,7 +377,7 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
       for (int k = 0; k < 5; ++k) {
         int c = 2 * k + half;
         c = c < 9 ? c : 8;  // the spare 16th plane receives a copy of plane 8
-        const T* src = job.U + c * cells + cell0 + pair;
+        const T* src = job.U + c * cells + cellp;
         __builtin_amdgcn_global_load_lds((const void*)src,
                                          (__attribute__((address_space(3))) void*)&ug[wv][6 + 2 * k][0], 16, 0, 0);
       }
@@ -443,15 +460,16 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   if (job.speed) job.speed[idx] = speed;
   if (job.partial) {
     T m = block_max<T>(speed, red);
-    if (tid == 0) job.partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = m;
+    if (tid == 0) job.partial[blockIdx.x] = m;
   }
 }
 
-// Tile shapes: 64 x 4 cells for n >= 64; whole rows below that.
+// Tile shapes: 32 x 8 cells (contiguous x row axis) for n >= 32; whole rows
+// below that.  A 1D grid; the kernel maps block ids to tiles.
 static void flux_tile(int n, int& tc, int& tr) {
-  if (n >= 64) {
-    tc = 64;
-    tr = 4;
+  if (n >= 32) {
+    tc = 32;
+    tr = 8;
   } else {
     tc = n;
     tr = 256 / n < n ? 256 / n : n;
@@ -462,8 +480,7 @@ static void flux_geometry(int d, int n, dim3& g, dim3& b) {
   int tc, tr;
   flux_tile(n, tc, tr);
   b = dim3(tc, tr, 1);
-  if (d == 3) g = dim3(n / tc, n / tr, n);
-  else g = dim3(n / tc, n / tr, 1);
+  g = dim3((d == 3 ? n : 1) * (n / tc) * (n / tr), 1, 1);
 }
 
 int flux_nblocks(int d, int n) {
@@ -472,24 +489,15 @@ int flux_nblocks(int d, int n) {
   return (int)(g.x * g.y * g.z);
 }
 
-template <int TC, int TR, bool PRE>
+template <int TC, int TR>
 static void launch_flux_t(const FluxJob& job, dim3 g, dim3 b) {
   if (job.d == 3) {
-    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_ONLY, TC, TR, PRE>), g, b, 0, job.stream, job);
-    else hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_CLIMB, TC, TR, PRE>), g, b, 0, job.stream, job);
+    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_ONLY, TC, TR, true>), g, b, 0, job.stream, job);
+    else hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_CLIMB, TC, TR, true>), g, b, 0, job.stream, job);
   } else {
-    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_ONLY, TC, TR, PRE>), g, b, 0, job.stream, job);
-    else hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_CLIMB, TC, TR, PRE>), g, b, 0, job.stream, job);
+    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_ONLY, TC, TR, true>), g, b, 0, job.stream, job);
+    else hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_CLIMB, TC, TR, true>), g, b, 0, job.stream, job);
   }
-}
-
-static int flux_variant() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = getenv("CDD_FLUX_VARIANT");
-    v = e ? atoi(e) : 0;
-  }
-  return v;
 }
 
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
@@ -497,15 +505,10 @@ hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
   dim3 g, b;
   flux_geometry(job.d, job.n, g, b);
   if (nblocks_out) *nblocks_out = (int)(g.x * g.y * g.z);
-  const bool pre = flux_variant() == 0;
   switch (b.x) {
-    case 64:
-      if (pre) launch_flux_t<64, 4, true>(job, g, b);
-      else launch_flux_t<64, 4, false>(job, g, b);
-      break;
-    case 32: launch_flux_t<32, 8, true>(job, g, b); break;
-    case 16: launch_flux_t<16, 16, true>(job, g, b); break;
-    case 8: launch_flux_t<8, 8, true>(job, g, b); break;
+    case 32: launch_flux_t<32, 8>(job, g, b); break;
+    case 16: launch_flux_t<16, 16>(job, g, b); break;
+    case 8: launch_flux_t<8, 8>(job, g, b); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
