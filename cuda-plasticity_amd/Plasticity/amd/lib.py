@@ -43,6 +43,9 @@ EXPORTS = [
     "cdd_plan_create", "cdd_plan_destroy", "cdd_set_stream", "cdd_set_state", "cdd_get_state",
     "cdd_set_time", "cdd_get_clock", "cdd_sigma", "cdd_rho", "cdd_flux", "cdd_step", "cdd_energy",
     "cdd_state_ptr", "cdd_last_error", "cdd_timing_enable", "cdd_timing_read",
+    "cdd_batch_create", "cdd_batch_destroy", "cdd_batch_set_stream", "cdd_batch_size", "cdd_batch_set_state",
+    "cdd_batch_get_state", "cdd_batch_set_time", "cdd_batch_get_clock", "cdd_batch_step", "cdd_batch_set_tile",
+    "cdd_batch_time_flux",
 ]
 KERNEL_KINDS = ["r2c", "cols_fwd", "contract", "cols_inv", "c2r", "flux_stage1", "cfl_reduce",
                 "flux_stage2", "flux_stage3", "advance"]
@@ -72,6 +75,17 @@ def _load():
         "cdd_state_ptr": ([P, ctypes.POINTER(P)], I),
         "cdd_last_error": ([], ctypes.c_char_p),
         "cdd_timing_enable": ([P, I], I),
+        "cdd_batch_create": ([ctypes.POINTER(Config), ctypes.POINTER(I), I, ctypes.POINTER(P)], I),
+        "cdd_batch_destroy": ([P], I),
+        "cdd_batch_set_stream": ([P, P], I),
+        "cdd_batch_size": ([P, I, ctypes.POINTER(I)], I),
+        "cdd_batch_set_state": ([P, I, P, I], I),
+        "cdd_batch_get_state": ([P, I, P, I], I),
+        "cdd_batch_set_time": ([P, I, D, D, D], I),
+        "cdd_batch_get_clock": ([P, I, dp, dp, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_longlong)], I),
+        "cdd_batch_step": ([P, I], I),
+        "cdd_batch_set_tile": ([P, I, I], I),
+        "cdd_batch_time_flux": ([P, I, I, I, dp], I),
         "cdd_timing_read": ([P, dp, ctypes.POINTER(I), I], I),
     }
     for name, (args, res) in sig.items():
@@ -216,3 +230,74 @@ class Plan:
         if tuple(t.shape) != (9,) + self.shape:
             raise CDDError("expected shape %s, got %s" % ((9,) + self.shape, tuple(t.shape)))
         return t
+
+
+class Batch:
+    """Many independent 2D realisations of uneven sides advanced together
+    (one hipGraph-captured step per launch, per-instance CFL clocks)."""
+
+    def __init__(self, sizes, dynamics=GLIDE_ONLY, rk_order=2, Lambda=0.0, mu=1.0, nu=0.3, cfl=0.25, device=0):
+        import torch
+        self.torch = torch
+        cfg = Config()
+        cfg.ndim, cfg.n, cfg.dynamics, cfg.rk_order = 2, 0, dynamics, rk_order
+        cfg.Lambda, cfg.mu, cfg.nu, cfg.cfl, cfg.device = Lambda, mu, nu, cfl, device
+        self.sizes = [int(s) for s in sizes]
+        arr = (ctypes.c_int * len(self.sizes))(*self.sizes)
+        h = ctypes.c_void_p()
+        check(lib.cdd_batch_create(ctypes.byref(cfg), arr, len(self.sizes), ctypes.byref(h)))
+        self.h = h
+        self.dev = torch.device("cuda", device)
+        s = torch.cuda.current_stream(self.dev).cuda_stream
+        check(lib.cdd_batch_set_stream(self.h, ctypes.c_void_p(s)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.cdd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def set_state(self, i, beta):
+        torch = self.torch
+        n = self.sizes[i]
+        if isinstance(beta, np.ndarray):
+            beta = torch.from_numpy(np.ascontiguousarray(beta, dtype=np.float64))
+        t = beta.to(device=self.dev, dtype=torch.float64).contiguous()
+        if tuple(t.shape) != (9, n, n):
+            raise CDDError("instance %d expects shape %s" % (i, (9, n, n)))
+        check(lib.cdd_batch_set_state(self.h, i, _ptr(t), 1))
+
+    def get_state(self, i):
+        n = self.sizes[i]
+        out = self.torch.empty((9, n, n), dtype=self.torch.float64, device=self.dev)
+        check(lib.cdd_batch_get_state(self.h, i, _ptr(out), 1))
+        return out
+
+    def set_time(self, i, t, t_end=float("inf"), fixed_dt=0.0):
+        check(lib.cdd_batch_set_time(self.h, i, t, t_end, fixed_dt))
+
+    def clock(self, i):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        done, ns = ctypes.c_int(), ctypes.c_longlong()
+        check(lib.cdd_batch_get_clock(self.h, i, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(done),
+                                      ctypes.byref(ns)))
+        return t.value, dt.value, bool(done.value), ns.value
+
+    def step(self, nsteps=1):
+        check(lib.cdd_batch_step(self.h, int(nsteps)))
+
+    def set_tile(self, n, variant):
+        check(lib.cdd_batch_set_tile(self.h, int(n), int(variant)))
+
+    def time_flux(self, n, variant, reps=5):
+        ms = ctypes.c_double()
+        check(lib.cdd_batch_time_flux(self.h, int(n), int(variant), int(reps), ctypes.byref(ms)))
+        return ms.value

@@ -1,0 +1,442 @@
+// Batched stepping of many independent 2D realisations of uneven sizes
+// (include/cdd.h, cdd_batch_*).  The reference runs one realisation per
+// process per GPU (Plasticity/exec_batch.py:26-58); here all of them advance
+// together: instances are grouped by grid side n, every group's buffers have
+// uniform per-instance strides so each kernel launch covers all instances of
+// a size (instance = blockIdx.y), each instance keeps its own device clock
+// (own CFL dt and end time), and one full step of every group is captured
+// once into a hipGraph -- each size group on its own forked capture stream,
+// so the groups are independent branches that run concurrently -- and
+// replayed for every step.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/cdd.h"
+#include "cdd_kernels.h"
+
+using namespace cdd;
+
+extern int cdd_internal_fail(int code, const char* msg);
+
+#define BCHK(expr)                                                                \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      char b_[256];                                                               \
+      snprintf(b_, sizeof b_, "%s failed: %s", #expr, hipGetErrorString(e_));     \
+      return cdd_internal_fail(CDD_ERR_HIP, b_);                                  \
+    }                                                                             \
+  } while (0)
+
+namespace {
+
+struct Group {
+  int n = 0, count = 0, tile = 0, nblocks = 0;
+  long long cells = 0, fstride = 0, gstride = 0, sstride = 0;
+  double* state[2] = {nullptr, nullptr};
+  double* L0 = nullptr;
+  double* buf = nullptr;
+  double* G = nullptr;
+  double* partial = nullptr;
+  void* spec = nullptr;
+  void* tw = nullptr;
+  Clock* clk = nullptr;
+  hipStream_t st = nullptr;
+};
+
+}  // namespace
+
+struct cdd_batch {
+  cdd_config cfg;
+  GreenParams gp0;  // mu/nu/lam/A/C (norm per group)
+  std::vector<Group> groups;
+  std::vector<int> inst_group, inst_slot;
+  std::vector<long long> set_global, set_nsteps;  // ping-pong bookkeeping (RK2)
+  long long global_steps = 0;
+  hipStream_t own = nullptr, stream = nullptr;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  std::vector<hipEvent_t> ev;
+};
+
+static void free_graphs(cdd_batch* b) {
+  for (int k = 0; k < 2; ++k) {
+    if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
+    if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+    b->exec[k] = nullptr;
+    b->graph[k] = nullptr;
+  }
+}
+
+// One full RK step of group g reading state[phase] (RK2 writes state[phase^1]).
+static hipError_t group_step(cdd_batch* b, Group& g, int phase, hipStream_t st) {
+  hipError_t e;
+  const int rk = b->cfg.rk_order;
+  double* u = g.state[rk == 3 ? 0 : phase];
+  auto spectral = [&](const double* A, const double* B, double* W) -> hipError_t {
+    SpectralJob j;
+    memset(&j, 0, sizeof j);
+    j.d = 2;
+    j.n = g.n;
+    j.nc = 6;
+    j.cells = g.cells;
+    j.src = SrcField<double>{A, B, g.clk, g.cells, W, g.fstride};
+    j.spec = g.spec;
+    j.tw = g.tw;
+    j.out = g.G;
+    j.gp = b->gp0;
+    j.gp.norm = 1.0 / (double)g.cells;
+    j.gp.clk = g.clk;
+    j.stream = st;
+    j.count = g.count;
+    j.sstride = g.sstride;
+    j.ostride = g.gstride;
+    return launch_spectral(j);
+  };
+  auto flux = [&](const double* A, const double* U, double* out, int mode, double cu, double cf,
+                  double* partial) -> hipError_t {
+    FluxJob f;
+    memset(&f, 0, sizeof f);
+    f.d = 2;
+    f.n = g.n;
+    f.dyn = b->cfg.dynamics;
+    f.cells = g.cells;
+    f.A = A;
+    f.G = g.G;
+    f.U = U;
+    f.out = out;
+    f.partial = partial;
+    f.clk = g.clk;
+    f.mode = mode;
+    f.cu = cu;
+    f.cf = cf;
+    f.Lambda = b->cfg.dynamics == CDD_GLIDE_ONLY ? 0.0 : b->cfg.Lambda;
+    f.mu = b->gp0.mu;
+    f.lam = b->gp0.lam;
+    f.stream = st;
+    f.count = g.count;
+    f.fstride = g.fstride;
+    f.gstride = g.gstride;
+    f.pstride = g.nblocks;
+    f.tile = g.tile;
+    return launch_flux(f, nullptr);
+  };
+  if ((e = spectral(u, nullptr, nullptr)) != hipSuccess) return e;
+  if ((e = flux(u, nullptr, g.L0, 0, 0.0, 0.0, g.partial)) != hipSuccess) return e;
+  if ((e = launch_finalize_dt(g.partial, g.nblocks, g.clk, (double)g.n, st, g.count, g.nblocks)) != hipSuccess)
+    return e;
+  if ((e = spectral(u, g.L0, g.L0)) != hipSuccess) return e;  // forms F0 = u + dt L0 over L0
+  if (rk == 3) {
+    if ((e = flux(g.L0, u, g.buf, 1, 0.75, 0.25, nullptr)) != hipSuccess) return e;
+    if ((e = spectral(g.buf, nullptr, nullptr)) != hipSuccess) return e;
+    if ((e = flux(g.buf, u, u, 1, 1.0 / 3.0, 2.0 / 3.0, nullptr)) != hipSuccess) return e;
+  } else {
+    if ((e = flux(g.L0, u, g.state[phase ^ 1], 1, 0.5, 0.5, nullptr)) != hipSuccess) return e;
+  }
+  return launch_advance(g.clk, st, g.count);
+}
+
+// Capture one step of every group into graph[phase]: fork one capture
+// stream per group from the batch stream, join them back.
+static int capture(cdd_batch* b, int phase) {
+  // capture on the private stream (the caller's stream may be the null
+  // stream, which cannot be captured); the graph is launched on the caller's
+  hipStream_t main = b->own;
+  BCHK(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
+  hipEvent_t fork = b->ev[0];
+  hipError_t err = hipEventRecord(fork, main);
+  for (size_t gi = 0; gi < b->groups.size() && err == hipSuccess; ++gi) {
+    Group& g = b->groups[gi];
+    err = hipStreamWaitEvent(g.st, fork, 0);
+    if (err == hipSuccess) err = group_step(b, g, phase, g.st);
+    if (err == hipSuccess) err = hipEventRecord(b->ev[1 + gi], g.st);
+    if (err == hipSuccess) err = hipStreamWaitEvent(main, b->ev[1 + gi], 0);
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t e2 = hipStreamEndCapture(main, &graph);
+  if (err != hipSuccess || e2 != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    char m[160];
+    snprintf(m, sizeof m, "graph capture failed: %s", hipGetErrorString(err != hipSuccess ? err : e2));
+    return cdd_internal_fail(CDD_ERR_HIP, m);
+  }
+  b->graph[phase] = graph;
+  BCHK(hipGraphInstantiate(&b->exec[phase], graph, nullptr, nullptr, 0));
+  return CDD_OK;
+}
+
+static int sync_clocks(cdd_batch* b, std::vector<Clock>& out) {
+  BCHK(hipStreamSynchronize(b->stream));
+  out.resize(b->inst_group.size());
+  for (size_t i = 0; i < b->inst_group.size(); ++i) {
+    Group& g = b->groups[b->inst_group[i]];
+    BCHK(hipMemcpy(&out[i], g.clk + b->inst_slot[i], sizeof(Clock), hipMemcpyDeviceToHost));
+  }
+  return CDD_OK;
+}
+
+// buffer holding instance i's state
+static double* inst_state(cdd_batch* b, int i, const Clock& c) {
+  Group& g = b->groups[b->inst_group[i]];
+  int buf = 0;
+  if (b->cfg.rk_order != 3) buf = (int)((b->set_global[i] + (c.nsteps - b->set_nsteps[i])) & 1);
+  return g.state[buf] + (long long)b->inst_slot[i] * g.fstride;
+}
+
+extern "C" {
+
+int cdd_batch_create(const cdd_config* cfg, const int* sizes, int count, cdd_batch** out) {
+  if (!cfg || !sizes || !out || count < 1 || count > 4096) return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
+  if (cfg->ndim != 2) return cdd_internal_fail(CDD_ERR_UNSUPPORTED, "batched stepping is 2D");
+  if (cfg->rk_order != 2 && cfg->rk_order != 3) return cdd_internal_fail(CDD_ERR_ARG, "rk_order must be 2 or 3");
+  if (cfg->load_kind != CDD_LOAD_NONE) return cdd_internal_fail(CDD_ERR_UNSUPPORTED, "batched stepping has no load");
+  const int ok2[] = {16, 32, 64, 128, 256, 384, 512, 768, 1024};
+  std::map<int, int> gidx;
+  for (int i = 0; i < count; ++i) {
+    bool ok = false;
+    for (int v : ok2) ok |= (v == sizes[i]);
+    if (!ok) return cdd_internal_fail(CDD_ERR_UNSUPPORTED, "unsupported instance size");
+  }
+  BCHK(hipSetDevice(cfg->device));
+  cdd_batch* b = new cdd_batch();
+  b->cfg = *cfg;
+  const double mu = cfg->mu, nu = cfg->nu, lam = 2.0 * mu * nu / (1.0 - 2.0 * nu);
+  memset(&b->gp0, 0, sizeof b->gp0);
+  b->gp0.A = lam / (lam + 2.0 * mu);
+  b->gp0.C = 2.0 * (lam + mu) / (lam + 2.0 * mu);
+  b->gp0.mu = mu;
+  b->gp0.lam = lam;
+  b->gp0.el = Elastic{mu, nu, lam};
+  for (int i = 0; i < count; ++i) {
+    auto it = gidx.find(sizes[i]);
+    if (it == gidx.end()) {
+      gidx[sizes[i]] = (int)b->groups.size();
+      Group g;
+      g.n = sizes[i];
+      b->groups.push_back(g);
+      it = gidx.find(sizes[i]);
+    }
+    Group& g = b->groups[it->second];
+    b->inst_group.push_back(it->second);
+    b->inst_slot.push_back(g.count++);
+  }
+  b->set_global.assign(count, 0);
+  b->set_nsteps.assign(count, 0);
+  auto bail = [&](const char* m) {
+    cdd_batch_destroy(b);
+    return cdd_internal_fail(CDD_ERR_HIP, m);
+  };
+  for (Group& g : b->groups) {
+    const int n = g.n;
+    g.cells = (long long)n * n;
+    g.fstride = 9 * g.cells;
+    g.gstride = 6 * g.cells;
+    g.sstride = 6LL * (n / 2 + 1) * n;
+    g.nblocks = flux_nblocks_tile(2, n, 0);
+    for (int t = 1; t <= 2; ++t) g.nblocks = flux_nblocks_tile(2, n, t) > g.nblocks ? flux_nblocks_tile(2, n, t) : g.nblocks;
+    const size_t fb = sizeof(double) * (size_t)g.fstride * g.count;
+    if (hipMalloc((void**)&g.state[0], fb) != hipSuccess) return bail("hipMalloc failed");
+    if (cfg->rk_order == 2 && hipMalloc((void**)&g.state[1], fb) != hipSuccess) return bail("hipMalloc failed");
+    if (hipMalloc((void**)&g.L0, fb) != hipSuccess) return bail("hipMalloc failed");
+    if (cfg->rk_order == 3 && hipMalloc((void**)&g.buf, fb) != hipSuccess) return bail("hipMalloc failed");
+    if (hipMalloc((void**)&g.G, sizeof(double) * (size_t)g.gstride * g.count) != hipSuccess) return bail("hipMalloc failed");
+    if (hipMalloc(&g.spec, sizeof(double) * 2 * (size_t)g.sstride * g.count) != hipSuccess) return bail("hipMalloc failed");
+    if (hipMalloc((void**)&g.partial, sizeof(double) * (size_t)g.nblocks * g.count) != hipSuccess)
+      return bail("hipMalloc failed");
+    if (hipMalloc((void**)&g.clk, sizeof(Clock) * g.count) != hipSuccess) return bail("hipMalloc failed");
+    if (hipMalloc(&g.tw, sizeof(double) * 2 * n) != hipSuccess) return bail("hipMalloc failed");
+    std::vector<double> h(2 * n);
+    for (int e = 0; e < n; ++e) {
+      long double th = 2.0L * 3.14159265358979323846264338327950288L * (long double)e / (long double)n;
+      h[2 * e] = (double)cosl(th);
+      h[2 * e + 1] = (double)(-sinl(th));
+    }
+    if (hipMemcpy(g.tw, h.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice) != hipSuccess)
+      return bail("twiddle upload failed");
+    std::vector<Clock> c0(g.count);
+    memset(c0.data(), 0, sizeof(Clock) * g.count);
+    for (auto& c : c0) {
+      c.t_end = INFINITY;
+      c.cfl = cfg->cfl;
+    }
+    if (hipMemcpy(g.clk, c0.data(), sizeof(Clock) * g.count, hipMemcpyHostToDevice) != hipSuccess)
+      return bail("clock upload failed");
+    if (hipMemset(g.state[0], 0, fb) != hipSuccess) return bail("memset failed");
+    if (g.state[1] && hipMemset(g.state[1], 0, fb) != hipSuccess) return bail("memset failed");
+    if (hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking) != hipSuccess) return bail("stream create failed");
+  }
+  if (hipStreamCreateWithFlags(&b->own, hipStreamNonBlocking) != hipSuccess) return bail("stream create failed");
+  b->stream = b->own;
+  b->ev.resize(1 + b->groups.size());
+  for (auto& e : b->ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return bail("event create failed");
+  *out = b;
+  return CDD_OK;
+}
+
+int cdd_batch_destroy(cdd_batch* b) {
+  if (!b) return CDD_OK;
+  (void)hipSetDevice(b->cfg.device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  free_graphs(b);
+  for (Group& g : b->groups) {
+    if (g.st) (void)hipStreamSynchronize(g.st);
+    (void)hipFree(g.state[0]);
+    (void)hipFree(g.state[1]);
+    (void)hipFree(g.L0);
+    (void)hipFree(g.buf);
+    (void)hipFree(g.G);
+    (void)hipFree(g.spec);
+    (void)hipFree(g.partial);
+    (void)hipFree(g.clk);
+    (void)hipFree(g.tw);
+    if (g.st) (void)hipStreamDestroy(g.st);
+  }
+  for (auto e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->own) (void)hipStreamDestroy(b->own);
+  delete b;
+  return CDD_OK;
+}
+
+int cdd_batch_set_stream(cdd_batch* b, void* stream) {
+  if (!b) return cdd_internal_fail(CDD_ERR_ARG, "null batch");
+  b->stream = (hipStream_t)stream;
+  return CDD_OK;
+}
+
+int cdd_batch_size(cdd_batch* b, int i, int* n) {
+  if (!b || !n || i < 0 || i >= (int)b->inst_group.size()) return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
+  *n = b->groups[b->inst_group[i]].n;
+  return CDD_OK;
+}
+
+int cdd_batch_set_state(cdd_batch* b, int i, const double* beta, int on_device) {
+  if (!b || !beta || i < 0 || i >= (int)b->inst_group.size()) return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
+  BCHK(hipSetDevice(b->cfg.device));
+  std::vector<Clock> c;
+  if (sync_clocks(b, c)) return CDD_ERR_HIP;
+  Group& g = b->groups[b->inst_group[i]];
+  b->set_global[i] = b->global_steps;
+  b->set_nsteps[i] = c[i].nsteps;
+  double* dst = inst_state(b, i, c[i]);
+  BCHK(hipMemcpy(dst, beta, sizeof(double) * (size_t)g.fstride, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+  return CDD_OK;
+}
+
+int cdd_batch_get_state(cdd_batch* b, int i, double* beta, int on_device) {
+  if (!b || !beta || i < 0 || i >= (int)b->inst_group.size()) return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
+  BCHK(hipSetDevice(b->cfg.device));
+  std::vector<Clock> c;
+  if (sync_clocks(b, c)) return CDD_ERR_HIP;
+  Group& g = b->groups[b->inst_group[i]];
+  BCHK(hipMemcpy(beta, inst_state(b, i, c[i]), sizeof(double) * (size_t)g.fstride,
+                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+  return CDD_OK;
+}
+
+int cdd_batch_set_time(cdd_batch* b, int i, double t, double t_end, double fixed_dt) {
+  if (!b || i < -1 || i >= (int)b->inst_group.size()) return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
+  BCHK(hipSetDevice(b->cfg.device));
+  std::vector<Clock> c;
+  if (sync_clocks(b, c)) return CDD_ERR_HIP;
+  for (int k = 0; k < (int)b->inst_group.size(); ++k) {
+    if (i >= 0 && k != i) continue;
+    Clock cc = c[k];
+    cc.t = t;
+    cc.t_end = t_end;
+    cc.fixed_dt = fixed_dt;
+    cc.done = t >= t_end ? 1 : 0;
+    Group& g = b->groups[b->inst_group[k]];
+    BCHK(hipMemcpy(g.clk + b->inst_slot[k], &cc, sizeof(Clock), hipMemcpyHostToDevice));
+  }
+  return CDD_OK;
+}
+
+int cdd_batch_get_clock(cdd_batch* b, int i, double* t, double* last_dt, int* done, long long* nsteps) {
+  if (!b || i < 0 || i >= (int)b->inst_group.size()) return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
+  BCHK(hipSetDevice(b->cfg.device));
+  std::vector<Clock> c;
+  if (sync_clocks(b, c)) return CDD_ERR_HIP;
+  if (t) *t = c[i].t;
+  if (last_dt) *last_dt = c[i].dt;
+  if (done) *done = c[i].done;
+  if (nsteps) *nsteps = c[i].nsteps;
+  return CDD_OK;
+}
+
+int cdd_batch_set_tile(cdd_batch* b, int n, int variant) {
+  if (!b || variant < 0 || variant > 2) return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
+  for (Group& g : b->groups)
+    if (g.n == n) g.tile = variant;
+  free_graphs(b);
+  return CDD_OK;
+}
+
+int cdd_batch_step(cdd_batch* b, int nsteps) {
+  if (!b || nsteps < 0) return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
+  BCHK(hipSetDevice(b->cfg.device));
+  for (int s = 0; s < nsteps; ++s) {
+    const int phase = b->cfg.rk_order == 3 ? 0 : (int)(b->global_steps & 1);
+    if (!b->exec[phase]) {
+      int rc = capture(b, phase);
+      if (rc) return rc;
+    }
+    BCHK(hipGraphLaunch(b->exec[phase], b->stream));
+    b->global_steps += 1;
+  }
+  return CDD_OK;
+}
+
+// Autotuner hook: mean time (ms) of `reps` stage-1 flux launches of size
+// group n with tile variant `variant` on the current states (no state change).
+int cdd_batch_time_flux(cdd_batch* b, int n, int variant, int reps, double* ms) {
+  if (!b || !ms || reps < 1 || variant < 0 || variant > 2) return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
+  BCHK(hipSetDevice(b->cfg.device));
+  Group* gp = nullptr;
+  for (Group& g : b->groups)
+    if (g.n == n) gp = &g;
+  if (!gp) return cdd_internal_fail(CDD_ERR_ARG, "no instances of this size");
+  Group& g = *gp;
+  FluxJob f;
+  memset(&f, 0, sizeof f);
+  f.d = 2;
+  f.n = g.n;
+  f.dyn = b->cfg.dynamics;
+  f.cells = g.cells;
+  f.A = g.state[0];
+  f.G = g.G;
+  f.out = g.L0;
+  f.partial = g.partial;
+  f.mode = 0;
+  f.Lambda = b->cfg.dynamics == CDD_GLIDE_ONLY ? 0.0 : b->cfg.Lambda;
+  f.mu = b->gp0.mu;
+  f.lam = b->gp0.lam;
+  f.stream = b->stream;
+  f.count = g.count;
+  f.fstride = g.fstride;
+  f.gstride = g.gstride;
+  f.pstride = g.nblocks;
+  f.tile = variant;
+  hipEvent_t e0, e1;
+  BCHK(hipEventCreate(&e0));
+  BCHK(hipEventCreate(&e1));
+  BCHK(launch_flux(f, nullptr));  // warm-up
+  BCHK(hipEventRecord(e0, b->stream));
+  for (int r = 0; r < reps; ++r) BCHK(launch_flux(f, nullptr));
+  BCHK(hipEventRecord(e1, b->stream));
+  BCHK(hipEventSynchronize(e1));
+  float t = 0.f;
+  BCHK(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = t / reps;
+  return CDD_OK;
+}
+
+}  // extern "C"

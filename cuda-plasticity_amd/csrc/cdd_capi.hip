@@ -19,6 +19,8 @@ static int fail(int code, const char* msg) {
   return code;
 }
 
+int cdd_internal_fail(int code, const char* msg) { return fail(code, msg); }
+
 #define HIPCHK(expr)                                                               \
   do {                                                                             \
     hipError_t e_ = (expr);                                                        \
@@ -287,6 +289,7 @@ int cdd_get_clock(cdd_plan* p, double* t, double* last_dt, int* done, long long*
 static SpectralJob spectral_job(cdd_plan* p, int nc, const double* A, const double* B, const Clock* clk, double t,
                                 double* out) {
   SpectralJob j;
+  memset(&j, 0, sizeof j);
   j.d = p->d;
   j.n = p->n;
   j.nc = nc;

@@ -295,6 +295,16 @@ template <typename T, int D, int DYN, int FTC, int FTR, bool PRE>
 __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   using FT = FluxTile<FTC, FTR>;
   constexpr int FPW = FT::PW, FTILE = FT::TILE, FNT = FT::NT, FFILL = FT::FILL;
+  if (blockIdx.y) {  // batched 2D: instance blockIdx.y
+    const long long i = blockIdx.y;
+    job.A += i * job.fstride;
+    if (job.U) job.U += i * job.fstride;
+    job.out += i * job.fstride;
+    job.G += i * job.gstride;
+    if (job.partial) job.partial += i * job.pstride;
+    if (job.speed) job.speed += i * (job.fstride / 9);
+  }
+  if (job.clk) job.clk += blockIdx.y;
   if (job.clk && job.clk->done) return;
   __shared__ T tile[9][FTILE];
   __shared__ T vsh[4 * (1 << D)][FNT];
@@ -464,10 +474,17 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   }
 }
 
-// Tile shapes: 32 x 8 cells (contiguous x row axis) for n >= 32; whole rows
-// below that.  A 1D grid; the kernel maps block ids to tiles.
-static void flux_tile(int n, int& tc, int& tr) {
-  if (n >= 32) {
+// Tile shapes (cells of the contiguous x row axis): variant 0 = 32 x 8,
+// 1 = 64 x 4, 2 = 16 x 16 where n allows; whole rows for small n.  A 1D
+// grid per instance (grid.y = instances); the kernel maps block ids to tiles.
+static void flux_tile(int n, int variant, int& tc, int& tr) {
+  if (variant == 1 && n >= 64) {
+    tc = 64;
+    tr = 4;
+  } else if (variant == 2 && n >= 16) {
+    tc = 16;
+    tr = 16;
+  } else if (n >= 32) {
     tc = 32;
     tr = 8;
   } else {
@@ -476,18 +493,20 @@ static void flux_tile(int n, int& tc, int& tr) {
   }
 }
 
-static void flux_geometry(int d, int n, dim3& g, dim3& b) {
+static void flux_geometry(int d, int n, int variant, int count, dim3& g, dim3& b) {
   int tc, tr;
-  flux_tile(n, tc, tr);
+  flux_tile(n, variant, tc, tr);
   b = dim3(tc, tr, 1);
-  g = dim3((d == 3 ? n : 1) * (n / tc) * (n / tr), 1, 1);
+  g = dim3((d == 3 ? n : 1) * (n / tc) * (n / tr), count > 0 ? count : 1, 1);
 }
 
-int flux_nblocks(int d, int n) {
+int flux_nblocks_tile(int d, int n, int tile) {
   dim3 g, b;
-  flux_geometry(d, n, g, b);
-  return (int)(g.x * g.y * g.z);
+  flux_geometry(d, n, tile, 1, g, b);
+  return (int)g.x;
 }
+
+int flux_nblocks(int d, int n) { return flux_nblocks_tile(d, n, 0); }
 
 template <int TC, int TR>
 static void launch_flux_t(const FluxJob& job, dim3 g, dim3 b) {
@@ -502,21 +521,25 @@ static void launch_flux_t(const FluxJob& job, dim3 g, dim3 b) {
 
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
   if (job.B) return hipErrorInvalidValue;  // stage inputs are materialised before the flux pass
+  if (job.count > 1 && job.d != 2) return hipErrorInvalidValue;
   dim3 g, b;
-  flux_geometry(job.d, job.n, g, b);
-  if (nblocks_out) *nblocks_out = (int)(g.x * g.y * g.z);
-  switch (b.x) {
-    case 32: launch_flux_t<32, 8>(job, g, b); break;
-    case 16: launch_flux_t<16, 16>(job, g, b); break;
-    case 8: launch_flux_t<8, 8>(job, g, b); break;
-    default: return hipErrorInvalidValue;
-  }
+  flux_geometry(job.d, job.n, job.tile, job.count, g, b);
+  if (nblocks_out) *nblocks_out = (int)g.x;
+  if (b.x == 64 && b.y == 4) launch_flux_t<64, 4>(job, g, b);
+  else if (b.x == 32 && b.y == 8) launch_flux_t<32, 8>(job, g, b);
+  else if (b.x == 16 && b.y == 16) launch_flux_t<16, 16>(job, g, b);
+  else if (b.x == 8 && b.y == 8) launch_flux_t<8, 8>(job, g, b);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
 // dt = CFL / (max(gridShape) max speed), clamped to the end time
 // (FieldMover.py:94-111).  One block.
-__global__ void __launch_bounds__(1024) k_finalize_dt(const double* partial, int nb, Clock* clk, double nmax) {
+__global__ void __launch_bounds__(1024) k_finalize_dt(const double* partial, int nb, Clock* clk, double nmax,
+                                                     long long pstride) {
+  // one block per instance
+  partial += blockIdx.x * pstride;
+  clk += blockIdx.x;
   if (clk->done) return;
   __shared__ double red[16];
   double m = -INFINITY;
@@ -534,7 +557,9 @@ __global__ void __launch_bounds__(1024) k_finalize_dt(const double* partial, int
   }
 }
 
-__global__ void k_fixed_dt(Clock* clk) {
+__global__ void k_fixed_dt(Clock* clk, int count) {
+  if ((int)threadIdx.x >= count) return;
+  clk += threadIdx.x;
   if (clk->done) return;
   double dt = clk->fixed_dt;
   if (clk->t + dt > clk->t_end) dt = clk->t_end - clk->t;
@@ -542,21 +567,26 @@ __global__ void k_fixed_dt(Clock* clk) {
 }
 
 // t += dt after the step; stop once t >= endTime (FieldMover.py:30-36)
-__global__ void k_advance(Clock* clk) {
+__global__ void k_advance(Clock* clk, int count) {
+  if ((int)threadIdx.x >= count) return;
+  clk += threadIdx.x;
   if (clk->done) return;
   clk->t += clk->dt;
   clk->nsteps += 1;
   if (clk->t >= clk->t_end) clk->done = 1;
 }
 
-hipError_t launch_finalize_dt(const double* partial, int nb, Clock* clk, double nmax, hipStream_t st) {
-  if (partial) hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(1024), 0, st, partial, nb, clk, nmax);
-  else hipLaunchKernelGGL(k_fixed_dt, dim3(1), dim3(1), 0, st, clk);
+hipError_t launch_finalize_dt(const double* partial, int nb, Clock* clk, double nmax, hipStream_t st, int count,
+                              long long pstride) {
+  if (count < 1 || count > 1024) return hipErrorInvalidValue;
+  if (partial) hipLaunchKernelGGL(k_finalize_dt, dim3(count), dim3(1024), 0, st, partial, nb, clk, nmax, pstride);
+  else hipLaunchKernelGGL(k_fixed_dt, dim3(1), dim3(count), 0, st, clk, count);
   return hipGetLastError();
 }
 
-hipError_t launch_advance(Clock* clk, hipStream_t st) {
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, clk);
+hipError_t launch_advance(Clock* clk, hipStream_t st, int count) {
+  if (count < 1 || count > 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(count), 0, st, clk, count);
   return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@ struct SrcField {
   const Clock* clk;
   long long cs;  // component stride (cells)
   T* W;          // optional: the formed A + dt B is also stored here (may alias B)
+  long long istride;  // batched 2D: per-instance stride of A, B, W (elements)
 };
 
 struct GreenParams {
@@ -43,6 +44,11 @@ struct SpectralJob {
   double* out;           // nc real planes
   GreenParams gp;
   hipStream_t stream;
+  // batched 2D: `count` instances of the same n with uniform strides; the
+  // instance index is blockIdx.y of every 2D pass and clocks are gp.clk[i]
+  int count;
+  long long sstride;     // spectrum (complex elements)
+  long long ostride;     // out (elements)
 };
 
 hipError_t launch_spectral(const SpectralJob& job);
@@ -66,12 +72,18 @@ struct FluxJob {
   double cu, cf;
   double Lambda, mu, lam;
   hipStream_t stream;
+  // batched 2D (instance = blockIdx.y): strides of A/U/out, G and partial
+  int count;
+  long long fstride, gstride, pstride;
+  int tile;            // flux tile variant (0: default, 1: 64x4, 2: 16x16)
 };
 
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out);
 int flux_nblocks(int d, int n);
-hipError_t launch_finalize_dt(const double* partial, int nblocks, Clock* clk, double nmax, hipStream_t st);
-hipError_t launch_advance(Clock* clk, hipStream_t st);
+hipError_t launch_finalize_dt(const double* partial, int nblocks, Clock* clk, double nmax, hipStream_t st,
+                              int count = 1, long long pstride = 0);
+hipError_t launch_advance(Clock* clk, hipStream_t st, int count = 1);
+int flux_nblocks_tile(int d, int n, int tile);
 hipError_t launch_energy(const double* G, const double* beta, long long cells, double mu, double nu, double lam,
                          double* partial, int* nblocks, hipStream_t st);
 hipError_t launch_reduce_sum(const double* partial, int n, double* out, hipStream_t st);

@@ -75,15 +75,25 @@ __device__ __forceinline__ T src_comp(const SrcField<T>& s, int c, long long idx
 // grid: (N / TY, NO, npairs); block: TY * TT.  NO = N in 3D (outer x), 1 in 2D.
 template <typename T, int N, bool SYM>
 __global__ void __launch_bounds__(Tiles<N>::TY * Tiles<N>::TT)
-k_r2c_last(SrcField<T> src, cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, int NO, int NC) {
+k_r2c_last(SrcField<T> src, cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, int NO, int NC,
+           long long sstride) {
   using TL = Tiles<N>;
   constexpr int TT = TL::TT, TY = TL::TY, P = TL::P, V = FFTPlan<N>::V, H = N / 2 + 1;
+  // 3D: blockIdx.y is the outer x index; batched 2D: it is the instance
+  const int inst = (NO == 1) ? (int)blockIdx.y : 0;
+  const int o = (NO == 1) ? 0 : (int)blockIdx.y;
+  if (inst) {
+    src.A += inst * src.istride;
+    if (src.B) src.B += inst * src.istride;
+    if (src.W) src.W += inst * src.istride;
+    spec += inst * sstride;
+  }
+  if (src.clk) src.clk += inst;
   if (src.clk && src.clk->done) return;
   __shared__ cx<T> lds[TY * P];
   const int tid = threadIdx.x;
   const int r = tid / TT, j = tid % TT;
   const int r0 = blockIdx.x * TY;
-  const int o = blockIdx.y;
   const int ca = 2 * blockIdx.z, cb = 2 * blockIdx.z + 1;
   const bool hasb = cb < NC;
   const T dt = (src.B && src.clk) ? T(src.clk->dt) : T(0);
@@ -121,14 +131,18 @@ k_r2c_last(SrcField<T> src, cx<T>* __restrict__ spec, const cx<T>* __restrict__ 
 template <typename T, int N>
 __global__ void __launch_bounds__(Tiles<N>::TY * Tiles<N>::TT)
 k_c2r_last(const cx<T>* __restrict__ spec, T* __restrict__ out, long long cs, const cx<T>* __restrict__ tw,
-           int NO, int NC, const Clock* clk) {
+           int NO, int NC, const Clock* clk, long long sstride, long long ostride) {
   using TL = Tiles<N>;
   constexpr int TT = TL::TT, TY = TL::TY, P = TL::P, V = FFTPlan<N>::V, H = N / 2 + 1;
+  const int inst = (NO == 1) ? (int)blockIdx.y : 0;
+  const int o = (NO == 1) ? 0 : (int)blockIdx.y;
+  spec += inst * sstride;
+  out += inst * ostride;
+  if (clk) clk += inst;
   if (clk && clk->done) return;
   __shared__ cx<T> lds[TY * P];
   const int tid = threadIdx.x;
   const int r0 = blockIdx.x * TY;
-  const int o = blockIdx.y;
   const int ca = 2 * blockIdx.z, cb = 2 * blockIdx.z + 1;
   const bool hasb = cb < NC;
   {
@@ -242,10 +256,12 @@ __device__ __forceinline__ cx<T> green_rho(const cx<T>* b, int c, T kx, T ky, T 
 // 3D: row = kz * N + x, e = y;  2D: row = ky, e = x.
 template <typename T, int N, int NC, int D>
 __global__ void __launch_bounds__((NC * ContractTiles<N, NC>::TR * ContractTiles<N, NC>::TT))
-k_contract(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, GreenParams gp) {
+k_contract(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, GreenParams gp, long long sstride) {
   using CT = ContractTiles<N, NC>;
   constexpr int TT = CT::TT, TR = CT::TR, P = LdsRow<N>::P, V = FFTPlan<N>::V, H = N / 2 + 1;
   constexpr long long NR = (D == 3) ? (long long)H * N : (long long)H;
+  spec += (long long)blockIdx.y * sstride;  // batched 2D instance
+  if (gp.clk) gp.clk += blockIdx.y;
   if (gp.clk && gp.clk->done) return;
   __shared__ cx<T> lds[NC * TR * P];
   const int tid = threadIdx.x;
@@ -352,14 +368,16 @@ static hipError_t launch_spectral_n(const SpectralJob& job) {
   const cx<T>* tw = reinterpret_cast<const cx<T>*>(job.tw);
   cx<T>* spec = reinterpret_cast<cx<T>*>(job.spec);
 
-  dim3 g1(N / TL::TY, NO, npairs), b1(TL::TY * TL::TT);
+  const int count = job.count > 0 ? job.count : 1;
+  if (count > 1 && d != 2) return hipErrorInvalidValue;
+  dim3 g1(N / TL::TY, d == 3 ? NO : count, npairs), b1(TL::TY * TL::TT);
   auto hook = [&](int kind) {
     if (job.hook) job.hook(job.hook_ctx, kind);
   };
   if (NC == 6)
-    hipLaunchKernelGGL((k_r2c_last<T, N, true>), g1, b1, 0, st, job.src, spec, tw, NO, NC);
+    hipLaunchKernelGGL((k_r2c_last<T, N, true>), g1, b1, 0, st, job.src, spec, tw, NO, NC, job.sstride);
   else
-    hipLaunchKernelGGL((k_r2c_last<T, N, false>), g1, b1, 0, st, job.src, spec, tw, NO, NC);
+    hipLaunchKernelGGL((k_r2c_last<T, N, false>), g1, b1, 0, st, job.src, spec, tw, NO, NC, job.sstride);
   hook(KK_R2C);
   if (d == 3) {
     dim3 g2(N / TL::TW, H, NC), b2(TL::TW * TL::TT);
@@ -369,14 +387,14 @@ static hipError_t launch_spectral_n(const SpectralJob& job) {
   const long long NR = d == 3 ? (long long)H * N : H;
   if (NC == 6) {
     using CT = ContractTiles<N, 6>;
-    dim3 g3((unsigned)((NR + CT::TR - 1) / CT::TR)), b3(6 * CT::TR * CT::TT);
-    if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 6, 3>), g3, b3, 0, st, spec, tw, job.gp);
-    else hipLaunchKernelGGL((k_contract<T, N, 6, 2>), g3, b3, 0, st, spec, tw, job.gp);
+    dim3 g3((unsigned)((NR + CT::TR - 1) / CT::TR), count), b3(6 * CT::TR * CT::TT);
+    if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 6, 3>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+    else hipLaunchKernelGGL((k_contract<T, N, 6, 2>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
   } else {
     using CT = ContractTiles<N, 9>;
-    dim3 g3((unsigned)((NR + CT::TR - 1) / CT::TR)), b3(9 * CT::TR * CT::TT);
-    if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 9, 3>), g3, b3, 0, st, spec, tw, job.gp);
-    else hipLaunchKernelGGL((k_contract<T, N, 9, 2>), g3, b3, 0, st, spec, tw, job.gp);
+    dim3 g3((unsigned)((NR + CT::TR - 1) / CT::TR), count), b3(9 * CT::TR * CT::TT);
+    if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 9, 3>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+    else hipLaunchKernelGGL((k_contract<T, N, 9, 2>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
   }
   hook(KK_CONTRACT);
   if (d == 3) {
@@ -384,7 +402,8 @@ static hipError_t launch_spectral_n(const SpectralJob& job) {
     hipLaunchKernelGGL((k_fft_cols<T, N, true>), g2, b2, 0, st, spec, tw, job.gp.clk);
     hook(KK_COLS_INV);
   }
-  hipLaunchKernelGGL((k_c2r_last<T, N>), g1, b1, 0, st, spec, job.out, cs, tw, NO, NC, job.gp.clk);
+  hipLaunchKernelGGL((k_c2r_last<T, N>), g1, b1, 0, st, spec, job.out, cs, tw, NO, NC, job.gp.clk, job.sstride,
+                     job.ostride);
   hook(KK_C2R);
   return hipGetLastError();
 }

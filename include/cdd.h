@@ -133,6 +133,30 @@ int cdd_state_ptr(cdd_plan* plan, double** beta_dev);
 int cdd_timing_enable(cdd_plan* plan, int max_events);
 int cdd_timing_read(cdd_plan* plan, double* ms, int* counts, int nkinds);
 
+/* ---- Batched independent realisations (2D) ------------------------------
+ * The reference advances independent realisations one process per GPU
+ * (Plasticity/exec_batch.py:26-58, RunCUDA.simulation per device/seed).
+ * A batch advances `count` 2D realisations of possibly different sides
+ * (sizes[i] in 16..1024 incl. 384, 768) together: one hipGraph-captured step
+ * per launch advances every instance with its own CFL time step and end
+ * time.  cfg gives the shared physics/stepping (ndim must be 2, no load). */
+typedef struct cdd_batch cdd_batch;
+int cdd_batch_create(const cdd_config* cfg, const int* sizes, int count, cdd_batch** out);
+int cdd_batch_destroy(cdd_batch* batch);
+int cdd_batch_set_stream(cdd_batch* batch, void* stream);
+int cdd_batch_size(cdd_batch* batch, int i, int* n);
+int cdd_batch_set_state(cdd_batch* batch, int i, const double* beta, int on_device);
+int cdd_batch_get_state(cdd_batch* batch, int i, double* beta, int on_device);
+/* i = -1 sets every instance */
+int cdd_batch_set_time(cdd_batch* batch, int i, double t, double t_end, double fixed_dt);
+int cdd_batch_get_clock(cdd_batch* batch, int i, double* t, double* last_dt, int* done, long long* nsteps);
+/* Replays the captured step nsteps times (no host synchronisation). */
+int cdd_batch_step(cdd_batch* batch, int nsteps);
+/* Autotuner hooks: flux tile variant per size class (0: 32x8, 1: 64x4,
+ * 2: 16x16 cells) and the timing of one variant on the current states. */
+int cdd_batch_set_tile(cdd_batch* batch, int n, int variant);
+int cdd_batch_time_flux(cdd_batch* batch, int n, int variant, int reps, double* ms);
+
 const char* cdd_last_error(void);
 
 #ifdef __cplusplus
