@@ -136,6 +136,99 @@ def bench_batched(args):
     print(json.dumps(out))
 
 
+def bench_adjoint(args):
+    """2D 512^2, 400 SSP-RK2 steps, glide + climb with climb mobility 0.1
+    (Lambda = 0.9), uniform shear sigma_xy = 0.005: gradient of the final
+    total elastic energy w.r.t. the initial beta^P and the climb mobility by
+    the hand-written adjoint (checkpoints every 20 steps), its cost relative
+    to the plain forward run, and central finite differences on 32 random
+    unit directions (eps = 1e-6) replaying the recorded time steps."""
+    import numpy as np
+    import torch
+    from Plasticity.amd import lib as L
+    from Plasticity.amd.autograd import run_with_grad, elastic_energy
+    from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
+    n, nsteps, Lam = 512, (args.steps if args.steps != 20 else 400), 0.9
+    shear = dict(load_kind=L.LOAD_TENSOR, load_init=(0, 0.005, 0, 0.005, 0, 0, 0, 0, 0))
+    beta0 = torch.from_numpy(SpectralGaussianField((n, n), 4.0, 0.01, seed=4)).cuda()
+    plan = L.Plan(2, n, dynamics=L.GLIDE_CLIMB, rk_order=2, Lambda=Lam, **shear)
+    log = torch.zeros(nsteps, dtype=torch.float64, device="cuda")
+
+    def forward(record):
+        plan.set_state(beta0)
+        plan.set_time(0.0)
+        if record:
+            plan.dt_record(log)
+        plan.step(nsteps)
+        e = plan.energy(plan.get_state(), 0.0)
+        if record:
+            plan.dt_record(None)
+        return e
+    forward(True)  # warm-up, records the schedule
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e_fwd = forward(False)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    b = beta0.clone().requires_grad_(True)
+    lt = torch.tensor(Lam, dtype=torch.float64, device="cuda", requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E = elastic_energy(run_with_grad(b, lt, nsteps, checkpoint_every=20, rk_order=2, load=shear), load=shear)
+    E.backward()
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    grad = b.grad.detach()
+    mob_grad = -lt.grad.item()  # climb mobility = 1 - Lambda
+
+    def replay(b0):
+        plan.set_state(b0)
+        plan.set_time(0.0)
+        plan.dt_replay(log)
+        plan.step(nsteps)
+        plan.dt_replay(None)
+        return plan.energy(plan.get_state(), 0.0)
+    e_base = replay(beta0)
+    rng = np.random.default_rng(32)
+    ndir = args.fd_dirs
+    dirs = []
+    for _ in range(ndir):
+        d = torch.from_numpy(rng.standard_normal((9, n, n))).cuda()
+        dirs.append(d / torch.linalg.norm(d))
+    fd_stats = {}
+    for eps in ((1e-6, 1e-8) if ndir else ()):
+        errs, bracketed = [], 0
+        for d in dirs:
+            ep, em = replay(beta0 + eps * d), replay(beta0 - eps * d)
+            cen, fwd, bwd = (ep - em) / (2 * eps), (ep - e_base) / eps, (e_base - em) / eps
+            ad = float((grad * d).sum())
+            errs.append(abs(cen - ad) / abs(ad))
+            # a minmod / max-speed kink within +-eps puts the central difference
+            # off by half the gap between the one-sided differences
+            bracketed += abs(cen - ad) <= 1e-6 * abs(ad) + 0.5 * abs(fwd - bwd)
+        fd_stats["%.0e" % eps] = {"rel_err_max": max(errs), "rel_err_median": float(np.median(errs)),
+                                  "within_1e-6": int(sum(e <= 1e-6 for e in errs)),
+                                  "within_1e-6_or_kink_bracket": int(bracketed)}
+        print("fd eps %.0e: %s" % (eps, fd_stats["%.0e" % eps]), file=sys.stderr, flush=True)
+    out = {
+        "metric": "adjoint/forward cost (2D 512^2, 400 RK2 steps, energy gradient)",
+        "value": t_adj / t_fwd,
+        "unit": "x (forward+backward time / forward time)",
+        "n_gpus": 1, "steps": nsteps, "warmup": 1,
+        "ms_per_step": 1e3 * t_fwd / nsteps,
+        "adjoint_ms_per_step": 1e3 * t_adj / nsteps,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded Gaussian random beta^P (L=4 cells, RMS 0.01)",
+        "config": {"workload": "cdd-2d-512-adjoint", "grid": [n, n], "dynamics": "glide+climb", "Lambda": Lam,
+                   "rk_order": 2, "cfl": 0.25, "load": "sigma_xy = sigma_yx = 0.005", "checkpoint_every": 20},
+        "energy": e_fwd,
+        "dE_dmobility": mob_grad,
+        "fd_directions": ndir,
+        "fd_vs_adjoint": fd_stats,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,7 +236,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="3d", choices=["3d", "batched"],
+    ap.add_argument("--fd-dirs", type=int, default=32, help="adjoint workload: finite-difference directions")
+    ap.add_argument("--workload", default="3d", choices=["3d", "batched", "adjoint"],
                     help="3d: the headline 256^3 stepper (default); batched: 48 uneven 2D realisations, "
                          "batched hipGraph step vs a serial loop of plain plans")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -164,6 +258,8 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
 
     import numpy as np
     import torch

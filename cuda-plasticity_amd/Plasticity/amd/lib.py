@@ -45,7 +45,8 @@ EXPORTS = [
     "cdd_state_ptr", "cdd_last_error", "cdd_timing_enable", "cdd_timing_read",
     "cdd_batch_create", "cdd_batch_destroy", "cdd_batch_set_stream", "cdd_batch_size", "cdd_batch_set_state",
     "cdd_batch_get_state", "cdd_batch_set_time", "cdd_batch_get_clock", "cdd_batch_step", "cdd_batch_set_tile",
-    "cdd_batch_time_flux",
+    "cdd_batch_time_flux", "cdd_flux_vjp", "cdd_step_vjp", "cdd_energy_grad", "cdd_dt_record", "cdd_dt_replay",
+    "cdd_segment_vjp", "cdd_lambda_bar",
 ]
 KERNEL_KINDS = ["r2c", "cols_fwd", "contract", "cols_inv", "c2r", "flux_stage1", "cfl_reduce",
                 "flux_stage2", "flux_stage3", "advance"]
@@ -75,6 +76,13 @@ def _load():
         "cdd_state_ptr": ([P, ctypes.POINTER(P)], I),
         "cdd_last_error": ([], ctypes.c_char_p),
         "cdd_timing_enable": ([P, I], I),
+        "cdd_flux_vjp": ([P, P, D, P, P, dp], I),
+        "cdd_step_vjp": ([P, P, D, D, P, P, dp], I),
+        "cdd_energy_grad": ([P, P, D, P, dp], I),
+        "cdd_dt_record": ([P, P, ctypes.c_longlong], I),
+        "cdd_dt_replay": ([P, P, ctypes.c_longlong], I),
+        "cdd_segment_vjp": ([P, P, D, P, I, P, P], I),
+        "cdd_lambda_bar": ([P, dp, I], I),
         "cdd_batch_create": ([ctypes.POINTER(Config), ctypes.POINTER(I), I, ctypes.POINTER(P)], I),
         "cdd_batch_destroy": ([P], I),
         "cdd_batch_set_stream": ([P, P], I),
@@ -211,6 +219,68 @@ class Plan:
         p = _ptr(self._as_dev(beta)) if beta is not None else None
         check(lib.cdd_energy(self.h, p, t, ctypes.byref(e)))
         return e.value
+
+    # ---- adjoint (frozen time-step schedule)
+    def flux_vjp(self, beta, t, rbar):
+        b, r = self._as_dev(beta), self._as_dev(rbar)
+        out = self.empty()
+        lam = ctypes.c_double()
+        check(lib.cdd_flux_vjp(self.h, _ptr(b), t, _ptr(r), _ptr(out), ctypes.byref(lam)))
+        return out, lam.value
+
+    def step_vjp(self, u, t, dt, ubar_next):
+        a, g = self._as_dev(u), self._as_dev(ubar_next)
+        out = self.empty()
+        lam = ctypes.c_double()
+        check(lib.cdd_step_vjp(self.h, _ptr(a), t, dt, _ptr(g), _ptr(out), ctypes.byref(lam)))
+        return out, lam.value
+
+    def energy_grad(self, beta, t=0.0):
+        b = self._as_dev(beta)
+        out = self.empty()
+        e = ctypes.c_double()
+        check(lib.cdd_energy_grad(self.h, _ptr(b), t, _ptr(out), ctypes.byref(e)))
+        return out, e.value
+
+    def segment_vjp(self, u0, t0, dts, ubar_end, out=None):
+        """Reverse pass over one checkpoint segment: recompute dts.numel()
+        steps from the checkpoint u0 at time t0 replaying the recorded time
+        steps dts (float64 device tensor), then return the cotangent of u0
+        given the cotangent ubar_end of the segment's final state.  `out` may
+        be ubar_end itself.  The Lambda cotangent accumulates on the device
+        (lambda_bar)."""
+        a, g = self._as_dev(u0), self._as_dev(ubar_end)
+        if dts.dtype != self.torch.float64 or not dts.is_contiguous() or dts.device != self.dev:
+            raise CDDError("dts must be a contiguous float64 tensor on the plan's device")
+        if out is None:
+            out = self.empty()
+        elif (out.dtype != self.torch.float64 or not out.is_contiguous() or out.device != self.dev
+              or tuple(out.shape) != (9,) + self.shape):
+            raise CDDError("out must be a contiguous float64 (9,) + grid tensor on the plan's device")
+        check(lib.cdd_segment_vjp(self.h, _ptr(a), float(t0), _ptr(dts), int(dts.numel()), _ptr(g), _ptr(out)))
+        return out
+
+    def lambda_bar(self, reset=True):
+        """Lambda cotangent accumulated by segment_vjp since the last reset."""
+        v = ctypes.c_double()
+        check(lib.cdd_lambda_bar(self.h, ctypes.byref(v), 1 if reset else 0))
+        return v.value
+
+    def dt_record(self, log):
+        """Log the next log.numel() time steps into the float64 device tensor `log`
+        (None stops logging)."""
+        if log is None:
+            check(lib.cdd_dt_record(self.h, None, 0))
+        else:
+            check(lib.cdd_dt_record(self.h, _ptr(log), log.numel()))
+
+    def dt_replay(self, sched):
+        """Take the next sched.numel() time steps from the float64 device tensor
+        `sched` (None returns to the CFL rule)."""
+        if sched is None:
+            check(lib.cdd_dt_replay(self.h, None, 0))
+        else:
+            check(lib.cdd_dt_replay(self.h, _ptr(sched), sched.numel()))
 
     def timing_enable(self, max_events):
         check(lib.cdd_timing_enable(self.h, int(max_events)))

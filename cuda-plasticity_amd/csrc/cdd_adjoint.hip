@@ -1,0 +1,656 @@
+// Hand-written adjoint (vector-Jacobian product) of the CDD flux and of one
+// TVD Runge-Kutta step, with the time-step schedule frozen from the forward
+// run (the CFL rule of FieldMover.py:94-96 is a non-differentiable global
+// max; the adjoint treats the recorded dt as a constant, and finite-difference
+// checks replay the same schedule).
+//
+// L(beta) = flux(beta, sigma(beta)) per RK stage (CentralUpwindHJ3D.py:63-271
+// with the BetaP velocities/Hamiltonians).  Its transpose applied to a
+// cotangent rbar is assembled from three pieces:
+//   1. k_flux_vjp_cell: per cell, recompute the forward (one-sided derivatives
+//      u, stress, corner velocities, speeds, weights) and propagate rbar back
+//      to ubar (the 54 derivative cotangents), to the stress cotangent
+//      (symmetrised, 9 planes) and to Lambda.  The max/min speed selections
+//      and the minmod branches are replayed from the forward values.
+//   2. k_flux_vjp_gather: the transpose of the limited-derivative stencils,
+//      written as a gather (each cell collects from the 5 cells whose stencil
+//      reaches it) so no atomics are needed.
+//   3. the stress adjoint: the spectral passes with the transposed Green's
+//      contraction (green_sigma_T) applied to the symmetrised stress
+//      cotangent, plus the transpose of the local part
+//      -lam tr(beta) d - mu (beta + beta^T).
+#include "cdd_kernels.h"
+
+namespace cdd {
+
+template <typename T>
+__device__ __forceinline__ T mm3(T a, T b, T c) {
+  const T mn = fmin(a, fmin(b, c));
+  const T mx = fmax(a, fmax(b, c));
+  return fmax(mn, T(0)) + fmin(mx, T(0));
+}
+
+// Which argument minmod3(a, b, c) returns: 0, 1, 2, or -1 for the zero case
+// (NumericalMethods.py:367-378); ties resolve to the first argument.
+template <typename T>
+__device__ __forceinline__ int mm3_sel(T a, T b, T c) {
+  const T mn = fmin(a, fmin(b, c));
+  const T mx = fmax(a, fmax(b, c));
+  if (mn > T(0)) return a == mn ? 0 : (b == mn ? 1 : 2);
+  if (mx <= T(0)) return a == mx ? 0 : (b == mx ? 1 : 2);
+  return -1;
+}
+
+template <typename T>
+__device__ __forceinline__ void lderivs(const T f[5], T N, T& up, T& um) {
+  const T Dm2 = f[1] - f[0], Dm1 = f[2] - f[1], D0 = f[3] - f[2], D1 = f[4] - f[3];
+  const T m_i = mm3(D1 - D0, (D1 - Dm1) * T(0.5), D0 - Dm1);
+  const T m_im1 = mm3(D0 - Dm1, (D0 - Dm2) * T(0.5), Dm1 - Dm2);
+  up = N * fma(T(-0.5), m_i, D0);
+  um = N * fma(T(0.5), m_im1, Dm1);
+}
+
+// ------------------------------------------------------------------ R1
+template <typename T, int D, int DYN>
+__device__ void cell_flux_vjp(const T (&u)[D][2][9], const T (&sg)[9], T Lambda, const T (&rb)[9],
+                              T (&ub)[D][2][9], T (&sgb)[9], T& lamb) {
+  constexpr int NCOR = 1 << D;
+  const T ME = T(kME);
+  const bool climb_sup = (DYN == DYN_GLIDE_CLIMB) && (Lambda != T(0));
+  const T st = climb_sup ? Lambda * (sg[0] + sg[4] + sg[8]) / T(3) : T(0);
+  const T jfac = -Lambda / T(3);
+  // ---------------- forward recompute
+  T w[D][2][3], zz[D][2], tr[D][2], P[D][2];
+  for (int a = 0; a < D; ++a)
+    for (int s = 0; s < 2; ++s) {
+      const T* ua = u[a][s];
+      for (int l = 0; l < 3; ++l)
+        w[a][s][l] = ua[3 * l] * sg[3 * a] + ua[3 * l + 1] * sg[3 * a + 1] + ua[3 * l + 2] * sg[3 * a + 2];
+      T z = T(0);
+      for (int c = 0; c < 9; ++c) z += ua[c] * sg[c];
+      zz[a][s] = z;
+      tr[a][s] = ua[0] + ua[4] + ua[8];
+      T p = T(0);
+      for (int m = 0; m < 3; ++m)
+        if (m != a)
+          for (int jj = 0; jj < 3; ++jj) p += ua[3 * m + jj] * ua[3 * m + jj];
+      P[a][s] = p;
+    }
+  T R[NCOR][3], q[NCOR][3], v0p[NCOR][3], v0[NCOR][3], v[NCOR][3], rinv[NCOR], J[NCOR];
+  T rr[NCOR], rv[NCOR], fpr[NCOR];
+  for (int c = 0; c < NCOR; ++c) {
+    int sg_[3] = {0, 0, 0};
+    for (int a = 0; a < D; ++a) sg_[a] = (c >> (D - 1 - a)) & 1;
+    for (int l = 0; l < 3; ++l) {
+      T r = T(0), vv = T(0);
+      for (int a = 0; a < D; ++a) {
+        r += u[a][sg_[a]][3 * l + a];
+        vv += w[a][sg_[a]][l];
+      }
+      R[c][l] = r;
+      v0p[c][l] = vv;
+    }
+    for (int a = 0; a < D; ++a) v0p[c][a] -= zz[a][sg_[a]];
+    for (int a = 0; a < 3; ++a) q[c][a] = (a < D ? tr[a % D][sg_[a]] : T(0)) - R[c][a];
+    for (int l = 0; l < 3; ++l) v0[c][l] = v0p[c][l];
+    rr[c] = rv[c] = fpr[c] = T(0);
+    if (DYN == DYN_GLIDE_CLIMB) {
+      if (climb_sup) {
+        for (int a = 0; a < D; ++a) v0[c][a] += st * tr[a][sg_[a]];
+        for (int l = 0; l < 3; ++l) v0[c][l] -= st * R[c][l];
+      }
+    } else {
+      T rrr = ME;
+      for (int a = 0; a < D; ++a) rrr += tr[a][sg_[a]] * tr[a][sg_[a]] - T(2) * tr[a][sg_[a]] * R[c][a];
+      for (int l = 0; l < 3; ++l) rrr += R[c][l] * R[c][l];
+      const T rvv = ME + v0p[c][0] * q[c][0] + v0p[c][1] * q[c][1] + v0p[c][2] * q[c][2];
+      rr[c] = rrr;
+      rv[c] = rvv;
+      fpr[c] = rvv / rrr;
+      for (int a = 0; a < 3; ++a) v0[c][a] += q[c][a] * fpr[c];
+    }
+    T r2 = ME;
+    for (int a = 0; a < D; ++a) r2 += P[a][sg_[a]];
+    T cross = T(0);
+    for (int jj = 0; jj < 3; ++jj) {
+      cross += u[0][sg_[0]][3 + jj] * u[1][sg_[1]][jj];
+      if (D == 3) {
+        cross += u[1][sg_[1]][6 + jj] * u[D - 1][sg_[D - 1]][3 + jj];
+        cross += u[D - 1][sg_[D - 1]][jj] * u[0][sg_[0]][6 + jj];
+      }
+    }
+    r2 -= T(2) * cross;
+    rinv[c] = T(1) / sqrt(r2);
+    for (int l = 0; l < 3; ++l) v[c][l] = v0[c][l] * rinv[c];
+    J[c] = climb_sup ? jfac * (v[c][0] * q[c][0] + v[c][1] * q[c][1] + v[c][2] * q[c][2]) : T(0);
+  }
+  T ap[3] = {0, 0, 0}, am[3] = {0, 0, 0}, at[3] = {1, 1, 1};
+  int cmax[3] = {0, 0, 0}, cmin[3] = {0, 0, 0};
+  for (int a = 0; a < D; ++a) {
+    T mxv = v[0][a], mnv = v[0][a];
+    for (int c = 1; c < NCOR; ++c) {
+      if (v[c][a] > mxv) {
+        mxv = v[c][a];
+        cmax[a] = c;
+      }
+      if (v[c][a] < mnv) {
+        mnv = v[c][a];
+        cmin[a] = c;
+      }
+    }
+    ap[a] = fmax(mxv, T(0));
+    am[a] = -fmin(mnv, T(0));
+    at[a] = ap[a] + am[a] + ME;
+  }
+  T Y[D][2][3], Jsum = T(0), wcs[NCOR];
+  for (int a = 0; a < D; ++a)
+    for (int s = 0; s < 2; ++s)
+      for (int l = 0; l < 3; ++l) Y[a][s][l] = T(0);
+  for (int c = 0; c < NCOR; ++c) {
+    T wc = T(1);
+    for (int a = 0; a < D; ++a) wc *= ((c >> (D - 1 - a)) & 1) == 0 ? am[a] : ap[a];
+    wcs[c] = wc;
+    for (int a = 0; a < D; ++a)
+      for (int l = 0; l < 3; ++l) Y[a][(c >> (D - 1 - a)) & 1][l] += wc * v[c][l];
+    Jsum += wc * J[c];
+  }
+  T inv = T(1);
+  for (int a = 0; a < D; ++a) inv *= at[a];
+  inv = T(1) / inv;
+  T twv[3] = {0, 0, 0};
+  for (int a = 0; a < D; ++a) twv[a] = am[a] * ap[a] / at[a];
+  T Hs[9];
+  for (int i = 0; i < 3; ++i)
+    for (int jj = 0; jj < 3; ++jj) {
+      const int cc = 3 * i + jj;
+      T h = (i == jj) ? Jsum : T(0);
+      for (int a = 0; a < D; ++a)
+        for (int s = 0; s < 2; ++s) h += u[a][s][cc] * Y[a][s][a];
+      if (i < D)
+        for (int s = 0; s < 2; ++s)
+          for (int l = 0; l < 3; ++l) h -= u[i % D][s][3 * l + jj] * Y[i % D][s][l];
+      Hs[cc] = h;
+    }
+  // ---------------- reverse
+  for (int a = 0; a < D; ++a)
+    for (int s = 0; s < 2; ++s)
+      for (int c = 0; c < 9; ++c) ub[a][s][c] = T(0);
+  for (int c = 0; c < 9; ++c) sgb[c] = T(0);
+  lamb = T(0);
+  T Hb[9], invb = T(0), twb[3] = {0, 0, 0};
+  for (int cc = 0; cc < 9; ++cc) {
+    Hb[cc] = -rb[cc] * inv;
+    invb -= rb[cc] * Hs[cc];
+    for (int a = 0; a < D; ++a) {
+      twb[a] += rb[cc] * (u[a][0][cc] - u[a][1][cc]);
+      ub[a][0][cc] += rb[cc] * twv[a];
+      ub[a][1][cc] -= rb[cc] * twv[a];
+    }
+  }
+  const T Jsumb = Hb[0] + Hb[4] + Hb[8];
+  T Yb[D][2][3];
+  for (int a = 0; a < D; ++a)
+    for (int s = 0; s < 2; ++s)
+      for (int l = 0; l < 3; ++l) Yb[a][s][l] = T(0);
+  for (int i = 0; i < 3; ++i)
+    for (int jj = 0; jj < 3; ++jj) {
+      const int cc = 3 * i + jj;
+      for (int a = 0; a < D; ++a)
+        for (int s = 0; s < 2; ++s) {
+          ub[a][s][cc] += Hb[cc] * Y[a][s][a];
+          Yb[a][s][a] += Hb[cc] * u[a][s][cc];
+        }
+      if (i < D)
+        for (int s = 0; s < 2; ++s)
+          for (int l = 0; l < 3; ++l) {
+            ub[i % D][s][3 * l + jj] -= Hb[cc] * Y[i % D][s][l];
+            Yb[i % D][s][l] -= Hb[cc] * u[i % D][s][3 * l + jj];
+          }
+    }
+  T apb[3] = {0, 0, 0}, amb[3] = {0, 0, 0};
+  for (int a = 0; a < D; ++a) {
+    amb[a] += twb[a] * ap[a] / at[a];
+    apb[a] += twb[a] * am[a] / at[a];
+    T atb = -twb[a] * am[a] * ap[a] / (at[a] * at[a]) - invb * inv / at[a];
+    apb[a] += atb;
+    amb[a] += atb;
+  }
+  T vb[NCOR][3], Jb[NCOR];
+  for (int c = 0; c < NCOR; ++c) {
+    T wcb = Jsumb * J[c];
+    Jb[c] = Jsumb * wcs[c];
+    for (int l = 0; l < 3; ++l) {
+      T acc = T(0);
+      for (int a = 0; a < D; ++a) {
+        const T yb = Yb[a][(c >> (D - 1 - a)) & 1][l];
+        acc += yb;
+        wcb += yb * v[c][l];
+      }
+      vb[c][l] = acc * wcs[c];
+    }
+    for (int a = 0; a < D; ++a) {
+      T others = T(1);
+      for (int b2 = 0; b2 < D; ++b2)
+        if (b2 != a) others *= ((c >> (D - 1 - b2)) & 1) == 0 ? am[b2] : ap[b2];
+      if (((c >> (D - 1 - a)) & 1) == 0) amb[a] += wcb * others;
+      else apb[a] += wcb * others;
+    }
+  }
+  for (int a = 0; a < D; ++a) {
+    if (ap[a] > T(0)) vb[cmax[a]][a] += apb[a];
+    if (am[a] > T(0)) vb[cmin[a]][a] -= amb[a];
+  }
+  T Pb[D][2], wb[D][2][3], zzb[D][2], trb[D][2];
+  for (int a = 0; a < D; ++a)
+    for (int s = 0; s < 2; ++s) {
+      Pb[a][s] = zzb[a][s] = trb[a][s] = T(0);
+      for (int l = 0; l < 3; ++l) wb[a][s][l] = T(0);
+    }
+  T stb = T(0);
+  for (int c = 0; c < NCOR; ++c) {
+    int sg_[3] = {0, 0, 0};
+    for (int a = 0; a < D; ++a) sg_[a] = (c >> (D - 1 - a)) & 1;
+    T qb[3] = {0, 0, 0}, Rb[3] = {0, 0, 0};
+    if (climb_sup) {
+      const T vq = v[c][0] * q[c][0] + v[c][1] * q[c][1] + v[c][2] * q[c][2];
+      for (int l = 0; l < 3; ++l) {
+        vb[c][l] += Jb[c] * jfac * q[c][l];
+        qb[l] += Jb[c] * jfac * v[c][l];
+      }
+      lamb += Jb[c] * (-vq / T(3));
+    }
+    T v0b[3], rinvb = T(0);
+    for (int l = 0; l < 3; ++l) {
+      v0b[l] = vb[c][l] * rinv[c];
+      rinvb += vb[c][l] * v0[c][l];
+    }
+    const T r2b = rinvb * T(-0.5) * rinv[c] * rinv[c] * rinv[c];
+    for (int a = 0; a < D; ++a) Pb[a][sg_[a]] += r2b;
+    const T crossb = T(-2) * r2b;
+    for (int jj = 0; jj < 3; ++jj) {
+      ub[0][sg_[0]][3 + jj] += crossb * u[1][sg_[1]][jj];
+      ub[1][sg_[1]][jj] += crossb * u[0][sg_[0]][3 + jj];
+      if (D == 3) {
+        ub[1][sg_[1]][6 + jj] += crossb * u[D - 1][sg_[D - 1]][3 + jj];
+        ub[D - 1][sg_[D - 1]][3 + jj] += crossb * u[1][sg_[1]][6 + jj];
+        ub[D - 1][sg_[D - 1]][jj] += crossb * u[0][sg_[0]][6 + jj];
+        ub[0][sg_[0]][6 + jj] += crossb * u[D - 1][sg_[D - 1]][jj];
+      }
+    }
+    T v0pb[3] = {v0b[0], v0b[1], v0b[2]};
+    if (DYN == DYN_GLIDE_CLIMB) {
+      if (climb_sup) {
+        for (int a = 0; a < D; ++a) {
+          stb += v0b[a] * tr[a][sg_[a]];
+          trb[a][sg_[a]] += v0b[a] * st;
+        }
+        for (int l = 0; l < 3; ++l) {
+          stb -= v0b[l] * R[c][l];
+          Rb[l] -= v0b[l] * st;
+        }
+      }
+    } else {
+      T fb = T(0);
+      for (int a = 0; a < 3; ++a) {
+        fb += v0b[a] * q[c][a];
+        qb[a] += v0b[a] * fpr[c];
+      }
+      const T rvb = fb / rr[c];
+      const T rrb = -fb * rv[c] / (rr[c] * rr[c]);
+      for (int a = 0; a < 3; ++a) {
+        v0pb[a] += rvb * q[c][a];
+        qb[a] += rvb * v0p[c][a];
+      }
+      for (int a = 0; a < D; ++a) {
+        trb[a][sg_[a]] += rrb * (T(2) * tr[a][sg_[a]] - T(2) * R[c][a]);
+        Rb[a] += rrb * (T(-2) * tr[a][sg_[a]]);
+      }
+      for (int l = 0; l < 3; ++l) Rb[l] += rrb * T(2) * R[c][l];
+    }
+    for (int a = 0; a < 3; ++a) {
+      if (a < D) trb[a % D][sg_[a]] += qb[a];
+      Rb[a] -= qb[a];
+    }
+    for (int l = 0; l < 3; ++l)
+      for (int a = 0; a < D; ++a) wb[a][sg_[a]][l] += v0pb[l];
+    for (int a = 0; a < D; ++a) zzb[a][sg_[a]] -= v0pb[a];
+    for (int l = 0; l < 3; ++l)
+      for (int a = 0; a < D; ++a) ub[a][sg_[a]][3 * l + a] += Rb[l];
+  }
+  if (climb_sup) {
+    sgb[0] += stb * Lambda / T(3);
+    sgb[4] += stb * Lambda / T(3);
+    sgb[8] += stb * Lambda / T(3);
+    lamb += stb * (sg[0] + sg[4] + sg[8]) / T(3);
+  }
+  for (int a = 0; a < D; ++a)
+    for (int s = 0; s < 2; ++s) {
+      for (int m = 0; m < 3; ++m)
+        if (m != a)
+          for (int jj = 0; jj < 3; ++jj) ub[a][s][3 * m + jj] += T(2) * Pb[a][s] * u[a][s][3 * m + jj];
+      for (int l = 0; l < 3; ++l)
+        for (int nn = 0; nn < 3; ++nn) {
+          ub[a][s][3 * l + nn] += wb[a][s][l] * sg[3 * a + nn];
+          sgb[3 * a + nn] += wb[a][s][l] * u[a][s][3 * l + nn];
+        }
+      for (int c = 0; c < 9; ++c) {
+        ub[a][s][c] += zzb[a][s] * sg[c];
+        sgb[c] += zzb[a][s] * u[a][s][c];
+      }
+      ub[a][s][0] += trb[a][s];
+      ub[a][s][4] += trb[a][s];
+      ub[a][s][8] += trb[a][s];
+    }
+}
+
+static __device__ __forceinline__ int wrapi(int p, int n) { return p < 0 ? p + n : (p >= n ? p - n : p); }
+
+// Tile mapping shared by the two flux-adjoint kernels: blocks of TX x TY
+// threads (TX = min(32, n) cells along the contiguous axis, TY = min(8, n)
+// along the next one), one plane per block in 3D.  Hardware block b runs on
+// XCD b % 8; logical tile L = (b % 8)(nb / 8) + b / 8 hands every XCD one
+// contiguous range of tiles, so the stencil neighbours of a tile (the tiles
+// above/below it and, in 3D, the neighbouring planes) are mostly in flight
+// on the same XCD and served from its L2.
+template <int D>
+__device__ __forceinline__ long long adj_tile_cell(int n, int co[3], long long str[3]) {
+  const int TX = blockDim.x, TY = blockDim.y;
+  const int nct = n / TX, nrt = n / TY;
+  const int nb = (D == 3 ? n : 1) * nct * nrt;
+  const int bl = blockIdx.x;
+  const int L = (nb % 8 == 0) ? (bl % 8) * (nb / 8) + bl / 8 : bl;
+  const int px = (D == 3) ? L % n : 0;
+  const int tyz = (D == 3) ? L / n : L;
+  const int col = (tyz % nct) * TX + threadIdx.x, row = (tyz / nct) * TY + threadIdx.y;
+  if (D == 3) {
+    co[0] = px;
+    co[1] = row;
+    co[2] = col;
+    str[0] = (long long)n * n;
+    str[1] = n;
+    str[2] = 1;
+    return ((long long)px * n + row) * n + col;
+  }
+  co[0] = row;
+  co[1] = col;
+  co[2] = 0;
+  str[0] = n;
+  str[1] = 1;
+  str[2] = 0;
+  return (long long)row * n + col;
+}
+
+template <typename T, int D, int DYN>
+__global__ void __launch_bounds__(256) k_flux_vjp_cell(VjpJob job) {
+  __shared__ T red[4];
+  const int n = job.n;
+  const long long cells = job.cells;
+  int co[3];
+  long long str[3];
+  const long long ii = adj_tile_cell<D>(n, co, str);
+  const T N = T(n);
+  T u[D][2][9], fc[9];
+  for (int a = 0; a < D; ++a) {
+    long long off[5];
+    for (int k = 0; k < 5; ++k) off[k] = ii + (long long)(wrapi(co[a] + k - 2, n) - co[a]) * str[a];
+    for (int c = 0; c < 9; ++c) {
+      T f[5];
+      for (int k = 0; k < 5; ++k) f[k] = job.beta[c * cells + off[k]];
+      if (a == 0) fc[c] = f[2];
+      lderivs<T>(f, N, u[a][0][c], u[a][1][c]);
+    }
+  }
+  T sg[9];
+  {
+    const T tr = fc[0] + fc[4] + fc[8];
+    const T mu = T(job.mu), lam = T(job.lam);
+    const int SC[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+    for (int c = 0; c < 9; ++c) {
+      const int i = c / 3, j = c % 3;
+      sg[c] = job.G[SC[c] * cells + ii] - mu * (fc[c] + fc[3 * j + i]) - (i == j ? lam * tr : T(0));
+    }
+  }
+  // rbar scaled by rscale (times the step's dt when dtp is given)
+  const T rs = T(job.rscale) * (job.dtp ? T(*job.dtp) : T(1));
+  T rb[9];
+  for (int c = 0; c < 9; ++c) rb[c] = rs * job.rbar[c * cells + ii];
+  T ub[D][2][9], sgb[9], lamb;
+  cell_flux_vjp<T, D, DYN>(u, sg, T(job.Lambda), rb, ub, sgb, lamb);
+  for (int a = 0; a < D; ++a)
+    for (int s = 0; s < 2; ++s)
+      for (int c = 0; c < 9; ++c) job.ubar[((a * 2 + s) * 9 + c) * cells + ii] = ub[a][s][c];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) job.sbar[(3 * i + j) * cells + ii] = T(0.5) * (sgb[3 * i + j] + sgb[3 * j + i]);
+  for (int off = 32; off > 0; off >>= 1) lamb += __shfl_xor(lamb, off, 64);
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  if ((tid & 63) == 0) red[tid >> 6] = lamb;
+  __syncthreads();
+  if (tid == 0) {
+    T t = T(0);
+    for (int k = 0; k < (int)(blockDim.x * blockDim.y + 63) / 64; ++k) t += red[k];
+    job.lpart[blockIdx.x] = t;
+  }
+}
+
+// ------------------------------------------------------------------ R2
+// betabar(j) = sum over the cells i in j-2..j+2 of the transposed limited-
+// derivative stencils of i applied to ubar(i), per axis and component,
+// with the minmod selections replayed from beta; plus the local stress
+// transpose -lam tr(sbar) d - 2 mu sbar (sbar is symmetric), the spectral
+// part ebar when given, and the Runge-Kutta combination sum_k c_k a_k
+// (read at the thread's own cell only, so out may alias an a_k).
+template <typename T, int D>
+__global__ void __launch_bounds__(256) k_flux_vjp_gather(GatherJob g) {
+  const int n = g.n;
+  const long long cells = g.cells;
+  const T* __restrict__ beta = g.beta;
+  const T* __restrict__ ubar = g.ubar;
+  int co[3];
+  long long str[3];
+  const long long idx = adj_tile_cell<D>(n, co, str);
+  const T N = T(n);
+  T acc[9];
+  for (int c = 0; c < 9; ++c) acc[c] = T(0);
+  for (int a = 0; a < D; ++a) {
+    long long off[9];
+    for (int k = 0; k < 9; ++k) off[k] = idx + (long long)(wrapi(co[a] + k - 4, n) - co[a]) * str[a];
+    for (int c = 0; c < 9; ++c) {
+      T f[9];
+      for (int k = 0; k < 9; ++k) f[k] = beta[c * cells + off[k]];
+      // Dv[k] = D_{j-4+k} = f_{j-3+k} - f_{j-4+k}, k = 0..7 (D_{j-4} .. D_{j+3})
+      T Dv[8];
+      for (int k = 0; k < 8; ++k) Dv[k] = f[k + 1] - f[k];
+      // cotangents of D_{j-1} (index 3) and D_j (index 4)
+      T Dbm1 = T(0), Db0 = T(0);
+      for (int di = -2; di <= 2; ++di) {
+        // source cell i = j + di; its D_k lives at Dv[k - (j - 4)] = Dv[di + 4 + (k - i)]
+        const long long oi = off[di + 4];
+        const T upb = ubar[((a * 2 + 0) * 9 + c) * cells + oi];
+        const T umb = ubar[((a * 2 + 1) * 9 + c) * cells + oi];
+        T db[4] = {T(0), T(0), T(0), T(0)};  // contributions to D_{i-2}, D_{i-1}, D_i, D_{i+1}
+        const int b = di + 4;                // Dv[b] = D_i
+        db[2] += N * upb;
+        db[1] += N * umb;
+        const T mb_i = T(-0.5) * N * upb;
+        const T mb_im1 = T(0.5) * N * umb;
+        {  // m_i = minmod(D_{i+1} - D_i, (D_{i+1} - D_{i-1}) / 2, D_i - D_{i-1})
+          const int sel = mm3_sel(Dv[b + 1] - Dv[b], (Dv[b + 1] - Dv[b - 1]) * T(0.5), Dv[b] - Dv[b - 1]);
+          if (sel == 0) {
+            db[3] += mb_i;
+            db[2] -= mb_i;
+          } else if (sel == 1) {
+            db[3] += T(0.5) * mb_i;
+            db[1] -= T(0.5) * mb_i;
+          } else if (sel == 2) {
+            db[2] += mb_i;
+            db[1] -= mb_i;
+          }
+        }
+        {  // m_{i-1} = minmod(D_i - D_{i-1}, (D_i - D_{i-2}) / 2, D_{i-1} - D_{i-2})
+          const int sel = mm3_sel(Dv[b] - Dv[b - 1], (Dv[b] - Dv[b - 2]) * T(0.5), Dv[b - 1] - Dv[b - 2]);
+          if (sel == 0) {
+            db[2] += mb_im1;
+            db[1] -= mb_im1;
+          } else if (sel == 1) {
+            db[2] += T(0.5) * mb_im1;
+            db[0] -= T(0.5) * mb_im1;
+          } else if (sel == 2) {
+            db[1] += mb_im1;
+            db[0] -= mb_im1;
+          }
+        }
+        // db[q] is the cotangent of D_{i-2+q} = D_{j+di-2+q}
+        for (int qq = 0; qq < 4; ++qq) {
+          const int k = di - 2 + qq;  // offset from j
+          if (k == -1) Dbm1 += db[qq];
+          if (k == 0) Db0 += db[qq];
+        }
+      }
+      acc[c] += Dbm1 - Db0;  // D_k = f_{k+1} - f_k
+    }
+  }
+  T s[9];
+  for (int c = 0; c < 9; ++c) s[c] = g.sbar[c * cells + idx];
+  const T trs = s[0] + s[4] + s[8];
+  const int SC[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+  for (int c = 0; c < 9; ++c) {
+    const int i = c / 3, j = c % 3;
+    T v = acc[c] - T(2) * T(g.mu) * s[c] - (i == j ? T(g.lam) * trs : T(0));
+    if (g.ebar) v += g.ebar[SC[c] * cells + idx];
+    for (int k = 0; k < g.nterm; ++k) v += T(g.coef[k]) * g.term[k][c * cells + idx];
+    acc[c] = v;
+  }
+  for (int c = 0; c < 9; ++c) g.out[c * cells + idx] = acc[c];
+}
+
+// sbar = S : sigma (the energy density's stress gradient, PlasticityState.py:
+// 317-330: e = 1/2 S_ijkm sigma_ij sigma_km), per-block energy sums.
+__global__ void __launch_bounds__(256) k_energy_grad(const double* G, const double* beta, long long cells, double mu,
+                                                    double nu, double lam, double* sbar, double* partial) {
+  __shared__ double red[4];
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0;
+  if (idx < cells) {
+    double b[9], s[9];
+    for (int c = 0; c < 9; ++c) b[c] = beta[c * cells + idx];
+    const double tr = b[0] + b[4] + b[8];
+    const int SC[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+    for (int c = 0; c < 9; ++c) {
+      const int i = c / 3, j = c % 3;
+      s[c] = G[SC[c] * cells + idx] - mu * (b[c] + b[3 * j + i]) - (i == j ? lam * tr : 0.0);
+    }
+    const double trs = s[0] + s[4] + s[8];
+    double ss = 0.0, sst = 0.0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        ss += s[3 * i + j] * s[3 * i + j];
+        sst += s[3 * i + j] * s[3 * j + i];
+      }
+    e = 0.5 * (2.0 * ss - sst - nu / (1.0 + nu) * trs * trs) / (2.0 * mu);
+    // de/dsigma_ij = (2 s_ij - s_ji - nu/(1+nu) tr d_ij) / (2 mu), symmetrised
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        const double g1 = (2.0 * s[3 * i + j] - s[3 * j + i] - (i == j ? nu / (1.0 + nu) * trs : 0.0)) / (2.0 * mu);
+        const double g2 = (2.0 * s[3 * j + i] - s[3 * i + j] - (i == j ? nu / (1.0 + nu) * trs : 0.0)) / (2.0 * mu);
+        sbar[(3 * i + j) * cells + idx] = 0.5 * (g1 + g2);
+      }
+  }
+  for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) t += red[k];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// out = sum_k c[k] in[k] (k < nin), elementwise over len values
+__global__ void k_lincomb(long long len, int nin, const double* a0, double c0, const double* a1, double c1,
+                          const double* a2, double c2, const double* a3, double c3, double* out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  double v = c0 * a0[i];
+  if (nin > 1) v += c1 * a1[i];
+  if (nin > 2) v += c2 * a2[i];
+  if (nin > 3) v += c3 * a3[i];
+  out[i] = v;
+}
+
+// Segment replay clock: mode 0 saves the plan's clock in clk[1] and sets up
+// a replay of `len` recorded steps from time t0; mode 1 restores clk[1].
+__global__ void k_clock_segment(Clock* clk, int mode, double t0, const double* dts, long long len) {
+  if (threadIdx.x != 0) return;
+  if (mode == 0) {
+    clk[1] = clk[0];
+    Clock c = clk[0];
+    c.t = t0;
+    c.t_end = INFINITY;
+    c.done = 0;
+    c.fixed_dt = 0.0;
+    c.dt_sched = dts;
+    c.sched_base = c.nsteps;
+    c.sched_len = len;
+    c.dt_log = nullptr;
+    c.log_base = 0;
+    c.log_cap = 0;
+    clk[0] = c;
+  } else {
+    clk[0] = clk[1];
+  }
+}
+
+hipError_t launch_clock_segment(Clock* clk, int mode, double t0, const double* dts, long long len, hipStream_t st) {
+  hipLaunchKernelGGL(k_clock_segment, dim3(1), dim3(64), 0, st, clk, mode, t0, dts, len);
+  return hipGetLastError();
+}
+
+static dim3 adj_block(int n) { return dim3(n < 32 ? n : 32, n < 8 ? n : 8); }
+
+int flux_vjp_nblocks(int d, int n) {
+  const dim3 b = adj_block(n);
+  return (d == 3 ? n : 1) * (n / (int)b.x) * (n / (int)b.y);
+}
+
+hipError_t launch_flux_vjp_cell(const VjpJob& j, int* nblocks, hipStream_t st) {
+  const dim3 bs = adj_block(j.n);
+  const int nb = flux_vjp_nblocks(j.d, j.n);
+  *nblocks = nb;
+  if (j.d == 3) {
+    if (j.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux_vjp_cell<double, 3, DYN_GLIDE_ONLY>), dim3(nb), bs, 0, st, j);
+    else hipLaunchKernelGGL((k_flux_vjp_cell<double, 3, DYN_GLIDE_CLIMB>), dim3(nb), bs, 0, st, j);
+  } else {
+    if (j.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux_vjp_cell<double, 2, DYN_GLIDE_ONLY>), dim3(nb), bs, 0, st, j);
+    else hipLaunchKernelGGL((k_flux_vjp_cell<double, 2, DYN_GLIDE_CLIMB>), dim3(nb), bs, 0, st, j);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_flux_vjp_gather(const GatherJob& g, hipStream_t st) {
+  if (g.nterm < 0 || g.nterm > 3) return hipErrorInvalidValue;
+  const dim3 bs = adj_block(g.n);
+  const int nb = flux_vjp_nblocks(g.d, g.n);
+  if (g.d == 3)
+    hipLaunchKernelGGL((k_flux_vjp_gather<double, 3>), dim3(nb), bs, 0, st, g);
+  else
+    hipLaunchKernelGGL((k_flux_vjp_gather<double, 2>), dim3(nb), bs, 0, st, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy_grad(const double* G, const double* beta, long long cells, double mu, double nu, double lam,
+                              double* sbar, double* partial, int* nblocks, hipStream_t st) {
+  const int nb = (int)((cells + 255) / 256);
+  *nblocks = nb;
+  hipLaunchKernelGGL(k_energy_grad, dim3(nb), dim3(256), 0, st, G, beta, cells, mu, nu, lam, sbar, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_lincomb(long long len, int nin, const double* a0, double c0, const double* a1, double c1,
+                          const double* a2, double c2, const double* a3, double c3, double* out, hipStream_t st) {
+  const int bs = 256;
+  hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((len + bs - 1) / bs)), dim3(bs), 0, st, len, nin, a0, c0, a1, c1, a2, c2,
+                     a3, c3, out);
+  return hipGetLastError();
+}
+
+}  // namespace cdd

@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -54,6 +55,18 @@ struct cdd_plan {
   int nblocks;
   int npartial;
   GreenParams gp;
+  // adjoint workspace (allocated on first use): 6 fields, the 2d*9
+  // derivative-cotangent planes, the stress cotangent, per-block partials
+  double* adj;
+  double* ubar;
+  double* sbar;
+  double* lpart;
+  // segment reverse pass: per-step stage states and stress planes, and the
+  // per-block Lambda cotangents of every stage of the segment
+  double* seg;
+  size_t seg_bytes;
+  double* lseg;
+  size_t lseg_len;
   // HIP-event timing of every kernel launched by cdd_step (optional)
   bool timing;
   std::vector<hipEvent_t>* ev;
@@ -147,7 +160,7 @@ int cdd_plan_create(const cdd_config* cfg, cdd_plan** out) {
   ALLOC(p->scalar, sizeof(double) * 8);
   ALLOC(p->spec, sizeof(double) * 2 * (size_t)p->spec_elems);
   ALLOC(p->tw, sizeof(double) * 2 * (size_t)n);
-  ALLOC(p->clk, sizeof(Clock));
+  ALLOC(p->clk, 2 * sizeof(Clock));  // [1]: saved clock of a segment replay
 #undef ALLOC
   if (cfg->rk_order == 3) p->state[1] = p->buf;
   // twiddle table W_n^e = exp(-2 pi i e / n)
@@ -171,6 +184,7 @@ int cdd_plan_create(const cdd_config* cfg, cdd_plan** out) {
   }
   p->stream = p->own;
   HIPCHK(hipMemsetAsync(p->state[0], 0, fb, p->stream));
+  HIPCHK(hipMemsetAsync(p->scalar, 0, sizeof(double) * 8, p->stream));
   // elastic / Green's constants
   const double mu = cfg->mu, nu = cfg->nu;
   const double lam = 2.0 * mu * nu / (1.0 - 2.0 * nu);
@@ -214,6 +228,12 @@ int cdd_plan_destroy(cdd_plan* p) {
   (void)hipFree(p->spec);
   (void)hipFree(p->tw);
   (void)hipFree(p->clk);
+  (void)hipFree(p->adj);
+  (void)hipFree(p->ubar);
+  (void)hipFree(p->sbar);
+  (void)hipFree(p->lpart);
+  (void)hipFree(p->seg);
+  (void)hipFree(p->lseg);
   if (p->own) (void)hipStreamDestroy(p->own);
   if (p->ev) {
     for (hipEvent_t e : *p->ev) (void)hipEventDestroy(e);
@@ -353,71 +373,93 @@ int cdd_flux(cdd_plan* p, const double* beta, double t, double* rhs, double* spe
   return CDD_OK;
 }
 
+// Buffers of one TVD-RK step: the stage states F0 (= u + dt L(u)) and F1
+// (RK3: 3/4 u + 1/4 (F0 + dt L(F0))), the stress planes of each stage and the
+// output.  Plain stepping shares one stress buffer and forms F0 in place
+// over L0; the adjoint's segment recomputation keeps every stage.
+struct StepBufs {
+  const double* u;
+  double* out;
+  double* G[3];
+  double* F0;
+  double* F1;
+};
+
+static int launch_step(cdd_plan* p, const StepBufs& b) {
+  const double nmax = (double)p->n;
+  double* u = const_cast<double*>(b.u);
+  // stage 1: L0 = L(u); dt from the stage-1 speeds
+  if (p->timing) timing_hook(p, -1);
+  HIPCHK(launch_spectral(spectral_job(p, 6, u, nullptr, p->clk, 0.0, b.G[0])));
+  FluxJob f = flux_job(p);
+  f.G = b.G[0];
+  f.clk = p->clk;
+  f.A = u;
+  f.out = p->L0;
+  f.mode = 0;
+  f.partial = p->partial;
+  int nb = 0;
+  HIPCHK(launch_flux(f, &nb));
+  if (p->timing) timing_hook(p, KK_FLUX1);
+  HIPCHK(launch_finalize_dt(p->partial, nb, p->clk, nmax, p->stream));
+  if (p->timing) timing_hook(p, KK_FINALIZE);
+  // stage 2 on F0 = u + dt L0: the R2C pass forms F0 while reading u and
+  // L0 for the stress solve and stores it in F0 (which may be L0 itself)
+  {
+    SpectralJob j2 = spectral_job(p, 6, u, p->L0, p->clk, 0.0, b.G[1]);
+    j2.src.W = b.F0;
+    HIPCHK(launch_spectral(j2));
+  }
+  FluxJob f2 = flux_job(p);
+  f2.G = b.G[1];
+  f2.clk = p->clk;
+  f2.A = b.F0;
+  f2.U = u;
+  f2.mode = 1;
+  if (p->cfg.rk_order == 3) {
+    f2.out = b.F1;
+    f2.cu = 0.75;
+    f2.cf = 0.25;
+    HIPCHK(launch_flux(f2, nullptr));
+    if (p->timing) timing_hook(p, KK_FLUX2);
+    // stage 3 on F1
+    HIPCHK(launch_spectral(spectral_job(p, 6, b.F1, nullptr, p->clk, 0.0, b.G[2])));
+    FluxJob f3 = flux_job(p);
+    f3.G = b.G[2];
+    f3.clk = p->clk;
+    f3.A = b.F1;
+    f3.U = u;
+    f3.out = b.out;  // may be u: u is only read at the thread's own cell
+    f3.mode = 1;
+    f3.cu = 1.0 / 3.0;
+    f3.cf = 2.0 / 3.0;
+    HIPCHK(launch_flux(f3, nullptr));
+    if (p->timing) timing_hook(p, KK_FLUX3);
+  } else {
+    f2.out = b.out;
+    f2.cu = 0.5;
+    f2.cf = 0.5;
+    HIPCHK(launch_flux(f2, nullptr));
+    if (p->timing) timing_hook(p, KK_FLUX2);
+  }
+  HIPCHK(launch_advance(p->clk, p->stream));
+  if (p->timing) timing_hook(p, KK_ADVANCE);
+  return CDD_OK;
+}
+
 int cdd_step(cdd_plan* p, int nsteps) {
   if (!p || nsteps < 0) return fail(CDD_ERR_ARG, "bad argument");
   HIPCHK(hipSetDevice(p->cfg.device));
-  const double nmax = (double)p->n;
   for (int s = 0; s < nsteps; ++s) {
     const int ci = cur_state(p);
-    double* u = p->state[ci];
-    // stage 1: L0 = L(u); dt from the stage-1 speeds
-    if (p->timing) timing_hook(p, -1);
-    HIPCHK(launch_spectral(spectral_job(p, 6, u, nullptr, p->clk, 0.0, p->G)));
-    FluxJob f = flux_job(p);
-    f.clk = p->clk;
-    f.A = u;
-    f.out = p->L0;
-    f.mode = 0;
-    f.partial = p->clk ? p->partial : nullptr;
-    int nb = 0;
-    HIPCHK(launch_flux(f, &nb));
-    if (p->timing) timing_hook(p, KK_FLUX1);
-    HIPCHK(launch_finalize_dt(p->partial, nb, p->clk, nmax, p->stream));
-    if (p->timing) timing_hook(p, KK_FINALIZE);
-    // stage 2 on F0 = u + dt L0: the R2C pass forms F0 while reading u and
-    // L0 for the stress solve and stores it over L0
-    {
-      SpectralJob j2 = spectral_job(p, 6, u, p->L0, p->clk, 0.0, p->G);
-      j2.src.W = p->L0;
-      HIPCHK(launch_spectral(j2));
-    }
-    if (p->cfg.rk_order == 3) {
-      FluxJob f2 = flux_job(p);
-      f2.clk = p->clk;
-      f2.A = p->L0;
-      f2.U = u;
-      f2.out = p->buf;
-      f2.mode = 1;
-      f2.cu = 0.75;
-      f2.cf = 0.25;
-      HIPCHK(launch_flux(f2, nullptr));
-      if (p->timing) timing_hook(p, KK_FLUX2);
-      // stage 3 on F1
-      HIPCHK(launch_spectral(spectral_job(p, 6, p->buf, nullptr, p->clk, 0.0, p->G)));
-      FluxJob f3 = flux_job(p);
-      f3.clk = p->clk;
-      f3.A = p->buf;
-      f3.U = u;
-      f3.out = u;  // in place: u is only read at the thread's own cell
-      f3.mode = 1;
-      f3.cu = 1.0 / 3.0;
-      f3.cf = 2.0 / 3.0;
-      HIPCHK(launch_flux(f3, nullptr));
-      if (p->timing) timing_hook(p, KK_FLUX3);
-    } else {
-      FluxJob f2 = flux_job(p);
-      f2.clk = p->clk;
-      f2.A = p->L0;
-      f2.U = u;
-      f2.out = p->state[ci ^ 1];
-      f2.mode = 1;
-      f2.cu = 0.5;
-      f2.cf = 0.5;
-      HIPCHK(launch_flux(f2, nullptr));
-      if (p->timing) timing_hook(p, KK_FLUX2);
-    }
-    HIPCHK(launch_advance(p->clk, p->stream));
-    if (p->timing) timing_hook(p, KK_ADVANCE);
+    StepBufs b;
+    b.u = p->state[ci];
+    b.out = p->cfg.rk_order == 3 ? p->state[ci] : p->state[ci ^ 1];
+    b.G[0] = b.G[1] = b.G[2] = p->G;
+    b.F0 = p->L0;
+    b.F1 = p->buf;
+    int rc = launch_step(p, b);
+    if (rc) return rc;
     p->steps_launched += 1;
   }
   return CDD_OK;
@@ -433,9 +475,328 @@ int cdd_energy(cdd_plan* p, const double* beta, double t, double* energy) {
   HIPCHK(launch_spectral(spectral_job(p, 6, beta, nullptr, nullptr, t, p->G)));
   int nb = 0;
   HIPCHK(launch_energy(p->G, beta, p->cells, p->gp.mu, p->gp.el.nu, p->gp.lam, p->partial, &nb, p->stream));
-  HIPCHK(launch_reduce_sum(p->partial, nb, p->scalar, p->stream));
+  HIPCHK(launch_reduce_sum(p->partial, nb, p->scalar, p->stream, 0));
   HIPCHK(hipMemcpyAsync(energy, p->scalar, sizeof(double), hipMemcpyDeviceToHost, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
+  return CDD_OK;
+}
+
+// ------------------------------------------------------------- adjoint
+static int adj_alloc(cdd_plan* p) {
+  if (p->adj) return CDD_OK;
+  const size_t fb = sizeof(double) * 9 * (size_t)p->cells;
+  if (p->cfg.load_kind == CDD_LOAD_UNIAXIAL)
+    return fail(CDD_ERR_UNSUPPORTED, "the adjoint supports loads that do not depend on beta (none/strain/stress/tensor)");
+  if (hipMalloc((void**)&p->adj, 6 * fb) != hipSuccess || hipMalloc((void**)&p->ubar, 2 * p->d * fb) != hipSuccess ||
+      hipMalloc((void**)&p->sbar, fb) != hipSuccess ||
+      hipMalloc((void**)&p->lpart, sizeof(double) * (size_t)std::max<long long>((p->cells + 255) / 256,
+                                                                                flux_vjp_nblocks(p->d, p->n))) !=
+          hipSuccess)
+    return fail(CDD_ERR_HIP, "hipMalloc failed (adjoint workspace)");
+  return CDD_OK;
+}
+
+static double* adjx(cdd_plan* p, int k) { return p->adj + (size_t)k * 9 * p->cells; }
+
+// out = J_L(beta)^T rbar; Lambda cotangent accumulated into scalar[1]
+static int flux_vjp_impl(cdd_plan* p, const double* beta, double t, const double* rbar, double* out) {
+  HIPCHK(launch_spectral(spectral_job(p, 6, beta, nullptr, nullptr, t, p->G)));
+  VjpJob j;
+  memset(&j, 0, sizeof j);
+  j.d = p->d;
+  j.n = p->n;
+  j.dyn = p->cfg.dynamics;
+  j.cells = p->cells;
+  j.beta = beta;
+  j.G = p->G;
+  j.rbar = rbar;
+  j.rscale = 1.0;
+  j.dtp = nullptr;
+  j.ubar = p->ubar;
+  j.sbar = p->sbar;
+  j.lpart = p->lpart;
+  j.Lambda = p->cfg.dynamics == CDD_GLIDE_ONLY ? 0.0 : p->cfg.Lambda;
+  j.mu = p->gp.mu;
+  j.lam = p->gp.lam;
+  int nb = 0;
+  HIPCHK(launch_flux_vjp_cell(j, &nb, p->stream));
+  HIPCHK(launch_reduce_sum(p->lpart, nb, p->scalar + 1, p->stream, 1));
+  SpectralJob jt = spectral_job(p, 6, p->sbar, nullptr, nullptr, t, p->G);
+  jt.trans = 1;
+  HIPCHK(launch_spectral(jt));
+  GatherJob g;
+  memset(&g, 0, sizeof g);
+  g.d = p->d;
+  g.n = p->n;
+  g.cells = p->cells;
+  g.beta = beta;
+  g.ubar = p->ubar;
+  g.sbar = p->sbar;
+  g.ebar = p->G;
+  g.mu = p->gp.mu;
+  g.lam = p->gp.lam;
+  g.out = out;
+  HIPCHK(launch_flux_vjp_gather(g, p->stream));
+  return CDD_OK;
+}
+
+static int lincomb(cdd_plan* p, double* out, int nin, const double* a0, double c0, const double* a1 = nullptr,
+                   double c1 = 0, const double* a2 = nullptr, double c2 = 0, const double* a3 = nullptr, double c3 = 0) {
+  HIPCHK(launch_lincomb(9 * p->cells, nin, a0, c0, a1, c1, a2, c2, a3, c3, out, p->stream));
+  return CDD_OK;
+}
+
+static int flux_into(cdd_plan* p, const double* beta, double t, double* out) {
+  HIPCHK(launch_spectral(spectral_job(p, 6, beta, nullptr, nullptr, t, p->G)));
+  FluxJob f = flux_job(p);
+  f.A = beta;
+  f.out = out;
+  f.mode = 0;
+  HIPCHK(launch_flux(f, nullptr));
+  return CDD_OK;
+}
+
+#define RCHK(x)          \
+  do {                   \
+    int rc_ = (x);       \
+    if (rc_) return rc_; \
+  } while (0)
+
+int cdd_flux_vjp(cdd_plan* p, const double* beta, double t, const double* rbar, double* betabar, double* lambar) {
+  if (!p || !beta || !rbar || !betabar) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  RCHK(adj_alloc(p));
+  HIPCHK(hipMemsetAsync(p->scalar + 1, 0, sizeof(double), p->stream));
+  RCHK(flux_vjp_impl(p, beta, t, rbar, betabar));
+  if (lambar) {
+    HIPCHK(hipMemcpyAsync(lambar, p->scalar + 1, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+  }
+  return CDD_OK;
+}
+
+int cdd_step_vjp(cdd_plan* p, const double* u, double t, double dt, const double* ubar_next, double* ubar,
+                 double* lambar) {
+  if (!p || !u || !ubar_next || !ubar) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  RCHK(adj_alloc(p));
+  HIPCHK(hipMemsetAsync(p->scalar + 1, 0, sizeof(double), p->stream));
+  double *x0 = adjx(p, 0), *x1 = adjx(p, 1), *x2 = adjx(p, 2), *x3 = adjx(p, 3), *x4 = adjx(p, 4),
+         *x5 = adjx(p, 5);
+  if (p->cfg.rk_order == 2) {
+    // forward: F0 = u + dt L(u)
+    RCHK(flux_into(p, u, t, x0));
+    RCHK(lincomb(p, x1, 2, u, 1.0, x0, dt));
+    // reverse of u' = u/2 + (F0 + dt L(F0))/2
+    RCHK(lincomb(p, x2, 1, ubar_next, 0.5 * dt));
+    RCHK(flux_vjp_impl(p, x1, t, x2, x3));
+    RCHK(lincomb(p, x4, 2, ubar_next, 0.5, x3, 1.0));  // F0bar
+    RCHK(lincomb(p, x2, 1, x4, dt));
+    RCHK(flux_vjp_impl(p, u, t, x2, x3));
+    RCHK(lincomb(p, ubar, 3, ubar_next, 0.5, x4, 1.0, x3, 1.0));
+  } else {
+    // forward: F0 = u + dt L(u), F1 = 3/4 u + 1/4 (F0 + dt L(F0))
+    RCHK(flux_into(p, u, t, x0));
+    RCHK(lincomb(p, x1, 2, u, 1.0, x0, dt));
+    RCHK(flux_into(p, x1, t, x0));
+    RCHK(lincomb(p, x2, 3, u, 0.75, x1, 0.25, x0, 0.25 * dt));
+    // reverse of u' = u/3 + 2/3 (F1 + dt L(F1))
+    RCHK(lincomb(p, x3, 1, ubar_next, 2.0 / 3.0 * dt));
+    RCHK(flux_vjp_impl(p, x2, t, x3, x0));
+    RCHK(lincomb(p, x4, 2, ubar_next, 2.0 / 3.0, x0, 1.0));  // F1bar
+    RCHK(lincomb(p, x3, 1, x4, 0.25 * dt));
+    RCHK(flux_vjp_impl(p, x1, t, x3, x0));
+    RCHK(lincomb(p, x5, 2, x4, 0.25, x0, 1.0));  // F0bar
+    RCHK(lincomb(p, x3, 1, x5, dt));
+    RCHK(flux_vjp_impl(p, u, t, x3, x0));
+    RCHK(lincomb(p, ubar, 4, ubar_next, 1.0 / 3.0, x4, 0.75, x5, 1.0, x0, 1.0));
+  }
+  if (lambar) {
+    HIPCHK(hipMemcpyAsync(lambar, p->scalar + 1, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+  }
+  return CDD_OK;
+}
+
+int cdd_energy_grad(cdd_plan* p, const double* beta, double t, double* betabar, double* energy) {
+  if (!p || !beta || !betabar) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  RCHK(adj_alloc(p));
+  HIPCHK(launch_spectral(spectral_job(p, 6, beta, nullptr, nullptr, t, p->G)));
+  int nb = 0;
+  HIPCHK(launch_energy_grad(p->G, beta, p->cells, p->gp.mu, p->gp.el.nu, p->gp.lam, p->sbar, p->lpart, &nb,
+                            p->stream));
+  HIPCHK(launch_reduce_sum(p->lpart, nb, p->scalar + 2, p->stream, 0));
+  SpectralJob jt = spectral_job(p, 6, p->sbar, nullptr, nullptr, t, p->G);
+  jt.trans = 1;
+  HIPCHK(launch_spectral(jt));
+  HIPCHK(launch_sigma_finalize(p->G, p->sbar, betabar, p->cells, p->gp.mu, p->gp.lam, p->stream));
+  if (energy) {
+    HIPCHK(hipMemcpyAsync(energy, p->scalar + 2, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+  }
+  return CDD_OK;
+}
+
+// One RK stage of the segment reverse pass: out = sum_k coef_k term_k +
+// J_L(beta)^T (rscale dt rbar), with beta's stress planes G kept from the
+// recomputation and the per-block Lambda cotangents written to lpart.
+static int stage_vjp(cdd_plan* p, const double* beta, const double* G, const double* rbar, double rscale,
+                     const double* dtp, double* lpart, double* out, int nterm, const double* t0, double c0,
+                     const double* t1 = nullptr, double c1 = 0.0, const double* t2 = nullptr, double c2 = 0.0) {
+  VjpJob j;
+  memset(&j, 0, sizeof j);
+  j.d = p->d;
+  j.n = p->n;
+  j.dyn = p->cfg.dynamics;
+  j.cells = p->cells;
+  j.beta = beta;
+  j.G = G;
+  j.rbar = rbar;
+  j.rscale = rscale;
+  j.dtp = dtp;
+  j.ubar = p->ubar;
+  j.sbar = p->sbar;
+  j.lpart = lpart;
+  j.Lambda = p->cfg.dynamics == CDD_GLIDE_ONLY ? 0.0 : p->cfg.Lambda;
+  j.mu = p->gp.mu;
+  j.lam = p->gp.lam;
+  int nb = 0;
+  HIPCHK(launch_flux_vjp_cell(j, &nb, p->stream));
+  SpectralJob jt = spectral_job(p, 6, p->sbar, nullptr, nullptr, 0.0, p->G);
+  jt.trans = 1;
+  HIPCHK(launch_spectral(jt));
+  GatherJob g;
+  memset(&g, 0, sizeof g);
+  g.d = p->d;
+  g.n = p->n;
+  g.cells = p->cells;
+  g.beta = beta;
+  g.ubar = p->ubar;
+  g.sbar = p->sbar;
+  g.ebar = p->G;
+  g.mu = p->gp.mu;
+  g.lam = p->gp.lam;
+  g.nterm = nterm;
+  g.term[0] = t0;
+  g.coef[0] = c0;
+  g.term[1] = t1;
+  g.coef[1] = c1;
+  g.term[2] = t2;
+  g.coef[2] = c2;
+  g.out = out;
+  HIPCHK(launch_flux_vjp_gather(g, p->stream));
+  return CDD_OK;
+}
+
+static int seg_alloc(cdd_plan* p, int nsteps) {
+  const int S = p->cfg.rk_order;
+  const size_t plane = sizeof(double) * (size_t)p->cells;
+  const size_t need = (size_t)nsteps * S * 15 * plane + 9 * plane;
+  const size_t lneed = (size_t)S * nsteps * flux_vjp_nblocks(p->d, p->n);
+  if (need > p->seg_bytes || lneed > p->lseg_len) HIPCHK(hipStreamSynchronize(p->stream));
+  if (need > p->seg_bytes) {
+    (void)hipFree(p->seg);
+    p->seg = nullptr;
+    p->seg_bytes = 0;
+    if (hipMalloc((void**)&p->seg, need) != hipSuccess)
+      return fail(CDD_ERR_HIP, ("hipMalloc failed (segment workspace of " + std::to_string(need) +
+                                " bytes; use shorter checkpoint intervals)").c_str());
+    p->seg_bytes = need;
+  }
+  if (lneed > p->lseg_len) {
+    (void)hipFree(p->lseg);
+    p->lseg = nullptr;
+    p->lseg_len = 0;
+    if (hipMalloc((void**)&p->lseg, sizeof(double) * lneed) != hipSuccess)
+      return fail(CDD_ERR_HIP, "hipMalloc failed (segment Lambda partials)");
+    p->lseg_len = lneed;
+  }
+  return CDD_OK;
+}
+
+int cdd_segment_vjp(cdd_plan* p, const double* u0, double t0, const double* dts_dev, int nsteps,
+                    const double* ubar_end, double* ubar0) {
+  if (!p || !u0 || !dts_dev || !ubar_end || !ubar0 || nsteps < 1) return fail(CDD_ERR_ARG, "bad argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  RCHK(adj_alloc(p));
+  RCHK(seg_alloc(p, nsteps));
+  const int S = p->cfg.rk_order;
+  const long long cells = p->cells;
+  auto st = [&](int j, int k) { return p->seg + ((size_t)j * S + k) * 9 * cells; };
+  double* gbase = p->seg + (size_t)nsteps * S * 9 * cells;
+  auto gs = [&](int j, int k) { return gbase + ((size_t)j * S + k) * 6 * cells; };
+  double* ufinal = gbase + (size_t)nsteps * S * 6 * cells;
+  // recompute the segment from u0, replaying the recorded time steps and
+  // keeping every stage state and its stress planes
+  HIPCHK(launch_clock_segment(p->clk, 0, t0, dts_dev, nsteps, p->stream));
+  for (int j = 0; j < nsteps; ++j) {
+    StepBufs b;
+    b.u = j == 0 ? u0 : st(j, 0);
+    b.out = j + 1 < nsteps ? st(j + 1, 0) : ufinal;
+    for (int k = 0; k < 3; ++k) b.G[k] = gs(j, k < S ? k : S - 1);
+    b.F0 = st(j, 1);
+    b.F1 = S == 3 ? st(j, 2) : nullptr;
+    RCHK(launch_step(p, b));
+  }
+  HIPCHK(launch_clock_segment(p->clk, 1, 0.0, nullptr, 0, p->stream));
+  // reverse, last step first
+  const int nb = flux_vjp_nblocks(p->d, p->n);
+  double *A = adjx(p, 0), *B = adjx(p, 1), *F0b = adjx(p, 2), *F1b = adjx(p, 3);
+  const double* gin = ubar_end;
+  for (int j = nsteps - 1; j >= 0; --j) {
+    double* gout = j == 0 ? ubar0 : (gin == A ? B : A);
+    const double* dtp = dts_dev + j;
+    double* lp = p->lseg + (size_t)j * S * nb;
+    const double* u = j == 0 ? u0 : st(j, 0);
+    if (S == 2) {
+      // u' = u/2 + (F0 + dt L(F0))/2,  F0 = u + dt L(u)
+      RCHK(stage_vjp(p, st(j, 1), gs(j, 1), gin, 0.5, dtp, lp + nb, F0b, 1, gin, 0.5));
+      RCHK(stage_vjp(p, u, gs(j, 0), F0b, 1.0, dtp, lp, gout, 2, gin, 0.5, F0b, 1.0));
+    } else {
+      // u' = u/3 + 2/3 (F1 + dt L(F1)),  F1 = 3/4 u + 1/4 (F0 + dt L(F0))
+      RCHK(stage_vjp(p, st(j, 2), gs(j, 2), gin, 2.0 / 3.0, dtp, lp + 2 * nb, F1b, 1, gin, 2.0 / 3.0));
+      RCHK(stage_vjp(p, st(j, 1), gs(j, 1), F1b, 0.25, dtp, lp + nb, F0b, 1, F1b, 0.25));
+      RCHK(stage_vjp(p, u, gs(j, 0), F0b, 1.0, dtp, lp, gout, 3, gin, 1.0 / 3.0, F1b, 0.75, F0b, 1.0));
+    }
+    gin = gout;
+  }
+  HIPCHK(launch_reduce_sum(p->lseg, S * nsteps * nb, p->scalar + 3, p->stream, 1));
+  return CDD_OK;
+}
+
+int cdd_lambda_bar(cdd_plan* p, double* value, int reset) {
+  if (!p) return fail(CDD_ERR_ARG, "null plan");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  if (value) {
+    HIPCHK(hipMemcpyAsync(value, p->scalar + 3, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  }
+  if (reset) HIPCHK(hipMemsetAsync(p->scalar + 3, 0, sizeof(double), p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return CDD_OK;
+}
+
+int cdd_dt_record(cdd_plan* p, double* log_dev, long long cap) {
+  if (!p || cap < 0) return fail(CDD_ERR_ARG, "bad argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  Clock c;
+  if (sync_clock(p, &c)) return CDD_ERR_HIP;
+  c.dt_log = cap > 0 ? log_dev : nullptr;
+  c.log_base = c.nsteps;
+  c.log_cap = cap;
+  HIPCHK(hipMemcpy(p->clk, &c, sizeof c, hipMemcpyHostToDevice));
+  return CDD_OK;
+}
+
+int cdd_dt_replay(cdd_plan* p, const double* sched_dev, long long len) {
+  if (!p || len < 0) return fail(CDD_ERR_ARG, "bad argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  Clock c;
+  if (sync_clock(p, &c)) return CDD_ERR_HIP;
+  c.dt_sched = len > 0 ? sched_dev : nullptr;
+  c.sched_base = c.nsteps;
+  c.sched_len = len;
+  HIPCHK(hipMemcpy(p->clk, &c, sizeof c, hipMemcpyHostToDevice));
   return CDD_OK;
 }
 

@@ -44,6 +44,12 @@ struct Clock {
   int done;         // 1 once t >= t_end: every step kernel returns at once
   int pad;
   long long nsteps;  // completed steps
+  // frozen time-step schedule (adjoint replay): dt = dt_sched[nsteps - sched_base]
+  const double* dt_sched;
+  long long sched_base, sched_len;
+  // optional log of the time steps taken: dt_log[nsteps - log_base] = dt
+  double* dt_log;
+  long long log_base, log_cap;
 };
 
 // Uniform load added to sigma (it enters the stress solve as the k = 0 mode).

@@ -551,6 +551,8 @@ __global__ void __launch_bounds__(1024) k_finalize_dt(const double* partial, int
     double mm = -INFINITY;
     for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) mm = fmax(mm, red[k]);
     double dt = clk->fixed_dt > 0.0 ? clk->fixed_dt : clk->cfl / (nmax * mm);
+    const long long si = clk->nsteps - clk->sched_base;
+    if (clk->dt_sched && si >= 0 && si < clk->sched_len) dt = clk->dt_sched[si];
     if (clk->t + dt > clk->t_end) dt = clk->t_end - clk->t;
     clk->dt = dt;
     clk->last_max = mm;
@@ -562,6 +564,8 @@ __global__ void k_fixed_dt(Clock* clk, int count) {
   clk += threadIdx.x;
   if (clk->done) return;
   double dt = clk->fixed_dt;
+  const long long si = clk->nsteps - clk->sched_base;
+  if (clk->dt_sched && si >= 0 && si < clk->sched_len) dt = clk->dt_sched[si];
   if (clk->t + dt > clk->t_end) dt = clk->t_end - clk->t;
   clk->dt = dt;
 }
@@ -571,6 +575,8 @@ __global__ void k_advance(Clock* clk, int count) {
   if ((int)threadIdx.x >= count) return;
   clk += threadIdx.x;
   if (clk->done) return;
+  const long long li = clk->nsteps - clk->log_base;
+  if (clk->dt_log && li >= 0 && li < clk->log_cap) clk->dt_log[li] = clk->dt;
   clk->t += clk->dt;
   clk->nsteps += 1;
   if (clk->t >= clk->t_end) clk->done = 1;
@@ -626,7 +632,7 @@ __global__ void __launch_bounds__(256) k_energy(const double* G, const double* b
   }
 }
 
-__global__ void __launch_bounds__(1024) k_reduce_sum(const double* partial, int nb, double* out) {
+__global__ void __launch_bounds__(1024) k_reduce_sum(const double* partial, int nb, double* out, int accumulate) {
   __shared__ double red[16];
   double m = 0.0;
   for (int k = threadIdx.x; k < nb; k += blockDim.x) m += partial[k];
@@ -636,7 +642,7 @@ __global__ void __launch_bounds__(1024) k_reduce_sum(const double* partial, int 
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) t += red[k];
-    *out = t;
+    *out = accumulate ? *out + t : t;
   }
 }
 
@@ -648,8 +654,8 @@ hipError_t launch_energy(const double* G, const double* beta, long long cells, d
   return hipGetLastError();
 }
 
-hipError_t launch_reduce_sum(const double* partial, int n, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(1024), 0, st, partial, n, out);
+hipError_t launch_reduce_sum(const double* partial, int n, double* out, hipStream_t st, int accumulate) {
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(1024), 0, st, partial, n, out, accumulate);
   return hipGetLastError();
 }
 

@@ -49,6 +49,7 @@ struct SpectralJob {
   int count;
   long long sstride;     // spectrum (complex elements)
   long long ostride;     // out (elements)
+  int trans;             // 1: transposed Green's contraction (adjoint), no load mode
 };
 
 hipError_t launch_spectral(const SpectralJob& job);
@@ -86,6 +87,43 @@ hipError_t launch_advance(Clock* clk, hipStream_t st, int count = 1);
 int flux_nblocks_tile(int d, int n, int tile);
 hipError_t launch_energy(const double* G, const double* beta, long long cells, double mu, double nu, double lam,
                          double* partial, int* nblocks, hipStream_t st);
-hipError_t launch_reduce_sum(const double* partial, int n, double* out, hipStream_t st);
+hipError_t launch_reduce_sum(const double* partial, int n, double* out, hipStream_t st, int accumulate = 0);
+
+struct VjpJob {
+  int d, n, dyn;
+  long long cells;
+  const double* beta;  // state the flux is linearised at
+  const double* G;     // its non-local stress planes
+  const double* rbar;  // cotangent of the flux (9 planes), used as rscale * dt * rbar
+  double rscale;
+  const double* dtp;   // optional device dt (the factor is 1 when null)
+  double* ubar;        // out: 2 d 9 derivative-cotangent planes
+  double* sbar;        // out: symmetrised stress cotangent (9 planes)
+  double* lpart;       // out: per-block Lambda cotangent
+  double Lambda, mu, lam;
+};
+// out = J_L(beta)^T rbar assembled from the derivative cotangents, the stress
+// cotangent and (optional) its spectral transpose ebar, plus sum_k coef_k term_k
+struct GatherJob {
+  int d, n;
+  long long cells;
+  const double* beta;
+  const double* ubar;
+  const double* sbar;
+  const double* ebar;
+  double mu, lam;
+  int nterm;
+  const double* term[3];
+  double coef[3];
+  double* out;
+};
+int flux_vjp_nblocks(int d, int n);
+hipError_t launch_clock_segment(Clock* clk, int mode, double t0, const double* dts, long long len, hipStream_t st);
+hipError_t launch_flux_vjp_cell(const VjpJob& j, int* nblocks, hipStream_t st);
+hipError_t launch_flux_vjp_gather(const GatherJob& g, hipStream_t st);
+hipError_t launch_energy_grad(const double* G, const double* beta, long long cells, double mu, double nu, double lam,
+                              double* sbar, double* partial, int* nblocks, hipStream_t st);
+hipError_t launch_lincomb(long long len, int nin, const double* a0, double c0, const double* a1, double c1,
+                          const double* a2, double c2, const double* a3, double c3, double* out, hipStream_t st);
 
 }  // namespace cdd

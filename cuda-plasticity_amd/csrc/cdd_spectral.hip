@@ -241,6 +241,38 @@ __device__ __forceinline__ cx<T> green_sigma(const cx<T>* e, int c, T kx, T ky, 
   return out * T(gp.norm);
 }
 
+// Transpose of green_sigma w.r.t. the Frobenius product of symmetric
+// matrices: with Gb (the 6 components of a symmetric cotangent), tg = tr Gb,
+// gk = Gb k, qg = k.Gb.k,
+//   (K^T Gb)_ij = d_ij (lam tg kSq A/K2 + 2 mu qg A/K2)
+//               + k_i k_j (-lam tg kSq C/K4 + 2 lam tg/K2 - 2 mu qg C/K4)
+//               + 2 mu/K2 (gk_i k_j + k_i gk_j).
+// (It equals green_sigma up to the ME regularisation of K2, K4.)
+template <typename T>
+__device__ __forceinline__ cx<T> green_sigma_T(const cx<T>* g, int c, T kx, T ky, T kz, const GreenParams& gp) {
+  const T kSq = kx * kx + ky * ky + kz * kz;
+  const T K2 = kSq + T(kME);
+  const T K4 = kSq * kSq + T(kME);
+  const T iK2 = T(1) / K2, iK4 = T(1) / K4;
+  const cx<T> tg = g[0] + g[1] + g[2];
+  cx<T> gk[3];
+  gk[0] = g[0] * kx + g[3] * ky + g[4] * kz;
+  gk[1] = g[3] * kx + g[1] * ky + g[5] * kz;
+  gk[2] = g[4] * kx + g[5] * ky + g[2] * kz;
+  const cx<T> qg = gk[0] * kx + gk[1] * ky + gk[2] * kz;
+  const T lam = T(gp.lam), mu = T(gp.mu), A = T(gp.A), C = T(gp.C);
+  const int i = c < 3 ? c : (c == 5 ? 1 : 0);
+  const int j = c < 3 ? c : (c == 3 ? 1 : 2);
+  const T ki = i == 0 ? kx : (i == 1 ? ky : kz);
+  const T kj = j == 0 ? kx : (j == 1 ? ky : kz);
+  const cx<T> gi = i == 0 ? gk[0] : (i == 1 ? gk[1] : gk[2]);
+  const cx<T> gj = j == 0 ? gk[0] : (j == 1 ? gk[1] : gk[2]);
+  cx<T> out = (tg * (-lam * kSq * C * iK4 + T(2) * lam * iK2) - qg * (T(2) * mu * C * iK4)) * (ki * kj) +
+              (gi * kj + gj * ki) * (T(2) * mu * iK2);
+  if (i == j) out = out + tg * (lam * kSq * A * iK2) + qg * (T(2) * mu * A * iK2);
+  return out * T(gp.norm);
+}
+
 // Nye tensor rho_ij = -eps_ilm d_l beta_mj in k space (times 1/N^d).
 template <typename T>
 __device__ __forceinline__ cx<T> green_rho(const cx<T>* b, int c, T kx, T ky, T kz, T norm) {
@@ -254,7 +286,7 @@ __device__ __forceinline__ cx<T> green_rho(const cx<T>* b, int c, T kx, T ky, T 
 
 // Rows of spec laid out as (c * NR + row) * N + e.  grid.x = ceil(NR / TR).
 // 3D: row = kz * N + x, e = y;  2D: row = ky, e = x.
-template <typename T, int N, int NC, int D>
+template <typename T, int N, int NC, int D, bool TRANS = false>
 __global__ void __launch_bounds__((NC * ContractTiles<N, NC>::TR * ContractTiles<N, NC>::TT))
 k_contract(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, GreenParams gp, long long sstride) {
   using CT = ContractTiles<N, NC>;
@@ -307,7 +339,9 @@ k_contract(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, GreenParams g
     cx<T> o;
     if constexpr (NC == 6) {
       const bool k0 = (row == 0) && (e == 0);
-      if (!k0) {
+      if (TRANS) {
+        o = k0 ? cx<T>{T(0), T(0)} : green_sigma_T<T>(in, c, kx, ky, kz, gp);
+      } else if (!k0) {
         o = green_sigma<T>(in, c, kx, ky, kz, gp);
       } else {
         // k = 0: kmask removes the Green's term; the uniform load enters here
@@ -388,8 +422,14 @@ static hipError_t launch_spectral_n(const SpectralJob& job) {
   if (NC == 6) {
     using CT = ContractTiles<N, 6>;
     dim3 g3((unsigned)((NR + CT::TR - 1) / CT::TR), count), b3(6 * CT::TR * CT::TT);
-    if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 6, 3>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
-    else hipLaunchKernelGGL((k_contract<T, N, 6, 2>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+    if (job.trans) {
+      if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 6, 3, true>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+      else hipLaunchKernelGGL((k_contract<T, N, 6, 2, true>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+    } else if (d == 3) {
+      hipLaunchKernelGGL((k_contract<T, N, 6, 3>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+    } else {
+      hipLaunchKernelGGL((k_contract<T, N, 6, 2>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+    }
   } else {
     using CT = ContractTiles<N, 9>;
     dim3 g3((unsigned)((NR + CT::TR - 1) / CT::TR), count), b3(9 * CT::TR * CT::TT);

@@ -133,6 +133,41 @@ int cdd_state_ptr(cdd_plan* plan, double** beta_dev);
 int cdd_timing_enable(cdd_plan* plan, int max_events);
 int cdd_timing_read(cdd_plan* plan, double* ms, int* counts, int nkinds);
 
+/* ---- Adjoint (no reference counterpart: the reference has no gradients) --
+ * Vector-Jacobian products with the time-step schedule frozen: the CFL
+ * reduction (FieldMover.py:94-96) is treated as a constant.  Loads that
+ * depend on beta (CDD_LOAD_UNIAXIAL) are not supported.
+ *   cdd_flux_vjp:    betabar = (dL/dbeta)^T rbar for the flux L of
+ *                    BetaPDynamics.CalculateFlux at time t; *lambar =
+ *                    (dL/dLambda) . rbar (nullable).
+ *   cdd_step_vjp:    ubar = (du'/du)^T ubar_next for one TVD-RK step
+ *                    u -> u' with step dt at time t (the plan's rk_order);
+ *                    *lambar = (du'/dLambda) . ubar_next.
+ *   cdd_energy_grad: total elastic energy of beta at t and its gradient.
+ *   cdd_dt_record:   log every step's dt into a device array (cap entries).
+ *   cdd_dt_replay:   take the next len steps' dt from a device array
+ *                    (replaying a recorded schedule exactly).
+ *   cdd_segment_vjp: reverse pass over one checkpoint segment: recompute
+ *                    nsteps steps from the checkpoint u0 at time t0
+ *                    replaying dts_dev (the recorded dt of those steps),
+ *                    keeping every stage state and its stress planes, then
+ *                    carry the cotangent of the segment's final state
+ *                    (ubar_end) back to ubar0 (may alias ubar_end).  The
+ *                    plan's clock and state are left unchanged; the Lambda
+ *                    cotangent is added to a device accumulator.  No host
+ *                    synchronisation.  Workspace: 15 rk_order planes per step.
+ *   cdd_lambda_bar:  read (nullable) and optionally reset that accumulator. */
+int cdd_flux_vjp(cdd_plan* plan, const double* beta_dev, double t, const double* rbar_dev, double* betabar_dev,
+                 double* lambar);
+int cdd_step_vjp(cdd_plan* plan, const double* u_dev, double t, double dt, const double* ubar_next_dev,
+                 double* ubar_dev, double* lambar);
+int cdd_energy_grad(cdd_plan* plan, const double* beta_dev, double t, double* betabar_dev, double* energy);
+int cdd_dt_record(cdd_plan* plan, double* log_dev, long long cap);
+int cdd_dt_replay(cdd_plan* plan, const double* sched_dev, long long len);
+int cdd_segment_vjp(cdd_plan* plan, const double* u0_dev, double t0, const double* dts_dev, int nsteps,
+                    const double* ubar_end_dev, double* ubar0_dev);
+int cdd_lambda_bar(cdd_plan* plan, double* value, int reset);
+
 /* ---- Batched independent realisations (2D) ------------------------------
  * The reference advances independent realisations one process per GPU
  * (Plasticity/exec_batch.py:26-58, RunCUDA.simulation per device/seed).
