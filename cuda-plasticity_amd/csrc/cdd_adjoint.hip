@@ -13,8 +13,9 @@
 //      (symmetrised, 9 planes) and to Lambda.  The max/min speed selections
 //      and the minmod branches are replayed from the forward values.
 //   2. k_flux_vjp_gather: the transpose of the limited-derivative stencils,
-//      written as a gather (each cell collects from the 5 cells whose stencil
-//      reaches it) so no atomics are needed.
+//      written as a gather over limiter sites (each cell collects from the 4
+//      sites and 5 cells whose stencils reach it) so no atomics are needed;
+//      the minmod selections are the codes stored by the cell kernel.
 //   3. the stress adjoint: the spectral passes with the transposed Green's
 //      contraction (green_sigma_T) applied to the symmetrised stress
 //      cotangent, plus the transpose of the local part
@@ -390,6 +391,9 @@ __global__ void __launch_bounds__(256) k_flux_vjp_cell(VjpJob job) {
   const long long ii = adj_tile_cell<D>(n, co, str);
   const T N = T(n);
   T u[D][2][9], fc[9];
+  // minmod selection of this cell's own limiter site m_i, per axis and
+  // component: 2 bits at 2 (9 a + c), 3 = the zero case
+  unsigned long long code = 0ull;
   for (int a = 0; a < D; ++a) {
     long long off[5];
     for (int k = 0; k < 5; ++k) off[k] = ii + (long long)(wrapi(co[a] + k - 2, n) - co[a]) * str[a];
@@ -398,8 +402,12 @@ __global__ void __launch_bounds__(256) k_flux_vjp_cell(VjpJob job) {
       for (int k = 0; k < 5; ++k) f[k] = job.beta[c * cells + off[k]];
       if (a == 0) fc[c] = f[2];
       lderivs<T>(f, N, u[a][0][c], u[a][1][c]);
+      const T Dm1 = f[2] - f[1], D0 = f[3] - f[2], D1 = f[4] - f[3];
+      const int sel = mm3_sel(D1 - D0, (D1 - Dm1) * T(0.5), D0 - Dm1);
+      code |= (unsigned long long)(sel < 0 ? 3 : sel) << (2 * (9 * a + c));
     }
   }
+  job.codes[ii] = code;
   T sg[9];
   {
     const T tr = fc[0] + fc[4] + fc[8];
@@ -418,7 +426,7 @@ __global__ void __launch_bounds__(256) k_flux_vjp_cell(VjpJob job) {
   cell_flux_vjp<T, D, DYN>(u, sg, T(job.Lambda), rb, ub, sgb, lamb);
   for (int a = 0; a < D; ++a)
     for (int s = 0; s < 2; ++s)
-      for (int c = 0; c < 9; ++c) job.ubar[((a * 2 + s) * 9 + c) * cells + ii] = ub[a][s][c];
+      for (int c = 0; c < 9; ++c) job.ubar[((a * 2 + s) * 9 + c) * cells + ii] = N * ub[a][s][c];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) job.sbar[(3 * i + j) * cells + ii] = T(0.5) * (sgb[3 * i + j] + sgb[3 * j + i]);
   for (int off = 32; off > 0; off >>= 1) lamb += __shfl_xor(lamb, off, 64);
@@ -433,80 +441,61 @@ __global__ void __launch_bounds__(256) k_flux_vjp_cell(VjpJob job) {
 }
 
 // ------------------------------------------------------------------ R2
-// betabar(j) = sum over the cells i in j-2..j+2 of the transposed limited-
-// derivative stencils of i applied to ubar(i), per axis and component,
-// with the minmod selections replayed from beta; plus the local stress
-// transpose -lam tr(sbar) d - 2 mu sbar (sbar is symmetric), the spectral
-// part ebar when given, and the Runge-Kutta combination sum_k c_k a_k
-// (read at the thread's own cell only, so out may alias an a_k).
+// Transpose of the limited-derivative stencils, in site form.  With
+// D_k = f_{k+1} - f_k, up_i = N (D_i - m_i / 2), um_i = N (D_{i-1} + m_{i-1} / 2)
+// and the limiter site m_k = minmod(D_{k+1} - D_k, (D_{k+1} - D_{k-1}) / 2,
+// D_k - D_{k-1}), the cotangents P_i = N upbar_i and Q_i = N umbar_i give
+//   Mbar_k = -P_k / 2 + Q_{k+1} / 2                       (site k)
+//   Dbar_k = P_k + Q_{k+1} + up(k-1) + mid(k) + dn(k+1)    (site terms)
+// where site k passes (up, mid, dn) = Mbar_k x {(1, -1, 0), (1/2, 0, -1/2),
+// (0, 1, -1), 0} for its selection {first, second, third, zero} to
+// (D_{k+1}, D_k, D_{k-1}); then betabar_j = Dbar_{j-1} - Dbar_j.  The
+// selections come from the codes the cell kernel stored, so no limiter is
+// re-evaluated here.  Added: the local stress transpose
+// -lam tr(sbar) d - 2 mu sbar (sbar symmetric), the spectral part ebar when
+// given, and the Runge-Kutta combination sum_k c_k a_k (read at the thread's
+// own cell only, so out may alias an a_k).
+template <typename T>
+__device__ __forceinline__ void site_coef(unsigned sel, T& up, T& mid, T& dn) {
+  up = sel == 0u ? T(1) : (sel == 1u ? T(0.5) : T(0));
+  mid = sel == 0u ? T(-1) : (sel == 2u ? T(1) : T(0));
+  dn = sel == 1u ? T(-0.5) : (sel == 2u ? T(-1) : T(0));
+}
+
 template <typename T, int D>
 __global__ void __launch_bounds__(256) k_flux_vjp_gather(GatherJob g) {
   const int n = g.n;
   const long long cells = g.cells;
-  const T* __restrict__ beta = g.beta;
   const T* __restrict__ ubar = g.ubar;
   int co[3];
   long long str[3];
   const long long idx = adj_tile_cell<D>(n, co, str);
-  const T N = T(n);
   T acc[9];
   for (int c = 0; c < 9; ++c) acc[c] = T(0);
   for (int a = 0; a < D; ++a) {
-    long long off[9];
-    for (int k = 0; k < 9; ++k) off[k] = idx + (long long)(wrapi(co[a] + k - 4, n) - co[a]) * str[a];
+    long long off[5];  // cells j-2 .. j+2 along axis a
+    for (int k = 0; k < 5; ++k) off[k] = idx + (long long)(wrapi(co[a] + k - 2, n) - co[a]) * str[a];
+    unsigned long long code[4];  // sites j-2 .. j+1
+    for (int s = 0; s < 4; ++s) code[s] = g.codes[off[s]];
     for (int c = 0; c < 9; ++c) {
-      T f[9];
-      for (int k = 0; k < 9; ++k) f[k] = beta[c * cells + off[k]];
-      // Dv[k] = D_{j-4+k} = f_{j-3+k} - f_{j-4+k}, k = 0..7 (D_{j-4} .. D_{j+3})
-      T Dv[8];
-      for (int k = 0; k < 8; ++k) Dv[k] = f[k + 1] - f[k];
-      // cotangents of D_{j-1} (index 3) and D_j (index 4)
-      T Dbm1 = T(0), Db0 = T(0);
-      for (int di = -2; di <= 2; ++di) {
-        // source cell i = j + di; its D_k lives at Dv[k - (j - 4)] = Dv[di + 4 + (k - i)]
-        const long long oi = off[di + 4];
-        const T upb = ubar[((a * 2 + 0) * 9 + c) * cells + oi];
-        const T umb = ubar[((a * 2 + 1) * 9 + c) * cells + oi];
-        T db[4] = {T(0), T(0), T(0), T(0)};  // contributions to D_{i-2}, D_{i-1}, D_i, D_{i+1}
-        const int b = di + 4;                // Dv[b] = D_i
-        db[2] += N * upb;
-        db[1] += N * umb;
-        const T mb_i = T(-0.5) * N * upb;
-        const T mb_im1 = T(0.5) * N * umb;
-        {  // m_i = minmod(D_{i+1} - D_i, (D_{i+1} - D_{i-1}) / 2, D_i - D_{i-1})
-          const int sel = mm3_sel(Dv[b + 1] - Dv[b], (Dv[b + 1] - Dv[b - 1]) * T(0.5), Dv[b] - Dv[b - 1]);
-          if (sel == 0) {
-            db[3] += mb_i;
-            db[2] -= mb_i;
-          } else if (sel == 1) {
-            db[3] += T(0.5) * mb_i;
-            db[1] -= T(0.5) * mb_i;
-          } else if (sel == 2) {
-            db[2] += mb_i;
-            db[1] -= mb_i;
-          }
-        }
-        {  // m_{i-1} = minmod(D_i - D_{i-1}, (D_i - D_{i-2}) / 2, D_{i-1} - D_{i-2})
-          const int sel = mm3_sel(Dv[b] - Dv[b - 1], (Dv[b] - Dv[b - 2]) * T(0.5), Dv[b - 1] - Dv[b - 2]);
-          if (sel == 0) {
-            db[2] += mb_im1;
-            db[1] -= mb_im1;
-          } else if (sel == 1) {
-            db[2] += T(0.5) * mb_im1;
-            db[0] -= T(0.5) * mb_im1;
-          } else if (sel == 2) {
-            db[1] += mb_im1;
-            db[0] -= mb_im1;
-          }
-        }
-        // db[q] is the cotangent of D_{i-2+q} = D_{j+di-2+q}
-        for (int qq = 0; qq < 4; ++qq) {
-          const int k = di - 2 + qq;  // offset from j
-          if (k == -1) Dbm1 += db[qq];
-          if (k == 0) Db0 += db[qq];
-        }
+      const T* Pp = ubar + ((a * 2 + 0) * 9 + c) * cells;
+      const T* Qp = ubar + ((a * 2 + 1) * 9 + c) * cells;
+      T P[4], Q[4];  // P at cells j-2..j+1, Q at cells j-1..j+2
+      for (int s = 0; s < 4; ++s) {
+        P[s] = Pp[off[s]];
+        Q[s] = Qp[off[s + 1]];
       }
-      acc[c] += Dbm1 - Db0;  // D_k = f_{k+1} - f_k
+      T up[4], mid[4], dn[4];
+      for (int s = 0; s < 4; ++s) {
+        const T Mb = T(0.5) * (Q[s] - P[s]);
+        site_coef<T>((unsigned)(code[s] >> (2 * (9 * a + c))) & 3u, up[s], mid[s], dn[s]);
+        up[s] *= Mb;
+        mid[s] *= Mb;
+        dn[s] *= Mb;
+      }
+      const T Dbm1 = P[1] + Q[1] + up[0] + mid[1] + dn[2];
+      const T Db0 = P[2] + Q[2] + up[1] + mid[2] + dn[3];
+      acc[c] += Dbm1 - Db0;
     }
   }
   T s[9];

@@ -61,6 +61,7 @@ struct cdd_plan {
   double* ubar;
   double* sbar;
   double* lpart;
+  unsigned long long* codes;
   // segment reverse pass: per-step stage states and stress planes, and the
   // per-block Lambda cotangents of every stage of the segment
   double* seg;
@@ -232,6 +233,7 @@ int cdd_plan_destroy(cdd_plan* p) {
   (void)hipFree(p->ubar);
   (void)hipFree(p->sbar);
   (void)hipFree(p->lpart);
+  (void)hipFree(p->codes);
   (void)hipFree(p->seg);
   (void)hipFree(p->lseg);
   if (p->own) (void)hipStreamDestroy(p->own);
@@ -489,6 +491,7 @@ static int adj_alloc(cdd_plan* p) {
     return fail(CDD_ERR_UNSUPPORTED, "the adjoint supports loads that do not depend on beta (none/strain/stress/tensor)");
   if (hipMalloc((void**)&p->adj, 6 * fb) != hipSuccess || hipMalloc((void**)&p->ubar, 2 * p->d * fb) != hipSuccess ||
       hipMalloc((void**)&p->sbar, fb) != hipSuccess ||
+      hipMalloc((void**)&p->codes, sizeof(unsigned long long) * (size_t)p->cells) != hipSuccess ||
       hipMalloc((void**)&p->lpart, sizeof(double) * (size_t)std::max<long long>((p->cells + 255) / 256,
                                                                                 flux_vjp_nblocks(p->d, p->n))) !=
           hipSuccess)
@@ -513,6 +516,7 @@ static int flux_vjp_impl(cdd_plan* p, const double* beta, double t, const double
   j.rscale = 1.0;
   j.dtp = nullptr;
   j.ubar = p->ubar;
+  j.codes = p->codes;
   j.sbar = p->sbar;
   j.lpart = p->lpart;
   j.Lambda = p->cfg.dynamics == CDD_GLIDE_ONLY ? 0.0 : p->cfg.Lambda;
@@ -529,7 +533,7 @@ static int flux_vjp_impl(cdd_plan* p, const double* beta, double t, const double
   g.d = p->d;
   g.n = p->n;
   g.cells = p->cells;
-  g.beta = beta;
+  g.codes = p->codes;
   g.ubar = p->ubar;
   g.sbar = p->sbar;
   g.ebar = p->G;
@@ -656,6 +660,7 @@ static int stage_vjp(cdd_plan* p, const double* beta, const double* G, const dou
   j.rscale = rscale;
   j.dtp = dtp;
   j.ubar = p->ubar;
+  j.codes = p->codes;
   j.sbar = p->sbar;
   j.lpart = lpart;
   j.Lambda = p->cfg.dynamics == CDD_GLIDE_ONLY ? 0.0 : p->cfg.Lambda;
@@ -671,7 +676,7 @@ static int stage_vjp(cdd_plan* p, const double* beta, const double* G, const dou
   g.d = p->d;
   g.n = p->n;
   g.cells = p->cells;
-  g.beta = beta;
+  g.codes = p->codes;
   g.ubar = p->ubar;
   g.sbar = p->sbar;
   g.ebar = p->G;

@@ -97,7 +97,8 @@ struct VjpJob {
   const double* rbar;  // cotangent of the flux (9 planes), used as rscale * dt * rbar
   double rscale;
   const double* dtp;   // optional device dt (the factor is 1 when null)
-  double* ubar;        // out: 2 d 9 derivative-cotangent planes
+  double* ubar;        // out: 2 d 9 derivative-cotangent planes, scaled by n
+  unsigned long long* codes;  // out: per-cell minmod selections (2 bits per axis and component)
   double* sbar;        // out: symmetrised stress cotangent (9 planes)
   double* lpart;       // out: per-block Lambda cotangent
   double Lambda, mu, lam;
@@ -107,7 +108,7 @@ struct VjpJob {
 struct GatherJob {
   int d, n;
   long long cells;
-  const double* beta;
+  const unsigned long long* codes;
   const double* ubar;
   const double* sbar;
   const double* ebar;
