@@ -46,7 +46,7 @@ EXPORTS = [
     "cdd_batch_create", "cdd_batch_destroy", "cdd_batch_set_stream", "cdd_batch_size", "cdd_batch_set_state",
     "cdd_batch_get_state", "cdd_batch_set_time", "cdd_batch_get_clock", "cdd_batch_step", "cdd_batch_set_tile",
     "cdd_batch_time_flux", "cdd_flux_vjp", "cdd_step_vjp", "cdd_energy_grad", "cdd_dt_record", "cdd_dt_replay",
-    "cdd_segment_vjp", "cdd_lambda_bar",
+    "cdd_segment_vjp", "cdd_lambda_bar", "cdd_stress_record",
 ]
 KERNEL_KINDS = ["r2c", "cols_fwd", "contract", "cols_inv", "c2r", "flux_stage1", "cfl_reduce",
                 "flux_stage2", "flux_stage3", "advance"]
@@ -81,8 +81,9 @@ def _load():
         "cdd_energy_grad": ([P, P, D, P, dp], I),
         "cdd_dt_record": ([P, P, ctypes.c_longlong], I),
         "cdd_dt_replay": ([P, P, ctypes.c_longlong], I),
-        "cdd_segment_vjp": ([P, P, D, P, I, P, P], I),
+        "cdd_segment_vjp": ([P, P, D, P, I, P, P, P], I),
         "cdd_lambda_bar": ([P, dp, I], I),
+        "cdd_stress_record": ([P, P, ctypes.c_longlong], I),
         "cdd_batch_create": ([ctypes.POINTER(Config), ctypes.POINTER(I), I, ctypes.POINTER(P)], I),
         "cdd_batch_destroy": ([P], I),
         "cdd_batch_set_stream": ([P, P], I),
@@ -242,13 +243,15 @@ class Plan:
         check(lib.cdd_energy_grad(self.h, _ptr(b), t, _ptr(out), ctypes.byref(e)))
         return out, e.value
 
-    def segment_vjp(self, u0, t0, dts, ubar_end, out=None):
+    def segment_vjp(self, u0, t0, dts, ubar_end, out=None, src=None):
         """Reverse pass over one checkpoint segment: recompute dts.numel()
         steps from the checkpoint u0 at time t0 replaying the recorded time
         steps dts (float64 device tensor), then return the cotangent of u0
         given the cotangent ubar_end of the segment's final state.  `out` may
         be ubar_end itself.  The Lambda cotangent accumulates on the device
-        (lambda_bar)."""
+        (lambda_bar).  src: optional (dts.numel(), 9) float64 device tensor,
+        the uniform gradient of a time-distributed objective at the state
+        that starts each step."""
         a, g = self._as_dev(u0), self._as_dev(ubar_end)
         if dts.dtype != self.torch.float64 or not dts.is_contiguous() or dts.device != self.dev:
             raise CDDError("dts must be a contiguous float64 tensor on the plan's device")
@@ -257,7 +260,13 @@ class Plan:
         elif (out.dtype != self.torch.float64 or not out.is_contiguous() or out.device != self.dev
               or tuple(out.shape) != (9,) + self.shape):
             raise CDDError("out must be a contiguous float64 (9,) + grid tensor on the plan's device")
-        check(lib.cdd_segment_vjp(self.h, _ptr(a), float(t0), _ptr(dts), int(dts.numel()), _ptr(g), _ptr(out)))
+        sp = None
+        if src is not None:
+            if (src.dtype != self.torch.float64 or not src.is_contiguous() or src.device != self.dev
+                    or tuple(src.shape) != (dts.numel(), 9)):
+                raise CDDError("src must be a contiguous float64 (nsteps, 9) tensor on the plan's device")
+            sp = _ptr(src)
+        check(lib.cdd_segment_vjp(self.h, _ptr(a), float(t0), _ptr(dts), int(dts.numel()), sp, _ptr(g), _ptr(out)))
         return out
 
     def lambda_bar(self, reset=True):
@@ -273,6 +282,17 @@ class Plan:
             check(lib.cdd_dt_record(self.h, None, 0))
         else:
             check(lib.cdd_dt_record(self.h, _ptr(log), log.numel()))
+
+    def stress_record(self, log):
+        """Append one stress-strain record per step to the float64 device tensor
+        `log` of shape (cap, 8): t, mean sigma (xx, yy, zz, xy, xz, yz, load
+        included) at the step start, mean beta_zz.  None stops recording."""
+        if log is None:
+            check(lib.cdd_stress_record(self.h, None, 0))
+        else:
+            if log.dtype != self.torch.float64 or not log.is_contiguous() or log.shape[-1] != 8:
+                raise CDDError("stress log must be a contiguous float64 (cap, 8) tensor")
+            check(lib.cdd_stress_record(self.h, _ptr(log), log.numel() // 8))
 
     def dt_replay(self, sched):
         """Take the next sched.numel() time steps from the float64 device tensor

@@ -58,8 +58,9 @@ __device__ void cell_flux_vjp(const T (&u)[D][2][9], const T (&sg)[9], T Lambda,
   constexpr int NCOR = 1 << D;
   const T ME = T(kME);
   const bool climb_sup = (DYN == DYN_GLIDE_CLIMB) && (Lambda != T(0));
-  const T st = climb_sup ? Lambda * (sg[0] + sg[4] + sg[8]) / T(3) : T(0);
-  const T jfac = -Lambda / T(3);
+  const T third = T(1.0 / 3.0);
+  const T st = climb_sup ? Lambda * (sg[0] + sg[4] + sg[8]) * third : T(0);
+  const T jfac = -Lambda * third;
   // ---------------- forward recompute
   T w[D][2][3], zz[D][2], tr[D][2], P[D][2];
   for (int a = 0; a < D; ++a)
@@ -121,7 +122,7 @@ __device__ void cell_flux_vjp(const T (&u)[D][2][9], const T (&sg)[9], T Lambda,
       }
     }
     r2 -= T(2) * cross;
-    rinv[c] = T(1) / sqrt(r2);
+    rinv[c] = rsqrt_nr(r2);
     for (int l = 0; l < 3; ++l) v[c][l] = v0[c][l] * rinv[c];
     J[c] = climb_sup ? jfac * (v[c][0] * q[c][0] + v[c][1] * q[c][1] + v[c][2] * q[c][2]) : T(0);
   }
@@ -159,7 +160,14 @@ __device__ void cell_flux_vjp(const T (&u)[D][2][9], const T (&sg)[9], T Lambda,
   for (int a = 0; a < D; ++a) inv *= at[a];
   inv = T(1) / inv;
   T twv[3] = {0, 0, 0};
-  for (int a = 0; a < D; ++a) twv[a] = am[a] * ap[a] / at[a];
+  T iat[3] = {1, 1, 1};
+  for (int a = 0; a < D; ++a) {
+    T o = inv;
+    for (int b = 0; b < D; ++b)
+      if (b != a) o *= at[b];
+    iat[a] = o;
+  }
+  for (int a = 0; a < D; ++a) twv[a] = am[a] * ap[a] * iat[a];
   T Hs[9];
   for (int i = 0; i < 3; ++i)
     for (int jj = 0; jj < 3; ++jj) {
@@ -210,9 +218,9 @@ __device__ void cell_flux_vjp(const T (&u)[D][2][9], const T (&sg)[9], T Lambda,
     }
   T apb[3] = {0, 0, 0}, amb[3] = {0, 0, 0};
   for (int a = 0; a < D; ++a) {
-    amb[a] += twb[a] * ap[a] / at[a];
-    apb[a] += twb[a] * am[a] / at[a];
-    T atb = -twb[a] * am[a] * ap[a] / (at[a] * at[a]) - invb * inv / at[a];
+    amb[a] += twb[a] * ap[a] * iat[a];
+    apb[a] += twb[a] * am[a] * iat[a];
+    T atb = -twb[a] * am[a] * ap[a] * iat[a] * iat[a] - invb * inv * iat[a];
     apb[a] += atb;
     amb[a] += atb;
   }
@@ -258,7 +266,7 @@ __device__ void cell_flux_vjp(const T (&u)[D][2][9], const T (&sg)[9], T Lambda,
         vb[c][l] += Jb[c] * jfac * q[c][l];
         qb[l] += Jb[c] * jfac * v[c][l];
       }
-      lamb += Jb[c] * (-vq / T(3));
+      lamb -= Jb[c] * vq * third;
     }
     T v0b[3], rinvb = T(0);
     for (int l = 0; l < 3; ++l) {
@@ -319,10 +327,10 @@ __device__ void cell_flux_vjp(const T (&u)[D][2][9], const T (&sg)[9], T Lambda,
       for (int a = 0; a < D; ++a) ub[a][sg_[a]][3 * l + a] += Rb[l];
   }
   if (climb_sup) {
-    sgb[0] += stb * Lambda / T(3);
-    sgb[4] += stb * Lambda / T(3);
-    sgb[8] += stb * Lambda / T(3);
-    lamb += stb * (sg[0] + sg[4] + sg[8]) / T(3);
+    sgb[0] += stb * Lambda * third;
+    sgb[4] += stb * Lambda * third;
+    sgb[8] += stb * Lambda * third;
+    lamb += stb * (sg[0] + sg[4] + sg[8]) * third;
   }
   for (int a = 0; a < D; ++a)
     for (int s = 0; s < 2; ++s) {
@@ -429,14 +437,27 @@ __global__ void __launch_bounds__(256) k_flux_vjp_cell(VjpJob job) {
       for (int c = 0; c < 9; ++c) job.ubar[((a * 2 + s) * 9 + c) * cells + ii] = N * ub[a][s][c];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) job.sbar[(3 * i + j) * cells + ii] = T(0.5) * (sgb[3 * i + j] + sgb[3 * j + i]);
-  for (int off = 32; off > 0; off >>= 1) lamb += __shfl_xor(lamb, off, 64);
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const int nw = (int)(blockDim.x * blockDim.y + 63) / 64;
+  for (int off = 32; off > 0; off >>= 1) lamb += __shfl_xor(lamb, off, 64);
   if ((tid & 63) == 0) red[tid >> 6] = lamb;
   __syncthreads();
   if (tid == 0) {
     T t = T(0);
-    for (int k = 0; k < (int)(blockDim.x * blockDim.y + 63) / 64; ++k) t += red[k];
+    for (int k = 0; k < nw; ++k) t += red[k];
     job.lpart[blockIdx.x] = t;
+  }
+  if (job.upart) {
+    T uz = T(job.lam) * (sgb[0] + sgb[4] + sgb[8]) + T(2) * T(job.mu) * sgb[8];
+    for (int off = 32; off > 0; off >>= 1) uz += __shfl_xor(uz, off, 64);
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = uz;
+    __syncthreads();
+    if (tid == 0) {
+      T t = T(0);
+      for (int k = 0; k < nw; ++k) t += red[k];
+      job.upart[blockIdx.x] = t;
+    }
   }
 }
 
@@ -507,8 +528,10 @@ __global__ void __launch_bounds__(256) k_flux_vjp_gather(GatherJob g) {
     T v = acc[c] - T(2) * T(g.mu) * s[c] - (i == j ? T(g.lam) * trs : T(0));
     if (g.ebar) v += g.ebar[SC[c] * cells + idx];
     for (int k = 0; k < g.nterm; ++k) v += T(g.coef[k]) * g.term[k][c * cells + idx];
+    if (g.csrc) v += g.csrc[c];
     acc[c] = v;
   }
+  if (g.uzz) acc[8] += T(g.uzz_scale) * g.uzz[0];
   for (int c = 0; c < 9; ++c) g.out[c * cells + idx] = acc[c];
 }
 
@@ -554,6 +577,40 @@ __global__ void __launch_bounds__(256) k_energy_grad(const double* G, const doub
   }
 }
 
+// Per-block sums of lam tr(sbar) + 2 mu sbar_zz: the cotangent of the
+// uniaxial control's betaE_zz = rate t - (<beta_zz> - init)
+// (RunLoadingMPI.py:68-83), whose stress enters every cell.
+__global__ void __launch_bounds__(256) k_uniaxial_partial(const double* sbar, long long cells, double lam, double mu,
+                                                         double* partial) {
+  __shared__ double red[4];
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double u = 0.0;
+  if (idx < cells)
+    u = lam * (sbar[idx] + sbar[4 * cells + idx] + sbar[8 * cells + idx]) + 2.0 * mu * sbar[8 * cells + idx];
+  for (int off = 32; off > 0; off >>= 1) u += __shfl_xor(u, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = u;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void k_add_uniform_zz(double* out, long long cells, const double* s, double scale) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < cells) out[8 * cells + idx] += scale * s[0];
+}
+
+hipError_t launch_uniaxial_partial(const double* sbar, long long cells, double lam, double mu, double* partial,
+                                   int* nblocks, hipStream_t st) {
+  const int nb = (int)((cells + 255) / 256);
+  *nblocks = nb;
+  hipLaunchKernelGGL(k_uniaxial_partial, dim3(nb), dim3(256), 0, st, sbar, cells, lam, mu, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_uniform_zz(double* out, long long cells, const double* s, double scale, hipStream_t st) {
+  hipLaunchKernelGGL(k_add_uniform_zz, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, out, cells, s, scale);
+  return hipGetLastError();
+}
+
 // out = sum_k c[k] in[k] (k < nin), elementwise over len values
 __global__ void k_lincomb(long long len, int nin, const double* a0, double c0, const double* a1, double c1,
                           const double* a2, double c2, const double* a3, double c3, double* out) {
@@ -583,6 +640,9 @@ __global__ void k_clock_segment(Clock* clk, int mode, double t0, const double* d
     c.dt_log = nullptr;
     c.log_base = 0;
     c.log_cap = 0;
+    c.slog = nullptr;
+    c.slog_base = 0;
+    c.slog_cap = 0;
     clk[0] = c;
   } else {
     clk[0] = clk[1];

@@ -61,6 +61,7 @@ struct cdd_plan {
   double* ubar;
   double* sbar;
   double* lpart;
+  double* upart;  // uniaxial control: per-block cotangent of betaE_zz
   unsigned long long* codes;
   // segment reverse pass: per-step stage states and stress planes, and the
   // per-block Lambda cotangents of every stage of the segment
@@ -233,6 +234,7 @@ int cdd_plan_destroy(cdd_plan* p) {
   (void)hipFree(p->ubar);
   (void)hipFree(p->sbar);
   (void)hipFree(p->lpart);
+  (void)hipFree(p->upart);
   (void)hipFree(p->codes);
   (void)hipFree(p->seg);
   (void)hipFree(p->lseg);
@@ -392,7 +394,11 @@ static int launch_step(cdd_plan* p, const StepBufs& b) {
   double* u = const_cast<double*>(b.u);
   // stage 1: L0 = L(u); dt from the stage-1 speeds
   if (p->timing) timing_hook(p, -1);
-  HIPCHK(launch_spectral(spectral_job(p, 6, u, nullptr, p->clk, 0.0, b.G[0])));
+  {
+    SpectralJob j1 = spectral_job(p, 6, u, nullptr, p->clk, 0.0, b.G[0]);
+    j1.gp.log_stress = 1;
+    HIPCHK(launch_spectral(j1));
+  }
   FluxJob f = flux_job(p);
   f.G = b.G[0];
   f.clk = p->clk;
@@ -487,17 +493,22 @@ int cdd_energy(cdd_plan* p, const double* beta, double t, double* energy) {
 static int adj_alloc(cdd_plan* p) {
   if (p->adj) return CDD_OK;
   const size_t fb = sizeof(double) * 9 * (size_t)p->cells;
-  if (p->cfg.load_kind == CDD_LOAD_UNIAXIAL)
-    return fail(CDD_ERR_UNSUPPORTED, "the adjoint supports loads that do not depend on beta (none/strain/stress/tensor)");
   if (hipMalloc((void**)&p->adj, 6 * fb) != hipSuccess || hipMalloc((void**)&p->ubar, 2 * p->d * fb) != hipSuccess ||
       hipMalloc((void**)&p->sbar, fb) != hipSuccess ||
       hipMalloc((void**)&p->codes, sizeof(unsigned long long) * (size_t)p->cells) != hipSuccess ||
       hipMalloc((void**)&p->lpart, sizeof(double) * (size_t)std::max<long long>((p->cells + 255) / 256,
                                                                                 flux_vjp_nblocks(p->d, p->n))) !=
+          hipSuccess ||
+      hipMalloc((void**)&p->upart, sizeof(double) * (size_t)std::max<long long>((p->cells + 255) / 256,
+                                                                                flux_vjp_nblocks(p->d, p->n))) !=
           hipSuccess)
     return fail(CDD_ERR_HIP, "hipMalloc failed (adjoint workspace)");
   return CDD_OK;
 }
+
+// The uniaxial strain-rate control's stress depends on <beta_zz>
+// (RunLoadingMPI.py:68-83): its transpose adds a uniform term to betabar_zz.
+static bool uniaxial(const cdd_plan* p) { return p->cfg.load_kind == CDD_LOAD_UNIAXIAL; }
 
 static double* adjx(cdd_plan* p, int k) { return p->adj + (size_t)k * 9 * p->cells; }
 
@@ -519,12 +530,14 @@ static int flux_vjp_impl(cdd_plan* p, const double* beta, double t, const double
   j.codes = p->codes;
   j.sbar = p->sbar;
   j.lpart = p->lpart;
+  j.upart = uniaxial(p) ? p->upart : nullptr;
   j.Lambda = p->cfg.dynamics == CDD_GLIDE_ONLY ? 0.0 : p->cfg.Lambda;
   j.mu = p->gp.mu;
   j.lam = p->gp.lam;
   int nb = 0;
   HIPCHK(launch_flux_vjp_cell(j, &nb, p->stream));
   HIPCHK(launch_reduce_sum(p->lpart, nb, p->scalar + 1, p->stream, 1));
+  if (j.upart) HIPCHK(launch_reduce_sum(p->upart, nb, p->scalar + 4, p->stream, 0));
   SpectralJob jt = spectral_job(p, 6, p->sbar, nullptr, nullptr, t, p->G);
   jt.trans = 1;
   HIPCHK(launch_spectral(jt));
@@ -539,6 +552,10 @@ static int flux_vjp_impl(cdd_plan* p, const double* beta, double t, const double
   g.ebar = p->G;
   g.mu = p->gp.mu;
   g.lam = p->gp.lam;
+  if (j.upart) {
+    g.uzz = p->scalar + 4;
+    g.uzz_scale = -1.0 / (double)p->cells;
+  }
   g.out = out;
   HIPCHK(launch_flux_vjp_gather(g, p->stream));
   return CDD_OK;
@@ -631,10 +648,17 @@ int cdd_energy_grad(cdd_plan* p, const double* beta, double t, double* betabar, 
   HIPCHK(launch_energy_grad(p->G, beta, p->cells, p->gp.mu, p->gp.el.nu, p->gp.lam, p->sbar, p->lpart, &nb,
                             p->stream));
   HIPCHK(launch_reduce_sum(p->lpart, nb, p->scalar + 2, p->stream, 0));
+  if (uniaxial(p)) {
+    int nu = 0;
+    HIPCHK(launch_uniaxial_partial(p->sbar, p->cells, p->gp.lam, p->gp.mu, p->upart, &nu, p->stream));
+    HIPCHK(launch_reduce_sum(p->upart, nu, p->scalar + 5, p->stream, 0));
+  }
   SpectralJob jt = spectral_job(p, 6, p->sbar, nullptr, nullptr, t, p->G);
   jt.trans = 1;
   HIPCHK(launch_spectral(jt));
   HIPCHK(launch_sigma_finalize(p->G, p->sbar, betabar, p->cells, p->gp.mu, p->gp.lam, p->stream));
+  if (uniaxial(p))
+    HIPCHK(launch_add_uniform_zz(betabar, p->cells, p->scalar + 5, -1.0 / (double)p->cells, p->stream));
   if (energy) {
     HIPCHK(hipMemcpyAsync(energy, p->scalar + 2, sizeof(double), hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
@@ -646,8 +670,9 @@ int cdd_energy_grad(cdd_plan* p, const double* beta, double t, double* betabar, 
 // J_L(beta)^T (rscale dt rbar), with beta's stress planes G kept from the
 // recomputation and the per-block Lambda cotangents written to lpart.
 static int stage_vjp(cdd_plan* p, const double* beta, const double* G, const double* rbar, double rscale,
-                     const double* dtp, double* lpart, double* out, int nterm, const double* t0, double c0,
-                     const double* t1 = nullptr, double c1 = 0.0, const double* t2 = nullptr, double c2 = 0.0) {
+                     const double* dtp, double* lpart, double* out, const double* csrc, int nterm, const double* t0,
+                     double c0, const double* t1 = nullptr, double c1 = 0.0, const double* t2 = nullptr,
+                     double c2 = 0.0) {
   VjpJob j;
   memset(&j, 0, sizeof j);
   j.d = p->d;
@@ -663,11 +688,13 @@ static int stage_vjp(cdd_plan* p, const double* beta, const double* G, const dou
   j.codes = p->codes;
   j.sbar = p->sbar;
   j.lpart = lpart;
+  j.upart = uniaxial(p) ? p->upart : nullptr;
   j.Lambda = p->cfg.dynamics == CDD_GLIDE_ONLY ? 0.0 : p->cfg.Lambda;
   j.mu = p->gp.mu;
   j.lam = p->gp.lam;
   int nb = 0;
   HIPCHK(launch_flux_vjp_cell(j, &nb, p->stream));
+  if (j.upart) HIPCHK(launch_reduce_sum(p->upart, nb, p->scalar + 4, p->stream, 0));
   SpectralJob jt = spectral_job(p, 6, p->sbar, nullptr, nullptr, 0.0, p->G);
   jt.trans = 1;
   HIPCHK(launch_spectral(jt));
@@ -689,6 +716,11 @@ static int stage_vjp(cdd_plan* p, const double* beta, const double* G, const dou
   g.coef[1] = c1;
   g.term[2] = t2;
   g.coef[2] = c2;
+  g.csrc = csrc;
+  if (j.upart) {
+    g.uzz = p->scalar + 4;
+    g.uzz_scale = -1.0 / (double)p->cells;
+  }
   g.out = out;
   HIPCHK(launch_flux_vjp_gather(g, p->stream));
   return CDD_OK;
@@ -721,7 +753,7 @@ static int seg_alloc(cdd_plan* p, int nsteps) {
 }
 
 int cdd_segment_vjp(cdd_plan* p, const double* u0, double t0, const double* dts_dev, int nsteps,
-                    const double* ubar_end, double* ubar0) {
+                    const double* src_dev, const double* ubar_end, double* ubar0) {
   if (!p || !u0 || !dts_dev || !ubar_end || !ubar0 || nsteps < 1) return fail(CDD_ERR_ARG, "bad argument");
   HIPCHK(hipSetDevice(p->cfg.device));
   RCHK(adj_alloc(p));
@@ -754,15 +786,17 @@ int cdd_segment_vjp(cdd_plan* p, const double* u0, double t0, const double* dts_
     const double* dtp = dts_dev + j;
     double* lp = p->lseg + (size_t)j * S * nb;
     const double* u = j == 0 ? u0 : st(j, 0);
+    // the objective's own gradient at u_j (a uniform 9-vector) joins the last stage
+    const double* sj = src_dev ? src_dev + 9 * (size_t)j : nullptr;
     if (S == 2) {
       // u' = u/2 + (F0 + dt L(F0))/2,  F0 = u + dt L(u)
-      RCHK(stage_vjp(p, st(j, 1), gs(j, 1), gin, 0.5, dtp, lp + nb, F0b, 1, gin, 0.5));
-      RCHK(stage_vjp(p, u, gs(j, 0), F0b, 1.0, dtp, lp, gout, 2, gin, 0.5, F0b, 1.0));
+      RCHK(stage_vjp(p, st(j, 1), gs(j, 1), gin, 0.5, dtp, lp + nb, F0b, nullptr, 1, gin, 0.5));
+      RCHK(stage_vjp(p, u, gs(j, 0), F0b, 1.0, dtp, lp, gout, sj, 2, gin, 0.5, F0b, 1.0));
     } else {
       // u' = u/3 + 2/3 (F1 + dt L(F1)),  F1 = 3/4 u + 1/4 (F0 + dt L(F0))
-      RCHK(stage_vjp(p, st(j, 2), gs(j, 2), gin, 2.0 / 3.0, dtp, lp + 2 * nb, F1b, 1, gin, 2.0 / 3.0));
-      RCHK(stage_vjp(p, st(j, 1), gs(j, 1), F1b, 0.25, dtp, lp + nb, F0b, 1, F1b, 0.25));
-      RCHK(stage_vjp(p, u, gs(j, 0), F0b, 1.0, dtp, lp, gout, 3, gin, 1.0 / 3.0, F1b, 0.75, F0b, 1.0));
+      RCHK(stage_vjp(p, st(j, 2), gs(j, 2), gin, 2.0 / 3.0, dtp, lp + 2 * nb, F1b, nullptr, 1, gin, 2.0 / 3.0));
+      RCHK(stage_vjp(p, st(j, 1), gs(j, 1), F1b, 0.25, dtp, lp + nb, F0b, nullptr, 1, F1b, 0.25));
+      RCHK(stage_vjp(p, u, gs(j, 0), F0b, 1.0, dtp, lp, gout, sj, 3, gin, 1.0 / 3.0, F1b, 0.75, F0b, 1.0));
     }
     gin = gout;
   }
@@ -789,6 +823,18 @@ int cdd_dt_record(cdd_plan* p, double* log_dev, long long cap) {
   c.dt_log = cap > 0 ? log_dev : nullptr;
   c.log_base = c.nsteps;
   c.log_cap = cap;
+  HIPCHK(hipMemcpy(p->clk, &c, sizeof c, hipMemcpyHostToDevice));
+  return CDD_OK;
+}
+
+int cdd_stress_record(cdd_plan* p, double* log_dev, long long cap) {
+  if (!p || cap < 0) return fail(CDD_ERR_ARG, "bad argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  Clock c;
+  if (sync_clock(p, &c)) return CDD_ERR_HIP;
+  c.slog = cap > 0 ? log_dev : nullptr;
+  c.slog_base = c.nsteps;
+  c.slog_cap = cap;
   HIPCHK(hipMemcpy(p->clk, &c, sizeof c, hipMemcpyHostToDevice));
   return CDD_OK;
 }

@@ -32,6 +32,15 @@ __device__ __forceinline__ cx<T> mul_ni(cx<T> a) { return {a.y, -a.x}; }
 template <typename T>
 __device__ __forceinline__ cx<T> mul_pi(cx<T> a) { return {-a.y, a.x}; }
 
+// 1/sqrt(x) for x > 0: the hardware reciprocal square root refined by one
+// Newton step y <- y (3/2 - x y^2 / 2), accurate to a few ulp.  The
+// forward flux and the adjoint's recomputation share it, so both linearise
+// at the same point.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  return y * (1.5 - 0.5 * x * y * y);
+}
+
 // Run-time clock of one stepper instance, resident in device memory so the
 // CFL time step never round-trips through the host.
 struct Clock {
@@ -50,6 +59,10 @@ struct Clock {
   // optional log of the time steps taken: dt_log[nsteps - log_base] = dt
   double* dt_log;
   long long log_base, log_cap;
+  // optional stress-strain log: 8 doubles per step (t, mean sigma xx yy zz
+  // xy xz yz with the load, mean beta_zz) at slog[8 (nsteps - slog_base)]
+  double* slog;
+  long long slog_base, slog_cap;
 };
 
 // Uniform load added to sigma (it enters the stress solve as the k = 0 mode).

@@ -100,8 +100,9 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
       P[a][s] = p;
     }
   }
-  const T st = (DYN == DYN_GLIDE_CLIMB) ? Lambda * (sg[0] + sg[4] + sg[8]) / T(3) : T(0);
-  const T jfac = -Lambda / T(3);
+  const T third = T(1.0 / 3.0);
+  const T st = (DYN == DYN_GLIDE_CLIMB) ? Lambda * (sg[0] + sg[4] + sg[8]) * third : T(0);
+  const T jfac = -Lambda * third;
   const bool climb_sup = (DYN == DYN_GLIDE_CLIMB) && (Lambda != T(0));
 
   T amax[3], amin[3];
@@ -182,7 +183,7 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
       }
     }
     r2 -= T(2) * cross;
-    const T rinv = T(1) / sqrt(r2);
+    const T rinv = rsqrt_nr(r2);
     T vv[3];
 #pragma unroll
     for (int l = 0; l < 3; ++l) vv[l] = v0[l] * rinv;
@@ -233,12 +234,22 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
 #pragma unroll
   for (int a = 0; a < D; ++a) inv *= at[a];
   inv = T(1) / inv;
+  // 1 / at[a] from the one division: inv times the other axes' at
+  T iat[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    T o = inv;
+#pragma unroll
+    for (int b = 0; b < D; ++b)
+      if (b != a) o *= at[b];
+    iat[a] = o;
+  }
   speed = T(0);
 #pragma unroll
   for (int a = 0; a < D; ++a) speed += at[a];
   T tw[D];
 #pragma unroll
-  for (int a = 0; a < D; ++a) tw[a] = am[a] * ap[a] / at[a];
+  for (int a = 0; a < D; ++a) tw[a] = am[a] * ap[a] * iat[a];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
 #pragma unroll

@@ -24,6 +24,7 @@ struct GreenParams {
   const Clock* clk;
   LoadParams load;
   Elastic el;
+  int log_stress;   // 1: the k = 0 thread appends the clock's stress-strain record
 };
 
 // Called after each kernel launch with the kernel kind (HIP-event timing).
@@ -101,6 +102,8 @@ struct VjpJob {
   unsigned long long* codes;  // out: per-cell minmod selections (2 bits per axis and component)
   double* sbar;        // out: symmetrised stress cotangent (9 planes)
   double* lpart;       // out: per-block Lambda cotangent
+  double* upart;       // out (optional): per-block sum of lam tr(sbar) + 2 mu sbar_zz,
+                       // the cotangent of the uniaxial control's betaE_zz
   double Lambda, mu, lam;
 };
 // out = J_L(beta)^T rbar assembled from the derivative cotangents, the stress
@@ -116,8 +119,16 @@ struct GatherJob {
   int nterm;
   const double* term[3];
   double coef[3];
+  const double* uzz;    // optional device scalar: uzz_scale * (*uzz) is added to component zz
+  double uzz_scale;
+  const double* csrc;   // optional: 9 device constants added per component
   double* out;
 };
+// per-block partials of lam tr(sbar) + 2 mu sbar_zz (uniaxial control)
+hipError_t launch_uniaxial_partial(const double* sbar, long long cells, double lam, double mu, double* partial,
+                                   int* nblocks, hipStream_t st);
+// out[zz] += scale * (*s) at every cell
+hipError_t launch_add_uniform_zz(double* out, long long cells, const double* s, double scale, hipStream_t st);
 int flux_vjp_nblocks(int d, int n);
 hipError_t launch_clock_segment(Clock* clk, int mode, double t0, const double* dts, long long len, hipStream_t st);
 hipError_t launch_flux_vjp_cell(const VjpJob& j, int* nblocks, hipStream_t st);

@@ -348,6 +348,27 @@ k_contract(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, GreenParams g
         double s[9];
         const double mean_bzz = (double)in[2].x * gp.norm;
         load_shift(gp.load, gp.el, tnow, mean_bzz, s);
+        if (gp.log_stress && c == 0 && gp.clk && gp.clk->slog) {
+          // stress-strain record of the step start: t, mean sigma (load
+          // included; the non-local part has zero mean), mean beta_zz
+          const Clock* ck = gp.clk;
+          const long long li = ck->nsteps - ck->slog_base;
+          if (li >= 0 && li < ck->slog_cap) {
+            double e[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) e[q] = (double)in[q].x * gp.norm;
+            const double tr = e[0] + e[1] + e[2], lam = gp.el.lam, mu2 = 2.0 * gp.el.mu;
+            double* rec = ck->slog + 8 * li;
+            rec[0] = tnow;
+            rec[1] = s[0] - lam * tr - mu2 * e[0];
+            rec[2] = s[4] - lam * tr - mu2 * e[1];
+            rec[3] = s[8] - lam * tr - mu2 * e[2];
+            rec[4] = s[1] - mu2 * e[3];
+            rec[5] = s[2] - mu2 * e[4];
+            rec[6] = s[5] - mu2 * e[5];
+            rec[7] = mean_bzz;
+          }
+        }
         const int cs9 = c == 0 ? 0 : c == 1 ? 4 : c == 2 ? 8 : c == 3 ? 1 : c == 4 ? 2 : 5;
         T sv = T(0);
 #pragma unroll

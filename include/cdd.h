@@ -130,13 +130,22 @@ int cdd_state_ptr(cdd_plan* plan, double** beta_dev);
  * forward, 2 Green's contraction pass, 3 column FFT inverse, 4 c2r,
  * 5 flux stage 1, 6 CFL reduction, 7 flux stage 2, 8 flux stage 3,
  * 9 clock advance. */
+/* Stress-strain log (replaces the per-state post-processing of
+ * PlottingTools.StrainStress, Plasticity/PlottingTools.py:69-100, and the
+ * sigma_zz of RunLoadingMPI's EnergyDownhillChecking,
+ * Plasticity/MattTools/Run/RunLoadingMPI.py:20-58): from the next step on,
+ * every step appends 8 doubles to log_dev (device, cap records): the step's
+ * start time t, the mean stress including the external load (xx, yy, zz,
+ * xy, xz, yz) of the state at t, and mean beta_zz.  Recorded by the stress
+ * solve itself (its k = 0 mode): no extra pass, no host sync.  cap = 0 stops. */
+int cdd_stress_record(cdd_plan* plan, double* log_dev, long long cap);
+
 int cdd_timing_enable(cdd_plan* plan, int max_events);
 int cdd_timing_read(cdd_plan* plan, double* ms, int* counts, int nkinds);
 
 /* ---- Adjoint (no reference counterpart: the reference has no gradients) --
  * Vector-Jacobian products with the time-step schedule frozen: the CFL
- * reduction (FieldMover.py:94-96) is treated as a constant.  Loads that
- * depend on beta (CDD_LOAD_UNIAXIAL) are not supported.
+ * reduction (FieldMover.py:94-96) is treated as a constant.
  *   cdd_flux_vjp:    betabar = (dL/dbeta)^T rbar for the flux L of
  *                    BetaPDynamics.CalculateFlux at time t; *lambar =
  *                    (dL/dLambda) . rbar (nullable).
@@ -152,10 +161,17 @@ int cdd_timing_read(cdd_plan* plan, double* ms, int* counts, int nkinds);
  *                    replaying dts_dev (the recorded dt of those steps),
  *                    keeping every stage state and its stress planes, then
  *                    carry the cotangent of the segment's final state
- *                    (ubar_end) back to ubar0 (may alias ubar_end).  The
- *                    plan's clock and state are left unchanged; the Lambda
- *                    cotangent is added to a device accumulator.  No host
- *                    synchronisation.  Workspace: 15 rk_order planes per step.
+ *                    (ubar_end) back to ubar0 (may alias ubar_end).
+ *                    src_dev (nullable, nsteps x 9 device doubles): a
+ *                    uniform per-component gradient of the objective at the
+ *                    state starting step j, added to its cotangent (time-
+ *                    distributed objectives such as a mean stress over the
+ *                    last steps).  The plan's clock and state are left
+ *                    unchanged; the Lambda cotangent is added to a device
+ *                    accumulator.  No host synchronisation.  Workspace:
+ *                    15 rk_order planes per step.
+ * All loads are supported; the uniaxial strain-rate control's dependence on
+ * <beta_zz> is differentiated (a uniform term in betabar_zz).
  *   cdd_lambda_bar:  read (nullable) and optionally reset that accumulator. */
 int cdd_flux_vjp(cdd_plan* plan, const double* beta_dev, double t, const double* rbar_dev, double* betabar_dev,
                  double* lambar);
@@ -165,7 +181,7 @@ int cdd_energy_grad(cdd_plan* plan, const double* beta_dev, double t, double* be
 int cdd_dt_record(cdd_plan* plan, double* log_dev, long long cap);
 int cdd_dt_replay(cdd_plan* plan, const double* sched_dev, long long len);
 int cdd_segment_vjp(cdd_plan* plan, const double* u0_dev, double t0, const double* dts_dev, int nsteps,
-                    const double* ubar_end_dev, double* ubar0_dev);
+                    const double* src_dev, const double* ubar_end_dev, double* ubar0_dev);
 int cdd_lambda_bar(cdd_plan* plan, double* value, int reset);
 
 /* ---- Batched independent realisations (2D) ------------------------------

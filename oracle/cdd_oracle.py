@@ -608,6 +608,17 @@ class Dynamics:
         return central_upwind_flux(beta, self.get_sigma(beta, t), self.dynamics, self.Lambda)
 
 
+def mean_stress(beta, t, mu=1.0, nu=0.3, load=None):
+    """Mean stress of the state at time t, external load included: the
+    stress-strain point of PlottingTools.StrainStress (PlottingTools.py:69-100:
+    sigma(source='betaE') averaged, plus lam tr(e) d + 2 mu e of the imposed
+    strain) and, for the uniaxial strain-rate control, the mean of the
+    loaded sigma of RunLoadingMPI.py:68-83.  Returns (xx, yy, zz, xy, xz, yz)."""
+    dyn = Dynamics(mu=mu, nu=nu, load=load)
+    s = dyn.get_sigma(beta, t)
+    return np.array([s[cidx(i, j)].mean() for i, j in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))])
+
+
 # ---------------------------------------------------------------------------
 # Time stepping: FieldMovers/FieldMover.py:14-141
 # ---------------------------------------------------------------------------

@@ -259,3 +259,67 @@ def test_end_to_end_3d_rk2_matches_torch_autograd(L):
     TR.energy(u, MU, NU, shear_shift()).backward()
     assert rel(b.grad.cpu().numpy(), bt.grad.numpy()) < 1e-9
     assert abs(lt.grad.item() - ltt.grad.item()) <= 1e-9 * abs(ltt.grad.item())
+
+
+UNIAX = dict(load_kind=3, load_init_strain=0.001, load_uniaxial_rate=0.05)
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_uniaxial_step_vjp_and_energy_grad_match_autograd(L, order):
+    """The uniaxial strain-rate control's stress depends on <beta_zz>; its
+    transpose (a uniform term in betabar_zz) in the step VJP and the energy
+    gradient, against torch autograd."""
+    say("uniaxial step vjp", order)
+    n, t, dt, Lam = 16, 0.3, 2e-3, 0.9
+    beta = SpectralGaussianField((n, n), 2.0, 0.01, seed=50)
+    beta[8] += 0.002
+    p = L.Plan(2, n, dynamics=0, rk_order=order, Lambda=Lam, mu=MU, nu=NU, **UNIAX)
+    gbar = np.random.default_rng(5).standard_normal((9, n, n))
+    got, lam = p.step_vjp(torch.from_numpy(beta).cuda(), t, dt, torch.from_numpy(gbar).cuda())
+    bt = torch.from_numpy(beta).requires_grad_(True)
+    lt = torch.tensor(Lam, dtype=torch.float64, requires_grad=True)
+    shift = TR.uniaxial_shift(t, 0.05, 0.001, MU, NU)
+    out = TR.rk_step(bt, dt, lt, False, MU, NU, order, shift)
+    (out * torch.from_numpy(gbar)).sum().backward()
+    assert rel(got.cpu().numpy(), bt.grad.numpy()) < 1e-10
+    assert abs(lam - lt.grad.item()) <= 1e-10 * abs(lt.grad.item())
+    g, e = p.energy_grad(torch.from_numpy(beta).cuda(), t)
+    bt2 = torch.from_numpy(beta).requires_grad_(True)
+    E = TR.energy(bt2, MU, NU, shift)
+    E.backward()
+    assert abs(e - E.item()) <= 1e-12 * abs(E.item())
+    assert rel(g.cpu().numpy(), bt2.grad.numpy()) < 1e-11
+
+
+def test_mean_stress_window_matches_torch_autograd(L):
+    """Stress-strain objective under the uniaxial control: <sigma_zz> averaged
+    over the last 3 of 6 RK3 steps, gradient w.r.t. beta0 and Lambda through
+    the checkpointed adjoint (per-step sources), against torch autograd."""
+    say("mean stress window vs torch")
+    from Plasticity.amd.autograd import mean_stress_window
+    n, nsteps, W, Lam = 16, 6, 3, 0.9
+    beta0 = SpectralGaussianField((n, n), 2.0, 0.01, seed=51)
+    b = torch.from_numpy(beta0).cuda().requires_grad_(True)
+    lt = torch.tensor(Lam, dtype=torch.float64, device="cuda", requires_grad=True)
+    J = mean_stress_window(b, lt, nsteps, W, "zz", t0=0.1, checkpoint_every=4, rk_order=3, load=UNIAX)
+    J.backward()
+    plan = L.Plan(2, n, dynamics=0, rk_order=3, Lambda=Lam, mu=MU, nu=NU, **UNIAX)
+    log = torch.zeros(nsteps, dtype=torch.float64, device="cuda")
+    plan.set_state(beta0)
+    plan.set_time(0.1)
+    plan.dt_record(log)
+    plan.step(nsteps)
+    dts = log.cpu().tolist()
+    bt = torch.from_numpy(beta0).requires_grad_(True)
+    ltt = torch.tensor(Lam, dtype=torch.float64, requires_grad=True)
+    u, t, acc = bt, 0.1, 0.0
+    for j, dt in enumerate(dts):
+        u = TR.rk_step(u, dt, ltt, False, MU, NU, 3, TR.uniaxial_shift(t, 0.05, 0.001, MU, NU))
+        t += dt
+        if j + 1 > nsteps - W:
+            acc = acc + TR.mean_stress(u, MU, NU, TR.uniaxial_shift(t, 0.05, 0.001, MU, NU))[2]
+    Jt = acc / W
+    Jt.backward()
+    assert abs(J.item() - Jt.item()) <= 1e-12 * abs(Jt.item())
+    assert rel(b.grad.cpu().numpy(), bt.grad.numpy()) < 1e-9
+    assert abs(lt.grad.item() - ltt.grad.item()) <= 1e-9 * abs(ltt.grad.item())

@@ -147,3 +147,69 @@ def test_reference_api_drop_in(amd):
     # stress through the state API
     s = state.CalculateSigma().numpy()
     assert rel_l2(s, O.sigma(beta, MU, NU)) < 1e-13
+
+
+LOADS = {
+    "strain": (dict(load_kind=1, load_init=(0.001, -0.002, 0.003) + (0,) * 6, load_rate=(-0.005, -0.005, 0.01) + (0,) * 6,
+                    load_t0=0.1),
+               dict(kind="strain", initial=(0.001, -0.002, 0.003), rate=(-0.005, -0.005, 0.01), t0=0.1)),
+    "stress": (dict(load_kind=2, load_init=(0.002, 0.0, -0.001) + (0,) * 6, load_rate=(0.0, 0.01, 0.02) + (0,) * 6),
+               dict(kind="stress", initial=(0.002, 0.0, -0.001), rate=(0.0, 0.01, 0.02))),
+    "uniaxial": (dict(load_kind=3, load_init_strain=0.001, load_uniaxial_rate=0.05),
+                 dict(kind="uniaxial", init_strain=0.001, uniaxial_rate=0.05)),
+    "tensor": (dict(load_kind=4, load_init=(0, 0.005, 0, 0.005, 0, 0, 0, 0, 0), load_rate=(0, 0, 0.01, 0, 0, 0, 0.01, 0, 0)),
+               dict(kind="tensor", tensor0=(0, 0.005, 0, 0.005, 0, 0, 0, 0, 0), tensor_rate=(0, 0, 0.01, 0, 0, 0, 0.01, 0, 0))),
+}
+
+
+@pytest.mark.parametrize("kind", sorted(LOADS))
+def test_loaded_steps_and_stress_log_match_oracle(amd, kind):
+    """Loaded RK3 steps (3D 16^3, glide+climb) against the oracle, and the
+    per-step stress-strain record against the oracle's mean stress of the
+    GPU's own state at each step start."""
+    import torch
+    say("loaded steps", kind)
+    pk, ok = LOADS[kind]
+    n, nsteps = 16, 6
+    beta = SpectralGaussianField((n,) * 3, 2.0, 0.01, seed=17)
+    p = _plan(amd, 3, n, 0, 0.9, rk=3, **pk)
+    p.set_state(beta)
+    p.set_time(0.2)
+    log = torch.zeros((nsteps, 8), dtype=torch.float64, device="cuda")
+    p.stress_record(log)
+    states = []
+    for _ in range(nsteps):
+        states.append(p.get_state().cpu().numpy())
+        p.step(1)
+    p.stress_record(None)
+    got = p.get_state().cpu().numpy()
+    load = O.Load(**ok)
+    od = O.Dynamics(O.GLIDE_CLIMB, 0.9, MU, NU, load=load)
+    ref, tref, dts = O.run(od, beta, 0.2, nsteps, 0.25, order=3)
+    assert rel_l2(got, ref) < 1e-11
+    rec = log.cpu().numpy()
+    t = 0.2
+    for j in range(nsteps):
+        assert abs(rec[j, 0] - t) <= 1e-13 * t
+        want = O.mean_stress(states[j], rec[j, 0], MU, NU, load)
+        np.testing.assert_allclose(rec[j, 1:7], want, rtol=1e-11, atol=1e-15)
+        assert abs(rec[j, 7] - states[j][8].mean()) <= 1e-15
+        t += dts[j]
+
+
+def test_config1_2d_128_lvp_rk2_100_steps(amd):
+    """2D 128^2, seed 0, L = 4 cells, RMS 0.01, mu = 1, nu = 0.3, glide-only,
+    TVD-RK2, CFL 0.25, 100 adaptive steps: 1e-11 relative L2 to the oracle."""
+    say("config 1: 2D 128 lvp RK2 x100")
+    n, nsteps = 128, 100
+    beta = SpectralGaussianField((n, n), 4.0, 0.01, seed=0)
+    p = _plan(amd, 2, n, 1, 0.0, rk=2)
+    p.set_state(beta)
+    p.set_time(0.0)
+    p.step(nsteps)
+    got = p.get_state().cpu().numpy()
+    t, _, _, ns = p.clock()
+    od = O.Dynamics(O.GLIDE_ONLY, 0.0, MU, NU)
+    ref, tref, _ = O.run(od, beta, 0.0, nsteps, 0.25, order=2)
+    assert ns == nsteps and abs(t - tref) <= 1e-12 * tref
+    assert rel_l2(got, ref) < 1e-11

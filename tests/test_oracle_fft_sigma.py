@@ -198,3 +198,30 @@ def test_loads_equal_stress_of_uniform_distortion():
     B = np.zeros((9,) + shape)
     B[8] = -bzz
     np.testing.assert_allclose(got, O.sigma(B, mu, nu), atol=1e-15)
+
+
+@pytest.mark.parametrize("kind", ["none", "strain", "uniaxial"])
+def test_mean_stress_is_elastic_response_to_mean_distortion(kind):
+    """Fixed-volume boundary: the mean elastic distortion is -<beta^P>, so
+    the mean stress is -(lam tr<beta> d + 2 mu sym<beta>) plus the load
+    shift, whatever the fluctuations (PlottingTools.py:69-100)."""
+    from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
+    mu, nu, t = 0.5, 0.3, 1.7
+    lam = 2 * mu * nu / (1 - 2 * nu)
+    beta = SpectralGaussianField((8, 8, 8), 2.0, 0.01, seed=11)
+    beta += np.arange(9).reshape(9, 1, 1, 1) * 1e-3  # nonzero means
+    load = {"none": O.Load(),
+            "strain": O.Load("strain", initial=(0.0, 0.0, 0.0), rate=(-0.005, -0.005, 0.01)),
+            "uniaxial": O.Load("uniaxial", init_strain=0.001, uniaxial_rate=0.01)}[kind]
+    got = O.mean_stress(beta, t, mu, nu, load)
+    mb = beta.reshape(9, -1).mean(axis=1).reshape(3, 3)
+    eps = 0.5 * (mb + mb.T)
+    sig = -(lam * np.trace(mb) * np.eye(3) + 2 * mu * eps)
+    if kind == "strain":  # StrainStress: + lam tr(e) d + 2 mu e of the imposed strain
+        e = np.diag(np.array([-0.005, -0.005, 0.01]) * t)
+        sig = sig + lam * np.trace(e) * np.eye(3) + 2 * mu * e
+    elif kind == "uniaxial":  # RunLoadingMPI: (lam d + 2 mu d_iz d_jz) betaE_zz
+        bzz = 0.01 * t - (mb[2, 2] - 0.001)
+        sig = sig + bzz * (lam * np.eye(3) + 2 * mu * np.diag([0.0, 0.0, 1.0]))
+    want = np.array([sig[0, 0], sig[1, 1], sig[2, 2], sig[0, 1], sig[0, 2], sig[1, 2]])
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-15)

@@ -17,6 +17,8 @@ def _k(shape):
 
 
 def sigma(beta, mu, nu, load_shift=None):
+    if callable(load_shift):  # a load that depends on the state (uniaxial control)
+        load_shift = load_shift(beta)
     shape = tuple(beta.shape[1:])
     d = len(shape)
     k, kSq0, kSqSq0, kmask = _k(shape)
@@ -170,6 +172,24 @@ def flux(beta, sig, Lambda, lvp):
     for a in range(d):
         deriv = deriv + (dv[a][0] - dv[a][1]) * sm[a] * sp[a] / at[a]
     return deriv
+
+
+def uniaxial_shift(t, rate, init_strain, mu, nu):
+    """RunLoadingMPI.py:68-83: sigma += (lam d_ij + 2 mu d_iz d_jz) betaE_zz with
+    betaE_zz = rate t - (<beta_zz> - init), as a function of the state."""
+    lam = 2.0 * mu * nu / (1.0 - 2.0 * nu)
+    w = torch.tensor([lam, 0, 0, 0, lam, 0, 0, 0, lam + 2.0 * mu], dtype=torch.float64)
+
+    def shift(beta):
+        bzz = rate * t - (beta[8].mean() - init_strain)
+        return w * bzz
+    return shift
+
+
+def mean_stress(beta, mu, nu, load_shift=None):
+    """Mean of the loaded stress, components xx yy zz xy xz yz."""
+    s = sigma(beta, mu, nu, load_shift)
+    return torch.stack([s[c].mean() for c in (0, 4, 8, 1, 2, 5)])
 
 
 def energy(beta, mu, nu, load_shift=None):
