@@ -1,0 +1,78 @@
+"""GPU parity at larger grids and longer runs, against the C/OpenMP
+restatement of the oracle (oracle/cdd_oracle.c, itself checked against the
+NumPy oracle in tests/test_oracle_c.py).  Glide + climb, climb mobility 0.1
+(Lambda = 0.9), TVD-RK3, CFL 0.25, the seeded field of the 3D workloads
+(L = 6 cells, RMS 0.01); 1e-11 relative L2 after the run."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import cdd_oracle as O
+from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def env():
+    import os
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from oracle import cdd_oracle_c as OC
+    if not os.path.exists(OC.LIB):
+        pytest.skip("the C oracle is not built (run __graft_entry__.build())")
+    from Plasticity import amd as A
+    return A, OC
+
+
+def _compare(env, n, nsteps, seed=2):
+    A, OC = env
+    beta = SpectralGaussianField((n, n, n), 6.0, 0.01, seed=seed)
+    p = A.Plan(3, n, dynamics=0, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25)
+    p.set_state(beta)
+    p.set_time(0.0)
+    p.step(nsteps)
+    got = p.get_state().cpu().numpy()
+    t, _, _, ns = p.clock()
+    say("  GPU done; C oracle ...")
+    import time
+    dyn = O.Dynamics(O.GLIDE_CLIMB, 0.9, 1.0, 0.3)
+    # the C oracle in chunks of 10 steps (same step sequence), with progress
+    ref, tref, dts = beta, 0.0, []
+    t0 = time.perf_counter()
+    while len(dts) < nsteps:
+        k = min(10, nsteps - len(dts))
+        ref, tref, d = OC.run(dyn, ref, tref, k, 0.25, order=3)
+        dts.extend(d)
+        say("  C oracle %d/%d steps, %.1f s" % (len(dts), nsteps, time.perf_counter() - t0))
+    err = np.linalg.norm((got - ref).ravel()) / np.linalg.norm(ref.ravel())
+    say("  rel L2 %.3e, t %.17g vs %.17g" % (err, t, tref))
+    assert ns == nsteps and len(dts) == nsteps
+    assert abs(t - tref) <= 1e-12 * tref
+    assert err < 1e-11
+    return err
+
+
+def test_3d_64_50_steps(env):
+    say("3D 64^3 x 50 RK3 steps vs C oracle")
+    _compare(env, 64, 50)
+
+
+def test_3d_128_5_steps(env):
+    say("3D 128^3 x 5 RK3 steps vs C oracle")
+    _compare(env, 128, 5)
+
+
+@pytest.mark.full_size
+def test_config2_3d_128_200_steps(env):
+    """The full 3D workload check: 128^3, 200 RK3 steps (about ten minutes of
+    host CPU for the C oracle on 16 threads)."""
+    say("config 2: 3D 128^3 x 200 RK3 steps vs C oracle")
+    _compare(env, 128, 200)
