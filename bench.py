@@ -237,6 +237,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fd-dirs", type=int, default=32, help="adjoint workload: finite-difference directions")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
+                    help="3d workload: arithmetic of the plan (fp32 also reports its error against fp64)")
     ap.add_argument("--workload", default="3d", choices=["3d", "batched", "adjoint"],
                     help="3d: the headline 256^3 stepper (default); batched: 48 uneven 2D realisations, "
                          "batched hipGraph step vs a serial loop of plain plans")
@@ -280,13 +282,17 @@ def main():
     probe.close()
     del speed
     rate = 0.01 / (1000.0 * dt0)
+    loadkw = dict(load_kind=L.LOAD_UNIAXIAL, load_init_strain=init_strain, load_uniaxial_rate=rate)
     plan = L.Plan(3, n, dynamics=L.GLIDE_CLIMB, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25,
-                  load_kind=L.LOAD_UNIAXIAL, load_init_strain=init_strain, load_uniaxial_rate=rate)
+                  precision=args.precision, **loadkw)
     plan.set_state(beta_t)
     plan.set_time(0.0)
     plan.step(args.warmup)
     torch.cuda.synchronize()
     plan.timing_enable(args.steps * 24 + 8)
+    # stress-strain record of every timed step (written by the stress solve itself)
+    slog = torch.zeros((args.steps, 8), dtype=torch.float64, device=dev)
+    plan.stress_record(slog)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     plan.step(args.steps)
@@ -294,6 +300,7 @@ def main():
     t1 = time.perf_counter()
     kt = plan.timing_read()
     plan.timing_enable(0)
+    plan.stress_record(None)
     t, dt, done, nsteps = plan.clock()
     state = plan.get_state()
     finite = bool(torch.isfinite(state).all())
@@ -301,7 +308,19 @@ def main():
     ms_step = 1e3 * elapsed / args.steps
     value = cells * args.steps / elapsed
 
-    per_cell = spectral_bytes(n)
+    fp32_err = None
+    if args.precision == "fp32":
+        # the same run in float64: the fp32 variant's error is stated against it
+        ref = L.Plan(3, n, dynamics=L.GLIDE_CLIMB, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25, **loadkw)
+        ref.set_state(beta_t)
+        ref.set_time(0.0)
+        ref.step(args.warmup + args.steps)
+        s64 = ref.get_state()
+        fp32_err = float(torch.linalg.norm(state.double() - s64) / torch.linalg.norm(s64))
+        ref.close()
+        del s64
+    vb = 0.5 if args.precision == "fp32" else 1.0  # bytes per value relative to fp64
+    per_cell = {k: v * vb for k, v in spectral_bytes(n).items()}
     kernels = {}
     for k, (ms, cnt) in kt.items():
         if cnt == 0:
@@ -316,14 +335,15 @@ def main():
     achieved = dom_bytes / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_3d256.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and args.precision == "fp64" and n == 256:
         try:
             traffic = json.load(open(tpath)).get(dom)
         except Exception:
             traffic = None
-    bpc = canonical_bytes_per_cell_update(n)
+    bpc = canonical_bytes_per_cell_update(n) * vb
+    rec = slog.cpu().numpy()
     out = {
-        "metric": "cell-updates/s (3D 256^3 fp64 glide+climb TVD-RK3)",
+        "metric": "cell-updates/s (3D %d^3 %s glide+climb TVD-RK3)" % (n, args.precision),
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -333,9 +353,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if args.precision == "fp32" else "f64",
         "data": "synthetic: seeded Gaussian random beta^P (k-filter exp(-k^2 L^2/2), L=6 cells, RMS 0.01)",
-        "config": {"workload": "cdd-3d-256-glide-climb-rk3", "grid": [n, n, n], "dynamics": "glide+climb",
+        "config": {"workload": "cdd-3d-%d-glide-climb-rk3%s" % (n, "-fp32" if args.precision == "fp32" else ""),
+                   "grid": [n, n, n], "dynamics": "glide+climb",
                    "Lambda": 0.9, "rk_order": 3, "cfl": 0.25, "mu": 1.0, "nu": 0.3,
                    "load": "uniaxial strain rate %.4g (0.01 after ~1000 steps)" % rate},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
@@ -346,7 +367,12 @@ def main():
         "sim_time": t,
         "last_dt": dt,
         "state_finite": finite,
+        "stress_strain": {"note": "per-step record of the timed steps: (strain = rate t, mean sigma_zz with load)",
+                          "first": [float(rate * rec[0, 0]), float(rec[0, 3])],
+                          "last": [float(rate * rec[-1, 0]), float(rec[-1, 3])]},
     }
+    if fp32_err is not None:
+        out["fp32_rel_l2_vs_fp64"] = fp32_err
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

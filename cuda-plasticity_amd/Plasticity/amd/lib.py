@@ -12,6 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libcdd.so")
 
 GLIDE_CLIMB, GLIDE_ONLY = 0, 1
+FP64, FP32 = 0, 1
 LOAD_NONE, LOAD_STRAIN, LOAD_STRESS, LOAD_UNIAXIAL, LOAD_TENSOR = 0, 1, 2, 3, 4
 
 
@@ -36,6 +37,7 @@ class Config(ctypes.Structure):
         ("load_t0", ctypes.c_double),
         ("load_init_strain", ctypes.c_double),
         ("load_uniaxial_rate", ctypes.c_double),
+        ("precision", ctypes.c_int),
     ]
 
 
@@ -113,11 +115,12 @@ def check(rc):
 
 
 def _ptr(t):
-    """Device pointer of a torch tensor (must be contiguous float64 on the GPU)."""
+    """Device pointer of a torch tensor (contiguous float64, or float32 for the
+    fields of an fp32 plan, on the GPU)."""
     if not t.is_cuda:
         raise CDDError("expected a device tensor")
-    if t.dtype.__repr__() != "torch.float64":
-        raise CDDError("expected float64")
+    if t.dtype.__repr__() not in ("torch.float64", "torch.float32"):
+        raise CDDError("expected float64 or float32")
     if not t.is_contiguous():
         raise CDDError("expected a contiguous tensor")
     return ctypes.c_void_p(t.data_ptr())
@@ -128,10 +131,15 @@ class Plan:
 
     def __init__(self, ndim, n, dynamics=GLIDE_CLIMB, rk_order=3, Lambda=0.0, mu=1.0, nu=0.3, cfl=0.25,
                  load_kind=LOAD_NONE, load_init=None, load_rate=None, load_t0=0.0, load_init_strain=0.0,
-                 load_uniaxial_rate=0.0, device=0):
+                 load_uniaxial_rate=0.0, device=0, precision="fp64"):
         import torch
         self.torch = torch
+        if precision not in ("fp64", "fp32"):
+            raise CDDError("precision must be 'fp64' or 'fp32'")
+        self.precision = precision
+        self.dtype = torch.float32 if precision == "fp32" else torch.float64
         cfg = Config()
+        cfg.precision = FP32 if precision == "fp32" else FP64
         cfg.ndim, cfg.n, cfg.dynamics, cfg.rk_order = ndim, n, dynamics, rk_order
         cfg.Lambda, cfg.mu, cfg.nu, cfg.cfl = Lambda, mu, nu, cfl
         cfg.load_kind, cfg.device = load_kind, device
@@ -173,7 +181,7 @@ class Plan:
             pass
 
     def empty(self, ncomp=9):
-        return self.torch.empty((ncomp,) + self.shape, dtype=self.torch.float64, device=self.dev)
+        return self.torch.empty((ncomp,) + self.shape, dtype=self.dtype, device=self.dev)
 
     def set_state(self, beta):
         t = self._as_dev(beta)
@@ -211,7 +219,7 @@ class Plan:
     def flux(self, beta, t=0.0):
         b = self._as_dev(beta)
         rhs = self.empty()
-        speed = self.torch.empty(self.shape, dtype=self.torch.float64, device=self.dev)
+        speed = self.torch.empty(self.shape, dtype=self.dtype, device=self.dev)
         check(lib.cdd_flux(self.h, _ptr(b), t, _ptr(rhs), _ptr(speed)))
         return rhs, speed
 
@@ -316,7 +324,7 @@ class Plan:
         torch = self.torch
         if isinstance(beta, np.ndarray):
             beta = torch.from_numpy(np.ascontiguousarray(beta, dtype=np.float64))
-        t = beta.to(device=self.dev, dtype=torch.float64).contiguous()
+        t = beta.to(device=self.dev, dtype=self.dtype).contiguous()
         if tuple(t.shape) != (9,) + self.shape:
             raise CDDError("expected shape %s, got %s" % ((9,) + self.shape, tuple(t.shape)))
         return t

@@ -35,6 +35,8 @@ int cdd_internal_fail(int code, const char* msg) { return fail(code, msg); }
 struct cdd_plan {
   cdd_config cfg;
   int d, n;
+  int fp32;   // fields stored and computed in float (CDD_FP32)
+  size_t es;  // bytes per field value
   long long cells;
   long long spec_elems;  // complex elements for 9 components
   double* state[2];
@@ -131,19 +133,22 @@ int cdd_plan_create(const cdd_config* cfg, cdd_plan** out) {
   if (cfg->dynamics != CDD_GLIDE_CLIMB && cfg->dynamics != CDD_GLIDE_ONLY) return fail(CDD_ERR_ARG, "bad dynamics");
   if (cfg->load_kind < 0 || cfg->load_kind > 4) return fail(CDD_ERR_ARG, "bad load_kind");
   if (!(cfg->mu > 0) || !(cfg->nu < 0.5) || !(cfg->cfl > 0)) return fail(CDD_ERR_ARG, "bad mu/nu/cfl");
+  if (cfg->precision != CDD_FP64 && cfg->precision != CDD_FP32) return fail(CDD_ERR_ARG, "bad precision");
   HIPCHK(hipSetDevice(cfg->device));
   cdd_plan* p = new cdd_plan();
   memset(p, 0, sizeof(*p));
   p->cfg = *cfg;
   p->d = cfg->ndim;
   p->n = cfg->n;
+  p->fp32 = cfg->precision == CDD_FP32;
+  p->es = p->fp32 ? sizeof(float) : sizeof(double);
   const int n = cfg->n;
   p->cells = (long long)n * n * (p->d == 3 ? n : 1);
   const long long H = n / 2 + 1;
   p->spec_elems = 9LL * H * (p->d == 3 ? (long long)n * n : (long long)n);
   p->nblocks = flux_nblocks(p->d, n);
   p->npartial = p->nblocks > (int)((p->cells + 255) / 256) ? p->nblocks : (int)((p->cells + 255) / 256);
-  const size_t fb = sizeof(double) * 9 * (size_t)p->cells;
+  const size_t fb = p->es * 9 * (size_t)p->cells;
 #define ALLOC(ptr, bytes)                                 \
   do {                                                    \
     hipError_t e_ = hipMalloc((void**)&(ptr), (bytes));   \
@@ -156,12 +161,12 @@ int cdd_plan_create(const cdd_config* cfg, cdd_plan** out) {
   if (cfg->rk_order == 2) ALLOC(p->state[1], fb);
   ALLOC(p->L0, fb);
   ALLOC(p->buf, fb);
-  ALLOC(p->G, sizeof(double) * 6 * (size_t)p->cells);
+  ALLOC(p->G, p->es * 6 * (size_t)p->cells);
   ALLOC(p->work9, fb);
   ALLOC(p->partial, sizeof(double) * (size_t)p->npartial);
   ALLOC(p->scalar, sizeof(double) * 8);
-  ALLOC(p->spec, sizeof(double) * 2 * (size_t)p->spec_elems);
-  ALLOC(p->tw, sizeof(double) * 2 * (size_t)n);
+  ALLOC(p->spec, p->es * 2 * (size_t)p->spec_elems);
+  ALLOC(p->tw, p->es * 2 * (size_t)n);
   ALLOC(p->clk, 2 * sizeof(Clock));  // [1]: saved clock of a segment replay
 #undef ALLOC
   if (cfg->rk_order == 3) p->state[1] = p->buf;
@@ -173,7 +178,15 @@ int cdd_plan_create(const cdd_config* cfg, cdd_plan** out) {
       h[2 * e] = (double)cosl(th);
       h[2 * e + 1] = (double)(-sinl(th));
     }
-    hipError_t e_ = hipMemcpy(p->tw, h, sizeof(double) * 2 * n, hipMemcpyHostToDevice);
+    hipError_t e_;
+    if (p->fp32) {
+      float* hf = new float[2 * n];
+      for (int e = 0; e < 2 * n; ++e) hf[e] = (float)h[e];
+      e_ = hipMemcpy(p->tw, hf, sizeof(float) * 2 * n, hipMemcpyHostToDevice);
+      delete[] hf;
+    } else {
+      e_ = hipMemcpy(p->tw, h, sizeof(double) * 2 * n, hipMemcpyHostToDevice);
+    }
     delete[] h;
     if (e_ != hipSuccess) {
       cdd_plan_destroy(p);
@@ -255,33 +268,33 @@ int cdd_set_stream(cdd_plan* p, void* stream) {
   return CDD_OK;
 }
 
-int cdd_set_state(cdd_plan* p, const double* beta, int on_device) {
+int cdd_set_state(cdd_plan* p, const void* beta, int on_device) {
   if (!p || !beta) return fail(CDD_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(p->cfg.device));
   Clock c;
   if (sync_clock(p, &c)) return CDD_ERR_HIP;
-  const size_t fb = sizeof(double) * 9 * (size_t)p->cells;
+  const size_t fb = p->es * 9 * (size_t)p->cells;
   HIPCHK(hipMemcpyAsync(p->state[cur_state(p)], beta, fb, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                         p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
   return CDD_OK;
 }
 
-int cdd_get_state(cdd_plan* p, double* beta, int on_device) {
+int cdd_get_state(cdd_plan* p, void* beta, int on_device) {
   if (!p || !beta) return fail(CDD_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(p->cfg.device));
   if (sync_clock(p, nullptr)) return CDD_ERR_HIP;
-  const size_t fb = sizeof(double) * 9 * (size_t)p->cells;
+  const size_t fb = p->es * 9 * (size_t)p->cells;
   HIPCHK(hipMemcpyAsync(beta, p->state[cur_state(p)], fb, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                         p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
   return CDD_OK;
 }
 
-int cdd_state_ptr(cdd_plan* p, double** beta_dev) {
+int cdd_state_ptr(cdd_plan* p, void** beta_dev) {
   if (!p || !beta_dev) return fail(CDD_ERR_ARG, "null argument");
   if (sync_clock(p, nullptr)) return CDD_ERR_HIP;
-  *beta_dev = p->state[cur_state(p)];
+  *beta_dev = (void*)p->state[cur_state(p)];
   return CDD_OK;
 }
 
@@ -328,6 +341,7 @@ static SpectralJob spectral_job(cdd_plan* p, int nc, const double* A, const doub
   j.stream = p->stream;
   j.hook = p->timing ? timing_hook : nullptr;
   j.hook_ctx = p;
+  j.fp32 = p->fp32;
   return j;
 }
 
@@ -343,19 +357,24 @@ static FluxJob flux_job(cdd_plan* p) {
   f.mu = p->gp.mu;
   f.lam = p->gp.lam;
   f.stream = p->stream;
+  f.fp32 = p->fp32;
   return f;
 }
 
-int cdd_sigma(cdd_plan* p, const double* beta, double t, double* sigma) {
+int cdd_sigma(cdd_plan* p, const void* beta_v, double t, void* sigma_v) {
+  const double* beta = static_cast<const double*>(beta_v);  // float data for fp32 plans
+  double* sigma = static_cast<double*>(sigma_v);
   if (!p || !beta || !sigma) return fail(CDD_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(p->cfg.device));
   SpectralJob j = spectral_job(p, 6, beta, nullptr, nullptr, t, p->G);
   HIPCHK(launch_spectral(j));
-  HIPCHK(launch_sigma_finalize(p->G, beta, sigma, p->cells, p->gp.mu, p->gp.lam, p->stream));
+  HIPCHK(launch_sigma_finalize(p->G, beta, sigma, p->cells, p->gp.mu, p->gp.lam, p->stream, p->fp32));
   return CDD_OK;
 }
 
-int cdd_rho(cdd_plan* p, const double* beta, double* rho) {
+int cdd_rho(cdd_plan* p, const void* beta_v, void* rho_v) {
+  const double* beta = static_cast<const double*>(beta_v);
+  double* rho = static_cast<double*>(rho_v);
   if (!p || !beta || !rho) return fail(CDD_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(p->cfg.device));
   SpectralJob j = spectral_job(p, 9, beta, nullptr, nullptr, 0.0, rho);
@@ -363,7 +382,10 @@ int cdd_rho(cdd_plan* p, const double* beta, double* rho) {
   return CDD_OK;
 }
 
-int cdd_flux(cdd_plan* p, const double* beta, double t, double* rhs, double* speed) {
+int cdd_flux(cdd_plan* p, const void* beta_v, double t, void* rhs_v, void* speed_v) {
+  const double* beta = static_cast<const double*>(beta_v);
+  double* rhs = static_cast<double*>(rhs_v);
+  double* speed = static_cast<double*>(speed_v);
   if (!p || !beta || !rhs) return fail(CDD_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(p->cfg.device));
   SpectralJob j = spectral_job(p, 6, beta, nullptr, nullptr, t, p->G);
@@ -473,7 +495,8 @@ int cdd_step(cdd_plan* p, int nsteps) {
   return CDD_OK;
 }
 
-int cdd_energy(cdd_plan* p, const double* beta, double t, double* energy) {
+int cdd_energy(cdd_plan* p, const void* beta_v, double t, double* energy) {
+  const double* beta = static_cast<const double*>(beta_v);
   if (!p || !energy) return fail(CDD_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(p->cfg.device));
   if (!beta) {
@@ -482,7 +505,7 @@ int cdd_energy(cdd_plan* p, const double* beta, double t, double* energy) {
   }
   HIPCHK(launch_spectral(spectral_job(p, 6, beta, nullptr, nullptr, t, p->G)));
   int nb = 0;
-  HIPCHK(launch_energy(p->G, beta, p->cells, p->gp.mu, p->gp.el.nu, p->gp.lam, p->partial, &nb, p->stream));
+  HIPCHK(launch_energy(p->G, beta, p->cells, p->gp.mu, p->gp.el.nu, p->gp.lam, p->partial, &nb, p->stream, p->fp32));
   HIPCHK(launch_reduce_sum(p->partial, nb, p->scalar, p->stream, 0));
   HIPCHK(hipMemcpyAsync(energy, p->scalar, sizeof(double), hipMemcpyDeviceToHost, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
@@ -491,6 +514,7 @@ int cdd_energy(cdd_plan* p, const double* beta, double t, double* energy) {
 
 // ------------------------------------------------------------- adjoint
 static int adj_alloc(cdd_plan* p) {
+  if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "the adjoint is float64 only");
   if (p->adj) return CDD_OK;
   const size_t fb = sizeof(double) * 9 * (size_t)p->cells;
   if (hipMalloc((void**)&p->adj, 6 * fb) != hipSuccess || hipMalloc((void**)&p->ubar, 2 * p->d * fb) != hipSuccess ||

@@ -40,6 +40,10 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
   double y = __builtin_amdgcn_rsq(x);
   return y * (1.5 - 0.5 * x * y * y);
 }
+__device__ __forceinline__ float rsqrt_nr(float x) {
+  float y = __builtin_amdgcn_rsqf(x);
+  return y * (1.5f - 0.5f * x * y * y);
+}
 
 // Run-time clock of one stepper instance, resident in device memory so the
 // CFL time step never round-trips through the host.

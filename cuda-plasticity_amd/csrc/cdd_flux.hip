@@ -306,14 +306,21 @@ template <typename T, int D, int DYN, int FTC, int FTR, bool PRE>
 __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   using FT = FluxTile<FTC, FTR>;
   constexpr int FPW = FT::PW, FTILE = FT::TILE, FNT = FT::NT, FFILL = FT::FILL;
+  // the job's field pointers carry T data (float for fp32 plans)
+  const T* __restrict__ A = reinterpret_cast<const T*>(job.A);
+  const T* Gp = reinterpret_cast<const T*>(job.G);
+  const T* Up = reinterpret_cast<const T*>(job.U);
+  T* outp = reinterpret_cast<T*>(job.out);
+  T* speedp = reinterpret_cast<T*>(job.speed);
+  double* partialp = job.partial;
   if (blockIdx.y) {  // batched 2D: instance blockIdx.y
     const long long i = blockIdx.y;
-    job.A += i * job.fstride;
-    if (job.U) job.U += i * job.fstride;
-    job.out += i * job.fstride;
-    job.G += i * job.gstride;
-    if (job.partial) job.partial += i * job.pstride;
-    if (job.speed) job.speed += i * (job.fstride / 9);
+    A += i * job.fstride;
+    if (Up) Up += i * job.fstride;
+    outp += i * job.fstride;
+    Gp += i * job.gstride;
+    if (partialp) partialp += i * job.pstride;
+    if (speedp) speedp += i * (job.fstride / 9);
   }
   if (job.clk) job.clk += blockIdx.y;
   if (job.clk && job.clk->done) return;
@@ -322,7 +329,7 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   __shared__ T red[(FNT + 63) / 64];
   // 3D: per-wave LDS landing zone for G (planes 0-5) and U (planes 6-15),
   // filled by asynchronous global->LDS loads issued at kernel start
-  __shared__ T ug[(D == 3) ? FNT / 64 : 1][16][64];
+  __shared__ T ug[(D == 3 && sizeof(T) == 8) ? FNT / 64 : 1][16][64];
   const int n = job.n;
   const long long cells = job.cells;
   const int tc = threadIdx.x, trr = threadIdx.y;
@@ -354,7 +361,6 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   const T invDx = T(n);
   const T halfDx = T(0.5) * (T(1) / T(n));
   const T dt = job.clk ? T(job.clk->dt) : T(0);
-  const T* __restrict__ A = job.A;
 
   // One batch of loads for every component (a single memory round trip):
   // the plane tile with its 2-cell halo (periodic wrap) and, in 3D, the
@@ -379,7 +385,9 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
     p = p < 0 ? p + n : (p >= n ? p - n : p);
     xoff[k] = (long long)p * splane + (long long)myrow * srow + mycol;
   }
-  if constexpr (D == 3) {
+  // 3D fp64: G and U land in LDS by asynchronous DMA (16 B = two cells per lane)
+  constexpr bool DMA = (D == 3) && (sizeof(T) == 8);
+  if constexpr (DMA) {
     // Each wave's 64 cells are contiguous in memory (TC = 64, or TC = n);
     // every lane moves 16 B (two cells) straight into LDS, no VGPRs used.
     // lane l moves cells (2 l', 2 l' + 1), l' = l mod 32, of the wave's 64
@@ -389,7 +397,7 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
     const long long cellp = pbase + (long long)(r0 + e / FTC) * srow + c0 + e % FTC;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const T* src = job.G + (2 * k + half) * cells + cellp;
+      const T* src = Gp + (2 * k + half) * cells + cellp;
       __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)&ug[wv][2 * k][0], 16,
                                        0, 0);
     }
@@ -398,7 +406,7 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
       for (int k = 0; k < 5; ++k) {
         int c = 2 * k + half;
         c = c < 9 ? c : 8;  // the spare 16th plane receives a copy of plane 8
-        const T* src = job.U + c * cells + cellp;
+        const T* src = Up + c * cells + cellp;
         __builtin_amdgcn_global_load_lds((const void*)src,
                                          (__attribute__((address_space(3))) void*)&ug[wv][6 + 2 * k][0], 16, 0, 0);
       }
@@ -445,14 +453,14 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
     }
   }
   // the asynchronous G/U loads must have landed before LDS is read
-  if constexpr (D == 3) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  if constexpr (DMA) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   auto Gv = [&](int c) -> T {
-    if constexpr (D == 3) return ug[tid >> 6][c][tid & 63];
-    else return job.G[c * cells + idx];
+    if constexpr (DMA) return ug[tid >> 6][c][tid & 63];
+    else return Gp[c * cells + idx];
   };
   auto Uv = [&](int c) -> T {
-    if constexpr (D == 3) return ug[tid >> 6][6 + c][tid & 63];
-    else return job.U[c * cells + idx];
+    if constexpr (DMA) return ug[tid >> 6][6 + c][tid & 63];
+    else return Up[c * cells + idx];
   };
   // sigma = G - lam tr(beta) d - mu (beta + beta^T) at the cell
   T sg[9];
@@ -472,16 +480,16 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   cell_flux<T, D, DYN, PRE>(u, sg, T(job.Lambda), rhs, speed, &vsh[0][tid], FNT);
   if (job.mode == 0) {
 #pragma unroll
-    for (int c = 0; c < 9; ++c) job.out[c * cells + idx] = rhs[c];
+    for (int c = 0; c < 9; ++c) outp[c * cells + idx] = rhs[c];
   } else {
     const T cu = T(job.cu), cf = T(job.cf);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) job.out[c * cells + idx] = cu * Uv(c) + cf * (fc[c] + dt * rhs[c]);
+    for (int c = 0; c < 9; ++c) outp[c * cells + idx] = cu * Uv(c) + cf * (fc[c] + dt * rhs[c]);
   }
-  if (job.speed) job.speed[idx] = speed;
-  if (job.partial) {
+  if (speedp) speedp[idx] = speed;
+  if (partialp) {
     T m = block_max<T>(speed, red);
-    if (tid == 0) job.partial[blockIdx.x] = m;
+    if (tid == 0) partialp[blockIdx.x] = (double)m;
   }
 }
 
@@ -519,15 +527,21 @@ int flux_nblocks_tile(int d, int n, int tile) {
 
 int flux_nblocks(int d, int n) { return flux_nblocks_tile(d, n, 0); }
 
+template <typename T, int TC, int TR>
+static void launch_flux_tt(const FluxJob& job, dim3 g, dim3 b) {
+  if (job.d == 3) {
+    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<T, 3, DYN_GLIDE_ONLY, TC, TR, true>), g, b, 0, job.stream, job);
+    else hipLaunchKernelGGL((k_flux<T, 3, DYN_GLIDE_CLIMB, TC, TR, true>), g, b, 0, job.stream, job);
+  } else {
+    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<T, 2, DYN_GLIDE_ONLY, TC, TR, true>), g, b, 0, job.stream, job);
+    else hipLaunchKernelGGL((k_flux<T, 2, DYN_GLIDE_CLIMB, TC, TR, true>), g, b, 0, job.stream, job);
+  }
+}
+
 template <int TC, int TR>
 static void launch_flux_t(const FluxJob& job, dim3 g, dim3 b) {
-  if (job.d == 3) {
-    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_ONLY, TC, TR, true>), g, b, 0, job.stream, job);
-    else hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_CLIMB, TC, TR, true>), g, b, 0, job.stream, job);
-  } else {
-    if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_ONLY, TC, TR, true>), g, b, 0, job.stream, job);
-    else hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_CLIMB, TC, TR, true>), g, b, 0, job.stream, job);
-  }
+  if (job.fp32) launch_flux_tt<float, TC, TR>(job, g, b);
+  else launch_flux_tt<double, TC, TR>(job, g, b);
 }
 
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
@@ -609,20 +623,21 @@ hipError_t launch_advance(Clock* clk, hipStream_t st, int count) {
 
 // Elastic energy density e = 1/2 S_ijkm sigma_ij sigma_km
 // (PlasticityState.py:317-330), summed per block.
-__global__ void __launch_bounds__(256) k_energy(const double* G, const double* beta, long long cells, double mu,
-                                               double nu, double lam, double* partial) {
+template <typename T>
+__global__ void __launch_bounds__(256) k_energy(const T* G, const T* beta, long long cells, double mu, double nu,
+                                               double lam, double* partial) {
   __shared__ double red[4];
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double e = 0.0;
   if (idx < cells) {
     double b[9];
-    for (int c = 0; c < 9; ++c) b[c] = beta[c * cells + idx];
+    for (int c = 0; c < 9; ++c) b[c] = (double)beta[c * cells + idx];
     const double tr = b[0] + b[4] + b[8];
     double s[9];
     const int SC[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
     for (int c = 0; c < 9; ++c) {
       const int i = c / 3, j = c % 3;
-      s[c] = G[SC[c] * cells + idx] - mu * (b[c] + b[3 * j + i]) - (i == j ? lam * tr : 0.0);
+      s[c] = (double)G[SC[c] * cells + idx] - mu * (b[c] + b[3 * j + i]) - (i == j ? lam * tr : 0.0);
     }
     double ss = 0.0, sst = 0.0;
     for (int i = 0; i < 3; ++i)
@@ -658,10 +673,14 @@ __global__ void __launch_bounds__(1024) k_reduce_sum(const double* partial, int 
 }
 
 hipError_t launch_energy(const double* G, const double* beta, long long cells, double mu, double nu, double lam,
-                         double* partial, int* nblocks, hipStream_t st) {
+                         double* partial, int* nblocks, hipStream_t st, int fp32) {
   const int nb = (int)((cells + 255) / 256);
   *nblocks = nb;
-  hipLaunchKernelGGL(k_energy, dim3(nb), dim3(256), 0, st, G, beta, cells, mu, nu, lam, partial);
+  if (fp32)
+    hipLaunchKernelGGL((k_energy<float>), dim3(nb), dim3(256), 0, st, reinterpret_cast<const float*>(G),
+                       reinterpret_cast<const float*>(beta), cells, mu, nu, lam, partial);
+  else
+    hipLaunchKernelGGL((k_energy<double>), dim3(nb), dim3(256), 0, st, G, beta, cells, mu, nu, lam, partial);
   return hipGetLastError();
 }
 

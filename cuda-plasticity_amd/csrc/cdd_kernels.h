@@ -51,11 +51,12 @@ struct SpectralJob {
   long long sstride;     // spectrum (complex elements)
   long long ostride;     // out (elements)
   int trans;             // 1: transposed Green's contraction (adjoint), no load mode
+  int fp32;              // 1: src, spec, tw and out hold float data (src typed as a carrier)
 };
 
 hipError_t launch_spectral(const SpectralJob& job);
 hipError_t launch_sigma_finalize(const double* G, const double* beta, double* sig, long long cells, double mu,
-                                 double lam, hipStream_t st);
+                                 double lam, hipStream_t st, int fp32 = 0);
 
 enum Dyn { DYN_GLIDE_CLIMB = 0, DYN_GLIDE_ONLY = 1 };
 
@@ -78,6 +79,7 @@ struct FluxJob {
   int count;
   long long fstride, gstride, pstride;
   int tile;            // flux tile variant (0: default, 1: 64x4, 2: 16x16)
+  int fp32;            // 1: A, G, U, out, speed hold float data (partial stays double)
 };
 
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out);
@@ -87,7 +89,7 @@ hipError_t launch_finalize_dt(const double* partial, int nblocks, Clock* clk, do
 hipError_t launch_advance(Clock* clk, hipStream_t st, int count = 1);
 int flux_nblocks_tile(int d, int n, int tile);
 hipError_t launch_energy(const double* G, const double* beta, long long cells, double mu, double nu, double lam,
-                         double* partial, int* nblocks, hipStream_t st);
+                         double* partial, int* nblocks, hipStream_t st, int fp32 = 0);
 hipError_t launch_reduce_sum(const double* partial, int n, double* out, hipStream_t st, int accumulate = 0);
 
 struct VjpJob {

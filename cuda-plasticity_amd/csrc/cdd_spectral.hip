@@ -409,10 +409,19 @@ __global__ void k_sigma_finalize(const T* __restrict__ G, const T* __restrict__ 
 }
 
 // ------------------------------------------------------------- launchers
-template <int N>
-static hipError_t launch_spectral_n(const SpectralJob& job) {
-  using T = double;
+// The job's SrcField<double> carries the source pointers; for fp32 jobs they
+// hold float data and are reinterpreted here.
+template <typename T>
+static SrcField<T> src_as(const SrcField<double>& s) {
+  return SrcField<T>{reinterpret_cast<const T*>(s.A), reinterpret_cast<const T*>(s.B), s.clk, s.cs,
+                     reinterpret_cast<T*>(s.W), s.istride};
+}
+
+template <typename T, int N>
+static hipError_t launch_spectral_t(const SpectralJob& job) {
   using TL = Tiles<N>;
+  const SrcField<T> src = src_as<T>(job.src);
+  T* out = reinterpret_cast<T*>(job.out);
   const int d = job.d;
   const int NO = d == 3 ? N : 1;
   const int NC = job.nc;
@@ -430,9 +439,9 @@ static hipError_t launch_spectral_n(const SpectralJob& job) {
     if (job.hook) job.hook(job.hook_ctx, kind);
   };
   if (NC == 6)
-    hipLaunchKernelGGL((k_r2c_last<T, N, true>), g1, b1, 0, st, job.src, spec, tw, NO, NC, job.sstride);
+    hipLaunchKernelGGL((k_r2c_last<T, N, true>), g1, b1, 0, st, src, spec, tw, NO, NC, job.sstride);
   else
-    hipLaunchKernelGGL((k_r2c_last<T, N, false>), g1, b1, 0, st, job.src, spec, tw, NO, NC, job.sstride);
+    hipLaunchKernelGGL((k_r2c_last<T, N, false>), g1, b1, 0, st, src, spec, tw, NO, NC, job.sstride);
   hook(KK_R2C);
   if (d == 3) {
     dim3 g2(N / TL::TW, H, NC), b2(TL::TW * TL::TT);
@@ -463,10 +472,15 @@ static hipError_t launch_spectral_n(const SpectralJob& job) {
     hipLaunchKernelGGL((k_fft_cols<T, N, true>), g2, b2, 0, st, spec, tw, job.gp.clk);
     hook(KK_COLS_INV);
   }
-  hipLaunchKernelGGL((k_c2r_last<T, N>), g1, b1, 0, st, spec, job.out, cs, tw, NO, NC, job.gp.clk, job.sstride,
+  hipLaunchKernelGGL((k_c2r_last<T, N>), g1, b1, 0, st, spec, out, cs, tw, NO, NC, job.gp.clk, job.sstride,
                      job.ostride);
   hook(KK_C2R);
   return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_spectral_n(const SpectralJob& job) {
+  return job.fp32 ? launch_spectral_t<float, N>(job) : launch_spectral_t<double, N>(job);
 }
 
 hipError_t launch_spectral(const SpectralJob& job) {
@@ -486,10 +500,15 @@ hipError_t launch_spectral(const SpectralJob& job) {
 }
 
 hipError_t launch_sigma_finalize(const double* G, const double* beta, double* sig, long long cells, double mu,
-                                 double lam, hipStream_t st) {
+                                 double lam, hipStream_t st, int fp32) {
   const int bs = 256;
-  hipLaunchKernelGGL((k_sigma_finalize<double>), dim3((unsigned)((cells + bs - 1) / bs)), dim3(bs), 0, st, G, beta,
-                     sig, cells, mu, lam);
+  const dim3 g((unsigned)((cells + bs - 1) / bs));
+  if (fp32)
+    hipLaunchKernelGGL((k_sigma_finalize<float>), g, dim3(bs), 0, st, reinterpret_cast<const float*>(G),
+                       reinterpret_cast<const float*>(beta), reinterpret_cast<float*>(sig), cells, (float)mu,
+                       (float)lam);
+  else
+    hipLaunchKernelGGL((k_sigma_finalize<double>), g, dim3(bs), 0, st, G, beta, sig, cells, mu, lam);
   return hipGetLastError();
 }
 

@@ -47,6 +47,13 @@ enum {
 
 enum { CDD_GLIDE_CLIMB = 0, CDD_GLIDE_ONLY = 1 };
 
+/* Arithmetic and storage type of a plan's fields.  The reference computes in
+ * float64 only (plasticity.h DOUBLE, numpy float64); CDD_FP32 is an added
+ * variant whose error is stated against the float64 run.  Field buffers
+ * (void* below) hold float64, or float32 for a CDD_FP32 plan; the adjoint
+ * and batched entry points are float64 only. */
+enum { CDD_FP64 = 0, CDD_FP32 = 1 };
+
 enum {
   CDD_LOAD_NONE = 0,
   CDD_LOAD_STRAIN = 1,   /* RunPython.py:21-26,44-49 ExternalStrain, strain(t) = init + rate (t - t0) */
@@ -70,6 +77,7 @@ typedef struct cdd_config {
   double load_t0;
   double load_init_strain;
   double load_uniaxial_rate;
+  int precision;     /* CDD_FP64 (default, 0) or CDD_FP32 */
 } cdd_config;
 
 typedef struct cdd_plan cdd_plan;
@@ -87,8 +95,8 @@ int cdd_set_stream(cdd_plan* plan, void* stream);
 /* State transfer (replaces the .ics write / .plas read, RunCUDA.py:9-22 and
  * plasticity.cu:139-206,268-270).  on_device: 1 if the pointer is a device
  * pointer, 0 for host memory. */
-int cdd_set_state(cdd_plan* plan, const double* beta, int on_device);
-int cdd_get_state(cdd_plan* plan, double* beta, int on_device);
+int cdd_set_state(cdd_plan* plan, const void* beta, int on_device);
+int cdd_get_state(cdd_plan* plan, void* beta, int on_device);
 
 /* Clock: time at the start of the next step and Run's endTime
  * (FieldMover.py:20-36).  fixed_dt > 0 selects CalculateOneFixedTimeStep. */
@@ -98,16 +106,16 @@ int cdd_get_clock(cdd_plan* plan, double* t, double* last_dt, int* done, long lo
 /* sigma = PlasticityState.CalculateSigma(type='k', source='betaE') of beta
  * (PlasticityState.py:85-127,357-390) plus the plan's load at time t (the
  * GetSigma override, RunPython.py:44-49).  beta_dev, sigma_dev: (9, N^d). */
-int cdd_sigma(cdd_plan* plan, const double* beta_dev, double t, double* sigma_dev);
+int cdd_sigma(cdd_plan* plan, const void* beta_dev, double t, void* sigma_dev);
 
 /* rho = PlasticityState.CalculateRhoFourier(type='k') (PlasticityState.py:264-304). */
-int cdd_rho(cdd_plan* plan, const double* beta_dev, double* rho_dev);
+int cdd_rho(cdd_plan* plan, const void* beta_dev, void* rho_dev);
 
 /* rhs (9, N^d) and the CFL speed field a_tot + b_tot (+ c_tot) (N^d,
  * nullable) of BetaPDynamics.CalculateFlux(t, state, CFLCondition=True)
  * (CentralUpwindHJBetaPDynamics3D.py:214-221 / GlideOnly :255-262 ->
  * CentralUpwindHJ3D.py:63-271). */
-int cdd_flux(cdd_plan* plan, const double* beta_dev, double t, double* rhs_dev, double* speed_dev);
+int cdd_flux(cdd_plan* plan, const void* beta_dev, double t, void* rhs_dev, void* speed_dev);
 
 /* Advance the plan's state by nsteps TVD-RK steps
  * (TVDRungeKutta_FieldMover.CalculateOneAdaptiveTimeStep, FieldMover.py:61-141;
@@ -117,11 +125,11 @@ int cdd_step(cdd_plan* plan, int nsteps);
 
 /* Total elastic energy sum_cells 1/2 S sigma sigma of beta (nullable: the
  * plan's state) at time t (PlasticityState.py:317-330, Observer.py:86). */
-int cdd_energy(cdd_plan* plan, const double* beta_dev, double t, double* energy);
+int cdd_energy(cdd_plan* plan, const void* beta_dev, double t, double* energy);
 
 /* Device pointer of the plan's current state (9, N^d) after the last
  * synchronising call. */
-int cdd_state_ptr(cdd_plan* plan, double** beta_dev);
+int cdd_state_ptr(cdd_plan* plan, void** beta_dev);
 
 /* HIP-event timing of the kernels cdd_step launches (the tracing the
  * reference does with cutil timers, plasticity.cu:169-170,227-228,320-350):

@@ -213,3 +213,34 @@ def test_config1_2d_128_lvp_rk2_100_steps(amd):
     ref, tref, _ = O.run(od, beta, 0.0, nsteps, 0.25, order=2)
     assert ns == nsteps and abs(t - tref) <= 1e-12 * tref
     assert rel_l2(got, ref) < 1e-11
+
+
+@pytest.mark.parametrize("d,n,dyn,Lambda", [(3, 32, 0, 0.9), (2, 64, 1, 0.0)])
+def test_fp32_variant_error_against_fp64(amd, d, n, dyn, Lambda):
+    """The float32 plan against the float64 plan on the same field: stress,
+    flux and 20 RK3 steps, errors within a few hundred float32 ulps
+    (relative L2; float32 eps = 1.19e-7)."""
+    import torch
+    say("fp32", d, n, dyn, Lambda)
+    beta = SpectralGaussianField((n,) * d, 3.0, 0.01, seed=18)
+    p64 = _plan(amd, d, n, dyn, Lambda, rk=3)
+    p32 = _plan(amd, d, n, dyn, Lambda, rk=3, precision="fp32")
+    s64 = p64.sigma(beta).cpu().numpy()
+    s32 = p32.sigma(beta).cpu().numpy()
+    assert p32.get_state().dtype == torch.float32
+    assert rel_l2(s32, s64) < 1e-5
+    r64, v64 = p64.flux(beta)
+    r32, v32 = p32.flux(beta)
+    assert rel_l2(r32.cpu().numpy(), r64.cpu().numpy()) < 1e-4
+    assert rel_l2(v32.cpu().numpy(), v64.cpu().numpy()) < 1e-5
+    for p in (p64, p32):
+        p.set_state(beta)
+        p.set_time(0.0)
+        p.step(20)
+    e = rel_l2(p32.get_state().cpu().numpy(), p64.get_state().cpu().numpy())
+    say("  fp32 vs fp64 after 20 steps: %.2e" % e)
+    assert e < 1e-4
+    t64, t32 = p64.clock()[0], p32.clock()[0]
+    assert abs(t32 - t64) < 1e-4 * t64
+    with pytest.raises(amd.CDDError):
+        p32.flux_vjp(beta, 0.0, beta)
