@@ -68,6 +68,29 @@ def cpu_baseline(seconds_hint=15.0):
                       "single-threaded" % dt}
 
 
+def cpu_baseline_openmp():
+    """The C/OpenMP restatement of the oracle on every host thread
+    (OMP_NUM_THREADS, else os.cpu_count()): 3D 128^3, 2 RK3 steps -- the
+    stronger CPU baseline, reported beside the single-core one."""
+    from oracle import cdd_oracle as O
+    from oracle import cdd_oracle_c as OC
+    from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
+    if not os.path.exists(OC.LIB):
+        return None
+    n, steps = 128, 2
+    b = SpectralGaussianField((n,) * 3, 6.0, 0.01, seed=0)
+    dyn = O.Dynamics(O.GLIDE_CLIMB, 0.9, 1.0, 0.3,
+                     O.Load("uniaxial", init_strain=float(b[8].mean()), uniaxial_rate=0.01))
+    OC.run(dyn, b, 0.0, 1, 0.25)  # warm-up (thread pool, twiddles)
+    t0 = time.perf_counter()
+    OC.run(dyn, b, 0.0, steps, 0.25)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": "C/OpenMP oracle, 3D 128^3 glide+climb Lambda=0.9 uniaxial load, 2 TVD-RK3 steps "
+                      "(%.1f s) on %d threads" % (dt, cores)}
+
+
 def bench_batched(args):
     """48 independent 2D realisations, sides drawn from {256,384,512,768,1024},
     per-instance seeds, L ~ U[3, 8] cells, RMS 0.01, glide-only ('lvp'),
@@ -375,6 +398,9 @@ def main():
         out["fp32_rel_l2_vs_fp64"] = fp32_err
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
+        omp = cpu_baseline_openmp()
+        if omp is not None:
+            out["cpu_baseline_openmp"] = omp
     print(json.dumps(out))
 
 
