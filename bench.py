@@ -91,6 +91,103 @@ def cpu_baseline_openmp():
                       "(%.1f s) on %d threads" % (dt, cores)}
 
 
+def bench_loading(args):
+    """3D n^3 (default 256) glide+climb, climb mobility 0.1, under the uniaxial
+    strain-rate control ramped to a strain of 0.01 over 1000 TVD-RK3 steps:
+    the forward run with its per-step stress-strain record, then the
+    checkpointed adjoint of the final mean stress sigma_zz with respect to the
+    beta^P that starts the last 100 steps (back through those 100 steps,
+    checkpoints every 10)."""
+    import numpy as np
+    import torch
+    from Plasticity.amd import lib as L
+    from Plasticity.amd.autograd import mean_stress_window
+    from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
+    n, total, window, ck = args.n, 1000, 100, 10
+    cells = n ** 3
+    dev = torch.device("cuda", 0)
+    beta = SpectralGaussianField((n,) * 3, 6.0, 0.01, seed=0)
+    init_strain = float(beta[8].mean())
+    beta_t = torch.from_numpy(beta).to(dev)
+    del beta
+    probe = L.Plan(3, n, dynamics=L.GLIDE_CLIMB, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25)
+    _, speed = probe.flux(beta_t)
+    dt0 = 0.25 / (n * float(speed.max()))
+    probe.close()
+    del speed
+    rate = 0.01 / (total * dt0)
+    load = dict(load_kind=L.LOAD_UNIAXIAL, load_init_strain=init_strain, load_uniaxial_rate=rate)
+    plan = L.Plan(3, n, dynamics=L.GLIDE_CLIMB, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25, **load)
+    plan.set_state(beta_t)
+    # fixed steps of the initial CFL step (CalculateOneFixedTimeStep), so the
+    # ramp reaches strain 0.01 exactly at step 1000; the speeds only fall
+    # during the run, so every step stays within the CFL bound
+    plan.set_time(0.0, float("inf"), dt0)
+    slog = torch.zeros((total, 8), dtype=torch.float64, device=dev)
+    plan.stress_record(slog)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.step(total - window)
+    torch.cuda.synchronize()
+    t_mid = plan.clock()[0]
+    beta_mid = plan.get_state()  # the state that starts the last 100 steps
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    plan.step(window)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    plan.stress_record(None)
+    t_end = plan.clock()[0]
+    fwd_s = (t1 - t0) + (t2 - t1)
+    rec = slog.cpu().numpy()
+    print("forward %d steps: %.2f s" % (total, fwd_s), file=sys.stderr, flush=True)
+    curve = [[float(rate * rec[k, 0]), float(rec[k, 3])] for k in range(0, total, 100)]
+    final_state = plan.get_state()
+    plan.close()
+    # adjoint over the last window steps
+    b = beta_mid.clone().requires_grad_(True)
+    lt = torch.tensor(0.9, dtype=torch.float64, device=dev, requires_grad=True)
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    J = mean_stress_window(b, lt, window, 1, "zz", t0=t_mid, checkpoint_every=ck, rk_order=3, load=load,
+                           fixed_dt=dt0)
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    J.backward()
+    torch.cuda.synchronize()
+    tc = time.perf_counter()
+    print("adjoint: forward %.2f s, backward %.2f s" % (tb - ta, tc - tb), file=sys.stderr, flush=True)
+    g = b.grad
+    out = {
+        "metric": "config 5: 3D %d^3 loading ramp, %d RK3 steps fp64 + adjoint of the final stress over the last %d"
+                  % (n, total, window),
+        "value": cells * total / fwd_s,
+        "unit": "cell-updates/s (forward)",
+        "n_gpus": 1, "steps": total, "warmup": 0,
+        "ms_per_step": 1e3 * fwd_s / total,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded Gaussian random beta^P (L=6 cells, RMS 0.01)",
+        "config": {"workload": "cdd-3d-%d-loading" % n, "grid": [n, n, n], "dynamics": "glide+climb",
+                   "Lambda": 0.9, "rk_order": 3, "cfl": 0.25,
+                   "load": "uniaxial strain-rate control, rate %.4g: strain 0.01 at step %d (fixed dt %.4g)"
+                           % (rate, total, dt0),
+                   "adjoint_window": window, "checkpoint_every": ck},
+        "stress_strain_every_100_steps": curve,
+        "final_strain": rate * t_end,
+        "final_mean_sigma_zz": float(J.item()),
+        "adjoint": {
+            "objective": "mean sigma_zz (load included) of the final state",
+            "wrt": "beta^P at the start of the last %d steps, and Lambda" % window,
+            "forward_s": tb - ta, "backward_s": tc - tb,
+            "cost_vs_forward": (tc - ta) / (tb - ta),
+            "grad_l2": float(torch.linalg.norm(g)), "dJ_dLambda": float(lt.grad.item()),
+            "grad_mean_zz": float(g[8].mean()),
+        },
+        "state_finite": bool(torch.isfinite(final_state).all()),
+    }
+    print(json.dumps(out))
+
+
 def bench_batched(args):
     """48 independent 2D realisations, sides drawn from {256,384,512,768,1024},
     per-instance seeds, L ~ U[3, 8] cells, RMS 0.01, glide-only ('lvp'),
@@ -262,7 +359,7 @@ def main():
     ap.add_argument("--fd-dirs", type=int, default=32, help="adjoint workload: finite-difference directions")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
                     help="3d workload: arithmetic of the plan (fp32 also reports its error against fp64)")
-    ap.add_argument("--workload", default="3d", choices=["3d", "batched", "adjoint"],
+    ap.add_argument("--workload", default="3d", choices=["3d", "batched", "adjoint", "loading"],
                     help="3d: the headline 256^3 stepper (default); batched: 48 uneven 2D realisations, "
                          "batched hipGraph step vs a serial loop of plain plans")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -285,6 +382,8 @@ def main():
         return bench_batched(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "loading":
+        return bench_loading(args)
 
     import numpy as np
     import torch

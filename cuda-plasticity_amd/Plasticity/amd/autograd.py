@@ -34,7 +34,7 @@ class _Run(torch.autograd.Function):
                      opts.get("load"), beta0.device.index or 0)
         log = torch.zeros(nsteps, dtype=torch.float64, device=beta0.device)
         plan.set_state(beta0.detach())
-        plan.set_time(float(t0))
+        plan.set_time(float(t0), float("inf"), float(opts.get("fixed_dt", 0.0)))
         plan.dt_record(log)
         checkpoints = [beta0.detach().clone()]
         done_steps = 0
@@ -72,13 +72,14 @@ class _Run(torch.autograd.Function):
 
 
 def run_with_grad(beta0, Lambda, nsteps, t0=0.0, checkpoint_every=20, dynamics=L.GLIDE_CLIMB, rk_order=2, mu=1.0,
-                  nu=0.3, cfl=0.25, load=None):
+                  nu=0.3, cfl=0.25, load=None, fixed_dt=0.0):
     """Final state after nsteps steps; differentiable w.r.t. beta0 and Lambda
     (a 0-dim float64 tensor).  load: optional dict of Plan load_* arguments
-    (every load kind, the uniaxial strain-rate control included)."""
+    (every load kind, the uniaxial strain-rate control included).  fixed_dt > 0:
+    fixed time steps (CalculateOneFixedTimeStep) instead of the CFL rule."""
     if not torch.is_tensor(Lambda):
         Lambda = torch.tensor(float(Lambda), dtype=torch.float64, device=beta0.device)
-    opts = dict(dynamics=dynamics, rk_order=rk_order, mu=mu, nu=nu, cfl=cfl, load=load)
+    opts = dict(dynamics=dynamics, rk_order=rk_order, mu=mu, nu=nu, cfl=cfl, load=load, fixed_dt=fixed_dt)
     return _Run.apply(beta0, Lambda, int(nsteps), float(t0), int(checkpoint_every), opts)
 
 
@@ -162,7 +163,7 @@ class _StressWindow(torch.autograd.Function):
         log = torch.zeros(nsteps, dtype=torch.float64, device=beta0.device)
         slog = torch.zeros((nsteps, 8), dtype=torch.float64, device=beta0.device)
         plan.set_state(beta0.detach())
-        plan.set_time(float(t0))
+        plan.set_time(float(t0), float("inf"), float(opts.get("fixed_dt", 0.0)))
         plan.dt_record(log)
         plan.stress_record(slog)
         checkpoints = [beta0.detach().clone()]
@@ -215,7 +216,7 @@ class _StressWindow(torch.autograd.Function):
 
 
 def mean_stress_window(beta0, Lambda, nsteps, window, comp="zz", t0=0.0, checkpoint_every=20,
-                       dynamics=L.GLIDE_CLIMB, rk_order=3, mu=1.0, nu=0.3, cfl=0.25, load=None):
+                       dynamics=L.GLIDE_CLIMB, rk_order=3, mu=1.0, nu=0.3, cfl=0.25, load=None, fixed_dt=0.0):
     """Stress-strain objective: the mean stress component `comp` (external load
     included, as PlottingTools.StrainStress reports it, PlottingTools.py:69-100)
     averaged over the states after each of the last `window` of nsteps steps;
@@ -226,7 +227,7 @@ def mean_stress_window(beta0, Lambda, nsteps, window, comp="zz", t0=0.0, checkpo
         raise ValueError("comp must be one of %s" % sorted(_COMPS))
     if not torch.is_tensor(Lambda):
         Lambda = torch.tensor(float(Lambda), dtype=torch.float64, device=beta0.device)
-    opts = dict(dynamics=dynamics, rk_order=rk_order, mu=mu, nu=nu, cfl=cfl, load=load)
+    opts = dict(dynamics=dynamics, rk_order=rk_order, mu=mu, nu=nu, cfl=cfl, load=load, fixed_dt=fixed_dt)
     return _StressWindow.apply(beta0, Lambda, int(nsteps), int(window), comp, float(t0), int(checkpoint_every), opts)
 
 
