@@ -389,25 +389,69 @@ __device__ __forceinline__ long long adj_tile_cell(int n, int co[3], long long s
   return (long long)row * n + col;
 }
 
-template <typename T, int D, int DYN>
-__global__ void __launch_bounds__(256) k_flux_vjp_cell(VjpJob job) {
+// Block = TX x TY cells of the two fastest axes (one plane in 3D), the tile
+// order of adj_tile_cell.  The plane tile of all nine components with its
+// 2-cell halo is staged in LDS with one batch of coalesced loads; in 3D each
+// thread's four slow-axis neighbours come straight from global memory.
+template <typename T, int D, int DYN, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
+  constexpr int PW = TX + 4, TILE = PW * (TY + 4), NT = TX * TY, FILL = (TILE + NT - 1) / NT;
   __shared__ T red[4];
+  __shared__ T tile[9][TILE];
   const int n = job.n;
   const long long cells = job.cells;
   int co[3];
   long long str[3];
   const long long ii = adj_tile_cell<D>(n, co, str);
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+  // tile origin (contiguous axis = last)
+  const int col0 = co[D - 1] - tx, row0 = co[D - 2] - ty;
+  const long long pbase = (D == 3) ? (long long)co[0] * n * n : 0;
   const T N = T(n);
+  T xn[D == 3 ? 9 : 1][4];
+  if constexpr (D == 3) {
+    long long xo[4];
+    for (int k = 0; k < 4; ++k) xo[k] = ii + (long long)(wrapi(co[0] + (k < 2 ? k - 2 : k - 1), n) - co[0]) * str[0];
+    for (int c = 0; c < 9; ++c)
+      for (int k = 0; k < 4; ++k) xn[c][k] = job.beta[c * cells + xo[k]];
+  }
+  {
+    long long foff[FILL];
+    bool fval[FILL];
+    for (int k = 0; k < FILL; ++k) {
+      const int e = tid + k * NT;
+      fval[k] = e < TILE;
+      const int er = e / PW, ec = e % PW;
+      foff[k] = pbase + (long long)wrapi(row0 + er - 2, n) * n + wrapi(col0 + ec - 2, n);
+    }
+    T tf[9][FILL];
+    for (int c = 0; c < 9; ++c)
+      for (int k = 0; k < FILL; ++k) tf[c][k] = fval[k] ? job.beta[c * cells + foff[k]] : T(0);
+    for (int c = 0; c < 9; ++c)
+      for (int k = 0; k < FILL; ++k)
+        if (fval[k]) tile[c][tid + k * NT] = tf[c][k];
+  }
+  __syncthreads();
   T u[D][2][9], fc[9];
   // minmod selection of this cell's own limiter site m_i, per axis and
   // component: 2 bits at 2 (9 a + c), 3 = the zero case
   unsigned long long code = 0ull;
+  const int lr = ty + 2, lc = tx + 2;
   for (int a = 0; a < D; ++a) {
-    long long off[5];
-    for (int k = 0; k < 5; ++k) off[k] = ii + (long long)(wrapi(co[a] + k - 2, n) - co[a]) * str[a];
     for (int c = 0; c < 9; ++c) {
+      const T* tb = tile[c];
       T f[5];
-      for (int k = 0; k < 5; ++k) f[k] = job.beta[c * cells + off[k]];
+      if (a == D - 1) {
+        for (int k = 0; k < 5; ++k) f[k] = tb[lr * PW + lc + k - 2];
+      } else if (a == D - 2) {
+        for (int k = 0; k < 5; ++k) f[k] = tb[(lr + k - 2) * PW + lc];
+      } else {
+        f[0] = xn[c][0];
+        f[1] = xn[c][1];
+        f[2] = tb[lr * PW + lc];
+        f[3] = xn[c][2];
+        f[4] = xn[c][3 % 4];
+      }
       if (a == 0) fc[c] = f[2];
       lderivs<T>(f, N, u[a][0][c], u[a][1][c]);
       const T Dm1 = f[2] - f[1], D0 = f[3] - f[2], D1 = f[4] - f[3];
@@ -437,7 +481,6 @@ __global__ void __launch_bounds__(256) k_flux_vjp_cell(VjpJob job) {
       for (int c = 0; c < 9; ++c) job.ubar[((a * 2 + s) * 9 + c) * cells + ii] = N * ub[a][s][c];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) job.sbar[(3 * i + j) * cells + ii] = T(0.5) * (sgb[3 * i + j] + sgb[3 * j + i]);
-  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   const int nw = (int)(blockDim.x * blockDim.y + 63) / 64;
   for (int off = 32; off > 0; off >>= 1) lamb += __shfl_xor(lamb, off, 64);
   if ((tid & 63) == 0) red[tid >> 6] = lamb;
@@ -661,17 +704,26 @@ int flux_vjp_nblocks(int d, int n) {
   return (d == 3 ? n : 1) * (n / (int)b.x) * (n / (int)b.y);
 }
 
+template <int TX, int TY>
+static void launch_vjp_cell_t(const VjpJob& j, int nb, hipStream_t st) {
+  const dim3 bs(TX, TY);
+  if (j.d == 3) {
+    if (j.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux_vjp_cell<double, 3, DYN_GLIDE_ONLY, TX, TY>), dim3(nb), bs, 0, st, j);
+    else hipLaunchKernelGGL((k_flux_vjp_cell<double, 3, DYN_GLIDE_CLIMB, TX, TY>), dim3(nb), bs, 0, st, j);
+  } else {
+    if (j.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux_vjp_cell<double, 2, DYN_GLIDE_ONLY, TX, TY>), dim3(nb), bs, 0, st, j);
+    else hipLaunchKernelGGL((k_flux_vjp_cell<double, 2, DYN_GLIDE_CLIMB, TX, TY>), dim3(nb), bs, 0, st, j);
+  }
+}
+
 hipError_t launch_flux_vjp_cell(const VjpJob& j, int* nblocks, hipStream_t st) {
   const dim3 bs = adj_block(j.n);
   const int nb = flux_vjp_nblocks(j.d, j.n);
   *nblocks = nb;
-  if (j.d == 3) {
-    if (j.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux_vjp_cell<double, 3, DYN_GLIDE_ONLY>), dim3(nb), bs, 0, st, j);
-    else hipLaunchKernelGGL((k_flux_vjp_cell<double, 3, DYN_GLIDE_CLIMB>), dim3(nb), bs, 0, st, j);
-  } else {
-    if (j.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux_vjp_cell<double, 2, DYN_GLIDE_ONLY>), dim3(nb), bs, 0, st, j);
-    else hipLaunchKernelGGL((k_flux_vjp_cell<double, 2, DYN_GLIDE_CLIMB>), dim3(nb), bs, 0, st, j);
-  }
+  if (bs.x == 32 && bs.y == 8) launch_vjp_cell_t<32, 8>(j, nb, st);
+  else if (bs.x == 16 && bs.y == 8) launch_vjp_cell_t<16, 8>(j, nb, st);
+  else if (bs.x == 8 && bs.y == 8) launch_vjp_cell_t<8, 8>(j, nb, st);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
