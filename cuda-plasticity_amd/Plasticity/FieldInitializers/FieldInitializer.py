@@ -37,3 +37,54 @@ def SpectralGaussianField(gridShape, lengthscale, rms=0.01, seed=0):
         r = np.sqrt(np.mean(b * b))
         out[c] = b * (rms / r)
     return out
+
+
+def _reverse_axes(a):
+    """Every grid axis reversed: the component layout of the reference's .ics
+    and .plas files (Python axis 0 fastest)."""
+    return np.ascontiguousarray(a.transpose(tuple(reversed(range(a.ndim)))))
+
+
+def LoadStateRaw(filename, N, dim, time=None, dtype='float64', hastimes=True, device=None):
+    """Load (t, state) from a raw file of [t, 9 N^dim] records in the reversed
+    component layout (FieldInitializer.py:61-101): the last record, or the
+    first whose time is >= `time`; hastimes=False reads a bare .ics file
+    (RunCUDA.py:9-22) with t = 0.  Like the reference, the records are
+    transposed on load, so this reads the CUDA executable's .plas output
+    and .ics files (and not the untransposed RecordStateObserverRaw output,
+    for which LoadStateRawUntransposed is provided)."""
+    from Plasticity.PlasticityStates.PlasticityState import PlasticityState
+    t, data = _read_record(filename, N, dim, time, dtype, hastimes)
+    field = np.stack([_reverse_axes(data[c]) for c in range(9)]).astype(np.float64)
+    return t, PlasticityState(tuple([N] * dim), field=field, device=device)
+
+
+def LoadStateRawUntransposed(filename, N, dim, time=None, dtype='float64', device=None):
+    """(t, state) from a RecordStateObserverRaw file (Observer.py:51-69), which
+    stores the components in their own (NumPy) layout."""
+    from Plasticity.PlasticityStates.PlasticityState import PlasticityState
+    t, data = _read_record(filename, N, dim, time, dtype, True)
+    return t, PlasticityState(tuple([N] * dim), field=data.astype(np.float64), device=device)
+
+
+def _read_record(filename, N, dim, time, dtype, hastimes):
+    dtype = np.dtype(dtype)
+    es = dtype.itemsize
+    datasize = 9 * N ** dim
+    rec = es * (datasize + (1 if hastimes else 0))
+    with open(filename, 'rb') as f:
+        f.seek(0, 2)
+        cnt = f.tell() // rec
+        if cnt < 1:
+            raise ValueError("%s holds no complete record for N=%d, dim=%d" % (filename, N, dim))
+        k = cnt - 1
+        if hastimes and time is not None:
+            for i in range(cnt):
+                f.seek(i * rec)
+                if np.frombuffer(f.read(es), dtype=dtype)[0] >= time:
+                    k = i
+                    break
+        f.seek(k * rec)
+        t = float(np.frombuffer(f.read(es), dtype=dtype)[0]) if hastimes else 0.0
+        data = np.frombuffer(f.read(es * datasize), dtype=dtype).reshape((9,) + (N,) * dim)
+    return t, data

@@ -48,7 +48,7 @@ EXPORTS = [
     "cdd_batch_create", "cdd_batch_destroy", "cdd_batch_set_stream", "cdd_batch_size", "cdd_batch_set_state",
     "cdd_batch_get_state", "cdd_batch_set_time", "cdd_batch_get_clock", "cdd_batch_step", "cdd_batch_set_tile",
     "cdd_batch_time_flux", "cdd_flux_vjp", "cdd_step_vjp", "cdd_energy_grad", "cdd_dt_record", "cdd_dt_replay",
-    "cdd_segment_vjp", "cdd_lambda_bar", "cdd_stress_record",
+    "cdd_segment_vjp", "cdd_lambda_bar", "cdd_stress_record", "cdd_plas_append", "cdd_plas_resume",
 ]
 KERNEL_KINDS = ["r2c", "cols_fwd", "contract", "cols_inv", "c2r", "flux_stage1", "cfl_reduce",
                 "flux_stage2", "flux_stage3", "advance"]
@@ -86,6 +86,8 @@ def _load():
         "cdd_segment_vjp": ([P, P, D, P, I, P, P, P], I),
         "cdd_lambda_bar": ([P, dp, I], I),
         "cdd_stress_record": ([P, P, ctypes.c_longlong], I),
+        "cdd_plas_append": ([P, ctypes.c_char_p], I),
+        "cdd_plas_resume": ([P, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(I)], I),
         "cdd_batch_create": ([ctypes.POINTER(Config), ctypes.POINTER(I), I, ctypes.POINTER(P)], I),
         "cdd_batch_destroy": ([P], I),
         "cdd_batch_set_stream": ([P, P], I),
@@ -290,6 +292,17 @@ class Plan:
             check(lib.cdd_dt_record(self.h, None, 0))
         else:
             check(lib.cdd_dt_record(self.h, _ptr(log), log.numel()))
+
+    def plas_append(self, path):
+        """Append [t, state] to a .plas file in the reference's layout."""
+        check(lib.cdd_plas_append(self.h, os.fsencode(path)))
+
+    def plas_resume(self, path):
+        """Load the last complete record of a .plas file into the state and
+        clock; returns (t, complete records, torn tail)."""
+        t, n, torn = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_int()
+        check(lib.cdd_plas_resume(self.h, os.fsencode(path), ctypes.byref(t), ctypes.byref(n), ctypes.byref(torn)))
+        return t.value, n.value, bool(torn.value)
 
     def stress_record(self, log):
         """Append one stress-strain record per step to the float64 device tensor

@@ -20,7 +20,7 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-ffp-contract=fast", "-Wno-unused-result",
 ]
-SOURCES = ["cdd_spectral.hip", "cdd_flux.hip", "cdd_capi.hip", "cdd_batch.hip", "cdd_adjoint.hip"]
+SOURCES = ["cdd_spectral.hip", "cdd_flux.hip", "cdd_capi.hip", "cdd_batch.hip", "cdd_adjoint.hip", "cdd_io.hip"]
 
 
 def _compile(src, extra):

@@ -71,6 +71,8 @@ struct cdd_plan {
   size_t seg_bytes;
   double* lseg;
   size_t lseg_len;
+  // pinned host staging of one .plas record (allocated on first use)
+  void* host;
   // HIP-event timing of every kernel launched by cdd_step (optional)
   bool timing;
   std::vector<hipEvent_t>* ev;
@@ -250,6 +252,7 @@ int cdd_plan_destroy(cdd_plan* p) {
   (void)hipFree(p->upart);
   (void)hipFree(p->codes);
   (void)hipFree(p->seg);
+  if (p->host) (void)hipHostFree(p->host);
   (void)hipFree(p->lseg);
   if (p->own) (void)hipStreamDestroy(p->own);
   if (p->ev) {
@@ -758,6 +761,7 @@ static int seg_alloc(cdd_plan* p, int nsteps) {
   if (need > p->seg_bytes || lneed > p->lseg_len) HIPCHK(hipStreamSynchronize(p->stream));
   if (need > p->seg_bytes) {
     (void)hipFree(p->seg);
+  if (p->host) (void)hipHostFree(p->host);
     p->seg = nullptr;
     p->seg_bytes = 0;
     if (hipMalloc((void**)&p->seg, need) != hipSuccess)
@@ -836,6 +840,90 @@ int cdd_lambda_bar(cdd_plan* p, double* value, int reset) {
   }
   if (reset) HIPCHK(hipMemsetAsync(p->scalar + 3, 0, sizeof(double), p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
+  return CDD_OK;
+}
+
+// ------------------------------------------------------------- .plas files
+static int host_alloc(cdd_plan* p) {
+  if (p->host) return CDD_OK;
+  const size_t bytes = p->es * (1 + 9 * (size_t)p->cells);
+  if (hipHostMalloc(&p->host, bytes) != hipSuccess) return fail(CDD_ERR_HIP, "hipHostMalloc failed (record staging)");
+  return CDD_OK;
+}
+
+int cdd_plas_append(cdd_plan* p, const char* path) {
+  if (!p || !path) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  RCHK(host_alloc(p));
+  Clock c;
+  if (sync_clock(p, &c)) return CDD_ERR_HIP;
+  const size_t fb = p->es * 9 * (size_t)p->cells;
+  HIPCHK(launch_reverse_axes(p->state[cur_state(p)], p->work9, p->d, p->n, 9, p->fp32, p->stream));
+  char* h = static_cast<char*>(p->host);
+  HIPCHK(hipMemcpyAsync(h + p->es, p->work9, fb, hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  if (p->fp32) {
+    const float tf = (float)c.t;
+    memcpy(h, &tf, sizeof tf);
+  } else {
+    memcpy(h, &c.t, sizeof c.t);
+  }
+  FILE* f = fopen(path, "ab");
+  if (!f) return fail(CDD_ERR_ARG, (std::string("cannot open ") + path).c_str());
+  const size_t want = p->es + fb;
+  const size_t got = fwrite(h, 1, want, f);
+  const int fl = fflush(f);
+  const int cl = fclose(f);
+  if (got != want || fl != 0 || cl != 0) return fail(CDD_ERR_ARG, "short write to the .plas file");
+  return CDD_OK;
+}
+
+int cdd_plas_resume(cdd_plan* p, const char* path, double* t, long long* records, int* torn) {
+  if (!p || !path) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  RCHK(host_alloc(p));
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(CDD_ERR_ARG, (std::string("cannot open ") + path).c_str());
+  const size_t fb = p->es * 9 * (size_t)p->cells;
+  const long long rec = (long long)(p->es + fb);
+  if (fseeko(f, 0, SEEK_END) != 0) {
+    fclose(f);
+    return fail(CDD_ERR_ARG, "cannot seek in the .plas file");
+  }
+  const long long size = (long long)ftello(f);
+  const long long nrec = size / rec;
+  if (records) *records = nrec;
+  if (torn) *torn = (size % rec) != 0;
+  if (nrec < 1) {
+    fclose(f);
+    return fail(CDD_ERR_ARG, "no complete record in the .plas file (wrong grid size or precision?)");
+  }
+  // the last complete record; a torn tail (an interrupted append) is skipped
+  if (fseeko(f, (off_t)((nrec - 1) * rec), SEEK_SET) != 0) {
+    fclose(f);
+    return fail(CDD_ERR_ARG, "cannot seek in the .plas file");
+  }
+  char* h = static_cast<char*>(p->host);
+  const size_t got = fread(h, 1, (size_t)rec, f);
+  fclose(f);
+  if ((long long)got != rec) return fail(CDD_ERR_ARG, "short read from the .plas file");
+  double tr;
+  if (p->fp32) {
+    float tf;
+    memcpy(&tf, h, sizeof tf);
+    tr = tf;
+  } else {
+    memcpy(&tr, h, sizeof tr);
+  }
+  Clock c;
+  if (sync_clock(p, &c)) return CDD_ERR_HIP;
+  HIPCHK(hipMemcpyAsync(p->work9, h + p->es, fb, hipMemcpyHostToDevice, p->stream));
+  HIPCHK(launch_reverse_axes(p->work9, p->state[cur_state(p)], p->d, p->n, 9, p->fp32, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  c.t = tr;
+  c.done = (tr >= c.t_end) ? 1 : 0;
+  HIPCHK(hipMemcpy(p->clk, &c, sizeof c, hipMemcpyHostToDevice));
+  if (t) *t = tr;
   return CDD_OK;
 }
 

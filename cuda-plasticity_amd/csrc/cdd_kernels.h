@@ -91,6 +91,9 @@ int flux_nblocks_tile(int d, int n, int tile);
 hipError_t launch_energy(const double* G, const double* beta, long long cells, double mu, double nu, double lam,
                          double* partial, int* nblocks, hipStream_t st, int fp32 = 0);
 hipError_t launch_reduce_sum(const double* partial, int n, double* out, hipStream_t st, int accumulate = 0);
+// reverse all grid axes of ncomp components (NumPy layout <-> the reference's
+// transposed .ics/.plas layout); in and out must not alias
+hipError_t launch_reverse_axes(const void* in, void* out, int d, int n, int ncomp, int fp32, hipStream_t st);
 
 struct VjpJob {
   int d, n, dyn;

@@ -148,6 +148,19 @@ int cdd_state_ptr(cdd_plan* plan, void** beta_dev);
  * solve itself (its k = 0 mode): no extra pass, no host sync.  cap = 0 stops. */
 int cdd_stress_record(cdd_plan* plan, double* log_dev, long long cap);
 
+/* The reference's native output format (plasticity.cu:102-160,239,270;
+ * assist.h:101-192): a .plas file is a sequence of records [t, 9 N^d], each
+ * component stored with every grid axis reversed (the layout of
+ * RunCUDA.convertStateToCUDA, Plasticity/RunCUDA.py:9-22), in the plan's
+ * precision.  cdd_plas_append appends the current state and time (a
+ * synchronising call).  cdd_plas_resume replaces the CONTINUE_RUN restart
+ * (plasticity.cu:107-125): it loads the last complete record into the plan's
+ * state and clock; unlike the reference's seek-from-end it checks that the
+ * file holds whole records and skips a torn tail left by an interrupted
+ * append (*torn = 1; *records = complete records; both nullable). */
+int cdd_plas_append(cdd_plan* plan, const char* path);
+int cdd_plas_resume(cdd_plan* plan, const char* path, double* t, long long* records, int* torn);
+
 int cdd_timing_enable(cdd_plan* plan, int max_events);
 int cdd_timing_read(cdd_plan* plan, double* ms, int* counts, int nkinds);
 
