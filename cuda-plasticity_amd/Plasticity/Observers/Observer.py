@@ -56,3 +56,30 @@ class EnergyDownhillObserver(Observer):
         if self.verbose:
             print("%d/%d" % (self.countBad, self.countTotal))
         self.PreviousEnergy = energy
+
+
+class LargeKInstabilityObserver(Observer):
+    """The instability check of LargeKInstabilityChecking_FieldMover
+    (Plasticity/FieldMovers/LargeKInstabilityChecking_FieldMover.py:73-89):
+    |FFT(|rho|)| at the mode (N/2, ..., N/2), relative to its value at the
+    first update, must stay below `limit`; ``stop`` turns True (and the
+    offending state is kept in ``state``) once it does not.  Evaluated on the
+    GPU (cdd_instability)."""
+
+    def __init__(self, limit=1.e5):
+        self.limit = limit
+        self.initialChecking = None
+        self.stop = False
+        self.state = None
+
+    def Update(self, time, state):
+        from Plasticity.amd import plans
+        beta = state.GetOrderParameterField().data
+        p = plans.analysis_plan(state.gridShape, state.mu, state.nu, beta.device)
+        value = p.instability(beta)
+        if self.initialChecking is None:
+            self.initialChecking = value
+        elif not (value / self.initialChecking < self.limit):
+            self.stop = True
+            self.state = state
+        return value

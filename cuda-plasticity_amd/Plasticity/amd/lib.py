@@ -49,6 +49,7 @@ EXPORTS = [
     "cdd_batch_get_state", "cdd_batch_set_time", "cdd_batch_get_clock", "cdd_batch_step", "cdd_batch_set_tile",
     "cdd_batch_time_flux", "cdd_flux_vjp", "cdd_step_vjp", "cdd_energy_grad", "cdd_dt_record", "cdd_dt_replay",
     "cdd_segment_vjp", "cdd_lambda_bar", "cdd_stress_record", "cdd_plas_append", "cdd_plas_resume",
+    "cdd_energy_record", "cdd_status", "cdd_instability",
 ]
 KERNEL_KINDS = ["r2c", "cols_fwd", "contract", "cols_inv", "c2r", "flux_stage1", "cfl_reduce",
                 "flux_stage2", "flux_stage3", "advance"]
@@ -86,6 +87,9 @@ def _load():
         "cdd_segment_vjp": ([P, P, D, P, I, P, P, P], I),
         "cdd_lambda_bar": ([P, dp, I], I),
         "cdd_stress_record": ([P, P, ctypes.c_longlong], I),
+        "cdd_energy_record": ([P, P, ctypes.c_longlong], I),
+        "cdd_status": ([P, ctypes.POINTER(I)], I),
+        "cdd_instability": ([P, P, dp], I),
         "cdd_plas_append": ([P, ctypes.c_char_p], I),
         "cdd_plas_resume": ([P, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(I)], I),
         "cdd_batch_create": ([ctypes.POINTER(Config), ctypes.POINTER(I), I, ctypes.POINTER(P)], I),
@@ -303,6 +307,29 @@ class Plan:
         t, n, torn = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_int()
         check(lib.cdd_plas_resume(self.h, os.fsencode(path), ctypes.byref(t), ctypes.byref(n), ctypes.byref(torn)))
         return t.value, n.value, bool(torn.value)
+
+    def energy_record(self, log):
+        """Append the total elastic energy of every step's start state to the
+        float64 device tensor `log` (None stops)."""
+        if log is None:
+            check(lib.cdd_energy_record(self.h, None, 0))
+        else:
+            if log.dtype != self.torch.float64 or not log.is_contiguous() or log.device != self.dev:
+                raise CDDError("energy log must be a contiguous float64 device tensor")
+            check(lib.cdd_energy_record(self.h, _ptr(log), log.numel()))
+
+    def status(self):
+        """0, or 1 if a non-finite speed stopped the run on the device."""
+        v = ctypes.c_int()
+        check(lib.cdd_status(self.h, ctypes.byref(v)))
+        return v.value
+
+    def instability(self, beta=None):
+        """|FFT(|rho|)| at the (N/2, ..., N/2) mode of beta (default: the state)."""
+        v = ctypes.c_double()
+        b = _ptr(self._as_dev(beta)) if beta is not None else None
+        check(lib.cdd_instability(self.h, b, ctypes.byref(v)))
+        return v.value
 
     def stress_record(self, log):
         """Append one stress-strain record per step to the float64 device tensor

@@ -686,6 +686,10 @@ __global__ void k_clock_segment(Clock* clk, int mode, double t0, const double* d
     c.slog = nullptr;
     c.slog_base = 0;
     c.slog_cap = 0;
+    c.elog = nullptr;
+    c.elog_base = 0;
+    c.elog_cap = 0;
+    c.status = 0;
     clk[0] = c;
   } else {
     clk[0] = clk[1];

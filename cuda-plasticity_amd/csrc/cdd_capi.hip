@@ -73,6 +73,9 @@ struct cdd_plan {
   size_t lseg_len;
   // pinned host staging of one .plas record (allocated on first use)
   void* host;
+  // per-step energy record: k_energy partials of the step-start state
+  double* epart;
+  bool elog_on;
   // HIP-event timing of every kernel launched by cdd_step (optional)
   bool timing;
   std::vector<hipEvent_t>* ev;
@@ -253,6 +256,7 @@ int cdd_plan_destroy(cdd_plan* p) {
   (void)hipFree(p->codes);
   (void)hipFree(p->seg);
   if (p->host) (void)hipHostFree(p->host);
+  (void)hipFree(p->epart);
   (void)hipFree(p->lseg);
   if (p->own) (void)hipStreamDestroy(p->own);
   if (p->ev) {
@@ -310,6 +314,7 @@ int cdd_set_time(cdd_plan* p, double t, double t_end, double fixed_dt) {
   c.t_end = t_end;
   c.fixed_dt = fixed_dt;
   c.done = (t >= t_end) ? 1 : 0;
+  c.status = 0;
   HIPCHK(hipMemcpy(p->clk, &c, sizeof c, hipMemcpyHostToDevice));
   return CDD_OK;
 }
@@ -423,6 +428,11 @@ static int launch_step(cdd_plan* p, const StepBufs& b) {
     SpectralJob j1 = spectral_job(p, 6, u, nullptr, p->clk, 0.0, b.G[0]);
     j1.gp.log_stress = 1;
     HIPCHK(launch_spectral(j1));
+  }
+  if (p->elog_on) {  // energy record of the step-start state (loads included)
+    int nbe = 0;
+    HIPCHK(launch_energy(b.G[0], u, p->cells, p->gp.mu, p->gp.el.nu, p->gp.lam, p->epart, &nbe, p->stream, p->fp32));
+    HIPCHK(launch_energy_log(p->epart, nbe, p->clk, p->stream));
   }
   FluxJob f = flux_job(p);
   f.G = b.G[0];
@@ -762,6 +772,7 @@ static int seg_alloc(cdd_plan* p, int nsteps) {
   if (need > p->seg_bytes) {
     (void)hipFree(p->seg);
   if (p->host) (void)hipHostFree(p->host);
+  (void)hipFree(p->epart);
     p->seg = nullptr;
     p->seg_bytes = 0;
     if (hipMalloc((void**)&p->seg, need) != hipSuccess)
@@ -948,6 +959,50 @@ int cdd_stress_record(cdd_plan* p, double* log_dev, long long cap) {
   c.slog_base = c.nsteps;
   c.slog_cap = cap;
   HIPCHK(hipMemcpy(p->clk, &c, sizeof c, hipMemcpyHostToDevice));
+  return CDD_OK;
+}
+
+int cdd_energy_record(cdd_plan* p, double* log_dev, long long cap) {
+  if (!p || cap < 0) return fail(CDD_ERR_ARG, "bad argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  if (cap > 0 && !p->epart &&
+      hipMalloc((void**)&p->epart, sizeof(double) * (size_t)((p->cells + 255) / 256)) != hipSuccess)
+    return fail(CDD_ERR_HIP, "hipMalloc failed (energy partials)");
+  Clock c;
+  if (sync_clock(p, &c)) return CDD_ERR_HIP;
+  c.elog = cap > 0 ? log_dev : nullptr;
+  c.elog_base = c.nsteps;
+  c.elog_cap = cap;
+  HIPCHK(hipMemcpy(p->clk, &c, sizeof c, hipMemcpyHostToDevice));
+  p->elog_on = cap > 0;
+  return CDD_OK;
+}
+
+int cdd_status(cdd_plan* p, int* status) {
+  if (!p || !status) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  Clock c;
+  if (sync_clock(p, &c)) return CDD_ERR_HIP;
+  *status = c.status;
+  return CDD_OK;
+}
+
+int cdd_instability(cdd_plan* p, const void* beta_v, double* value) {
+  if (!p || !value) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  const double* beta = static_cast<const double*>(beta_v);
+  if (!beta) {
+    if (sync_clock(p, nullptr)) return CDD_ERR_HIP;
+    beta = p->state[cur_state(p)];
+  }
+  HIPCHK(launch_spectral(spectral_job(p, 9, beta, nullptr, nullptr, 0.0, p->work9)));
+  int nb = 0;
+  HIPCHK(launch_nyquist_mode(p->work9, p->d, p->n, p->cells, p->fp32, p->partial, &nb, p->stream));
+  HIPCHK(launch_reduce_sum(p->partial, nb, p->scalar + 6, p->stream, 0));
+  double v = 0.0;
+  HIPCHK(hipMemcpyAsync(&v, p->scalar + 6, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  *value = fabs(v);
   return CDD_OK;
 }
 

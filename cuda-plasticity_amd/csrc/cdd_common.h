@@ -54,8 +54,8 @@ struct Clock {
   double fixed_dt;  // > 0: fixed step (CalculateOneFixedTimeStep)
   double cfl;       // CFLsafeFactor
   double last_max;  // max speed seen by the last CFL reduction
-  int done;         // 1 once t >= t_end: every step kernel returns at once
-  int pad;
+  int done;         // 1 once t >= t_end (or on a failure): every step kernel returns at once
+  int status;       // 0 ok; 1 a non-finite speed stopped the run (the state is the last step's start)
   long long nsteps;  // completed steps
   // frozen time-step schedule (adjoint replay): dt = dt_sched[nsteps - sched_base]
   const double* dt_sched;
@@ -67,6 +67,9 @@ struct Clock {
   // xy xz yz with the load, mean beta_zz) at slog[8 (nsteps - slog_base)]
   double* slog;
   long long slog_base, slog_cap;
+  // optional energy log: total elastic energy of each step's start state
+  double* elog;
+  long long elog_base, elog_cap;
 };
 
 // Uniform load added to sigma (it enters the stress solve as the k = 0 mode).

@@ -488,7 +488,12 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   }
   if (speedp) speedp[idx] = speed;
   if (partialp) {
-    T m = block_max<T>(speed, red);
+    // a non-finite speed or rhs must reach the CFL reduction (fmax drops NaN,
+    // also inside the speeds, so the rhs is checked too)
+    bool fin = isfinite(speed);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) fin = fin && isfinite(rhs[c]);
+    T m = block_max<T>(fin ? speed : T(INFINITY), red);
     if (tid == 0) partialp[blockIdx.x] = (double)m;
   }
 }
@@ -575,13 +580,77 @@ __global__ void __launch_bounds__(1024) k_finalize_dt(const double* partial, int
   if (threadIdx.x == 0) {
     double mm = -INFINITY;
     for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) mm = fmax(mm, red[k]);
+    clk->last_max = mm;
+    if (!(mm < INFINITY)) {  // a non-finite speed: stop before the state is updated
+      clk->status = 1;
+      clk->done = 1;
+      clk->dt = 0.0;
+      return;
+    }
     double dt = clk->fixed_dt > 0.0 ? clk->fixed_dt : clk->cfl / (nmax * mm);
     const long long si = clk->nsteps - clk->sched_base;
     if (clk->dt_sched && si >= 0 && si < clk->sched_len) dt = clk->dt_sched[si];
     if (clk->t + dt > clk->t_end) dt = clk->t_end - clk->t;
     clk->dt = dt;
-    clk->last_max = mm;
   }
+}
+
+// Energy record of the step-start state: the per-block energy partials of
+// k_energy summed and appended to the clock's energy log.
+__global__ void __launch_bounds__(1024) k_energy_log(const double* partial, int nb, Clock* clk) {
+  if (clk->done || !clk->elog) return;
+  __shared__ double red[16];
+  double e = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) e += partial[k];
+  for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) t += red[k];
+    const long long li = clk->nsteps - clk->elog_base;
+    if (li >= 0 && li < clk->elog_cap) clk->elog[li] = t;
+  }
+}
+
+hipError_t launch_energy_log(const double* partial, int nb, Clock* clk, hipStream_t st) {
+  hipLaunchKernelGGL(k_energy_log, dim3(1), dim3(1024), 0, st, partial, nb, clk);
+  return hipGetLastError();
+}
+
+// Large-k instability measure: the mode (N/2, ..., N/2) of the FFT of |rho|,
+// which is sum over cells of (-1)^(i+j[+k]) |rho| (per-block partials).
+template <typename T>
+__global__ void __launch_bounds__(256) k_nyquist_mode(const T* rho, int d, int n, long long cells, double* partial) {
+  __shared__ double red[4];
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (idx < cells) {
+    double s = 0.0;
+    for (int c = 0; c < 9; ++c) {
+      const double r = (double)rho[c * cells + idx];
+      s += r * r;
+    }
+    int par = (int)(idx % n) + (int)((idx / n) % n);
+    if (d == 3) par += (int)(idx / ((long long)n * n));
+    v = (par & 1) ? -sqrt(s) : sqrt(s);
+  }
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+hipError_t launch_nyquist_mode(const void* rho, int d, int n, long long cells, int fp32, double* partial, int* nb,
+                               hipStream_t st) {
+  *nb = (int)((cells + 255) / 256);
+  if (fp32)
+    hipLaunchKernelGGL((k_nyquist_mode<float>), dim3(*nb), dim3(256), 0, st, static_cast<const float*>(rho), d, n,
+                       cells, partial);
+  else
+    hipLaunchKernelGGL((k_nyquist_mode<double>), dim3(*nb), dim3(256), 0, st, static_cast<const double*>(rho), d, n,
+                       cells, partial);
+  return hipGetLastError();
 }
 
 __global__ void k_fixed_dt(Clock* clk, int count) {

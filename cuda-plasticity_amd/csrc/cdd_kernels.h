@@ -91,6 +91,11 @@ int flux_nblocks_tile(int d, int n, int tile);
 hipError_t launch_energy(const double* G, const double* beta, long long cells, double mu, double nu, double lam,
                          double* partial, int* nblocks, hipStream_t st, int fp32 = 0);
 hipError_t launch_reduce_sum(const double* partial, int n, double* out, hipStream_t st, int accumulate = 0);
+// append the sum of the k_energy partials to the clock's energy log
+hipError_t launch_energy_log(const double* partial, int nb, Clock* clk, hipStream_t st);
+// per-block partials of sum over cells of (-1)^(i+j[+k]) |rho|
+hipError_t launch_nyquist_mode(const void* rho, int d, int n, long long cells, int fp32, double* partial, int* nb,
+                               hipStream_t st);
 // reverse all grid axes of ncomp components (NumPy layout <-> the reference's
 // transposed .ics/.plas layout); in and out must not alias
 hipError_t launch_reverse_axes(const void* in, void* out, int d, int n, int ncomp, int fp32, hipStream_t st);

@@ -148,6 +148,24 @@ int cdd_state_ptr(cdd_plan* plan, void** beta_dev);
  * solve itself (its k = 0 mode): no extra pass, no host sync.  cap = 0 stops. */
 int cdd_stress_record(cdd_plan* plan, double* log_dev, long long cap);
 
+/* Device-side observers and failure detectors (no per-step host sync):
+ *   cdd_energy_record: every step appends the total elastic energy of its
+ *     start state (loads included) to log_dev (cap doubles; 0 stops): the
+ *     series behind Observer.EnergyDownhillObserver (Observer.py:89-104) and
+ *     RunLoadingMPI's EnergyDownhillChecking (RunLoadingMPI.py:20-58).
+ *   cdd_status: 0, or 1 when a non-finite speed or right-hand side appeared
+ *     in the step's first stage; the run then stops on the device with the
+ *     state left at that step's start (a NaN guard the reference does not
+ *     have).
+ *   cdd_instability: |FFT(|rho|)| at the mode (N/2, ..., N/2) of beta
+ *     (nullable: the plan's state), the measure of
+ *     LargeKInstabilityChecking_FieldMover.CheckInstability
+ *     (Plasticity/FieldMovers/LargeKInstabilityChecking_FieldMover.py:73-89),
+ *     computed as sum over cells of (-1)^(i+j[+k]) |rho|. */
+int cdd_energy_record(cdd_plan* plan, double* log_dev, long long cap);
+int cdd_status(cdd_plan* plan, int* status);
+int cdd_instability(cdd_plan* plan, const void* beta_dev, double* value);
+
 /* The reference's native output format (plasticity.cu:102-160,239,270;
  * assist.h:101-192): a .plas file is a sequence of records [t, 9 N^d], each
  * component stored with every grid axis reversed (the layout of

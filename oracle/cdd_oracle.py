@@ -608,6 +608,19 @@ class Dynamics:
         return central_upwind_flux(beta, self.get_sigma(beta, t), self.dynamics, self.Lambda)
 
 
+def instability(beta):
+    """LargeKInstabilityChecking_FieldMover.CheckInstability's measure
+    (Plasticity/FieldMovers/LargeKInstabilityChecking_FieldMover.py:73-89):
+    rhoM = |rho| per cell (CalculateRhoFourier().modulus()), KrhoM =
+    fftn(rhoM), value = |KrhoM[N/2, ..., N/2]|.  Pinned by the closed form
+    sum over cells of (-1)^(i+j[+k]) rhoM."""
+    r = rho_fourier(beta)
+    rhoM = np.sqrt((r * r).sum(axis=0))
+    K = np.fft.fftn(rhoM)
+    mid = tuple(n // 2 for n in rhoM.shape)
+    return float(np.sqrt(K[mid].real ** 2 + K[mid].imag ** 2))
+
+
 def mean_stress(beta, t, mu=1.0, nu=0.3, load=None):
     """Mean stress of the state at time t, external load included: the
     stress-strain point of PlottingTools.StrainStress (PlottingTools.py:69-100:

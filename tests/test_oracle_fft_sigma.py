@@ -225,3 +225,20 @@ def test_mean_stress_is_elastic_response_to_mean_distortion(kind):
         sig = sig + bzz * (lam * np.eye(3) + 2 * mu * np.diag([0.0, 0.0, 1.0]))
     want = np.array([sig[0, 0], sig[1, 1], sig[2, 2], sig[0, 1], sig[0, 2], sig[1, 2]])
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("shape", [(16, 16), (8, 8, 8)])
+def test_instability_measure_is_alternating_sum(shape):
+    """The (N/2, ..., N/2) Fourier mode of |rho| equals the sum over cells of
+    (-1)^(i+j[+k]) |rho| (the exponent e^{-i pi (i+j+k)})."""
+    from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
+    beta = SpectralGaussianField(shape, 1.5, 0.01, seed=21)
+    r = O.rho_fourier(beta)
+    rhoM = np.sqrt((r * r).sum(axis=0))
+    sign = np.ones(shape)
+    for a, n in enumerate(shape):
+        idx = [1] * len(shape)
+        idx[a] = n
+        sign = sign * ((-1.0) ** np.arange(n)).reshape(idx)
+    want = abs((sign * rhoM).sum())
+    assert abs(O.instability(beta) - want) <= 1e-12 * want
