@@ -88,3 +88,99 @@ def _read_record(filename, N, dim, time, dtype, hastimes):
         t = float(np.frombuffer(f.read(es), dtype=dtype)[0]) if hastimes else 0.0
         data = np.frombuffer(f.read(es * datasize), dtype=dtype).reshape((9,) + (N,) * dim)
     return t, data
+
+
+# ---------------------------------------------------------------------------
+# The reference's own initialisers (host side, NumPy/SciPy), so runs can start
+# from the same fields as reference runs with the same seed.
+# ---------------------------------------------------------------------------
+def _ktools_kSq(gridShape):
+    """kSq of FourierSpaceTools (GridArray/FourierSpaceTools.py:25-53) on the
+    half spectrum: 2D scales both axes by gridShape[1], 3D each axis by its
+    own size; theta - 2 pi [theta > pi] keeps the Nyquist at +pi."""
+    def ident(a):
+        return a - 2 * np.pi * (a > np.pi).astype(float)
+    shape = tuple(gridShape)
+    half = shape[:-1] + (shape[-1] // 2 + 1,)
+    idx = np.meshgrid(*[np.arange(s) for s in half], indexing="ij")
+    if len(shape) == 2:
+        ks = [shape[1] * ident(2.0 * np.pi * idx[a] / float(shape[a])) for a in range(2)]
+    else:
+        ks = [shape[a] * ident(2.0 * np.pi * idx[a] / float(shape[a])) for a in range(len(shape))]
+    return sum(k * k for k in ks)
+
+
+def ResampleArray(arr, N, Dim=2):
+    """scipy.signal.resample along every axis (FieldInitializer.py:421-428)."""
+    from scipy import signal
+    out = np.asarray(arr)
+    for a in range(Dim):
+        out = signal.resample(out, N, axis=a)
+    return out
+
+
+def _state_like(gridShape, field, device=None):
+    from Plasticity.PlasticityStates.PlasticityState import PlasticityState
+    return PlasticityState(tuple(gridShape), field=np.ascontiguousarray(field, dtype=np.float64), device=device)
+
+
+def ResizeState(state, N, Dim=2, device=None):
+    """Resample every component to N^Dim (FieldInitializer.py:430-442)."""
+    f = state.GetOrderParameterField().numpy()
+    new = np.stack([ResampleArray(f[c], N, Dim) for c in range(9)])
+    return _state_like((N,) * Dim, new, device)
+
+
+def GaussianRandomInitializer(gridShape, sigma=0.2, seed=None, device=None):
+    """The reference's random initial state (FieldInitializer.py:153-209):
+    the fields are always generated on a 128^d grid -- per component in
+    sorted order, white noise numpy.random.normal(scale=128) from the legacy
+    global generator seeded with `seed` (0 when None), filtered in k-space
+    by (sqrt(pi) sigma)^d exp(-kSq sigma^2 / 4) -- and resampled to the
+    requested size, so a seed gives the same field at every resolution.
+    Only the plain PlasticityState branch is restated (no slip systems,
+    vacancies or smectic states)."""
+    d = len(gridShape)
+    g = (128,) * d
+    Ksq = _ktools_kSq(g)
+    rs = np.random.RandomState(0 if seed is None else seed)
+    field = np.empty((9,) + g)
+    for c in range(9):
+        temp = rs.normal(scale=g[0], size=g)
+        ktemp = np.fft.rfftn(temp) * (np.sqrt(np.pi) * sigma) ** d * np.exp(-Ksq * sigma ** 2 / 4.)
+        field[c] = np.real(np.fft.irfftn(ktemp, s=g, axes=tuple(range(d))))
+    if tuple(gridShape)[0] != 128:
+        field = np.stack([ResampleArray(field[c], gridShape[0], d) for c in range(9)])
+    return _state_like(tuple(gridShape), field, device)
+
+
+def RotateState_90degree(state, device=None):
+    """Rotate by 90 degrees about z (FieldInitializer.py:488-503): the
+    component map of R = [[0, 1, 0], [-1, 0, 0], [0, 0, 1]] followed by
+    scipy.ndimage.rotate(., 90, mode='wrap') of every component."""
+    from scipy import ndimage as ndi
+    b = state.GetOrderParameterField().numpy()
+    src = {0: (4, 1), 1: (3, -1), 2: (5, 1), 3: (1, -1), 4: (0, 1), 5: (2, -1), 6: (7, 1), 7: (6, -1), 8: (8, 1)}
+    out = np.stack([ndi.rotate(sgn * b[s], 90, mode='wrap') for s, sgn in (src[c] for c in range(9))])
+    return _state_like(state.gridShape, out, device)
+
+
+def zeropadding2D(N, filename):
+    """Double a raw 2D state file by zero-padding its spectrum
+    (FieldInitializer.py:321-334): reads 9 N^2 float64 values, writes the
+    2N^2 field to filename with str(N) replaced by str(2N)."""
+    a = np.fromfile(filename).reshape(9, N, N)
+    b = np.zeros((9, 2 * N, 2 * N), float)
+    h = N // 2
+    for i in range(9):
+        ka = np.fft.fft2(a[i])
+        kb = np.zeros((2 * N, 2 * N), complex)
+        kb[:h, :h] = ka[:h, :h]
+        kb[-h:, :h] = ka[-h:, :h]
+        kb[:h, -h:] = ka[:h, -h:]
+        kb[-h:, -h:] = ka[-h:, -h:]
+        b[i] = np.real(np.fft.ifft2(kb))
+    b *= 4.
+    out = filename.replace(str(N), str(2 * N))
+    b.tofile(out)
+    return out
