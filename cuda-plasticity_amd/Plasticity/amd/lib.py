@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libcdd.so")
 
-GLIDE_CLIMB, GLIDE_ONLY = 0, 1
+GLIDE_CLIMB, GLIDE_ONLY, GLIDE_CLIMB_WOJZZ = 0, 1, 2
 FP64, FP32 = 0, 1
 LOAD_NONE, LOAD_STRAIN, LOAD_STRESS, LOAD_UNIAXIAL, LOAD_TENSOR = 0, 1, 2, 3, 4
 
@@ -55,10 +55,33 @@ KERNEL_KINDS = ["r2c", "cols_fwd", "contract", "cols_inv", "c2r", "flux_stage1",
                 "flux_stage2", "flux_stage3", "advance"]
 
 
+def _hip_runtimes():
+    """Paths of every HIP runtime library mapped into this process."""
+    paths = set()
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                p = line.split()[-1]
+                if "libamdhip64" in os.path.basename(p):
+                    paths.add(os.path.realpath(p))
+    except OSError:
+        pass
+    return paths
+
+
 def _load():
     if not os.path.exists(LIBPATH):
         raise CDDError("libcdd.so is not built (%s); run __graft_entry__.build()" % LIBPATH)
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7).
+    # Loading torch first lets libcdd.so bind to that copy; loading libcdd.so
+    # first would map a second runtime from /opt/rocm, and the two then fail
+    # to share the device ("no ROCm-capable device" in whichever starts last).
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIBPATH)
+    rts = _hip_runtimes()
+    if len(rts) > 1:
+        raise CDDError("two HIP runtimes are mapped into this process (%s); import torch before "
+                       "loading libcdd.so" % ", ".join(sorted(rts)))
     P = ctypes.c_void_p
     D = ctypes.c_double
     I = ctypes.c_int

@@ -16,3 +16,11 @@ def test_library_exports_header_symbols():
     for n in sorted(names):
         assert hasattr(L, n), n
     assert names == set(EXPORTS)
+
+
+def test_single_hip_runtime():
+    """The binding maps exactly one HIP runtime (torch's): a second copy from
+    /opt/rocm makes the later of the two fail to find the device."""
+    from Plasticity.amd import lib
+    rts = lib._hip_runtimes()
+    assert len(rts) == 1, rts
