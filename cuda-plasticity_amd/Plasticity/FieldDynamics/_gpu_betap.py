@@ -45,7 +45,8 @@ class GpuBetaPDynamics(FieldDynamics):
 
     def plan_kwargs(self):
         self._check_overrides()
-        kw = dict(dynamics=self.dynamics_kind, Lambda=float(self.Lambda) if self.dynamics_kind == L.GLIDE_CLIMB else 0.0)
+        kw = dict(dynamics=self.dynamics_kind,
+                  Lambda=float(self.Lambda) if self.dynamics_kind != L.GLIDE_ONLY else 0.0)
         kw.update(self.load)
         return kw
 

@@ -196,6 +196,8 @@ int cdd_batch_create(const cdd_config* cfg, const int* sizes, int count, cdd_bat
   if (cfg->rk_order != 2 && cfg->rk_order != 3) return cdd_internal_fail(CDD_ERR_ARG, "rk_order must be 2 or 3");
   if (cfg->load_kind != CDD_LOAD_NONE) return cdd_internal_fail(CDD_ERR_UNSUPPORTED, "batched stepping has no load");
   if (cfg->precision != CDD_FP64) return cdd_internal_fail(CDD_ERR_UNSUPPORTED, "batched stepping is float64 only");
+  if (cfg->dynamics != CDD_GLIDE_CLIMB && cfg->dynamics != CDD_GLIDE_ONLY)
+    return cdd_internal_fail(CDD_ERR_UNSUPPORTED, "batched stepping supports glide+climb and glide-only");
   const int ok2[] = {16, 32, 64, 128, 256, 384, 512, 768, 1024};
   std::map<int, int> gidx;
   for (int i = 0; i < count; ++i) {

@@ -135,7 +135,10 @@ int cdd_plan_create(const cdd_config* cfg, cdd_plan** out) {
   if (!cfg || !out) return fail(CDD_ERR_ARG, "null argument");
   if (!valid_n(cfg->ndim, cfg->n)) return fail(CDD_ERR_UNSUPPORTED, "unsupported grid dimension/size");
   if (cfg->rk_order != 2 && cfg->rk_order != 3) return fail(CDD_ERR_ARG, "rk_order must be 2 or 3");
-  if (cfg->dynamics != CDD_GLIDE_CLIMB && cfg->dynamics != CDD_GLIDE_ONLY) return fail(CDD_ERR_ARG, "bad dynamics");
+  if (cfg->dynamics != CDD_GLIDE_CLIMB && cfg->dynamics != CDD_GLIDE_ONLY && cfg->dynamics != CDD_GLIDE_CLIMB_WOJZZ)
+    return fail(CDD_ERR_ARG, "bad dynamics");
+  if (cfg->dynamics == CDD_GLIDE_CLIMB_WOJZZ && cfg->ndim != 2)
+    return fail(CDD_ERR_UNSUPPORTED, "the woJzz dynamics is 2D only");
   if (cfg->load_kind < 0 || cfg->load_kind > 4) return fail(CDD_ERR_ARG, "bad load_kind");
   if (!(cfg->mu > 0) || !(cfg->nu < 0.5) || !(cfg->cfl > 0)) return fail(CDD_ERR_ARG, "bad mu/nu/cfl");
   if (cfg->precision != CDD_FP64 && cfg->precision != CDD_FP32) return fail(CDD_ERR_ARG, "bad precision");
@@ -528,6 +531,8 @@ int cdd_energy(cdd_plan* p, const void* beta_v, double t, double* energy) {
 // ------------------------------------------------------------- adjoint
 static int adj_alloc(cdd_plan* p) {
   if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "the adjoint is float64 only");
+  if (p->cfg.dynamics == CDD_GLIDE_CLIMB_WOJZZ)
+    return fail(CDD_ERR_UNSUPPORTED, "the adjoint supports the glide+climb and glide-only dynamics");
   if (p->adj) return CDD_OK;
   const size_t fb = sizeof(double) * 9 * (size_t)p->cells;
   if (hipMalloc((void**)&p->adj, 6 * fb) != hipSuccess || hipMalloc((void**)&p->ubar, 2 * p->d * fb) != hipSuccess ||

@@ -101,9 +101,13 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
     }
   }
   const T third = T(1.0 / 3.0);
-  const T st = (DYN == DYN_GLIDE_CLIMB) ? Lambda * (sg[0] + sg[4] + sg[8]) * third : T(0);
-  const T jfac = -Lambda * third;
-  const bool climb_sup = (DYN == DYN_GLIDE_CLIMB) && (Lambda != T(0));
+  // climb suppression: glide + climb uses Lambda tr(sigma)/3 and q; the woJzz
+  // variant (2D) Lambda tr(sigma)/2 and w' (CentralUpwindHJBetaPDynamics_woJzz.py:71-75)
+  const T st = (DYN == DYN_GLIDE_CLIMB) ? Lambda * (sg[0] + sg[4] + sg[8]) * third
+               : (DYN == DYN_WOJZZ)    ? (sg[0] + sg[4] + sg[8]) * T(0.5) * Lambda
+                                       : T(0);
+  const T jfac = (DYN == DYN_WOJZZ) ? -Lambda * T(0.5) : -Lambda * third;
+  const bool climb_sup = (DYN == DYN_GLIDE_CLIMB || DYN == DYN_WOJZZ) && (Lambda != T(0));
 
   T amax[3], amin[3];
 #pragma unroll
@@ -150,12 +154,24 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
       const T tra = (a < D) ? tr[a % D][sgn[a]] : T(0);
       q[a] = tra - R[a];
     }
+    // woJzz climb direction w' = (ux_yy - uy_xy, uy_xx - ux_yx, -(ux_zx + uy_zy))
+    T wz[3] = {T(0), T(0), T(0)};
+    if (DYN == DYN_WOJZZ) {
+      wz[0] = u[0][sgn[0]][4] - u[D - 1][sgn[D - 1]][1];
+      wz[1] = u[D - 1][sgn[D - 1]][0] - u[0][sgn[0]][3];
+      wz[2] = -(u[0][sgn[0]][6] + u[D - 1][sgn[D - 1]][7]);
+    }
     if (DYN == DYN_GLIDE_CLIMB) {
       if (climb_sup) {
 #pragma unroll
         for (int a = 0; a < D; ++a) v0[a] += st * tr[a][sgn[a]];
 #pragma unroll
         for (int l = 0; l < 3; ++l) v0[l] -= st * R[l];
+      }
+    } else if (DYN == DYN_WOJZZ) {
+      if (climb_sup) {
+#pragma unroll
+        for (int l = 0; l < 3; ++l) v0[l] += st * wz[l];
       }
     } else {
       // lvp projection (CentralUpwindHJBetaPGlideOnlyDynamics.py:170-195)
@@ -187,7 +203,9 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
     T vv[3];
 #pragma unroll
     for (int l = 0; l < 3; ++l) vv[l] = v0[l] * rinv;
-    const T J = climb_sup ? jfac * (vv[0] * q[0] + vv[1] * q[1] + vv[2] * q[2]) : T(0);
+    const T J = !climb_sup              ? T(0)
+                : (DYN == DYN_WOJZZ) ? jfac * (vv[0] * wz[0] + vv[1] * wz[1] + vv[2] * wz[2])
+                                     : jfac * (vv[0] * q[0] + vv[1] * q[1] + vv[2] * q[2]);
 #pragma unroll
     for (int l = 0; l < 3; ++l) vbuf[(4 * c + l) * vstride] = vv[l];
     vbuf[(4 * c + 3) * vstride] = J;
@@ -255,7 +273,8 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
 #pragma unroll
     for (int jj = 0; jj < 3; ++jj) {
       const int cc = 3 * i + jj;
-      T h = (i == jj) ? Jsum : T(0);
+      // the climb term joins H_xx, H_yy, H_zz (woJzz: H_xx, H_yy only)
+      T h = (i == jj && (DYN != DYN_WOJZZ || i < 2)) ? Jsum : T(0);
 #pragma unroll
       for (int a = 0; a < D; ++a)
 #pragma unroll
@@ -269,7 +288,7 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
       T r = -h * inv;
 #pragma unroll
       for (int a = 0; a < D; ++a) r += (u[a][0][cc] - u[a][1][cc]) * tw[a];
-      rhs[cc] = r;
+      rhs[cc] = (DYN == DYN_WOJZZ && cc == 8) ? T(0) : r;  // woJzz holds beta_zz fixed
     }
   }
 }
@@ -534,6 +553,10 @@ int flux_nblocks(int d, int n) { return flux_nblocks_tile(d, n, 0); }
 
 template <typename T, int TC, int TR>
 static void launch_flux_tt(const FluxJob& job, dim3 g, dim3 b) {
+  if (job.dyn == DYN_WOJZZ) {  // 2D only (checked at plan creation)
+    hipLaunchKernelGGL((k_flux<T, 2, DYN_WOJZZ, TC, TR, true>), g, b, 0, job.stream, job);
+    return;
+  }
   if (job.d == 3) {
     if (job.dyn == DYN_GLIDE_ONLY) hipLaunchKernelGGL((k_flux<T, 3, DYN_GLIDE_ONLY, TC, TR, true>), g, b, 0, job.stream, job);
     else hipLaunchKernelGGL((k_flux<T, 3, DYN_GLIDE_CLIMB, TC, TR, true>), g, b, 0, job.stream, job);
@@ -551,6 +574,7 @@ static void launch_flux_t(const FluxJob& job, dim3 g, dim3 b) {
 
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
   if (job.B) return hipErrorInvalidValue;  // stage inputs are materialised before the flux pass
+  if (job.dyn == DYN_WOJZZ && job.d != 2) return hipErrorInvalidValue;
   if (job.count > 1 && job.d != 2) return hipErrorInvalidValue;
   dim3 g, b;
   flux_geometry(job.d, job.n, job.tile, job.count, g, b);

@@ -58,7 +58,7 @@ hipError_t launch_spectral(const SpectralJob& job);
 hipError_t launch_sigma_finalize(const double* G, const double* beta, double* sig, long long cells, double mu,
                                  double lam, hipStream_t st, int fp32 = 0);
 
-enum Dyn { DYN_GLIDE_CLIMB = 0, DYN_GLIDE_ONLY = 1 };
+enum Dyn { DYN_GLIDE_CLIMB = 0, DYN_GLIDE_ONLY = 1, DYN_WOJZZ = 2 };
 
 struct FluxJob {
   int d, n, dyn;

@@ -45,7 +45,12 @@ enum {
   CDD_ERR_UNSUPPORTED = -3
 };
 
-enum { CDD_GLIDE_CLIMB = 0, CDD_GLIDE_ONLY = 1 };
+/* CDD_GLIDE_CLIMB_WOJZZ: the 2D variant of
+ * Plasticity/FieldDynamics/CentralUpwindHJBetaPDynamics_woJzz.py (climb
+ * suppression with tr(sigma)/2 along (ux_yy - uy_xy, uy_xx - ux_yx,
+ * -(ux_zx + uy_zy)), its J term on H_xx and H_yy only, beta_zz held fixed);
+ * the adjoint and batched entry points do not support it. */
+enum { CDD_GLIDE_CLIMB = 0, CDD_GLIDE_ONLY = 1, CDD_GLIDE_CLIMB_WOJZZ = 2 };
 
 /* Arithmetic and storage type of a plan's fields.  The reference computes in
  * float64 only (plasticity.h DOUBLE, numpy float64); CDD_FP32 is an added

@@ -461,17 +461,65 @@ def hamiltonian(us, V, Lambda):
     return H
 
 
+def velocity_wojzz(us, sig, Lambda):
+    """BetaP2D_Velocity of the woJzz variant
+    (FieldDynamics/CentralUpwindHJBetaPDynamics_woJzz.py:54-83, 2D only):
+    the climb velocity plus, for Lambda != 0,
+    sigma_tr = tr(sigma)/2 Lambda times (ux_yy - uy_xy, uy_xx - ux_yx,
+    -(ux_zx + uy_zy)), divided by |rho|.  Pinned by: Lambda = 0 reduces to
+    the glide + climb velocity; rotation equivariance of the flux."""
+    ux, uy = us
+    v = [np.zeros(sig.shape[1:]) for _ in range(3)]
+    for n in range(3):
+        for l in range(3):
+            v[l] = v[l] + ux[cidx(l, n)] * sig[cidx(X, n)]
+            v[l] = v[l] + uy[cidx(l, n)] * sig[cidx(Y, n)]
+            v[X] = v[X] - ux[cidx(l, n)] * sig[cidx(l, n)]
+            v[Y] = v[Y] - uy[cidx(l, n)] * sig[cidx(l, n)]
+    if Lambda != 0:
+        sigma_tr = (sig[0] + sig[4] + sig[8]) / 2. * Lambda
+        v[X] = v[X] + sigma_tr * (ux[cidx(Y, Y)] - uy[cidx(X, Y)])
+        v[Y] = v[Y] + sigma_tr * (uy[cidx(X, X)] - ux[cidx(Y, X)])
+        v[Z] = v[Z] - sigma_tr * (ux[cidx(Z, X)] + uy[cidx(Z, Y)])
+    rm = _rhomod(us)
+    return [vl / rm for vl in v]
+
+
+def hamiltonian_wojzz(us, V, Lambda):
+    """H_2D of the woJzz variant (CentralUpwindHJBetaPDynamics_woJzz.py:95-106):
+    the glide + climb Hamiltonian without its Lambda term, then
+    J = Lambda/2 (V_x (ux_yy - uy_xy) + V_y (uy_xx - ux_yx) - V_z (ux_zx + uy_zy))
+    subtracted from H_xx and H_yy only."""
+    ux, uy = us
+    H = hamiltonian(us, V, 0.0)
+    if Lambda != 0:
+        J = Lambda / 2. * (V[0] * (ux[cidx(Y, Y)] - uy[cidx(X, Y)]) + V[1] * (uy[cidx(X, X)] - ux[cidx(Y, X)])
+                           - V[2] * (ux[cidx(Z, X)] + uy[cidx(Z, Y)]))
+        for i in (X, Y):
+            H[cidx(i, i)] = H[cidx(i, i)] - J
+    return H
+
+
 # ---------------------------------------------------------------------------
 # Central-upwind flux: CentralUpwindHJ3D.py:63-271
 # ---------------------------------------------------------------------------
 GLIDE_CLIMB = "glide_climb"
 GLIDE_ONLY = "lvp"
+GLIDE_CLIMB_WOJZZ = "wojzz"  # 2D only; rhs_zz is held at 0
 
 
 def _velocity(dynamics, us, sig, Lambda):
     if dynamics == GLIDE_ONLY:
         return velocity_lvp(us, sig)
+    if dynamics == GLIDE_CLIMB_WOJZZ:
+        return velocity_wojzz(us, sig, Lambda)
     return velocity_glide_climb(us, sig, Lambda)
+
+
+def _hamiltonian(dynamics, us, V, Lambda):
+    if dynamics == GLIDE_CLIMB_WOJZZ:
+        return hamiltonian_wojzz(us, V, Lambda)
+    return hamiltonian(us, V, Lambda)
 
 
 def central_upwind_flux(beta, sig, dynamics=GLIDE_CLIMB, Lambda=0.0, Dx=None):
@@ -485,6 +533,8 @@ def central_upwind_flux(beta, sig, dynamics=GLIDE_CLIMB, Lambda=0.0, Dx=None):
     """
     d = beta.ndim - 1
     lam = 0.0 if dynamics == GLIDE_ONLY else Lambda
+    if dynamics == GLIDE_CLIMB_WOJZZ and d != 2:
+        raise ValueError("the woJzz dynamics is 2D only")
     if d == 2:
         dxp, dxm = find_derivatives(beta, 0, Dx)
         dyp, dym = find_derivatives(beta, 1, Dx)
@@ -498,12 +548,14 @@ def central_upwind_flux(beta, sig, dynamics=GLIDE_CLIMB, Lambda=0.0, Dx=None):
         b_m = np.fabs(min5(v_mm[1], v_mp[1], v_pm[1], v_pp[1], 0.))
         a_tot = a_p + a_m + ME
         b_tot = b_p + b_m + ME
-        H_pp = hamiltonian([dxp, dyp], v_pp, lam)
-        H_pm = hamiltonian([dxp, dym], v_pm, lam)
-        H_mp = hamiltonian([dxm, dyp], v_mp, lam)
-        H_mm = hamiltonian([dxm, dym], v_mm, lam)
+        H_pp = _hamiltonian(dynamics, [dxp, dyp], v_pp, lam)
+        H_pm = _hamiltonian(dynamics, [dxp, dym], v_pm, lam)
+        H_mp = _hamiltonian(dynamics, [dxm, dyp], v_mp, lam)
+        H_mm = _hamiltonian(dynamics, [dxm, dym], v_mm, lam)
         rhs = (-(H_pp * a_m * b_m + H_pm * a_m * b_p + H_mp * a_p * b_m + H_mm * a_p * b_p) / a_tot / b_tot
                + ((dxp - dxm) * a_m * a_p / a_tot) + ((dyp - dym) * b_m * b_p / b_tot))
+        if dynamics == GLIDE_CLIMB_WOJZZ:
+            rhs[cidx(Z, Z)] = 0.  # CentralUpwindHJBetaPDynamics_woJzz.py:128-135
         return rhs, a_tot + b_tot
     dxp, dxm = find_derivatives(beta, 0, Dx)
     dyp, dym = find_derivatives(beta, 1, Dx)

@@ -244,3 +244,44 @@ def test_fp32_variant_error_against_fp64(amd, d, n, dyn, Lambda):
     assert abs(t32 - t64) < 1e-4 * t64
     with pytest.raises(amd.CDDError):
         p32.flux_vjp(beta, 0.0, beta)
+
+
+@pytest.mark.parametrize("Lambda", [0.0, 0.6, 1.0])
+def test_wojzz_flux_and_steps_match_oracle(amd, Lambda):
+    say("woJzz", Lambda)
+    n = 32
+    beta = SpectralGaussianField((n, n), 2.5, 0.01, seed=19)
+    p = _plan(amd, 2, n, 2, Lambda, rk=3)
+    rhs, speed = p.flux(beta)
+    od = O.Dynamics(O.GLIDE_CLIMB_WOJZZ, Lambda, MU, NU)
+    ref_rhs, ref_speed = od.flux(0.0, beta)
+    assert rel_l2(rhs.cpu().numpy(), ref_rhs) < 1e-12
+    assert rel_l2(speed.cpu().numpy(), ref_speed) < 1e-12
+    assert not rhs[8].any()
+    p.set_state(beta)
+    p.set_time(0.0)
+    p.step(10)
+    ref, tref, _ = O.run(od, beta, 0.0, 10, 0.25, order=3)
+    assert rel_l2(p.get_state().cpu().numpy(), ref) < 1e-11
+    with pytest.raises(amd.CDDError):
+        p.flux_vjp(beta, 0.0, beta)
+    with pytest.raises(amd.CDDError):
+        _plan(amd, 3, 8, 2, Lambda)
+
+
+def test_wojzz_reference_api(amd):
+    from Plasticity import PlasticitySystem
+    from Plasticity.FieldDynamics import CentralUpwindHJBetaPDynamics_woJzz as W
+    from Plasticity.FieldMovers import FieldMover
+    from Plasticity.Observers import Observer
+    from Plasticity.PlasticityStates.PlasticityState import PlasticityState
+    n = 16
+    beta = SpectralGaussianField((n, n), 2.0, 0.01, seed=27)
+    state = PlasticityState((n, n), field=beta, mu=MU, nu=NU)
+    rec = Observer.RecallStateObserver()
+    sysm = PlasticitySystem.PlasticitySystem((n, n), state, FieldMover.TVDRungeKutta_FieldMover(CFLsafeFactor=0.25),
+                                             W.BetaPDynamics(Lambda=0.5), [rec])
+    _, t1, dts = O.run(O.Dynamics(O.GLIDE_CLIMB_WOJZZ, 0.5, MU, NU), beta, 0.0, 3, 0.25)
+    sysm.Run(startTime=0.0, endTime=t1)
+    ref, _, _ = O.run(O.Dynamics(O.GLIDE_CLIMB_WOJZZ, 0.5, MU, NU), beta, 0.0, 3, 0.25, t_end=t1)
+    assert rel_l2(rec.state.GetOrderParameterField().numpy(), ref) < 1e-11

@@ -237,3 +237,30 @@ def test_energy_goes_downhill_small_cfl():
         t += dt
         e.append(O.elastic_energy(u))
     assert all(e[i + 1] <= e[i] * (1 + 1e-12) for i in range(5))
+
+
+def test_wojzz_reduces_to_glide_climb_at_lambda_zero_and_freezes_zz():
+    from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
+    beta = SpectralGaussianField((16, 16), 2.5, 0.01, seed=40)
+    sig = O.sigma(beta)
+    r0, s0 = O.central_upwind_flux(beta, sig, O.GLIDE_CLIMB, 0.0)
+    rw, sw = O.central_upwind_flux(beta, sig, O.GLIDE_CLIMB_WOJZZ, 0.0)
+    assert np.array_equal(rw[:8], r0[:8]) and np.array_equal(sw, s0)
+    assert not rw[8].any()
+    rw, _ = O.central_upwind_flux(beta, sig, O.GLIDE_CLIMB_WOJZZ, 0.7)
+    assert not rw[8].any()
+    # the Lambda term changes the flux
+    assert np.abs(rw[:8] - r0[:8]).max() > 1e-6 * np.abs(r0).max()
+
+
+def test_wojzz_rotation_equivariance():
+    """x and y enter the woJzz climb term symmetrically: the flux commutes
+    with the 90-degree rotation of FieldInitializer.RotateState_90degree."""
+    from Plasticity.FieldInitializers.FieldInitializer import RotateState_90degree, SpectralGaussianField
+    from Plasticity.PlasticityStates.PlasticityState import PlasticityState
+    beta = SpectralGaussianField((16, 16), 2.5, 0.01, seed=41)
+    rot = lambda a: RotateState_90degree(PlasticityState((16, 16), field=a, device="cpu"), device="cpu").GetOrderParameterField().numpy()
+    dyn = O.Dynamics(O.GLIDE_CLIMB_WOJZZ, 0.6)
+    r = dyn.flux(0.0, beta)[0]
+    rr = dyn.flux(0.0, rot(beta))[0]
+    np.testing.assert_allclose(rr, rot(r), rtol=0, atol=1e-12 * np.abs(r).max())
