@@ -452,7 +452,14 @@ def main():
             b = per_cell[k] * cells
             entry["gbs"] = round(b / (ms / cnt * 1e-3) / 1e9, 1)
         kernels[k] = entry
-    dom = max((k for k in kernels if k in per_cell), key=lambda k: kernels[k]["total_ms"])
+    # the dominant kernel by total time; the three flux stages are launches of
+    # one kernel and count together (quoted on stage 2, the steady RK stage)
+    fam = {}
+    for k in kernels:
+        if k in per_cell:
+            f = "flux_stage2" if k.startswith("flux_stage") else k
+            fam[f] = fam.get(f, 0.0) + kernels[k]["total_ms"]
+    dom = max(fam, key=fam.get)
     dom_bytes = per_cell[dom] * cells
     achieved = dom_bytes / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
     traffic = None

@@ -517,6 +517,233 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   }
 }
 
+// Plane-marching variant of k_flux for the 3D fp64 stepper (no per-cell speed
+// output, one instance).  A persistent block owns a 32 x 8 column of the two
+// fast axes and walks `chunk` consecutive planes of the slow axis (axis 0).
+// While plane p is computed, the next plane's halo tile (all nine
+// components), its six stress planes G and this plane's U cells stream into
+// LDS by asynchronous global->LDS DMA, and the plane three ahead is loaded
+// into a register ring that holds this thread's cell at planes p-2, p-1,
+// p+1, p+2; plane p's outputs are stored one step later, so no wait in the
+// loop covers a store.  Each plane is read once per column (plus its halo),
+// and a block's load latency hides behind the previous plane's arithmetic.
+// The per-cell arithmetic is cell_flux, exactly as in k_flux.
+namespace march {
+constexpr int TC = 32, TR = 8, NT = TC * TR;
+constexpr int PW = TC + 4, PH = TR + 4, TILE = PW * PH;  // 36 x 12 halo tile
+constexpr int NPAIR = 9 * TILE / 2;                     // 16-byte DMA pairs per tile
+constexpr int NDMA = (NPAIR + NT - 1) / NT;             // DMA instructions per thread
+static_assert(PW % 2 == 0, "pairs must not straddle tile rows");
+}  // namespace march
+
+template <int DYN>
+__global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk) {
+  using T = double;
+  using namespace march;
+  typedef __attribute__((address_space(3))) void lds_void;
+  const T* __restrict__ A = job.A;
+  const T* __restrict__ Gp = job.G;
+  const T* __restrict__ Up = job.U;
+  T* __restrict__ outp = job.out;
+  if (job.clk && job.clk->done) return;
+  __shared__ T tile[9 * TILE];
+  __shared__ T vsh[4 * 8][NT];
+  __shared__ T gb[2][NT / 64][6][64];  // stress planes, double buffered
+  __shared__ T ub[NT / 64][10][64];    // U cells (plane 9 repeats component 8)
+  __shared__ T red[NT / 64];
+  const int n = job.n;
+  const long long cells = job.cells, splane = (long long)n * n;
+  const int tid = threadIdx.x, tc = tid % TC, trr = tid / TC;
+  const int wv = tid >> 6, lane = tid & 63;
+  // block -> (chunk of planes, column); each XCD (hardware block b runs on
+  // XCD b % 8) takes one contiguous range of logical blocks, i.e. adjacent
+  // columns at the same planes, so shared halo rows are served from its L2
+  const int nct = n / TC, ncol = nct * (n / TR);
+  const int nb = ncol * (n / chunk);
+  const int b = blockIdx.x;
+  const int L = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+  const int col = L % ncol, x0 = (L / ncol) * chunk;
+  const int c0 = (col % nct) * TC, r0 = (col / nct) * TR;
+  const T invDx = T(n), halfDx = T(0.5) / T(n);
+  const T dt = job.clk ? T(job.clk->dt) : T(0);
+  const T mu = T(job.mu), lam = T(job.lam), Lambda = T(job.Lambda);
+  const T cu = T(job.cu), cf = T(job.cf);
+  const bool comb = job.mode != 0;
+  auto wrapp = [n](int p) { return p < 0 ? p + n : (p >= n ? p - n : p); };
+
+  // per-lane element offsets inside one plane (plane base added per step):
+  // halo-tile pairs (component, row, even column; n even, so a pair never
+  // straddles the periodic wrap), the G/U cells of this lane's pair, and
+  // this thread's own cell
+  unsigned toff[NDMA];
+#pragma unroll
+  for (int k = 0; k < NDMA; ++k) {
+    const int e = k * NT + wv * 64 + lane;
+    const int ee = e < NPAIR ? e : 0;
+    const int comp = ee / (TILE / 2), rem = ee % (TILE / 2);
+    const int gr = wrapp(r0 - 2 + rem / (PW / 2)), gc = wrapp(c0 - 2 + 2 * (rem % (PW / 2)));
+    toff[k] = (unsigned)(comp * cells + (long long)gr * n + gc);
+  }
+  const int half = lane >> 5, e2 = 64 * wv + 2 * (lane & 31);
+  const unsigned poff = (unsigned)((r0 + e2 / TC) * n + c0 + e2 % TC);
+  const unsigned xoff = (unsigned)((r0 + trr) * n + c0 + tc);
+
+  auto dma_tile = [&](int p) {
+    const T* base = A + (long long)p * splane;
+#pragma unroll
+    for (int k = 0; k < NDMA; ++k) {
+      if (k * NT + wv * 64 + lane < NPAIR)
+        __builtin_amdgcn_global_load_lds((const void*)(base + toff[k]), (lds_void*)&tile[2 * (k * NT + wv * 64)], 16,
+                                         0, 0);
+    }
+  };
+  auto dma_g = [&](int p) {
+    const T* base = Gp + (long long)p * splane + poff;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      __builtin_amdgcn_global_load_lds((const void*)(base + (2 * k + half) * cells), (lds_void*)&gb[p & 1][wv][2 * k][0],
+                                       16, 0, 0);
+  };
+  auto dma_u = [&](int p) {
+    const T* base = Up + (long long)p * splane + poff;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const int c = 2 * k + half < 9 ? 2 * k + half : 8;
+      __builtin_amdgcn_global_load_lds((const void*)(base + c * cells), (lds_void*)&ub[wv][2 * k][0], 16, 0, 0);
+    }
+  };
+  auto center = [&](int p, T (&v)[9]) {
+    const T* base = A + (long long)wrapp(p) * splane + xoff;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) v[c] = base[c * cells];
+  };
+
+  // prologue: plane x0's tile and stress, the ring around it
+  dma_tile(x0);
+  dma_g(x0);
+  T xr[4][9];
+  center(x0 - 2, xr[0]);
+  center(x0 - 1, xr[1]);
+  center(x0 + 1, xr[2]);
+  center(x0 + 2, xr[3]);
+  T pend[9];                 // outputs of the previous plane, stored one step late
+  long long pend_idx = -1;   // their cell index (-1: none yet)
+  T run_max = -INFINITY;
+  const int lr = trr + 2, lc = tc + 2;
+  for (int p = x0; p < x0 + chunk; ++p) {
+    // tile(p), G(p) and the ring's plane p+2 have landed (issued one step
+    // earlier, or in the prologue); the barrier publishes every wave's DMA
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    T u[3][2][9], fc[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      const T* tb = tile + c * TILE;
+      const T ctr = tb[lr * PW + lc];
+      fc[c] = ctr;
+      {
+        T f[5] = {tb[lr * PW + lc - 2], tb[lr * PW + lc - 1], ctr, tb[lr * PW + lc + 1], tb[lr * PW + lc + 2]};
+        limited_derivs<T>(f, invDx, halfDx, u[2][0][c], u[2][1][c]);
+      }
+      {
+        T f[5] = {tb[(lr - 2) * PW + lc], tb[(lr - 1) * PW + lc], ctr, tb[(lr + 1) * PW + lc], tb[(lr + 2) * PW + lc]};
+        limited_derivs<T>(f, invDx, halfDx, u[1][0][c], u[1][1][c]);
+      }
+      {
+        T f[5] = {xr[0][c], xr[1][c], ctr, xr[2][c], xr[3][c]};
+        limited_derivs<T>(f, invDx, halfDx, u[0][0][c], u[0][1][c]);
+      }
+    }
+    T sg[9];
+    {
+      const T tr = fc[0] + fc[4] + fc[8];
+      const T* g = &gb[p & 1][wv][0][lane];
+      const T g0 = g[0], g1 = g[64], g2 = g[128], g3 = g[192], g4 = g[256], g5 = g[320];
+      sg[0] = g0 - lam * tr - mu * (fc[0] + fc[0]);
+      sg[4] = g1 - lam * tr - mu * (fc[4] + fc[4]);
+      sg[8] = g2 - lam * tr - mu * (fc[8] + fc[8]);
+      sg[1] = sg[3] = g3 - mu * (fc[1] + fc[3]);
+      sg[2] = sg[6] = g4 - mu * (fc[2] + fc[6]);
+      sg[5] = sg[7] = g5 - mu * (fc[5] + fc[7]);
+    }
+    // materialise the derivatives here: left to itself the compiler sinks
+    // their arithmetic past the barrier and keeps all 81 raw stencil values
+    // per thread live instead of the 54 derivatives
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int c = 0; c < 9; ++c) asm volatile("" : "+v"(u[a][s][c]));
+#pragma unroll
+    for (int c = 0; c < 9; ++c) asm volatile("" : "+v"(sg[c]), "+v"(fc[c]));
+    // every wave is done with tile(p) and gb[(p + 1) & 1] before they refill
+    __syncthreads();
+    if (p + 1 < x0 + chunk) {
+      dma_tile(p + 1);
+      dma_g(p + 1);
+    }
+    if (comb) dma_u(p);
+    T nx[9];
+    center(p + 3, nx);
+    if (pend_idx >= 0) {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) outp[c * cells + pend_idx] = pend[c];
+    }
+    T rhs[9], speed;
+    cell_flux<T, 3, DYN, true>(u, sg, Lambda, rhs, speed, &vsh[0][tid], NT);
+    bool fin = isfinite(speed);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) fin = fin && isfinite(rhs[c]);
+    run_max = fmax(run_max, fin ? speed : T(INFINITY));
+    // U(p) (this wave's own cells) and the ring's next plane
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    if (comb) {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) pend[c] = cu * ub[wv][c][lane] + cf * (fc[c] + dt * rhs[c]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) pend[c] = rhs[c];
+    }
+    pend_idx = (long long)p * splane + xoff;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      xr[0][c] = xr[1][c];
+      xr[1][c] = fc[c];
+      xr[2][c] = xr[3][c];
+      xr[3][c] = nx[c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 9; ++c) outp[c * cells + pend_idx] = pend[c];
+  if (job.partial) {
+    const T m = block_max<T>(run_max, red);
+    if (tid == 0) job.partial[blockIdx.x] = m;
+  }
+}
+
+// Planes per marching block: the largest power of two that still gives at
+// least 1024 blocks (four per CU), but no fewer than 16 planes.  The
+// CDD_FLUX_CHUNK environment variable overrides it (0 selects k_flux).
+static int march_chunk(int n) {
+  static int env = -2;
+  if (env == -2) {
+    const char* s = getenv("CDD_FLUX_CHUNK");
+    env = s ? atoi(s) : -1;
+  }
+  if (env == 0) return 0;
+  if (env > 0) return (env <= n && n % env == 0) ? env : 0;
+  const int ncol = (n / march::TC) * (n / march::TR);
+  int chunk = n;
+  while (chunk > 16 && ncol * (n / chunk) < 1024) chunk /= 2;
+  return chunk;
+}
+
+static bool march_ok(const FluxJob& job) {
+  return job.d == 3 && !job.fp32 && !job.speed && job.count <= 1 && job.tile == 0 && job.n >= 32 &&
+         job.dyn != DYN_WOJZZ && (job.mode == 0 || job.U) && march_chunk(job.n) > 0;
+}
+
 // Tile shapes (cells of the contiguous x row axis): variant 0 = 32 x 8,
 // 1 = 64 x 4, 2 = 16 x 16 where n allows; whole rows for small n.  A 1D
 // grid per instance (grid.y = instances); the kernel maps block ids to tiles.
@@ -576,6 +803,16 @@ hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
   if (job.B) return hipErrorInvalidValue;  // stage inputs are materialised before the flux pass
   if (job.dyn == DYN_WOJZZ && job.d != 2) return hipErrorInvalidValue;
   if (job.count > 1 && job.d != 2) return hipErrorInvalidValue;
+  if (march_ok(job)) {
+    const int chunk = march_chunk(job.n);
+    const int nb = (job.n / march::TC) * (job.n / march::TR) * (job.n / chunk);
+    if (nblocks_out) *nblocks_out = nb;
+    if (job.dyn == DYN_GLIDE_ONLY)
+      hipLaunchKernelGGL((k_flux_march<DYN_GLIDE_ONLY>), dim3(nb), dim3(march::NT), 0, job.stream, job, chunk);
+    else
+      hipLaunchKernelGGL((k_flux_march<DYN_GLIDE_CLIMB>), dim3(nb), dim3(march::NT), 0, job.stream, job, chunk);
+    return hipGetLastError();
+  }
   dim3 g, b;
   flux_geometry(job.d, job.n, job.tile, job.count, g, b);
   if (nblocks_out) *nblocks_out = (int)g.x;

@@ -32,10 +32,10 @@ def env():
     return A, OC
 
 
-def _compare(env, n, nsteps, seed=2):
+def _compare(env, n, nsteps, seed=2, dynamics=0, Lambda=0.9, rk=3):
     A, OC = env
     beta = SpectralGaussianField((n, n, n), 6.0, 0.01, seed=seed)
-    p = A.Plan(3, n, dynamics=0, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25)
+    p = A.Plan(3, n, dynamics=dynamics, rk_order=rk, Lambda=Lambda, mu=1.0, nu=0.3, cfl=0.25)
     p.set_state(beta)
     p.set_time(0.0)
     p.step(nsteps)
@@ -43,13 +43,13 @@ def _compare(env, n, nsteps, seed=2):
     t, _, _, ns = p.clock()
     say("  GPU done; C oracle ...")
     import time
-    dyn = O.Dynamics(O.GLIDE_CLIMB, 0.9, 1.0, 0.3)
+    dyn = O.Dynamics(O.GLIDE_ONLY if dynamics == 1 else O.GLIDE_CLIMB, Lambda, 1.0, 0.3)
     # the C oracle in chunks of 10 steps (same step sequence), with progress
     ref, tref, dts = beta, 0.0, []
     t0 = time.perf_counter()
     while len(dts) < nsteps:
         k = min(10, nsteps - len(dts))
-        ref, tref, d = OC.run(dyn, ref, tref, k, 0.25, order=3)
+        ref, tref, d = OC.run(dyn, ref, tref, k, 0.25, order=rk)
         dts.extend(d)
         say("  C oracle %d/%d steps, %.1f s" % (len(dts), nsteps, time.perf_counter() - t0))
     err = np.linalg.norm((got - ref).ravel()) / np.linalg.norm(ref.ravel())
@@ -58,6 +58,15 @@ def _compare(env, n, nsteps, seed=2):
     assert abs(t - tref) <= 1e-12 * tref
     assert err < 1e-11
     return err
+
+
+@pytest.mark.parametrize("rk", [2, 3])
+@pytest.mark.parametrize("dynamics,Lambda", [(0, 0.9), (0, 0.0), (1, 0.0)])
+def test_3d_32_steps(env, dynamics, Lambda, rk):
+    """32^3: the smallest grid the plane-marching flux kernel runs on (two
+    16-plane chunks per column), both dynamics, both RK orders."""
+    say("3D 32^3 x 12 steps vs C oracle", dynamics, Lambda, rk)
+    _compare(env, 32, 12, seed=5, dynamics=dynamics, Lambda=Lambda, rk=rk)
 
 
 def test_3d_64_50_steps(env):
