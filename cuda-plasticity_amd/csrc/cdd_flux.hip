@@ -28,51 +28,52 @@
 
 namespace cdd {
 
+// Two-argument minmod: min if both > 0, max if both <= 0, else 0, written as
+// max(min, 0) + min(max, 0) (the other term is +0 in every case) without
+// compares or selects.
 template <typename T>
-__device__ __forceinline__ T minmod3(T a, T b, T c) {
-  // NumericalMethods.py:367-378: min if all > 0, max if all <= 0, else 0;
-  // written as max(min, 0) + min(max, 0), which is exactly that (the other
-  // term is +0 in every case) without compares or selects.
-  const T mn = fmin(a, fmin(b, c));
-  const T mx = fmax(a, fmax(b, c));
-  return fmax(mn, T(0)) + fmin(mx, T(0));
+__device__ __forceinline__ T minmod2(T a, T c) {
+  return fmax(fmin(a, c), T(0)) + fmin(fmax(a, c), T(0));
 }
 
 // One-sided limited derivatives of f at i from f[i-2..i+2]
-// (FindDerivatives, CentralUpwindHJ3D.py:11-34, theta = 1, Dx = 1/N_last).
-// The reference scales the minmod arguments by 1/Dx^2 and the result by
-// Dx/2; with N a power of two these scalings are exact, commute with the
-// minmod, and the result is bit-identical to
-//   u+ = N (D_i - m_i / 2),  u- = N (D_{i-1} + m_{i-1} / 2),
-//   m_i = minmod(D_{i+1} - D_i, (D_{i+1} - D_{i-1}) / 2, D_i - D_{i-1}).
+// (FindDerivatives, CentralUpwindHJ3D.py:11-34, theta = 1, Dx = 1/N_last),
+// returned in units of 1/N: the reference's u+ = N (D_i - m_i / 2) and
+// u- = N (D_{i-1} + m_{i-1} / 2), with D_i = f_{i+1} - f_i and
+//   m_i = minmod3(D_{i+1} - D_i, (D_{i+1} - D_{i-1}) / 2, D_i - D_{i-1})
+// (NumericalMethods.py:367-378 MinMod3; the reference's 1/Dx^2 and Dx/2
+// scalings are powers of two and commute with it).  With theta = 1 the
+// middle argument is the mean of the other two, so it lies between them
+// whenever they share a sign and minmod3 equals minmod2 of the outer two;
+// the rounded mean can step outside by an ulp, so the two forms agree to
+// the last bit except at such ties.  The factor N is applied once to the
+// flux: every term of the right-hand side is homogeneous of degree one in
+// the derivatives once the |rho| regulariser is taken in the same units
+// (ME / N^2); for N a power of two the rescaling is exact and the result is
+// bit-identical to scaling each derivative.
 template <typename T>
-__device__ __forceinline__ void limited_derivs(const T f[5], T invDx, T halfDx, T& up, T& um) {
-  (void)halfDx;
+__device__ __forceinline__ void limited_derivs(const T f[5], T& up, T& um) {
   const T Dm2 = f[1] - f[0], Dm1 = f[2] - f[1], D0 = f[3] - f[2], D1 = f[4] - f[3];
-  const T s1_i = D1 - D0;
-  const T s1_im1 = D0 - Dm1;
-  const T s1_im2 = Dm1 - Dm2;
-  const T s2_i = (D1 - Dm1) * T(0.5);
-  const T s2_im1 = (D0 - Dm2) * T(0.5);
-  const T m_i = minmod3(s1_i, s2_i, s1_im1);
-  const T m_im1 = minmod3(s1_im1, s2_im1, s1_im2);
-  up = invDx * fma(T(-0.5), m_i, D0);
-  um = invDx * fma(T(0.5), m_im1, Dm1);
+  const T a0 = D0 - Dm1;
+  const T m_i = minmod2(D1 - D0, a0);
+  const T m_im1 = minmod2(a0, Dm1 - Dm2);
+  up = fma(T(-0.5), m_i, D0);
+  um = fma(T(0.5), m_im1, Dm1);
 }
 
 // Per-cell flux.  u[a][s][c]: derivative along axis a, s = 0 ('+') or 1
-// ('-'), component c = 3 i + j.  sg: full (symmetric) stress.
-// Per-cell flux.  u[a][s][c]: derivative along axis a, s = 0 ('+') or 1
-// ('-'), component c = 3 i + j.  sg: full (symmetric) stress.  vbuf: this
+// ('-'), component c = 3 i + j, in units of 1/N (limited_derivs); rhs comes
+// out in the reference's units.  sg: full (symmetric) stress.  vbuf: this
 // thread's slot of an LDS array [4 * 2^D][blockDim] that parks the corner
 // velocities between the speed pass and the Hamiltonian pass.
 // PRE: keep the per-axis velocity numerators w, z in registers (fewer
 // flops) or recompute them at every corner (fewer live registers).
 template <typename T, int D, int DYN, bool PRE>
 __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9], T Lambda, T (&rhs)[9],
-                                          T& speed, T* __restrict__ vbuf, int vstride) {
+                                          T& speed, T* __restrict__ vbuf, int vstride, T nscale) {
   constexpr int NCOR = 1 << D;
   const T ME = T(kME);
+  const T MEU = ME / (nscale * nscale);  // ME in the units of u^2
   // per axis / sign quantities
   T w[D][2][3], zz[D][2], tr[D][2], P[D][2];
 #pragma unroll
@@ -175,18 +176,18 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
       }
     } else {
       // lvp projection (CentralUpwindHJBetaPGlideOnlyDynamics.py:170-195)
-      T rr = ME;
+      T rr = MEU;
 #pragma unroll
       for (int a = 0; a < D; ++a) rr += tr[a][sgn[a]] * tr[a][sgn[a]] - T(2) * tr[a][sgn[a]] * R[a];
 #pragma unroll
       for (int l = 0; l < 3; ++l) rr += R[l] * R[l];
-      const T rv = ME + v0[0] * q[0] + v0[1] * q[1] + v0[2] * q[2];
+      const T rv = MEU + v0[0] * q[0] + v0[1] * q[1] + v0[2] * q[2];
       const T f = rv / rr;
 #pragma unroll
       for (int a = 0; a < 3; ++a) v0[a] += q[a] * f;
     }
     // |rho|^2 (BetaP3D_RhoMod, CentralUpwindHJBetaPDynamics3D.py:45-58)
-    T r2 = ME;
+    T r2 = MEU;
 #pragma unroll
     for (int a = 0; a < D; ++a) r2 += P[a][sgn[a]];
     T cross = T(0);
@@ -288,7 +289,7 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
       T r = -h * inv;
 #pragma unroll
       for (int a = 0; a < D; ++a) r += (u[a][0][cc] - u[a][1][cc]) * tw[a];
-      rhs[cc] = (DYN == DYN_WOJZZ && cc == 8) ? T(0) : r;  // woJzz holds beta_zz fixed
+      rhs[cc] = (DYN == DYN_WOJZZ && cc == 8) ? T(0) : r * nscale;  // woJzz holds beta_zz fixed
     }
   }
 }
@@ -377,8 +378,6 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   const long long pbase = (D == 3) ? (long long)px * splane : 0;
   const int mycol = c0 + tc, myrow = r0 + trr;
   const long long idx = pbase + (long long)myrow * srow + mycol;
-  const T invDx = T(n);
-  const T halfDx = T(0.5) * (T(1) / T(n));
   const T dt = job.clk ? T(job.clk->dt) : T(0);
 
   // One batch of loads for every component (a single memory round trip):
@@ -459,16 +458,16 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
     fc[c] = ctr;
     {  // contiguous axis (last)
       T f[5] = {tb[lr * FPW + lc - 2], tb[lr * FPW + lc - 1], ctr, tb[lr * FPW + lc + 1], tb[lr * FPW + lc + 2]};
-      limited_derivs<T>(f, invDx, halfDx, u[D - 1][0][c], u[D - 1][1][c]);
+      limited_derivs<T>(f, u[D - 1][0][c], u[D - 1][1][c]);
     }
     {  // row axis (second last)
       T f[5] = {tb[(lr - 2) * FPW + lc], tb[(lr - 1) * FPW + lc], ctr, tb[(lr + 1) * FPW + lc],
                 tb[(lr + 2) * FPW + lc]};
-      limited_derivs<T>(f, invDx, halfDx, u[D - 2][0][c], u[D - 2][1][c]);
+      limited_derivs<T>(f, u[D - 2][0][c], u[D - 2][1][c]);
     }
     if constexpr (D == 3) {
       T f[5] = {xn[c][0], xn[c][1], ctr, xn[c][2], xn[c][3 % NX]};
-      limited_derivs<T>(f, invDx, halfDx, u[0][0][c], u[0][1][c]);
+      limited_derivs<T>(f, u[0][0][c], u[0][1][c]);
     }
   }
   // the asynchronous G/U loads must have landed before LDS is read
@@ -496,7 +495,7 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
     sg[5] = sg[7] = g5 - mu * (fc[5] + fc[7]);
   }
   T rhs[9], speed;
-  cell_flux<T, D, DYN, PRE>(u, sg, T(job.Lambda), rhs, speed, &vsh[0][tid], FNT);
+  cell_flux<T, D, DYN, PRE>(u, sg, T(job.Lambda), rhs, speed, &vsh[0][tid], FNT, T(n));
   if (job.mode == 0) {
 #pragma unroll
     for (int c = 0; c < 9; ++c) outp[c * cells + idx] = rhs[c];
@@ -564,7 +563,6 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
   const int L = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
   const int col = L % ncol, x0 = (L / ncol) * chunk;
   const int c0 = (col % nct) * TC, r0 = (col / nct) * TR;
-  const T invDx = T(n), halfDx = T(0.5) / T(n);
   const T dt = job.clk ? T(job.clk->dt) : T(0);
   const T mu = T(job.mu), lam = T(job.lam), Lambda = T(job.Lambda);
   const T cu = T(job.cu), cf = T(job.cf);
@@ -643,15 +641,15 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
       fc[c] = ctr;
       {
         T f[5] = {tb[lr * PW + lc - 2], tb[lr * PW + lc - 1], ctr, tb[lr * PW + lc + 1], tb[lr * PW + lc + 2]};
-        limited_derivs<T>(f, invDx, halfDx, u[2][0][c], u[2][1][c]);
+        limited_derivs<T>(f, u[2][0][c], u[2][1][c]);
       }
       {
         T f[5] = {tb[(lr - 2) * PW + lc], tb[(lr - 1) * PW + lc], ctr, tb[(lr + 1) * PW + lc], tb[(lr + 2) * PW + lc]};
-        limited_derivs<T>(f, invDx, halfDx, u[1][0][c], u[1][1][c]);
+        limited_derivs<T>(f, u[1][0][c], u[1][1][c]);
       }
       {
         T f[5] = {xr[0][c], xr[1][c], ctr, xr[2][c], xr[3][c]};
-        limited_derivs<T>(f, invDx, halfDx, u[0][0][c], u[0][1][c]);
+        limited_derivs<T>(f, u[0][0][c], u[0][1][c]);
       }
     }
     T sg[9];
@@ -691,7 +689,7 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
       for (int c = 0; c < 9; ++c) outp[c * cells + pend_idx] = pend[c];
     }
     T rhs[9], speed;
-    cell_flux<T, 3, DYN, true>(u, sg, Lambda, rhs, speed, &vsh[0][tid], NT);
+    cell_flux<T, 3, DYN, true>(u, sg, Lambda, rhs, speed, &vsh[0][tid], NT, T(n));
     bool fin = isfinite(speed);
 #pragma unroll
     for (int c = 0; c < 9; ++c) fin = fin && isfinite(rhs[c]);
@@ -722,9 +720,10 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
   }
 }
 
-// Planes per marching block: the largest power of two that still gives at
-// least 1024 blocks (four per CU), but no fewer than 16 planes.  The
-// CDD_FLUX_CHUNK environment variable overrides it (0 selects k_flux).
+// Planes per marching block: the longest chunk (a power of two, at least 16
+// planes) that still gives every CU of the current device a block; the LDS
+// footprint admits one block per CU, so longer chunks only save prologues.
+// The CDD_FLUX_CHUNK environment variable overrides it (0 selects k_flux).
 static int march_chunk(int n) {
   static int env = -2;
   if (env == -2) {
@@ -733,9 +732,17 @@ static int march_chunk(int n) {
   }
   if (env == 0) return 0;
   if (env > 0) return (env <= n && n % env == 0) ? env : 0;
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if (cus[dev] <= 0) {
+    int c = 0;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+    cus[dev] = c;
+  }
   const int ncol = (n / march::TC) * (n / march::TR);
   int chunk = n;
-  while (chunk > 16 && ncol * (n / chunk) < 1024) chunk /= 2;
+  while (chunk > 16 && ncol * (n / chunk) < cus[dev]) chunk /= 2;
   return chunk;
 }
 
