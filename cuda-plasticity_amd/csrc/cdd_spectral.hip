@@ -209,10 +209,14 @@ __device__ __forceinline__ double kwrap(int i, int n) {
   return 6.283185307179586 * (double)(i <= n / 2 ? i : i - n);
 }
 
-// Green's contraction for output symmetric component c at one k.
+// Symmetric component c = 0..5 (xx yy zz xy xz yz) -> (i, j).
+__device__ __forceinline__ constexpr int sym_i(int c) { return c < 3 ? c : (c == 5 ? 1 : 0); }
+__device__ __forceinline__ constexpr int sym_j(int c) { return c < 3 ? c : (c == 3 ? 1 : 2); }
+
+// Green's contraction at one k, all six symmetric output components.
 // e[6] = FFT of (xx, yy, zz, xy, xz, yz) of the symmetric part of beta.
 template <typename T>
-__device__ __forceinline__ cx<T> green_sigma(const cx<T>* e, int c, T kx, T ky, T kz, const GreenParams& gp) {
+__device__ __forceinline__ void green_sigma(const cx<T>* e, cx<T>* out, T kx, T ky, T kz, const GreenParams& gp) {
   const T kSq = kx * kx + ky * ky + kz * kz;
   const T K2 = kSq + T(kME);
   const T K4 = kSq * kSq + T(kME);
@@ -224,21 +228,17 @@ __device__ __forceinline__ cx<T> green_sigma(const cx<T>* e, int c, T kx, T ky, 
   const cx<T> q = p[0] * kx + p[1] * ky + p[2] * kz;
   const T iK2 = T(1) / K2, iK4 = T(1) / K4;
   const cx<T> a1 = t * (T(gp.A) * iK2) - q * (T(gp.C) * iK4);
-  // symmetric component c -> (i, j): xx yy zz xy xz yz
-  const int i = c < 3 ? c : (c == 5 ? 1 : 0);
-  const int j = c < 3 ? c : (c == 3 ? 1 : 2);
-  const T ki = i == 0 ? kx : (i == 1 ? ky : kz);
-  const T kj = j == 0 ? kx : (j == 1 ? ky : kz);
-  const cx<T> pi_ = i == 0 ? p[0] : (i == 1 ? p[1] : p[2]);
-  const cx<T> pj_ = j == 0 ? p[0] : (j == 1 ? p[1] : p[2]);
-  // S_ij = T_ij + T_ji
-  cx<T> S = a1 * (T(2) * ki * kj) + (pj_ * ki + pi_ * kj) * (T(2) * iK2);
-  cx<T> out = S * T(gp.mu);
-  if (i == j) {
-    cx<T> Tll = a1 * kSq + q * (T(2) * iK2);
-    out = out + Tll * T(gp.lam);
+  const cx<T> Tll = a1 * kSq + q * (T(2) * iK2);
+  const T k3[3] = {kx, ky, kz};
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const int i = sym_i(c), j = sym_j(c);
+    // S_ij = T_ij + T_ji
+    const cx<T> S = a1 * (T(2) * k3[i] * k3[j]) + (p[j] * k3[i] + p[i] * k3[j]) * (T(2) * iK2);
+    cx<T> o = S * T(gp.mu);
+    if (i == j) o = o + Tll * T(gp.lam);
+    out[c] = o * T(gp.norm);
   }
-  return out * T(gp.norm);
 }
 
 // Transpose of green_sigma w.r.t. the Frobenius product of symmetric
@@ -249,7 +249,7 @@ __device__ __forceinline__ cx<T> green_sigma(const cx<T>* e, int c, T kx, T ky, 
 //               + 2 mu/K2 (gk_i k_j + k_i gk_j).
 // (It equals green_sigma up to the ME regularisation of K2, K4.)
 template <typename T>
-__device__ __forceinline__ cx<T> green_sigma_T(const cx<T>* g, int c, T kx, T ky, T kz, const GreenParams& gp) {
+__device__ __forceinline__ void green_sigma_T(const cx<T>* g, cx<T>* out, T kx, T ky, T kz, const GreenParams& gp) {
   const T kSq = kx * kx + ky * ky + kz * kz;
   const T K2 = kSq + T(kME);
   const T K4 = kSq * kSq + T(kME);
@@ -261,36 +261,42 @@ __device__ __forceinline__ cx<T> green_sigma_T(const cx<T>* g, int c, T kx, T ky
   gk[2] = g[4] * kx + g[5] * ky + g[2] * kz;
   const cx<T> qg = gk[0] * kx + gk[1] * ky + gk[2] * kz;
   const T lam = T(gp.lam), mu = T(gp.mu), A = T(gp.A), C = T(gp.C);
-  const int i = c < 3 ? c : (c == 5 ? 1 : 0);
-  const int j = c < 3 ? c : (c == 3 ? 1 : 2);
-  const T ki = i == 0 ? kx : (i == 1 ? ky : kz);
-  const T kj = j == 0 ? kx : (j == 1 ? ky : kz);
-  const cx<T> gi = i == 0 ? gk[0] : (i == 1 ? gk[1] : gk[2]);
-  const cx<T> gj = j == 0 ? gk[0] : (j == 1 ? gk[1] : gk[2]);
-  cx<T> out = (tg * (-lam * kSq * C * iK4 + T(2) * lam * iK2) - qg * (T(2) * mu * C * iK4)) * (ki * kj) +
-              (gi * kj + gj * ki) * (T(2) * mu * iK2);
-  if (i == j) out = out + tg * (lam * kSq * A * iK2) + qg * (T(2) * mu * A * iK2);
-  return out * T(gp.norm);
+  const cx<T> kk = tg * (-lam * kSq * C * iK4 + T(2) * lam * iK2) - qg * (T(2) * mu * C * iK4);
+  const cx<T> dg = tg * (lam * kSq * A * iK2) + qg * (T(2) * mu * A * iK2);
+  const T k3[3] = {kx, ky, kz};
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const int i = sym_i(c), j = sym_j(c);
+    cx<T> o = kk * (k3[i] * k3[j]) + (gk[i] * k3[j] + gk[j] * k3[i]) * (T(2) * mu * iK2);
+    if (i == j) o = o + dg;
+    out[c] = o * T(gp.norm);
+  }
 }
 
-// Nye tensor rho_ij = -eps_ilm d_l beta_mj in k space (times 1/N^d).
+// Nye tensor rho_ij = -eps_ilm d_l beta_mj in k space (times 1/N^d), all
+// nine components.
 template <typename T>
-__device__ __forceinline__ cx<T> green_rho(const cx<T>* b, int c, T kx, T ky, T kz, T norm) {
-  const int i = c / 3, j = c % 3;
-  cx<T> r;
-  if (i == 0) r = b[3 + j] * kz - b[6 + j] * ky;
-  else if (i == 1) r = b[6 + j] * kx - b[j] * kz;
-  else r = b[j] * ky - b[3 + j] * kx;
-  return mul_pi(r) * norm;
+__device__ __forceinline__ void green_rho(const cx<T>* b, cx<T>* out, T kx, T ky, T kz, T norm) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    out[j] = mul_pi(b[3 + j] * kz - b[6 + j] * ky) * norm;
+    out[3 + j] = mul_pi(b[6 + j] * kx - b[j] * kz) * norm;
+    out[6 + j] = mul_pi(b[j] * ky - b[3 + j] * kx) * norm;
+  }
 }
 
 // Rows of spec laid out as (c * NR + row) * N + e.  grid.x = ceil(NR / TR).
 // 3D: row = kz * N + x, e = y;  2D: row = ky, e = x.
+// Forward FFT of every component row (one transform per TT threads), then
+// the contraction with each thread owning whole k points (the k-dependent
+// terms once per k, all outputs at once, in place in LDS), then the
+// inverse FFT of every output row.
 template <typename T, int N, int NC, int D, bool TRANS = false>
 __global__ void __launch_bounds__((NC * ContractTiles<N, NC>::TR * ContractTiles<N, NC>::TT))
 k_contract(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, GreenParams gp, long long sstride) {
   using CT = ContractTiles<N, NC>;
   constexpr int TT = CT::TT, TR = CT::TR, P = LdsRow<N>::P, V = FFTPlan<N>::V, H = N / 2 + 1;
+  constexpr int NTH = NC * TR * TT;
   constexpr long long NR = (D == 3) ? (long long)H * N : (long long)H;
   spec += (long long)blockIdx.y * sstride;  // batched 2D instance
   if (gp.clk) gp.clk += blockIdx.y;
@@ -315,72 +321,74 @@ k_contract(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, GreenParams g
   for (int i = 0; i < V; ++i) myrow[lds_pad(fft_out_index<N>(j, i))] = v[i];
   __syncthreads();
 
-  // k vector of this row
-  T kx, ky, kz;
-  int kzi = 0, rowi = 0;
-  if constexpr (D == 3) {
-    kzi = (int)(row / N);
-    rowi = (int)(row % N);  // x index
-    kz = T(6.283185307179586 * kzi);
-    kx = T(kwrap(rowi, N));
-  } else {
-    ky = T(6.283185307179586 * (double)row);
-    kz = T(0);
-  }
-  double tnow = gp.clk ? gp.clk->t : gp.t;
+  const double tnow = gp.clk ? gp.clk->t : gp.t;
+  for (int kp = tid; kp < TR * N; kp += NTH) {
+    const int rr = kp / N, e = kp % N;
+    const long long rowk = (long long)blockIdx.x * TR + rr;
+    if (rowk >= NR) continue;
+    T kx, ky, kz;
+    if constexpr (D == 3) {
+      kz = T(6.283185307179586 * (double)(rowk / N));
+      kx = T(kwrap((int)(rowk % N), N));
+      ky = T(kwrap(e, N));
+    } else {
+      ky = T(6.283185307179586 * (double)rowk);
+      kz = T(0);
+      kx = T(kwrap(e, N));
+    }
+    cx<T> in[NC], o[NC];
 #pragma unroll
-  for (int i = 0; i < V; ++i) {
-    const int e = fft_in_index<N>(j, i);
-    cx<T> in[NC];
-#pragma unroll
-    for (int cc = 0; cc < NC; ++cc) in[cc] = lds[(cc * TR + r) * P + lds_pad(e)];
-    if constexpr (D == 3) ky = T(kwrap(e, N));
-    else kx = T(kwrap(e, N));
-    cx<T> o;
+    for (int cc = 0; cc < NC; ++cc) in[cc] = lds[(cc * TR + rr) * P + lds_pad(e)];
     if constexpr (NC == 6) {
-      const bool k0 = (row == 0) && (e == 0);
+      const bool k0 = (rowk == 0) && (e == 0);
       if (TRANS) {
-        o = k0 ? cx<T>{T(0), T(0)} : green_sigma_T<T>(in, c, kx, ky, kz, gp);
+        if (k0) {
+#pragma unroll
+          for (int cc = 0; cc < 6; ++cc) o[cc] = cx<T>{T(0), T(0)};
+        } else {
+          green_sigma_T<T>(in, o, kx, ky, kz, gp);
+        }
       } else if (!k0) {
-        o = green_sigma<T>(in, c, kx, ky, kz, gp);
+        green_sigma<T>(in, o, kx, ky, kz, gp);
       } else {
         // k = 0: kmask removes the Green's term; the uniform load enters here
         double s[9];
         const double mean_bzz = (double)in[2].x * gp.norm;
         load_shift(gp.load, gp.el, tnow, mean_bzz, s);
-        if (gp.log_stress && c == 0 && gp.clk && gp.clk->slog) {
+        if (gp.log_stress && gp.clk && gp.clk->slog) {
           // stress-strain record of the step start: t, mean sigma (load
           // included; the non-local part has zero mean), mean beta_zz
           const Clock* ck = gp.clk;
           const long long li = ck->nsteps - ck->slog_base;
           if (li >= 0 && li < ck->slog_cap) {
-            double e[6];
+            double em[6];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) e[q] = (double)in[q].x * gp.norm;
-            const double tr = e[0] + e[1] + e[2], lam = gp.el.lam, mu2 = 2.0 * gp.el.mu;
+            for (int q = 0; q < 6; ++q) em[q] = (double)in[q].x * gp.norm;
+            const double tr = em[0] + em[1] + em[2], lam = gp.el.lam, mu2 = 2.0 * gp.el.mu;
             double* rec = ck->slog + 8 * li;
             rec[0] = tnow;
-            rec[1] = s[0] - lam * tr - mu2 * e[0];
-            rec[2] = s[4] - lam * tr - mu2 * e[1];
-            rec[3] = s[8] - lam * tr - mu2 * e[2];
-            rec[4] = s[1] - mu2 * e[3];
-            rec[5] = s[2] - mu2 * e[4];
-            rec[6] = s[5] - mu2 * e[5];
+            rec[1] = s[0] - lam * tr - mu2 * em[0];
+            rec[2] = s[4] - lam * tr - mu2 * em[1];
+            rec[3] = s[8] - lam * tr - mu2 * em[2];
+            rec[4] = s[1] - mu2 * em[3];
+            rec[5] = s[2] - mu2 * em[4];
+            rec[6] = s[5] - mu2 * em[5];
             rec[7] = mean_bzz;
           }
         }
-        const int cs9 = c == 0 ? 0 : c == 1 ? 4 : c == 2 ? 8 : c == 3 ? 1 : c == 4 ? 2 : 5;
-        T sv = T(0);
+        constexpr int CS9[6] = {0, 4, 8, 1, 2, 5};
 #pragma unroll
-        for (int q9 = 0; q9 < 9; ++q9)
-          if (q9 == cs9) sv = T(s[q9]);
-        o = cx<T>{sv, T(0)};
+        for (int cc = 0; cc < 6; ++cc) o[cc] = cx<T>{T(s[CS9[cc]]), T(0)};
       }
     } else {
-      o = green_rho<T>(in, c, kx, ky, kz, T(gp.norm));
+      green_rho<T>(in, o, kx, ky, kz, T(gp.norm));
     }
-    v[i] = conj(o);
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) lds[(cc * TR + rr) * P + lds_pad(e)] = conj(o[cc]);
   }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < V; ++i) v[i] = myrow[lds_pad(fft_in_index<N>(j, i))];
   fft_forward<T, N>(v, j, tw, ex);
   if (valid) {
 #pragma unroll
