@@ -48,33 +48,27 @@ struct ContractTiles {
   static constexpr int TR = TR0 >= 16 ? 16 : (TR0 >= 8 ? 8 : (TR0 >= 4 ? 4 : (TR0 >= 2 ? 2 : 1)));
 };
 
-// value of source component c (SYM: symmetric-part index) at cell idx
-template <typename T, bool SYM>
-__device__ __forceinline__ T src_comp(const SrcField<T>& s, int c, long long idx, T dt) {
-  auto b = [&](int k) -> T {
-    T v = s.A[k * s.cs + idx];
-    if (s.B) v += dt * s.B[k * s.cs + idx];
-    if (s.W) s.W[k * s.cs + idx] = v;  // every component is read exactly once per cell
-    return v;
-  };
+// Source components of transform component c: SYM (the symmetric part
+// e_mn = (beta_mn + beta_nm) / 2, order xx yy zz xy xz yz) averages two beta
+// components for the off-diagonal ones; otherwise c itself.  -1: none.
+template <bool SYM>
+__device__ __forceinline__ void src_comps(int c, int& k0, int& k1) {
   if constexpr (SYM) {
-    switch (c) {
-      case 0: return b(0);
-      case 1: return b(4);
-      case 2: return b(8);
-      case 3: return T(0.5) * (b(1) + b(3));
-      case 4: return T(0.5) * (b(2) + b(6));
-      default: return T(0.5) * (b(5) + b(7));
-    }
+    constexpr int K0[6] = {0, 4, 8, 1, 2, 5}, K1[6] = {-1, -1, -1, 3, 6, 7};
+    k0 = K0[c];
+    k1 = K1[c];
   } else {
-    return b(c);
+    k0 = c;
+    k1 = -1;
   }
 }
 
 // --------------------------------------------------------------- r2c_last
 // grid: (N / TY, NO, npairs); block: TY * TT.  NO = N in 3D (outer x), 1 in 2D.
+// (two waves per SIMD: the LDS tile admits two blocks per CU, and the
+// register budget is held to match)
 template <typename T, int N, bool SYM>
-__global__ void __launch_bounds__(Tiles<N>::TY * Tiles<N>::TT)
+__global__ void __launch_bounds__(Tiles<N>::TY * Tiles<N>::TT, 2)
 k_r2c_last(SrcField<T> src, cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, int NO, int NC,
            long long sstride) {
   using TL = Tiles<N>;
@@ -94,17 +88,64 @@ k_r2c_last(SrcField<T> src, cx<T>* __restrict__ spec, const cx<T>* __restrict__ 
   const int tid = threadIdx.x;
   const int r = tid / TT, j = tid % TT;
   const int r0 = blockIdx.x * TY;
-  const int ca = 2 * blockIdx.z, cb = 2 * blockIdx.z + 1;
+  // transform pairs: SYM pairs a diagonal with an off-diagonal component,
+  // (xx, yz), (yy, xz), (zz, xy), so every block reads three beta components
+  const int ca = SYM ? (int)blockIdx.z : 2 * (int)blockIdx.z;
+  const int cb = SYM ? 5 - (int)blockIdx.z : 2 * (int)blockIdx.z + 1;
   const bool hasb = cb < NC;
   const T dt = (src.B && src.clk) ? T(src.clk->dt) : T(0);
 
-  cx<T> v[V];
+  // Source slots: SYM reads the diagonal component ca and the two beta
+  // components averaged into the off-diagonal cb; otherwise ca and cb.
+  // Every load comes before the W stores (W may alias B: F0 formed in place
+  // over L0), so the loads of all slots can be in flight together; where
+  // that would not fit the register budget (V >= 24) the slots are loaded
+  // one after another.  Each source component is read (and W written) by
+  // exactly one pair block of a row tile.
+  constexpr int NS = SYM ? 3 : 2;
+  constexpr bool STAGED = 2 * NS * V > 96;
   const long long rowbase = ((long long)o * N + (r0 + r)) * N;
+  int ks[NS];
+  if constexpr (SYM) {
+    int unused;
+    src_comps<true>(ca, ks[0], unused);
+    src_comps<true>(cb, ks[1], ks[2]);
+  } else {
+    ks[0] = ca;
+    ks[1] = hasb ? cb : -1;
+  }
+  T a[NS][V];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    if (ks[q] < 0) continue;
+    const T* Ak = src.A + ks[q] * src.cs + rowbase;
+#pragma unroll
+    for (int i = 0; i < V; ++i) a[q][i] = Ak[fft_in_index<N>(j, i)];
+    if (src.B) {
+      const T* Bk = src.B + ks[q] * src.cs + rowbase;
+      T b[V];
+#pragma unroll
+      for (int i = 0; i < V; ++i) b[i] = Bk[fft_in_index<N>(j, i)];
+#pragma unroll
+      for (int i = 0; i < V; ++i) a[q][i] += dt * b[i];
+    }
+    if constexpr (STAGED) asm volatile("" ::: "memory");
+  }
+  if (src.W) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      if (ks[q] < 0) continue;
+      T* Wk = src.W + ks[q] * src.cs + rowbase;
+#pragma unroll
+      for (int i = 0; i < V; ++i) Wk[fft_in_index<N>(j, i)] = a[q][i];
+    }
+  }
+  cx<T> v[V];
 #pragma unroll
   for (int i = 0; i < V; ++i) {
-    const long long idx = rowbase + fft_in_index<N>(j, i);
-    v[i].x = src_comp<T, SYM>(src, ca, idx, dt);
-    v[i].y = hasb ? src_comp<T, SYM>(src, cb, idx, dt) : T(0);
+    v[i].x = a[0][i];
+    if constexpr (SYM) v[i].y = T(0.5) * (a[1][i] + a[2][i]);
+    else v[i].y = ks[1] >= 0 ? a[1][i] : T(0);
   }
   RowExchange<T> ex{lds + r * P};
   fft_forward<T, N>(v, j, tw, ex);
@@ -114,7 +155,10 @@ k_r2c_last(SrcField<T> src, cx<T>* __restrict__ spec, const cx<T>* __restrict__ 
   __syncthreads();
   // separate the two real transforms and store [c][k][o][r] (r contiguous)
   const int rr = tid % TY, g = tid / TY;
-  for (int k = g; k < H; k += TT) {
+#pragma unroll
+  for (int m = 0; m < (H + TT - 1) / TT; ++m) {
+    const int k = g + m * TT;
+    if (k >= H) break;
     cx<T> z1 = lds[rr * P + lds_pad(k)];
     cx<T> z2 = conj(lds[rr * P + lds_pad((N - k) % N)]);
     const long long oa = (((long long)ca * H + k) * NO + o) * N + r0 + rr;
@@ -146,10 +190,22 @@ k_c2r_last(const cx<T>* __restrict__ spec, T* __restrict__ out, long long cs, co
   const int ca = 2 * blockIdx.z, cb = 2 * blockIdx.z + 1;
   const bool hasb = cb < NC;
   {
+    // all loads of the transposed tile first (one memory round trip)
+    constexpr int NK = (H + TT - 1) / TT;
     const int rr = tid % TY, g = tid / TY;
-    for (int k = g; k < H; k += TT) {
-      cx<T> A = spec[(((long long)ca * H + k) * NO + o) * N + r0 + rr];
-      cx<T> B = hasb ? spec[(((long long)cb * H + k) * NO + o) * N + r0 + rr] : cx<T>{T(0), T(0)};
+    cx<T> Av[NK], Bv[NK];
+#pragma unroll
+    for (int m = 0; m < NK; ++m) {
+      const int k = g + m * TT;
+      const bool in = k < H;
+      Av[m] = in ? spec[(((long long)ca * H + k) * NO + o) * N + r0 + rr] : cx<T>{T(0), T(0)};
+      Bv[m] = (in && hasb) ? spec[(((long long)cb * H + k) * NO + o) * N + r0 + rr] : cx<T>{T(0), T(0)};
+    }
+#pragma unroll
+    for (int m = 0; m < NK; ++m) {
+      const int k = g + m * TT;
+      if (k >= H) break;
+      cx<T> A = Av[m], B = Bv[m];
       if (k == 0 || k == N / 2) {  // irfft ignores the imaginary part here
         A.y = T(0);
         B.y = T(0);
