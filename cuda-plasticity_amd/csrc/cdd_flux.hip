@@ -520,12 +520,13 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
 // output, one instance).  A persistent block owns a 32 x 8 column of the two
 // fast axes and walks `chunk` consecutive planes of the slow axis (axis 0).
 // While plane p is computed, the next plane's halo tile (all nine
-// components), its six stress planes G and this plane's U cells stream into
-// LDS by asynchronous global->LDS DMA, and the plane three ahead is loaded
-// into a register ring that holds this thread's cell at planes p-2, p-1,
-// p+1, p+2; plane p's outputs are stored one step later, so no wait in the
-// loop covers a store.  Each plane is read once per column (plus its halo),
-// and a block's load latency hides behind the previous plane's arithmetic.
+// components), its six stress planes G, this plane's U cells and the cells
+// three planes ahead stream into LDS by asynchronous global->LDS DMA; the
+// latter refill a register ring that holds this thread's cell at planes
+// p-2, p-1, p+1, p+2.  Each plane is read once per column (plus its halo),
+// and a block's load latency hides behind the previous plane's arithmetic:
+// the only wait in the loop follows the arithmetic, when the outputs
+// stored one step earlier have long completed.
 // The per-cell arithmetic is cell_flux, exactly as in k_flux.
 namespace march {
 constexpr int TC = 32, TR = 8, NT = TC * TR;
@@ -549,6 +550,7 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
   __shared__ T vsh[4 * 8][NT];
   __shared__ T gb[2][NT / 64][6][64];  // stress planes, double buffered
   __shared__ T ub[NT / 64][10][64];    // U cells (plane 9 repeats component 8)
+  __shared__ T nbuf[NT / 64][10][64];  // the ring's next plane, same layout
   __shared__ T red[NT / 64];
   const int n = job.n;
   const long long cells = job.cells, splane = (long long)n * n;
@@ -610,6 +612,14 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
       __builtin_amdgcn_global_load_lds((const void*)(base + c * cells), (lds_void*)&ub[wv][2 * k][0], 16, 0, 0);
     }
   };
+  auto dma_next = [&](int p) {
+    const T* base = A + (long long)wrapp(p) * splane + poff;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const int c = 2 * k + half < 9 ? 2 * k + half : 8;
+      __builtin_amdgcn_global_load_lds((const void*)(base + c * cells), (lds_void*)&nbuf[wv][2 * k][0], 16, 0, 0);
+    }
+  };
   auto center = [&](int p, T (&v)[9]) {
     const T* base = A + (long long)wrapp(p) * splane + xoff;
 #pragma unroll
@@ -624,14 +634,13 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
   center(x0 - 1, xr[1]);
   center(x0 + 1, xr[2]);
   center(x0 + 2, xr[3]);
-  T pend[9];                 // outputs of the previous plane, stored one step late
-  long long pend_idx = -1;   // their cell index (-1: none yet)
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   T run_max = -INFINITY;
   const int lr = trr + 2, lc = tc + 2;
   for (int p = x0; p < x0 + chunk; ++p) {
-    // tile(p), G(p) and the ring's plane p+2 have landed (issued one step
-    // earlier, or in the prologue); the barrier publishes every wave's DMA
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    // tile(p) and G(p) have landed (each wave waited for its own DMA at the
+    // end of the previous step, or in the prologue); the barrier publishes
+    // every wave's part
     __syncthreads();
     T u[3][2][9], fc[9];
 #pragma unroll
@@ -682,38 +691,32 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
       dma_g(p + 1);
     }
     if (comb) dma_u(p);
-    T nx[9];
-    center(p + 3, nx);
-    if (pend_idx >= 0) {
-#pragma unroll
-      for (int c = 0; c < 9; ++c) outp[c * cells + pend_idx] = pend[c];
-    }
+    dma_next(p + 3);
     T rhs[9], speed;
     cell_flux<T, 3, DYN, true>(u, sg, Lambda, rhs, speed, &vsh[0][tid], NT, T(n));
     bool fin = isfinite(speed);
 #pragma unroll
     for (int c = 0; c < 9; ++c) fin = fin && isfinite(rhs[c]);
     run_max = fmax(run_max, fin ? speed : T(INFINITY));
-    // U(p) (this wave's own cells) and the ring's next plane
+    // everything issued this step has landed: U(p) and the ring's next plane
+    // (this wave's own cells), tile(p + 1) and G(p + 1)
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    T* o = outp + (long long)p * splane + xoff;
     if (comb) {
 #pragma unroll
-      for (int c = 0; c < 9; ++c) pend[c] = cu * ub[wv][c][lane] + cf * (fc[c] + dt * rhs[c]);
+      for (int c = 0; c < 9; ++c) o[c * cells] = cu * ub[wv][c][lane] + cf * (fc[c] + dt * rhs[c]);
     } else {
 #pragma unroll
-      for (int c = 0; c < 9; ++c) pend[c] = rhs[c];
+      for (int c = 0; c < 9; ++c) o[c * cells] = rhs[c];
     }
-    pend_idx = (long long)p * splane + xoff;
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
       xr[0][c] = xr[1][c];
       xr[1][c] = fc[c];
       xr[2][c] = xr[3][c];
-      xr[3][c] = nx[c];
+      xr[3][c] = nbuf[wv][c][lane];
     }
   }
-#pragma unroll
-  for (int c = 0; c < 9; ++c) outp[c * cells + pend_idx] = pend[c];
   if (job.partial) {
     const T m = block_max<T>(run_max, red);
     if (tid == 0) job.partial[blockIdx.x] = m;
