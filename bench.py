@@ -260,9 +260,11 @@ def bench_adjoint(args):
     """2D 512^2, 400 SSP-RK2 steps, glide + climb with climb mobility 0.1
     (Lambda = 0.9), uniform shear sigma_xy = 0.005: gradient of the final
     total elastic energy w.r.t. the initial beta^P and the climb mobility by
-    the hand-written adjoint (checkpoints every 20 steps), its cost relative
-    to the plain forward run, and central finite differences on 32 random
-    unit directions (eps = 1e-6) replaying the recorded time steps."""
+    the hand-written adjoint (segments of 20 steps whose stage states are kept
+    in device memory; the checkpoint-recomputation cost is reported beside
+    it), its cost relative to the plain forward run, and central finite
+    differences on 32 random unit directions (eps = 1e-6) replaying the
+    recorded time steps."""
     import numpy as np
     import torch
     from Plasticity.amd import lib as L
@@ -290,14 +292,28 @@ def bench_adjoint(args):
     e_fwd = forward(False)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
-    b = beta0.clone().requires_grad_(True)
-    lt = torch.tensor(Lam, dtype=torch.float64, device="cuda", requires_grad=True)
+
+    def adjoint(store):
+        b = beta0.clone().requires_grad_(True)
+        lt = torch.tensor(Lam, dtype=torch.float64, device="cuda", requires_grad=True)
+        E = elastic_energy(run_with_grad(b, lt, nsteps, checkpoint_every=20, rk_order=2, load=shear, store=store),
+                           load=shear)
+        E.backward()
+        return b, lt
+    # stored trajectories (every segment fits: 400 x 63 MB) versus checkpoint
+    # recomputation; one untimed call first (plan set-up, device allocations)
+    adjoint("auto")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    E = elastic_energy(run_with_grad(b, lt, nsteps, checkpoint_every=20, rk_order=2, load=shear), load=shear)
-    E.backward()
+    b, lt = adjoint("auto")
     torch.cuda.synchronize()
     t_adj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    adjoint(False)
+    torch.cuda.synchronize()
+    t_adj_rc = time.perf_counter() - t0
+    store_bytes = L.Plan(2, n, dynamics=L.GLIDE_CLIMB, rk_order=2, Lambda=Lam, **shear).segment_store_bytes(20) * (
+        nsteps // 20)
     grad = b.grad.detach()
     mob_grad = -lt.grad.item()  # climb mobility = 1 - Lambda
 
@@ -337,10 +353,13 @@ def bench_adjoint(args):
         "n_gpus": 1, "steps": nsteps, "warmup": 1,
         "ms_per_step": 1e3 * t_fwd / nsteps,
         "adjoint_ms_per_step": 1e3 * t_adj / nsteps,
+        "stored_trajectory_bytes": store_bytes,
+        "cost_vs_forward_recompute": t_adj_rc / t_fwd,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: seeded Gaussian random beta^P (L=4 cells, RMS 0.01)",
         "config": {"workload": "cdd-2d-512-adjoint", "grid": [n, n], "dynamics": "glide+climb", "Lambda": Lam,
-                   "rk_order": 2, "cfl": 0.25, "load": "sigma_xy = sigma_yx = 0.005", "checkpoint_every": 20},
+                   "rk_order": 2, "cfl": 0.25, "load": "sigma_xy = sigma_yx = 0.005", "checkpoint_every": 20,
+                   "trajectory": "stored"},
         "energy": e_fwd,
         "dE_dmobility": mob_grad,
         "fd_directions": ndir,

@@ -7,12 +7,16 @@ parameter; climb mobility = 1 - Lambda).  ``elastic_energy(beta)`` is the
 total elastic energy with its hand-written gradient.
 
 Forward: the time steps are chosen by the CFL rule and recorded on the
-device; a checkpoint of the state is kept every ``checkpoint_every`` steps.
-Backward: segments are processed last to first by cdd_segment_vjp: each is
-recomputed from its checkpoint replaying the recorded time steps exactly
-(the adjoint treats the schedule as frozen), keeping every stage state and
-its stress planes, and the cotangent is carried back through it stage by
-stage on the device with no host synchronisation.
+device; the run is cut into segments of ``checkpoint_every`` steps.  While
+device memory allows (``store``), a segment keeps its whole trajectory (every
+stage state and its stress planes, cdd_segment_forward); otherwise only a
+checkpoint of its start state.
+Backward: segments are processed last to first: a stored segment goes
+straight into the reverse pass (cdd_segment_vjp_stored); a checkpointed one
+is first recomputed from its checkpoint replaying the recorded time steps
+exactly (cdd_segment_vjp; the adjoint treats the schedule as frozen).  The
+cotangent is carried back stage by stage on the device with no host
+synchronisation; both paths give bit-identical gradients.
 """
 import torch
 
@@ -23,6 +27,57 @@ def _plan(n, d, dynamics, rk_order, Lambda, mu, nu, cfl, load, device):
     kw = dict(dynamics=dynamics, rk_order=rk_order, Lambda=float(Lambda), mu=mu, nu=nu, cfl=cfl, device=device)
     kw.update(load or {})
     return L.Plan(d, n, **kw)
+
+
+def _store_budget(device, store):
+    """Device bytes the forward may keep in stored segments: store=False keeps
+    none, True or "auto" up to 60% of the free device memory (the rest serves
+    the recomputation workspace of the other segments), an int that many."""
+    if store is False or store is None:
+        return 0
+    if store is True or store == "auto":
+        free, _ = torch.cuda.mem_get_info(device)
+        return int(0.6 * free)
+    return int(store)
+
+
+def _forward_segments(plan, nsteps, ckpt, budget, device):
+    """Advance the plan by nsteps steps in segments of ckpt steps; each
+    segment keeps ("store", trajectory) while the budget allows, else
+    ("ckpt", start state)."""
+    segs, used, done = [], 0, 0
+    while done < nsteps:
+        k = min(ckpt, nsteps - done)
+        nb = plan.segment_store_bytes(k)
+        if used + nb <= budget:
+            st = torch.empty(nb, dtype=torch.uint8, device=device)
+            plan.segment_forward(k, st)
+            used += nb
+            segs.append(("store", st))
+        else:
+            segs.append(("ckpt", plan.get_state()))
+            plan.step(k)
+        done += k
+    return segs
+
+
+def _backward_segments(plan, segs, log, t0, ckpt, nsteps, g, src=None):
+    """Carry the cotangent g (in place) back through every segment, last
+    first; each store is released once used."""
+    dts = log.cpu().tolist()
+    times = [t0]
+    for dt in dts[:-1]:
+        times.append(times[-1] + dt)
+    for s in reversed(range(len(segs))):
+        first = s * ckpt
+        last = min(first + ckpt, nsteps)
+        sv = None if src is None else src[first:last].contiguous()
+        kind, obj = segs[s]
+        if kind == "store":
+            plan.segment_vjp_stored(obj, log[first:last], g, out=g, src=sv)
+        else:
+            plan.segment_vjp(obj, times[first], log[first:last], g, out=g, src=sv)
+        segs[s] = None
 
 
 class _Run(torch.autograd.Function):
@@ -36,50 +91,38 @@ class _Run(torch.autograd.Function):
         plan.set_state(beta0.detach())
         plan.set_time(float(t0), float("inf"), float(opts.get("fixed_dt", 0.0)))
         plan.dt_record(log)
-        checkpoints = [beta0.detach().clone()]
-        done_steps = 0
-        while done_steps < nsteps:
-            k = min(ckpt, nsteps - done_steps)
-            plan.step(k)
-            done_steps += k
-            if done_steps < nsteps:
-                checkpoints.append(plan.get_state())
+        segs = _forward_segments(plan, nsteps, ckpt, _store_budget(beta0.device, opts.get("store", "auto")),
+                                 beta0.device)
         out = plan.get_state()
         plan.dt_record(None)
         ctx.plan = plan
         ctx.log = log
-        ctx.checkpoints = checkpoints
+        ctx.segs = segs
         ctx.nsteps, ctx.t0, ctx.ckpt = nsteps, float(t0), ckpt
         return out
 
     @staticmethod
     def backward(ctx, gout):
-        plan, log, cps = ctx.plan, ctx.log, ctx.checkpoints
-        dts = log.cpu().tolist()
-        times = [ctx.t0]
-        for dt in dts[:-1]:
-            times.append(times[-1] + dt)
+        plan = ctx.plan
         g = gout.detach().contiguous().clone()
         plan.lambda_bar(reset=True)
-        for s in reversed(range(len(cps))):
-            first = s * ctx.ckpt
-            last = min(first + ctx.ckpt, ctx.nsteps)
-            # recompute the segment from its checkpoint replaying the recorded
-            # steps, then carry the cotangent back through it (in place)
-            plan.segment_vjp(cps[s], times[first], log[first:last], g, out=g)
+        _backward_segments(plan, ctx.segs, ctx.log, ctx.t0, ctx.ckpt, ctx.nsteps, g)
         lam_grad = torch.tensor(plan.lambda_bar(reset=True), dtype=torch.float64, device=gout.device)
         return g, lam_grad, None, None, None, None
 
 
 def run_with_grad(beta0, Lambda, nsteps, t0=0.0, checkpoint_every=20, dynamics=L.GLIDE_CLIMB, rk_order=2, mu=1.0,
-                  nu=0.3, cfl=0.25, load=None, fixed_dt=0.0):
+                  nu=0.3, cfl=0.25, load=None, fixed_dt=0.0, store="auto"):
     """Final state after nsteps steps; differentiable w.r.t. beta0 and Lambda
     (a 0-dim float64 tensor).  load: optional dict of Plan load_* arguments
     (every load kind, the uniaxial strain-rate control included).  fixed_dt > 0:
-    fixed time steps (CalculateOneFixedTimeStep) instead of the CFL rule."""
+    fixed time steps (CalculateOneFixedTimeStep) instead of the CFL rule.
+    store: device memory for stored segment trajectories ("auto", True,
+    False or a byte budget); segments beyond it are recomputed."""
     if not torch.is_tensor(Lambda):
         Lambda = torch.tensor(float(Lambda), dtype=torch.float64, device=beta0.device)
-    opts = dict(dynamics=dynamics, rk_order=rk_order, mu=mu, nu=nu, cfl=cfl, load=load, fixed_dt=fixed_dt)
+    opts = dict(dynamics=dynamics, rk_order=rk_order, mu=mu, nu=nu, cfl=cfl, load=load, fixed_dt=fixed_dt,
+                store=store)
     return _Run.apply(beta0, Lambda, int(nsteps), float(t0), int(checkpoint_every), opts)
 
 
@@ -166,14 +209,8 @@ class _StressWindow(torch.autograd.Function):
         plan.set_time(float(t0), float("inf"), float(opts.get("fixed_dt", 0.0)))
         plan.dt_record(log)
         plan.stress_record(slog)
-        checkpoints = [beta0.detach().clone()]
-        done_steps = 0
-        while done_steps < nsteps:
-            k = min(ckpt, nsteps - done_steps)
-            plan.step(k)
-            done_steps += k
-            if done_steps < nsteps:
-                checkpoints.append(plan.get_state())
+        segs = _forward_segments(plan, nsteps, ckpt, _store_budget(beta0.device, opts.get("store", "auto")),
+                                 beta0.device)
         final = plan.get_state()
         t_end = plan.clock()[0]
         plan.dt_record(None)
@@ -187,7 +224,7 @@ class _StressWindow(torch.autograd.Function):
         sh = _load_shift(opts.get("load"), t_end, mb[8], opts["mu"], opts["nu"])
         m_final = sh[i][j] - (lam * (mb[0] + mb[4] + mb[8]) if i == j else 0.0) - opts["mu"] * (mb[3 * i + j] + mb[3 * j + i])
         J = (slog[nsteps - window + 1:, col].sum().item() + m_final) / window
-        ctx.plan, ctx.log, ctx.checkpoints = plan, log, checkpoints
+        ctx.plan, ctx.log, ctx.segs = plan, log, segs
         ctx.nsteps, ctx.t0, ctx.ckpt, ctx.window = nsteps, float(t0), ckpt, window
         ctx.g = _mean_stress_grad(comp, opts.get("load"), opts["mu"], opts["nu"], final[0].numel())
         ctx.shape = tuple(final.shape)
@@ -195,28 +232,22 @@ class _StressWindow(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gJ):
-        plan, log, cps = ctx.plan, ctx.log, ctx.checkpoints
+        plan, log = ctx.plan, ctx.log
         dev = log.device
         w = gJ.item() / ctx.window
         gvec = torch.tensor(ctx.g, dtype=torch.float64, device=dev) * w
         g = gvec.reshape((9,) + (1,) * (len(ctx.shape) - 1)).expand(ctx.shape).contiguous()
         src = torch.zeros((ctx.nsteps, 9), dtype=torch.float64, device=dev)
         src[ctx.nsteps - ctx.window + 1:] = gvec
-        dts = log.cpu().tolist()
-        times = [ctx.t0]
-        for dt in dts[:-1]:
-            times.append(times[-1] + dt)
         plan.lambda_bar(reset=True)
-        for s in reversed(range(len(cps))):
-            first = s * ctx.ckpt
-            last = min(first + ctx.ckpt, ctx.nsteps)
-            plan.segment_vjp(cps[s], times[first], log[first:last], g, out=g, src=src[first:last].contiguous())
+        _backward_segments(plan, ctx.segs, log, ctx.t0, ctx.ckpt, ctx.nsteps, g, src=src)
         lam_grad = torch.tensor(plan.lambda_bar(reset=True), dtype=torch.float64, device=dev)
         return g, lam_grad, None, None, None, None, None, None
 
 
 def mean_stress_window(beta0, Lambda, nsteps, window, comp="zz", t0=0.0, checkpoint_every=20,
-                       dynamics=L.GLIDE_CLIMB, rk_order=3, mu=1.0, nu=0.3, cfl=0.25, load=None, fixed_dt=0.0):
+                       dynamics=L.GLIDE_CLIMB, rk_order=3, mu=1.0, nu=0.3, cfl=0.25, load=None, fixed_dt=0.0,
+                       store="auto"):
     """Stress-strain objective: the mean stress component `comp` (external load
     included, as PlottingTools.StrainStress reports it, PlottingTools.py:69-100)
     averaged over the states after each of the last `window` of nsteps steps;
@@ -227,7 +258,8 @@ def mean_stress_window(beta0, Lambda, nsteps, window, comp="zz", t0=0.0, checkpo
         raise ValueError("comp must be one of %s" % sorted(_COMPS))
     if not torch.is_tensor(Lambda):
         Lambda = torch.tensor(float(Lambda), dtype=torch.float64, device=beta0.device)
-    opts = dict(dynamics=dynamics, rk_order=rk_order, mu=mu, nu=nu, cfl=cfl, load=load, fixed_dt=fixed_dt)
+    opts = dict(dynamics=dynamics, rk_order=rk_order, mu=mu, nu=nu, cfl=cfl, load=load, fixed_dt=fixed_dt,
+                store=store)
     return _StressWindow.apply(beta0, Lambda, int(nsteps), int(window), comp, float(t0), int(checkpoint_every), opts)
 
 

@@ -49,7 +49,8 @@ EXPORTS = [
     "cdd_batch_get_state", "cdd_batch_set_time", "cdd_batch_get_clock", "cdd_batch_step", "cdd_batch_set_tile",
     "cdd_batch_time_flux", "cdd_flux_vjp", "cdd_step_vjp", "cdd_energy_grad", "cdd_dt_record", "cdd_dt_replay",
     "cdd_segment_vjp", "cdd_lambda_bar", "cdd_stress_record", "cdd_plas_append", "cdd_plas_resume",
-    "cdd_energy_record", "cdd_status", "cdd_instability",
+    "cdd_energy_record", "cdd_status", "cdd_instability", "cdd_segment_store_bytes", "cdd_segment_forward",
+    "cdd_segment_vjp_stored",
 ]
 KERNEL_KINDS = ["r2c", "cols_fwd", "contract", "cols_inv", "c2r", "flux_stage1", "cfl_reduce",
                 "flux_stage2", "flux_stage3", "advance"]
@@ -108,6 +109,9 @@ def _load():
         "cdd_dt_record": ([P, P, ctypes.c_longlong], I),
         "cdd_dt_replay": ([P, P, ctypes.c_longlong], I),
         "cdd_segment_vjp": ([P, P, D, P, I, P, P, P], I),
+        "cdd_segment_store_bytes": ([P, I, ctypes.POINTER(ctypes.c_ulonglong)], I),
+        "cdd_segment_forward": ([P, I, P, ctypes.c_ulonglong], I),
+        "cdd_segment_vjp_stored": ([P, P, ctypes.c_ulonglong, P, I, P, P, P], I),
         "cdd_lambda_bar": ([P, dp, I], I),
         "cdd_stress_record": ([P, P, ctypes.c_longlong], I),
         "cdd_energy_record": ([P, P, ctypes.c_longlong], I),
@@ -304,6 +308,51 @@ class Plan:
                 raise CDDError("src must be a contiguous float64 (nsteps, 9) tensor on the plan's device")
             sp = _ptr(src)
         check(lib.cdd_segment_vjp(self.h, _ptr(a), float(t0), _ptr(dts), int(dts.numel()), sp, _ptr(g), _ptr(out)))
+        return out
+
+    def segment_store_bytes(self, nsteps):
+        """Device bytes of one nsteps segment's stored trajectory."""
+        v = ctypes.c_ulonglong()
+        check(lib.cdd_segment_store_bytes(self.h, int(nsteps), ctypes.byref(v)))
+        return v.value
+
+    def _store_ptr(self, store, nsteps):
+        t = self.torch
+        if not store.is_cuda or store.device != self.dev or not store.is_contiguous():
+            raise CDDError("store must be a contiguous tensor on the plan's device")
+        nbytes = store.numel() * store.element_size()
+        if nbytes < self.segment_store_bytes(nsteps):
+            raise CDDError("store too small for %d steps (segment_store_bytes)" % nsteps)
+        return ctypes.c_void_p(store.data_ptr()), nbytes
+
+    def segment_forward(self, nsteps, store):
+        """Advance the state by nsteps steps like step(), keeping the
+        segment's start state, every stage state and its stress planes in
+        `store` (a device tensor of at least segment_store_bytes(nsteps)
+        bytes) for segment_vjp_stored."""
+        ptr, nbytes = self._store_ptr(store, nsteps)
+        check(lib.cdd_segment_forward(self.h, int(nsteps), ptr, nbytes))
+
+    def segment_vjp_stored(self, store, dts, ubar_end, out=None, src=None):
+        """segment_vjp over a segment kept by segment_forward: no
+        recomputation (same dts, ubar_end, out, src and Lambda cotangent)."""
+        g = self._as_dev(ubar_end)
+        if dts.dtype != self.torch.float64 or not dts.is_contiguous() or dts.device != self.dev:
+            raise CDDError("dts must be a contiguous float64 tensor on the plan's device")
+        nsteps = int(dts.numel())
+        ptr, nbytes = self._store_ptr(store, nsteps)
+        if out is None:
+            out = self.empty()
+        elif (out.dtype != self.torch.float64 or not out.is_contiguous() or out.device != self.dev
+              or tuple(out.shape) != (9,) + self.shape):
+            raise CDDError("out must be a contiguous float64 (9,) + grid tensor on the plan's device")
+        sp = None
+        if src is not None:
+            if (src.dtype != self.torch.float64 or not src.is_contiguous() or src.device != self.dev
+                    or tuple(src.shape) != (nsteps, 9)):
+                raise CDDError("src must be a contiguous float64 (nsteps, 9) tensor on the plan's device")
+            sp = _ptr(src)
+        check(lib.cdd_segment_vjp_stored(self.h, ptr, nbytes, _ptr(dts), nsteps, sp, _ptr(g), _ptr(out)))
         return out
 
     def lambda_bar(self, reset=True):

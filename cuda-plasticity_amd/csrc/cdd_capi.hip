@@ -768,24 +768,28 @@ static int stage_vjp(cdd_plan* p, const double* beta, const double* G, const dou
   return CDD_OK;
 }
 
-static int seg_alloc(cdd_plan* p, int nsteps) {
-  const int S = p->cfg.rk_order;
-  const size_t plane = sizeof(double) * (size_t)p->cells;
-  const size_t need = (size_t)nsteps * S * 15 * plane + 9 * plane;
-  const size_t lneed = (size_t)S * nsteps * flux_vjp_nblocks(p->d, p->n);
-  if (need > p->seg_bytes || lneed > p->lseg_len) HIPCHK(hipStreamSynchronize(p->stream));
-  if (need > p->seg_bytes) {
-    (void)hipFree(p->seg);
-  if (p->host) (void)hipHostFree(p->host);
-  (void)hipFree(p->epart);
-    p->seg = nullptr;
-    p->seg_bytes = 0;
-    if (hipMalloc((void**)&p->seg, need) != hipSuccess)
-      return fail(CDD_ERR_HIP, ("hipMalloc failed (segment workspace of " + std::to_string(need) +
-                                " bytes; use shorter checkpoint intervals)").c_str());
-    p->seg_bytes = need;
+// Layout of a segment's stored trajectory: the S stage states (9 planes)
+// of every step, then the S stress solves (6 planes) of every step.  The
+// recomputing reverse keeps one more 9-plane state (the segment's end) after
+// them.
+struct SegLayout {
+  double* base;
+  int S;
+  long long cells;
+  int nsteps;
+  double* st(int j, int k) const { return base + ((size_t)j * S + k) * 9 * cells; }
+  double* gs(int j, int k) const {
+    return base + (size_t)nsteps * S * 9 * cells + ((size_t)j * S + k) * 6 * cells;
   }
+  static size_t bytes(int nsteps, int S, long long cells) {
+    return sizeof(double) * (size_t)nsteps * S * 15 * (size_t)cells;
+  }
+};
+
+static int lseg_alloc(cdd_plan* p, int nsteps) {
+  const size_t lneed = (size_t)p->cfg.rk_order * nsteps * flux_vjp_nblocks(p->d, p->n);
   if (lneed > p->lseg_len) {
+    HIPCHK(hipStreamSynchronize(p->stream));
     (void)hipFree(p->lseg);
     p->lseg = nullptr;
     p->lseg_len = 0;
@@ -796,32 +800,28 @@ static int seg_alloc(cdd_plan* p, int nsteps) {
   return CDD_OK;
 }
 
-int cdd_segment_vjp(cdd_plan* p, const double* u0, double t0, const double* dts_dev, int nsteps,
-                    const double* src_dev, const double* ubar_end, double* ubar0) {
-  if (!p || !u0 || !dts_dev || !ubar_end || !ubar0 || nsteps < 1) return fail(CDD_ERR_ARG, "bad argument");
-  HIPCHK(hipSetDevice(p->cfg.device));
-  RCHK(adj_alloc(p));
-  RCHK(seg_alloc(p, nsteps));
-  const int S = p->cfg.rk_order;
-  const long long cells = p->cells;
-  auto st = [&](int j, int k) { return p->seg + ((size_t)j * S + k) * 9 * cells; };
-  double* gbase = p->seg + (size_t)nsteps * S * 9 * cells;
-  auto gs = [&](int j, int k) { return gbase + ((size_t)j * S + k) * 6 * cells; };
-  double* ufinal = gbase + (size_t)nsteps * S * 6 * cells;
-  // recompute the segment from u0, replaying the recorded time steps and
-  // keeping every stage state and its stress planes
-  HIPCHK(launch_clock_segment(p->clk, 0, t0, dts_dev, nsteps, p->stream));
-  for (int j = 0; j < nsteps; ++j) {
-    StepBufs b;
-    b.u = j == 0 ? u0 : st(j, 0);
-    b.out = j + 1 < nsteps ? st(j + 1, 0) : ufinal;
-    for (int k = 0; k < 3; ++k) b.G[k] = gs(j, k < S ? k : S - 1);
-    b.F0 = st(j, 1);
-    b.F1 = S == 3 ? st(j, 2) : nullptr;
-    RCHK(launch_step(p, b));
+static int seg_alloc(cdd_plan* p, int nsteps) {
+  const size_t plane = sizeof(double) * (size_t)p->cells;
+  const size_t need = SegLayout::bytes(nsteps, p->cfg.rk_order, p->cells) + 9 * plane;
+  if (need > p->seg_bytes) {
+    HIPCHK(hipStreamSynchronize(p->stream));
+    (void)hipFree(p->seg);
+    p->seg = nullptr;
+    p->seg_bytes = 0;
+    if (hipMalloc((void**)&p->seg, need) != hipSuccess)
+      return fail(CDD_ERR_HIP, ("hipMalloc failed (segment workspace of " + std::to_string(need) +
+                                " bytes; use shorter checkpoint intervals)").c_str());
+    p->seg_bytes = need;
   }
-  HIPCHK(launch_clock_segment(p->clk, 1, 0.0, nullptr, 0, p->stream));
-  // reverse, last step first
+  return lseg_alloc(p, nsteps);
+}
+
+// Reverse sweep over a stored segment (last step first): the cotangent of
+// the segment's end state ubar_end is carried to ubar0, the per-block Lambda
+// cotangents accumulate into the plan's Lambda accumulator.
+static int segment_reverse(cdd_plan* p, const SegLayout& L, const double* u0, const double* dts_dev, int nsteps,
+                           const double* src_dev, const double* ubar_end, double* ubar0) {
+  const int S = p->cfg.rk_order;
   const int nb = flux_vjp_nblocks(p->d, p->n);
   double *A = adjx(p, 0), *B = adjx(p, 1), *F0b = adjx(p, 2), *F1b = adjx(p, 3);
   const double* gin = ubar_end;
@@ -829,23 +829,93 @@ int cdd_segment_vjp(cdd_plan* p, const double* u0, double t0, const double* dts_
     double* gout = j == 0 ? ubar0 : (gin == A ? B : A);
     const double* dtp = dts_dev + j;
     double* lp = p->lseg + (size_t)j * S * nb;
-    const double* u = j == 0 ? u0 : st(j, 0);
+    const double* u = j == 0 ? u0 : L.st(j, 0);
     // the objective's own gradient at u_j (a uniform 9-vector) joins the last stage
     const double* sj = src_dev ? src_dev + 9 * (size_t)j : nullptr;
     if (S == 2) {
       // u' = u/2 + (F0 + dt L(F0))/2,  F0 = u + dt L(u)
-      RCHK(stage_vjp(p, st(j, 1), gs(j, 1), gin, 0.5, dtp, lp + nb, F0b, nullptr, 1, gin, 0.5));
-      RCHK(stage_vjp(p, u, gs(j, 0), F0b, 1.0, dtp, lp, gout, sj, 2, gin, 0.5, F0b, 1.0));
+      RCHK(stage_vjp(p, L.st(j, 1), L.gs(j, 1), gin, 0.5, dtp, lp + nb, F0b, nullptr, 1, gin, 0.5));
+      RCHK(stage_vjp(p, u, L.gs(j, 0), F0b, 1.0, dtp, lp, gout, sj, 2, gin, 0.5, F0b, 1.0));
     } else {
       // u' = u/3 + 2/3 (F1 + dt L(F1)),  F1 = 3/4 u + 1/4 (F0 + dt L(F0))
-      RCHK(stage_vjp(p, st(j, 2), gs(j, 2), gin, 2.0 / 3.0, dtp, lp + 2 * nb, F1b, nullptr, 1, gin, 2.0 / 3.0));
-      RCHK(stage_vjp(p, st(j, 1), gs(j, 1), F1b, 0.25, dtp, lp + nb, F0b, nullptr, 1, F1b, 0.25));
-      RCHK(stage_vjp(p, u, gs(j, 0), F0b, 1.0, dtp, lp, gout, sj, 3, gin, 1.0 / 3.0, F1b, 0.75, F0b, 1.0));
+      RCHK(stage_vjp(p, L.st(j, 2), L.gs(j, 2), gin, 2.0 / 3.0, dtp, lp + 2 * nb, F1b, nullptr, 1, gin, 2.0 / 3.0));
+      RCHK(stage_vjp(p, L.st(j, 1), L.gs(j, 1), F1b, 0.25, dtp, lp + nb, F0b, nullptr, 1, F1b, 0.25));
+      RCHK(stage_vjp(p, u, L.gs(j, 0), F0b, 1.0, dtp, lp, gout, sj, 3, gin, 1.0 / 3.0, F1b, 0.75, F0b, 1.0));
     }
     gin = gout;
   }
   HIPCHK(launch_reduce_sum(p->lseg, S * nsteps * nb, p->scalar + 3, p->stream, 1));
   return CDD_OK;
+}
+
+int cdd_segment_vjp(cdd_plan* p, const double* u0, double t0, const double* dts_dev, int nsteps,
+                    const double* src_dev, const double* ubar_end, double* ubar0) {
+  if (!p || !u0 || !dts_dev || !ubar_end || !ubar0 || nsteps < 1) return fail(CDD_ERR_ARG, "bad argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  RCHK(adj_alloc(p));
+  RCHK(seg_alloc(p, nsteps));
+  const int S = p->cfg.rk_order;
+  const SegLayout L{p->seg, S, p->cells, nsteps};
+  double* ufinal = L.gs(nsteps, 0);
+  // recompute the segment from u0, replaying the recorded time steps and
+  // keeping every stage state and its stress planes
+  HIPCHK(launch_clock_segment(p->clk, 0, t0, dts_dev, nsteps, p->stream));
+  for (int j = 0; j < nsteps; ++j) {
+    StepBufs b;
+    b.u = j == 0 ? u0 : L.st(j, 0);
+    b.out = j + 1 < nsteps ? L.st(j + 1, 0) : ufinal;
+    for (int k = 0; k < 3; ++k) b.G[k] = L.gs(j, k < S ? k : S - 1);
+    b.F0 = L.st(j, 1);
+    b.F1 = S == 3 ? L.st(j, 2) : nullptr;
+    RCHK(launch_step(p, b));
+  }
+  HIPCHK(launch_clock_segment(p->clk, 1, 0.0, nullptr, 0, p->stream));
+  return segment_reverse(p, L, u0, dts_dev, nsteps, src_dev, ubar_end, ubar0);
+}
+
+int cdd_segment_store_bytes(cdd_plan* p, int nsteps, unsigned long long* bytes) {
+  if (!p || !bytes || nsteps < 1) return fail(CDD_ERR_ARG, "bad argument");
+  *bytes = (unsigned long long)SegLayout::bytes(nsteps, p->cfg.rk_order, p->cells);
+  return CDD_OK;
+}
+
+int cdd_segment_forward(cdd_plan* p, int nsteps, void* store, unsigned long long store_bytes) {
+  if (!p || !store || nsteps < 1) return fail(CDD_ERR_ARG, "bad argument");
+  if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "stored trajectories are float64 only");
+  if (store_bytes < SegLayout::bytes(nsteps, p->cfg.rk_order, p->cells))
+    return fail(CDD_ERR_ARG, "segment store too small (cdd_segment_store_bytes)");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  const int S = p->cfg.rk_order;
+  const SegLayout L{static_cast<double*>(store), S, p->cells, nsteps};
+  // the segment's start state is kept in the store; the last step writes the
+  // buffer cdd_step would have ended on (RK2 ping-pongs by step parity)
+  const int ci = cur_state(p);
+  const int cf = S == 3 ? ci : ci ^ (nsteps & 1);
+  HIPCHK(hipMemcpyAsync(L.st(0, 0), p->state[ci], sizeof(double) * 9 * (size_t)p->cells, hipMemcpyDeviceToDevice,
+                        p->stream));
+  for (int j = 0; j < nsteps; ++j) {
+    StepBufs b;
+    b.u = L.st(j, 0);
+    b.out = j + 1 < nsteps ? L.st(j + 1, 0) : p->state[cf];
+    for (int k = 0; k < 3; ++k) b.G[k] = L.gs(j, k < S ? k : S - 1);
+    b.F0 = L.st(j, 1);
+    b.F1 = S == 3 ? L.st(j, 2) : nullptr;
+    RCHK(launch_step(p, b));
+  }
+  p->steps_launched += nsteps;
+  return CDD_OK;
+}
+
+int cdd_segment_vjp_stored(cdd_plan* p, const void* store, unsigned long long store_bytes, const double* dts_dev,
+                           int nsteps, const double* src_dev, const double* ubar_end, double* ubar0) {
+  if (!p || !store || !dts_dev || !ubar_end || !ubar0 || nsteps < 1) return fail(CDD_ERR_ARG, "bad argument");
+  if (store_bytes < SegLayout::bytes(nsteps, p->cfg.rk_order, p->cells))
+    return fail(CDD_ERR_ARG, "segment store too small (cdd_segment_store_bytes)");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  RCHK(adj_alloc(p));
+  RCHK(lseg_alloc(p, nsteps));
+  const SegLayout L{const_cast<double*>(static_cast<const double*>(store)), p->cfg.rk_order, p->cells, nsteps};
+  return segment_reverse(p, L, L.st(0, 0), dts_dev, nsteps, src_dev, ubar_end, ubar0);
 }
 
 int cdd_lambda_bar(cdd_plan* p, double* value, int reset) {

@@ -216,7 +216,20 @@ int cdd_timing_read(cdd_plan* plan, double* ms, int* counts, int nkinds);
  *                    15 rk_order planes per step.
  * All loads are supported; the uniaxial strain-rate control's dependence on
  * <beta_zz> is differentiated (a uniform term in betabar_zz).
- *   cdd_lambda_bar:  read (nullable) and optionally reset that accumulator. */
+ *   cdd_lambda_bar:  read (nullable) and optionally reset that accumulator.
+ * Stored trajectories (no counterpart in the reference, which has no
+ * adjoint): when device memory allows, the forward run keeps each segment's
+ * stage states so the reverse pass needs no recomputation.
+ *   cdd_segment_store_bytes: device bytes of one nsteps segment's store
+ *                    (15 rk_order planes per step).
+ *   cdd_segment_forward: advance the plan's state by nsteps steps exactly as
+ *                    cdd_step (CFL or fixed dt, dt record, loads, logs), and
+ *                    keep the segment's start state, every stage state and
+ *                    its stress planes in `store` (caller-owned device
+ *                    memory).  The segment must not reach the end time.
+ *   cdd_segment_vjp_stored: the reverse pass of cdd_segment_vjp over a
+ *                    segment kept by cdd_segment_forward (same dts_dev,
+ *                    src_dev, ubar_end, ubar0 and Lambda accumulator). */
 int cdd_flux_vjp(cdd_plan* plan, const double* beta_dev, double t, const double* rbar_dev, double* betabar_dev,
                  double* lambar);
 int cdd_step_vjp(cdd_plan* plan, const double* u_dev, double t, double dt, const double* ubar_next_dev,
@@ -227,6 +240,11 @@ int cdd_dt_replay(cdd_plan* plan, const double* sched_dev, long long len);
 int cdd_segment_vjp(cdd_plan* plan, const double* u0_dev, double t0, const double* dts_dev, int nsteps,
                     const double* src_dev, const double* ubar_end_dev, double* ubar0_dev);
 int cdd_lambda_bar(cdd_plan* plan, double* value, int reset);
+int cdd_segment_store_bytes(cdd_plan* plan, int nsteps, unsigned long long* bytes);
+int cdd_segment_forward(cdd_plan* plan, int nsteps, void* store_dev, unsigned long long store_bytes);
+int cdd_segment_vjp_stored(cdd_plan* plan, const void* store_dev, unsigned long long store_bytes,
+                           const double* dts_dev, int nsteps, const double* src_dev, const double* ubar_end_dev,
+                           double* ubar0_dev);
 
 /* ---- Batched independent realisations (2D) ------------------------------
  * The reference advances independent realisations one process per GPU

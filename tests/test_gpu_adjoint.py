@@ -323,3 +323,62 @@ def test_mean_stress_window_matches_torch_autograd(L):
     assert abs(J.item() - Jt.item()) <= 1e-12 * abs(Jt.item())
     assert rel(b.grad.cpu().numpy(), bt.grad.numpy()) < 1e-9
     assert abs(lt.grad.item() - ltt.grad.item()) <= 1e-9 * abs(ltt.grad.item())
+
+
+@pytest.mark.parametrize("order,k", [(2, 3), (2, 4), (3, 3)])
+def test_segment_forward_equals_step(L, order, k):
+    """segment_forward advances the plan exactly like step (state, clock and
+    the RK2 ping-pong buffer, odd and even segment lengths), and its store
+    starts with the segment's start state."""
+    say("segment_forward vs step", order, k)
+    n = 16
+    beta0 = SpectralGaussianField((n, n), 2.0, 0.01, seed=61)
+    outs = []
+    for stored in (False, True):
+        p = L.Plan(2, n, dynamics=0, rk_order=order, Lambda=0.9, mu=MU, nu=NU, **SHEAR)
+        p.set_state(beta0)
+        p.set_time(0.0)
+        p.step(1)
+        start = p.get_state()
+        if stored:
+            st = torch.empty(p.segment_store_bytes(k), dtype=torch.uint8, device="cuda")
+            p.segment_forward(k, st)
+            assert torch.equal(st[: start.numel() * 8].view(torch.float64).reshape(start.shape), start)
+        else:
+            p.step(k)
+        p.step(2)  # the plan keeps stepping from the right buffer
+        outs.append((p.get_state().cpu().numpy(), p.clock()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
+
+
+def test_stored_and_recomputed_segments_give_identical_gradients(L):
+    """Stored trajectories (no recomputation) and checkpoint recomputation
+    give bit-identical gradients, also with a budget that stores only some of
+    the segments; likewise for the time-distributed stress objective."""
+    say("stored vs recomputed segments")
+    from Plasticity.amd.autograd import run_with_grad, mean_stress_window
+    n, nsteps = 16, 7
+    beta0 = SpectralGaussianField((n, n), 2.0, 0.01, seed=62)
+    one = L.Plan(2, n, dynamics=0, rk_order=2, Lambda=0.9, mu=MU, nu=NU).segment_store_bytes(3)
+    res = []
+    for store in (False, True, one + 1):
+        b = torch.from_numpy(beta0).cuda().requires_grad_(True)
+        lt = torch.tensor(0.9, dtype=torch.float64, device="cuda", requires_grad=True)
+        final = run_with_grad(b, lt, nsteps, checkpoint_every=3, rk_order=2, load=SHEAR, store=store)
+        (final * final).sum().backward()
+        res.append((final.detach().cpu().numpy(), b.grad.cpu().numpy(), lt.grad.item()))
+    for r in res[1:]:
+        assert np.array_equal(r[0], res[0][0])
+        assert np.array_equal(r[1], res[0][1])
+        assert r[2] == res[0][2]
+    res = []
+    for store in (False, True):
+        b = torch.from_numpy(beta0).cuda().requires_grad_(True)
+        lt = torch.tensor(0.9, dtype=torch.float64, device="cuda", requires_grad=True)
+        J = mean_stress_window(b, lt, 6, 3, "zz", t0=0.1, checkpoint_every=4, rk_order=3, load=UNIAX, store=store)
+        J.backward()
+        res.append((J.item(), b.grad.cpu().numpy(), lt.grad.item()))
+    assert res[0][0] == res[1][0]
+    assert np.array_equal(res[0][1], res[1][1])
+    assert res[0][2] == res[1][2]
