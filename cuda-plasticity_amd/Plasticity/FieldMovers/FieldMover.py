@@ -62,3 +62,49 @@ class TVDRungeKutta_FieldMover(FieldMover):
         self.lastState = PlasticityState(state.gridShape, field=TensorField(state.gridShape, data=p.get_state()),
                                          inherit=state)
         return self.lastState
+
+
+class OperatorSplittingTVDRK_FieldMover(TVDRungeKutta_FieldMover):
+    """TVD-RK mover of a VacancyState with BetaP_VacancyDynamics
+    (FieldMovers/FieldMover.py:318-380): each step advances beta^P with the
+    concentration c of the step start, then c += beta (tr beta^P' -
+    tr beta^P) and c diffuses by exp(-gamma dt k^2); the adaptive dt is
+    capped at dtBound (here the cap is applied; the plain mover leaves it
+    commented out).  The whole step runs on the device (cdd_set_vacancy).
+    The reference's fixed-step variant writes back the step-start beta^P
+    (:347, its F_2 commented out); here every step keeps the advanced one."""
+
+    def Run(self, state, observers, dynamics=None, startTime=0., endTime=10., fixedTimeStep=None):
+        from Plasticity.Fields.Fields import TensorScalarMixedField
+        from Plasticity.PlasticityStates.VacancyState import VacancyState
+        import torch
+        p = dynamics._vacancy_plan(state, cfl=float(self.CFLsafeFactor), rk_order=int(self.order),
+                                   dt_bound=float(self.dtBound) if self.dtBound else None)
+        betaPstate, c = state.DecoupleState()
+        p.set_state(betaPstate.betaP.data)
+        p.vacancy_set(c)
+        p.set_time(float(startTime), float(endTime), float(fixedTimeStep) if fixedTimeStep else 0.0)
+        steps0 = p.clock()[3]
+
+        def snapshot():
+            data = torch.cat([p.vacancy_get()[None], p.get_state()])
+            return VacancyState(state.gridShape, field=TensorScalarMixedField(state.gridShape, data=data),
+                                inherit=state)
+        if observers:
+            while True:
+                p.step(1)
+                t, dt, done, ns = p.clock()
+                cur = snapshot()
+                for observer in observers:
+                    observer.Update(t, cur)
+                if done:
+                    break
+        else:
+            while True:
+                p.step(self.chunk)
+                t, dt, done, ns = p.clock()
+                if done:
+                    break
+        self.numberOfSteps = p.clock()[3] - steps0
+        self.lastState = snapshot()
+        return self.lastState

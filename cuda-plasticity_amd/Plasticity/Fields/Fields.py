@@ -102,3 +102,16 @@ class TensorField(Field):
         arr = np.ascontiguousarray(arr, dtype=np.float64)
         dev = device if device is not None else torch.device("cuda", 0)
         return cls(arr.shape[1:], data=torch.from_numpy(arr).to(dev))
+
+
+MIXED_COMPONENTS = list(TENSOR_COMPONENTS) + [('s', 's')]
+
+
+class TensorScalarMixedField(Field):
+    """Nine tensor components plus the scalar ('s', 's') of a VacancyState
+    (Fields.py:348-358); in sorted order ('s', 's') is component 0."""
+
+    def __init__(self, gridShape, components=None, data=None, device=None, kspace=False):
+        if kspace:
+            raise NotImplementedError("k-space fields stay inside the HIP stress solve")
+        super(TensorScalarMixedField, self).__init__(gridShape, components or list(MIXED_COMPONENTS), data, device)

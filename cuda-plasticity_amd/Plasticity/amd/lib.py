@@ -50,7 +50,7 @@ EXPORTS = [
     "cdd_batch_time_flux", "cdd_flux_vjp", "cdd_step_vjp", "cdd_energy_grad", "cdd_dt_record", "cdd_dt_replay",
     "cdd_segment_vjp", "cdd_lambda_bar", "cdd_stress_record", "cdd_plas_append", "cdd_plas_resume",
     "cdd_energy_record", "cdd_status", "cdd_instability", "cdd_segment_store_bytes", "cdd_segment_forward",
-    "cdd_segment_vjp_stored",
+    "cdd_segment_vjp_stored", "cdd_set_vacancy", "cdd_vacancy_set", "cdd_vacancy_get",
 ]
 KERNEL_KINDS = ["r2c", "cols_fwd", "contract", "cols_inv", "c2r", "flux_stage1", "cfl_reduce",
                 "flux_stage2", "flux_stage3", "advance"]
@@ -113,6 +113,9 @@ def _load():
         "cdd_segment_forward": ([P, I, P, ctypes.c_ulonglong], I),
         "cdd_segment_vjp_stored": ([P, P, ctypes.c_ulonglong, P, I, P, P, P], I),
         "cdd_lambda_bar": ([P, dp, I], I),
+        "cdd_set_vacancy": ([P, I, D, D, D, D], I),
+        "cdd_vacancy_set": ([P, P, I], I),
+        "cdd_vacancy_get": ([P, P, I], I),
         "cdd_stress_record": ([P, P, ctypes.c_longlong], I),
         "cdd_energy_record": ([P, P, ctypes.c_longlong], I),
         "cdd_status": ([P, ctypes.POINTER(I)], I),
@@ -308,6 +311,29 @@ class Plan:
                 raise CDDError("src must be a contiguous float64 (nsteps, 9) tensor on the plan's device")
             sp = _ptr(src)
         check(lib.cdd_segment_vjp(self.h, _ptr(a), float(t0), _ptr(dts), int(dts.numel()), sp, _ptr(g), _ptr(out)))
+        return out
+
+    def set_vacancy(self, alpha=1.0, beta=1.0, gamma=1.0, dt_bound=None, enable=True):
+        """Vacancy coupling (BetaP_VacancyDynamics with the operator-splitting
+        mover): stress solves see sigma_ii - alpha c, and after every step
+        c += beta (tr beta_new - tr beta_old) and c diffuses by
+        exp(-gamma dt k^2); dt_bound caps the adaptive step (None: no cap).
+        c starts at zero (vacancy_set)."""
+        check(lib.cdd_set_vacancy(self.h, 1 if enable else 0, float(alpha), float(beta), float(gamma),
+                                  float(dt_bound) if dt_bound else 0.0))
+
+    def vacancy_set(self, c):
+        """Set the vacancy concentration field (gridShape, float64)."""
+        t = self.torch.as_tensor(c, dtype=self.torch.float64)
+        if tuple(t.shape) != self.shape:
+            raise CDDError("c must have the grid shape %s" % (self.shape,))
+        t = t.to(self.dev).contiguous()
+        check(lib.cdd_vacancy_set(self.h, ctypes.c_void_p(t.data_ptr()), 1))
+
+    def vacancy_get(self):
+        """The vacancy concentration field as a device tensor."""
+        out = self.torch.empty(self.shape, dtype=self.torch.float64, device=self.dev)
+        check(lib.cdd_vacancy_get(self.h, ctypes.c_void_p(out.data_ptr()), 1))
         return out
 
     def segment_store_bytes(self, nsteps):

@@ -76,6 +76,11 @@ struct cdd_plan {
   // per-step energy record: k_energy partials of the step-start state
   double* epart;
   bool elog_on;
+  // vacancy coupling (cdd_set_vacancy): the concentration field and the
+  // operator-splitting mover's parameters
+  double* vac;
+  int vacancy;
+  double valpha, vbeta, vgamma, dt_bound;
   // HIP-event timing of every kernel launched by cdd_step (optional)
   bool timing;
   std::vector<hipEvent_t>* ev;
@@ -261,6 +266,7 @@ int cdd_plan_destroy(cdd_plan* p) {
   if (p->host) (void)hipHostFree(p->host);
   (void)hipFree(p->epart);
   (void)hipFree(p->lseg);
+  (void)hipFree(p->vac);
   if (p->own) (void)hipStreamDestroy(p->own);
   if (p->ev) {
     for (hipEvent_t e : *p->ev) (void)hipEventDestroy(e);
@@ -356,6 +362,15 @@ static SpectralJob spectral_job(cdd_plan* p, int nc, const double* A, const doub
   return j;
 }
 
+// The vacancies' chemical stress joins the stepper's (and the dynamics
+// flux's) stress solves, not the state's CalculateSigma.
+static void vac_stress(const cdd_plan* p, SpectralJob& j) {
+  if (p->vacancy) {
+    j.vac = p->vac;
+    j.valpha = p->valpha;
+  }
+}
+
 static FluxJob flux_job(cdd_plan* p) {
   FluxJob f;
   memset(&f, 0, sizeof f);
@@ -400,6 +415,7 @@ int cdd_flux(cdd_plan* p, const void* beta_v, double t, void* rhs_v, void* speed
   if (!p || !beta || !rhs) return fail(CDD_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(p->cfg.device));
   SpectralJob j = spectral_job(p, 6, beta, nullptr, nullptr, t, p->G);
+  vac_stress(p, j);
   HIPCHK(launch_spectral(j));
   FluxJob f = flux_job(p);
   f.A = beta;
@@ -430,6 +446,7 @@ static int launch_step(cdd_plan* p, const StepBufs& b) {
   {
     SpectralJob j1 = spectral_job(p, 6, u, nullptr, p->clk, 0.0, b.G[0]);
     j1.gp.log_stress = 1;
+    vac_stress(p, j1);
     HIPCHK(launch_spectral(j1));
   }
   if (p->elog_on) {  // energy record of the step-start state (loads included)
@@ -447,13 +464,14 @@ static int launch_step(cdd_plan* p, const StepBufs& b) {
   int nb = 0;
   HIPCHK(launch_flux(f, &nb));
   if (p->timing) timing_hook(p, KK_FLUX1);
-  HIPCHK(launch_finalize_dt(p->partial, nb, p->clk, nmax, p->stream));
+  HIPCHK(launch_finalize_dt(p->partial, nb, p->clk, nmax, p->stream, 1, 0, p->vacancy ? p->dt_bound : 0.0));
   if (p->timing) timing_hook(p, KK_FINALIZE);
   // stage 2 on F0 = u + dt L0: the R2C pass forms F0 while reading u and
   // L0 for the stress solve and stores it in F0 (which may be L0 itself)
   {
     SpectralJob j2 = spectral_job(p, 6, u, p->L0, p->clk, 0.0, b.G[1]);
     j2.src.W = b.F0;
+    vac_stress(p, j2);
     HIPCHK(launch_spectral(j2));
   }
   FluxJob f2 = flux_job(p);
@@ -469,7 +487,9 @@ static int launch_step(cdd_plan* p, const StepBufs& b) {
     HIPCHK(launch_flux(f2, nullptr));
     if (p->timing) timing_hook(p, KK_FLUX2);
     // stage 3 on F1
-    HIPCHK(launch_spectral(spectral_job(p, 6, b.F1, nullptr, p->clk, 0.0, b.G[2])));
+    SpectralJob j3 = spectral_job(p, 6, b.F1, nullptr, p->clk, 0.0, b.G[2]);
+    vac_stress(p, j3);
+    HIPCHK(launch_spectral(j3));
     FluxJob f3 = flux_job(p);
     f3.G = b.G[2];
     f3.clk = p->clk;
@@ -479,14 +499,29 @@ static int launch_step(cdd_plan* p, const StepBufs& b) {
     f3.mode = 1;
     f3.cu = 1.0 / 3.0;
     f3.cf = 2.0 / 3.0;
+    if (p->vacancy) {
+      f3.vac = p->vac;
+      f3.vbeta = p->vbeta;
+    }
     HIPCHK(launch_flux(f3, nullptr));
     if (p->timing) timing_hook(p, KK_FLUX3);
   } else {
     f2.out = b.out;
     f2.cu = 0.5;
     f2.cf = 0.5;
+    if (p->vacancy) {
+      f2.vac = p->vac;
+      f2.vbeta = p->vbeta;
+    }
     HIPCHK(launch_flux(f2, nullptr));
     if (p->timing) timing_hook(p, KK_FLUX2);
+  }
+  if (p->vacancy) {
+    // operator splitting: diffuse the vacancies over the step
+    // (DiffusionEvolution, VacancyDynamics.py:40-70), in place
+    SpectralJob jd = spectral_job(p, 1, p->vac, nullptr, p->clk, 0.0, p->vac);
+    jd.gp.vgamma = p->vgamma;
+    HIPCHK(launch_spectral(jd));
   }
   HIPCHK(launch_advance(p->clk, p->stream));
   if (p->timing) timing_hook(p, KK_ADVANCE);
@@ -511,6 +546,50 @@ int cdd_step(cdd_plan* p, int nsteps) {
   return CDD_OK;
 }
 
+int cdd_set_vacancy(cdd_plan* p, int enable, double alpha, double beta, double gamma, double dt_bound) {
+  if (!p) return fail(CDD_ERR_ARG, "null plan");
+  if (!enable) {
+    p->vacancy = 0;
+    return CDD_OK;
+  }
+  if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "the vacancy coupling is float64 only");
+  if (!(gamma >= 0.0) || !(dt_bound >= 0.0) || !isfinite(alpha) || !isfinite(beta))
+    return fail(CDD_ERR_ARG, "bad vacancy parameters");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  if (!p->vac) {
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (hipMalloc((void**)&p->vac, sizeof(double) * (size_t)p->cells) != hipSuccess)
+      return fail(CDD_ERR_HIP, "hipMalloc failed (vacancy field)");
+    HIPCHK(hipMemsetAsync(p->vac, 0, sizeof(double) * (size_t)p->cells, p->stream));
+  }
+  p->vacancy = 1;
+  p->valpha = alpha;
+  p->vbeta = beta;
+  p->vgamma = gamma;
+  p->dt_bound = dt_bound;
+  return CDD_OK;
+}
+
+int cdd_vacancy_set(cdd_plan* p, const void* c, int on_device) {
+  if (!p || !c) return fail(CDD_ERR_ARG, "null argument");
+  if (!p->vac) return fail(CDD_ERR_ARG, "the vacancy coupling is not enabled (cdd_set_vacancy)");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  HIPCHK(hipMemcpyAsync(p->vac, c, sizeof(double) * (size_t)p->cells,
+                        on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return CDD_OK;
+}
+
+int cdd_vacancy_get(cdd_plan* p, void* c, int on_device) {
+  if (!p || !c) return fail(CDD_ERR_ARG, "null argument");
+  if (!p->vac) return fail(CDD_ERR_ARG, "the vacancy coupling is not enabled (cdd_set_vacancy)");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  HIPCHK(hipMemcpyAsync(c, p->vac, sizeof(double) * (size_t)p->cells,
+                        on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return CDD_OK;
+}
+
 int cdd_energy(cdd_plan* p, const void* beta_v, double t, double* energy) {
   const double* beta = static_cast<const double*>(beta_v);
   if (!p || !energy) return fail(CDD_ERR_ARG, "null argument");
@@ -531,6 +610,7 @@ int cdd_energy(cdd_plan* p, const void* beta_v, double t, double* energy) {
 // ------------------------------------------------------------- adjoint
 static int adj_alloc(cdd_plan* p) {
   if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "the adjoint is float64 only");
+  if (p->vacancy) return fail(CDD_ERR_UNSUPPORTED, "the adjoint does not include the vacancy coupling");
   if (p->cfg.dynamics == CDD_GLIDE_CLIMB_WOJZZ)
     return fail(CDD_ERR_UNSUPPORTED, "the adjoint supports the glide+climb and glide-only dynamics");
   if (p->adj) return CDD_OK;
@@ -882,6 +962,7 @@ int cdd_segment_store_bytes(cdd_plan* p, int nsteps, unsigned long long* bytes) 
 int cdd_segment_forward(cdd_plan* p, int nsteps, void* store, unsigned long long store_bytes) {
   if (!p || !store || nsteps < 1) return fail(CDD_ERR_ARG, "bad argument");
   if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "stored trajectories are float64 only");
+  if (p->vacancy) return fail(CDD_ERR_UNSUPPORTED, "the adjoint does not include the vacancy coupling");
   if (store_bytes < SegLayout::bytes(nsteps, p->cfg.rk_order, p->cells))
     return fail(CDD_ERR_ARG, "segment store too small (cdd_segment_store_bytes)");
   HIPCHK(hipSetDevice(p->cfg.device));

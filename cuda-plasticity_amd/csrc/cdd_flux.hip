@@ -501,8 +501,19 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
     for (int c = 0; c < 9; ++c) outp[c * cells + idx] = rhs[c];
   } else {
     const T cu = T(job.cu), cf = T(job.cf);
+    T ov[9], uv[9];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) outp[c * cells + idx] = cu * Uv(c) + cf * (fc[c] + dt * rhs[c]);
+    for (int c = 0; c < 9; ++c) {
+      uv[c] = Uv(c);
+      ov[c] = cu * uv[c] + cf * (fc[c] + dt * rhs[c]);
+      outp[c * cells + idx] = ov[c];
+    }
+    if constexpr (sizeof(T) == 8) {
+      // vacancy coupling after the last stage: c += beta (tr F2 - tr u)
+      // (OperatorSplittingTVDRK_FieldMover, FieldMover.py:351,375-376,
+      // VacancyDynamics.py:37-39)
+      if (job.vac) job.vac[idx] += job.vbeta * ((ov[0] + ov[4] + ov[8]) - (uv[0] + uv[4] + uv[8]));
+    }
   }
   if (speedp) speedp[idx] = speed;
   if (partialp) {
@@ -703,8 +714,16 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     T* o = outp + (long long)p * splane + xoff;
     if (comb) {
+      T ov[9], uv[9];
 #pragma unroll
-      for (int c = 0; c < 9; ++c) o[c * cells] = cu * ub[wv][c][lane] + cf * (fc[c] + dt * rhs[c]);
+      for (int c = 0; c < 9; ++c) {
+        uv[c] = ub[wv][c][lane];
+        ov[c] = cu * uv[c] + cf * (fc[c] + dt * rhs[c]);
+        o[c * cells] = ov[c];
+      }
+      // vacancy coupling after the last stage: c += beta (tr F2 - tr u)
+      if (job.vac)
+        job.vac[(long long)p * splane + xoff] += job.vbeta * ((ov[0] + ov[4] + ov[8]) - (uv[0] + uv[4] + uv[8]));
     } else {
 #pragma unroll
       for (int c = 0; c < 9; ++c) o[c * cells] = rhs[c];
@@ -837,7 +856,7 @@ hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
 // dt = CFL / (max(gridShape) max speed), clamped to the end time
 // (FieldMover.py:94-111).  One block.
 __global__ void __launch_bounds__(1024) k_finalize_dt(const double* partial, int nb, Clock* clk, double nmax,
-                                                     long long pstride) {
+                                                     long long pstride, double dt_bound) {
   // one block per instance
   partial += blockIdx.x * pstride;
   clk += blockIdx.x;
@@ -859,6 +878,9 @@ __global__ void __launch_bounds__(1024) k_finalize_dt(const double* partial, int
       return;
     }
     double dt = clk->fixed_dt > 0.0 ? clk->fixed_dt : clk->cfl / (nmax * mm);
+    // the operator-splitting mover's bound on the adaptive step
+    // (FieldMover.py:359-360; the plain TVD mover leaves it commented out)
+    if (clk->fixed_dt <= 0.0 && dt_bound > 0.0 && dt > dt_bound) dt = dt_bound;
     const long long si = clk->nsteps - clk->sched_base;
     if (clk->dt_sched && si >= 0 && si < clk->sched_len) dt = clk->dt_sched[si];
     if (clk->t + dt > clk->t_end) dt = clk->t_end - clk->t;
@@ -948,9 +970,10 @@ __global__ void k_advance(Clock* clk, int count) {
 }
 
 hipError_t launch_finalize_dt(const double* partial, int nb, Clock* clk, double nmax, hipStream_t st, int count,
-                              long long pstride) {
+                              long long pstride, double dt_bound) {
   if (count < 1 || count > 1024) return hipErrorInvalidValue;
-  if (partial) hipLaunchKernelGGL(k_finalize_dt, dim3(count), dim3(1024), 0, st, partial, nb, clk, nmax, pstride);
+  if (partial)
+    hipLaunchKernelGGL(k_finalize_dt, dim3(count), dim3(1024), 0, st, partial, nb, clk, nmax, pstride, dt_bound);
   else hipLaunchKernelGGL(k_fixed_dt, dim3(1), dim3(count), 0, st, clk, count);
   return hipGetLastError();
 }

@@ -25,6 +25,7 @@ struct GreenParams {
   LoadParams load;
   Elastic el;
   int log_stress;   // 1: the k = 0 thread appends the clock's stress-strain record
+  double vgamma;    // nc = 1: diffusion filter exp(-vgamma dt k^2) (dt from clk)
 };
 
 // Called after each kernel launch with the kernel kind (HIP-event timing).
@@ -52,6 +53,10 @@ struct SpectralJob {
   long long ostride;     // out (elements)
   int trans;             // 1: transposed Green's contraction (adjoint), no load mode
   int fp32;              // 1: src, spec, tw and out hold float data (src typed as a carrier)
+  // vacancy coupling (fp64, nc = 6): the diagonal stress planes of out get
+  // -valpha * vac (the vacancies' chemical stress)
+  const double* vac;
+  double valpha;
 };
 
 hipError_t launch_spectral(const SpectralJob& job);
@@ -80,12 +85,16 @@ struct FluxJob {
   long long fstride, gstride, pstride;
   int tile;            // flux tile variant (0: default, 1: 64x4, 2: 16x16)
   int fp32;            // 1: A, G, U, out, speed hold float data (partial stays double)
+  // vacancy coupling (fp64, mode 1, the step's last stage): vac += vbeta
+  // (tr out - tr U) at every cell
+  double* vac;
+  double vbeta;
 };
 
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out);
 int flux_nblocks(int d, int n);
 hipError_t launch_finalize_dt(const double* partial, int nblocks, Clock* clk, double nmax, hipStream_t st,
-                              int count = 1, long long pstride = 0);
+                              int count = 1, long long pstride = 0, double dt_bound = 0.0);
 hipError_t launch_advance(Clock* clk, hipStream_t st, int count = 1);
 int flux_nblocks_tile(int d, int n, int tile);
 hipError_t launch_energy(const double* G, const double* beta, long long cells, double mu, double nu, double lam,

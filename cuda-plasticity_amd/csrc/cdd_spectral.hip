@@ -175,7 +175,8 @@ k_r2c_last(SrcField<T> src, cx<T>* __restrict__ spec, const cx<T>* __restrict__ 
 template <typename T, int N>
 __global__ void __launch_bounds__(Tiles<N>::TY * Tiles<N>::TT)
 k_c2r_last(const cx<T>* __restrict__ spec, T* __restrict__ out, long long cs, const cx<T>* __restrict__ tw,
-           int NO, int NC, const Clock* clk, long long sstride, long long ostride) {
+           int NO, int NC, const Clock* clk, long long sstride, long long ostride, const T* __restrict__ vac,
+           T valpha) {
   using TL = Tiles<N>;
   constexpr int TT = TL::TT, TY = TL::TY, P = TL::P, V = FFTPlan<N>::V, H = N / 2 + 1;
   const int inst = (NO == 1) ? (int)blockIdx.y : 0;
@@ -222,6 +223,20 @@ k_c2r_last(const cx<T>* __restrict__ spec, T* __restrict__ out, long long cs, co
   RowExchange<T> ex{lds + r * P};
   fft_forward<T, N>(v, j, tw, ex);
   const long long rowbase = ((long long)o * N + (r0 + r)) * N;
+  if (vac && (ca < 3 || (hasb && cb < 3))) {
+    // vacancy coupling of the stepper's stress solves: the diagonal stress
+    // planes (xx, yy, zz) get -alpha c (VacancyDynamics.py:24-28)
+    T cv[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) cv[i] = valpha * vac[rowbase + fft_out_index<N>(j, i)];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const long long idx = rowbase + fft_out_index<N>(j, i);
+      out[ca * cs + idx] = ca < 3 ? v[i].x - cv[i] : v[i].x;
+      if (hasb) out[cb * cs + idx] = cb < 3 ? -v[i].y - cv[i] : -v[i].y;
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < V; ++i) {
     const long long idx = rowbase + fft_out_index<N>(j, i);
@@ -436,8 +451,14 @@ k_contract(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, GreenParams g
 #pragma unroll
         for (int cc = 0; cc < 6; ++cc) o[cc] = cx<T>{T(s[CS9[cc]]), T(0)};
       }
-    } else {
+    } else if constexpr (NC == 9) {
       green_rho<T>(in, o, kx, ky, kz, T(gp.norm));
+    } else {
+      // diffusion filter of the vacancy concentration (VacancyDynamics.py:
+      // 40-70): kc exp(kSq * -gamma * dt), normalised for the inverse pass
+      const T kSq = kx * kx + ky * ky + kz * kz;
+      const double dtv = gp.clk ? gp.clk->dt : 0.0;
+      o[0] = in[0] * (T(exp(((double)kSq * -gp.vgamma) * dtv)) * T(gp.norm));
     }
 #pragma unroll
     for (int cc = 0; cc < NC; ++cc) lds[(cc * TR + rr) * P + lds_pad(e)] = conj(o[cc]);
@@ -524,11 +545,18 @@ static hipError_t launch_spectral_t(const SpectralJob& job) {
     } else {
       hipLaunchKernelGGL((k_contract<T, N, 6, 2>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
     }
-  } else {
+  } else if (NC == 9) {
     using CT = ContractTiles<N, 9>;
     dim3 g3((unsigned)((NR + CT::TR - 1) / CT::TR), count), b3(9 * CT::TR * CT::TT);
     if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 9, 3>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
     else hipLaunchKernelGGL((k_contract<T, N, 9, 2>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+  } else if (NC == 1) {
+    using CT = ContractTiles<N, 1>;
+    dim3 g3((unsigned)((NR + CT::TR - 1) / CT::TR), count), b3(CT::TR * CT::TT);
+    if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 1, 3>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+    else hipLaunchKernelGGL((k_contract<T, N, 1, 2>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+  } else {
+    return hipErrorInvalidValue;
   }
   hook(KK_CONTRACT);
   if (d == 3) {
@@ -536,8 +564,9 @@ static hipError_t launch_spectral_t(const SpectralJob& job) {
     hipLaunchKernelGGL((k_fft_cols<T, N, true>), g2, b2, 0, st, spec, tw, job.gp.clk);
     hook(KK_COLS_INV);
   }
+  if (job.vac && (NC != 6 || sizeof(T) != 8)) return hipErrorInvalidValue;
   hipLaunchKernelGGL((k_c2r_last<T, N>), g1, b1, 0, st, spec, out, cs, tw, NO, NC, job.gp.clk, job.sstride,
-                     job.ostride);
+                     job.ostride, reinterpret_cast<const T*>(job.vac), T(job.valpha));
   hook(KK_C2R);
   return hipGetLastError();
 }

@@ -128,6 +128,27 @@ int cdd_flux(cdd_plan* plan, const void* beta_dev, double t, void* rhs_dev, void
  * synchronisation. */
 int cdd_step(cdd_plan* plan, int nsteps);
 
+/* Vacancy coupling: switches the plan's stepping to the operator-splitting
+ * mover of a VacancyState (OperatorSplittingTVDRK_FieldMover, FieldMover.py:
+ * 318-380, with BetaP_VacancyDynamics, VacancyDynamics.py:16-70; the CUDA
+ * executable's calculateKDiffusion, plasticity_kernel.cu:700-775).  Every
+ * stress solve of a step sees sigma_ii - alpha c with the concentration c of
+ * the step start (GetSigma); after the step c += beta (tr beta_new -
+ * tr beta_old) and c is diffused by exp(-gamma dt k^2) in k space
+ * (DiffusionEvolution); the adaptive dt is capped at dt_bound (> 0; the
+ * mover's dtBound) before the end-time clamp.  cdd_flux includes the
+ * -alpha c term (BetaP_VacancyDynamics.CalculateFlux); cdd_sigma does not
+ * (PlasticityState.CalculateSigma).  float64 plans only; the adjoint refuses
+ * such a plan.  c starts at zero; enable = 0 switches the coupling off and
+ * keeps c.  The reference's fixed-step variant (:322-348) recouples the
+ * step-start beta^P instead of the advanced one; here both keep the
+ * advanced state (its adaptive variant, :349-380).
+ *   cdd_vacancy_set / cdd_vacancy_get: the concentration field c (N^d
+ *   doubles, the VacancyState's ('s','s') component), host or device. */
+int cdd_set_vacancy(cdd_plan* plan, int enable, double alpha, double beta, double gamma, double dt_bound);
+int cdd_vacancy_set(cdd_plan* plan, const void* c, int on_device);
+int cdd_vacancy_get(cdd_plan* plan, void* c, int on_device);
+
 /* Total elastic energy sum_cells 1/2 S sigma sigma of beta (nullable: the
  * plan's state) at time t (PlasticityState.py:317-330, Observer.py:86). */
 int cdd_energy(cdd_plan* plan, const void* beta_dev, double t, double* energy);

@@ -692,14 +692,18 @@ def cfl_dt(speed, shape, cfl):
     return cfl / (max(shape) * speed.max())
 
 
-def tvd_rk3_step(flux, t, u, t_end, cfl, fixed_dt=None):
+def tvd_rk3_step(flux, t, u, t_end, cfl, fixed_dt=None, dt_bound=None):
     """CalculateOneAdaptiveTimeStep (FieldMover.py:61-141) or, with fixed_dt,
     CalculateOneFixedTimeStep (:38-58).  flux(t, u) -> (rhs, speed).
     Every stage is evaluated at the step's start time (:123,133).
+    dt_bound caps the adaptive step (the operator-splitting mover,
+    FieldMover.py:359-360; the plain mover has it commented out, :107-108).
     Returns (u_new, dt).  Pinned by the RK3 amplification factor.
     """
     rhs, speed = flux(t, u)
     dt = fixed_dt if fixed_dt is not None else cfl_dt(speed, u.shape[1:], cfl)
+    if fixed_dt is None and dt_bound is not None and dt > dt_bound:
+        dt = dt_bound
     if t + dt > t_end:
         dt = t_end - t
     L0 = dt * rhs
@@ -715,7 +719,7 @@ def tvd_rk3_step(flux, t, u, t_end, cfl, fixed_dt=None):
     return F2, dt
 
 
-def tvd_rk2_step(flux, t, u, t_end, cfl, fixed_dt=None):
+def tvd_rk2_step(flux, t, u, t_end, cfl, fixed_dt=None, dt_bound=None):
     """SSP (TVD) RK2 -- no reference counterpart; same CFL rule and stage-time
     convention as the reference RK3 (FieldMover.py:94-111,123):
     u1 = u + dt L(u);  u' = u/2 + (u1 + dt L(u1))/2.
@@ -723,6 +727,8 @@ def tvd_rk2_step(flux, t, u, t_end, cfl, fixed_dt=None):
     """
     rhs, speed = flux(t, u)
     dt = fixed_dt if fixed_dt is not None else cfl_dt(speed, u.shape[1:], cfl)
+    if fixed_dt is None and dt_bound is not None and dt > dt_bound:
+        dt = dt_bound
     if t + dt > t_end:
         dt = t_end - t
     u1 = u + dt * rhs
@@ -750,3 +756,75 @@ def run(dyn, beta, t0, nsteps, cfl, order=3, t_end=np.inf, fixed_dt=None, observ
         if t >= t_end:
             break
     return u, t, np.array(dts)
+
+
+# ---------------------------------------------------------------------------
+# Vacancy coupling: BetaP_VacancyDynamics (FieldDynamics/VacancyDynamics.py:
+# 16-70) advanced by OperatorSplittingTVDRK_FieldMover (FieldMovers/
+# FieldMover.py:318-380)
+# ---------------------------------------------------------------------------
+class Vacancy:
+    """alpha: chemical stress sigma_ii -= alpha c (GetSigma, :24-28); beta:
+    c += beta (tr beta_new - tr beta_old) after a step (CalculateVFlux,
+    :37-39); gamma: diffusion constant (DiffusionEvolution, :40-70);
+    dt_bound: the mover's dtBound (FieldMover.py:359-360), None for none."""
+
+    def __init__(self, alpha=1.0, beta=1.0, gamma=1.0, dt_bound=None):
+        self.alpha = alpha
+        self.beta = beta
+        self.gamma = gamma
+        self.dt_bound = dt_bound
+
+
+def vacancy_diffuse(c, dt, gamma):
+    """DiffusionEvolution (VacancyDynamics.py:40-70):
+    irfftn(rfftn(c) exp(kSq * -gamma * dt)).  Pinned by: a single Fourier
+    mode decays by exp(-gamma dt k^2) exactly, the mean is kept
+    (tests/test_oracle_vacancy.py)."""
+    kt = ktools(c.shape)
+    kc = rfftn(c)
+    k = kt.kSq * -gamma * dt
+    return irfftn(kc * np.exp(k), c.shape)
+
+
+def vacancy_sigma(dyn, vac, beta, c, t):
+    """GetSigma of the vacancy dynamics (VacancyDynamics.py:24-28; the loaded
+    variant RunPython.py:93-101 adds the load afterwards): sigma of the
+    state, sigma_ii -= alpha c, then the load."""
+    s = sigma(beta, dyn.mu, dyn.nu).copy()
+    for i in range(3):
+        s[cidx(i, i)] = s[cidx(i, i)] - vac.alpha * c
+    return dyn.load.apply(s, beta, t, dyn.mu, dyn.nu)
+
+
+def run_vacancy(dyn, vac, beta, c, t0, nsteps, cfl, order=3, t_end=np.inf, fixed_dt=None):
+    """OperatorSplittingTVDRK_FieldMover.Run restricted to nsteps steps
+    (FieldMover.py:20-36 loop, CalculateOneAdaptiveTimeStep :349-380): every
+    stage's flux sees the stress of vacancy_sigma with the concentration of
+    the step start; dt from the first stage's speeds capped at dtBound and
+    clamped to the end time; after the step c += beta (tr F2 - tr u), then c
+    diffuses over dt.  The fixed-step variant (:322-348) recouples the
+    step-start beta^P (apparently unintended: its F_2 is commented out);
+    here both variants keep the advanced state.  Returns (u, c, t, dts).
+    Pinned by tests/test_oracle_vacancy.py (diffusion closed form, stress
+    shift, trace balance, Lambda = 1 trace conservation, dt cap)."""
+    if np.shape(c) != np.shape(beta)[1:]:
+        raise ValueError("c must have the grid shape")
+    step = tvd_rk3_step if order == 3 else tvd_rk2_step
+    t, u, dts = t0, beta, []
+    for _ in range(nsteps):
+        cs = c
+
+        def f(tt, uu):
+            return central_upwind_flux(uu, vacancy_sigma(dyn, vac, uu, cs, tt), dyn.dynamics, dyn.Lambda)
+
+        prev = u[cidx(X, X)] + u[cidx(Y, Y)] + u[cidx(Z, Z)]
+        u, dt = step(f, t, u, t_end, cfl, fixed_dt, dt_bound=vac.dt_bound)
+        tr = u[cidx(X, X)] + u[cidx(Y, Y)] + u[cidx(Z, Z)]
+        c = c + vac.beta * (tr - prev)
+        c = vacancy_diffuse(c, dt, vac.gamma)
+        t += dt
+        dts.append(dt)
+        if t >= t_end:
+            break
+    return u, c, t, np.array(dts)
