@@ -533,8 +533,7 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
 // While plane p is computed, the next plane's halo tile (all nine
 // components), its six stress planes G, this plane's U cells and the cells
 // three planes ahead stream into LDS by asynchronous global->LDS DMA; the
-// latter refill a register ring that holds this thread's cell at planes
-// p-2, p-1, p+1, p+2.  Each plane is read once per column (plus its halo),
+// slow-axis derivatives are carried in registers from plane to plane.  Each plane is read once per column (plus its halo),
 // and a block's load latency hides behind the previous plane's arithmetic:
 // the only wait in the loop follows the arithmetic, when the outputs
 // stored one step earlier have long completed.
@@ -559,7 +558,7 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
   if (job.clk && job.clk->done) return;
   __shared__ T tile[9 * TILE];
   __shared__ T vsh[4 * 8][NT];
-  __shared__ T gb[2][NT / 64][6][64];  // stress planes, double buffered
+  __shared__ T gb[NT / 64][6][64];     // stress planes
   __shared__ T ub[NT / 64][10][64];    // U cells (plane 9 repeats component 8)
   __shared__ T nbuf[NT / 64][10][64];  // the ring's next plane, same layout
   __shared__ T red[NT / 64];
@@ -612,8 +611,8 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
     const T* base = Gp + (long long)p * splane + poff;
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-      __builtin_amdgcn_global_load_lds((const void*)(base + (2 * k + half) * cells), (lds_void*)&gb[p & 1][wv][2 * k][0],
-                                       16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(base + (2 * k + half) * cells), (lds_void*)&gb[wv][2 * k][0], 16,
+                                       0, 0);
   };
   auto dma_u = [&](int p) {
     const T* base = Up + (long long)p * splane + poff;
@@ -637,14 +636,31 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
     for (int c = 0; c < 9; ++c) v[c] = base[c * cells];
   };
 
-  // prologue: plane x0's tile and stress, the ring around it
+  // Slow-axis derivatives are carried from plane to plane: with
+  // D_k = f(k+1) - f(k), plane p needs D_{p+1} = f(p+2) - f(p+1), the
+  // carried f(p+1), D_p, s_p = D_p - D_{p-1} and u-(p) = D_{p-1} + m_{p-1}/2;
+  // it forms m_p = minmod(D_{p+1} - D_p, s_p) and u+(p) = D_p - m_p/2 and
+  // hands on D_{p+1} - D_p and D_p + m_p/2.  Every value comes from the same
+  // operands by the same operation as in limited_derivs, so the result is
+  // bit-identical to it.  f(p+2) arrives in nbuf one plane ahead.
   dma_tile(x0);
   dma_g(x0);
-  T xr[4][9];
-  center(x0 - 2, xr[0]);
-  center(x0 - 1, xr[1]);
-  center(x0 + 1, xr[2]);
-  center(x0 + 2, xr[3]);
+  dma_next(x0 + 2);
+  T xf[9], xd[9], xs[9], xum[9];
+  {
+    T fm2[9], fm1[9], f0[9];
+    center(x0 - 2, fm2);
+    center(x0 - 1, fm1);
+    center(x0, f0);
+    center(x0 + 1, xf);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      const T Dm2 = fm1[c] - fm2[c], Dm1 = f0[c] - fm1[c];
+      xd[c] = xf[c] - f0[c];
+      xs[c] = xd[c] - Dm1;
+      xum[c] = fma(T(0.5), minmod2(xs[c], Dm1 - Dm2), Dm1);
+    }
+  }
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   T run_max = -INFINITY;
   const int lr = trr + 2, lc = tc + 2;
@@ -667,15 +683,22 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
         T f[5] = {tb[(lr - 2) * PW + lc], tb[(lr - 1) * PW + lc], ctr, tb[(lr + 1) * PW + lc], tb[(lr + 2) * PW + lc]};
         limited_derivs<T>(f, u[1][0][c], u[1][1][c]);
       }
-      {
-        T f[5] = {xr[0][c], xr[1][c], ctr, xr[2][c], xr[3][c]};
-        limited_derivs<T>(f, u[0][0][c], u[0][1][c]);
+      {  // slow axis, from the carried differences
+        const T D1 = nbuf[wv][c][lane] - xf[c];
+        const T s1 = D1 - xd[c];
+        const T m = minmod2(s1, xs[c]);
+        u[0][0][c] = fma(T(-0.5), m, xd[c]);
+        u[0][1][c] = xum[c];
+        xum[c] = fma(T(0.5), m, xd[c]);
+        xf[c] = nbuf[wv][c][lane];
+        xs[c] = s1;
+        xd[c] = D1;
       }
     }
     T sg[9];
     {
       const T tr = fc[0] + fc[4] + fc[8];
-      const T* g = &gb[p & 1][wv][0][lane];
+      const T* g = &gb[wv][0][lane];
       const T g0 = g[0], g1 = g[64], g2 = g[128], g3 = g[192], g4 = g[256], g5 = g[320];
       sg[0] = g0 - lam * tr - mu * (fc[0] + fc[0]);
       sg[4] = g1 - lam * tr - mu * (fc[4] + fc[4]);
@@ -695,22 +718,22 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
         for (int c = 0; c < 9; ++c) asm volatile("" : "+v"(u[a][s][c]));
 #pragma unroll
     for (int c = 0; c < 9; ++c) asm volatile("" : "+v"(sg[c]), "+v"(fc[c]));
-    // every wave is done with tile(p) and gb[(p + 1) & 1] before they refill
+    // every wave is done with tile(p), G(p) and its nbuf cells before they refill
     __syncthreads();
     if (p + 1 < x0 + chunk) {
       dma_tile(p + 1);
       dma_g(p + 1);
     }
     if (comb) dma_u(p);
-    dma_next(p + 3);
+    dma_next(p + 3);  // f(p + 3): read at the next plane
     T rhs[9], speed;
     cell_flux<T, 3, DYN, true>(u, sg, Lambda, rhs, speed, &vsh[0][tid], NT, T(n));
     bool fin = isfinite(speed);
 #pragma unroll
     for (int c = 0; c < 9; ++c) fin = fin && isfinite(rhs[c]);
     run_max = fmax(run_max, fin ? speed : T(INFINITY));
-    // everything issued this step has landed: U(p) and the ring's next plane
-    // (this wave's own cells), tile(p + 1) and G(p + 1)
+    // everything issued this step has landed: U(p) and plane p + 3's cells
+    // (this wave's own), tile(p + 1) and G(p + 1)
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     T* o = outp + (long long)p * splane + xoff;
     if (comb) {
@@ -727,13 +750,6 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
     } else {
 #pragma unroll
       for (int c = 0; c < 9; ++c) o[c * cells] = rhs[c];
-    }
-#pragma unroll
-    for (int c = 0; c < 9; ++c) {
-      xr[0][c] = xr[1][c];
-      xr[1][c] = fc[c];
-      xr[2][c] = xr[3][c];
-      xr[3][c] = nbuf[wv][c][lane];
     }
   }
   if (job.partial) {
