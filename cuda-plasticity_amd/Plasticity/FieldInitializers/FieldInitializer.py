@@ -72,7 +72,9 @@ def _read_record(filename, N, dim, time, dtype, hastimes):
     es = dtype.itemsize
     datasize = 9 * N ** dim
     rec = es * (datasize + (1 if hastimes else 0))
-    with open(filename, 'rb') as f:
+    own = not hasattr(filename, "read")  # a path, or an open file (e.g. a tar member)
+    f = open(filename, 'rb') if own else filename
+    try:
         f.seek(0, 2)
         cnt = f.tell() // rec
         if cnt < 1:
@@ -87,7 +89,74 @@ def _read_record(filename, N, dim, time, dtype, hastimes):
         f.seek(k * rec)
         t = float(np.frombuffer(f.read(es), dtype=dtype)[0]) if hastimes else 0.0
         data = np.frombuffer(f.read(es * datasize), dtype=dtype).reshape((9,) + (N,) * dim)
+    finally:
+        if own:
+            f.close()
     return t, data
+
+
+# Globals a state pickle may name: the state and field classes of this
+# package (also under the reference's module paths, which are the same) and
+# numpy's array reconstruction; anything else is refused, so loading a
+# pickle cannot run arbitrary code.
+_PICKLE_ALLOWED = {
+    ("Plasticity.PlasticityStates.PlasticityState", "PlasticityState"),
+    ("Plasticity.PlasticityStates.VacancyState", "VacancyState"),
+    ("Plasticity.Fields.Fields", "Field"),
+    ("Plasticity.Fields.Fields", "TensorField"),
+    ("Plasticity.Fields.Fields", "TensorScalarMixedField"),
+    ("numpy.core.multiarray", "_reconstruct"),
+    ("numpy._core.multiarray", "_reconstruct"),
+    ("numpy.core.multiarray", "scalar"),
+    ("numpy._core.multiarray", "scalar"),
+    ("numpy", "ndarray"),
+    ("numpy", "dtype"),
+    ("copy_reg", "_reconstructor"),
+    ("copyreg", "_reconstructor"),
+    ("__builtin__", "object"),
+    ("builtins", "object"),
+    ("_codecs", "encode"),  # protocol-2 pickles written by Python 3 carry array bytes this way
+}
+
+
+def _state_unpickler(f):
+    import importlib
+    import pickle
+
+    class _Restricted(pickle.Unpickler):
+        def find_class(self, module, name):
+            if (module, name) not in _PICKLE_ALLOWED:
+                raise pickle.UnpicklingError("refusing %s.%s in a state pickle" % (module, name))
+            mod = {"copy_reg": "copyreg", "__builtin__": "builtins"}.get(module, module)
+            if mod == "numpy.core.multiarray":
+                mod = "numpy._core.multiarray" if hasattr(np, "_core") else mod
+            return getattr(importlib.import_module(mod), name)
+    # Python 2 pickles (the reference's) carry numpy buffers as latin-1 strings
+    return _Restricted(f, encoding="latin1")
+
+
+def LoadState(filename, time=None):
+    """(time, state) from a RecordStateObserver file of consecutive pickled
+    (time, state) pairs (FieldInitializer.py:29-59): the last pair, or the
+    first whose time is >= `time`.  Only state, field and numpy classes may
+    appear in the pickle.  Reference-written files (Python 2 pickles of its
+    PlasticityState, whose field holds per-component arrays) use the same
+    module paths and layout; no such file is available here, so that path is
+    exercised with a synthetic pickle of the same structure only."""
+    T, state = None, None
+    with open(filename, "rb") as f:
+        while True:
+            try:
+                u = _state_unpickler(f)
+                T = u.load()
+                state = u.load()
+            except EOFError:
+                break
+            if time is not None and T >= time:
+                return T, state
+    if state is None or time is not None:
+        raise ValueError("no state%s found in %s" % ("" if time is None else " at or after t=%g" % time, filename))
+    return T, state
 
 
 # ---------------------------------------------------------------------------

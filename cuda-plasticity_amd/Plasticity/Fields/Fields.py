@@ -29,6 +29,24 @@ class Field(object):
     def __getitem__(self, index):
         return self.data[self._index[index]]
 
+    # pickling: host arrays per component, the reference's Field layout
+    # (Fields.py:7-50: gridShape, components, data = {component: array})
+    def __getstate__(self):
+        arr = self.data.detach().cpu().numpy()
+        return {"gridShape": self.gridShape, "components": list(self.components),
+                "data": {c: arr[k].copy() for c, k in self._index.items()}}
+
+    def __setstate__(self, d):
+        import torch
+        comps = sorted(d["components"])
+        data = d["data"]
+        arr = np.stack([np.asarray(data[c], dtype=np.float64) for c in comps])
+        self.gridShape = tuple(d["gridShape"])
+        self.components = comps
+        self._index = {c: k for k, c in enumerate(comps)}
+        dev = torch.device("cuda", 0) if torch.cuda.is_available() else torch.device("cpu")
+        self.data = torch.from_numpy(arr).to(dev)
+
     def __setitem__(self, index, value):
         self.data[self._index[index]] = value
 

@@ -22,6 +22,21 @@ class RecallStateObserver(Observer):
         self.state = state
 
 
+class RecordStateObserver(Observer):
+    """Pickles (time, state) after every step, protocol 2 (:23-48); read
+    back with FieldInitializer.LoadState.  The state's field is pickled as
+    host arrays per component, the reference's layout."""
+
+    def __init__(self, filename=None, mode='wb'):
+        self.file = open(filename, mode) if filename is not None else sys.stdout.buffer
+
+    def Update(self, time, state):
+        import pickle
+        pickle.dump(time, self.file, protocol=2)
+        pickle.dump(state, self.file, protocol=2)
+        self.file.flush()
+
+
 class RecordStateObserverRaw(Observer):
     """Appends [time, 9 components] as raw float64 (:51-69)."""
 

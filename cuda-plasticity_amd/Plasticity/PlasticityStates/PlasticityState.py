@@ -32,6 +32,24 @@ class PlasticityState:
         else:  # NumPy array (9, *gridShape)
             self.betaP = TensorField.from_numpy(field, device=device)
 
+    # pickling (PlasticityState.py:41-60): the order parameter field as host
+    # arrays under 'field'; a pickle written by the reference (its __dict__
+    # with 'field' a Field of per-component arrays) loads the same way
+    def __getstate__(self):
+        d = {k: v for k, v in self.__dict__.items() if k not in ("betaP", "betaP_V")}
+        d["field"] = self.GetOrderParameterField()
+        return d
+
+    def __setstate__(self, d):
+        d = dict(d)
+        field = d.pop("field")
+        for k in ("ktools", "sinktools"):
+            d.pop(k, None)
+        self.__dict__.update(d)
+        self.gridShape = tuple(self.gridShape)
+        self.dimension = len(self.gridShape)
+        self.UpdateOrderParameterField(field)
+
     def _plan(self):
         return plans.analysis_plan(self.gridShape, self.mu, self.nu, self.betaP.data.device)
 
