@@ -828,3 +828,250 @@ def run_vacancy(dyn, vac, beta, c, t0, nsteps, cfl, order=3, t_end=np.inf, fixed
         if t >= t_end:
             break
     return u, c, t, np.array(dts)
+
+
+# ---------------------------------------------------------------------------
+# The CUDA executable's numerics (plasticity_kernel.cu centralHJ :118-697,
+# findDerivatives :53-90, simpleTVD :1112-1137, TVD3rd :1140-1206, with the
+# defaults of plasticity.h: theta = 1, LLF, PYTHON_COMPATIBILITY_DIVIDE, DOUBLE;
+# RunCUDA.py:54-81 adds lambda (gcd 0, mdp 1, lvp 0 + NEWGLIDEONLY) and
+# CFLsafeFactor 0.5).  Same physics as the Python path, different numerics:
+# one local Lax-Friedrichs speed per cell, |rho| + sqrt(ME) in place of
+# sqrt(|rho|^2 + ME), its own second-difference form, forward Euler.
+# ---------------------------------------------------------------------------
+def cuda_minmod3(a, b, c):
+    """minmod3 of plasticity_kernel.cu:41-51 (min if all > 0, max if all < 0,
+    else 0)."""
+    mn = np.minimum(np.minimum(a, b), c)
+    mx = np.maximum(np.maximum(a, b), c)
+    return np.where((a > 0) & (b > 0) & (c > 0), mn, np.where((a < 0) & (b < 0) & (c < 0), mx, 0.0))
+
+
+def cuda_derivatives(field, axis, N):
+    """findDerivatives with theta (plasticity_kernel.cu:53-90): returns the
+    (+, -) one-sided derivatives in units of the grid (times N).  Pinned
+    against find_derivatives (the same limiter written with differences of
+    differences) to rounding."""
+    ax = axis + 1
+    r = np.roll
+    val, val_r, val_l = field, r(field, -1, ax), r(field, 1, ax)
+    val_rr, val_ll = r(field, -2, ax), r(field, 2, ax)
+    diff_p = val_r - val
+    diff_m = val - val_l
+    d2 = THETA * (val_r + val_l - 2 * val)
+    dp = (diff_p - 0.5 * cuda_minmod3(THETA * (val_rr + val - 2 * val_r), d2, 0.5 * (val_rr - val_r - val + val_l))) * N
+    dm = (diff_m + 0.5 * cuda_minmod3(THETA * (val_ll + val - 2 * val_l), d2, 0.5 * (val_r - val - val_l + val_ll))) * N
+    return dp, dm
+
+
+def cuda_velocity_numerator(us, sig, lam, glide_only):
+    """The corner velocity before the |rho| division (plasticity_kernel.cu:
+    289-456): the glide + climb velocity with climb suppression lambda or,
+    glide_only (NEWGLIDEONLY, :389-452), the lvp projection.  The 3D v_z
+    subtracts uz[y,n] sigma[z,n] where its comment (and the Python path)
+    have uz[y,n] sigma[y,n] (:364-369); restated as written."""
+    d = len(us)
+
+    def U(a, i, j):
+        return us[a][cidx(i, j)]
+
+    def S(i, j):
+        return sig[cidx(i, j)]
+    vx = U(1, 0, 0) * S(1, 0)
+    vx = vx - U(0, 1, 0) * S(1, 0)
+    vx = vx - U(0, 2, 0) * S(2, 0)
+    vx = vx + U(1, 0, 1) * S(1, 1)
+    vx = vx - U(0, 1, 1) * S(1, 1)
+    vx = vx - U(0, 2, 1) * S(2, 1)
+    vx = vx + U(1, 0, 2) * S(1, 2)
+    vx = vx - U(0, 1, 2) * S(1, 2)
+    vx = vx - U(0, 2, 2) * S(2, 2)
+    if d == 3:
+        vx = vx + U(2, 0, 0) * S(2, 0)
+        vx = vx + U(2, 0, 1) * S(2, 1)
+        vx = vx + U(2, 0, 2) * S(2, 2)
+    vy = U(0, 1, 0) * S(0, 0)
+    vy = vy - U(1, 0, 0) * S(0, 0)
+    vy = vy - U(1, 2, 0) * S(2, 0)
+    vy = vy + U(0, 1, 1) * S(0, 1)
+    vy = vy - U(1, 0, 1) * S(0, 1)
+    vy = vy - U(1, 2, 1) * S(2, 1)
+    vy = vy + U(0, 1, 2) * S(0, 2)
+    vy = vy - U(1, 0, 2) * S(0, 2)
+    vy = vy - U(1, 2, 2) * S(2, 2)
+    if d == 3:
+        vy = vy + U(2, 1, 0) * S(2, 0)
+        vy = vy + U(2, 1, 1) * S(2, 1)
+        vy = vy + U(2, 1, 2) * S(2, 2)
+    vz = U(0, 2, 0) * S(0, 0)
+    vz = vz + U(1, 2, 0) * S(1, 0)
+    vz = vz + U(0, 2, 1) * S(0, 1)
+    vz = vz + U(1, 2, 1) * S(1, 1)
+    vz = vz + U(0, 2, 2) * S(0, 2)
+    vz = vz + U(1, 2, 2) * S(1, 2)
+    if d == 3:
+        vz = vz - U(2, 0, 0) * S(0, 0)
+        vz = vz - U(2, 0, 1) * S(0, 1)
+        vz = vz - U(2, 0, 2) * S(0, 2)
+        vz = vz - U(2, 1, 0) * S(2, 0)
+        vz = vz - U(2, 1, 1) * S(2, 1)
+        vz = vz - U(2, 1, 2) * S(2, 2)
+    st = (S(0, 0) + S(1, 1) + S(2, 2)) / 3. * lam
+    vx = vx + st * (U(0, 0, 0) + U(0, 1, 1) + U(0, 2, 2))
+    vy = vy + st * (U(1, 0, 0) + U(1, 1, 1) + U(1, 2, 2))
+    vx = vx - st * (U(0, 0, 0) + U(1, 0, 1))
+    vy = vy - st * (U(0, 1, 0) + U(1, 1, 1))
+    vz = vz - st * (U(0, 2, 0) + U(1, 2, 1))
+    if d == 3:
+        vz = vz + st * (U(2, 0, 0) + U(2, 1, 1) + U(2, 2, 2))
+        vx = vx - st * U(2, 0, 2)
+        vy = vy - st * U(2, 1, 2)
+        vz = vz - st * U(2, 2, 2)
+    if glide_only:
+        txr = U(0, 0, 0) + U(0, 1, 1) + U(0, 2, 2)
+        tyr = U(1, 0, 0) + U(1, 1, 1) + U(1, 2, 2)
+        if d == 2:
+            rr = ME + (txr * txr + tyr * tyr)
+            rr = rr + (U(0, 0, 0) + U(1, 0, 1)) * (U(0, 0, 0) + U(1, 0, 1))
+            rr = rr + (U(0, 1, 0) + U(1, 1, 1)) * (U(0, 1, 0) + U(1, 1, 1))
+            rr = rr + (U(0, 2, 0) + U(1, 2, 1)) * (U(0, 2, 0) + U(1, 2, 1))
+            rr = rr - 2 * txr * (U(0, 0, 0) + U(1, 0, 1))
+            rr = rr - 2 * tyr * (U(0, 1, 0) + U(1, 1, 1))
+            rv = ME + vx * (U(0, 2, 2) + U(0, 1, 1) - U(1, 0, 1))
+            rv = rv + vy * (U(1, 0, 0) + U(1, 2, 2) - U(0, 1, 0))
+            rv = rv - vz * (U(0, 2, 0) + U(1, 2, 1))
+            rv = rv / rr
+            vx = vx + (U(0, 1, 1) + U(0, 2, 2) - U(1, 0, 1)) * rv
+            vy = vy + (U(1, 0, 0) + U(1, 2, 2) - U(0, 1, 0)) * rv
+            vz = vz - (U(0, 2, 0) + U(1, 2, 1)) * rv
+        else:
+            tzr = U(2, 0, 0) + U(2, 1, 1) + U(2, 2, 2)
+            rx = U(0, 0, 0) + U(1, 0, 1) + U(2, 0, 2)
+            ry = U(0, 1, 0) + U(1, 1, 1) + U(2, 1, 2)
+            rz = U(0, 2, 0) + U(1, 2, 1) + U(2, 2, 2)
+            rr = ME + (txr * txr + tyr * tyr + tzr * tzr)
+            rr = rr + rx * rx
+            rr = rr + ry * ry
+            rr = rr + rz * rz
+            rr = rr - 2 * txr * rx
+            rr = rr - 2 * tyr * ry
+            rr = rr - 2 * tzr * rz
+            qx = U(0, 2, 2) + U(0, 1, 1) - U(1, 0, 1) - U(2, 0, 2)
+            qy = U(1, 0, 0) + U(1, 2, 2) - U(0, 1, 0) - U(2, 1, 2)
+            qz = U(2, 0, 0) + U(2, 1, 1) - U(0, 2, 0) - U(1, 2, 1)
+            rv = ME + vx * qx
+            rv = rv + vy * qy
+            rv = rv + vz * qz
+            rv = rv / rr
+            vx = vx + (U(0, 1, 1) + U(0, 2, 2) - U(1, 0, 1) - U(2, 0, 2)) * rv
+            vy = vy + (U(1, 0, 0) + U(1, 2, 2) - U(0, 1, 0) - U(2, 1, 2)) * rv
+            vz = vz + (U(2, 0, 0) + U(2, 1, 1) - U(0, 2, 0) - U(1, 2, 1)) * rv
+    return [vx, vy, vz]
+
+
+def cuda_rho_norm(us):
+    """|rho| of a corner without ME, negative values clamped (:232-287)."""
+    d = len(us)
+    r = 0.0
+    for a in range(d):
+        for i in range(3):
+            if i == a:
+                continue
+            for j in range(3):
+                r = r + us[a][cidx(i, j)] * us[a][cidx(i, j)]
+    for j in range(3):
+        r = r - 2 * us[0][cidx(1, j)] * us[1][cidx(0, j)]
+    if d == 3:
+        for j in range(3):
+            r = r - 2 * us[0][cidx(2, j)] * us[2][cidx(0, j)]
+        for j in range(3):
+            r = r - 2 * us[1][cidx(2, j)] * us[2][cidx(1, j)]
+    return np.sqrt(np.maximum(r, 0.0))
+
+
+def cuda_llf_flux(beta, sig, lam, glide_only=False):
+    """centralHJ with LLF (plasticity_kernel.cu:118-697): corners k (x sign
+    k % 2, y sign (k % 4) / 2, z sign k / 4; 0 is '+'), velocities divided by
+    |rho| + sqrt(ME), one speed a = max over corners and axes of |v|, then
+    rhs = sum_k -H_k a^d / prod(2a + ME) + sum_axes (u+ - u-) a^2 / (2a + ME)
+    and speed = sum_axes (2a + ME).  Pinned by tests/test_oracle_cuda.py."""
+    d = beta.ndim - 1
+    N = beta.shape[-1]
+    du = []
+    for a in range(d):
+        p, m = cuda_derivatives(beta, a, N)
+        du += [p, m]
+    ncor = 1 << d
+    V, US = [], []
+    for k in range(ncor):
+        s = [k % 2, (k % 4) // 2, k // 4]
+        us = [du[2 * a + s[a]] for a in range(d)]
+        v = cuda_velocity_numerator(us, sig, lam, glide_only)
+        irho = cuda_rho_norm(us) + np.sqrt(ME)
+        V.append([vv / irho for vv in v])
+        US.append(us)
+    amax = np.zeros(beta.shape[1:])
+    for k in range(ncor):
+        for l in range(3):
+            amax = np.maximum(amax, np.abs(V[k][l]))
+    at = amax + amax + ME
+    deriv = np.zeros_like(beta)
+    for k in range(ncor):
+        us, v = US[k], V[k]
+        for i in range(3):
+            for j in range(3):
+                c = cidx(i, j)
+                h = us[0][c] * v[0] + us[1][c] * v[1]
+                if d == 3:
+                    h = h + us[2][c] * v[2]
+                if i < d:
+                    h = h - v[0] * us[i][cidx(0, j)]
+                    h = h - v[1] * us[i][cidx(1, j)]
+                    h = h - v[2] * us[i][cidx(2, j)]
+                if i == j:
+                    q = (v[0] * (us[0][cidx(1, 1)] + us[0][cidx(2, 2)] - us[1][cidx(0, 1)])
+                         + v[1] * (us[1][cidx(0, 0)] + us[1][cidx(2, 2)] - us[0][cidx(1, 0)])
+                         - v[2] * (us[0][cidx(2, 0)] + us[1][cidx(2, 1)]))
+                    if d == 3:
+                        q = q - v[0] * us[2][cidx(0, 2)] - v[1] * us[2][cidx(1, 2)] + v[2] * (
+                            us[2][cidx(0, 0)] + us[2][cidx(1, 1)])
+                    if not glide_only:
+                        h = h - q / 3. * lam
+                w = -h * amax * amax
+                if d == 3:
+                    w = w * amax
+                for a in range(d):
+                    w = w / at
+                deriv[c] = deriv[c] + w
+    for a in range(d):
+        deriv = deriv + (du[2 * a] - du[2 * a + 1]) * amax * amax / at
+    return deriv, at * d
+
+
+def cuda_euler_step(flux, t, u, t_end, cfl, N):
+    """simpleTVD (plasticity_kernel.cu:1112-1137): dt = CFL / max speed / N,
+    past the end time dt = t_end - t + ME; u += dt rhs."""
+    rhs, speed = flux(t, u)
+    dt = cfl / speed.max() / N
+    if t + dt > t_end:
+        dt = t_end - t + ME
+    return u + dt * rhs, dt
+
+
+def run_cuda(dyn, beta, t0, nsteps, cfl=0.5, t_end=np.inf):
+    """The CUDA executable's run loop (plasticity.cu:242-271) for a fixed
+    number of steps with the LLF flux and forward Euler; dyn gives Lambda
+    (its 'lambda'), glide-only (NEWGLIDEONLY), elastic constants and the load.
+    Returns (u, t, dts)."""
+    glide_only = dyn.dynamics == GLIDE_ONLY
+    lam = 0.0 if glide_only else dyn.Lambda
+    N = beta.shape[-1]
+    t, u, dts = t0, beta, []
+    for _ in range(nsteps):
+        u, dt = cuda_euler_step(lambda tt, uu: cuda_llf_flux(uu, dyn.get_sigma(uu, tt), lam, glide_only),
+                                t, u, t_end, cfl, N)
+        t += dt
+        dts.append(dt)
+        if t >= t_end:
+            break
+    return u, t, np.array(dts)
