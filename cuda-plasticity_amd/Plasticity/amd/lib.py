@@ -14,6 +14,10 @@ LIBPATH = os.path.join(_HERE, "libcdd.so")
 GLIDE_CLIMB, GLIDE_ONLY, GLIDE_CLIMB_WOJZZ = 0, 1, 2
 FP64, FP32 = 0, 1
 LOAD_NONE, LOAD_STRAIN, LOAD_STRESS, LOAD_UNIAXIAL, LOAD_TENSOR = 0, 1, 2, 3, 4
+# flux discretisations: the Python stepper's central-upwind scheme, and the
+# CUDA executable's local Lax-Friedrichs scheme (rk_order 1 = its simpleTVD)
+SCHEME_CENTRAL_UPWIND, SCHEME_LLF = 0, 1
+SCHEMES = {"central-upwind": SCHEME_CENTRAL_UPWIND, "llf": SCHEME_LLF}
 
 
 class CDDError(RuntimeError):
@@ -38,6 +42,7 @@ class Config(ctypes.Structure):
         ("load_init_strain", ctypes.c_double),
         ("load_uniaxial_rate", ctypes.c_double),
         ("precision", ctypes.c_int),
+        ("scheme", ctypes.c_int),
     ]
 
 
@@ -167,7 +172,7 @@ class Plan:
 
     def __init__(self, ndim, n, dynamics=GLIDE_CLIMB, rk_order=3, Lambda=0.0, mu=1.0, nu=0.3, cfl=0.25,
                  load_kind=LOAD_NONE, load_init=None, load_rate=None, load_t0=0.0, load_init_strain=0.0,
-                 load_uniaxial_rate=0.0, device=0, precision="fp64"):
+                 load_uniaxial_rate=0.0, device=0, precision="fp64", scheme="central-upwind"):
         import torch
         self.torch = torch
         if precision not in ("fp64", "fp32"):
@@ -176,6 +181,10 @@ class Plan:
         self.dtype = torch.float32 if precision == "fp32" else torch.float64
         cfg = Config()
         cfg.precision = FP32 if precision == "fp32" else FP64
+        if scheme not in SCHEMES:
+            raise CDDError("scheme must be one of %s" % sorted(SCHEMES))
+        cfg.scheme = SCHEMES[scheme]
+        self.scheme = scheme
         cfg.ndim, cfg.n, cfg.dynamics, cfg.rk_order = ndim, n, dynamics, rk_order
         cfg.Lambda, cfg.mu, cfg.nu, cfg.cfl = Lambda, mu, nu, cfl
         cfg.load_kind, cfg.device = load_kind, device

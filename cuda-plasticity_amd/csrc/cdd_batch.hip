@@ -194,6 +194,8 @@ int cdd_batch_create(const cdd_config* cfg, const int* sizes, int count, cdd_bat
   if (!cfg || !sizes || !out || count < 1 || count > 4096) return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
   if (cfg->ndim != 2) return cdd_internal_fail(CDD_ERR_UNSUPPORTED, "batched stepping is 2D");
   if (cfg->rk_order != 2 && cfg->rk_order != 3) return cdd_internal_fail(CDD_ERR_ARG, "rk_order must be 2 or 3");
+  if (cfg->scheme != CDD_SCHEME_CENTRAL_UPWIND)
+    return cdd_internal_fail(CDD_ERR_UNSUPPORTED, "batched stepping uses the central-upwind scheme");
   if (cfg->load_kind != CDD_LOAD_NONE) return cdd_internal_fail(CDD_ERR_UNSUPPORTED, "batched stepping has no load");
   if (cfg->precision != CDD_FP64) return cdd_internal_fail(CDD_ERR_UNSUPPORTED, "batched stepping is float64 only");
   if (cfg->dynamics != CDD_GLIDE_CLIMB && cfg->dynamics != CDD_GLIDE_ONLY)

@@ -111,8 +111,8 @@ static bool valid_n(int d, int n) {
 }
 
 static int cur_state(cdd_plan* p) {
-  // RK2 ping-pongs between state[0] and state[1]; RK3 updates in place
-  if (p->cfg.rk_order == 3) return p->cur;
+  // RK2 ping-pongs between state[0] and state[1]; RK3 and Euler update in place
+  if (p->cfg.rk_order != 2) return p->cur;
   return p->cur ^ (int)((p->steps_launched - p->steps_base) & 1);
 }
 
@@ -122,7 +122,7 @@ static int sync_clock(cdd_plan* p, Clock* host) {
   HIPCHK(hipStreamSynchronize(p->stream));
   Clock c;
   HIPCHK(hipMemcpy(&c, p->clk, sizeof(Clock), hipMemcpyDeviceToHost));
-  if (p->cfg.rk_order != 3) {
+  if (p->cfg.rk_order == 2) {
     long long executed = c.nsteps;
     p->cur = p->cur ^ (int)((executed - p->steps_base) & 1);
     p->steps_base = executed;
@@ -139,7 +139,16 @@ const char* cdd_last_error(void) { return g_err.c_str(); }
 int cdd_plan_create(const cdd_config* cfg, cdd_plan** out) {
   if (!cfg || !out) return fail(CDD_ERR_ARG, "null argument");
   if (!valid_n(cfg->ndim, cfg->n)) return fail(CDD_ERR_UNSUPPORTED, "unsupported grid dimension/size");
-  if (cfg->rk_order != 2 && cfg->rk_order != 3) return fail(CDD_ERR_ARG, "rk_order must be 2 or 3");
+  if (cfg->scheme != CDD_SCHEME_CENTRAL_UPWIND && cfg->scheme != CDD_SCHEME_LLF) return fail(CDD_ERR_ARG, "bad scheme");
+  if (cfg->scheme == CDD_SCHEME_CENTRAL_UPWIND && cfg->rk_order != 2 && cfg->rk_order != 3)
+    return fail(CDD_ERR_ARG, "rk_order must be 2 or 3");
+  if (cfg->scheme == CDD_SCHEME_LLF) {
+    // the CUDA executable's simpleTVD (1) and TVD3rd (3), float64 (DOUBLE)
+    if (cfg->rk_order != 1 && cfg->rk_order != 3) return fail(CDD_ERR_ARG, "the LLF scheme's rk_order is 1 or 3");
+    if (cfg->precision != CDD_FP64) return fail(CDD_ERR_UNSUPPORTED, "the LLF scheme is float64 only");
+    if (cfg->dynamics == CDD_GLIDE_CLIMB_WOJZZ)
+      return fail(CDD_ERR_UNSUPPORTED, "the LLF scheme supports the glide+climb and glide-only dynamics");
+  }
   if (cfg->dynamics != CDD_GLIDE_CLIMB && cfg->dynamics != CDD_GLIDE_ONLY && cfg->dynamics != CDD_GLIDE_CLIMB_WOJZZ)
     return fail(CDD_ERR_ARG, "bad dynamics");
   if (cfg->dynamics == CDD_GLIDE_CLIMB_WOJZZ && cfg->ndim != 2)
@@ -182,7 +191,7 @@ int cdd_plan_create(const cdd_config* cfg, cdd_plan** out) {
   ALLOC(p->tw, p->es * 2 * (size_t)n);
   ALLOC(p->clk, 2 * sizeof(Clock));  // [1]: saved clock of a segment replay
 #undef ALLOC
-  if (cfg->rk_order == 3) p->state[1] = p->buf;
+  if (cfg->rk_order != 2) p->state[1] = p->buf;
   // twiddle table W_n^e = exp(-2 pi i e / n)
   {
     double* h = new double[2 * n];
@@ -235,6 +244,7 @@ int cdd_plan_create(const cdd_config* cfg, cdd_plan** out) {
   c0.t = 0.0;
   c0.t_end = INFINITY;
   c0.cfl = cfg->cfl;
+  c0.dt_rule = cfg->scheme == CDD_SCHEME_LLF ? 1 : 0;
   HIPCHK(hipMemcpy(p->clk, &c0, sizeof c0, hipMemcpyHostToDevice));
   HIPCHK(hipStreamSynchronize(p->stream));
   *out = p;
@@ -384,6 +394,7 @@ static FluxJob flux_job(cdd_plan* p) {
   f.lam = p->gp.lam;
   f.stream = p->stream;
   f.fp32 = p->fp32;
+  f.llf = p->cfg.scheme == CDD_SCHEME_LLF;
   return f;
 }
 
@@ -466,6 +477,13 @@ static int launch_step(cdd_plan* p, const StepBufs& b) {
   if (p->timing) timing_hook(p, KK_FLUX1);
   HIPCHK(launch_finalize_dt(p->partial, nb, p->clk, nmax, p->stream, 1, 0, p->vacancy ? p->dt_bound : 0.0));
   if (p->timing) timing_hook(p, KK_FINALIZE);
+  if (p->cfg.rk_order == 1) {
+    // forward Euler (simpleTVD, plasticity_kernel.cu:1112-1137), in place
+    HIPCHK(launch_euler(b.out, p->L0, 9 * p->cells, p->clk, p->stream));
+    HIPCHK(launch_advance(p->clk, p->stream));
+    if (p->timing) timing_hook(p, KK_ADVANCE);
+    return CDD_OK;
+  }
   // stage 2 on F0 = u + dt L0: the R2C pass forms F0 while reading u and
   // L0 for the stress solve and stores it in F0 (which may be L0 itself)
   {
@@ -535,7 +553,7 @@ int cdd_step(cdd_plan* p, int nsteps) {
     const int ci = cur_state(p);
     StepBufs b;
     b.u = p->state[ci];
-    b.out = p->cfg.rk_order == 3 ? p->state[ci] : p->state[ci ^ 1];
+    b.out = p->cfg.rk_order != 2 ? p->state[ci] : p->state[ci ^ 1];
     b.G[0] = b.G[1] = b.G[2] = p->G;
     b.F0 = p->L0;
     b.F1 = p->buf;
@@ -553,6 +571,8 @@ int cdd_set_vacancy(cdd_plan* p, int enable, double alpha, double beta, double g
     return CDD_OK;
   }
   if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "the vacancy coupling is float64 only");
+  if (p->cfg.scheme != CDD_SCHEME_CENTRAL_UPWIND)
+    return fail(CDD_ERR_UNSUPPORTED, "the vacancy coupling uses the central-upwind scheme");
   if (!(gamma >= 0.0) || !(dt_bound >= 0.0) || !isfinite(alpha) || !isfinite(beta))
     return fail(CDD_ERR_ARG, "bad vacancy parameters");
   HIPCHK(hipSetDevice(p->cfg.device));
@@ -611,6 +631,8 @@ int cdd_energy(cdd_plan* p, const void* beta_v, double t, double* energy) {
 static int adj_alloc(cdd_plan* p) {
   if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "the adjoint is float64 only");
   if (p->vacancy) return fail(CDD_ERR_UNSUPPORTED, "the adjoint does not include the vacancy coupling");
+  if (p->cfg.scheme != CDD_SCHEME_CENTRAL_UPWIND)
+    return fail(CDD_ERR_UNSUPPORTED, "the adjoint differentiates the central-upwind scheme");
   if (p->cfg.dynamics == CDD_GLIDE_CLIMB_WOJZZ)
     return fail(CDD_ERR_UNSUPPORTED, "the adjoint supports the glide+climb and glide-only dynamics");
   if (p->adj) return CDD_OK;
@@ -963,6 +985,8 @@ int cdd_segment_forward(cdd_plan* p, int nsteps, void* store, unsigned long long
   if (!p || !store || nsteps < 1) return fail(CDD_ERR_ARG, "bad argument");
   if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "stored trajectories are float64 only");
   if (p->vacancy) return fail(CDD_ERR_UNSUPPORTED, "the adjoint does not include the vacancy coupling");
+  if (p->cfg.scheme != CDD_SCHEME_CENTRAL_UPWIND)
+    return fail(CDD_ERR_UNSUPPORTED, "the adjoint differentiates the central-upwind scheme");
   if (store_bytes < SegLayout::bytes(nsteps, p->cfg.rk_order, p->cells))
     return fail(CDD_ERR_ARG, "segment store too small (cdd_segment_store_bytes)");
   HIPCHK(hipSetDevice(p->cfg.device));

@@ -70,6 +70,10 @@ struct Clock {
   // optional energy log: total elastic energy of each step's start state
   double* elog;
   long long elog_base, elog_cap;
+  // 0: dt = CFL / (N max speed), the end step lands on t_end
+  //    (FieldMover.py:94-111); 1: dt = CFL / max speed / N, the end step is
+  //    t_end - t + ME (plasticity_kernel.cu simpleTVD :1121-1128)
+  int dt_rule;
 };
 
 // Uniform load added to sigma (it enters the stress solve as the k = 0 mode).

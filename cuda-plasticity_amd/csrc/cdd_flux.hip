@@ -294,6 +294,180 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
   }
 }
 
+// ---------------------------------------------------------------------------
+// The CUDA executable's numerics (plasticity_kernel.cu centralHJ :118-697 with
+// the defaults of plasticity.h: LLF, theta = 1, DOUBLE): the same physics as
+// cell_flux, with one local Lax-Friedrichs speed per cell, velocities divided
+// by |rho| + sqrt(ME), and its own form of the limited derivatives.
+
+// minmod3 of plasticity_kernel.cu:41-51: the smallest if all three are
+// positive, the largest if all are negative, else 0.
+template <typename T>
+__host__ __device__ __forceinline__ T minmod3_cmp(T a, T b, T c) {
+  if (a > T(0) && b > T(0) && c > T(0)) return fmin(fmin(a, b), c);
+  if (a < T(0) && b < T(0) && c < T(0)) return fmax(fmax(a, b), c);
+  return T(0);
+}
+
+// findDerivatives, theta = 1 (plasticity_kernel.cu:53-90): the one-sided
+// limited derivatives from f[i-2..i+2], in the grid's units (times N).
+template <typename T>
+__host__ __device__ __forceinline__ void llf_derivs(const T f[5], T N, T& up, T& um) {
+  const T vll = f[0], vl = f[1], v = f[2], vr = f[3], vrr = f[4];
+  const T d2 = (vr + vl) - T(2) * v;
+  const T mp = minmod3_cmp<T>((vrr + v) - T(2) * vr, d2, T(0.5) * (((vrr - vr) - v) + vl));
+  const T mm = minmod3_cmp<T>((vll + v) - T(2) * vl, d2, T(0.5) * (((vr - v) - vl) + vll));
+  up = ((vr - v) - T(0.5) * mp) * N;
+  um = ((v - vl) + T(0.5) * mm) * N;
+}
+
+// Per-cell LLF flux.  u[a][s][c]: derivative along axis a, s = 0 ('+') or 1
+// ('-'), in the grid's units.  Corner k takes the x sign k & 1, the y sign
+// (k >> 1) & 1 and the z sign k >> 2.  With the single speed a = max over
+// corners and axes of |v|, at = 2 a + ME:
+//   rhs = -(a^d / at^d) sum_k H_k + sum_axes (u+ - u-) a^2 / at,
+//   speed = d at.
+// H_k is linear in the corner velocity v_k, so sum_k H_k is formed from
+// Y[a][s][l] = sum over corners with sign s on axis a of v_k[l]
+// (sum_k u_k[a][c] v_k[l] = sum_s u[a][s][c] Y[a][s][l]).  vbuf parks the
+// 3 2^D corner velocities (stride vstride).
+template <typename T, int D, int DYN>
+__host__ __device__ __forceinline__ void cell_flux_llf(const T (&u)[D][2][9], const T (&sg)[9], T Lambda, T (&rhs)[9],
+                                              T& speed, T* __restrict__ vbuf, int vstride) {
+  constexpr int NCOR = 1 << D;
+  const T ME = T(kME);
+  const T sqME = sqrt(ME);
+  // climb suppression lambda tr(sigma) / 3 (:297-300, :336-345, :371-386)
+  const T st = ((sg[0] + sg[4]) + sg[8]) / T(3) * Lambda;
+  T amax = T(0);
+#pragma unroll
+  for (int k = 0; k < NCOR; ++k) {
+    const T* U0 = u[0][k & 1];
+    const T* U1 = u[1][(k >> 1) & 1];
+    const T* U2 = u[D - 1][(k >> 2) & 1];  // read in 3D only
+    // velocity numerator (:289-456); the 3D v_z subtracts uz[y,n] sigma[z,n]
+    // as the executable does
+    T vx = U1[0] * sg[3] - U0[3] * sg[3] - U0[6] * sg[6] + U1[1] * sg[4] - U0[4] * sg[4] - U0[7] * sg[7] +
+           U1[2] * sg[5] - U0[5] * sg[5] - U0[8] * sg[8];
+    T vy = U0[3] * sg[0] - U1[0] * sg[0] - U1[6] * sg[6] + U0[4] * sg[1] - U1[1] * sg[1] - U1[7] * sg[7] +
+           U0[5] * sg[2] - U1[2] * sg[2] - U1[8] * sg[8];
+    T vz = U0[6] * sg[0] + U1[6] * sg[3] + U0[7] * sg[1] + U1[7] * sg[4] + U0[8] * sg[2] + U1[8] * sg[5];
+    if constexpr (D == 3) {
+      vx += U2[0] * sg[6] + U2[1] * sg[7] + U2[2] * sg[8];
+      vy += U2[3] * sg[6] + U2[4] * sg[7] + U2[5] * sg[8];
+      vz -= U2[0] * sg[0] + U2[1] * sg[1] + U2[2] * sg[2] + U2[3] * sg[6] + U2[4] * sg[7] + U2[5] * sg[8];
+    }
+    vx += st * ((U0[0] + U0[4]) + U0[8]);
+    vy += st * ((U1[0] + U1[4]) + U1[8]);
+    vx -= st * (U0[0] + U1[1]);
+    vy -= st * (U0[3] + U1[4]);
+    vz -= st * (U0[6] + U1[7]);
+    if constexpr (D == 3) {
+      vz += st * ((U2[0] + U2[4]) + U2[8]);
+      vx -= st * U2[2];
+      vy -= st * U2[5];
+      vz -= st * U2[8];
+    }
+    if constexpr (DYN == DYN_GLIDE_ONLY) {
+      // lvp projection (NEWGLIDEONLY, :389-452)
+      const T txr = (U0[0] + U0[4]) + U0[8], tyr = (U1[0] + U1[4]) + U1[8];
+      T rx = U0[0] + U1[1], ry = U0[3] + U1[4], rz = U0[6] + U1[7];
+      T qx = (U0[4] + U0[8]) - U1[1], qy = (U1[0] + U1[8]) - U0[3], qz = -rz;
+      T rr = ME + txr * txr + tyr * tyr;
+      if constexpr (D == 3) {
+        const T tzr = (U2[0] + U2[4]) + U2[8];
+        rx += U2[2];
+        ry += U2[5];
+        rz += U2[8];
+        qx -= U2[2];
+        qy -= U2[5];
+        qz = (U2[0] + U2[4]) - U0[6] - U1[7];
+        rr += tzr * tzr - T(2) * tzr * rz;
+      }
+      rr += rx * rx + ry * ry + rz * rz - T(2) * txr * rx - T(2) * tyr * ry;
+      const T rv = (ME + vx * qx + vy * qy + vz * qz) / rr;
+      vx += qx * rv;
+      vy += qy * rv;
+      vz += qz * rv;
+    }
+    // |rho| without ME, negative values clamped (:232-287)
+    T r = T(0);
+#pragma unroll
+    for (int c = 3; c < 9; ++c) r += U0[c] * U0[c];
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      if (c < 3 || c >= 6) r += U1[c] * U1[c];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r -= T(2) * U0[3 + j] * U1[j];
+    if constexpr (D == 3) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) r += U2[c] * U2[c];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) r -= T(2) * U0[6 + j] * U2[j] + T(2) * U1[6 + j] * U2[3 + j];
+    }
+    const T irho = T(1) / (sqrt(fmax(r, T(0))) + sqME);
+    const T v[3] = {vx * irho, vy * irho, vz * irho};
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      vbuf[(3 * k + l) * vstride] = v[l];
+      amax = fmax(amax, fabs(v[l]));
+    }
+  }
+  T Y[D][2][3];
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) Y[a][s][l] = T(0);
+#pragma unroll
+  for (int k = 0; k < NCOR; ++k) {
+    const int sgn[3] = {k & 1, (k >> 1) & 1, (k >> 2) & 1};
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      const T v = vbuf[(3 * k + l) * vstride];
+#pragma unroll
+      for (int a = 0; a < D; ++a) Y[a][sgn[a]][l] += v;
+    }
+  }
+  // Z(a, c, l) = sum_k u_k[a][c] v_k[l]
+  auto Z = [&](int a, int c, int l) -> T { return u[a][0][c] * Y[a][0][l] + u[a][1][c] * Y[a][1][l]; };
+  T Jsum = T(0);
+  if constexpr (DYN == DYN_GLIDE_CLIMB) {
+    // sum over corners of the climb rate q (:468-545)
+    T Q = Z(0, 4, 0) + Z(0, 8, 0) - Z(1, 1, 0) + Z(1, 0, 1) + Z(1, 8, 1) - Z(0, 3, 1) - Z(0, 6, 2) - Z(1, 7, 2);
+    if constexpr (D == 3) Q += -Z(2, 2, 0) - Z(2, 5, 1) + Z(2, 0, 2) + Z(2, 4, 2);
+    Jsum = Q / T(3) * Lambda;
+  }
+  const T at = amax + amax + ME;
+  T aD = amax * amax, atD = at * at;
+  if constexpr (D == 3) {
+    aD *= amax;
+    atD *= at;
+  }
+  const T wH = aD / atD;
+  const T wd = amax * amax / at;
+  speed = at * T(D);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int cc = 3 * i + j;
+      T h = (i == j) ? -Jsum : T(0);
+#pragma unroll
+      for (int a = 0; a < D; ++a) h += Z(a, cc, a);
+      if (i < D) {
+#pragma unroll
+        for (int l = 0; l < 3; ++l) h -= Z(i % D, 3 * l + j, l);
+      }
+      T r = -h * wH;
+#pragma unroll
+      for (int a = 0; a < D; ++a) r += (u[a][0][cc] - u[a][1][cc]) * wd;
+      rhs[cc] = r;
+    }
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T block_max(T v, T* red) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
@@ -322,7 +496,9 @@ struct FluxTile {
   static constexpr int FILL = (TILE + NT - 1) / NT; // fill loads per thread
 };
 
-template <typename T, int D, int DYN, int FTC, int FTR, bool PRE>
+// LLF: the CUDA executable's flux (llf_derivs, cell_flux_llf) in place of
+// the central-upwind one.
+template <typename T, int D, int DYN, int FTC, int FTR, bool PRE, bool LLF = false>
 __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   using FT = FluxTile<FTC, FTR>;
   constexpr int FPW = FT::PW, FTILE = FT::TILE, FNT = FT::NT, FFILL = FT::FILL;
@@ -458,16 +634,19 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
     fc[c] = ctr;
     {  // contiguous axis (last)
       T f[5] = {tb[lr * FPW + lc - 2], tb[lr * FPW + lc - 1], ctr, tb[lr * FPW + lc + 1], tb[lr * FPW + lc + 2]};
-      limited_derivs<T>(f, u[D - 1][0][c], u[D - 1][1][c]);
+      if constexpr (LLF) llf_derivs<T>(f, T(n), u[D - 1][0][c], u[D - 1][1][c]);
+      else limited_derivs<T>(f, u[D - 1][0][c], u[D - 1][1][c]);
     }
     {  // row axis (second last)
       T f[5] = {tb[(lr - 2) * FPW + lc], tb[(lr - 1) * FPW + lc], ctr, tb[(lr + 1) * FPW + lc],
                 tb[(lr + 2) * FPW + lc]};
-      limited_derivs<T>(f, u[D - 2][0][c], u[D - 2][1][c]);
+      if constexpr (LLF) llf_derivs<T>(f, T(n), u[D - 2][0][c], u[D - 2][1][c]);
+      else limited_derivs<T>(f, u[D - 2][0][c], u[D - 2][1][c]);
     }
     if constexpr (D == 3) {
       T f[5] = {xn[c][0], xn[c][1], ctr, xn[c][2], xn[c][3 % NX]};
-      limited_derivs<T>(f, u[0][0][c], u[0][1][c]);
+      if constexpr (LLF) llf_derivs<T>(f, T(n), u[0][0][c], u[0][1][c]);
+      else limited_derivs<T>(f, u[0][0][c], u[0][1][c]);
     }
   }
   // the asynchronous G/U loads must have landed before LDS is read
@@ -495,7 +674,8 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
     sg[5] = sg[7] = g5 - mu * (fc[5] + fc[7]);
   }
   T rhs[9], speed;
-  cell_flux<T, D, DYN, PRE>(u, sg, T(job.Lambda), rhs, speed, &vsh[0][tid], FNT, T(n));
+  if constexpr (LLF) cell_flux_llf<T, D, DYN>(u, sg, T(job.Lambda), rhs, speed, &vsh[0][tid], FNT);
+  else cell_flux<T, D, DYN, PRE>(u, sg, T(job.Lambda), rhs, speed, &vsh[0][tid], FNT, T(n));
   if (job.mode == 0) {
 #pragma unroll
     for (int c = 0; c < 9; ++c) outp[c * cells + idx] = rhs[c];
@@ -785,7 +965,7 @@ static int march_chunk(int n) {
 }
 
 static bool march_ok(const FluxJob& job) {
-  return job.d == 3 && !job.fp32 && !job.speed && job.count <= 1 && job.tile == 0 && job.n >= 32 &&
+  return job.d == 3 && !job.fp32 && !job.llf && !job.speed && job.count <= 1 && job.tile == 0 && job.n >= 32 &&
          job.dyn != DYN_WOJZZ && (job.mode == 0 || job.U) && march_chunk(job.n) > 0;
 }
 
@@ -844,10 +1024,35 @@ static void launch_flux_t(const FluxJob& job, dim3 g, dim3 b) {
   else launch_flux_tt<double, TC, TR>(job, g, b);
 }
 
+// The LLF flux: fp64, one instance, glide + climb or glide only, default tile.
+template <int TC, int TR>
+static void launch_flux_llf(const FluxJob& job, dim3 g, dim3 b) {
+  if (job.d == 3) {
+    if (job.dyn == DYN_GLIDE_ONLY)
+      hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_ONLY, TC, TR, true, true>), g, b, 0, job.stream, job);
+    else hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_CLIMB, TC, TR, true, true>), g, b, 0, job.stream, job);
+  } else {
+    if (job.dyn == DYN_GLIDE_ONLY)
+      hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_ONLY, TC, TR, true, true>), g, b, 0, job.stream, job);
+    else hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_CLIMB, TC, TR, true, true>), g, b, 0, job.stream, job);
+  }
+}
+
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
   if (job.B) return hipErrorInvalidValue;  // stage inputs are materialised before the flux pass
   if (job.dyn == DYN_WOJZZ && job.d != 2) return hipErrorInvalidValue;
   if (job.count > 1 && job.d != 2) return hipErrorInvalidValue;
+  if (job.llf) {
+    if (job.fp32 || job.count > 1 || job.dyn == DYN_WOJZZ) return hipErrorInvalidValue;
+    dim3 g, b;
+    flux_geometry(job.d, job.n, 0, 1, g, b);
+    if (nblocks_out) *nblocks_out = (int)g.x;
+    if (b.x == 32 && b.y == 8) launch_flux_llf<32, 8>(job, g, b);
+    else if (b.x == 16 && b.y == 16) launch_flux_llf<16, 16>(job, g, b);
+    else if (b.x == 8 && b.y == 8) launch_flux_llf<8, 8>(job, g, b);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+  }
   if (march_ok(job)) {
     const int chunk = march_chunk(job.n);
     const int nb = (job.n / march::TC) * (job.n / march::TR) * (job.n / chunk);
@@ -893,13 +1098,13 @@ __global__ void __launch_bounds__(1024) k_finalize_dt(const double* partial, int
       clk->dt = 0.0;
       return;
     }
-    double dt = clk->fixed_dt > 0.0 ? clk->fixed_dt : clk->cfl / (nmax * mm);
+    double dt = clk->fixed_dt > 0.0 ? clk->fixed_dt : clk->dt_rule ? clk->cfl / mm / nmax : clk->cfl / (nmax * mm);
     // the operator-splitting mover's bound on the adaptive step
     // (FieldMover.py:359-360; the plain TVD mover leaves it commented out)
     if (clk->fixed_dt <= 0.0 && dt_bound > 0.0 && dt > dt_bound) dt = dt_bound;
     const long long si = clk->nsteps - clk->sched_base;
     if (clk->dt_sched && si >= 0 && si < clk->sched_len) dt = clk->dt_sched[si];
-    if (clk->t + dt > clk->t_end) dt = clk->t_end - clk->t;
+    if (clk->t + dt > clk->t_end) dt = clk->t_end - clk->t + (clk->dt_rule ? kME : 0.0);
     clk->dt = dt;
   }
 }
@@ -969,7 +1174,7 @@ __global__ void k_fixed_dt(Clock* clk, int count) {
   double dt = clk->fixed_dt;
   const long long si = clk->nsteps - clk->sched_base;
   if (clk->dt_sched && si >= 0 && si < clk->sched_len) dt = clk->dt_sched[si];
-  if (clk->t + dt > clk->t_end) dt = clk->t_end - clk->t;
+  if (clk->t + dt > clk->t_end) dt = clk->t_end - clk->t + (clk->dt_rule ? kME : 0.0);
   clk->dt = dt;
 }
 
@@ -991,6 +1196,35 @@ hipError_t launch_finalize_dt(const double* partial, int nb, Clock* clk, double 
   if (partial)
     hipLaunchKernelGGL(k_finalize_dt, dim3(count), dim3(1024), 0, st, partial, nb, clk, nmax, pstride, dt_bound);
   else hipLaunchKernelGGL(k_fixed_dt, dim3(1), dim3(count), 0, st, clk, count);
+  return hipGetLastError();
+}
+
+// u += dt L over n values (simpleTVD's updateField, plasticity_kernel.cu:1105-1109,
+// 1130), two values per thread
+__global__ void __launch_bounds__(256) k_euler(double* __restrict__ u, const double* __restrict__ L, long long n,
+                                              const Clock* clk) {
+  if (clk->done) return;
+  const double dt = clk->dt;
+  const long long stride = 2LL * gridDim.x * blockDim.x;
+  for (long long i = 2LL * ((long long)blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) {
+    if (i + 1 < n) {
+      double2 a = *reinterpret_cast<const double2*>(u + i);
+      const double2 l = *reinterpret_cast<const double2*>(L + i);
+      a.x += dt * l.x;
+      a.y += dt * l.y;
+      *reinterpret_cast<double2*>(u + i) = a;
+    } else {
+      u[i] += dt * L[i];
+    }
+  }
+}
+
+hipError_t launch_euler(double* u, const double* L, long long n, const Clock* clk, hipStream_t st) {
+  if (n <= 0 || (reinterpret_cast<uintptr_t>(u) | reinterpret_cast<uintptr_t>(L)) % 16) return hipErrorInvalidValue;
+  long long blocks = (n / 2 + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_euler, dim3((unsigned)blocks), dim3(256), 0, st, u, L, n, clk);
   return hipGetLastError();
 }
 

@@ -89,6 +89,9 @@ struct FluxJob {
   // (tr out - tr U) at every cell
   double* vac;
   double vbeta;
+  // 1: the CUDA executable's local Lax-Friedrichs flux (fp64; glide + climb
+  // or glide only), plasticity_kernel.cu centralHJ :118-697
+  int llf;
 };
 
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out);
@@ -96,6 +99,8 @@ int flux_nblocks(int d, int n);
 hipError_t launch_finalize_dt(const double* partial, int nblocks, Clock* clk, double nmax, hipStream_t st,
                               int count = 1, long long pstride = 0, double dt_bound = 0.0);
 hipError_t launch_advance(Clock* clk, hipStream_t st, int count = 1);
+// forward Euler update u += dt L with the clock's dt (skipped once done)
+hipError_t launch_euler(double* u, const double* L, long long n, const Clock* clk, hipStream_t st);
 int flux_nblocks_tile(int d, int n, int tile);
 hipError_t launch_energy(const double* G, const double* beta, long long cells, double mu, double nu, double lam,
                          double* partial, int* nblocks, hipStream_t st, int fp32 = 0);

@@ -67,11 +67,21 @@ enum {
   CDD_LOAD_TENSOR = 4    /* sigma_ij += init_ij + rate_ij (t - t0); e.g. uniform shear */
 };
 
+/* Discretisation of the Hamilton-Jacobi flux.  CDD_SCHEME_CENTRAL_UPWIND is
+ * the Python stepper's (CentralUpwindHJ3D.py:63-271: one-sided speeds per
+ * axis, sqrt(|rho|^2 + ME)); CDD_SCHEME_LLF is the CUDA executable's
+ * (plasticity_kernel.cu centralHJ :118-697 under plasticity.h's LLF, DOUBLE,
+ * theta = 1: one local Lax-Friedrichs speed per cell, |rho| + sqrt(ME), its
+ * own limiter form, dt = CFL / max speed / N and an end step of
+ * t_end - t + ME, simpleTVD :1112-1137). */
+enum { CDD_SCHEME_CENTRAL_UPWIND = 0, CDD_SCHEME_LLF = 1 };
+
 typedef struct cdd_config {
   int ndim;          /* 2 or 3 */
   int n;             /* grid side: 8..512 powers of two in 3D; 8..1024 and 384, 768 in 2D */
   int dynamics;      /* CDD_GLIDE_CLIMB ('mdp' Lambda=1, 'gcd' Lambda=0, partial climb) or CDD_GLIDE_ONLY ('lvp') */
-  int rk_order;      /* 3: Shu-Osher TVD-RK3 (FieldMover.py:61-141); 2: SSP-RK2 */
+  int rk_order;      /* 3: Shu-Osher TVD-RK3 (FieldMover.py:61-141, plasticity_kernel.cu TVD3rd
+                        :1140-1206); 2: SSP-RK2; 1: forward Euler (simpleTVD, CDD_SCHEME_LLF only) */
   double Lambda;     /* climb suppression (CentralUpwindHJBetaPDynamics3D.py:126-133) */
   double mu, nu;     /* isotropic elasticity (PlasticityState.py:23) */
   double cfl;        /* CFLsafeFactor (FieldMover.py:16,96) */
@@ -83,6 +93,8 @@ typedef struct cdd_config {
   double load_init_strain;
   double load_uniaxial_rate;
   int precision;     /* CDD_FP64 (default, 0) or CDD_FP32 */
+  int scheme;        /* CDD_SCHEME_CENTRAL_UPWIND (default, 0) or CDD_SCHEME_LLF (float64; glide+climb
+                        or glide-only; no batching, vacancies or adjoint) */
 } cdd_config;
 
 typedef struct cdd_plan cdd_plan;

@@ -1058,18 +1058,37 @@ def cuda_euler_step(flux, t, u, t_end, cfl, N):
     return u + dt * rhs, dt
 
 
-def run_cuda(dyn, beta, t0, nsteps, cfl=0.5, t_end=np.inf):
+def cuda_tvd3_step(flux, t, u, t_end, cfl, N):
+    """TVD3rd (plasticity_kernel.cu:1140-1206): the Shu-Osher stages of the
+    Python mover (every stage at the step's start time) with simpleTVD's dt
+    rule and end clamp.  Pinned against tvd_rk3_step, whose stages it shares,
+    by tests/test_oracle_cuda.py."""
+    rhs, speed = flux(t, u)
+    dt = cfl / speed.max() / N
+    if t + dt > t_end:
+        dt = t_end - t + ME
+    L0 = dt * rhs
+    F0 = u + L0
+    rhs, _ = flux(t, F0)
+    F1 = 3. / 4. * u + 1. / 4. * F0 + 1. / 4. * (dt * rhs)
+    rhs, _ = flux(t, F1)
+    return 1. / 3. * u + 2. / 3. * F1 + 2. / 3. * (dt * rhs), dt
+
+
+def run_cuda(dyn, beta, t0, nsteps, cfl=0.5, t_end=np.inf, order=1):
     """The CUDA executable's run loop (plasticity.cu:242-271) for a fixed
-    number of steps with the LLF flux and forward Euler; dyn gives Lambda
-    (its 'lambda'), glide-only (NEWGLIDEONLY), elastic constants and the load.
+    number of steps with the LLF flux and forward Euler (TVDstep = simpleTVD,
+    plasticity.h:9) or, order 3, TVD3rd; dyn gives Lambda (its 'lambda'),
+    glide-only (NEWGLIDEONLY), elastic constants and the load.
     Returns (u, t, dts)."""
     glide_only = dyn.dynamics == GLIDE_ONLY
     lam = 0.0 if glide_only else dyn.Lambda
     N = beta.shape[-1]
+    step = cuda_euler_step if order == 1 else cuda_tvd3_step
     t, u, dts = t0, beta, []
     for _ in range(nsteps):
-        u, dt = cuda_euler_step(lambda tt, uu: cuda_llf_flux(uu, dyn.get_sigma(uu, tt), lam, glide_only),
-                                t, u, t_end, cfl, N)
+        u, dt = step(lambda tt, uu: cuda_llf_flux(uu, dyn.get_sigma(uu, tt), lam, glide_only),
+                     t, u, t_end, cfl, N)
         t += dt
         dts.append(dt)
         if t >= t_end:

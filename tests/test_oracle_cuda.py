@@ -117,3 +117,26 @@ def test_euler_step_and_end_clamp():
     assert dts[0] == dt and np.array_equal(u1, b + dt * rhs)
     u2, t2, dts2 = O.run_cuda(dyn, b, 0.0, 5, 0.5, t_end=0.5 * dt)
     assert len(dts2) == 1 and dts2[0] == 0.5 * dt + O.ME
+
+
+def test_tvd3_step_is_the_python_rk3_with_the_executables_dt():
+    """With a flux whose speed makes both dt rules agree (N a power of two)
+    and no end clamp, TVD3rd equals the Python mover's RK3 step."""
+    b = SpectralGaussianField((16, 16), 2.0, 0.01, seed=7)
+    dyn = O.Dynamics(O.GLIDE_CLIMB, 0.3, MU, NU)
+
+    def f(tt, uu):
+        return O.cuda_llf_flux(uu, dyn.get_sigma(uu, tt), 0.3)
+    u3, dt3 = O.cuda_tvd3_step(f, 0.0, b, np.inf, 0.5, 16)
+    ur, dtr = O.tvd_rk3_step(f, 0.0, b, np.inf, 0.5)
+    assert dt3 == dtr
+    assert relmax(u3, ur) < 1e-14
+    # end clamp lands ME past the end time
+    u4, dt4 = O.cuda_tvd3_step(f, 0.0, b, 0.25 * dt3, 0.5, 16)
+    assert dt4 == 0.25 * dt3 + O.ME
+    # the RK3 amplification on a linear flux L(u) = -k u: 1 - z + z^2/2 - z^3/6
+    k = 3.0
+    lin = lambda tt, uu: (-k * uu, np.full(uu.shape[1:], 1.0))
+    u5, dt5 = O.cuda_tvd3_step(lin, 0.0, b, np.inf, 0.5, 16)
+    z = k * dt5
+    assert relmax(u5, b * (1 - z + z * z / 2 - z ** 3 / 6)) < 1e-14
