@@ -56,6 +56,7 @@ EXPORTS = [
     "cdd_segment_vjp", "cdd_lambda_bar", "cdd_stress_record", "cdd_plas_append", "cdd_plas_resume",
     "cdd_energy_record", "cdd_status", "cdd_instability", "cdd_segment_store_bytes", "cdd_segment_forward",
     "cdd_segment_vjp_stored", "cdd_set_vacancy", "cdd_vacancy_set", "cdd_vacancy_get",
+    "cdd_set_disorder", "cdd_disorder_get", "cdd_disorder_sigma", "cdd_disorder_flux",
 ]
 KERNEL_KINDS = ["r2c", "cols_fwd", "contract", "cols_inv", "c2r", "flux_stage1", "cfl_reduce",
                 "flux_stage2", "flux_stage3", "advance"]
@@ -121,6 +122,10 @@ def _load():
         "cdd_set_vacancy": ([P, I, D, D, D, D], I),
         "cdd_vacancy_set": ([P, P, I], I),
         "cdd_vacancy_get": ([P, P, I], I),
+        "cdd_set_disorder": ([P, P, I, I], I),
+        "cdd_disorder_get": ([P, P, I], I),
+        "cdd_disorder_sigma": ([P, P, P, D, P], I),
+        "cdd_disorder_flux": ([P, P, P, D, P, P, P], I),
         "cdd_stress_record": ([P, P, ctypes.c_longlong], I),
         "cdd_energy_record": ([P, P, ctypes.c_longlong], I),
         "cdd_status": ([P, ctypes.POINTER(I)], I),
@@ -344,6 +349,46 @@ class Plan:
         out = self.torch.empty(self.shape, dtype=self.torch.float64, device=self.dev)
         check(lib.cdd_vacancy_get(self.h, ctypes.c_void_p(out.data_ptr()), 1))
         return out
+
+    def _scalar_field(self, d):
+        t = self.torch.as_tensor(d, dtype=self.torch.float64)
+        if tuple(t.shape) != self.shape:
+            raise CDDError("the disorder field must have the grid shape %s" % (self.shape,))
+        return t.to(self.dev).contiguous()
+
+    def set_disorder(self, d, advect=True):
+        """Disorder coupling: every stress solve gains the disorder term of d
+        (DisorderState.CalculateSigma); advect=True carries d with the flux
+        and the RK stages as the ('s', 's') component of a DisorderState,
+        advect=False keeps it fixed (BetaP_DisorderDynamics).  None disables."""
+        if d is None:
+            check(lib.cdd_set_disorder(self.h, None, 0, 0))
+            return
+        t = self._scalar_field(d)
+        check(lib.cdd_set_disorder(self.h, _ptr(t), 1, 1 if advect else 0))
+
+    def disorder_get(self):
+        """The (advected) disorder field as a device tensor."""
+        out = self.torch.empty(self.shape, dtype=self.torch.float64, device=self.dev)
+        check(lib.cdd_disorder_get(self.h, _ptr(out), 1))
+        return out
+
+    def disorder_sigma(self, beta, d, t=0.0):
+        """The stress of a DisorderState (nine planes, not symmetric)."""
+        b, dd = self._as_dev(beta), self._scalar_field(d)
+        out = self.empty()
+        check(lib.cdd_disorder_sigma(self.h, _ptr(b), _ptr(dd), float(t), _ptr(out)))
+        return out
+
+    def disorder_flux(self, beta, d, t=0.0, advect=True):
+        """(rhs of beta, rhs of d or None, speed) of the disorder dynamics."""
+        b, dd = self._as_dev(beta), self._scalar_field(d)
+        rhs = self.empty()
+        rd = self.torch.empty(self.shape, dtype=self.torch.float64, device=self.dev) if advect else None
+        speed = self.torch.empty(self.shape, dtype=self.torch.float64, device=self.dev)
+        check(lib.cdd_disorder_flux(self.h, _ptr(b), _ptr(dd), float(t), _ptr(rhs), _ptr(rd) if advect else None,
+                                    _ptr(speed)))
+        return rhs, rd, speed
 
     def segment_store_bytes(self, nsteps):
         """Device bytes of one nsteps segment's stored trajectory."""

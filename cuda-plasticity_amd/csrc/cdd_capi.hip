@@ -32,6 +32,12 @@ int cdd_internal_fail(int code, const char* msg) { return fail(code, msg); }
     }                                                                              \
   } while (0)
 
+#define RCHK(x)          \
+  do {                   \
+    int rc_ = (x);       \
+    if (rc_) return rc_; \
+  } while (0)
+
 struct cdd_plan {
   cdd_config cfg;
   int d, n;
@@ -81,6 +87,14 @@ struct cdd_plan {
   double* vac;
   int vacancy;
   double valpha, vbeta, vgamma, dt_bound;
+  // disorder (cdd_set_disorder): the field d, advected by the flux
+  // (DisorderState) or fixed (BetaP_DisorderDynamics); F0 and F1 stage
+  // planes of the advected field; the nine planes of the disorder stress
+  int disorder, dis_adv;
+  double* dis_u;
+  double* dis_L;
+  double* dis_F1;
+  double* dsig;
   // HIP-event timing of every kernel launched by cdd_step (optional)
   bool timing;
   std::vector<hipEvent_t>* ev;
@@ -277,6 +291,10 @@ int cdd_plan_destroy(cdd_plan* p) {
   (void)hipFree(p->epart);
   (void)hipFree(p->lseg);
   (void)hipFree(p->vac);
+  (void)hipFree(p->dis_u);
+  (void)hipFree(p->dis_L);
+  (void)hipFree(p->dis_F1);
+  (void)hipFree(p->dsig);
   if (p->own) (void)hipStreamDestroy(p->own);
   if (p->ev) {
     for (hipEvent_t e : *p->ev) (void)hipEventDestroy(e);
@@ -381,6 +399,20 @@ static void vac_stress(const cdd_plan* p, SpectralJob& j) {
   }
 }
 
+// The disorder stress of the state beta with disorder field dfield, into
+// p->dsig (DisorderState.CalculateSigma's added term, DisorderState.py:
+// 17-29): the spectral Nye tensor, rho' = d rho / (ME + 2 |rho|^2) in place,
+// then sum_lt eps_lit irfftn(k_l rfftn(rho'_tj)) as a nine-component
+// spectral pass.  clk: the stepper's clock (the passes skip once it is done).
+static int disorder_stress(cdd_plan* p, const double* beta, const double* dfield, const Clock* clk) {
+  HIPCHK(launch_spectral(spectral_job(p, 9, beta, nullptr, clk, 0.0, p->dsig)));
+  HIPCHK(launch_disorder_weight(p->dsig, dfield, p->cells, p->stream));
+  SpectralJob jc = spectral_job(p, 9, p->dsig, nullptr, clk, 0.0, p->dsig);
+  jc.gp.curl = 1;
+  HIPCHK(launch_spectral(jc));
+  return CDD_OK;
+}
+
 static FluxJob flux_job(cdd_plan* p) {
   FluxJob f;
   memset(&f, 0, sizeof f);
@@ -472,6 +504,15 @@ static int launch_step(cdd_plan* p, const StepBufs& b) {
   f.out = p->L0;
   f.mode = 0;
   f.partial = p->partial;
+  if (p->disorder) {
+    int rc = disorder_stress(p, u, p->dis_u, p->clk);
+    if (rc) return rc;
+    f.S9 = p->dsig;
+    if (p->dis_adv) {
+      f.Ad = p->dis_u;
+      f.outd = p->dis_L;
+    }
+  }
   int nb = 0;
   HIPCHK(launch_flux(f, &nb));
   if (p->timing) timing_hook(p, KK_FLUX1);
@@ -498,6 +539,22 @@ static int launch_step(cdd_plan* p, const StepBufs& b) {
   f2.A = b.F0;
   f2.U = u;
   f2.mode = 1;
+  if (p->disorder) {
+    // the advected field's F0 = d + dt L0_d, formed over L0_d
+    const double* d2 = p->dis_u;
+    if (p->dis_adv) {
+      HIPCHK(launch_axpy_dt(p->dis_L, p->dis_u, p->dis_L, p->cells, p->clk, p->stream));
+      d2 = p->dis_L;
+    }
+    int rc = disorder_stress(p, b.F0, d2, p->clk);
+    if (rc) return rc;
+    f2.S9 = p->dsig;
+    if (p->dis_adv) {
+      f2.Ad = p->dis_L;
+      f2.Ud = p->dis_u;
+      f2.outd = p->cfg.rk_order == 3 ? p->dis_F1 : p->dis_u;  // in place: Ud is read at the own cell only
+    }
+  }
   if (p->cfg.rk_order == 3) {
     f2.out = b.F1;
     f2.cu = 0.75;
@@ -517,6 +574,16 @@ static int launch_step(cdd_plan* p, const StepBufs& b) {
     f3.mode = 1;
     f3.cu = 1.0 / 3.0;
     f3.cf = 2.0 / 3.0;
+    if (p->disorder) {
+      int rc = disorder_stress(p, b.F1, p->dis_adv ? p->dis_F1 : p->dis_u, p->clk);
+      if (rc) return rc;
+      f3.S9 = p->dsig;
+      if (p->dis_adv) {
+        f3.Ad = p->dis_F1;
+        f3.Ud = p->dis_u;
+        f3.outd = p->dis_u;
+      }
+    }
     if (p->vacancy) {
       f3.vac = p->vac;
       f3.vbeta = p->vbeta;
@@ -573,6 +640,7 @@ int cdd_set_vacancy(cdd_plan* p, int enable, double alpha, double beta, double g
   if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "the vacancy coupling is float64 only");
   if (p->cfg.scheme != CDD_SCHEME_CENTRAL_UPWIND)
     return fail(CDD_ERR_UNSUPPORTED, "the vacancy coupling uses the central-upwind scheme");
+  if (p->disorder) return fail(CDD_ERR_UNSUPPORTED, "the vacancy and disorder couplings do not combine");
   if (!(gamma >= 0.0) || !(dt_bound >= 0.0) || !isfinite(alpha) || !isfinite(beta))
     return fail(CDD_ERR_ARG, "bad vacancy parameters");
   HIPCHK(hipSetDevice(p->cfg.device));
@@ -587,6 +655,88 @@ int cdd_set_vacancy(cdd_plan* p, int enable, double alpha, double beta, double g
   p->vbeta = beta;
   p->vgamma = gamma;
   p->dt_bound = dt_bound;
+  return CDD_OK;
+}
+
+int cdd_set_disorder(cdd_plan* p, const void* dfield, int on_device, int advect) {
+  if (!p) return fail(CDD_ERR_ARG, "null plan");
+  if (!dfield) {
+    p->disorder = 0;
+    return CDD_OK;
+  }
+  if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "the disorder coupling is float64 only");
+  if (p->cfg.scheme != CDD_SCHEME_CENTRAL_UPWIND)
+    return fail(CDD_ERR_UNSUPPORTED, "the disorder coupling uses the central-upwind scheme");
+  if (p->vacancy) return fail(CDD_ERR_UNSUPPORTED, "the vacancy and disorder couplings do not combine");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  const size_t plane = sizeof(double) * (size_t)p->cells;
+  if (!p->dis_u && hipMalloc((void**)&p->dis_u, plane) != hipSuccess)
+    return fail(CDD_ERR_HIP, "hipMalloc failed (disorder field)");
+  if (!p->dsig && hipMalloc((void**)&p->dsig, 9 * plane) != hipSuccess)
+    return fail(CDD_ERR_HIP, "hipMalloc failed (disorder stress)");
+  if (advect) {
+    if (!p->dis_L && hipMalloc((void**)&p->dis_L, plane) != hipSuccess)
+      return fail(CDD_ERR_HIP, "hipMalloc failed (disorder stages)");
+    if (!p->dis_F1 && hipMalloc((void**)&p->dis_F1, plane) != hipSuccess)
+      return fail(CDD_ERR_HIP, "hipMalloc failed (disorder stages)");
+  }
+  HIPCHK(hipMemcpyAsync(p->dis_u, dfield, plane, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                        p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  p->disorder = 1;
+  p->dis_adv = advect ? 1 : 0;
+  return CDD_OK;
+}
+
+int cdd_disorder_get(cdd_plan* p, void* dfield, int on_device) {
+  if (!p || !dfield) return fail(CDD_ERR_ARG, "null argument");
+  if (!p->disorder) return fail(CDD_ERR_ARG, "the disorder coupling is not enabled (cdd_set_disorder)");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  HIPCHK(hipMemcpyAsync(dfield, p->dis_u, sizeof(double) * (size_t)p->cells,
+                        on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return CDD_OK;
+}
+
+static int dsig_alloc(cdd_plan* p) {
+  if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "the disorder coupling is float64 only");
+  if (!p->dsig && hipMalloc((void**)&p->dsig, 9 * sizeof(double) * (size_t)p->cells) != hipSuccess)
+    return fail(CDD_ERR_HIP, "hipMalloc failed (disorder stress)");
+  return CDD_OK;
+}
+
+int cdd_disorder_sigma(cdd_plan* p, const double* beta, const double* dfield, double t, double* sigma) {
+  if (!p || !beta || !dfield || !sigma) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  RCHK(dsig_alloc(p));
+  HIPCHK(launch_spectral(spectral_job(p, 6, beta, nullptr, nullptr, t, p->G)));
+  HIPCHK(launch_sigma_finalize(p->G, beta, sigma, p->cells, p->gp.mu, p->gp.lam, p->stream, 0));
+  RCHK(disorder_stress(p, beta, dfield, nullptr));
+  HIPCHK(launch_lincomb(9 * p->cells, 2, sigma, 1.0, p->dsig, 1.0, nullptr, 0.0, nullptr, 0.0, sigma, p->stream));
+  return CDD_OK;
+}
+
+int cdd_disorder_flux(cdd_plan* p, const double* beta, const double* dfield, double t, double* rhs, double* rhs_d,
+                      double* speed) {
+  if (!p || !beta || !dfield || !rhs) return fail(CDD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  RCHK(dsig_alloc(p));
+  if (p->cfg.scheme != CDD_SCHEME_CENTRAL_UPWIND)
+    return fail(CDD_ERR_UNSUPPORTED, "the disorder coupling uses the central-upwind scheme");
+  HIPCHK(launch_spectral(spectral_job(p, 6, beta, nullptr, nullptr, t, p->G)));
+  RCHK(disorder_stress(p, beta, dfield, nullptr));
+  FluxJob f = flux_job(p);
+  f.A = beta;
+  f.out = rhs;
+  f.speed = speed;
+  f.mode = 0;
+  f.S9 = p->dsig;
+  if (rhs_d) {
+    f.Ad = dfield;
+    f.outd = rhs_d;
+  }
+  HIPCHK(launch_flux(f, nullptr));
   return CDD_OK;
 }
 
@@ -631,6 +781,7 @@ int cdd_energy(cdd_plan* p, const void* beta_v, double t, double* energy) {
 static int adj_alloc(cdd_plan* p) {
   if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "the adjoint is float64 only");
   if (p->vacancy) return fail(CDD_ERR_UNSUPPORTED, "the adjoint does not include the vacancy coupling");
+  if (p->disorder) return fail(CDD_ERR_UNSUPPORTED, "the adjoint does not include the disorder coupling");
   if (p->cfg.scheme != CDD_SCHEME_CENTRAL_UPWIND)
     return fail(CDD_ERR_UNSUPPORTED, "the adjoint differentiates the central-upwind scheme");
   if (p->cfg.dynamics == CDD_GLIDE_CLIMB_WOJZZ)
@@ -720,12 +871,6 @@ static int flux_into(cdd_plan* p, const double* beta, double t, double* out) {
   HIPCHK(launch_flux(f, nullptr));
   return CDD_OK;
 }
-
-#define RCHK(x)          \
-  do {                   \
-    int rc_ = (x);       \
-    if (rc_) return rc_; \
-  } while (0)
 
 int cdd_flux_vjp(cdd_plan* p, const double* beta, double t, const double* rbar, double* betabar, double* lambar) {
   if (!p || !beta || !rbar || !betabar) return fail(CDD_ERR_ARG, "null argument");
@@ -985,6 +1130,7 @@ int cdd_segment_forward(cdd_plan* p, int nsteps, void* store, unsigned long long
   if (!p || !store || nsteps < 1) return fail(CDD_ERR_ARG, "bad argument");
   if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "stored trajectories are float64 only");
   if (p->vacancy) return fail(CDD_ERR_UNSUPPORTED, "the adjoint does not include the vacancy coupling");
+  if (p->disorder) return fail(CDD_ERR_UNSUPPORTED, "the adjoint does not include the disorder coupling");
   if (p->cfg.scheme != CDD_SCHEME_CENTRAL_UPWIND)
     return fail(CDD_ERR_UNSUPPORTED, "the adjoint differentiates the central-upwind scheme");
   if (store_bytes < SegLayout::bytes(nsteps, p->cfg.rk_order, p->cells))

@@ -68,9 +68,14 @@ __device__ __forceinline__ void limited_derivs(const T f[5], T& up, T& um) {
 // velocities between the speed pass and the Hamiltonian pass.
 // PRE: keep the per-axis velocity numerators w, z in registers (fewer
 // flops) or recompute them at every corner (fewer live registers).
-template <typename T, int D, int DYN, bool PRE>
+// SC: also the rhs of a scalar carried by the same flux (the disorder field
+// of a DisorderState: H_s = sum_a v_a s_a, no subtraction or climb term,
+// CentralUpwindHJBetaPDynamics3D.py:186-188); usc[a][s] its derivatives in
+// the units of u.
+template <typename T, int D, int DYN, bool PRE, bool SC = false>
 __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9], T Lambda, T (&rhs)[9],
-                                          T& speed, T* __restrict__ vbuf, int vstride, T nscale) {
+                                          T& speed, T* __restrict__ vbuf, int vstride, T nscale,
+                                          const T (*usc)[2] = nullptr, T* rhs_sc = nullptr) {
   constexpr int NCOR = 1 << D;
   const T ME = T(kME);
   const T MEU = ME / (nscale * nscale);  // ME in the units of u^2
@@ -292,6 +297,17 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
       rhs[cc] = (DYN == DYN_WOJZZ && cc == 8) ? T(0) : r * nscale;  // woJzz holds beta_zz fixed
     }
   }
+  if constexpr (SC) {
+    T h = T(0);
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int s = 0; s < 2; ++s) h += usc[a][s] * Y[a][s][a];
+    T r = -h * inv;
+#pragma unroll
+    for (int a = 0; a < D; ++a) r += (usc[a][0] - usc[a][1]) * tw[a];
+    *rhs_sc = r * nscale;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -498,7 +514,8 @@ struct FluxTile {
 
 // LLF: the CUDA executable's flux (llf_derivs, cell_flux_llf) in place of
 // the central-upwind one.
-template <typename T, int D, int DYN, int FTC, int FTR, bool PRE, bool LLF = false>
+// SC: the advected disorder scalar of FluxJob (Ad/Ud/outd) rides along.
+template <typename T, int D, int DYN, int FTC, int FTR, bool PRE, bool LLF = false, bool SC = false>
 __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   using FT = FluxTile<FTC, FTR>;
   constexpr int FPW = FT::PW, FTILE = FT::TILE, FNT = FT::NT, FFILL = FT::FILL;
@@ -520,7 +537,8 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   }
   if (job.clk) job.clk += blockIdx.y;
   if (job.clk && job.clk->done) return;
-  __shared__ T tile[9][FTILE];
+  constexpr int NCOMP = SC ? 10 : 9;
+  __shared__ T tile[NCOMP][FTILE];
   __shared__ T vsh[4 * (1 << D)][FNT];
   __shared__ T red[(FNT + 63) / 64];
   // 3D: per-wave LDS landing zone for G (planes 0-5) and U (planes 6-15),
@@ -606,19 +624,21 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
       }
     }
   }
-  T tf[9][FFILL];
-  T xn[9][NX];
+  T tf[NCOMP][FFILL];
+  T xn[NCOMP][NX];
 #pragma unroll
-  for (int c = 0; c < 9; ++c) {
+  for (int c = 0; c < NCOMP; ++c) {
+    // component 9: the advected scalar's plane
+    const T* Ac = (SC && c == 9) ? reinterpret_cast<const T*>(job.Ad) : A + c * cells;
 #pragma unroll
-    for (int k = 0; k < FFILL; ++k) tf[c][k] = fval[k] ? A[c * cells + foff[k]] : T(0);
+    for (int k = 0; k < FFILL; ++k) tf[c][k] = fval[k] ? Ac[foff[k]] : T(0);
     if constexpr (D == 3) {
 #pragma unroll
-      for (int k = 0; k < NX; ++k) xn[c][k] = A[c * cells + xoff[k]];
+      for (int k = 0; k < NX; ++k) xn[c][k] = Ac[xoff[k]];
     }
   }
 #pragma unroll
-  for (int c = 0; c < 9; ++c)
+  for (int c = 0; c < NCOMP; ++c)
 #pragma unroll
     for (int k = 0; k < FFILL; ++k)
       if (fval[k]) tile[c][tid + k * FNT] = tf[c][k];
@@ -649,6 +669,19 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
       else limited_derivs<T>(f, u[0][0][c], u[0][1][c]);
     }
   }
+  T usc[D][2], fsc = T(0);
+  if constexpr (SC) {
+    const T* tb = tile[9];
+    fsc = tb[lr * FPW + lc];
+    T f[5] = {tb[lr * FPW + lc - 2], tb[lr * FPW + lc - 1], fsc, tb[lr * FPW + lc + 1], tb[lr * FPW + lc + 2]};
+    limited_derivs<T>(f, usc[D - 1][0], usc[D - 1][1]);
+    T g[5] = {tb[(lr - 2) * FPW + lc], tb[(lr - 1) * FPW + lc], fsc, tb[(lr + 1) * FPW + lc], tb[(lr + 2) * FPW + lc]};
+    limited_derivs<T>(g, usc[D - 2][0], usc[D - 2][1]);
+    if constexpr (D == 3) {
+      T h[5] = {xn[9][0], xn[9][1], fsc, xn[9][2], xn[9][3 % NX]};
+      limited_derivs<T>(h, usc[0][0], usc[0][1]);
+    }
+  }
   // the asynchronous G/U loads must have landed before LDS is read
   if constexpr (DMA) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   auto Gv = [&](int c) -> T {
@@ -673,9 +706,21 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
     sg[2] = sg[6] = g4 - mu * (fc[2] + fc[6]);
     sg[5] = sg[7] = g5 - mu * (fc[5] + fc[7]);
   }
-  T rhs[9], speed;
+  if constexpr (sizeof(T) == 8) {
+    // the disorder stress (non-symmetric), DisorderState.py:26-29
+    if (job.S9) {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) sg[c] += job.S9[c * cells + idx];
+    }
+  }
+  T rhs[9], speed, rhs_sc = T(0);
   if constexpr (LLF) cell_flux_llf<T, D, DYN>(u, sg, T(job.Lambda), rhs, speed, &vsh[0][tid], FNT);
-  else cell_flux<T, D, DYN, PRE>(u, sg, T(job.Lambda), rhs, speed, &vsh[0][tid], FNT, T(n));
+  else cell_flux<T, D, DYN, PRE, SC>(u, sg, T(job.Lambda), rhs, speed, &vsh[0][tid], FNT, T(n), usc, &rhs_sc);
+  if constexpr (SC) {
+    double* od = job.outd;
+    if (job.mode == 0) od[idx] = (double)rhs_sc;
+    else od[idx] = job.cu * job.Ud[idx] + job.cf * ((double)fsc + (double)dt * (double)rhs_sc);
+  }
   if (job.mode == 0) {
 #pragma unroll
     for (int c = 0; c < 9; ++c) outp[c * cells + idx] = rhs[c];
@@ -699,7 +744,7 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   if (partialp) {
     // a non-finite speed or rhs must reach the CFL reduction (fmax drops NaN,
     // also inside the speeds, so the rhs is checked too)
-    bool fin = isfinite(speed);
+    bool fin = isfinite(speed) && isfinite(rhs_sc);
 #pragma unroll
     for (int c = 0; c < 9; ++c) fin = fin && isfinite(rhs[c]);
     T m = block_max<T>(fin ? speed : T(INFINITY), red);
@@ -965,7 +1010,8 @@ static int march_chunk(int n) {
 }
 
 static bool march_ok(const FluxJob& job) {
-  return job.d == 3 && !job.fp32 && !job.llf && !job.speed && job.count <= 1 && job.tile == 0 && job.n >= 32 &&
+  return job.d == 3 && !job.fp32 && !job.llf && !job.S9 && !job.Ad && !job.speed && job.count <= 1 &&
+         job.tile == 0 && job.n >= 32 &&
          job.dyn != DYN_WOJZZ && (job.mode == 0 || job.U) && march_chunk(job.n) > 0;
 }
 
@@ -1038,8 +1084,38 @@ static void launch_flux_llf(const FluxJob& job, dim3 g, dim3 b) {
   }
 }
 
+// The flux with the advected disorder scalar: fp64, one instance, default tile.
+template <int TC, int TR>
+static void launch_flux_sc(const FluxJob& job, dim3 g, dim3 b) {
+  if (job.dyn == DYN_WOJZZ) {
+    hipLaunchKernelGGL((k_flux<double, 2, DYN_WOJZZ, TC, TR, true, false, true>), g, b, 0, job.stream, job);
+  } else if (job.d == 3) {
+    if (job.dyn == DYN_GLIDE_ONLY)
+      hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_ONLY, TC, TR, true, false, true>), g, b, 0, job.stream, job);
+    else
+      hipLaunchKernelGGL((k_flux<double, 3, DYN_GLIDE_CLIMB, TC, TR, true, false, true>), g, b, 0, job.stream, job);
+  } else {
+    if (job.dyn == DYN_GLIDE_ONLY)
+      hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_ONLY, TC, TR, true, false, true>), g, b, 0, job.stream, job);
+    else
+      hipLaunchKernelGGL((k_flux<double, 2, DYN_GLIDE_CLIMB, TC, TR, true, false, true>), g, b, 0, job.stream, job);
+  }
+}
+
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
   if (job.B) return hipErrorInvalidValue;  // stage inputs are materialised before the flux pass
+  if ((job.S9 || job.Ad) && (job.fp32 || job.count > 1 || job.llf)) return hipErrorInvalidValue;
+  if (job.Ad && (!job.outd || (job.mode != 0 && !job.Ud))) return hipErrorInvalidValue;
+  if (job.Ad) {
+    dim3 g, b;
+    flux_geometry(job.d, job.n, 0, 1, g, b);
+    if (nblocks_out) *nblocks_out = (int)g.x;
+    if (b.x == 32 && b.y == 8) launch_flux_sc<32, 8>(job, g, b);
+    else if (b.x == 16 && b.y == 16) launch_flux_sc<16, 16>(job, g, b);
+    else if (b.x == 8 && b.y == 8) launch_flux_sc<8, 8>(job, g, b);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+  }
   if (job.dyn == DYN_WOJZZ && job.d != 2) return hipErrorInvalidValue;
   if (job.count > 1 && job.d != 2) return hipErrorInvalidValue;
   if (job.llf) {
@@ -1217,6 +1293,25 @@ __global__ void __launch_bounds__(256) k_euler(double* __restrict__ u, const dou
       u[i] += dt * L[i];
     }
   }
+}
+
+// out = u + dt L (the advected disorder field's F0 = u + dt L0, formed over
+// L0 in place)
+__global__ void __launch_bounds__(256) k_axpy_dt(double* out, const double* u, const double* L, long long n,
+                                                const Clock* clk) {
+  if (clk->done) return;
+  const double dt = clk->dt;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = u[i] + dt * L[i];
+}
+
+hipError_t launch_axpy_dt(double* out, const double* u, const double* L, long long n, const Clock* clk,
+                          hipStream_t st) {
+  if (n <= 0) return hipErrorInvalidValue;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_axpy_dt, dim3((unsigned)blocks), dim3(256), 0, st, out, u, L, n, clk);
+  return hipGetLastError();
 }
 
 hipError_t launch_euler(double* u, const double* L, long long n, const Clock* clk, hipStream_t st) {

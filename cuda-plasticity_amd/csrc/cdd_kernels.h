@@ -26,6 +26,8 @@ struct GreenParams {
   Elastic el;
   int log_stress;   // 1: the k = 0 thread appends the clock's stress-strain record
   double vgamma;    // nc = 1: diffusion filter exp(-vgamma dt k^2) (dt from clk)
+  int curl;         // nc = 9: the disorder stress sum_lt eps_lit k_l F_tj (real k_l) in
+                    // place of the Nye tensor's i k_l
 };
 
 // Called after each kernel launch with the kernel kind (HIP-event timing).
@@ -92,6 +94,14 @@ struct FluxJob {
   // 1: the CUDA executable's local Lax-Friedrichs flux (fp64; glide + climb
   // or glide only), plasticity_kernel.cu centralHJ :118-697
   int llf;
+  // disorder (fp64, one instance, per-tile kernel): S9 (nine planes) is
+  // added to the stress at every cell; Ad, the stencil source of an
+  // advected scalar (the disorder field), gives outd = its rhs (mode 0) or
+  // cu Ud + cf (Ad + dt rhs) (mode 1)
+  const double* S9;
+  const double* Ad;
+  const double* Ud;
+  double* outd;
 };
 
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out);
@@ -101,6 +111,11 @@ hipError_t launch_finalize_dt(const double* partial, int nblocks, Clock* clk, do
 hipError_t launch_advance(Clock* clk, hipStream_t st, int count = 1);
 // forward Euler update u += dt L with the clock's dt (skipped once done)
 hipError_t launch_euler(double* u, const double* L, long long n, const Clock* clk, hipStream_t st);
+// out = u + dt L with the clock's dt (out may alias L; skipped once done)
+hipError_t launch_axpy_dt(double* out, const double* u, const double* L, long long n, const Clock* clk,
+                          hipStream_t st);
+// rho'_ij = d rho_ij / (ME + sum_mn 2 rho_mn^2), in place over the nine planes
+hipError_t launch_disorder_weight(double* rho, const double* d, long long cells, hipStream_t st);
 int flux_nblocks_tile(int d, int n, int tile);
 hipError_t launch_energy(const double* G, const double* beta, long long cells, double mu, double nu, double lam,
                          double* partial, int* nblocks, hipStream_t st, int fp32 = 0);

@@ -356,6 +356,43 @@ __device__ __forceinline__ void green_rho(const cx<T>* b, cx<T>* out, T kx, T ky
   }
 }
 
+// The disorder stress of DisorderState.CalculateSigma (DisorderState.py:
+// 26-29): sum_{l,t} eps_lit k_l F_tj (times 1/N^d) -- the Nye tensor's
+// contraction without its factor i.  With the inverse passes reading the
+// kz = 0 and Nyquist planes as real (k_c2r_last), as numpy's irfftn does,
+// this is the reference's irfftn(k_l rfftn(rho'_tj)) for every l, t.
+template <typename T>
+__device__ __forceinline__ void green_curl(const cx<T>* b, cx<T>* out, T kx, T ky, T kz, T norm) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    out[j] = (b[3 + j] * kz - b[6 + j] * ky) * norm;
+    out[3 + j] = (b[6 + j] * kx - b[j] * kz) * norm;
+    out[6 + j] = (b[j] * ky - b[3 + j] * kx) * norm;
+  }
+}
+
+// rho'_ij = d rho_ij / rhomod, rhomod = ME + sum_mn 2 rho_mn rho_mn in
+// component order (DisorderState.py:19-25), in place.
+__global__ void __launch_bounds__(256) k_disorder_weight(double* __restrict__ rho, const double* __restrict__ d,
+                                                        long long cells) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= cells) return;
+  double r[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) r[c] = rho[c * cells + idx];
+  double m = kME;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) m = m + 2.0 * r[c] * r[c];
+  const double dv = d[idx];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) rho[c * cells + idx] = dv * r[c] / m;
+}
+
+hipError_t launch_disorder_weight(double* rho, const double* d, long long cells, hipStream_t st) {
+  hipLaunchKernelGGL(k_disorder_weight, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, rho, d, cells);
+  return hipGetLastError();
+}
+
 // Rows of spec laid out as (c * NR + row) * N + e.  grid.x = ceil(NR / TR).
 // 3D: row = kz * N + x, e = y;  2D: row = ky, e = x.
 // Forward FFT of every component row (one transform per TT threads), then
@@ -452,7 +489,8 @@ k_contract(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, GreenParams g
         for (int cc = 0; cc < 6; ++cc) o[cc] = cx<T>{T(s[CS9[cc]]), T(0)};
       }
     } else if constexpr (NC == 9) {
-      green_rho<T>(in, o, kx, ky, kz, T(gp.norm));
+      if (gp.curl) green_curl<T>(in, o, kx, ky, kz, T(gp.norm));
+      else green_rho<T>(in, o, kx, ky, kz, T(gp.norm));
     } else {
       // diffusion filter of the vacancy concentration (VacancyDynamics.py:
       // 40-70): kc exp(kSq * -gamma * dt), normalised for the inverse pass

@@ -161,6 +161,28 @@ int cdd_set_vacancy(cdd_plan* plan, int enable, double alpha, double beta, doubl
 int cdd_vacancy_set(cdd_plan* plan, const void* c, int on_device);
 int cdd_vacancy_get(cdd_plan* plan, void* c, int on_device);
 
+/* Disorder (FieldDynamics/DisorderDynamics.py:17-44,
+ * PlasticityStates/DisorderState.py:11-30, RunPython.py:196-214):
+ * the stress gains sum_{l,t} eps_lit irfftn(k_l rfftn(rho'_tj)) with
+ * rho' = d rho / (ME + 2 |rho|^2), rho the spectral Nye tensor, at every
+ * stage.  advect = 1: d is the state's ('s', 's') component of a
+ * DisorderState, carried by the same central-upwind flux (H_s = v . grad d)
+ * and RK stages; advect = 0: d is the fixed field of BetaP_DisorderDynamics.
+ * dfield: N^d float64 values; NULL disables.  float64, central-upwind
+ * scheme; excludes the vacancy coupling, the adjoint and batching.
+ * cdd_disorder_get reads the (advected) field back. */
+int cdd_set_disorder(cdd_plan* plan, const void* dfield, int on_device, int advect);
+int cdd_disorder_get(cdd_plan* plan, void* dfield, int on_device);
+/* DisorderState.CalculateSigma for beta (9 planes) and d (device pointers,
+ * float64): the state's stress plus the disorder term (9 planes, not
+ * symmetric). */
+int cdd_disorder_sigma(cdd_plan* plan, const double* beta, const double* dfield, double t, double* sigma);
+/* The flux of the disorder dynamics at (beta, d): rhs of beta (9 planes),
+ * rhs_d of the advected field (nullable: the fixed-field dynamics) and the
+ * CFL speed (nullable); device pointers. */
+int cdd_disorder_flux(cdd_plan* plan, const double* beta, const double* dfield, double t, double* rhs, double* rhs_d,
+                      double* speed);
+
 /* Total elastic energy sum_cells 1/2 S sigma sigma of beta (nullable: the
  * plan's state) at time t (PlasticityState.py:317-330, Observer.py:86). */
 int cdd_energy(cdd_plan* plan, const void* beta_dev, double t, double* energy);

@@ -831,6 +831,76 @@ def run_vacancy(dyn, vac, beta, c, t0, nsteps, cfl, order=3, t_end=np.inf, fixed
 
 
 # ---------------------------------------------------------------------------
+# Disorder: the stress of a disorder field d coupled to the dislocation
+# density (BetaP_DisorderDynamics.GetSigma, FieldDynamics/DisorderDynamics.py:
+# 22-39, with d held by the dynamics; DisorderState.CalculateSigma,
+# PlasticityStates/DisorderState.py:15-30, with d the state's ('s', 's')
+# component, the RunPython.py:196-214 'x' runs).
+# ---------------------------------------------------------------------------
+def disorder_weight(rho, d):
+    """rho'_ij = d rho_ij / rhomod, rhomod = ME + sum_mn 2 rho_mn rho_mn
+    accumulated in component order (DisorderState.py:19-25)."""
+    rhomod = ME
+    for c in range(9):
+        rhomod = rhomod + 2 * rho[c] * rho[c]
+    return np.stack([d * rho[c] / rhomod for c in range(9)])
+
+
+def disorder_curl(rhop, sig=None):
+    """sigma_ij += sum_{l,t} perm[l,i,t] irfftn(k_l rfftn(rho'_tj)) in the
+    loop order of DisorderState.py:26-29 (l = z skipped in 2D).  The k-space
+    factor is the real k_l, not i k_l, and irfftn keeps only the real part of
+    the kz = 0 and Nyquist planes, as written: along the last axis the
+    operator is |d/dz|, and along another axis a mode contributes only at
+    its Nyquist wavenumber.  Pinned by those single-mode closed forms
+    (tests/test_oracle_disorder.py)."""
+    shape = rhop.shape[1:]
+    d = len(shape)
+    kt = ktools(shape)
+    out = np.zeros((9,) + shape) if sig is None else sig.copy()
+    F = [rfftn(rhop[c]) for c in range(9)]
+    for i in range(3):
+        for j in range(3):
+            for l in range(3):
+                for t in range(3):
+                    if PERM[l, i, t] != 0 and (d == 3 or l != Z):
+                        out[cidx(i, j)] = out[cidx(i, j)] + PERM[l, i, t] * irfftn(kt.k[l] * F[cidx(t, j)], shape)
+    return out
+
+
+def disorder_sigma(beta, d, mu, nu):
+    """DisorderState.CalculateSigma (DisorderState.py:15-30): the state's
+    stress (source 'betaE') plus the disorder term of disorder_weight(rho, d)
+    with rho the spectral Nye tensor.  beta: the nine tensor components."""
+    return disorder_curl(disorder_weight(rho_fourier(beta[:9]), d), sigma(beta[:9], mu, nu))
+
+
+class DisorderDynamics(Dynamics):
+    """The BetaP dynamics with the disorder stress.  disorder=None: d is the
+    state's tenth component u[9], advected by the flux with the Hamiltonian
+    H_s = sum_a v_a d_a (H_2D/H_3D apply deriv_a * V[a] to every component of
+    the state's field, CentralUpwindHJBetaPDynamics3D.py:186-188, and the
+    RK combinations to all ten); else d is the given fixed field
+    (BetaP_DisorderDynamics) and the state has nine components.  A load,
+    which the reference's disorder classes do not apply, is added after the
+    disorder term as ExternalStrain/ExternalStress would be."""
+
+    def __init__(self, dynamics=GLIDE_CLIMB, Lambda=0.0, mu=1.0, nu=0.3, load=None, disorder=None):
+        Dynamics.__init__(self, dynamics, Lambda, mu, nu, load)
+        self.disorder = disorder
+
+    def field_of(self, u):
+        return self.disorder if self.disorder is not None else u[9]
+
+    def get_sigma(self, u, t):
+        s = disorder_sigma(u[:9], self.field_of(u), self.mu, self.nu)
+        return self.load.apply(s, u[:9], t, self.mu, self.nu)
+
+    def flux(self, t, u):
+        return central_upwind_flux(u, self.get_sigma(u, t), self.dynamics, self.Lambda)
+
+
+# ---------------------------------------------------------------------------
 # The CUDA executable's numerics (plasticity_kernel.cu centralHJ :118-697,
 # findDerivatives :53-90, simpleTVD :1112-1137, TVD3rd :1140-1206, with the
 # defaults of plasticity.h: theta = 1, LLF, PYTHON_COMPATIBILITY_DIVIDE, DOUBLE;
