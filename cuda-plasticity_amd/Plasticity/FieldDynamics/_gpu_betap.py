@@ -58,17 +58,56 @@ class GpuBetaPDynamics(FieldDynamics):
         kw.update(extra)
         return plans.get_plan(len(state.gridShape), n, state.betaP.data.device, mu=state.mu, nu=state.nu, **kw)
 
+    def _private_plan(self, state, **extra):
+        """A plan owned by this dynamics object, outside the shared plan cache,
+        for couplings whose fields live on the plan (disorder)."""
+        n = state.gridShape[0]
+        if any(s != n for s in state.gridShape):
+            raise ValueError("the GPU stepper supports square grids only")
+        kw = self.plan_kwargs()
+        kw.update(extra)
+        dev = state.betaP.data.device
+        key = (len(state.gridShape), n, str(dev), state.mu, state.nu) + tuple(
+            sorted((k, tuple(v) if isinstance(v, (list, tuple)) else v) for k, v in kw.items()))
+        own = self.__dict__.setdefault("_private_plans", {})
+        p = own.get(key)
+        if p is None:
+            p = L.Plan(len(state.gridShape), n, device=dev.index or 0, mu=state.mu, nu=state.nu, **kw)
+            own[key] = p
+        p.bind_stream()
+        return p
+
     def GetSigma(self, state, time=0.0):
+        """BetaPDynamics.GetSigma = state.CalculateSigma(): a DisorderState's
+        includes its disorder term (DisorderState.py:15-30)."""
         p = self._plan(state)
+        d = _disorder_field(state)
+        if d is not None:
+            return TensorField(state.gridShape, data=p.disorder_sigma(state.betaP.data, d, time))
         return TensorField(state.gridShape, data=p.sigma(state.betaP.data, time))
 
     def CalculateFlux(self, time, state, CFLCondition=False):
+        """For a DisorderState the flux covers all ten components of its
+        field: the disorder field is carried with H_s = v . grad d (H_2D/H_3D
+        apply to every component, CentralUpwindHJBetaPDynamics3D.py:186-188)."""
         p = self._plan(state)
-        rhs, speed = p.flux(state.betaP.data, time)
-        rhs = TensorField(state.gridShape, data=rhs)
+        d = _disorder_field(state)
+        if d is not None:
+            import torch
+            from Plasticity.Fields.Fields import TensorScalarMixedField
+            rhs, rd, speed = p.disorder_flux(state.betaP.data, d, time, advect=True)
+            rhs = TensorScalarMixedField(state.gridShape, data=torch.cat([rd[None], rhs]))
+        else:
+            rhs, speed = p.flux(state.betaP.data, time)
+            rhs = TensorField(state.gridShape, data=rhs)
         if CFLCondition:
             return rhs, speed
         return rhs
+
+
+def _disorder_field(state):
+    from Plasticity.PlasticityStates.DisorderState import DisorderState
+    return state.DisorderField() if isinstance(state, DisorderState) else None
 
 
 class GpuGlideOnlyDynamics(GpuBetaPDynamics):

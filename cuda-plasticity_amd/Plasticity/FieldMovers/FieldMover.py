@@ -38,6 +38,9 @@ class TVDRungeKutta_FieldMover(FieldMover):
         return dynamics._plan(state, cfl=float(self.CFLsafeFactor), rk_order=int(self.order))
 
     def Run(self, state, observers, dynamics=None, startTime=0., endTime=10., fixedTimeStep=None):
+        from Plasticity.PlasticityStates.DisorderState import DisorderState
+        if isinstance(state, DisorderState):
+            return self._run_disorder(state, observers, dynamics, startTime, endTime, fixedTimeStep)
         p = self._plan(state, dynamics)
         p.set_state(state.betaP.data)
         p.set_time(float(startTime), float(endTime), float(fixedTimeStep) if fixedTimeStep else 0.0)
@@ -61,6 +64,45 @@ class TVDRungeKutta_FieldMover(FieldMover):
         self.numberOfSteps = p.clock()[3] - steps0
         self.lastState = PlasticityState(state.gridShape, field=TensorField(state.gridShape, data=p.get_state()),
                                          inherit=state)
+        return self.lastState
+
+    def _run_disorder(self, state, observers, dynamics, startTime, endTime, fixedTimeStep):
+        """A DisorderState with the plain BetaP dynamics: its disorder field
+        joins every stage's stress and is carried by the flux and the RK
+        stages like the tensor components (the mover's arithmetic on the
+        state's ten-component field, FieldMover.py:115-135)."""
+        import torch
+        from Plasticity.Fields.Fields import TensorScalarMixedField
+        from Plasticity.PlasticityStates.DisorderState import DisorderState
+        if not hasattr(dynamics, "_private_plan") or hasattr(dynamics, "disorder"):
+            raise NotImplementedError("a DisorderState runs with the plain BetaP dynamics")
+        p = dynamics._private_plan(state, cfl=float(self.CFLsafeFactor), rk_order=int(self.order))
+        p.set_state(state.betaP.data)
+        p.set_disorder(state.DisorderField(), advect=True)
+        p.set_time(float(startTime), float(endTime), float(fixedTimeStep) if fixedTimeStep else 0.0)
+        steps0 = p.clock()[3]
+
+        def snapshot():
+            data = torch.cat([p.disorder_get()[None], p.get_state()])
+            return DisorderState(state.gridShape, field=TensorScalarMixedField(state.gridShape, data=data),
+                                 inherit=state)
+        if observers:
+            while True:
+                p.step(1)
+                t, dt, done, ns = p.clock()
+                cur = snapshot()
+                for observer in observers:
+                    observer.Update(t, cur)
+                if done:
+                    break
+        else:
+            while True:
+                p.step(self.chunk)
+                t, dt, done, ns = p.clock()
+                if done:
+                    break
+        self.numberOfSteps = p.clock()[3] - steps0
+        self.lastState = snapshot()
         return self.lastState
 
 

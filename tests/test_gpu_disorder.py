@@ -145,3 +145,64 @@ def test_disorder_refusals(amd):
     r = amd.Plan(2, n, rk_order=1, scheme="llf")
     with pytest.raises(CDDError):
         r.set_disorder(dis)
+
+
+def test_reference_api_disorder_state(amd):
+    """DisorderState + BetaPDynamics(Lambda=1) + TVDRungeKutta_FieldMover
+    through PlasticitySystem (the RunPython.py:196-257 'x' runs): the state's
+    stress, the ten-component flux and the run against the oracle."""
+    say("disorder state API")
+    from Plasticity import PlasticitySystem
+    from Plasticity.FieldDynamics import CentralUpwindHJBetaPDynamics
+    from Plasticity.FieldMovers import FieldMover
+    from Plasticity.Observers import Observer
+    from Plasticity.PlasticityStates.DisorderState import DisorderState
+    n = 32
+    beta, dis = _fields(2, n, 57)
+    state = DisorderState((n, n), field=np.concatenate([dis[None], beta]), mu=MU, nu=NU)
+    assert rel_l2(state.CalculateSigma().numpy(), O.disorder_sigma(beta, dis, MU, NU)) < 1e-12
+    dyn = CentralUpwindHJBetaPDynamics.BetaPDynamics(Lambda=1)
+    rhs, speed = dyn.CalculateFlux(0.0, state, CFLCondition=True)
+    u = np.concatenate([beta, dis[None]])
+    ref, sp = _odyn(0, 1.0).flux(0.0, u)
+    got = rhs.numpy()
+    assert got.shape == (10, n, n)
+    assert rel_l2(got[1:], ref[:9]) < 1e-12 and rel_l2(got[0], ref[9]) < 1e-12
+    assert rel_l2(speed.cpu().numpy(), sp) < 1e-12
+    rec = Observer.RecallStateObserver()
+    mover = FieldMover.TVDRungeKutta_FieldMover(CFLsafeFactor=0.25)
+    _, _, dts = O.run(_odyn(0, 1.0), u, 0.0, 5, 0.25)
+    t_end = float(np.sum(dts[:4]) + 0.5 * dts[4])
+    PlasticitySystem.PlasticitySystem((n, n), state, mover, dyn, [rec]).Run(startTime=0.0, endTime=t_end)
+    ref_u, tref, dref = O.run(_odyn(0, 1.0), u, 0.0, 100, 0.25, t_end=t_end)
+    assert mover.numberOfSteps == len(dref)
+    g = rec.state.GetOrderParameterField().numpy()
+    assert rel_l2(g[1:], ref_u[:9]) < 1e-11 and rel_l2(g[0], ref_u[9]) < 1e-11
+    mover2 = FieldMover.TVDRungeKutta_FieldMover(CFLsafeFactor=0.25)
+    mover2.Run(state, [], dyn, 0.0, t_end)
+    g2 = mover2.lastState.GetOrderParameterField().numpy()
+    assert rel_l2(g2[1:], ref_u[:9]) < 1e-11 and rel_l2(g2[0], ref_u[9]) < 1e-11
+
+
+def test_reference_api_fixed_disorder_dynamics(amd):
+    """BetaP_DisorderDynamics(disorder=d) on a PlasticityState
+    (DisorderDynamics.py:17-44): GetSigma, CalculateFlux and a mover run."""
+    say("fixed disorder API")
+    from Plasticity.FieldDynamics.DisorderDynamics import BetaP_DisorderDynamics
+    from Plasticity.FieldMovers import FieldMover
+    from Plasticity.PlasticityStates.PlasticityState import PlasticityState
+    n = 16
+    beta, dis = _fields(3, n, 58)
+    state = PlasticityState((n,) * 3, field=beta, mu=MU, nu=NU)
+    dyn = BetaP_DisorderDynamics(Lambda=0.5, disorder=dis)
+    assert rel_l2(dyn.GetSigma(state, 0.0).numpy(), O.disorder_sigma(beta, dis, MU, NU)) < 1e-12
+    od = _odyn(0, 0.5, disorder=dis)
+    ref, _ = od.flux(0.0, beta)
+    assert rel_l2(dyn.CalculateFlux(0.0, state).numpy(), ref) < 1e-12
+    _, _, dts = O.run(od, beta, 0.0, 3, 0.25)
+    t_end = float(np.sum(dts[:2]) + 0.5 * dts[2])
+    mover = FieldMover.TVDRungeKutta_FieldMover(CFLsafeFactor=0.25)
+    mover.Run(state, [], dyn, 0.0, t_end)
+    ref_u, _, dref = O.run(od, beta, 0.0, 100, 0.25, t_end=t_end)
+    assert mover.numberOfSteps == len(dref)
+    assert rel_l2(mover.lastState.betaP.numpy(), ref_u) < 1e-11
