@@ -53,7 +53,7 @@ __device__ __forceinline__ void lderivs(const T f[5], T N, T& up, T& um) {
 
 // ------------------------------------------------------------------ R1
 template <typename T, int D, int DYN>
-__device__ void cell_flux_vjp(const T (&u)[D][2][9], const T (&sg)[9], T Lambda, const T (&rb)[9],
+__host__ __device__ void cell_flux_vjp(const T (&u)[D][2][9], const T (&sg)[9], T Lambda, const T (&rb)[9],
                               T (&ub)[D][2][9], T (&sgb)[9], T& lamb) {
   constexpr int NCOR = 1 << D;
   const T ME = T(kME);
@@ -352,6 +352,461 @@ __device__ void cell_flux_vjp(const T (&u)[D][2][9], const T (&sg)[9], T Lambda,
     }
 }
 
+// The reverse of cell_flux with a small working set, for the 3D kernel
+// (cell_flux_vjp holds every corner's quantities and all 54 derivative
+// cotangents at once, which spills in 3D).  Every corner quantity depends on
+// the derivatives only through per-(axis, sign) sums F: the diagonal entries
+// u_a[l, a], the stress contractions w_a[l] = u_a[l, n] sigma[a, n] and
+// z_a = u_a : sigma, the traces, the |rho|^2 pieces P_a and the |rho| cross
+// products of two axes.  The forward pass forms F, parks each corner's
+// velocity and climb rate in vbuf (4 values per corner, stride vstride, as
+// the forward flux kernel does) and Y, Hb in ybuf; the reverse corner sweeps
+// read only F and accumulate the cotangents of F; then reload(u) refills the
+// derivatives (dead during the sweeps) and each derivative cotangent
+// ub[a][s][c] is formed once and handed to store(a, s, c, value).  Same
+// arithmetic as cell_flux_vjp up to the order of the sums.
+template <typename T, int D, int DYN, typename Store, typename Reload>
+__host__ __device__ __forceinline__ void cell_flux_vjp_lean(T (&u)[D][2][9], const T (&sg)[9], T Lambda,
+                                                            const T (&rb)[9], T* __restrict__ vbuf,
+                                                            T* __restrict__ ybuf, int vstride, T (&sgb)[9], T& lamb,
+                                                            Store store, Reload reload) {
+  constexpr int NCOR = 1 << D;
+  const T ME = T(kME);
+  const bool climb_sup = (DYN == DYN_GLIDE_CLIMB) && (Lambda != T(0));
+  const T third = T(1.0 / 3.0);
+  const T st = climb_sup ? Lambda * (sg[0] + sg[4] + sg[8]) * third : T(0);
+  const T jfac = -Lambda * third;
+  auto sgn = [](int c, int a) { return (c >> (D - 1 - a)) & 1; };
+  // ---------------- F
+  T Ru[D][2][3], w[D][2][3], zz[D][2], tr[D][2], P[D][2], C01[2][2], C12[2][2], C20[2][2];
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const T* ua = u[a][s];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        Ru[a][s][l] = ua[3 * l + a];
+        w[a][s][l] = ua[3 * l] * sg[3 * a] + ua[3 * l + 1] * sg[3 * a + 1] + ua[3 * l + 2] * sg[3 * a + 2];
+      }
+      T z = T(0);
+#pragma unroll
+      for (int cc = 0; cc < 9; ++cc) z += ua[cc] * sg[cc];
+      zz[a][s] = z;
+      tr[a][s] = ua[0] + ua[4] + ua[8];
+      T p = T(0);
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+        if (m != a)
+#pragma unroll
+          for (int jj = 0; jj < 3; ++jj) p += ua[3 * m + jj] * ua[3 * m + jj];
+      P[a][s] = p;
+    }
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      T c01 = T(0), c12 = T(0), c20 = T(0);
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) {
+        c01 += u[0][s][3 + jj] * u[1][t][jj];
+        if (D == 3) {
+          c12 += u[1][s][6 + jj] * u[D - 1][t][3 + jj];
+          c20 += u[D - 1][s][jj] * u[0][t][6 + jj];
+        }
+      }
+      C01[s][t] = c01;
+      C12[s][t] = c12;
+      C20[s][t] = c20;
+    }
+  // forward quantities of corner c (cell_flux_vjp's R, q, v0p, v0, rr, rv, fpr, rinv) from F
+  auto corner = [&](int c, T (&R)[3], T (&q)[3], T (&v0p)[3], T (&v0)[3], T& rr, T& rv, T& fpr, T& rinv) {
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      T r = T(0), vv = T(0);
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        r += Ru[a][sgn(c, a)][l];
+        vv += w[a][sgn(c, a)][l];
+      }
+      R[l] = r;
+      v0p[l] = vv;
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) v0p[a] -= zz[a][sgn(c, a)];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) q[a] = (a < D ? tr[a % D][sgn(c, a % D)] : T(0)) - R[a];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) v0[l] = v0p[l];
+    rr = rv = fpr = T(0);
+    if (DYN == DYN_GLIDE_CLIMB) {
+      if (climb_sup) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) v0[a] += st * tr[a][sgn(c, a)];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) v0[l] -= st * R[l];
+      }
+    } else {
+      T rrr = ME;
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        const T ta = tr[a][sgn(c, a)];
+        rrr += ta * ta - T(2) * ta * R[a];
+      }
+#pragma unroll
+      for (int l = 0; l < 3; ++l) rrr += R[l] * R[l];
+      rr = rrr;
+      rv = ME + v0p[0] * q[0] + v0p[1] * q[1] + v0p[2] * q[2];
+      fpr = rv / rr;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) v0[a] += q[a] * fpr;
+    }
+    T r2 = ME;
+#pragma unroll
+    for (int a = 0; a < D; ++a) r2 += P[a][sgn(c, a)];
+    T cross = C01[sgn(c, 0)][sgn(c, 1)];
+    if (D == 3) cross += C12[sgn(c, 1)][sgn(c, D - 1)] + C20[sgn(c, D - 1)][sgn(c, 0)];
+    r2 -= T(2) * cross;
+    rinv = rsqrt_nr(r2);
+  };
+  // the climb term's corner rate J = -Lambda/3 v . q, q from F
+  auto climb_rate = [&](int c) {
+    T j = T(0);
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      T r = T(0);
+#pragma unroll
+      for (int a = 0; a < D; ++a) r += Ru[a][sgn(c, a)][l];
+      const T ql = (l < D ? tr[l % D][sgn(c, l % D)] : T(0)) - r;
+      j += vbuf[(3 * c + l) * vstride] * ql;
+    }
+    return jfac * j;
+  };
+  auto weight = [&](int c, const T (&ap)[3], const T (&am)[3]) {
+    T wc = T(1);
+#pragma unroll
+    for (int a = 0; a < D; ++a) wc *= sgn(c, a) == 0 ? am[a] : ap[a];
+    return wc;
+  };
+  // ---------------- forward: corner velocities and climb rates, speeds
+  T mxv[3], mnv[3];
+  int cmax[3] = {0, 0, 0}, cmin[3] = {0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < NCOR; ++c) {
+    T R[3], q[3], v0p[3], v0[3], rr, rv, fpr, rinv;
+    corner(c, R, q, v0p, v0, rr, rv, fpr, rinv);
+    T v[3];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) v[l] = v0[l] * rinv;
+#pragma unroll
+    for (int l = 0; l < 3; ++l) vbuf[(3 * c + l) * vstride] = v[l];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      if (c == 0) {
+        mxv[a] = mnv[a] = v[a];
+      } else {
+        if (v[a] > mxv[a]) {
+          mxv[a] = v[a];
+          cmax[a] = c;
+        }
+        if (v[a] < mnv[a]) {
+          mnv[a] = v[a];
+          cmin[a] = c;
+        }
+      }
+    }
+  }
+  T ap[3] = {0, 0, 0}, am[3] = {0, 0, 0}, at[3] = {1, 1, 1};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    ap[a] = fmax(mxv[a], T(0));
+    am[a] = -fmin(mnv[a], T(0));
+    at[a] = ap[a] + am[a] + ME;
+  }
+  // Y and Hb wait in ybuf (stride vstride) for the final pass: Y at
+  // 6 a + 3 s + l, Hb at 6 D + c
+  auto yat = [&](int a, int s, int l) -> T& { return ybuf[(6 * a + 3 * s + l) * vstride]; };
+  auto hbat = [&](int c) -> T& { return ybuf[(6 * D + c) * vstride]; };
+  auto sgat = [&](int c) -> T& { return ybuf[(6 * D + 9 + c) * vstride]; };
+#pragma unroll
+  for (int c = 0; c < 9; ++c) sgat(c) = sg[c];
+  T invb = T(0);
+  {
+    T Y[D][2][3];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) Y[a][s][l] = T(0);
+    T Jsum = T(0);
+#pragma unroll
+    for (int c = 0; c < NCOR; ++c) {
+      const T wc = weight(c, ap, am);
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        const T vl = vbuf[(3 * c + l) * vstride];
+#pragma unroll
+        for (int a = 0; a < D; ++a) Y[a][sgn(c, a)][l] += wc * vl;
+      }
+      if (climb_sup) Jsum += wc * climb_rate(c);
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) yat(a, s, l) = Y[a][s][l];
+    T inv0 = T(1);
+#pragma unroll
+    for (int a = 0; a < D; ++a) inv0 *= at[a];
+    inv0 = T(1) / inv0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) {
+        const int cc = 3 * i + jj;
+        T h = (i == jj) ? Jsum : T(0);
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+          for (int s = 0; s < 2; ++s) h += u[a][s][cc] * Y[a][s][a];
+        if (i < D)
+#pragma unroll
+          for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) h -= u[i % D][s][3 * l + jj] * Y[i % D][s][l];
+        // H is needed only for the cotangent of 1 / prod(at) and, through
+        // Hb = -rbar / prod(at), for everything else
+        invb -= rb[cc] * h;
+        hbat(cc) = -rb[cc] * inv0;
+      }
+  }
+  T inv = T(1);
+#pragma unroll
+  for (int a = 0; a < D; ++a) inv *= at[a];
+  inv = T(1) / inv;
+  // tw2[a] = am ap / at_a times prod(at): the diffusion weight over the
+  // factor of Hb = -rbar / prod(at), so that the final pass takes rbar from Hb
+  T iat[3] = {1, 1, 1}, tw2[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    T o = inv, p = T(1);
+#pragma unroll
+    for (int b = 0; b < D; ++b)
+      if (b != a) {
+        o *= at[b];
+        p *= at[b];
+      }
+    iat[a] = o;
+    tw2[a] = am[a] * ap[a] * p;
+  }
+  // ---------------- reverse
+  T twb[3] = {0, 0, 0};
+#pragma unroll
+  for (int cc = 0; cc < 9; ++cc)
+#pragma unroll
+    for (int a = 0; a < D; ++a) twb[a] += rb[cc] * (u[a][0][cc] - u[a][1][cc]);
+  const T Jsumb = hbat(0) + hbat(4) + hbat(8);
+  // Yb[a][s][l] = [l == a] sum_c Hb[c] u[a][s][c] - sum_j Hb[3a + j] u[a][s][3l + j]
+  T Yb[D][2][3];
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        T y = T(0);
+        if (l == a)
+#pragma unroll
+          for (int cc = 0; cc < 9; ++cc) y += hbat(cc) * u[a][s][cc];
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) y -= hbat(3 * a + jj) * u[a][s][3 * l + jj];
+        Yb[a][s][l] = y;
+      }
+  T apb[3] = {0, 0, 0}, amb[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    amb[a] = twb[a] * ap[a] * iat[a];
+    apb[a] = twb[a] * am[a] * iat[a];
+    const T atb = -twb[a] * am[a] * ap[a] * iat[a] * iat[a] - invb * inv * iat[a];
+    apb[a] += atb;
+    amb[a] += atb;
+  }
+  // corner sweep 1: the weights' cotangents
+#pragma unroll
+  for (int c = 0; c < NCOR; ++c) {
+    T wcb = climb_sup ? Jsumb * climb_rate(c) : T(0);
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      const T vl = vbuf[(3 * c + l) * vstride];
+#pragma unroll
+      for (int a = 0; a < D; ++a) wcb += Yb[a][sgn(c, a)][l] * vl;
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      T others = T(1);
+#pragma unroll
+      for (int b2 = 0; b2 < D; ++b2)
+        if (b2 != a) others *= sgn(c, b2) == 0 ? am[b2] : ap[b2];
+      if (sgn(c, a) == 0) amb[a] += wcb * others;
+      else apb[a] += wcb * others;
+    }
+  }
+  // corner sweep 2: velocity cotangents back to the cotangents of F
+  T Pb[D][2], wb[D][2][3], zzb[D][2], trb[D][2], RbA[D][2][3], X01[2][2], X12[2][2], X20[2][2];
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      Pb[a][s] = zzb[a][s] = trb[a][s] = T(0);
+#pragma unroll
+      for (int l = 0; l < 3; ++l) wb[a][s][l] = RbA[a][s][l] = T(0);
+    }
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) X01[s][t] = X12[s][t] = X20[s][t] = T(0);
+  T stb = T(0);
+  lamb = T(0);
+#pragma unroll
+  for (int c = 0; c < NCOR; ++c) {
+    T R[3], q[3], v0p[3], v0[3], rr, rv, fpr, rinv;
+    corner(c, R, q, v0p, v0, rr, rv, fpr, rinv);
+    const T wc = weight(c, ap, am);
+    T vb[3];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      T acc = T(0);
+#pragma unroll
+      for (int a = 0; a < D; ++a) acc += Yb[a][sgn(c, a)][l];
+      vb[l] = acc * wc;
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      if (c == cmax[a] && ap[a] > T(0)) vb[a] += apb[a];
+      if (c == cmin[a] && am[a] > T(0)) vb[a] -= amb[a];
+    }
+    T qb[3] = {0, 0, 0}, Rb[3] = {0, 0, 0};
+    if (climb_sup) {
+      const T Jb = Jsumb * wc;
+      T v[3];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) v[l] = vbuf[(3 * c + l) * vstride];
+      const T vq = v[0] * q[0] + v[1] * q[1] + v[2] * q[2];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        vb[l] += Jb * jfac * q[l];
+        qb[l] += Jb * jfac * v[l];
+      }
+      lamb -= Jb * vq * third;
+    }
+    T v0b[3], rinvb = T(0);
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      v0b[l] = vb[l] * rinv;
+      rinvb += vb[l] * v0[l];
+    }
+    const T r2b = rinvb * T(-0.5) * rinv * rinv * rinv;
+#pragma unroll
+    for (int a = 0; a < D; ++a) Pb[a][sgn(c, a)] += r2b;
+    const T crossb = T(-2) * r2b;
+    X01[sgn(c, 0)][sgn(c, 1)] += crossb;
+    if (D == 3) {
+      X12[sgn(c, 1)][sgn(c, D - 1)] += crossb;
+      X20[sgn(c, D - 1)][sgn(c, 0)] += crossb;
+    }
+    T v0pb[3] = {v0b[0], v0b[1], v0b[2]};
+    if (DYN == DYN_GLIDE_CLIMB) {
+      if (climb_sup) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          stb += v0b[a] * tr[a][sgn(c, a)];
+          trb[a][sgn(c, a)] += v0b[a] * st;
+        }
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+          stb -= v0b[l] * R[l];
+          Rb[l] -= v0b[l] * st;
+        }
+      }
+    } else {
+      T fb = T(0);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        fb += v0b[a] * q[a];
+        qb[a] += v0b[a] * fpr;
+      }
+      const T rvb = fb / rr;
+      const T rrb = -fb * rv / (rr * rr);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        v0pb[a] += rvb * q[a];
+        qb[a] += rvb * v0p[a];
+      }
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        const T ta = tr[a][sgn(c, a)];
+        trb[a][sgn(c, a)] += rrb * (T(2) * ta - T(2) * R[a]);
+        Rb[a] += rrb * (T(-2) * ta);
+      }
+#pragma unroll
+      for (int l = 0; l < 3; ++l) Rb[l] += rrb * T(2) * R[l];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (a < D) trb[a % D][sgn(c, a % D)] += qb[a];
+      Rb[a] -= qb[a];
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        wb[a][sgn(c, a)][l] += v0pb[l];
+        RbA[a][sgn(c, a)][l] += Rb[l];
+      }
+      zzb[a][sgn(c, a)] -= v0pb[a];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 9; ++c) sgb[c] = T(0);
+  if (climb_sup) {
+    sgb[0] += stb * Lambda * third;
+    sgb[4] += stb * Lambda * third;
+    sgb[8] += stb * Lambda * third;
+    lamb += stb * st / Lambda;
+  }
+  // ---------------- every derivative cotangent, once
+  reload(u);
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) {
+        const int m = c / 3, nn = c % 3;
+        T val = (s == 0 ? -hbat(c) : hbat(c)) * tw2[a];  // +-rbar am ap / at_a
+        val += hbat(c) * yat(a, s, a) - hbat(3 * a + nn) * yat(a, s, m);
+        if (m != a) val += T(2) * Pb[a][s] * u[a][s][c];
+        val += wb[a][s][m] * sgat(3 * a + nn) + zzb[a][s] * sgat(c);
+        if (m == nn) val += trb[a][s];
+        if (nn == a) val += RbA[a][s][m];
+        // the |rho| cross terms, summed over the other axis' sign
+        if (a == 0 && m == 1) val += X01[s][0] * u[1][0][nn] + X01[s][1] * u[1][1][nn];
+        if (a == 1 && m == 0) val += X01[0][s] * u[0][0][3 + nn] + X01[1][s] * u[0][1][3 + nn];
+        if (D == 3) {
+          if (a == 0 && m == 2) val += X20[0][s] * u[D - 1][0][nn] + X20[1][s] * u[D - 1][1][nn];
+          if (a == 1 && m == 2) val += X12[s][0] * u[D - 1][0][3 + nn] + X12[s][1] * u[D - 1][1][3 + nn];
+          if (a == D - 1 && m == 1) val += X12[0][s] * u[1][0][6 + nn] + X12[1][s] * u[1][1][6 + nn];
+          if (a == D - 1 && m == 0) val += X20[s][0] * u[0][0][6 + nn] + X20[s][1] * u[0][1][6 + nn];
+        }
+        store(a, s, c, val);
+        sgb[3 * a + nn] += wb[a][s][m] * u[a][s][c];
+        sgb[c] += zzb[a][s] * u[a][s][c];
+      }
+    }
+}
+
 static __device__ __forceinline__ int wrapi(int p, int n) { return p < 0 ? p + n : (p >= n ? p - n : p); }
 
 // Tile mapping shared by the two flux-adjoint kernels: blocks of TX x TY
@@ -396,8 +851,14 @@ __device__ __forceinline__ long long adj_tile_cell(int n, int co[3], long long s
 template <typename T, int D, int DYN, int TX, int TY>
 __global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
   constexpr int PW = TX + 4, TILE = PW * (TY + 4), NT = TX * TY, FILL = (TILE + NT - 1) / NT;
+  // 3D glide + climb: the lean reverse (cell_flux_vjp_lean), its corner
+  // velocities and Y, Hb in LDS (fewer spilled registers there; the
+  // glide-only reverse spills less in the plain form)
+  constexpr bool LEAN = (D == 3);
   __shared__ T red[4];
   __shared__ T tile[9][TILE];
+  __shared__ T vsh[LEAN ? 3 * (1 << D) : 1][LEAN ? NT : 1];
+  __shared__ T ysh[LEAN ? 6 * D + 18 : 1][LEAN ? NT : 1];
   const int n = job.n;
   const long long cells = job.cells;
   int co[3];
@@ -474,11 +935,42 @@ __global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
   const T rs = T(job.rscale) * (job.dtp ? T(*job.dtp) : T(1));
   T rb[9];
   for (int c = 0; c < 9; ++c) rb[c] = rs * job.rbar[c * cells + ii];
-  T ub[D][2][9], sgb[9], lamb;
-  cell_flux_vjp<T, D, DYN>(u, sg, T(job.Lambda), rb, ub, sgb, lamb);
-  for (int a = 0; a < D; ++a)
-    for (int s = 0; s < 2; ++s)
-      for (int c = 0; c < 9; ++c) job.ubar[((a * 2 + s) * 9 + c) * cells + ii] = N * ub[a][s][c];
+  T sgb[9], lamb;
+  if constexpr (LEAN) {
+    double* ubar = job.ubar;
+    // the derivatives again for the final pass: the plane tile is still in
+    // LDS, the slow-axis neighbours are read again (from L2, mostly)
+    auto reload = [&](T (&uu)[D][2][9]) {
+      long long xo[4];
+      for (int k = 0; k < 4; ++k) xo[k] = ii + (long long)(wrapi(co[0] + (k < 2 ? k - 2 : k - 1), n) - co[0]) * str[0];
+      for (int a = 0; a < D; ++a)
+        for (int c = 0; c < 9; ++c) {
+          const T* tb = tile[c];
+          T f[5];
+          if (a == D - 1) {
+            for (int k = 0; k < 5; ++k) f[k] = tb[lr * PW + lc + k - 2];
+          } else if (a == D - 2) {
+            for (int k = 0; k < 5; ++k) f[k] = tb[(lr + k - 2) * PW + lc];
+          } else {
+            f[0] = job.beta[c * cells + xo[0]];
+            f[1] = job.beta[c * cells + xo[1]];
+            f[2] = tb[lr * PW + lc];
+            f[3] = job.beta[c * cells + xo[2]];
+            f[4] = job.beta[c * cells + xo[3]];
+          }
+          lderivs<T>(f, N, uu[a][0][c], uu[a][1][c]);
+        }
+    };
+    cell_flux_vjp_lean<T, D, DYN>(
+        u, sg, T(job.Lambda), rb, &vsh[0][tid], &ysh[0][tid], NT, sgb, lamb,
+        [&](int a, int s, int c, T val) { ubar[((a * 2 + s) * 9 + c) * cells + ii] = N * val; }, reload);
+  } else {
+    T ub[D][2][9];
+    cell_flux_vjp<T, D, DYN>(u, sg, T(job.Lambda), rb, ub, sgb, lamb);
+    for (int a = 0; a < D; ++a)
+      for (int s = 0; s < 2; ++s)
+        for (int c = 0; c < 9; ++c) job.ubar[((a * 2 + s) * 9 + c) * cells + ii] = N * ub[a][s][c];
+  }
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) job.sbar[(3 * i + j) * cells + ii] = T(0.5) * (sgb[3 * i + j] + sgb[3 * j + i]);
   const int nw = (int)(blockDim.x * blockDim.y + 63) / 64;
