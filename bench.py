@@ -21,6 +21,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "cuda-plasticity_amd")]
 
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E datasheet peak
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix) datasheet peak
 
 
 def spectral_bytes(n, d=3):
@@ -488,6 +489,18 @@ def main():
             traffic = json.load(open(tpath)).get(dom)
         except Exception:
             traffic = None
+    # the dominant kernel's FP64 rate from its measured FP64 operations per cell
+    fp64 = None
+    fpath = os.path.join(ROOT, "profiles", "fp64_ops_3d256.json")
+    if os.path.exists(fpath) and args.precision == "fp64" and n == 256:
+        try:
+            fpc = json.load(open(fpath)).get("flux" if dom.startswith("flux") else dom)
+        except Exception:
+            fpc = None
+        if fpc:
+            tf = fpc * cells / (kernels[dom]["avg_ms"] * 1e-3) / 1e12
+            fp64 = {"bound": "fp64", "kernel": dom, "flops_per_cell": fpc, "achieved": round(tf, 2),
+                    "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / PEAK_FP64_TFLOPS, 4)}
     bpc = canonical_bytes_per_cell_update(n) * vb
     rec = slog.cpu().numpy()
     out = {
@@ -510,6 +523,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": dom_bytes},
+        "roofline_fp64": fp64,
         "step_roofline": {"bytes_per_cell_update": bpc, "frac": round(bpc * value / (PEAK_HBM_GBS * 1e9), 4)},
         "kernels": kernels,
         "sim_time": t,
