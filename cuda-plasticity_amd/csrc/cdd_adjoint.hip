@@ -851,9 +851,9 @@ __device__ __forceinline__ long long adj_tile_cell(int n, int co[3], long long s
 template <typename T, int D, int DYN, int TX, int TY>
 __global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
   constexpr int PW = TX + 4, TILE = PW * (TY + 4), NT = TX * TY, FILL = (TILE + NT - 1) / NT;
-  // 3D glide + climb: the lean reverse (cell_flux_vjp_lean), its corner
-  // velocities and Y, Hb in LDS (fewer spilled registers there; the
-  // glide-only reverse spills less in the plain form)
+  // 3D: the lean reverse (cell_flux_vjp_lean) with its corner velocities,
+  // Y, Hb and sigma in LDS (the plain form spills in 3D; in 2D it does not
+  // and measures slightly faster)
   constexpr bool LEAN = (D == 3);
   __shared__ T red[4];
   __shared__ T tile[9][TILE];
