@@ -42,7 +42,8 @@ def shear_shift():
     return s
 
 
-CASES = [((16, 16), 1, 0.0), ((16, 16), 0, 0.9), ((8, 8, 8), 0, 0.9), ((8, 8, 8), 1, 0.0)]
+CASES = [((16, 16), 1, 0.0), ((16, 16), 0, 0.9), ((8, 8, 8), 0, 0.9), ((8, 8, 8), 1, 0.0),
+         ((32, 32, 32), 0, 0.9), ((32, 32, 32), 1, 0.0)]  # 32^3: the plane-marching gather
 
 
 @pytest.mark.parametrize("shape,dyn,Lam", CASES)
