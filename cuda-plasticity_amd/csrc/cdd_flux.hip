@@ -28,12 +28,14 @@
 
 namespace cdd {
 
-// Two-argument minmod: min if both > 0, max if both <= 0, else 0, written as
-// max(min, 0) + min(max, 0) (the other term is +0 in every case) without
-// compares or selects.
+// Two-argument minmod: min if both > 0, max if both <= 0, else 0.  That is
+// the median of (a, c, 0), max(min(a, c), min(max(a, c), 0)): four min/max
+// and no compares or selects.  min and max return one of their operands, so
+// the value is exactly the minmod (only the sign of a zero result may differ
+// from other formulations).
 template <typename T>
 __device__ __forceinline__ T minmod2(T a, T c) {
-  return fmax(fmin(a, c), T(0)) + fmin(fmax(a, c), T(0));
+  return fmax(fmin(a, c), fmin(fmax(a, c), T(0)));
 }
 
 // One-sided limited derivatives of f at i from f[i-2..i+2]
@@ -72,11 +74,22 @@ __device__ __forceinline__ void limited_derivs(const T f[5], T& up, T& um) {
 // of a DisorderState: H_s = sum_a v_a s_a, no subtraction or climb term,
 // CentralUpwindHJBetaPDynamics3D.py:186-188); usc[a][s] its derivatives in
 // the units of u.
-template <typename T, int D, int DYN, bool PRE, bool SC = false>
+// CS: climb suppression decided at compile time (1: Lambda != 0, 0: Lambda
+// == 0) or, for -1, from Lambda at run time; a run-time flag is applied as
+// selects on every climb term.
+// JY: form sum_c w_c J_c from the Y sums after the second pass instead of
+// parking J per corner (vbuf then holds 3 * 2^D rows, not 4 * 2^D).  With
+// J_c = jfac v(c).q(c), q_l(c) = [l<d] tr(u_l^{s_l}) - sum_a u_a^{s_a}[l,a],
+//   sum_c w_c J_c = jfac sum_l ([l<d] sum_s tr(u_l^s) Y_l^s[l]
+//                               - sum_a sum_s u_a^s[l,a] Y_a^s[l]).
+// Glide + climb only (the woJzz climb direction is not of this form).
+template <typename T, int D, int DYN, bool PRE, bool SC = false, int CS = -1, bool JY = false>
 __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9], T Lambda, T (&rhs)[9],
                                           T& speed, T* __restrict__ vbuf, int vstride, T nscale,
                                           const T (*usc)[2] = nullptr, T* rhs_sc = nullptr) {
   constexpr int NCOR = 1 << D;
+  static_assert(!JY || DYN == DYN_GLIDE_CLIMB || DYN == DYN_GLIDE_ONLY, "JY: glide + climb form only");
+  constexpr int VR = JY ? 3 : 4;  // vbuf rows per corner
   const T ME = T(kME);
   const T MEU = ME / (nscale * nscale);  // ME in the units of u^2
   // per axis / sign quantities
@@ -113,13 +126,16 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
                : (DYN == DYN_WOJZZ)    ? (sg[0] + sg[4] + sg[8]) * T(0.5) * Lambda
                                        : T(0);
   const T jfac = (DYN == DYN_WOJZZ) ? -Lambda * T(0.5) : -Lambda * third;
-  const bool climb_sup = (DYN == DYN_GLIDE_CLIMB || DYN == DYN_WOJZZ) && (Lambda != T(0));
+  const bool climb_sup =
+      (DYN == DYN_GLIDE_CLIMB || DYN == DYN_WOJZZ) && (CS < 0 ? (Lambda != T(0)) : (CS == 1));
 
+  // running max / min of the corner speeds, started at 0: the one-sided
+  // speeds are max(0, max v) and -min(0, min v) (Max5/Max9 with 0)
   T amax[3], amin[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    amax[a] = -INFINITY;
-    amin[a] = INFINITY;
+    amax[a] = T(0);
+    amin[a] = T(0);
   }
 #pragma unroll
   for (int c = 0; c < NCOR; ++c) {
@@ -213,8 +229,8 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
                 : (DYN == DYN_WOJZZ) ? jfac * (vv[0] * wz[0] + vv[1] * wz[1] + vv[2] * wz[2])
                                      : jfac * (vv[0] * q[0] + vv[1] * q[1] + vv[2] * q[2]);
 #pragma unroll
-    for (int l = 0; l < 3; ++l) vbuf[(4 * c + l) * vstride] = vv[l];
-    vbuf[(4 * c + 3) * vstride] = J;
+    for (int l = 0; l < 3; ++l) vbuf[(VR * c + l) * vstride] = vv[l];
+    if constexpr (!JY) vbuf[(4 * c + 3) * vstride] = J;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
       amax[a] = fmax(amax[a], vv[a]);
@@ -225,8 +241,8 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
   T ap[3], am[3], at[3];
 #pragma unroll
   for (int a = 0; a < D; ++a) {
-    ap[a] = fmax(amax[a], T(0));
-    am[a] = -fmin(amin[a], T(0));
+    ap[a] = amax[a];
+    am[a] = -amin[a];
     at[a] = ap[a] + am[a] + ME;
   }
   T Y[D][2][3], Jsum = T(0);
@@ -247,12 +263,31 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
     }
     T vv[3];
 #pragma unroll
-    for (int l = 0; l < 3; ++l) vv[l] = vbuf[(4 * c + l) * vstride];
+    for (int l = 0; l < 3; ++l) vv[l] = vbuf[(VR * c + l) * vstride];
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
       for (int l = 0; l < 3; ++l) Y[a][sgn[a]][l] += wc * vv[l];
-    if (climb_sup) Jsum += wc * vbuf[(4 * c + 3) * vstride];
+    if constexpr (!JY) {
+      if (climb_sup) Jsum += wc * vbuf[(4 * c + 3) * vstride];
+    }
+  }
+  if constexpr (JY) {
+    if (climb_sup) {
+      T js = T(0);
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        if (l < D) {
+#pragma unroll
+          for (int s = 0; s < 2; ++s) js = fma(tr[l % D][s], Y[l % D][s][l], js);
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+          for (int s = 0; s < 2; ++s) js = fma(-u[a][s][3 * l + a], Y[a][s][l], js);
+      }
+      Jsum = jfac * js;
+    }
   }
   T inv = T(1);
 #pragma unroll
@@ -768,10 +803,93 @@ constexpr int TC = 32, TR = 8, NT = TC * TR;
 constexpr int PW = TC + 4, PH = TR + 4, TILE = PW * PH;  // 36 x 12 halo tile
 constexpr int NPAIR = 9 * TILE / 2;                     // 16-byte DMA pairs per tile
 constexpr int NDMA = (NPAIR + NT - 1) / NT;             // DMA instructions per thread
+// The last DMA round is needed by waves 0..WLAST only; the waves above repeat
+// wave WLAST's (same cells to the same LDS words), so every wave issues NDMA
+// tile DMAs, and lanes past the tile fetch pair 0 into the buffer's tail.
+constexpr int WLAST = (NPAIR - 1 - (NDMA - 1) * NT) / 64;
+constexpr int TBUF = 2 * ((NDMA - 1) * NT + (WLAST + 1) * 64);  // tile buffer (doubles)
 static_assert(PW % 2 == 0, "pairs must not straddle tile rows");
+static_assert(NDMA >= 2 && WLAST >= 0 && WLAST < NT / 64, "tile DMA rounds");
+// Vector-memory instructions per wave and group, for the s_waitcnt vmcnt
+// counts of the pipeline.  Every group is issued whole, by every lane, on
+// every step (a conditional DMA makes the compiler's wait tracking merge
+// paths and wait for every DMA in flight).
+constexpr int N_T = NDMA;   // tile DMA
+constexpr int N_9 = 4 + 2;  // nine components: four 2-component DMAs + two 4-byte DMAs
+constexpr int N_GN = 3 + N_9;  // G DMA + next-plane DMA
+constexpr int N_ST = 9;        // output stores
 }  // namespace march
 
-template <int DYN>
+// s_waitcnt immediates (gfx9 encoding: vmcnt in bits 3:0 and 15:14, expcnt
+// 6:4, lgkmcnt 11:8): wait until at most N vector-memory operations of this
+// wave are outstanding; the compiler's own wait tracking sees these.
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+// Workgroup barrier for LDS data only: every LDS access of this wave has
+// completed; outstanding global stores and DMA stay in flight (their
+// consumers wait on vmcnt themselves).  The memory clobber keeps the
+// compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// LDS byte address of a __shared__ object.
+__device__ __forceinline__ unsigned lds_addr(const void* p) {
+  return (unsigned)(size_t)(const __attribute__((address_space(3))) char*)p;
+}
+// Reads of the per-wave DMA buffers (N doubles, 512 bytes apart, from LDS
+// byte address a), completed inside the statement (lgkmcnt 0).  They are
+// written here because the compiler's own wait tracking for LDS DMA makes a
+// plain read of these buffers wait for every DMA in flight, which would
+// drain the fetch pipeline; the kernel's vmcnt waits order them instead.
+__device__ __forceinline__ void lds_read6(unsigned a, double (&v)[6]) {
+  asm volatile(
+      "ds_read_b64 %0, %6\n\t"
+      "ds_read_b64 %1, %6 offset:512\n\t"
+      "ds_read_b64 %2, %6 offset:1024\n\t"
+      "ds_read_b64 %3, %6 offset:1536\n\t"
+      "ds_read_b64 %4, %6 offset:2048\n\t"
+      "ds_read_b64 %5, %6 offset:2560\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5])
+      : "v"(a)
+      : "memory");
+}
+__device__ __forceinline__ void lds_read9(unsigned a, double (&v)[9]) {
+  asm volatile(
+      "ds_read_b64 %0, %9\n\t"
+      "ds_read_b64 %1, %9 offset:512\n\t"
+      "ds_read_b64 %2, %9 offset:1024\n\t"
+      "ds_read_b64 %3, %9 offset:1536\n\t"
+      "ds_read_b64 %4, %9 offset:2048\n\t"
+      "ds_read_b64 %5, %9 offset:2560\n\t"
+      "ds_read_b64 %6, %9 offset:3072\n\t"
+      "ds_read_b64 %7, %9 offset:3584\n\t"
+      "ds_read_b64 %8, %9 offset:4096\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]), "=&v"(v[6]), "=&v"(v[7]),
+        "=&v"(v[8])
+      : "v"(a)
+      : "memory");
+}
+
+// Plane-marching flux + stage kernel (3D fp64, central upwind, glide + climb
+// or glide only).  A block owns a TC x TR column of cells (fast axes) and
+// marches a chunk of planes along the slow axis, one cell per thread.  Per
+// plane it DMAs (global_load_lds) into LDS the halo tile of the stage field,
+// the six non-local stress planes G, the slow axis's next plane and (stage
+// combinations) the step-start state U, and carries the slow-axis
+// differences in registers.
+// The fetches are pipelined so that the memory pipe never drains between
+// planes: the tile of plane p + 2 is fetched into the other of two tile
+// buffers while plane p is computed; G(p + 1) and f(p + 3) as soon as plane
+// p has read theirs, U(p + 1) as soon as U(p) is read.
+// CLIMB: Lambda != 0 (climb suppression terms compiled in); COMB: a stage
+// combination (mode != 0, reads U) rather than the bare rhs.  The host picks
+// the instance from the job; both are compile-time so that the compiler's
+// wait tracking sees one vector-memory sequence per step.
+template <int DYN, bool CLIMB, bool COMB>
 __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk) {
   using T = double;
   using namespace march;
@@ -781,11 +899,15 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
   const T* __restrict__ Up = job.U;
   T* __restrict__ outp = job.out;
   if (job.clk && job.clk->done) return;
-  __shared__ T tile[9 * TILE];
-  __shared__ T vsh[4 * 8][NT];
-  __shared__ T gb[NT / 64][6][64];     // stress planes
-  __shared__ T ub[NT / 64][10][64];    // U cells (plane 9 repeats component 8)
-  __shared__ T nbuf[NT / 64][10][64];  // the ring's next plane, same layout
+  // two tile buffers as separate arrays (planes of even / odd offset in the
+  // chunk), so the compiler sees that reads of one never wait for the DMA
+  // into the other; 161824 bytes of LDS in all
+  __shared__ T tile0[TBUF];
+  __shared__ T tile1[TBUF];
+  __shared__ T vsh[3 * 8][NT];
+  __shared__ T gb[NT / 64][6][64];    // stress planes
+  __shared__ T nbuf[NT / 64][9][64];  // the slow axis's next plane
+  __shared__ T ub[NT / 64][9][64];    // U cells
   __shared__ T red[NT / 64];
   const int n = job.n;
   const long long cells = job.cells, splane = (long long)n * n;
@@ -803,17 +925,18 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
   const T dt = job.clk ? T(job.clk->dt) : T(0);
   const T mu = T(job.mu), lam = T(job.lam), Lambda = T(job.Lambda);
   const T cu = T(job.cu), cf = T(job.cf);
-  const bool comb = job.mode != 0;
   auto wrapp = [n](int p) { return p < 0 ? p + n : (p >= n ? p - n : p); };
 
   // per-lane element offsets inside one plane (plane base added per step):
   // halo-tile pairs (component, row, even column; n even, so a pair never
-  // straddles the periodic wrap), the G/U cells of this lane's pair, and
-  // this thread's own cell
+  // straddles the periodic wrap), the cells of this lane's pair in the
+  // per-wave buffers, the cell of this lane's 4-byte word of component 8,
+  // and this thread's own cell
   unsigned toff[NDMA];
 #pragma unroll
   for (int k = 0; k < NDMA; ++k) {
-    const int e = k * NT + wv * 64 + lane;
+    const int wk = (k == NDMA - 1 && wv > WLAST) ? WLAST : wv;
+    const int e = k * NT + wk * 64 + lane;
     const int ee = e < NPAIR ? e : 0;
     const int comp = ee / (TILE / 2), rem = ee % (TILE / 2);
     const int gr = wrapp(r0 - 2 + rem / (PW / 2)), gc = wrapp(c0 - 2 + 2 * (rem % (PW / 2)));
@@ -821,39 +944,41 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
   }
   const int half = lane >> 5, e2 = 64 * wv + 2 * (lane & 31);
   const unsigned poff = (unsigned)((r0 + e2 / TC) * n + c0 + e2 % TC);
+  const unsigned woff = (unsigned)((r0 + 2 * wv) * n + c0 + (lane >> 1));  // + n: the wave's second row
   const unsigned xoff = (unsigned)((r0 + trr) * n + c0 + tc);
+  const unsigned a_gb = lds_addr(&gb[wv][0][lane]), a_nb = lds_addr(&nbuf[wv][0][lane]),
+                 a_ub = lds_addr(&ub[wv][0][lane]);
 
-  auto dma_tile = [&](int p) {
-    const T* base = A + (long long)p * splane;
+  // Planes wrap periodically, so a fetch past the chunk reads valid cells
+  // that are never used.
+  auto dma_tile = [&](int p, T* dst) {
+    const T* base = A + (long long)wrapp(p) * splane;
 #pragma unroll
     for (int k = 0; k < NDMA; ++k) {
-      if (k * NT + wv * 64 + lane < NPAIR)
-        __builtin_amdgcn_global_load_lds((const void*)(base + toff[k]), (lds_void*)&tile[2 * (k * NT + wv * 64)], 16,
-                                         0, 0);
+      const int wk = (k == NDMA - 1 && wv > WLAST) ? WLAST : wv;
+      __builtin_amdgcn_global_load_lds((const void*)(base + toff[k]), (lds_void*)&dst[2 * (k * NT + wk * 64)], 16, 0,
+                                       0);
     }
   };
+  // two components per 16-byte DMA: lanes 0-31 the even one, 32-63 the odd
   auto dma_g = [&](int p) {
-    const T* base = Gp + (long long)p * splane + poff;
+    const T* base = Gp + (long long)wrapp(p) * splane + poff;
 #pragma unroll
     for (int k = 0; k < 3; ++k)
       __builtin_amdgcn_global_load_lds((const void*)(base + (2 * k + half) * cells), (lds_void*)&gb[wv][2 * k][0], 16,
                                        0, 0);
   };
-  auto dma_u = [&](int p) {
-    const T* base = Up + (long long)p * splane + poff;
+  // nine components: 0-7 in pairs, 8 as 4-byte words (lane l: word l & 1 of
+  // cell l >> 1 of the wave's first, then second row)
+  auto dma_9 = [&](const T* src, int p, T (*dst)[64]) {
+    const T* plane = src + (long long)wrapp(p) * splane;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      const int c = 2 * k + half < 9 ? 2 * k + half : 8;
-      __builtin_amdgcn_global_load_lds((const void*)(base + c * cells), (lds_void*)&ub[wv][2 * k][0], 16, 0, 0);
-    }
-  };
-  auto dma_next = [&](int p) {
-    const T* base = A + (long long)wrapp(p) * splane + poff;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      const int c = 2 * k + half < 9 ? 2 * k + half : 8;
-      __builtin_amdgcn_global_load_lds((const void*)(base + c * cells), (lds_void*)&nbuf[wv][2 * k][0], 16, 0, 0);
-    }
+    for (int k = 0; k < 4; ++k)
+      __builtin_amdgcn_global_load_lds((const void*)(plane + poff + (2 * k + half) * cells), (lds_void*)&dst[2 * k][0],
+                                       16, 0, 0);
+    const char* w8 = (const char*)(plane + 8 * cells + woff) + 4 * (lane & 1);
+    __builtin_amdgcn_global_load_lds((const void*)w8, (lds_void*)&dst[8][0], 4, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void*)(w8 + 8 * (size_t)n), (lds_void*)&dst[8][32], 4, 0, 0);
   };
   auto center = [&](int p, T (&v)[9]) {
     const T* base = A + (long long)wrapp(p) * splane + xoff;
@@ -868,9 +993,11 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
   // hands on D_{p+1} - D_p and D_p + m_p/2.  Every value comes from the same
   // operands by the same operation as in limited_derivs, so the result is
   // bit-identical to it.  f(p+2) arrives in nbuf one plane ahead.
-  dma_tile(x0);
+  dma_tile(x0, tile0);
+  dma_tile(x0 + 1, tile1);
   dma_g(x0);
-  dma_next(x0 + 2);
+  dma_9(A, x0 + 2, nbuf[wv]);
+  if constexpr (COMB) dma_9(Up, x0, ub[wv]);
   T xf[9], xd[9], xs[9], xum[9];
   {
     T fm2[9], fm1[9], f0[9];
@@ -886,18 +1013,27 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
       xum[c] = fma(T(0.5), minmod2(xs[c], Dm1 - Dm2), Dm1);
     }
   }
-  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  wait_vmcnt<0>();
   T run_max = -INFINITY;
   const int lr = trr + 2, lc = tc + 2;
-  for (int p = x0; p < x0 + chunk; ++p) {
-    // tile(p) and G(p) have landed (each wave waited for its own DMA at the
-    // end of the previous step, or in the prologue); the barrier publishes
-    // every wave's part
-    __syncthreads();
+
+  // One plane.  Per wave, the vector-memory issue order of a step is
+  //   [G(p+1), next(p+3)] [tile(p+2)] [stores(p)] [U(p+1)]
+  // so U(p) (issued last on the step before) is complete once at most
+  // N_GN + N_T younger operations are outstanding, and G(p+1), next(p+3)
+  // and tile(p+1) (issued on the step before) once at most N_T + N_ST
+  // (+ N_9 for U(p+1)) are.
+  auto step = [&](int p, T* tcur) __attribute__((always_inline)) {
+    // tile(p), G(p) and next(p+2) (this wave's part) landed at the wait that
+    // closed the previous step; the barrier publishes every wave's part
+    lds_barrier();
+    T nv[9], gv[6];
+    lds_read9(a_nb, nv);
+    lds_read6(a_gb, gv);
     T u[3][2][9], fc[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
-      const T* tb = tile + c * TILE;
+      const T* tb = tcur + c * TILE;
       const T ctr = tb[lr * PW + lc];
       fc[c] = ctr;
       {
@@ -909,28 +1045,30 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
         limited_derivs<T>(f, u[1][0][c], u[1][1][c]);
       }
       {  // slow axis, from the carried differences
-        const T D1 = nbuf[wv][c][lane] - xf[c];
+        const T D1 = nv[c] - xf[c];
         const T s1 = D1 - xd[c];
         const T m = minmod2(s1, xs[c]);
         u[0][0][c] = fma(T(-0.5), m, xd[c]);
         u[0][1][c] = xum[c];
         xum[c] = fma(T(0.5), m, xd[c]);
-        xf[c] = nbuf[wv][c][lane];
+        xf[c] = nv[c];
         xs[c] = s1;
         xd[c] = D1;
       }
     }
+    // this wave's G and next-plane cells are read: refill them for the next
+    // plane (per-wave buffers, so no barrier is needed)
+    dma_g(p + 1);
+    dma_9(A, p + 3, nbuf[wv]);
     T sg[9];
     {
       const T tr = fc[0] + fc[4] + fc[8];
-      const T* g = &gb[wv][0][lane];
-      const T g0 = g[0], g1 = g[64], g2 = g[128], g3 = g[192], g4 = g[256], g5 = g[320];
-      sg[0] = g0 - lam * tr - mu * (fc[0] + fc[0]);
-      sg[4] = g1 - lam * tr - mu * (fc[4] + fc[4]);
-      sg[8] = g2 - lam * tr - mu * (fc[8] + fc[8]);
-      sg[1] = sg[3] = g3 - mu * (fc[1] + fc[3]);
-      sg[2] = sg[6] = g4 - mu * (fc[2] + fc[6]);
-      sg[5] = sg[7] = g5 - mu * (fc[5] + fc[7]);
+      sg[0] = gv[0] - lam * tr - mu * (fc[0] + fc[0]);
+      sg[4] = gv[1] - lam * tr - mu * (fc[4] + fc[4]);
+      sg[8] = gv[2] - lam * tr - mu * (fc[8] + fc[8]);
+      sg[1] = sg[3] = gv[3] - mu * (fc[1] + fc[3]);
+      sg[2] = sg[6] = gv[4] - mu * (fc[2] + fc[6]);
+      sg[5] = sg[7] = gv[5] - mu * (fc[5] + fc[7]);
     }
     // materialise the derivatives here: left to itself the compiler sinks
     // their arithmetic past the barrier and keeps all 81 raw stencil values
@@ -943,40 +1081,44 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
         for (int c = 0; c < 9; ++c) asm volatile("" : "+v"(u[a][s][c]));
 #pragma unroll
     for (int c = 0; c < 9; ++c) asm volatile("" : "+v"(sg[c]), "+v"(fc[c]));
-    // every wave is done with tile(p), G(p) and its nbuf cells before they refill
-    __syncthreads();
-    if (p + 1 < x0 + chunk) {
-      dma_tile(p + 1);
-      dma_g(p + 1);
-    }
-    if (comb) dma_u(p);
-    dma_next(p + 3);  // f(p + 3): read at the next plane
+    // every wave is done with tile(p) before tile(p + 2) refills its buffer
+    lds_barrier();
+    dma_tile(p + 2, tcur);
     T rhs[9], speed;
-    cell_flux<T, 3, DYN, true>(u, sg, Lambda, rhs, speed, &vsh[0][tid], NT, T(n));
+    cell_flux<T, 3, DYN, true, false, CLIMB ? 1 : 0, true>(u, sg, Lambda, rhs, speed, &vsh[0][tid], NT, T(n));
     bool fin = isfinite(speed);
 #pragma unroll
     for (int c = 0; c < 9; ++c) fin = fin && isfinite(rhs[c]);
     run_max = fmax(run_max, fin ? speed : T(INFINITY));
-    // everything issued this step has landed: U(p) and plane p + 3's cells
-    // (this wave's own), tile(p + 1) and G(p + 1)
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     T* o = outp + (long long)p * splane + xoff;
-    if (comb) {
-      T ov[9], uv[9];
+    if constexpr (COMB) {
+      wait_vmcnt<N_GN + N_T>();  // U(p)
+      T uv[9], ov[9];
+      lds_read9(a_ub, uv);
 #pragma unroll
       for (int c = 0; c < 9; ++c) {
-        uv[c] = ub[wv][c][lane];
         ov[c] = cu * uv[c] + cf * (fc[c] + dt * rhs[c]);
         o[c * cells] = ov[c];
       }
       // vacancy coupling after the last stage: c += beta (tr F2 - tr u)
-      if (job.vac)
+      if (job.vac) {
         job.vac[(long long)p * splane + xoff] += job.vbeta * ((ov[0] + ov[4] + ov[8]) - (uv[0] + uv[4] + uv[8]));
+        wait_vmcnt<0>();
+      }
+      dma_9(Up, p + 1, ub[wv]);
+      wait_vmcnt<N_T + N_ST + N_9>();
     } else {
 #pragma unroll
       for (int c = 0; c < 9; ++c) o[c * cells] = rhs[c];
+      wait_vmcnt<N_T + N_ST>();
     }
+  };
+  for (int p = x0; p < x0 + chunk; p += 2) {
+    step(p, tile0);
+    step(p + 1, tile1);
   }
+  // nothing may still write this block's LDS when it exits
+  wait_vmcnt<0>();
   if (job.partial) {
     const T m = block_max<T>(run_max, red);
     if (tid == 0) job.partial[blockIdx.x] = m;
@@ -986,7 +1128,8 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
 // Planes per marching block: the longest chunk (a power of two, at least 16
 // planes) that still gives every CU of the current device a block; the LDS
 // footprint admits one block per CU, so longer chunks only save prologues.
-// The CDD_FLUX_CHUNK environment variable overrides it (0 selects k_flux).
+// The CDD_FLUX_CHUNK environment variable overrides it (0 selects k_flux);
+// the kernel marches planes in pairs, so an override must be even.
 static int march_chunk(int n) {
   static int env = -2;
   if (env == -2) {
@@ -994,7 +1137,7 @@ static int march_chunk(int n) {
     env = s ? atoi(s) : -1;
   }
   if (env == 0) return 0;
-  if (env > 0) return (env <= n && n % env == 0) ? env : 0;
+  if (env > 0) return (env <= n && n % env == 0 && env % 2 == 0) ? env : 0;
   static int cus[64] = {0};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
@@ -1133,10 +1276,18 @@ hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
     const int chunk = march_chunk(job.n);
     const int nb = (job.n / march::TC) * (job.n / march::TR) * (job.n / chunk);
     if (nblocks_out) *nblocks_out = nb;
-    if (job.dyn == DYN_GLIDE_ONLY)
-      hipLaunchKernelGGL((k_flux_march<DYN_GLIDE_ONLY>), dim3(nb), dim3(march::NT), 0, job.stream, job, chunk);
-    else
-      hipLaunchKernelGGL((k_flux_march<DYN_GLIDE_CLIMB>), dim3(nb), dim3(march::NT), 0, job.stream, job, chunk);
+    const bool comb = job.mode != 0;
+#define CDD_MARCH(DYN_, CLIMB_)                                                                                     \
+  do {                                                                                                            \
+    if (comb)                                                                                                     \
+      hipLaunchKernelGGL((k_flux_march<DYN_, CLIMB_, true>), dim3(nb), dim3(march::NT), 0, job.stream, job, chunk);  \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_flux_march<DYN_, CLIMB_, false>), dim3(nb), dim3(march::NT), 0, job.stream, job, chunk); \
+  } while (0)
+    if (job.dyn == DYN_GLIDE_ONLY) CDD_MARCH(DYN_GLIDE_ONLY, false);
+    else if (job.Lambda != 0.0) CDD_MARCH(DYN_GLIDE_CLIMB, true);
+    else CDD_MARCH(DYN_GLIDE_CLIMB, false);
+#undef CDD_MARCH
     return hipGetLastError();
   }
   dim3 g, b;
