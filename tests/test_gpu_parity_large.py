@@ -32,9 +32,20 @@ def env():
     return A, OC
 
 
+_FIELDS = {}
+
+
+def _field(n, seed):
+    key = (n, seed)
+    if key not in _FIELDS:
+        _FIELDS.clear()
+        _FIELDS[key] = SpectralGaussianField((n, n, n), 6.0, 0.01, seed=seed)
+    return _FIELDS[key]
+
+
 def _compare(env, n, nsteps, seed=2, dynamics=0, Lambda=0.9, rk=3):
     A, OC = env
-    beta = SpectralGaussianField((n, n, n), 6.0, 0.01, seed=seed)
+    beta = _field(n, seed)
     p = A.Plan(3, n, dynamics=dynamics, rk_order=rk, Lambda=Lambda, mu=1.0, nu=0.3, cfl=0.25)
     p.set_state(beta)
     p.set_time(0.0)
@@ -77,6 +88,16 @@ def test_3d_64_50_steps(env):
 def test_3d_128_5_steps(env):
     say("3D 128^3 x 5 RK3 steps vs C oracle")
     _compare(env, 128, 5)
+
+
+@pytest.mark.parametrize("dynamics,Lambda", [(0, 0.9), (0, 0.0), (1, 0.0)])
+def test_3d_256_one_step(env, dynamics, Lambda):
+    """256^3, the default bench grid: every column's block marches the
+    whole 256-plane ring as one chunk (both tile buffers, the fetches past
+    the chunk wrapping round), through all three RK3 stages, for each
+    instance of the plane-marching kernel (climb on / off, glide only)."""
+    say("3D 256^3 x 1 RK3 step vs C oracle", dynamics, Lambda)
+    _compare(env, 256, 1, seed=3, dynamics=dynamics, Lambda=Lambda)
 
 
 @pytest.mark.full_size
