@@ -912,7 +912,9 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
   const int n = job.n;
   const long long cells = job.cells, splane = (long long)n * n;
   const int tid = threadIdx.x, tc = tid % TC, trr = tid / TC;
-  const int wv = tid >> 6, lane = tid & 63;
+  // the wave index is uniform: in a scalar register, the per-wave LDS
+  // addresses (the DMA destinations) are scalar arithmetic
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   // block -> (chunk of planes, column); each XCD (hardware block b runs on
   // XCD b % 8) takes one contiguous range of logical blocks, i.e. adjacent
   // columns at the same planes, so shared halo rows are served from its L2
