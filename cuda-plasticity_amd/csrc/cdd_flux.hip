@@ -787,17 +787,9 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   }
 }
 
-// Plane-marching variant of k_flux for the 3D fp64 stepper (no per-cell speed
-// output, one instance).  A persistent block owns a 32 x 8 column of the two
-// fast axes and walks `chunk` consecutive planes of the slow axis (axis 0).
-// While plane p is computed, the next plane's halo tile (all nine
-// components), its six stress planes G, this plane's U cells and the cells
-// three planes ahead stream into LDS by asynchronous global->LDS DMA; the
-// slow-axis derivatives are carried in registers from plane to plane.  Each plane is read once per column (plus its halo),
-// and a block's load latency hides behind the previous plane's arithmetic:
-// the only wait in the loop follows the arithmetic, when the outputs
-// stored one step earlier have long completed.
-// The per-cell arithmetic is cell_flux, exactly as in k_flux.
+// Plane-marching variant of k_flux for the 3D fp64 stepper (no per-cell
+// speed output, one instance), described at k_flux_march below.  The
+// per-cell arithmetic is cell_flux, as in k_flux.
 namespace march {
 constexpr int TC = 32, TR = 8, NT = TC * TR;
 constexpr int PW = TC + 4, PH = TR + 4, TILE = PW * PH;  // 36 x 12 halo tile
