@@ -793,23 +793,31 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
 namespace march {
 constexpr int TC = 32, TR = 8, NT = TC * TR;
 constexpr int PW = TC + 4, PH = TR + 4, TILE = PW * PH;  // 36 x 12 halo tile
-constexpr int NPAIR = 9 * TILE / 2;                     // 16-byte DMA pairs per tile
-constexpr int NDMA = (NPAIR + NT - 1) / NT;             // DMA instructions per thread
-// The last DMA round is needed by waves 0..WLAST only; the waves above repeat
-// wave WLAST's (same cells to the same LDS words), so every wave issues NDMA
-// tile DMAs, and lanes past the tile fetch pair 0 into the buffer's tail.
-constexpr int WLAST = (NPAIR - 1 - (NDMA - 1) * NT) / 64;
-constexpr int TBUF = 2 * ((NDMA - 1) * NT + (WLAST + 1) * 64);  // tile buffer (doubles)
-static_assert(PW % 2 == 0, "pairs must not straddle tile rows");
-static_assert(NDMA >= 2 && WLAST >= 0 && WLAST < NT / 64, "tile DMA rounds");
-// Vector-memory instructions per wave and group, for the s_waitcnt vmcnt
-// counts of the pipeline.  Every group is issued whole, by every lane, on
-// every step (a conditional DMA makes the compiler's wait tracking merge
-// paths and wait for every DMA in flight).
-constexpr int N_T = NDMA;   // tile DMA
-constexpr int N_9 = 4 + 2;  // nine components: four 2-component DMAs + two 4-byte DMAs
-constexpr int N_GN = 3 + N_9;  // G DMA + next-plane DMA
-constexpr int N_ST = 9;        // output stores
+// DMA geometry per element type: fp64 moves 16 bytes (two values of a tile
+// row) per lane, fp32 4 bytes (one value).
+template <typename T>
+struct Dma {
+  static constexpr int EPL = sizeof(T) == 8 ? 2 : 1;       // tile values per lane and DMA
+  static constexpr int BYTES = EPL * (int)sizeof(T);       // 16 or 4
+  static constexpr int NUNIT = 9 * TILE / EPL;             // lane units per tile
+  static constexpr int NDMA = (NUNIT + NT - 1) / NT;       // tile DMA instructions per thread
+  // The last DMA round is needed by waves 0..WLAST only; the waves above
+  // repeat wave WLAST's (same cells to the same LDS words), so every wave
+  // issues NDMA tile DMAs, and lanes past the tile fetch unit 0 into the
+  // buffer's tail.
+  static constexpr int WLAST = (NUNIT - 1 - (NDMA - 1) * NT) / 64;
+  static constexpr int TBUF = EPL * ((NDMA - 1) * NT + (WLAST + 1) * 64);  // tile buffer (values)
+  static_assert(PW % EPL == 0, "DMA units must not straddle tile rows");
+  static_assert(NDMA >= 2 && WLAST >= 0 && WLAST < NT / 64, "tile DMA rounds");
+  // Vector-memory instructions per wave and group, for the s_waitcnt vmcnt
+  // counts of the pipeline.  Every group is issued whole, by every lane, on
+  // every step (a conditional DMA makes the compiler's wait tracking merge
+  // paths and wait for every DMA in flight).
+  static constexpr int N_T = NDMA;                          // tile DMA
+  static constexpr int N_9 = sizeof(T) == 8 ? 4 + 2 : 9;    // nine per-wave components
+  static constexpr int N_GN = (sizeof(T) == 8 ? 3 : 6) + N_9;  // G + next plane
+  static constexpr int N_ST = 9;                            // output stores
+};
 }  // namespace march
 
 // s_waitcnt immediates (gfx9 encoding: vmcnt in bits 3:0 and 15:14, expcnt
@@ -865,9 +873,41 @@ __device__ __forceinline__ void lds_read9(unsigned a, double (&v)[9]) {
       : "v"(a)
       : "memory");
 }
+// The same for fp32 buffers (N floats, 256 bytes apart).
+__device__ __forceinline__ void lds_read6(unsigned a, float (&v)[6]) {
+  asm volatile(
+      "ds_read_b32 %0, %6\n\t"
+      "ds_read_b32 %1, %6 offset:256\n\t"
+      "ds_read_b32 %2, %6 offset:512\n\t"
+      "ds_read_b32 %3, %6 offset:768\n\t"
+      "ds_read_b32 %4, %6 offset:1024\n\t"
+      "ds_read_b32 %5, %6 offset:1280\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5])
+      : "v"(a)
+      : "memory");
+}
+__device__ __forceinline__ void lds_read9(unsigned a, float (&v)[9]) {
+  asm volatile(
+      "ds_read_b32 %0, %9\n\t"
+      "ds_read_b32 %1, %9 offset:256\n\t"
+      "ds_read_b32 %2, %9 offset:512\n\t"
+      "ds_read_b32 %3, %9 offset:768\n\t"
+      "ds_read_b32 %4, %9 offset:1024\n\t"
+      "ds_read_b32 %5, %9 offset:1280\n\t"
+      "ds_read_b32 %6, %9 offset:1536\n\t"
+      "ds_read_b32 %7, %9 offset:1792\n\t"
+      "ds_read_b32 %8, %9 offset:2048\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]), "=&v"(v[6]), "=&v"(v[7]),
+        "=&v"(v[8])
+      : "v"(a)
+      : "memory");
+}
 
-// Plane-marching flux + stage kernel (3D fp64, central upwind, glide + climb
-// or glide only).  A block owns a TC x TR column of cells (fast axes) and
+// Plane-marching flux + stage kernel body (3D, fp64 or fp32, central upwind,
+// glide + climb or glide only; the kernels k_flux_march / k_flux_march32
+// below).  A block owns a TC x TR column of cells (fast axes) and
 // marches a chunk of planes along the slow axis, one cell per thread.  Per
 // plane it DMAs (global_load_lds) into LDS the halo tile of the stage field,
 // the six non-local stress planes G, the slow axis's next plane and (stage
@@ -881,21 +921,24 @@ __device__ __forceinline__ void lds_read9(unsigned a, double (&v)[9]) {
 // combination (mode != 0, reads U) rather than the bare rhs.  The host picks
 // the instance from the job; both are compile-time so that the compiler's
 // wait tracking sees one vector-memory sequence per step.
-template <int DYN, bool CLIMB, bool COMB>
-__global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk) {
-  using T = double;
+template <typename T, int DYN, bool CLIMB, bool COMB>
+__device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
   using namespace march;
+  using DM = Dma<T>;
+  constexpr int NDMA = DM::NDMA, WLAST = DM::WLAST, EPL = DM::EPL, NUNIT = DM::NUNIT;
+  constexpr int N_T = DM::N_T, N_9 = DM::N_9, N_GN = DM::N_GN, N_ST = DM::N_ST;
   typedef __attribute__((address_space(3))) void lds_void;
-  const T* __restrict__ A = job.A;
-  const T* __restrict__ Gp = job.G;
-  const T* __restrict__ Up = job.U;
-  T* __restrict__ outp = job.out;
+  // the job's field pointers carry T data (float for fp32 plans)
+  const T* __restrict__ A = reinterpret_cast<const T*>(job.A);
+  const T* __restrict__ Gp = reinterpret_cast<const T*>(job.G);
+  const T* __restrict__ Up = reinterpret_cast<const T*>(job.U);
+  T* __restrict__ outp = reinterpret_cast<T*>(job.out);
   if (job.clk && job.clk->done) return;
   // two tile buffers as separate arrays (planes of even / odd offset in the
   // chunk), so the compiler sees that reads of one never wait for the DMA
-  // into the other; 161824 bytes of LDS in all
-  __shared__ T tile0[TBUF];
-  __shared__ T tile1[TBUF];
+  // into the other; 161824 bytes of LDS in all for fp64, 80 KB for fp32
+  __shared__ T tile0[DM::TBUF];
+  __shared__ T tile1[DM::TBUF];
   __shared__ T vsh[3 * 8][NT];
   __shared__ T gb[NT / 64][6][64];    // stress planes
   __shared__ T nbuf[NT / 64][9][64];  // the slow axis's next plane
@@ -922,18 +965,18 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
   auto wrapp = [n](int p) { return p < 0 ? p + n : (p >= n ? p - n : p); };
 
   // per-lane element offsets inside one plane (plane base added per step):
-  // halo-tile pairs (component, row, even column; n even, so a pair never
-  // straddles the periodic wrap), the cells of this lane's pair in the
-  // per-wave buffers, the cell of this lane's 4-byte word of component 8,
-  // and this thread's own cell
+  // halo-tile units (component, row, column; a fp64 unit is an even-column
+  // pair, and n is even, so a pair never straddles the periodic wrap), the
+  // cells of this lane in the per-wave buffers (fp64: its pair, and the
+  // cell of its 4-byte word of component 8), and this thread's own cell
   unsigned toff[NDMA];
 #pragma unroll
   for (int k = 0; k < NDMA; ++k) {
     const int wk = (k == NDMA - 1 && wv > WLAST) ? WLAST : wv;
     const int e = k * NT + wk * 64 + lane;
-    const int ee = e < NPAIR ? e : 0;
-    const int comp = ee / (TILE / 2), rem = ee % (TILE / 2);
-    const int gr = wrapp(r0 - 2 + rem / (PW / 2)), gc = wrapp(c0 - 2 + 2 * (rem % (PW / 2)));
+    const int ee = e < NUNIT ? e : 0;
+    const int comp = ee / (TILE / EPL), rem = ee % (TILE / EPL);
+    const int gr = wrapp(r0 - 2 + rem / (PW / EPL)), gc = wrapp(c0 - 2 + EPL * (rem % (PW / EPL)));
     toff[k] = (unsigned)(comp * cells + (long long)gr * n + gc);
   }
   const int half = lane >> 5, e2 = 64 * wv + 2 * (lane & 31);
@@ -950,29 +993,43 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
 #pragma unroll
     for (int k = 0; k < NDMA; ++k) {
       const int wk = (k == NDMA - 1 && wv > WLAST) ? WLAST : wv;
-      __builtin_amdgcn_global_load_lds((const void*)(base + toff[k]), (lds_void*)&dst[2 * (k * NT + wk * 64)], 16, 0,
-                                       0);
+      __builtin_amdgcn_global_load_lds((const void*)(base + toff[k]), (lds_void*)&dst[EPL * (k * NT + wk * 64)],
+                                       DM::BYTES, 0, 0);
     }
   };
-  // two components per 16-byte DMA: lanes 0-31 the even one, 32-63 the odd
+  // fp64: two components per 16-byte DMA, lanes 0-31 the even one, 32-63
+  // the odd; fp32: one component per 4-byte DMA, each lane its own cell
   auto dma_g = [&](int p) {
-    const T* base = Gp + (long long)wrapp(p) * splane + poff;
+    if constexpr (EPL == 2) {
+      const T* base = Gp + (long long)wrapp(p) * splane + poff;
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-      __builtin_amdgcn_global_load_lds((const void*)(base + (2 * k + half) * cells), (lds_void*)&gb[wv][2 * k][0], 16,
-                                       0, 0);
+      for (int k = 0; k < 3; ++k)
+        __builtin_amdgcn_global_load_lds((const void*)(base + (2 * k + half) * cells), (lds_void*)&gb[wv][2 * k][0],
+                                         16, 0, 0);
+    } else {
+      const T* base = Gp + (long long)wrapp(p) * splane + xoff;
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+        __builtin_amdgcn_global_load_lds((const void*)(base + c * cells), (lds_void*)&gb[wv][c][0], 4, 0, 0);
+    }
   };
-  // nine components: 0-7 in pairs, 8 as 4-byte words (lane l: word l & 1 of
-  // cell l >> 1 of the wave's first, then second row)
+  // nine components; fp64: 0-7 in pairs, 8 as 4-byte words (lane l: word
+  // l & 1 of cell l >> 1 of the wave's first, then second row)
   auto dma_9 = [&](const T* src, int p, T (*dst)[64]) {
     const T* plane = src + (long long)wrapp(p) * splane;
+    if constexpr (EPL == 2) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      __builtin_amdgcn_global_load_lds((const void*)(plane + poff + (2 * k + half) * cells), (lds_void*)&dst[2 * k][0],
-                                       16, 0, 0);
-    const char* w8 = (const char*)(plane + 8 * cells + woff) + 4 * (lane & 1);
-    __builtin_amdgcn_global_load_lds((const void*)w8, (lds_void*)&dst[8][0], 4, 0, 0);
-    __builtin_amdgcn_global_load_lds((const void*)(w8 + 8 * (size_t)n), (lds_void*)&dst[8][32], 4, 0, 0);
+      for (int k = 0; k < 4; ++k)
+        __builtin_amdgcn_global_load_lds((const void*)(plane + poff + (2 * k + half) * cells),
+                                         (lds_void*)&dst[2 * k][0], 16, 0, 0);
+      const char* w8 = (const char*)(plane + 8 * cells + woff) + 4 * (lane & 1);
+      __builtin_amdgcn_global_load_lds((const void*)w8, (lds_void*)&dst[8][0], 4, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(w8 + 8 * (size_t)n), (lds_void*)&dst[8][32], 4, 0, 0);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 9; ++c)
+        __builtin_amdgcn_global_load_lds((const void*)(plane + xoff + c * cells), (lds_void*)&dst[c][0], 4, 0, 0);
+    }
   };
   auto center = [&](int p, T (&v)[9]) {
     const T* base = A + (long long)wrapp(p) * splane + xoff;
@@ -1115,8 +1172,18 @@ __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk
   wait_vmcnt<0>();
   if (job.partial) {
     const T m = block_max<T>(run_max, red);
-    if (tid == 0) job.partial[blockIdx.x] = m;
+    if (tid == 0) job.partial[blockIdx.x] = (double)m;
   }
+}
+
+template <int DYN, bool CLIMB, bool COMB>
+__global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk) {
+  flux_march_body<double, DYN, CLIMB, COMB>(job, chunk);
+}
+// fp32 plans: the same kernel on float fields
+template <int DYN, bool CLIMB, bool COMB>
+__global__ void __launch_bounds__(march::NT) k_flux_march32(FluxJob job, int chunk) {
+  flux_march_body<float, DYN, CLIMB, COMB>(job, chunk);
 }
 
 // Planes per marching block: the longest chunk (a power of two, at least 16
@@ -1147,7 +1214,7 @@ static int march_chunk(int n) {
 }
 
 static bool march_ok(const FluxJob& job) {
-  return job.d == 3 && !job.fp32 && !job.llf && !job.S9 && !job.Ad && !job.speed && job.count <= 1 &&
+  return job.d == 3 && !job.llf && !job.S9 && !job.Ad && !job.speed && job.count <= 1 &&
          job.tile == 0 && job.n >= 32 &&
          job.dyn != DYN_WOJZZ && (job.mode == 0 || job.U) && march_chunk(job.n) > 0;
 }
@@ -1271,16 +1338,22 @@ hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
     const int nb = (job.n / march::TC) * (job.n / march::TR) * (job.n / chunk);
     if (nblocks_out) *nblocks_out = nb;
     const bool comb = job.mode != 0;
-#define CDD_MARCH(DYN_, CLIMB_)                                                                                     \
-  do {                                                                                                            \
-    if (comb)                                                                                                     \
-      hipLaunchKernelGGL((k_flux_march<DYN_, CLIMB_, true>), dim3(nb), dim3(march::NT), 0, job.stream, job, chunk);  \
-    else                                                                                                          \
-      hipLaunchKernelGGL((k_flux_march<DYN_, CLIMB_, false>), dim3(nb), dim3(march::NT), 0, job.stream, job, chunk); \
+#define CDD_MARCH(K_, DYN_, CLIMB_)                                                                        \
+  do {                                                                                                     \
+    if (comb)                                                                                              \
+      hipLaunchKernelGGL((K_<DYN_, CLIMB_, true>), dim3(nb), dim3(march::NT), 0, job.stream, job, chunk);  \
+    else                                                                                                   \
+      hipLaunchKernelGGL((K_<DYN_, CLIMB_, false>), dim3(nb), dim3(march::NT), 0, job.stream, job, chunk); \
   } while (0)
-    if (job.dyn == DYN_GLIDE_ONLY) CDD_MARCH(DYN_GLIDE_ONLY, false);
-    else if (job.Lambda != 0.0) CDD_MARCH(DYN_GLIDE_CLIMB, true);
-    else CDD_MARCH(DYN_GLIDE_CLIMB, false);
+#define CDD_MARCH_T(K_)                                           \
+  do {                                                            \
+    if (job.dyn == DYN_GLIDE_ONLY) CDD_MARCH(K_, DYN_GLIDE_ONLY, false); \
+    else if (job.Lambda != 0.0) CDD_MARCH(K_, DYN_GLIDE_CLIMB, true);    \
+    else CDD_MARCH(K_, DYN_GLIDE_CLIMB, false);                          \
+  } while (0)
+    if (!job.fp32) CDD_MARCH_T(k_flux_march);
+    else CDD_MARCH_T(k_flux_march32);
+#undef CDD_MARCH_T
 #undef CDD_MARCH
     return hipGetLastError();
   }
