@@ -1180,18 +1180,20 @@ template <int DYN, bool CLIMB, bool COMB>
 __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk) {
   flux_march_body<double, DYN, CLIMB, COMB>(job, chunk);
 }
-// fp32 plans: the same kernel on float fields
+// fp32 plans: the same kernel on float fields.  Its registers (at most 256
+// per lane) and LDS (80 KB) admit two blocks per CU, two waves per SIMD.
 template <int DYN, bool CLIMB, bool COMB>
-__global__ void __launch_bounds__(march::NT) k_flux_march32(FluxJob job, int chunk) {
+__global__ void __launch_bounds__(march::NT, 2) k_flux_march32(FluxJob job, int chunk) {
   flux_march_body<float, DYN, CLIMB, COMB>(job, chunk);
 }
 
 // Planes per marching block: the longest chunk (a power of two, at least 16
-// planes) that still gives every CU of the current device a block; the LDS
-// footprint admits one block per CU, so longer chunks only save prologues.
+// planes) that still gives every CU of the current device per_cu blocks (the
+// fp64 kernel's LDS admits one block per CU, the fp32 kernel's two), so
+// longer chunks only save prologues.
 // The CDD_FLUX_CHUNK environment variable overrides it (0 selects k_flux);
 // the kernel marches planes in pairs, so an override must be even.
-static int march_chunk(int n) {
+static int march_chunk(int n, int per_cu = 1) {
   static int env = -2;
   if (env == -2) {
     const char* s = getenv("CDD_FLUX_CHUNK");
@@ -1209,7 +1211,7 @@ static int march_chunk(int n) {
   }
   const int ncol = (n / march::TC) * (n / march::TR);
   int chunk = n;
-  while (chunk > 16 && ncol * (n / chunk) < cus[dev]) chunk /= 2;
+  while (chunk > 16 && ncol * (n / chunk) < per_cu * cus[dev]) chunk /= 2;
   return chunk;
 }
 
@@ -1334,7 +1336,7 @@ hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
     return hipGetLastError();
   }
   if (march_ok(job)) {
-    const int chunk = march_chunk(job.n);
+    const int chunk = march_chunk(job.n, job.fp32 ? 2 : 1);
     const int nb = (job.n / march::TC) * (job.n / march::TR) * (job.n / chunk);
     if (nblocks_out) *nblocks_out = nb;
     const bool comb = job.mode != 0;
