@@ -44,26 +44,34 @@ class TVDRungeKutta_FieldMover(FieldMover):
         p = self._plan(state, dynamics)
         p.set_state(state.betaP.data)
         p.set_time(float(startTime), float(endTime), float(fixedTimeStep) if fixedTimeStep else 0.0)
+
+        def snapshot():
+            return PlasticityState(state.gridShape, field=TensorField(state.gridShape, data=p.get_state()),
+                                   inherit=state)
+        return self._drive(p, observers, snapshot)
+
+    def _drive(self, p, observers, snapshot):
+        """Step the plan to its end time, calling every observer after every
+        step (chunks of self.chunk steps when there are none).  A non-finite
+        speed or right-hand side stops the run on the device with the state
+        at that step's start: that raises FloatingPointError here, before
+        any observer sees the unadvanced state again."""
         steps0 = p.clock()[3]
-        if observers:
-            while True:
-                p.step(1)
-                t, dt, done, ns = p.clock()
-                cur = PlasticityState(state.gridShape, field=TensorField(state.gridShape, data=p.get_state()),
-                                      inherit=state)
+        while True:
+            p.step(1 if observers else self.chunk)
+            t, dt, done, ns = p.clock()
+            if p.status():
+                self.numberOfSteps = ns - steps0
+                raise FloatingPointError("non-finite speed or flux at t = %.17g after %d steps; the state is "
+                                         "left at that step's start" % (t, ns - steps0))
+            if observers:
+                cur = snapshot()
                 for observer in observers:
                     observer.Update(t, cur)
-                if done:
-                    break
-        else:
-            while True:
-                p.step(self.chunk)
-                t, dt, done, ns = p.clock()
-                if done:
-                    break
+            if done:
+                break
         self.numberOfSteps = p.clock()[3] - steps0
-        self.lastState = PlasticityState(state.gridShape, field=TensorField(state.gridShape, data=p.get_state()),
-                                         inherit=state)
+        self.lastState = snapshot()
         return self.lastState
 
     def _run_disorder(self, state, observers, dynamics, startTime, endTime, fixedTimeStep):
@@ -80,30 +88,12 @@ class TVDRungeKutta_FieldMover(FieldMover):
         p.set_state(state.betaP.data)
         p.set_disorder(state.DisorderField(), advect=True)
         p.set_time(float(startTime), float(endTime), float(fixedTimeStep) if fixedTimeStep else 0.0)
-        steps0 = p.clock()[3]
 
         def snapshot():
             data = torch.cat([p.disorder_get()[None], p.get_state()])
             return DisorderState(state.gridShape, field=TensorScalarMixedField(state.gridShape, data=data),
                                  inherit=state)
-        if observers:
-            while True:
-                p.step(1)
-                t, dt, done, ns = p.clock()
-                cur = snapshot()
-                for observer in observers:
-                    observer.Update(t, cur)
-                if done:
-                    break
-        else:
-            while True:
-                p.step(self.chunk)
-                t, dt, done, ns = p.clock()
-                if done:
-                    break
-        self.numberOfSteps = p.clock()[3] - steps0
-        self.lastState = snapshot()
-        return self.lastState
+        return self._drive(p, observers, snapshot)
 
 
 class OperatorSplittingTVDRK_FieldMover(TVDRungeKutta_FieldMover):
@@ -126,27 +116,9 @@ class OperatorSplittingTVDRK_FieldMover(TVDRungeKutta_FieldMover):
         p.set_state(betaPstate.betaP.data)
         p.vacancy_set(c)
         p.set_time(float(startTime), float(endTime), float(fixedTimeStep) if fixedTimeStep else 0.0)
-        steps0 = p.clock()[3]
 
         def snapshot():
             data = torch.cat([p.vacancy_get()[None], p.get_state()])
             return VacancyState(state.gridShape, field=TensorScalarMixedField(state.gridShape, data=data),
                                 inherit=state)
-        if observers:
-            while True:
-                p.step(1)
-                t, dt, done, ns = p.clock()
-                cur = snapshot()
-                for observer in observers:
-                    observer.Update(t, cur)
-                if done:
-                    break
-        else:
-            while True:
-                p.step(self.chunk)
-                t, dt, done, ns = p.clock()
-                if done:
-                    break
-        self.numberOfSteps = p.clock()[3] - steps0
-        self.lastState = snapshot()
-        return self.lastState
+        return self._drive(p, observers, snapshot)

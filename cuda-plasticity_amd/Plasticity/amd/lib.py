@@ -441,13 +441,27 @@ class Plan:
         check(lib.cdd_lambda_bar(self.h, ctypes.byref(v), 1 if reset else 0))
         return v.value
 
+    def _f64_log(self, t, what, shape=None):
+        """Device pointer of a float64 log/schedule tensor the library reads or
+        writes as double[numel]; anything else is refused before a launch."""
+        if t.dtype != self.torch.float64:
+            raise CDDError("%s must be float64, got %s" % (what, t.dtype))
+        if not t.is_contiguous():
+            raise CDDError("%s must be contiguous" % what)
+        if shape is not None and (t.dim() != len(shape) or any(a is not None and a != b
+                                                             for a, b in zip(shape, t.shape))):
+            raise CDDError("%s must have shape %s, got %s" % (what, shape, tuple(t.shape)))
+        if not t.is_cuda or t.device != self.dev:
+            raise CDDError("%s must be on the plan's device %s, got %s" % (what, self.dev, t.device))
+        return ctypes.c_void_p(t.data_ptr())
+
     def dt_record(self, log):
         """Log the next log.numel() time steps into the float64 device tensor `log`
         (None stops logging)."""
         if log is None:
             check(lib.cdd_dt_record(self.h, None, 0))
         else:
-            check(lib.cdd_dt_record(self.h, _ptr(log), log.numel()))
+            check(lib.cdd_dt_record(self.h, self._f64_log(log, "dt log"), log.numel()))
 
     def plas_append(self, path):
         """Append [t, state] to a .plas file in the reference's layout."""
@@ -466,9 +480,7 @@ class Plan:
         if log is None:
             check(lib.cdd_energy_record(self.h, None, 0))
         else:
-            if log.dtype != self.torch.float64 or not log.is_contiguous() or log.device != self.dev:
-                raise CDDError("energy log must be a contiguous float64 device tensor")
-            check(lib.cdd_energy_record(self.h, _ptr(log), log.numel()))
+            check(lib.cdd_energy_record(self.h, self._f64_log(log, "energy log"), log.numel()))
 
     def status(self):
         """0, or 1 if a non-finite speed stopped the run on the device."""
@@ -490,9 +502,8 @@ class Plan:
         if log is None:
             check(lib.cdd_stress_record(self.h, None, 0))
         else:
-            if log.dtype != self.torch.float64 or not log.is_contiguous() or log.shape[-1] != 8:
-                raise CDDError("stress log must be a contiguous float64 (cap, 8) tensor")
-            check(lib.cdd_stress_record(self.h, _ptr(log), log.numel() // 8))
+            ptr = self._f64_log(log, "stress log", shape=(None, 8))
+            check(lib.cdd_stress_record(self.h, ptr, log.shape[0]))
 
     def dt_replay(self, sched):
         """Take the next sched.numel() time steps from the float64 device tensor
@@ -500,7 +511,7 @@ class Plan:
         if sched is None:
             check(lib.cdd_dt_replay(self.h, None, 0))
         else:
-            check(lib.cdd_dt_replay(self.h, _ptr(sched), sched.numel()))
+            check(lib.cdd_dt_replay(self.h, self._f64_log(sched, "dt schedule"), sched.numel()))
 
     def timing_enable(self, max_events):
         check(lib.cdd_timing_enable(self.h, int(max_events)))

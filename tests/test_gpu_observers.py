@@ -98,3 +98,32 @@ def test_non_finite_speed_stops_the_run(L):
     p.set_time(0.0)
     p.step(2)
     assert p.status() == 0 and p.clock()[3] == 2
+
+
+def test_mover_raises_on_non_finite_state(L):
+    """PlasticitySystem.Run (the reference API) stops with FloatingPointError
+    when the device guard trips: no observer sees the unadvanced state again
+    and Run does not return as if the interval had been covered."""
+    from Plasticity.PlasticitySystem import PlasticitySystem
+    from Plasticity.PlasticityStates.PlasticityState import PlasticityState
+    from Plasticity.Fields.Fields import TensorField
+    from Plasticity.FieldMovers.FieldMover import TVDRungeKutta_FieldMover
+    from Plasticity.FieldDynamics.CentralUpwindHJBetaPGlideOnlyDynamics import BetaPDynamics
+    say("mover nan guard")
+    n = 16
+    beta = SpectralGaussianField((n, n), 2.0, 0.01, seed=27)
+    beta[4, 7, 2] = np.nan
+    state = PlasticityState((n, n), field=TensorField((n, n), data=beta), mu=1.0, nu=0.3)
+    seen = []
+
+    class Count:
+        def Update(self, t, st):
+            seen.append(t)
+
+    for obs in ([Count()], []):
+        seen.clear()
+        mover = TVDRungeKutta_FieldMover(CFLsafeFactor=0.25, order=2)
+        system = PlasticitySystem((n, n), state, mover, BetaPDynamics(), obs)
+        with pytest.raises(FloatingPointError):
+            system.Run(startTime=0.0, endTime=1.0)
+        assert seen == [] and mover.numberOfSteps == 0
