@@ -10,6 +10,7 @@ TVDstep = simpleTVD), reading and writing the same .ics and .plas files.
 simulation() keeps RunCUDA.simulation's file and archive flow around it.
 """
 import ast
+import numbers
 import os
 import re
 import shutil as sh
@@ -39,86 +40,182 @@ def convertCUDAToState(filename, N, dim, time=None):
 
 
 # ------------------------------------------------------------ headers
-# (RunCUDA.py:24-100): a dict of #define names; a value of "" is a bare flag.
+# The executable is configured at compile time by a header of #define lines
+# (plasticity.h and the #ifdef blocks of plasticity.cu); RunCUDA.simulation
+# writes one from a dict (RunCUDA.py:24-100, 195-205).  Each value is a C
+# literal: a number as Python prints it, a string in double quotes, a
+# sequence in braces ({{0, 0, 1}, ...}), and a bare switch as the empty
+# string "".  HEADER_KEYS lists the keys the executable reads and the kind
+# of value each takes (the reader parses by the literal's own form, so
+# headers with other keys read too).
+FLAG = ""
+HEADER_KEYS = {
+    "N": "int",                   # grid side
+    "DIMENSION3": "flag",         # 3D grid (else 2D)
+    "CFLsafeFactor": "number",
+    "lambda": "number",           # climb mobility switch: 0 'gcd', 1 'mdp'
+    "NEWGLIDEONLY": "flag",       # 'lvp' glide-only velocity
+    "VACANCIES": "flag",
+    "DYNAMIC_NUCLEATION": "int",
+    "LOADING": "flag",
+    "LOAD_DEF": "matrix",         # 3x3 strain direction of the load
+    "LOADING_RATE": "number",
+    "LOAD_START": "number",
+    "FILE_INPUT": "string",       # sprintf patterns of (N, seed, lambda)
+    "FILE_OUTPUT": "string",
+    "TIME_STEP": "number",        # record spacing
+    "TIME_FINAL": "number",
+}
+# the CFL factor RunCUDA compiles every run with (RunCUDA.py:54-58)
+EXECUTABLE_CFL = 0.5
+# the switches each dynamics method of RunCUDA compiles to; 'slip' also
+# takes the slip system's own keys (the reference marks 'slip' and 'vac' as
+# incomplete, and the executable here refuses VACANCIES)
+METHOD_DEFINES = {
+    "gcd": {"lambda": 0},
+    "mdp": {"lambda": 1},
+    "lvp": {"lambda": 0, "NEWGLIDEONLY": FLAG},
+    "slip": {"lambda": 0},
+    "vac": {"VACANCIES": FLAG},
+}
+
+
+def c_literal(value):
+    """The C text of a header value."""
+    if isinstance(value, str):
+        return '"%s"' % value.replace("\\", "\\\\").replace('"', '\\"')
+    if isinstance(value, numbers.Integral):
+        return str(int(value))
+    if isinstance(value, numbers.Real):
+        return repr(float(value))
+    if isinstance(value, np.ndarray):
+        value = value.tolist()
+    if isinstance(value, (list, tuple)):
+        return "{" + ", ".join(c_literal(v) for v in value) + "}"
+    raise TypeError("no C literal for %r" % (value,))
+
+
+def parse_c_literal(text):
+    """Python data of a header value's C text (the inverse of c_literal);
+    text that is no literal (a C expression) is returned as it is."""
+    text = text.strip()
+    if text == "":
+        return FLAG
+    try:
+        value, rest = _parse_literal(text, 0)
+    except (ValueError, IndexError):
+        return text
+    return value if text[rest:].strip() == "" else text
+
+
+def _parse_literal(s, i):
+    while s[i] == " ":
+        i += 1
+    if s[i] == '"':
+        out, i = [], i + 1
+        while s[i] != '"':
+            if s[i] == "\\":
+                i += 1
+            out.append(s[i])
+            i += 1
+        return "".join(out), i + 1
+    if s[i] == "{":
+        items, i = [], i + 1
+        while True:
+            while s[i] == " ":
+                i += 1
+            if s[i] == "}":
+                return items, i + 1
+            v, i = _parse_literal(s, i)
+            items.append(v)
+            while s[i] == " ":
+                i += 1
+            if s[i] == ",":
+                i += 1
+    j = i
+    while j < len(s) and s[j] not in ",} ":
+        j += 1
+    tok = s[i:j]
+    try:
+        return int(tok), j
+    except ValueError:
+        return float(tok), j
+
+
 def write_header(dct, headername):
-    headerfile = open_if_not(headername, "w")
-    for key, val in dct.items():
-        headerfile.write("#define " + key + " " + list_to_crepr(val) + "\n")
-    if headerfile is not headername:
-        headerfile.close()
+    """One '#define KEY VALUE' line per entry, to a path or an open file."""
+    text = "".join("#define %s %s\n" % (key, c_literal(val)) for key, val in dct.items())
+    f = open_if_not(headername, "w")
+    f.write(text)
+    if f is not headername:
+        f.close()
 
 
 def read_header(header):
-    define_statement = re.compile(r"\#define\s([a-zA-Z0-9_\(\),]*)\s?(.*)$")
+    """The dict of a header's #define lines (path or open file), values as
+    Python data; other lines (comments, guards) are skipped."""
+    f = open_if_not(header, "r")
+    try:
+        lines = f.read().splitlines()
+    finally:
+        if f is not header:
+            f.close()
     dct = {}
-    headerfile = open_if_not(header, "r")
-    lines = headerfile.readlines()
-    if headerfile is not header:
-        headerfile.close()
     for line in lines:
-        m = define_statement.match(line.strip())
-        if m is None:
-            print("Not a valid line: ", line.strip())
-            continue
-        var, val = m.groups()
-        dct[var] = crepr_to_list(val)
+        words = line.strip().split(None, 2)
+        if len(words) >= 2 and words[0] == "#define":
+            dct[words[1]] = parse_c_literal(words[2] if len(words) == 3 else "")
     return dct
 
 
-def list_to_crepr(arr):
-    return repr(arr).replace("[", "{").replace("]", "}").replace("'", "\"")
+def run_defines(N=None, dim=None, method=None, nucleation=False, files=None, times=None, load=None, **extra):
+    """The #define set of one run, from the tables above: grid (N, dim),
+    dynamics method, dynamic nucleation, (input, output) file patterns,
+    (record spacing, final time) and load (direction, rate, start)."""
+    d = {}
+    if N is not None:
+        d.update(N=int(N), CFLsafeFactor=EXECUTABLE_CFL)
+        if dim == 3:
+            d["DIMENSION3"] = FLAG
+    if method is not None:
+        d.update(METHOD_DEFINES.get(method, {}))
+        if method == "slip":
+            d.update(extra)
+    if nucleation:
+        d["DYNAMIC_NUCLEATION"] = 0
+    if files is not None:
+        d["FILE_INPUT"], d["FILE_OUTPUT"] = files
+    if load is not None:
+        direction, rate, start = load
+        d.update(LOADING=FLAG, LOAD_DEF=np.asarray(direction).tolist(), LOADING_RATE=rate, LOAD_START=start)
+    if times is not None:
+        d["TIME_STEP"], d["TIME_FINAL"] = times
+    return d
 
 
-def crepr_to_list(arr):
-    return arr.replace("{", "[").replace("}", "]").replace("\"", "'")
-
-
+# the reference's header builders (RunCUDA.py:54-100), each adding its part
 def header_size(header, N, dim):
-    header.update({"N": N})
-    header.update({"CFLsafeFactor": 0.5})
-    if dim == 3:
-        header.update({"DIMENSION3": ""})
+    header.update(run_defines(N=N, dim=dim))
 
 
 def header_load(header, direction, rate, start=0.0):
-    header.update({"LOADING": ""})
-    if isinstance(direction, np.ndarray):
-        header.update({"LOAD_DEF": direction.tolist()})
-    else:
-        header.update({"LOAD_DEF": direction})
-    header.update({"LOADING_RATE": rate})
-    header.update({"LOAD_START": start})
+    header.update(run_defines(load=(direction, rate, start)))
 
 
 def header_dynamics(header, dyn, **kwargs):
-    if dyn == "gcd":
-        header.update({"lambda": 0})
-    if dyn == "mdp":
-        header.update({"lambda": 1})
-    if dyn == "lvp":
-        header.update({"lambda": 0})
-        header.update({"NEWGLIDEONLY": ""})
-    if dyn == "slip":
-        header.update({"lambda": 0})
-        header.update(kwargs)
-    if dyn == "vac":
-        header.update({"VACANCIES": ""})
+    header.update(run_defines(method=dyn, **kwargs))
 
 
 def header_dynamic_nucleation(header, dn=True):
-    if dn is True:
-        header.update({"DYNAMIC_NUCLEATION": 0})
+    header.update(run_defines(nucleation=dn is True))
 
 
 def header_files(header, infile, outfile):
-    header.update({"FILE_INPUT": infile})
-    header.update({"FILE_OUTPUT": outfile})
+    header.update(run_defines(files=(infile, outfile)))
 
 
 def header_times(header, step, end):
-    header.update({"TIME_STEP": step})
-    header.update({"TIME_FINAL": end})
-
+    header.update(run_defines(times=(step, end)))
 
 # ------------------------------------------------------------ the executable
 # Compile-time options of the executable that this runner does not carry out.
@@ -238,9 +335,10 @@ def run_header(header, state=None, seed=0, device=0, rk_order=1, directory=".", 
 
 # ------------------------------------------------------------ run archives
 def header_fromfile(filename):
-    from Plasticity.TarFile import read_json, tar_getfile
-    with tarfile.open(filename) as tar:
-        return read_json(tar_getfile(tar, ".json"))
+    """The header dict archived with a run (its .json member)."""
+    from Plasticity.TarFile import RunArchive, read_json
+    with RunArchive(filename) as arch:
+        return read_json(arch.open(".json"))
 
 
 def local_get(store, dir, file):
@@ -269,7 +367,7 @@ def simulation(dct, get_file=local_get, put_file=local_put, delete_intermediates
     and its LOAD_START is taken from the configuration's load_start or else
     from the previous archive's header, and stored as load_start in this
     run's configuration so that loading runs chain."""
-    from Plasticity.TarFile import LoadTarJSON, LoadTarState, tar_extract, tar_open
+    from Plasticity.TarFile import LoadTarJSON, LoadTarState, RunArchive
     conf = Configuration(dct)
     homedir = conf.homedir
     N, dim = conf.N, conf.dim
@@ -311,10 +409,8 @@ def simulation(dct, get_file=local_get, put_file=local_put, delete_intermediates
     if os.path.isfile(currfile) and resume:
         # resume from the archived records: the output exists, so the run
         # continues from its last record (CONTINUE_RUN)
-        with tar_open(currfile) as tar:
-            tar_extract(tar, ".plas")
-        sh.move(os.path.join(currstub, file_output), file_output)
-        sh.rmtree(currstub)
+        with RunArchive(currfile) as arch:
+            arch.copy_to(".plas", file_output)
         file_input = file_output
     else:
         if previous == "":

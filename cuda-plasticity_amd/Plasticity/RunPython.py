@@ -155,7 +155,7 @@ def simulation(dct, get_file=local_get, put_file=local_put, end_time=30., verbos
     left out too, so the records stay in the working directory.  Returns
     the name of the record file."""
     from Plasticity.PlasticityStates.DisorderState import DisorderState
-    from Plasticity.TarFile import LoadTarState, tar_extract, tar_open
+    from Plasticity.TarFile import LoadTarState, RunArchive
     conf = Configuration(dct)
     homedir, N, dim = conf.homedir, conf.N, conf.dim
     previous, postfix, method, seed = conf.previous, conf.postfix, conf.method, conf.seed
@@ -174,10 +174,8 @@ def simulation(dct, get_file=local_get, put_file=local_put, end_time=30., verbos
     if os.path.isfile(currfile):
         if method.endswith('x'):
             raise NotImplementedError("resuming a DisorderState run ('x') from its records")
-        with tar_open(currfile) as tar:
-            tar_extract(tar, ".plas")
-        sh.move(os.path.join(currstub, file_output), file_output)
-        sh.rmtree(currstub)
+        with RunArchive(currfile) as arch:
+            arch.copy_to(".plas", file_output)
         start, state = FieldInitializer.LoadStateRawUntransposed(file_output, N, dim)
     else:
         if previous == "":

@@ -129,3 +129,29 @@ def test_run_archive(tmp_path):
     assert t == 1.0 and np.array_equal(s.GetOrderParameterField().numpy(), recs[1][1])
     t, s = TarFile.LoadTarState(tarname, time=0.2, device=CPU)
     assert t == 0.5 and np.array_equal(s.GetOrderParameterField().numpy(), recs[0][1])
+
+
+def test_run_archive_refuses_unsafe_member_names(tmp_path):
+    """A crafted archive whose record member climbs out of the target
+    directory is refused before anything is written."""
+    import io
+    import tarfile
+    from Plasticity.TarFile import RunArchive, tar_extract
+    path = tmp_path / "evil.tar"
+    with tarfile.open(path, "w") as tar:
+        for name, data in (("../evil.plas", b"x" * 16), ("run/run.conf", b'{"N": 4, "dim": 2}')):
+            info = tarfile.TarInfo(name)
+            info.size = len(data)
+            tar.addfile(info, io.BytesIO(data))
+    dest = tmp_path / "sub"
+    dest.mkdir()
+    with tarfile.open(path) as tar:
+        with pytest.raises(ValueError, match="unsafe"):
+            tar_extract(tar, ".plas", path=str(dest))
+    assert not (tmp_path / "evil.plas").exists()
+    with RunArchive(str(path)) as arch:
+        with pytest.raises(ValueError):
+            arch.copy_to(".plas", str(dest / "x.plas"))
+        with pytest.raises(KeyError, match=".ics"):
+            arch.member(".ics")
+        assert arch.configuration() == {"N": 4, "dim": 2}

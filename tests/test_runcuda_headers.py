@@ -86,3 +86,44 @@ def test_unsupported_options_raise():
     h["LOAD_DEF"] = [[0, 1, 0], [0, 0, 0], [0, 0, 0]]
     with pytest.raises(NotImplementedError):
         R.header_options(h)
+
+
+def test_header_reads_reference_style_text():
+    """A header as the reference's writer prints it (repr-based C text, keys
+    in any order, a stray non-define line) reads back as data."""
+    text = ('#define blank "nothing"\n#define N 64\n#define CFLsafeFactor 0.5\n#define DIMENSION3 ""\n'
+            '#define lambda 1\n#define LOADING ""\n#define LOAD_DEF {{0, 0, 0}, {0, 0, 0}, {0, 0, 1}}\n'
+            '#define LOADING_RATE 0.05\n#define LOAD_START 0.0\n#define FILE_INPUT "in_%d.ics"\n'
+            '#define FILE_OUTPUT "out_%d_%d_%f.plas"\n#define TIME_STEP 2.0\n#define TIME_FINAL 50.0\n'
+            '#define DYNAMIC_NUCLEATION 0\n// a comment\n')
+    h = R.read_header(io.StringIO(text))
+    assert h["N"] == 64 and h["lambda"] == 1 and h["DIMENSION3"] == "" and h["LOADING"] == ""
+    assert h["LOAD_DEF"] == [[0, 0, 0], [0, 0, 0], [0, 0, 1]] and h["LOADING_RATE"] == 0.05
+    assert h["FILE_OUTPUT"] == "out_%d_%d_%f.plas" and h["DYNAMIC_NUCLEATION"] == 0
+    assert len(h) == 14
+    # and writes back to the same lines
+    buf = io.StringIO()
+    R.write_header(h, buf)
+    assert sorted(buf.getvalue().splitlines()) == sorted(text.splitlines()[:-1])
+
+
+def test_c_literals():
+    assert R.c_literal('a "q" \\ b') == '"a \\"q\\" \\\\ b"'
+    assert R.parse_c_literal(R.c_literal('a "q" \\ b')) == 'a "q" \\ b'
+    assert R.c_literal(np.float64(0.25)) == "0.25" and R.c_literal(np.int64(3)) == "3"
+    assert R.c_literal(np.eye(2)) == "{{1.0, 0.0}, {0.0, 1.0}}"
+    assert R.parse_c_literal("{ {1, -2.5e-3} ,{}}") == [[1, -0.0025], []]
+    assert R.parse_c_literal("(N*2)") == "(N*2)"
+    with pytest.raises(TypeError):
+        R.c_literal(object())
+
+
+def test_run_defines_tables():
+    h = R.run_defines(N=16, dim=2, method="lvp", files=("a.ics", "b.plas"), times=(1.0, 5.0),
+                      load=([[0, 0, 0], [0, 0, 0], [0, 0, 1]], 0.1, 0.2))
+    assert "DIMENSION3" not in h and h["NEWGLIDEONLY"] == "" and h["lambda"] == 0
+    assert (h["FILE_INPUT"], h["FILE_OUTPUT"], h["TIME_STEP"], h["TIME_FINAL"]) == ("a.ics", "b.plas", 1.0, 5.0)
+    assert h["LOADING"] == "" and (h["LOADING_RATE"], h["LOAD_START"]) == (0.1, 0.2)
+    assert set(h) <= set(R.HEADER_KEYS)
+    assert R.run_defines(method="slip", SLIP_PLANE=3) == {"lambda": 0, "SLIP_PLANE": 3}
+    assert R.run_defines(method="gcd", SLIP_PLANE=3) == {"lambda": 0}
