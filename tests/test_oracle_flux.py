@@ -264,3 +264,55 @@ def test_wojzz_rotation_equivariance():
     r = dyn.flux(0.0, beta)[0]
     rr = dyn.flux(0.0, rot(beta))[0]
     np.testing.assert_allclose(rr, rot(r), rtol=0, atol=1e-12 * np.abs(r).max())
+
+
+@pytest.mark.parametrize("d,n", [(2, 32), (3, 12)])
+@pytest.mark.parametrize("dyn,lam", [(O.GLIDE_CLIMB, 0.9), (O.GLIDE_CLIMB, 0.0), (O.GLIDE_ONLY, 0.0)])
+def test_one_signed_speeds_give_the_upwind_hamiltonian(d, n, dyn, lam):
+    """Where every corner velocity has one sign along each axis, the
+    central-upwind combination is the pure upwind scheme: rhs = -H at the
+    corner of the upwind one-sided derivatives (u^- where the speed is
+    positive, u^+ where it is negative), up to the ME regularisers of the
+    speed totals, and the numerical-diffusion terms vanish
+    (CentralUpwindHJ3D.py:99-188 with a_m = 0 or a_p = 0).  Exchanging the
+    corner weights of an axis (H_+ with a_p instead of a_m) gives -H at the
+    downwind corner instead, which this fails."""
+    beta = SpectralGaussianField((n,) * d, 1.5, 0.01, seed=41)
+    sig = O.sigma(beta, 1.0, 0.3)
+    sig[O.cidx(0, 1)] += 0.004
+    sig[O.cidx(1, 0)] += 0.004
+    sig[O.cidx(d - 1, d - 1)] -= 0.003
+    rhs, _ = O.central_upwind_flux(beta, sig, dyn, lam)
+    ders = [O.find_derivatives(beta, a) for a in range(d)]
+    corners = [tuple(c) for c in np.ndindex(*(2,) * d)]  # 0: '+', 1: '-'
+    V = {c: O._velocity(dyn, [ders[a][c[a]] for a in range(d)], sig, lam) for c in corners}
+    H = {c: O._hamiltonian(dyn, [ders[a][c[a]] for a in range(d)], V[c], lam) for c in corners}
+    vmax = [np.max([V[c][a] for c in corners], axis=0) for a in range(d)]
+    vmin = [np.min([V[c][a] for c in corners], axis=0) for a in range(d)]
+    # cells with one-signed speeds on every axis, well above ME
+    floor = 1e-6 * max(np.abs(v).max() for v in vmax + vmin)
+    ok = np.ones(beta.shape[1:], bool)
+    up = []
+    for a in range(d):
+        pos, neg = vmin[a] >= floor, vmax[a] <= -floor
+        ok &= pos | neg
+        up.append(np.where(pos, 1, 0))  # positive speed: u^- corner
+    assert ok.sum() >= 0.2 * ok.size, ok.sum()
+    want = np.zeros_like(rhs)
+    for c in corners:
+        sel = ok.copy()
+        for a in range(d):
+            sel &= up[a] == c[a]
+        want[:, sel] = -H[c][:, sel]
+    scale = np.abs(want[:, ok]).max()
+    err = np.abs(rhs[:, ok] - want[:, ok]).max()
+    assert err <= 1e-8 * scale, (err, scale)
+    # and the downwind corner (a weight exchange on any axis) is far off
+    for a in range(d):
+        wrong = np.zeros_like(rhs)
+        for c in corners:
+            sel = ok.copy()
+            for b in range(d):
+                sel &= (up[b] == c[b]) if b != a else (up[b] != c[b])
+            wrong[:, sel] = -H[c][:, sel]
+        assert np.abs(rhs[:, ok] - wrong[:, ok]).max() > 1e-3 * scale
