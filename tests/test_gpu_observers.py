@@ -106,14 +106,13 @@ def test_mover_raises_on_non_finite_state(L):
     and Run does not return as if the interval had been covered."""
     from Plasticity.PlasticitySystem import PlasticitySystem
     from Plasticity.PlasticityStates.PlasticityState import PlasticityState
-    from Plasticity.Fields.Fields import TensorField
     from Plasticity.FieldMovers.FieldMover import TVDRungeKutta_FieldMover
     from Plasticity.FieldDynamics.CentralUpwindHJBetaPGlideOnlyDynamics import BetaPDynamics
     say("mover nan guard")
     n = 16
     beta = SpectralGaussianField((n, n), 2.0, 0.01, seed=27)
     beta[4, 7, 2] = np.nan
-    state = PlasticityState((n, n), field=TensorField((n, n), data=beta), mu=1.0, nu=0.3)
+    state = PlasticityState((n, n), field=beta, mu=1.0, nu=0.3)
     seen = []
 
     class Count:
