@@ -16,13 +16,21 @@ from Plasticity.amd import lib as L
 from Plasticity.FieldDynamics._gpu_betap import GpuBetaPDynamics, GpuGlideOnlyDynamics
 
 
+# RunPython's module constants, which its ExternalStrain applies whatever the
+# state's elastic constants (Plasticity/RunPython.py:19-26)
+RUNPYTHON_MU, RUNPYTHON_NU = 0.5, 0.3
+
+
 def _load3(kind, initial, rate, initT):
     init = np.zeros(9)
     rt = np.zeros(9)
     init[:3] = np.asarray(initial, float)
     rt[:3] = np.asarray(rate, float)
-    return dict(load_kind=L.LOAD_STRAIN if kind == 'strain' else L.LOAD_STRESS,
-                load_init=tuple(init), load_rate=tuple(rt), load_t0=initT)
+    kw = dict(load_kind=L.LOAD_STRAIN if kind == 'strain' else L.LOAD_STRESS,
+              load_init=tuple(init), load_rate=tuple(rt), load_t0=initT)
+    if kind == 'strain':
+        kw.update(load_mu=RUNPYTHON_MU, load_nu=RUNPYTHON_NU)
+    return kw
 
 
 class UpwindLoadingBetaPDynamics(GpuBetaPDynamics):

@@ -24,7 +24,8 @@ class GpuBetaPDynamics(FieldDynamics):
     dynamics_kind = L.GLIDE_CLIMB
 
     def __init__(self, Dx=None, Lambda=0, coreEnergy=0, coreEnergyLog=0, load_kind=L.LOAD_NONE,
-                 load_init=None, load_rate=None, load_t0=0.0, load_init_strain=0.0, load_uniaxial_rate=0.0):
+                 load_init=None, load_rate=None, load_t0=0.0, load_init_strain=0.0, load_uniaxial_rate=0.0,
+                 load_mu=0.0, load_nu=0.0):
         if Dx is not None:
             raise NotImplementedError("Dx = 1/N_last (the reference default) is the only spacing on the GPU path")
         if coreEnergy or coreEnergyLog:
@@ -35,7 +36,8 @@ class GpuBetaPDynamics(FieldDynamics):
         self.coreEnergyLog = coreEnergyLog
         self.load = dict(load_kind=load_kind, load_init=tuple(load_init or (0.,) * 9),
                          load_rate=tuple(load_rate or (0.,) * 9), load_t0=load_t0,
-                         load_init_strain=load_init_strain, load_uniaxial_rate=load_uniaxial_rate)
+                         load_init_strain=load_init_strain, load_uniaxial_rate=load_uniaxial_rate,
+                         load_mu=float(load_mu), load_nu=float(load_nu))
 
     def _check_overrides(self):
         if type(self).GetSigma is not GpuBetaPDynamics.GetSigma:

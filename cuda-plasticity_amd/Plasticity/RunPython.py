@@ -48,8 +48,12 @@ def _load_kwargs(rate, type, initial, initT):
     if rate is None:
         return {}
     kind = L.LOAD_STRAIN if type == 'strain' else L.LOAD_STRESS
-    return dict(load_kind=kind, load_init=tuple(np.asarray(initial, float).ravel()[:3]),
-                load_rate=tuple(np.asarray(rate, float).ravel()[:3]), load_t0=float(initT))
+    kw = dict(load_kind=kind, load_init=tuple(np.asarray(initial, float).ravel()[:3]),
+              load_rate=tuple(np.asarray(rate, float).ravel()[:3]), load_t0=float(initT))
+    if kind == L.LOAD_STRAIN:
+        # ExternalStrain's Hooke's law takes this module's mu, nu, not the state's
+        kw.update(load_mu=mu, load_nu=nu)
+    return kw
 
 
 class UpwindLoadingBetaPDynamics(CentralUpwindHJBetaPDynamics.BetaPDynamics):

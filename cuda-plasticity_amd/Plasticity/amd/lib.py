@@ -43,6 +43,8 @@ class Config(ctypes.Structure):
         ("load_uniaxial_rate", ctypes.c_double),
         ("precision", ctypes.c_int),
         ("scheme", ctypes.c_int),
+        ("load_mu", ctypes.c_double),
+        ("load_nu", ctypes.c_double),
     ]
 
 
@@ -177,7 +179,8 @@ class Plan:
 
     def __init__(self, ndim, n, dynamics=GLIDE_CLIMB, rk_order=3, Lambda=0.0, mu=1.0, nu=0.3, cfl=0.25,
                  load_kind=LOAD_NONE, load_init=None, load_rate=None, load_t0=0.0, load_init_strain=0.0,
-                 load_uniaxial_rate=0.0, device=0, precision="fp64", scheme="central-upwind"):
+                 load_uniaxial_rate=0.0, device=0, precision="fp64", scheme="central-upwind", load_mu=0.0,
+                 load_nu=0.0):
         import torch
         self.torch = torch
         if precision not in ("fp64", "fp32"):
@@ -205,6 +208,8 @@ class Plan:
         cfg.load_t0 = load_t0
         cfg.load_init_strain = load_init_strain
         cfg.load_uniaxial_rate = load_uniaxial_rate
+        # the strain load's own elastic constants (0: the plan's mu, nu)
+        cfg.load_mu, cfg.load_nu = float(load_mu), float(load_nu)
         self.cfg = cfg
         self.ndim, self.n, self.device = ndim, n, device
         self.shape = (n,) * ndim

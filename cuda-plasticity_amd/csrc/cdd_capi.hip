@@ -169,6 +169,8 @@ int cdd_plan_create(const cdd_config* cfg, cdd_plan** out) {
     return fail(CDD_ERR_UNSUPPORTED, "the woJzz dynamics is 2D only");
   if (cfg->load_kind < 0 || cfg->load_kind > 4) return fail(CDD_ERR_ARG, "bad load_kind");
   if (!(cfg->mu > 0) || !(cfg->nu < 0.5) || !(cfg->cfl > 0)) return fail(CDD_ERR_ARG, "bad mu/nu/cfl");
+  if (cfg->load_mu < 0 || (cfg->load_mu > 0 && !(cfg->load_nu < 0.5)))
+    return fail(CDD_ERR_ARG, "bad load_mu/load_nu");
   if (cfg->precision != CDD_FP64 && cfg->precision != CDD_FP32) return fail(CDD_ERR_ARG, "bad precision");
   HIPCHK(hipSetDevice(cfg->device));
   cdd_plan* p = new cdd_plan();
@@ -253,6 +255,12 @@ int cdd_plan_create(const cdd_config* cfg, cdd_plan** out) {
   p->gp.load.t0 = cfg->load_t0;
   p->gp.load.init_strain = cfg->load_init_strain;
   p->gp.load.uniaxial_rate = cfg->load_uniaxial_rate;
+  {
+    const double smu = cfg->load_mu > 0 ? cfg->load_mu : mu;
+    const double snu = cfg->load_mu > 0 ? cfg->load_nu : nu;
+    p->gp.load.smu = smu;
+    p->gp.load.slam = 2.0 * smu * snu / (1.0 - 2.0 * snu);
+  }
   Clock c0;
   memset(&c0, 0, sizeof c0);
   c0.t = 0.0;

@@ -85,6 +85,7 @@ struct LoadParams {
   double t0;
   double init_strain;    // uniaxial: initial <beta_zz>
   double uniaxial_rate;  // uniaxial: strain rate
+  double smu, slam;      // strain load: its own mu and lam (RunPython's module constants)
 };
 
 struct Elastic {
@@ -101,7 +102,7 @@ __device__ __forceinline__ void load_shift(const LoadParams& L, const Elastic& E
     double e[3];
     for (int i = 0; i < 3; ++i) e[i] = L.init[i] + L.rate[i] * (t - L.t0);
     double tr = e[0] + e[1] + e[2];
-    for (int i = 0; i < 3; ++i) s[4 * i] = E.lam * tr + 2.0 * E.mu * e[i];
+    for (int i = 0; i < 3; ++i) s[4 * i] = L.slam * tr + 2.0 * L.smu * e[i];
   } else if (L.kind == 2) {  // RunPython.py:28-32
     for (int i = 0; i < 3; ++i) s[4 * i] = L.init[i] + L.rate[i] * (t - L.t0);
   } else if (L.kind == 3) {  // MattTools/Run/RunLoadingMPI.py:68-83

@@ -95,6 +95,11 @@ typedef struct cdd_config {
   int precision;     /* CDD_FP64 (default, 0) or CDD_FP32 */
   int scheme;        /* CDD_SCHEME_CENTRAL_UPWIND (default, 0) or CDD_SCHEME_LLF (float64; glide+climb
                         or glide-only; no batching, vacancies or adjoint) */
+  double load_mu;    /* elastic constants of the CDD_LOAD_STRAIN stress (lam tr e + 2 mu e); 0: the plan's
+                        mu, nu.  RunPython's ExternalStrain uses its module constants mu = 0.5, nu = 0.3
+                        whatever the state's (Plasticity/RunPython.py:19-26), as does
+                        PlottingTools.StrainStress (Plasticity/PlottingTools.py:73-74) */
+  double load_nu;
 } cdd_config;
 
 typedef struct cdd_plan cdd_plan;

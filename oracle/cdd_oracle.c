@@ -42,6 +42,7 @@ typedef struct {
   double Lambda, mu, nu, cfl;
   int load_kind; /* 0 none, 1 strain, 2 stress, 3 uniaxial, 4 tensor */
   double init[9], rate[9], t0, init_strain, uniaxial_rate;
+  double load_mu, load_nu; /* the strain load's elastic constants (0: mu, nu) */
 } oc_cfg;
 
 /* ------------------------------------------------------------ FFT */
@@ -215,7 +216,10 @@ void oc_sigma(const oc_cfg* c, const double* beta, double t, double* sig) {
     double e[3];
     for (int i = 0; i < 3; ++i) e[i] = c->init[i] + c->rate[i] * (t - c->t0);
     const double tr = e[0] + e[1] + e[2];
-    for (int i = 0; i < 3; ++i) shift[CI(i, i)] = lamb * tr + 2.0 * mu * e[i];
+    const double smu = c->load_mu > 0 ? c->load_mu : mu;
+    const double snu = c->load_mu > 0 ? c->load_nu : c->nu;
+    const double slam = 2.0 * smu * snu / (1.0 - 2.0 * snu);
+    for (int i = 0; i < 3; ++i) shift[CI(i, i)] = slam * tr + 2.0 * smu * e[i];
   } else if (c->load_kind == 2) {
     for (int i = 0; i < 3; ++i) shift[CI(i, i)] = c->init[i] + c->rate[i] * (t - c->t0);
   } else if (c->load_kind == 3) {

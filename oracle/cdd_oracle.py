@@ -601,11 +601,16 @@ class Load:
     'uniaxial' -- RunLoadingMPI.py:68-83 (betaE_zz = rate t - (<beta_zz> -
     init)); 'tensor' -- sigma_ij += s0_ij + rate_ij (t - t0) (the shear of
     the adjoint workload, no reference counterpart).
+    elastic: (mu, nu) of the 'strain' load's Hooke's law; RunPython's
+    ExternalStrain uses its module constants (0.5, 0.3) whatever the state's
+    (RunPython.py:19-26); None: the dynamics' own mu, nu.  The uniaxial
+    control uses the state's (RunLoadingMPI.py:76-77).
     """
 
     def __init__(self, kind="none", initial=(0., 0., 0.), rate=(0., 0., 0.), t0=0.0,
-                 init_strain=0.0, uniaxial_rate=0.0, tensor0=None, tensor_rate=None):
+                 init_strain=0.0, uniaxial_rate=0.0, tensor0=None, tensor_rate=None, elastic=None):
         self.kind = kind
+        self.elastic = None if elastic is None else (float(elastic[0]), float(elastic[1]))
         self.initial = np.asarray(initial, dtype=float)
         self.rate = np.asarray(rate, dtype=float)
         self.t0 = t0
@@ -620,10 +625,12 @@ class Load:
             return sig
         sig = sig.copy()
         if self.kind == "strain":
+            smu, snu = self.elastic if self.elastic is not None else (mu, nu)
+            slamb = lame_lambda(smu, snu)
             e = self.initial + self.rate * (t - self.t0)
             tr = e[0] + e[1] + e[2]
             for i in range(3):
-                sig[cidx(i, i)] += lamb * tr + 2.0 * mu * e[i]
+                sig[cidx(i, i)] += slamb * tr + 2.0 * smu * e[i]
         elif self.kind == "stress":
             s = self.initial + self.rate * (t - self.t0)
             for i in range(3):

@@ -32,7 +32,7 @@ class _Cfg(ctypes.Structure):
                 ("Lambda", ctypes.c_double), ("mu", ctypes.c_double), ("nu", ctypes.c_double),
                 ("cfl", ctypes.c_double), ("load_kind", ctypes.c_int), ("init", ctypes.c_double * 9),
                 ("rate", ctypes.c_double * 9), ("t0", ctypes.c_double), ("init_strain", ctypes.c_double),
-                ("uniaxial_rate", ctypes.c_double)]
+                ("uniaxial_rate", ctypes.c_double), ("load_mu", ctypes.c_double), ("load_nu", ctypes.c_double)]
 
 
 _lib = None
@@ -63,6 +63,8 @@ def _cfg(beta, dynamics, Lambda, mu, nu, cfl, order, load):
         if load.kind == "strain":
             c.load_kind = 1
             c.init[:3], c.rate[:3], c.t0 = list(load.initial), list(load.rate), load.t0
+            if load.elastic is not None:
+                c.load_mu, c.load_nu = load.elastic
         elif load.kind == "stress":
             c.load_kind = 2
             c.init[:3], c.rate[:3], c.t0 = list(load.initial), list(load.rate), load.t0

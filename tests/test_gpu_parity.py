@@ -153,6 +153,11 @@ LOADS = {
     "strain": (dict(load_kind=1, load_init=(0.001, -0.002, 0.003) + (0,) * 6, load_rate=(-0.005, -0.005, 0.01) + (0,) * 6,
                     load_t0=0.1),
                dict(kind="strain", initial=(0.001, -0.002, 0.003), rate=(-0.005, -0.005, 0.01), t0=0.1)),
+    # ExternalStrain's own Hooke's law (RunPython's mu = 0.5, nu = 0.3) on a mu = 1 plan
+    "strain_runpython": (dict(load_kind=1, load_init=(0.001, -0.002, 0.003) + (0,) * 6,
+                              load_rate=(-0.005, -0.005, 0.01) + (0,) * 6, load_t0=0.1, load_mu=0.5, load_nu=0.3),
+                         dict(kind="strain", initial=(0.001, -0.002, 0.003), rate=(-0.005, -0.005, 0.01), t0=0.1,
+                              elastic=(0.5, 0.3))),
     "stress": (dict(load_kind=2, load_init=(0.002, 0.0, -0.001) + (0,) * 6, load_rate=(0.0, 0.01, 0.02) + (0,) * 6),
                dict(kind="stress", initial=(0.002, 0.0, -0.001), rate=(0.0, 0.01, 0.02))),
     "uniaxial": (dict(load_kind=3, load_init_strain=0.001, load_uniaxial_rate=0.05),
@@ -287,3 +292,34 @@ def test_wojzz_reference_api(amd):
     sysm.Run(startTime=0.0, endTime=t1)
     ref, _, _ = O.run(O.Dynamics(O.GLIDE_CLIMB_WOJZZ, 0.5, MU, NU), beta, 0.0, 3, 0.25, t_end=t1)
     assert rel_l2(rec.state.GetOrderParameterField().numpy(), ref) < 1e-11
+
+
+def test_runpython_strain_loading_uses_its_own_elastic_constants(amd):
+    """UpwindLoadingBetaPDynamics (RunPython.py:35-49) on a mu = 1 state: the
+    external strain's stress takes RunPython's module constants mu = 0.5,
+    nu = 0.3 (RunPython.py:19-26), not the state's; the state's stress solve
+    keeps mu = 1."""
+    from Plasticity.RunPython import UpwindLoadingBetaPDynamics
+    from Plasticity.PlasticityStates.PlasticityState import PlasticityState
+    from Plasticity.FieldMovers import FieldMover
+    from Plasticity.PlasticitySystem import PlasticitySystem
+    from Plasticity.Observers.Observer import RecallStateObserver
+    say("runpython strain load at mu = 1")
+    n = 16
+    beta = SpectralGaussianField((n,) * 3, 2.0, 0.01, seed=19)
+    state = PlasticityState((n,) * 3, field=beta, mu=MU, nu=NU)
+    dyn = UpwindLoadingBetaPDynamics(rate=np.array([-0.005, -0.005, 0.01]), initial=np.array([0.001, 0.0, -0.002]))
+    obs = RecallStateObserver()
+    mover = FieldMover.TVDRungeKutta_FieldMover(CFLsafeFactor=0.25)
+    t_end = 0.02
+    PlasticitySystem((n,) * 3, state, mover, dyn, [obs]).Run(startTime=0.0, endTime=t_end)
+    load = O.Load(kind="strain", initial=(0.001, 0.0, -0.002), rate=(-0.005, -0.005, 0.01), elastic=(0.5, 0.3))
+    ref, tref, dts = O.run(O.Dynamics(O.GLIDE_CLIMB, 0.0, MU, NU, load=load), beta, 0.0, 10 ** 6, 0.25, t_end=t_end)
+    assert mover.numberOfSteps == len(dts)
+    assert rel_l2(obs.state.betaP.numpy(), ref) < 1e-11
+    # the flux at the state: against the state's own constants it would differ
+    rhs = dyn.CalculateFlux(0.01, state).numpy()
+    want = O.Dynamics(O.GLIDE_CLIMB, 0.0, MU, NU, load=load).flux(0.01, beta)[0]
+    wrong = O.Dynamics(O.GLIDE_CLIMB, 0.0, MU, NU, load=O.Load(kind="strain", initial=(0.001, 0.0, -0.002),
+                                                                 rate=(-0.005, -0.005, 0.01))).flux(0.01, beta)[0]
+    assert rel_l2(rhs, want) < 1e-12 and rel_l2(rhs, wrong) > 1e-6

@@ -84,7 +84,7 @@ def test_strain_loaded_run_matches_oracle(R, tmp_path, monkeypatch):
                             load_type="strain"), end_time=0.01, verbose=False)
     got = records(out, 9, 16, 2)
     beta0 = FieldInitializer.GaussianRandomInitializer((16, 16), 0.2, 2).GetOrderParameterField().numpy()
-    load = O.Load(kind="strain", initial=(0.001, 0.0, 0.0), rate=(0.0, 0.0, 0.5))
+    load = O.Load(kind="strain", initial=(0.001, 0.0, 0.0), rate=(0.0, 0.0, 0.5), elastic=(0.5, 0.3))
     want = oracle_intervals(R, O.Dynamics(O.GLIDE_CLIMB, 0.0, MU, NU, load=load), beta0, 0.01)
     _check(got, want)
 

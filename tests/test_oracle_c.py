@@ -32,6 +32,7 @@ def test_flux_matches_numpy_oracle(shape, dyn, Lam):
 
 
 LOADS = [O.Load(), O.Load("strain", initial=(0.001, 0.0, -0.001), rate=(-0.005, -0.005, 0.01), t0=0.1),
+         O.Load("strain", initial=(0.001, 0.0, -0.001), rate=(-0.005, -0.005, 0.01), t0=0.1, elastic=(0.5, 0.3)),
          O.Load("stress", initial=(0.002, 0.0, 0.0), rate=(0.0, 0.0, 0.01)),
          O.Load("uniaxial", init_strain=0.001, uniaxial_rate=0.05),
          O.Load("tensor", tensor0=(0, 0.005, 0, 0.005, 0, 0, 0, 0, 0))]

@@ -189,6 +189,13 @@ def test_loads_equal_stress_of_uniform_distortion():
     for i in range(3):
         B[O.cidx(i, i)] = -e_t[i]
     np.testing.assert_allclose(got, O.sigma(B, mu, nu), atol=1e-15)
+    # a strain load with its own constants (RunPython's ExternalStrain at
+    # mu = 0.5, nu = 0.3) on a mu = 1 state: the stress of the uniform
+    # distortion in a mu = 0.5 medium, whatever the state's mu
+    own = O.Load("strain", initial=eps, rate=(0.001, 0.0, 0.002), t0=0.5, elastic=(0.5, 0.3))
+    got = own.apply(np.zeros_like(beta), beta, t, mu, nu)
+    np.testing.assert_allclose(got, O.sigma(B, 0.5, 0.3), atol=1e-15)
+    assert np.abs(got - O.sigma(B, mu, nu)).max() > 1e-3
     # uniaxial strain-rate control: beta_E_zz = rate t - (<beta_zz> - init)
     beta = np.zeros((9,) + shape)
     beta[8] = 0.003
