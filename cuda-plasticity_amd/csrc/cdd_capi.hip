@@ -124,6 +124,43 @@ static bool valid_n(int d, int n) {
   return false;
 }
 
+namespace cdd {
+void fill_twiddles(double* h, int n) {
+  for (int e = 0; e < n; ++e) {
+    long double th = 2.0L * 3.14159265358979323846264338327950288L * (long double)e / (long double)n;
+    h[2 * e] = (double)cosl(th);
+    h[2 * e + 1] = (double)(-sinl(th));
+  }
+}
+
+GreenParams green_params(const cdd_config& cfg) {
+  GreenParams g;
+  memset(&g, 0, sizeof g);
+  const double mu = cfg.mu, nu = cfg.nu;
+  const double lam = 2.0 * mu * nu / (1.0 - 2.0 * nu);
+  g.A = lam / (lam + 2.0 * mu);
+  g.C = 2.0 * (lam + mu) / (lam + 2.0 * mu);
+  g.mu = mu;
+  g.lam = lam;
+  const long long cells = (long long)cfg.n * cfg.n * (cfg.ndim == 3 ? cfg.n : 1);
+  g.norm = 1.0 / (double)cells;
+  g.el = Elastic{mu, nu, lam};
+  g.load.kind = cfg.load_kind;
+  for (int c = 0; c < 9; ++c) {
+    g.load.init[c] = cfg.load_init[c];
+    g.load.rate[c] = cfg.load_rate[c];
+  }
+  g.load.t0 = cfg.load_t0;
+  g.load.init_strain = cfg.load_init_strain;
+  g.load.uniaxial_rate = cfg.load_uniaxial_rate;
+  const double smu = cfg.load_mu > 0 ? cfg.load_mu : mu;
+  const double snu = cfg.load_mu > 0 ? cfg.load_nu : nu;
+  g.load.smu = smu;
+  g.load.slam = 2.0 * smu * snu / (1.0 - 2.0 * snu);
+  return g;
+}
+}  // namespace cdd
+
 static int cur_state(cdd_plan* p) {
   // RK2 ping-pongs between state[0] and state[1]; RK3 and Euler update in place
   if (p->cfg.rk_order != 2) return p->cur;
@@ -211,11 +248,7 @@ int cdd_plan_create(const cdd_config* cfg, cdd_plan** out) {
   // twiddle table W_n^e = exp(-2 pi i e / n)
   {
     double* h = new double[2 * n];
-    for (int e = 0; e < n; ++e) {
-      long double th = 2.0L * 3.14159265358979323846264338327950288L * (long double)e / (long double)n;
-      h[2 * e] = (double)cosl(th);
-      h[2 * e + 1] = (double)(-sinl(th));
-    }
+    fill_twiddles(h, n);
     hipError_t e_;
     if (p->fp32) {
       float* hf = new float[2 * n];
@@ -238,29 +271,7 @@ int cdd_plan_create(const cdd_config* cfg, cdd_plan** out) {
   p->stream = p->own;
   HIPCHK(hipMemsetAsync(p->state[0], 0, fb, p->stream));
   HIPCHK(hipMemsetAsync(p->scalar, 0, sizeof(double) * 8, p->stream));
-  // elastic / Green's constants
-  const double mu = cfg->mu, nu = cfg->nu;
-  const double lam = 2.0 * mu * nu / (1.0 - 2.0 * nu);
-  p->gp.A = lam / (lam + 2.0 * mu);
-  p->gp.C = 2.0 * (lam + mu) / (lam + 2.0 * mu);
-  p->gp.mu = mu;
-  p->gp.lam = lam;
-  p->gp.norm = 1.0 / (double)p->cells;
-  p->gp.el = Elastic{mu, nu, lam};
-  p->gp.load.kind = cfg->load_kind;
-  for (int c = 0; c < 9; ++c) {
-    p->gp.load.init[c] = cfg->load_init[c];
-    p->gp.load.rate[c] = cfg->load_rate[c];
-  }
-  p->gp.load.t0 = cfg->load_t0;
-  p->gp.load.init_strain = cfg->load_init_strain;
-  p->gp.load.uniaxial_rate = cfg->load_uniaxial_rate;
-  {
-    const double smu = cfg->load_mu > 0 ? cfg->load_mu : mu;
-    const double snu = cfg->load_mu > 0 ? cfg->load_nu : nu;
-    p->gp.load.smu = smu;
-    p->gp.load.slam = 2.0 * smu * snu / (1.0 - 2.0 * snu);
-  }
+  p->gp = green_params(*cfg);
   Clock c0;
   memset(&c0, 0, sizeof c0);
   c0.t = 0.0;

@@ -3,6 +3,8 @@
 #include "cdd_common.h"
 #include "cdd_fft.h"
 
+struct cdd_config;
+
 namespace cdd {
 
 // A stencil/source field: value = A + dt * B (B optional, dt from clk).
@@ -62,6 +64,8 @@ struct SpectralJob {
 };
 
 hipError_t launch_spectral(const SpectralJob& job);
+// the twiddle table W_n^e = exp(-2 pi i e / n), e < n, as 2 n doubles (host)
+void fill_twiddles(double* h, int n);
 hipError_t launch_sigma_finalize(const double* G, const double* beta, double* sig, long long cells, double mu,
                                  double lam, hipStream_t st, int fp32 = 0);
 
@@ -105,6 +109,8 @@ struct FluxJob {
 };
 
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out);
+// the Green's-tensor, elastic and load constants of a configuration
+GreenParams green_params(const cdd_config& cfg);
 int flux_nblocks(int d, int n);
 hipError_t launch_finalize_dt(const double* partial, int nblocks, Clock* clk, double nmax, hipStream_t st,
                               int count = 1, long long pstride = 0, double dt_bound = 0.0);
