@@ -58,7 +58,7 @@ EXPORTS = [
     "cdd_segment_vjp", "cdd_lambda_bar", "cdd_stress_record", "cdd_plas_append", "cdd_plas_resume",
     "cdd_energy_record", "cdd_status", "cdd_instability", "cdd_segment_store_bytes", "cdd_segment_forward",
     "cdd_segment_vjp_stored", "cdd_set_vacancy", "cdd_vacancy_set", "cdd_vacancy_get",
-    "cdd_set_disorder", "cdd_disorder_get", "cdd_disorder_sigma", "cdd_disorder_flux",
+    "cdd_set_disorder", "cdd_disorder_get", "cdd_disorder_sigma", "cdd_disorder_flux", "cdd_sigma_noalloc",
 ]
 KERNEL_KINDS = ["r2c", "cols_fwd", "contract", "cols_inv", "c2r", "flux_stage1", "cfl_reduce",
                 "flux_stage2", "flux_stage3", "advance"]
@@ -105,6 +105,7 @@ def _load():
         "cdd_get_clock": ([P, dp, dp, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_longlong)], I),
         "cdd_sigma": ([P, P, D, P], I),
         "cdd_rho": ([P, P, P], I),
+        "cdd_sigma_noalloc": ([ctypes.POINTER(Config), P, D, P, P], I),
         "cdd_flux": ([P, P, D, P, P], I),
         "cdd_step": ([P, I], I),
         "cdd_energy": ([P, P, D, dp], I),
@@ -264,6 +265,18 @@ class Plan:
         out = self.empty()
         check(lib.cdd_sigma(self.h, _ptr(b), t, _ptr(out)))
         return out
+
+    def sigma_noalloc(self, beta, out, t=0.0, stream=None):
+        """cdd_sigma_noalloc with this plan's configuration: the stress of beta
+        into `out` (both (9,) + grid float64 device tensors) on `stream`
+        (default: torch's current stream), using the library's static
+        workspace instead of the plan's buffers."""
+        b, o = self._as_dev(beta), out
+        if o.dtype != self.torch.float64 or not o.is_contiguous() or tuple(o.shape) != (9,) + self.shape:
+            raise CDDError("out must be a contiguous float64 (9,) + grid tensor")
+        s = self.torch.cuda.current_stream(self.dev).cuda_stream if stream is None else stream
+        check(lib.cdd_sigma_noalloc(ctypes.byref(self.cfg), _ptr(b), float(t), _ptr(o), ctypes.c_void_p(s)))
+        return o
 
     def rho(self, beta):
         b = self._as_dev(beta)

@@ -20,7 +20,8 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-ffp-contract=fast", "-Wno-unused-result",
 ]
-SOURCES = ["cdd_spectral.hip", "cdd_flux.hip", "cdd_capi.hip", "cdd_batch.hip", "cdd_adjoint.hip", "cdd_io.hip"]
+SOURCES = ["cdd_spectral.hip", "cdd_flux.hip", "cdd_capi.hip", "cdd_batch.hip", "cdd_adjoint.hip", "cdd_io.hip",
+           "cdd_noalloc.hip"]
 
 
 def _compile(src, extra):

@@ -130,6 +130,19 @@ int cdd_get_clock(cdd_plan* plan, double* t, double* last_dt, int* done, long lo
  * GetSigma override, RunPython.py:44-49).  beta_dev, sigma_dev: (9, N^d). */
 int cdd_sigma(cdd_plan* plan, const void* beta_dev, double t, void* sigma_dev);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the stress of cdd_sigma for the configuration cfg
+ * (ndim, n, mu, nu, load, load_mu/load_nu, device; float64 only) without a
+ * plan and without any allocation.  beta_dev, sigma_dev: (9, N^d) float64
+ * device pointers on cfg->device; stream: a hipStream_t of that device
+ * (NULL: the null stream).  The workspace is a static device array of 64 MiB
+ * per GPU, held until the process ends; grids that do not fit are refused
+ * (2D up to 768^2, 3D up to 64^3).  The first call on a device fills the
+ * array's twiddle tables with a synchronous copy: make it before any graph
+ * capture.  Calls that overlap on one GPU share the array and race. */
+int cdd_sigma_noalloc(const cdd_config* cfg, const void* beta_dev, double t, void* sigma_dev, void* stream);
+
 /* rho = PlasticityState.CalculateRhoFourier(type='k') (PlasticityState.py:264-304). */
 int cdd_rho(cdd_plan* plan, const void* beta_dev, void* rho_dev);
 
