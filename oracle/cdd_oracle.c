@@ -91,8 +91,10 @@ static void fft1d(double complex* a, int n, int sign, const double complex* tw) 
   }
 }
 
-/* unnormalised transform of an n^d array along axis ax (sign -1 forward) */
-static void fft_axis(double complex* a, int d, int n, int ax, int sign) {
+/* unnormalised transform of an n^d array along axis ax (sign -1 forward);
+ * half = 1 skips the lines whose last-axis index exceeds n/2 (ax < d - 1),
+ * which the half-spectrum inverse never reads */
+static void fft_axis(double complex* a, int d, int n, int ax, int sign, int half) {
   const long long cells = d == 3 ? (long long)n * n * n : (long long)n * n;
   long long stride = 1;
   for (int b = ax + 1; b < d; ++b) stride *= n;
@@ -104,6 +106,7 @@ static void fft_axis(double complex* a, int d, int n, int ax, int sign) {
 #pragma omp for schedule(static)
     for (long long l = 0; l < lines; ++l) {
       const long long hi = l / stride, lo = l % stride;
+      if (half && lo % n > n / 2) continue;
       const long long base = hi * stride * n + lo;
       for (int i = 0; i < n; ++i) buf[i] = a[base + i * stride];
       fft1d(buf, n, sign, tw);
@@ -116,7 +119,7 @@ static void fft_axis(double complex* a, int d, int n, int ax, int sign) {
 /* numpy.fft.rfftn of a real field (kept as the full spectrum; only the
  * last-axis indices 0..n/2 are used afterwards) */
 static void rfftn_full(double complex* a, int d, int n) {
-  for (int ax = 0; ax < d; ++ax) fft_axis(a, d, n, ax, -1);
+  for (int ax = 0; ax < d; ++ax) fft_axis(a, d, n, ax, -1, 0);
 }
 
 /* numpy.fft.irfftn(s = grid) of the half spectrum stored in the last-axis
@@ -126,7 +129,7 @@ static void rfftn_full(double complex* a, int d, int n) {
  * (pocketfft's c2r).  The real result is left in the real parts. */
 static void irfftn_half(double complex* a, int d, int n) {
   const long long cells = d == 3 ? (long long)n * n * n : (long long)n * n;
-  for (int ax = 0; ax < d - 1; ++ax) fft_axis(a, d, n, ax, +1);
+  for (int ax = 0; ax < d - 1; ++ax) fft_axis(a, d, n, ax, +1, 1);
   const long long lines = cells / n;
   const double s = 1.0 / (double)cells;
   const double complex* tw = twiddles(n);
@@ -159,8 +162,16 @@ void oc_sigma(const oc_cfg* c, const double* beta, double t, double* sig) {
   const int d = c->d, n = c->n;
   const long long cells = d == 3 ? (long long)n * n * n : (long long)n * n;
   const double mu = c->mu, lamb = 2.0 * c->mu * c->nu / (1.0 - 2.0 * c->nu);
-  double complex* kb = (double complex*)malloc(sizeof(double complex) * 9 * cells);
-  double complex* ke = (double complex*)malloc(sizeof(double complex) * 9 * cells);
+  /* spectra kept between calls (the oracle is driven from one host thread) */
+  static double complex *kb = NULL, *ke = NULL;
+  static long long cap = 0;
+  if (cap < cells) {
+    free(kb);
+    free(ke);
+    kb = (double complex*)malloc(sizeof(double complex) * 9 * cells);
+    ke = (double complex*)malloc(sizeof(double complex) * 9 * cells);
+    cap = cells;
+  }
   for (int q = 0; q < 9; ++q) {
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < cells; ++i) kb[q * cells + i] = beta[q * cells + i];
@@ -186,17 +197,23 @@ void oc_sigma(const oc_cfg* c, const double* beta, double t, double* sig) {
       k[1] = n * ident(2.0 * M_PI * iy / (double)n);
       k[2] = n * ident(2.0 * M_PI * iz / (double)n);
     }
+    /* the half spectrum only: irfftn reads last-axis indices 0..n/2 */
+    if ((d == 2 ? iy : iz) > n / 2) continue;
     const double kSq0 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
     const double kSq = kSq0 + ME, kSqSq = kSq0 * kSq0 + ME;
     const double kmask = (ix == 0 && iy == 0 && iz == 0) ? 0.0 : 1.0;
+    /* T_ijmn = A k_i k_j / K2 d_mn + k_i / K2 (k_m d_jn + k_n d_jm) - C k_i k_j k_m k_n / K4,
+     * each factor evaluated in that order (the sub-products that do not
+     * depend on m, n are formed once) */
+    const double A = lamb / (lamb + 2.0 * mu), C = 2.0 * (lamb + mu) / (lamb + 2.0 * mu);
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) {
+        const double t1 = A * k[i] * k[j] / kSq, t2 = k[i] / kSq, kij = k[i] * k[j];
         double complex acc = 0.0;
         for (int m = 0; m < 3; ++m)
           for (int nn = 0; nn < 3; ++nn) {
-            const double T = lamb / (lamb + 2.0 * mu) * k[i] * k[j] / kSq * (double)(m == nn) +
-                             k[i] / kSq * ((k[m] * (double)(j == nn)) + (k[nn] * (double)(j == m))) -
-                             2.0 * (lamb + mu) / (lamb + 2.0 * mu) * (k[i] * k[j] * k[m] * k[nn]) / kSqSq;
+            const double T = t1 * (double)(m == nn) + t2 * ((k[m] * (double)(j == nn)) + (k[nn] * (double)(j == m))) -
+                             C * (kij * k[m] * k[nn]) / kSqSq;
             acc = acc + kb[CI(m, nn) * cells + idx] * T;
           }
         ke[CI(i, j) * cells + idx] = acc * kmask;
@@ -242,8 +259,6 @@ void oc_sigma(const oc_cfg* c, const double* beta, double t, double* sig) {
         sig[CI(a, b) * cells + i] = s;
       }
   }
-  free(kb);
-  free(ke);
 }
 
 /* ------------------------------------------------------------ limiter */
@@ -290,6 +305,26 @@ static void derivs(const double f[5], double Dx, double* dp, double* dm) {
   const double m_im1 = minmod3(d21_im1, d22_im1, d23_im1);
   *dp = diff_0 / Dx - 0.5 * Dx * m_i;
   *dm = diff_m1 / Dx + 0.5 * Dx * m_im1;
+}
+
+/* derivs over n cells whose five stencil values are rows r0..r4 (the same
+ * operations as derivs, in a form the compiler vectorises) */
+static void deriv_rows(const double* restrict r0, const double* restrict r1, const double* restrict r2,
+                       const double* restrict r3, const double* restrict r4, double* restrict dpo,
+                       double* restrict dmo, int n, double Dx) {
+  for (int z = 0; z < n; ++z) {
+    const double diff_m2 = r1[z] - r0[z], diff_m1 = r2[z] - r1[z], diff_0 = r3[z] - r2[z], diff_p1 = r4[z] - r3[z];
+    const double d21_i = 1.0 * (diff_p1 - diff_0) / Dx / Dx;
+    const double d22_i = (diff_p1 - diff_m1) / Dx / Dx * 0.5;
+    const double d21_im1 = 1.0 * (diff_0 - diff_m1) / Dx / Dx;
+    const double d23_i = d21_im1;
+    const double d22_im1 = (diff_0 - diff_m2) / Dx / Dx * 0.5;
+    const double d23_im1 = 1.0 * (diff_m1 - diff_m2) / Dx / Dx;
+    const double m_i = minmod3(d21_i, d22_i, d23_i);
+    const double m_im1 = minmod3(d21_im1, d22_im1, d23_im1);
+    dpo[z] = diff_0 / Dx - 0.5 * Dx * m_i;
+    dmo[z] = diff_m1 / Dx + 0.5 * Dx * m_im1;
+  }
 }
 
 /* ------------------------------------------------------------ velocities */
@@ -414,39 +449,65 @@ static void velocity(const oc_cfg* c, double us[3][9], const double* sg, double 
 }
 
 /* ------------------------------------------------------------ flux */
+/* The limited one-sided derivatives of every component along every axis,
+ * DP/DM[(a 9 + q) cells + idx]: each a function of the cell's five stencil
+ * values only, computed line by line along the contiguous axis so that the
+ * five source rows stream (the values do not depend on the loop order). */
+static void deriv_fields(int d, int n, long long cells, const double* beta, double* DP, double* DM) {
+  const double Dx = 1.0 / n;
+  const long long lines = cells / n;
+  for (int a = 0; a < d; ++a) {
+    const long long str = a == d - 1 ? 1 : (a == 0 ? cells / n : (long long)n);
+#pragma omp parallel for schedule(static) collapse(2)
+    for (int q = 0; q < 9; ++q)
+      for (long long l = 0; l < lines; ++l) {
+        const double* src = beta + q * cells;
+        const long long idx0 = l * n;
+        double* dpo = DP + (long long)(a * 9 + q) * cells + idx0;
+        double* dmo = DM + (long long)(a * 9 + q) * cells + idx0;
+        if (a == d - 1) {
+          const double* row = src + idx0;
+          for (int z = 0; z < n; ++z) {
+            double f[5];
+            for (int k = 0; k < 5; ++k) f[k] = row[(z + k - 2 + n) % n];
+            derivs(f, Dx, &dpo[z], &dmo[z]);
+          }
+        } else {
+          const int co = (int)((idx0 / str) % n);
+          const double* rows[5];
+          for (int k = 0; k < 5; ++k) {
+            const int p = (co + k - 2 + n) % n;
+            rows[k] = src + idx0 + (long long)(p - co) * str;
+          }
+          deriv_rows(rows[0], rows[1], rows[2], rows[3], rows[4], dpo, dmo, n, Dx);
+        }
+      }
+  }
+}
+
 /* central_upwind_flux: rhs (9 planes) and speed (1 plane; may be NULL) */
 void oc_flux(const oc_cfg* c, const double* beta, const double* sig, double* rhs, double* speed) {
   const int d = c->d, n = c->n;
   const long long cells = d == 3 ? (long long)n * n * n : (long long)n * n;
-  const double Dx = 1.0 / n;
   const double lam = c->dyn == 1 ? 0.0 : c->Lambda;
+  /* derivative fields kept between calls (one host thread drives the oracle) */
+  static double* dws = NULL;
+  static long long dcap = 0;
+  if (dcap < 2LL * d * 9 * cells) {
+    free(dws);
+    dws = (double*)malloc(sizeof(double) * 2 * d * 9 * cells);
+    dcap = 2LL * d * 9 * cells;
+  }
+  double* DP = dws;
+  double* DM = dws + (long long)d * 9 * cells;
+  deriv_fields(d, n, cells, beta, DP, DM);
 #pragma omp parallel for schedule(static)
   for (long long idx = 0; idx < cells; ++idx) {
-    int co[3] = {0, 0, 0};
-    long long str[3] = {0, 0, 0};
-    if (d == 2) {
-      co[0] = (int)(idx / n);
-      co[1] = (int)(idx % n);
-      str[0] = n;
-      str[1] = 1;
-    } else {
-      co[0] = (int)(idx / ((long long)n * n));
-      co[1] = (int)((idx / n) % n);
-      co[2] = (int)(idx % n);
-      str[0] = (long long)n * n;
-      str[1] = n;
-      str[2] = 1;
-    }
     double dp[3][9], dm[3][9], sg[9];
     for (int a = 0; a < d; ++a)
       for (int q = 0; q < 9; ++q) {
-        double f[5];
-        for (int k = 0; k < 5; ++k) {
-          int p = co[a] + k - 2;
-          p = p < 0 ? p + n : (p >= n ? p - n : p);
-          f[k] = beta[q * cells + idx + (long long)(p - co[a]) * str[a]];
-        }
-        derivs(f, Dx, &dp[a][q], &dm[a][q]);
+        dp[a][q] = DP[(long long)(a * 9 + q) * cells + idx];
+        dm[a][q] = DM[(long long)(a * 9 + q) * cells + idx];
       }
     for (int q = 0; q < 9; ++q) sg[q] = sig[q * cells + idx];
     double out[9], spd;
