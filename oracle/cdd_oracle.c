@@ -43,6 +43,7 @@ typedef struct {
   int load_kind; /* 0 none, 1 strain, 2 stress, 3 uniaxial, 4 tensor */
   double init[9], rate[9], t0, init_strain, uniaxial_rate;
   double load_mu, load_nu; /* the strain load's elastic constants (0: mu, nu) */
+  double fixed_dt;          /* > 0: CalculateOneFixedTimeStep (FieldMover.py:38-58) */
 } oc_cfg;
 
 /* ------------------------------------------------------------ FFT */
@@ -609,7 +610,7 @@ int oc_run(const oc_cfg* c, double* beta, double t0, int nsteps, double t_end, d
     L_of(&w, t, beta, 1);
     double mx = w.speed[0];
     for (long long i = 1; i < cells; ++i) mx = w.speed[i] > mx ? w.speed[i] : mx;
-    double dt = c->cfl / ((double)n * mx); /* FieldMover.py:94-96 */
+    double dt = c->fixed_dt > 0 ? c->fixed_dt : c->cfl / ((double)n * mx); /* FieldMover.py:94-96 */
     if (t + dt > t_end) dt = t_end - t;
     if (c->order == 3) {
       /* tvd_rk3_step, FieldMover.py:115-135 */

@@ -32,7 +32,8 @@ class _Cfg(ctypes.Structure):
                 ("Lambda", ctypes.c_double), ("mu", ctypes.c_double), ("nu", ctypes.c_double),
                 ("cfl", ctypes.c_double), ("load_kind", ctypes.c_int), ("init", ctypes.c_double * 9),
                 ("rate", ctypes.c_double * 9), ("t0", ctypes.c_double), ("init_strain", ctypes.c_double),
-                ("uniaxial_rate", ctypes.c_double), ("load_mu", ctypes.c_double), ("load_nu", ctypes.c_double)]
+                ("uniaxial_rate", ctypes.c_double), ("load_mu", ctypes.c_double), ("load_nu", ctypes.c_double),
+                ("fixed_dt", ctypes.c_double)]
 
 
 _lib = None
@@ -83,16 +84,28 @@ def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-def run(dyn, beta, t0, nsteps, cfl, order=3, t_end=np.inf):
+def run(dyn, beta, t0, nsteps, cfl, order=3, t_end=np.inf, fixed_dt=None):
     """Same contract as oracle.cdd_oracle.run for a cdd_oracle.Dynamics:
     returns (u, t, dts)."""
     L = _load()
     u = np.ascontiguousarray(beta, dtype=np.float64).copy()
     c = _cfg(u, dyn.dynamics, dyn.Lambda, dyn.mu, dyn.nu, cfl, order, dyn.load)
+    if fixed_dt is not None:
+        c.fixed_dt = float(fixed_dt)
     dts = np.zeros(nsteps)
     t = ctypes.c_double()
     k = L.oc_run(ctypes.byref(c), _dp(u), float(t0), int(nsteps), float(t_end), ctypes.byref(t), _dp(dts))
     return u, t.value, dts[:k]
+
+
+def sigma(dyn, t, beta):
+    """dyn.get_sigma(beta, t): the stress with the dynamics' load."""
+    L = _load()
+    u = np.ascontiguousarray(beta, dtype=np.float64)
+    c = _cfg(u, dyn.dynamics, dyn.Lambda, dyn.mu, dyn.nu, 0.25, 3, dyn.load)
+    sig = np.empty_like(u)
+    L.oc_sigma(ctypes.byref(c), _dp(u), float(t), _dp(sig))
+    return sig
 
 
 def flux(dyn, t, beta):

@@ -100,9 +100,9 @@ def test_3d_256_one_step(env, dynamics, Lambda):
     _compare(env, 256, 1, seed=3, dynamics=dynamics, Lambda=Lambda)
 
 
-@pytest.mark.full_size
+@pytest.mark.timeout(1200)
 def test_config2_3d_128_200_steps(env):
-    """The full 3D workload check: 128^3, 200 RK3 steps (about ten minutes of
-    host CPU for the C oracle on 16 threads)."""
+    """The full 3D workload check: 128^3, 200 RK3 steps (about four minutes
+    of host CPU for the C oracle on 16 threads)."""
     say("config 2: 3D 128^3 x 200 RK3 steps vs C oracle")
     _compare(env, 128, 200)

@@ -66,3 +66,13 @@ def test_end_time_clamp():
     u, t, dts = OC.run(dyn, beta, 0.0, 10, 0.25, t_end=t_end)
     ur, tr, dtr = O.run(dyn, beta, 0.0, 10, 0.25, t_end=t_end)
     assert len(dts) == len(dtr) == 3 and abs(t - t_end) < 1e-15 and rel(u, ur) < 1e-13
+
+
+def test_fixed_step_and_sigma_match_numpy_oracle():
+    beta = SpectralGaussianField((8, 8, 8), 2.0, 0.01, seed=34)
+    dyn = O.Dynamics(O.GLIDE_CLIMB, 0.9, load=O.Load("uniaxial", init_strain=0.001, uniaxial_rate=0.05))
+    u, t, dts = OC.run(dyn, beta, 0.1, 3, 0.25, order=3, fixed_dt=0.004)
+    ur, tr, dtr = O.run(dyn, beta, 0.1, 3, 0.25, order=3, fixed_dt=0.004)
+    assert list(dts) == [0.004] * 3 and list(dtr) == [0.004] * 3
+    assert abs(t - tr) <= 1e-15 and rel(u, ur) < 1e-13
+    assert rel(OC.sigma(dyn, 0.3, beta), dyn.get_sigma(beta, 0.3)) < 1e-13
