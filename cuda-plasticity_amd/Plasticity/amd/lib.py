@@ -59,6 +59,7 @@ EXPORTS = [
     "cdd_energy_record", "cdd_status", "cdd_instability", "cdd_segment_store_bytes", "cdd_segment_forward",
     "cdd_segment_vjp_stored", "cdd_set_vacancy", "cdd_vacancy_set", "cdd_vacancy_get",
     "cdd_set_disorder", "cdd_disorder_get", "cdd_disorder_sigma", "cdd_disorder_flux", "cdd_sigma_noalloc",
+    "cdd_flux_frozen", "cdd_flux_tangent",
 ]
 KERNEL_KINDS = ["r2c", "cols_fwd", "contract", "cols_inv", "c2r", "flux_stage1", "cfl_reduce",
                 "flux_stage2", "flux_stage3", "advance"]
@@ -114,6 +115,8 @@ def _load():
         "cdd_timing_enable": ([P, I], I),
         "cdd_flux_vjp": ([P, P, D, P, P, dp], I),
         "cdd_step_vjp": ([P, P, D, D, P, P, dp], I),
+        "cdd_flux_frozen": ([P, P, D, P, I, P], I),
+        "cdd_flux_tangent": ([P, P, P, D, D, P, P], I),
         "cdd_energy_grad": ([P, P, D, P, dp], I),
         "cdd_dt_record": ([P, P, ctypes.c_longlong], I),
         "cdd_dt_replay": ([P, P, ctypes.c_longlong], I),
@@ -304,6 +307,28 @@ class Plan:
         lam = ctypes.c_double()
         check(lib.cdd_flux_vjp(self.h, _ptr(b), t, _ptr(r), _ptr(out), ctypes.byref(lam)))
         return out, lam.value
+
+    def flux_frozen(self, beta, t, sel, record, out=None):
+        """cdd_flux_frozen: the flux with its minmod and speed branches
+        recorded into (record=True) or replayed from (False) `sel`, an int64
+        device tensor of 2 * cells words."""
+        b = self._as_dev(beta)
+        if (sel.dtype != self.torch.int64 or not sel.is_contiguous() or sel.device != self.dev
+                or sel.numel() != 2 * b[0].numel()):
+            raise CDDError("sel must be a contiguous int64 tensor of 2 * cells words on the plan's device")
+        if out is None:
+            out = self.empty()
+        check(lib.cdd_flux_frozen(self.h, _ptr(b), float(t), ctypes.c_void_p(sel.data_ptr()), 1 if record else 0,
+                                  _ptr(out)))
+        return out
+
+    def flux_tangent(self, beta, dbeta, t=0.0, dLambda=0.0):
+        """cdd_flux_tangent: (rhs, drhs), the flux at beta and its tangent
+        along (dbeta, dLambda)."""
+        b, db = self._as_dev(beta), self._as_dev(dbeta)
+        rhs, drhs = self.empty(), self.empty()
+        check(lib.cdd_flux_tangent(self.h, _ptr(b), _ptr(db), float(t), float(dLambda), _ptr(rhs), _ptr(drhs)))
+        return rhs, drhs
 
     def step_vjp(self, u, t, dt, ubar_next):
         a, g = self._as_dev(u), self._as_dev(ubar_next)

@@ -996,6 +996,298 @@ __global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
   }
 }
 
+// ------------------------------------------------------------------ frozen branches
+// k_flux_branch: the flux of cell_flux_vjp's forward recompute (the same
+// operations as k_flux_vjp_cell, derivatives in reference units) with its
+// branches -- the minmod selection of every limiter site and, per axis, the
+// corners of the largest and smallest velocity and whether they pass zero --
+//   MODE 0: decided at the state (the ordinary flux),
+//   MODE 1: decided and written to job.sel (FrozenJob),
+//   MODE 2: read from job.sel (the branch-frozen map, smooth in beta).
+// V = Dual carries a tangent with every value (forward-mode differentiation:
+// the derivative of each operation at its value, branches decided on the
+// values), so MODE 0 with Dual gives the tangent-linear map of the flux,
+// J_L(beta) dbeta + (dL/dLambda) dLambda, exactly.
+
+// A value and its tangent.
+struct Dual {
+  double v, d;
+};
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a) { return {-a.v, -a.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, double s) { return {a.v * s, a.d * s}; }
+__device__ __forceinline__ Dual operator*(double s, Dual a) { return {a.v * s, a.d * s}; }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+__device__ __forceinline__ Dual& operator+=(Dual& a, Dual b) { return a = a + b; }
+__device__ __forceinline__ Dual& operator-=(Dual& a, Dual b) { return a = a - b; }
+__device__ __forceinline__ Dual& operator*=(Dual& a, Dual b) { return a = a * b; }
+
+__device__ __forceinline__ double val(double x) { return x; }
+__device__ __forceinline__ double val(Dual x) { return x.v; }
+template <typename V>
+__device__ __forceinline__ V cst(double x) {
+  if constexpr (sizeof(V) == sizeof(double)) return x;
+  else return V{x, 0.0};
+}
+// s m + d (s a constant)
+__device__ __forceinline__ double sfma(double s, double m, double d) { return fma(s, m, d); }
+__device__ __forceinline__ Dual sfma(double s, Dual m, Dual d) { return {fma(s, m.v, d.v), s * m.d + d.d}; }
+__device__ __forceinline__ Dual rsqrt_nr(Dual x) {
+  const double y = rsqrt_nr(x.v);
+  return {y, -0.5 * x.d * y * y * y};
+}
+
+template <typename V>
+__device__ __forceinline__ V mm_pick(unsigned s, V a, V b, V c) {
+  return s == 0u ? a : (s == 1u ? b : (s == 2u ? c : cst<V>(0.0)));
+}
+
+template <typename V, int D, int DYN, int TX, int TY, int MODE>
+__global__ void __launch_bounds__(TX* TY) k_flux_branch(FrozenJob job) {
+  constexpr int PW = TX + 4, TILE = PW * (TY + 4), NT = TX * TY, FILL = (TILE + NT - 1) / NT;
+  constexpr int NCOR = 1 << D;
+  constexpr bool TAN = sizeof(V) != sizeof(double);
+  __shared__ double tile[9][TILE];
+  __shared__ double dtile[TAN ? 9 : 1][TAN ? TILE : 1];
+  const int n = job.n;
+  const long long cells = job.cells;
+  int co[3];
+  long long str[3];
+  const long long ii = adj_tile_cell<D>(n, co, str);
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+  const int col0 = co[D - 1] - tx, row0 = co[D - 2] - ty;
+  const long long pbase = (D == 3) ? (long long)co[0] * n * n : 0;
+  const double N = double(n);
+  auto load = [&](int c, long long off) -> V {
+    if constexpr (TAN) return V{job.beta[c * cells + off], job.dbeta[c * cells + off]};
+    else return job.beta[c * cells + off];
+  };
+  V xn[D == 3 ? 9 : 1][4];
+  if constexpr (D == 3) {
+    for (int k = 0; k < 4; ++k) {
+      const long long xo = ii + (long long)(wrapi(co[0] + (k < 2 ? k - 2 : k - 1), n) - co[0]) * str[0];
+      for (int c = 0; c < 9; ++c) xn[c][k] = load(c, xo);
+    }
+  }
+  for (int k = 0; k < FILL; ++k) {
+    const int e = tid + k * NT;
+    if (e < TILE) {
+      const int er = e / PW, ec = e % PW;
+      const long long off = pbase + (long long)wrapi(row0 + er - 2, n) * n + wrapi(col0 + ec - 2, n);
+      for (int c = 0; c < 9; ++c) {
+        tile[c][e] = job.beta[c * cells + off];
+        if constexpr (TAN) dtile[c][e] = job.dbeta[c * cells + off];
+      }
+    }
+  }
+  __syncthreads();
+  auto tv = [&](int c, int e) -> V {
+    if constexpr (TAN) return V{tile[c][e], dtile[c][e]};
+    else return tile[c][e];
+  };
+  // replay: this cell's words and, per axis, the site word of the cell before it
+  unsigned long long own = 0ull, spd = 0ull, prev[3] = {0ull, 0ull, 0ull};
+  if constexpr (MODE == 2) {
+    own = job.sel[ii];
+    spd = job.sel[cells + ii];
+    for (int a = 0; a < D; ++a) prev[a] = job.sel[ii + (long long)(wrapi(co[a] - 1, n) - co[a]) * str[a]];
+  }
+  V u[D][2][9], fc[9];
+  unsigned long long code = 0ull;
+  const int lr = ty + 2, lc = tx + 2;
+  for (int a = 0; a < D; ++a) {
+    for (int c = 0; c < 9; ++c) {
+      V f[5];
+      if (a == D - 1) {
+        for (int k = 0; k < 5; ++k) f[k] = tv(c, lr * PW + lc + k - 2);
+      } else if (a == D - 2) {
+        for (int k = 0; k < 5; ++k) f[k] = tv(c, (lr + k - 2) * PW + lc);
+      } else {
+        f[0] = xn[c][0];
+        f[1] = xn[c][1];
+        f[2] = tv(c, lr * PW + lc);
+        f[3] = xn[c][2];
+        f[4] = xn[c][3 % 4];
+      }
+      if (a == 0) fc[c] = f[2];
+      const V Dm2 = f[1] - f[0], Dm1 = f[2] - f[1], D0 = f[3] - f[2], D1 = f[4] - f[3];
+      const V a1 = D1 - D0, b1 = (D1 - Dm1) * 0.5, c1 = D0 - Dm1;
+      const V a2 = D0 - Dm1, b2 = (D0 - Dm2) * 0.5, c2 = Dm1 - Dm2;
+      unsigned s1, s2;
+      if constexpr (MODE == 2) {
+        const int sh = 2 * (9 * a + c);
+        s1 = (unsigned)(own >> sh) & 3u;
+        s2 = (unsigned)(prev[a] >> sh) & 3u;
+      } else {
+        const int k1 = mm3_sel(val(a1), val(b1), val(c1)), k2 = mm3_sel(val(a2), val(b2), val(c2));
+        s1 = k1 < 0 ? 3u : (unsigned)k1;
+        s2 = k2 < 0 ? 3u : (unsigned)k2;
+        if constexpr (MODE == 1) code |= (unsigned long long)s1 << (2 * (9 * a + c));
+      }
+      const V m_i = mm_pick<V>(s1, a1, b1, c1), m_im1 = mm_pick<V>(s2, a2, b2, c2);
+      u[a][0][c] = sfma(-0.5, m_i, D0) * N;
+      u[a][1][c] = sfma(0.5, m_im1, Dm1) * N;
+    }
+  }
+  V sg[9];
+  {
+    const V tr = fc[0] + fc[4] + fc[8];
+    const double mu = job.mu, lam = job.lam;
+    const int SC[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+    for (int c = 0; c < 9; ++c) {
+      const int i = c / 3, j = c % 3;
+      V g;
+      if constexpr (TAN) g = V{job.G[SC[c] * cells + ii], job.dG[SC[c] * cells + ii]};
+      else g = job.G[SC[c] * cells + ii];
+      sg[c] = g - mu * (fc[c] + fc[3 * j + i]) - (i == j ? lam * tr : cst<V>(0.0));
+    }
+  }
+  const double ME = kME;
+  V Lambda;
+  if constexpr (TAN) Lambda = V{job.Lambda, job.dLambda};
+  else Lambda = job.Lambda;
+  const bool climb_sup = (DYN == DYN_GLIDE_CLIMB) && (job.Lambda != 0.0);
+  const double third = 1.0 / 3.0;
+  const V st = climb_sup ? Lambda * (sg[0] + sg[4] + sg[8]) * third : cst<V>(0.0);
+  const V jfac = -Lambda * third;
+  V w[D][2][3], zz[D][2], trs[D][2], P[D][2];
+  for (int a = 0; a < D; ++a)
+    for (int s = 0; s < 2; ++s) {
+      const V* ua = u[a][s];
+      for (int l = 0; l < 3; ++l)
+        w[a][s][l] = ua[3 * l] * sg[3 * a] + ua[3 * l + 1] * sg[3 * a + 1] + ua[3 * l + 2] * sg[3 * a + 2];
+      V z = cst<V>(0.0);
+      for (int c = 0; c < 9; ++c) z += ua[c] * sg[c];
+      zz[a][s] = z;
+      trs[a][s] = ua[0] + ua[4] + ua[8];
+      V p = cst<V>(0.0);
+      for (int m = 0; m < 3; ++m)
+        if (m != a)
+          for (int jj = 0; jj < 3; ++jj) p += ua[3 * m + jj] * ua[3 * m + jj];
+      P[a][s] = p;
+    }
+  V v[NCOR][3], J[NCOR];
+  for (int c = 0; c < NCOR; ++c) {
+    int sg_[3] = {0, 0, 0};
+    for (int a = 0; a < D; ++a) sg_[a] = (c >> (D - 1 - a)) & 1;
+    V R[3], q[3], v0[3];
+    for (int l = 0; l < 3; ++l) {
+      V r = cst<V>(0.0), vv = cst<V>(0.0);
+      for (int a = 0; a < D; ++a) {
+        r += u[a][sg_[a]][3 * l + a];
+        vv += w[a][sg_[a]][l];
+      }
+      R[l] = r;
+      v0[l] = vv;
+    }
+    for (int a = 0; a < D; ++a) v0[a] -= zz[a][sg_[a]];
+    for (int a = 0; a < 3; ++a) q[a] = (a < D ? trs[a % D][sg_[a]] : cst<V>(0.0)) - R[a];
+    if (DYN == DYN_GLIDE_CLIMB) {
+      if (climb_sup) {
+        for (int a = 0; a < D; ++a) v0[a] += st * trs[a][sg_[a]];
+        for (int l = 0; l < 3; ++l) v0[l] -= st * R[l];
+      }
+    } else {
+      V rrr = cst<V>(ME);
+      for (int a = 0; a < D; ++a) rrr += trs[a][sg_[a]] * trs[a][sg_[a]] - 2.0 * trs[a][sg_[a]] * R[a];
+      for (int l = 0; l < 3; ++l) rrr += R[l] * R[l];
+      const V rvv = cst<V>(ME) + v0[0] * q[0] + v0[1] * q[1] + v0[2] * q[2];
+      const V fpr = rvv / rrr;
+      for (int a = 0; a < 3; ++a) v0[a] += q[a] * fpr;
+    }
+    V r2 = cst<V>(ME);
+    for (int a = 0; a < D; ++a) r2 += P[a][sg_[a]];
+    V cross = cst<V>(0.0);
+    for (int jj = 0; jj < 3; ++jj) {
+      cross += u[0][sg_[0]][3 + jj] * u[1][sg_[1]][jj];
+      if (D == 3) {
+        cross += u[1][sg_[1]][6 + jj] * u[D - 1][sg_[D - 1]][3 + jj];
+        cross += u[D - 1][sg_[D - 1]][jj] * u[0][sg_[0]][6 + jj];
+      }
+    }
+    r2 -= 2.0 * cross;
+    const V rinv = rsqrt_nr(r2);
+    for (int l = 0; l < 3; ++l) v[c][l] = v0[l] * rinv;
+    J[c] = climb_sup ? jfac * (v[c][0] * q[0] + v[c][1] * q[1] + v[c][2] * q[2]) : cst<V>(0.0);
+  }
+  V ap[3], am[3], at[3];
+  unsigned long long spd_out = 0ull;
+  for (int a = 0; a < D; ++a) {
+    int cmax, cmin;
+    bool pos, neg;
+    if constexpr (MODE == 2) {
+      const unsigned b = (unsigned)(spd >> (8 * a)) & 255u;
+      cmax = (int)(b & 7u);
+      cmin = (int)((b >> 3) & 7u);
+      pos = (b >> 6) & 1u;
+      neg = (b >> 7) & 1u;
+    } else {
+      cmax = cmin = 0;
+      double mxv = val(v[0][a]), mnv = val(v[0][a]);
+      for (int c = 1; c < NCOR; ++c) {
+        if (val(v[c][a]) > mxv) {
+          mxv = val(v[c][a]);
+          cmax = c;
+        }
+        if (val(v[c][a]) < mnv) {
+          mnv = val(v[c][a]);
+          cmin = c;
+        }
+      }
+      pos = mxv > 0.0;
+      neg = mnv < 0.0;
+      spd_out |= (unsigned long long)(cmax | (cmin << 3) | ((int)pos << 6) | ((int)neg << 7)) << (8 * a);
+    }
+    ap[a] = pos ? v[cmax][a] : cst<V>(0.0);
+    am[a] = neg ? -v[cmin][a] : cst<V>(0.0);
+    at[a] = ap[a] + am[a] + cst<V>(ME);
+  }
+  V Y[D][2][3], Jsum = cst<V>(0.0);
+  for (int a = 0; a < D; ++a)
+    for (int s = 0; s < 2; ++s)
+      for (int l = 0; l < 3; ++l) Y[a][s][l] = cst<V>(0.0);
+  for (int c = 0; c < NCOR; ++c) {
+    V wc = cst<V>(1.0);
+    for (int a = 0; a < D; ++a) wc *= ((c >> (D - 1 - a)) & 1) == 0 ? am[a] : ap[a];
+    for (int a = 0; a < D; ++a)
+      for (int l = 0; l < 3; ++l) Y[a][(c >> (D - 1 - a)) & 1][l] += wc * v[c][l];
+    Jsum += wc * J[c];
+  }
+  V inv = cst<V>(1.0);
+  for (int a = 0; a < D; ++a) inv *= at[a];
+  inv = cst<V>(1.0) / inv;
+  V twv[3];
+  for (int a = 0; a < D; ++a) {
+    V o = inv;
+    for (int b = 0; b < D; ++b)
+      if (b != a) o *= at[b];
+    twv[a] = am[a] * ap[a] * o;
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int jj = 0; jj < 3; ++jj) {
+      const int cc = 3 * i + jj;
+      V h = (i == jj) ? Jsum : cst<V>(0.0);
+      for (int a = 0; a < D; ++a)
+        for (int s = 0; s < 2; ++s) h += u[a][s][cc] * Y[a][s][a];
+      if (i < D)
+        for (int s = 0; s < 2; ++s)
+          for (int l = 0; l < 3; ++l) h -= u[i % D][s][3 * l + jj] * Y[i % D][s][l];
+      V r = -h * inv;
+      for (int a = 0; a < D; ++a) r += (u[a][0][cc] - u[a][1][cc]) * twv[a];
+      job.rhs[cc * cells + ii] = val(r);
+      if constexpr (TAN) job.drhs[cc * cells + ii] = r.d;
+    }
+  if constexpr (MODE == 1) {
+    job.sel[ii] = code;
+    job.sel[cells + ii] = spd_out;
+  }
+}
+
 // ------------------------------------------------------------------ R2
 // Transpose of the limited-derivative stencils, in site form.  With
 // D_k = f_{k+1} - f_k, up_i = N (D_i - m_i / 2), um_i = N (D_{i-1} + m_{i-1} / 2)
@@ -1219,6 +1511,39 @@ hipError_t launch_flux_vjp_cell(const VjpJob& j, int* nblocks, hipStream_t st) {
   if (bs.x == 32 && bs.y == 8) launch_vjp_cell_t<32, 8>(j, nb, st);
   else if (bs.x == 16 && bs.y == 8) launch_vjp_cell_t<16, 8>(j, nb, st);
   else if (bs.x == 8 && bs.y == 8) launch_vjp_cell_t<8, 8>(j, nb, st);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <typename V, int MODE, int TX, int TY>
+static void launch_branch_t(const FrozenJob& j, int nb, hipStream_t st) {
+  const dim3 bs(TX, TY);
+  if (j.d == 3) {
+    if (j.dyn == DYN_GLIDE_ONLY)
+      hipLaunchKernelGGL((k_flux_branch<V, 3, DYN_GLIDE_ONLY, TX, TY, MODE>), dim3(nb), bs, 0, st, j);
+    else
+      hipLaunchKernelGGL((k_flux_branch<V, 3, DYN_GLIDE_CLIMB, TX, TY, MODE>), dim3(nb), bs, 0, st, j);
+  } else {
+    if (j.dyn == DYN_GLIDE_ONLY)
+      hipLaunchKernelGGL((k_flux_branch<V, 2, DYN_GLIDE_ONLY, TX, TY, MODE>), dim3(nb), bs, 0, st, j);
+    else
+      hipLaunchKernelGGL((k_flux_branch<V, 2, DYN_GLIDE_CLIMB, TX, TY, MODE>), dim3(nb), bs, 0, st, j);
+  }
+}
+
+template <int TX, int TY>
+static void launch_branch_tiles(const FrozenJob& j, int nb, hipStream_t st) {
+  if (j.dbeta) launch_branch_t<Dual, 0, TX, TY>(j, nb, st);
+  else if (j.record) launch_branch_t<double, 1, TX, TY>(j, nb, st);
+  else launch_branch_t<double, 2, TX, TY>(j, nb, st);
+}
+
+hipError_t launch_flux_frozen(const FrozenJob& j, hipStream_t st) {
+  const dim3 bs = adj_block(j.n);
+  const int nb = flux_vjp_nblocks(j.d, j.n);
+  if (bs.x == 32 && bs.y == 8) launch_branch_tiles<32, 8>(j, nb, st);
+  else if (bs.x == 16 && bs.y == 8) launch_branch_tiles<16, 8>(j, nb, st);
+  else if (bs.x == 8 && bs.y == 8) launch_branch_tiles<8, 8>(j, nb, st);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }

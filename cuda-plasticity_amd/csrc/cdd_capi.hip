@@ -904,6 +904,70 @@ int cdd_flux_vjp(cdd_plan* p, const double* beta, double t, const double* rbar, 
   return CDD_OK;
 }
 
+int cdd_flux_frozen(cdd_plan* p, const double* beta, double t, unsigned long long* sel, int record, double* rhs) {
+  if (!p || !beta || !sel || !rhs) return fail(CDD_ERR_ARG, "null argument");
+  if (p->fp32 || p->cfg.scheme != CDD_SCHEME_CENTRAL_UPWIND || p->cfg.dynamics == CDD_GLIDE_CLIMB_WOJZZ ||
+      p->vacancy || p->disorder)
+    return fail(CDD_ERR_UNSUPPORTED,
+                "the branch-frozen flux covers float64 central-upwind glide+climb and glide-only plans");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  HIPCHK(launch_spectral(spectral_job(p, 6, beta, nullptr, nullptr, t, p->G)));
+  FrozenJob f;
+  memset(&f, 0, sizeof f);
+  f.d = p->d;
+  f.n = p->n;
+  f.dyn = p->cfg.dynamics;
+  f.cells = p->cells;
+  f.beta = beta;
+  f.G = p->G;
+  f.sel = sel;
+  f.record = record ? 1 : 0;
+  f.rhs = rhs;
+  f.Lambda = p->cfg.dynamics == CDD_GLIDE_ONLY ? 0.0 : p->cfg.Lambda;
+  f.mu = p->gp.mu;
+  f.lam = p->gp.lam;
+  HIPCHK(launch_flux_frozen(f, p->stream));
+  return CDD_OK;
+}
+
+int cdd_flux_tangent(cdd_plan* p, const double* beta, const double* dbeta, double t, double dLambda, double* rhs,
+                     double* drhs) {
+  if (!p || !beta || !dbeta || !rhs || !drhs) return fail(CDD_ERR_ARG, "null argument");
+  if (p->fp32 || p->cfg.scheme != CDD_SCHEME_CENTRAL_UPWIND || p->cfg.dynamics == CDD_GLIDE_CLIMB_WOJZZ ||
+      p->vacancy || p->disorder)
+    return fail(CDD_ERR_UNSUPPORTED, "the tangent flux covers float64 central-upwind glide+climb and glide-only plans");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  HIPCHK(launch_spectral(spectral_job(p, 6, beta, nullptr, nullptr, t, p->G)));
+  // the stress of dbeta: the Green's part and the load's tangent, which is
+  // nonzero only for the uniaxial control (its -(lam d + 2 mu d_zz d_zz)
+  // <dbeta_zz>, the control with rate and initial strain 0)
+  double* dG = p->work9;
+  SpectralJob jd = spectral_job(p, 6, dbeta, nullptr, nullptr, t, dG);
+  const bool uni = p->gp.load.kind == CDD_LOAD_UNIAXIAL;
+  memset(&jd.gp.load, 0, sizeof jd.gp.load);
+  if (uni) jd.gp.load.kind = CDD_LOAD_UNIAXIAL;
+  jd.gp.log_stress = 0;
+  HIPCHK(launch_spectral(jd));
+  FrozenJob f;
+  memset(&f, 0, sizeof f);
+  f.d = p->d;
+  f.n = p->n;
+  f.dyn = p->cfg.dynamics;
+  f.cells = p->cells;
+  f.beta = beta;
+  f.G = p->G;
+  f.rhs = rhs;
+  f.Lambda = p->cfg.dynamics == CDD_GLIDE_ONLY ? 0.0 : p->cfg.Lambda;
+  f.mu = p->gp.mu;
+  f.lam = p->gp.lam;
+  f.dbeta = dbeta;
+  f.dG = dG;
+  f.dLambda = p->cfg.dynamics == CDD_GLIDE_ONLY ? 0.0 : dLambda;
+  f.drhs = drhs;
+  HIPCHK(launch_flux_frozen(f, p->stream));
+  return CDD_OK;
+}
+
 int cdd_step_vjp(cdd_plan* p, const double* u, double t, double dt, const double* ubar_next, double* ubar,
                  double* lambar) {
   if (!p || !u || !ubar_next || !ubar) return fail(CDD_ERR_ARG, "null argument");

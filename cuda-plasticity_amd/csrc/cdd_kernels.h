@@ -180,6 +180,37 @@ hipError_t launch_flux_vjp_cell(const VjpJob& j, int* nblocks, hipStream_t st);
 hipError_t launch_flux_vjp_gather(const GatherJob& g, hipStream_t st);
 hipError_t launch_energy_grad(const double* G, const double* beta, long long cells, double mu, double nu, double lam,
                               double* sbar, double* partial, int* nblocks, hipStream_t st);
+// The flux with every branch of the scheme fixed (the branch-frozen map the
+// adjoint differentiates): two words per cell and stage in sel,
+//   sel[idx]:         the minmod selection of the cell's own limiter site
+//                     m_i per axis a and component c, 2 bits at 2 (9 a + c)
+//                     (0, 1, 2: the first, middle, last argument; 3: zero)
+//   sel[cells + idx]: per axis a, 8 bits at 8 a: the corner with the largest
+//                     velocity component a (bits 0-2), the corner with the
+//                     smallest (bits 3-5), whether the largest is > 0 (bit 6:
+//                     a_plus is it, else 0) and whether the smallest is < 0
+//                     (bit 7).
+// record = 1 decides the branches at beta (the ordinary flux) and writes
+// them; record = 0 takes them from sel, which makes the flux a smooth
+// function of beta (and of Lambda).  With dbeta set (the tangent mode) the
+// branches are decided at beta, sel is unused, and drhs receives the
+// tangent-linear flux J_L(beta) dbeta + (dL/dLambda) dLambda, dG being the
+// non-local stress planes of dbeta (its load's tangent included).
+struct FrozenJob {
+  int d, n, dyn;
+  long long cells;
+  const double* beta;
+  const double* G;      // the non-local stress planes of beta
+  unsigned long long* sel;
+  int record;
+  double* rhs;          // 9 planes
+  double Lambda, mu, lam;
+  const double* dbeta;  // tangent mode: the tangent of beta (9 planes)
+  const double* dG;     // and of G (6 planes)
+  double dLambda;
+  double* drhs;         // out: the tangent of rhs (9 planes)
+};
+hipError_t launch_flux_frozen(const FrozenJob& j, hipStream_t st);
 hipError_t launch_lincomb(long long len, int nin, const double* a0, double c0, const double* a1, double c1,
                           const double* a2, double c2, const double* a3, double c3, double* out, hipStream_t st);
 

@@ -305,6 +305,26 @@ int cdd_timing_read(cdd_plan* plan, double* ms, int* counts, int nkinds);
  *                    src_dev, ubar_end, ubar0 and Lambda accumulator). */
 int cdd_flux_vjp(cdd_plan* plan, const double* beta_dev, double t, const double* rbar_dev, double* betabar_dev,
                  double* lambar);
+/* Branch-frozen flux (no reference counterpart; for checking the adjoint
+ * against finite differences of a smooth map): the flux of cdd_flux (rhs_dev,
+ * 9 planes) with every branch of the scheme -- the minmod selection of each
+ * limiter site and, per axis, the corners of the largest and smallest
+ * velocity and whether they pass zero (Max9/Min9 with 0,
+ * CentralUpwindHJ3D.py:11-34,189-241) -- recorded into or replayed from
+ * sel_dev (2 N^d words).  record = 1 decides them at beta (the flux is then
+ * cdd_flux's to rounding) and writes them; record = 0 replays them, which
+ * makes the flux a smooth function of beta and Lambda: the map the adjoint
+ * differentiates.  float64 central-upwind glide+climb and glide-only plans. */
+int cdd_flux_frozen(cdd_plan* plan, const double* beta_dev, double t, unsigned long long* sel_dev, int record,
+                    double* rhs_dev);
+/* Tangent-linear flux (no reference counterpart; forward-mode
+ * differentiation of cdd_flux): rhs_dev = the flux at beta (cdd_flux's to
+ * rounding) and drhs_dev = J_L(beta) dbeta + (dL/dLambda) dLambda, with the
+ * branches decided at beta as the adjoint decides them.  The transpose of
+ * this map is cdd_flux_vjp's.  float64 central-upwind glide+climb and
+ * glide-only plans; all loads (the uniaxial control's tangent included). */
+int cdd_flux_tangent(cdd_plan* plan, const double* beta_dev, const double* dbeta_dev, double t, double dLambda,
+                     double* rhs_dev, double* drhs_dev);
 int cdd_step_vjp(cdd_plan* plan, const double* u_dev, double t, double dt, const double* ubar_next_dev,
                  double* ubar_dev, double* lambar);
 int cdd_energy_grad(cdd_plan* plan, const double* beta_dev, double t, double* betabar_dev, double* energy);
