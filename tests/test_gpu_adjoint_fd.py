@@ -192,3 +192,61 @@ def test_adjoint_is_the_transposed_tangent_512_400_steps(L):
     errl = abs(lam_grad - tl) / abs(lam_grad)
     say("  Lambda: adjoint %.14e tangent %.14e rel %.2e; worst over directions %.2e" % (lam_grad, tl, errl, worst))
     assert worst <= 1e-11 and errl <= 1e-11
+
+
+@pytest.mark.timeout(900)
+def test_stress_window_adjoint_is_the_transposed_tangent_3d_256(L):
+    """The 3D 256^3 uniaxial ramp (glide + climb, Lambda = 0.9, fixed steps
+    of the initial CFL step, strain 0.01 after 1000 RK3 steps): the adjoint
+    of the final mean sigma_zz (load included) with respect to the beta^P
+    that starts the last 100 steps, and to Lambda, against the tangent-linear
+    derivative carried on the stepper's own stage states, along 3 random unit
+    directions."""
+    import torch
+    from Plasticity.amd.autograd import mean_stress_window, _mean_stress_grad
+    from Plasticity.amd.frozen import tangent_stored
+    n, total, window, ck = 256, 1000, 100, 10
+    say("3D 256^3 ramp: adjoint of the final mean sigma_zz over the last %d steps vs tangent-linear" % window)
+    beta = SpectralGaussianField((n,) * 3, 6.0, 0.01, seed=0)
+    init_strain = float(beta[8].mean())
+    beta0 = torch.from_numpy(beta).cuda()
+    del beta
+    probe = L.Plan(3, n, dynamics=L.GLIDE_CLIMB, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25)
+    _, speed = probe.flux(beta0)
+    dt0 = 0.25 / (n * float(speed.max()))
+    probe.close()
+    del speed
+    load = dict(load_kind=L.LOAD_UNIAXIAL, load_init_strain=init_strain, load_uniaxial_rate=0.01 / (total * dt0))
+    plan = L.Plan(3, n, dynamics=L.GLIDE_CLIMB, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25, **load)
+    plan.set_state(beta0)
+    plan.set_time(0.0, float("inf"), dt0)
+    plan.step(total - window)
+    t_mid = plan.clock()[0]
+    beta_mid = plan.get_state()
+    del beta0
+    b = beta_mid.clone().requires_grad_(True)
+    lt = torch.tensor(0.9, dtype=torch.float64, device="cuda", requires_grad=True)
+    J = mean_stress_window(b, lt, window, 1, "zz", t0=t_mid, checkpoint_every=ck, rk_order=3, load=load,
+                           fixed_dt=dt0)
+    J.backward()
+    grad, lam_grad = b.grad.detach(), lt.grad.item()
+    del b
+    say("  J = %.15g, |grad| %.6e, dJ/dLambda %.6e" % (J.item(), float(grad.norm()), lam_grad))
+    gvec = torch.tensor(_mean_stress_grad("zz", load, 1.0, 0.3, n ** 3), dtype=torch.float64, device="cuda")
+
+    def dJ(dvec, dlam):
+        u, du, t = tangent_stored(plan, beta_mid, dvec, window, t0=t_mid, dLambda=dlam, fixed_dt=dt0, segment=ck)
+        return float((gvec * du.reshape(9, -1).sum(dim=1)).sum())
+    rng = np.random.default_rng(256)
+    worst = 0.0
+    for k in range(3):
+        dvec = torch.from_numpy(rng.standard_normal((9, n, n, n))).cuda()
+        dvec /= dvec.norm()
+        ad, tl = float((grad * dvec).sum()), dJ(dvec, 0.0)
+        worst = max(worst, abs(ad - tl) / abs(ad))
+        say("  dir %d: adjoint %.14e tangent %.14e rel %.2e" % (k, ad, tl, abs(ad - tl) / abs(ad)))
+        del dvec
+    tl = dJ(torch.zeros_like(beta_mid), 1.0)
+    errl = abs(tl - lam_grad) / abs(lam_grad)
+    say("  Lambda: adjoint %.14e tangent %.14e rel %.2e" % (lam_grad, tl, errl))
+    assert worst <= 1e-11 and errl <= 1e-11
