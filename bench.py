@@ -6,7 +6,11 @@ TVD-RK3, CFL 0.25, mu = 1, nu = 0.3 -- the workload the headline metric
 
 A step is one full TVD-RK3 step (three stress solves + three fused
 flux/stage kernels + the device-side CFL reduction) over the whole grid;
-inputs are resident in HBM before the timed region starts.
+inputs are resident in HBM before the timed region starts.  After the
+headline measurement the same line carries the benchmark's two other
+metrics, measured compactly: "adjoint_vs_forward" (the 2D 512^2 x 400-step
+energy gradient and the 3D 256^3 100-step stress window) and
+"batched_vs_serial" (48 uneven 2D realisations); --no-sub skips them.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
 Prints one JSON line.
@@ -51,45 +55,47 @@ def canonical_bytes_per_cell_update(n, d=3, order=3):
     return order * bfft + 408 + 264 * (order - 1)
 
 
-def cpu_baseline(seconds_hint=15.0):
+def cpu_baseline_numpy():
     """The NumPy oracle (the reference Python path restated) on a bounded
-    sample of the same dynamics: 3D 64^3, 2 RK3 steps, one core."""
-    import numpy as np
+    sample of the same dynamics: 3D 64^3, 2 RK3 steps, on one core (every
+    thread pool limited to one thread)."""
+    from threadpoolctl import threadpool_limits
     from oracle import cdd_oracle as O
     from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
     n, steps = 64, 2
     b = SpectralGaussianField((n,) * 3, 6.0, 0.01, seed=0)
     dyn = O.Dynamics(O.GLIDE_CLIMB, 0.9, 1.0, 0.3,
                      O.Load("uniaxial", init_strain=float(b[8].mean()), uniaxial_rate=0.01))
-    t0 = time.perf_counter()
-    O.run(dyn, b, 0.0, steps, 0.25)
-    dt = time.perf_counter() - t0
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        O.run(dyn, b, 0.0, steps, 0.25)
+        dt = time.perf_counter() - t0
     return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
             "sample": "NumPy oracle, 3D 64^3 glide+climb Lambda=0.9 uniaxial load, 2 TVD-RK3 steps (%.1f s), "
-                      "single-threaded" % dt}
+                      "one thread" % dt}
 
 
-def cpu_baseline_openmp():
-    """The C/OpenMP restatement of the oracle on every host thread
-    (OMP_NUM_THREADS, else os.cpu_count()): 3D 128^3, 2 RK3 steps -- the
-    stronger CPU baseline, reported beside the single-core one."""
+def cpu_baseline(n=256):
+    """The C/OpenMP restatement of the oracle on every host thread it is given
+    (OMP_NUM_THREADS, else os.cpu_count()): one RK3 step of the benchmark's
+    own workload, 3D n^3 glide + climb under the uniaxial control."""
     from oracle import cdd_oracle as O
     from oracle import cdd_oracle_c as OC
     from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
     if not os.path.exists(OC.LIB):
         return None
-    n, steps = 128, 2
+    steps = 1
     b = SpectralGaussianField((n,) * 3, 6.0, 0.01, seed=0)
     dyn = O.Dynamics(O.GLIDE_CLIMB, 0.9, 1.0, 0.3,
                      O.Load("uniaxial", init_strain=float(b[8].mean()), uniaxial_rate=0.01))
-    OC.run(dyn, b, 0.0, 1, 0.25)  # warm-up (thread pool, twiddles)
+    OC.flux(dyn, 0.0, b)  # warm-up: the oracle's kept workspaces and twiddles
     t0 = time.perf_counter()
     OC.run(dyn, b, 0.0, steps, 0.25)
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": "C/OpenMP oracle, 3D 128^3 glide+climb Lambda=0.9 uniaxial load, 2 TVD-RK3 steps "
-                      "(%.1f s) on %d threads" % (dt, cores)}
+            "sample": "C/OpenMP oracle, 3D %d^3 glide+climb Lambda=0.9 uniaxial load, %d TVD-RK3 step "
+                      "(%.1f s) on %d threads" % (n, steps, dt, cores)}
 
 
 def bench_loading(args):
@@ -189,26 +195,25 @@ def bench_loading(args):
     print(json.dumps(out))
 
 
-def bench_batched(args):
+def batched_vs_serial(K, W, count=48):
     """48 independent 2D realisations, sides drawn from {256,384,512,768,1024},
     per-instance seeds, L ~ U[3, 8] cells, RMS 0.01, glide-only ('lvp'),
-    SSP-RK2, CFL 0.25: the batched hipGraph step against a serial loop of
-    plain single-instance plans over the same instances."""
+    SSP-RK2, CFL 0.25: the batched hipGraph step (tiles autotuned) against a
+    serial loop of plain single-instance plans over the same instances, K
+    timed steps after W warm-up steps.  Returns a dict (ratio, times, the
+    batched step's share of the 2D step roofline)."""
     import numpy as np
     import torch
     from Plasticity.amd import lib as L
     from Plasticity.amd.autotune import autotune_batch
     from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
     rng = np.random.default_rng(2024)
-    count = 48
     sizes = [int(s) for s in rng.choice([256, 384, 512, 768, 1024], size=count)]
     lens = rng.uniform(3.0, 8.0, size=count)
     betas = [torch.from_numpy(SpectralGaussianField((n, n), float(lens[i]), 0.01, seed=i)).cuda()
              for i, n in enumerate(sizes)]
     cells = sum(n * n for n in sizes)
-    K, W = args.steps, args.warmup
     kw = dict(dynamics=L.GLIDE_ONLY, rk_order=2, cfl=0.25)
-    # batched
     b = L.Batch(sizes, **kw)
     for i in range(count):
         b.set_state(i, betas[i])
@@ -222,7 +227,6 @@ def bench_batched(args):
     tb = time.perf_counter() - t0
     steps_b = [b.clock(i)[3] for i in range(count)]
     b.close()
-    # serial loop of the plain stepper
     plans = []
     for i, n in enumerate(sizes):
         p = L.Plan(2, n, **kw)
@@ -239,39 +243,51 @@ def bench_batched(args):
     ts = time.perf_counter() - t0
     for p in plans:
         p.close()
+    del betas
+    torch.cuda.empty_cache()
+    # the 2D RK2 step's canonical bytes, summed over the instances
+    step_bytes = sum(n * n * canonical_bytes_per_cell_update(n, d=2, order=2) for n in sizes)
+    return {"value": ts / tb, "unit": "x (serial time / batched time)", "steps": K, "warmup": W,
+            "ms_per_step": 1e3 * tb / K, "serial_ms_per_step": 1e3 * ts / K,
+            "cell_updates_per_s_batched": cells * K / tb, "cell_updates_per_s_serial": cells * K / ts,
+            "step_roofline": {"bytes_per_step": step_bytes, "frac_batched": step_bytes * K / tb / (PEAK_HBM_GBS * 1e9),
+                              "frac_serial": step_bytes * K / ts / (PEAK_HBM_GBS * 1e9)},
+            "sizes": sizes, "cells": cells, "flux_tiles": {str(k): v for k, v in choices.items()},
+            "all_steps_ran": all(s == K + W for s in steps_b)}
+
+
+def bench_batched(args):
+    """The batched workload as its own line (batched_vs_serial)."""
+    r = batched_vs_serial(args.steps, args.warmup)
     out = {
         "metric": "batched vs serial (48 uneven 2D realisations, SSP-RK2)",
-        "value": ts / tb,
-        "unit": "x (serial time / batched time)",
-        "n_gpus": 1, "steps": K, "warmup": W,
-        "ms_per_step": 1e3 * tb / K,
-        "serial_ms_per_step": 1e3 * ts / K,
-        "cell_updates_per_s_batched": cells * K / tb,
-        "cell_updates_per_s_serial": cells * K / ts,
+        "value": r["value"], "unit": r["unit"],
+        "n_gpus": 1, "steps": r["steps"], "warmup": r["warmup"],
+        "ms_per_step": r["ms_per_step"], "serial_ms_per_step": r["serial_ms_per_step"],
+        "cell_updates_per_s_batched": r["cell_updates_per_s_batched"],
+        "cell_updates_per_s_serial": r["cell_updates_per_s_serial"],
+        "step_roofline": r["step_roofline"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: seeded Gaussian random beta^P per instance (L ~ U[3,8] cells, RMS 0.01)",
-        "config": {"workload": "cdd-2d-batched-48", "sizes": sizes, "cells": cells, "dynamics": "lvp",
-                   "rk_order": 2, "cfl": 0.25, "flux_tiles": {str(k): v for k, v in choices.items()}},
-        "all_steps_ran": all(s == K + W for s in steps_b),
+        "config": {"workload": "cdd-2d-batched-48", "sizes": r["sizes"], "cells": r["cells"], "dynamics": "lvp",
+                   "rk_order": 2, "cfl": 0.25, "flux_tiles": r["flux_tiles"]},
+        "all_steps_ran": r["all_steps_ran"],
     }
     print(json.dumps(out))
 
 
-def bench_adjoint(args):
-    """2D 512^2, 400 SSP-RK2 steps, glide + climb with climb mobility 0.1
-    (Lambda = 0.9), uniform shear sigma_xy = 0.005: gradient of the final
-    total elastic energy w.r.t. the initial beta^P and the climb mobility by
-    the hand-written adjoint (segments of 20 steps whose stage states are kept
-    in device memory; the checkpoint-recomputation cost is reported beside
-    it), its cost relative to the plain forward run, and central finite
-    differences on 32 random unit directions (eps = 1e-6) replaying the
-    recorded time steps."""
-    import numpy as np
+def adjoint_cost_2d(nsteps=400):
+    """2D 512^2, nsteps SSP-RK2 steps, glide + climb with climb mobility 0.1
+    (Lambda = 0.9), uniform shear sigma_xy = 0.005: the hand-written adjoint
+    of the final total elastic energy w.r.t. the initial beta^P and Lambda
+    (segments of 20 steps whose stage states are kept in device memory; the
+    checkpoint-recomputation cost beside it) against the plain forward run.
+    Returns (summary dict, context for the finite-difference report)."""
     import torch
     from Plasticity.amd import lib as L
     from Plasticity.amd.autograd import run_with_grad, elastic_energy
     from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
-    n, nsteps, Lam = 512, (args.steps if args.steps != 20 else 400), 0.9
+    n, Lam = 512, 0.9
     shear = dict(load_kind=L.LOAD_TENSOR, load_init=(0, 0.005, 0, 0.005, 0, 0, 0, 0, 0))
     beta0 = torch.from_numpy(SpectralGaussianField((n, n), 4.0, 0.01, seed=4)).cuda()
     plan = L.Plan(2, n, dynamics=L.GLIDE_CLIMB, rk_order=2, Lambda=Lam, **shear)
@@ -313,10 +329,85 @@ def bench_adjoint(args):
     adjoint(False)
     torch.cuda.synchronize()
     t_adj_rc = time.perf_counter() - t0
+    torch.cuda.empty_cache()
+    summary = {"value": t_adj / t_fwd, "unit": "x (forward+backward time / forward time)",
+               "forward_ms_per_step": 1e3 * t_fwd / nsteps, "adjoint_ms_per_step": 1e3 * t_adj / nsteps,
+               "cost_vs_forward_recompute": t_adj_rc / t_fwd, "steps": nsteps, "energy": e_fwd,
+               "dE_dmobility": -lt.grad.item(),
+               "config": {"workload": "cdd-2d-512-adjoint", "grid": [n, n], "dynamics": "glide+climb",
+                          "Lambda": Lam, "rk_order": 2, "cfl": 0.25, "load": "sigma_xy = sigma_yx = 0.005",
+                          "checkpoint_every": 20, "trajectory": "stored"}}
+    ctx = dict(plan=plan, beta0=beta0, log=log, grad=b.grad.detach(), n=n, Lam=Lam, shear=shear, t_fwd=t_fwd,
+               t_adj=t_adj, t_adj_rc=t_adj_rc, e_fwd=e_fwd, mob_grad=-lt.grad.item())
+    return summary, ctx
+
+
+def adjoint_cost_3d_window(n=256, window=100, ck=10):
+    """3D n^3 glide + climb (Lambda = 0.9) under the uniaxial strain-rate
+    control of the loading ramp (fixed steps of the initial CFL step): the
+    adjoint of the final mean sigma_zz over a window of `window` RK3 steps
+    (checkpoints every ck, stored while memory allows) against the plain
+    forward run of the same steps, from the ramp's start state."""
+    import torch
+    from Plasticity.amd import lib as L
+    from Plasticity.amd.autograd import mean_stress_window
+    from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
+    beta = SpectralGaussianField((n,) * 3, 6.0, 0.01, seed=0)
+    init_strain = float(beta[8].mean())
+    beta0 = torch.from_numpy(beta).cuda()
+    del beta
+    probe = L.Plan(3, n, dynamics=L.GLIDE_CLIMB, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25)
+    _, speed = probe.flux(beta0)
+    dt0 = 0.25 / (n * float(speed.max()))
+    probe.close()
+    del speed
+    load = dict(load_kind=L.LOAD_UNIAXIAL, load_init_strain=init_strain, load_uniaxial_rate=0.01 / (1000 * dt0))
+    plan = L.Plan(3, n, dynamics=L.GLIDE_CLIMB, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25, **load)
+    for rep in range(2):  # the first run warms up
+        plan.set_state(beta0)
+        plan.set_time(0.0, float("inf"), dt0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.step(window)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+    plan.close()
+    b = beta0.clone().requires_grad_(True)
+    lt = torch.tensor(0.9, dtype=torch.float64, device="cuda", requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = mean_stress_window(b, lt, window, 1, "zz", t0=0.0, checkpoint_every=ck, rk_order=3, load=load,
+                           fixed_dt=dt0)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    J.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    out = {"value": (t2 - t0) / t_fwd, "unit": "x (forward+backward time / forward time)",
+           "forward_s": t_fwd, "adjoint_forward_s": t1 - t0, "adjoint_backward_s": t2 - t1, "steps": window,
+           "objective": "final mean sigma_zz (load included)", "J": J.item(), "dJ_dLambda": lt.grad.item(),
+           "config": {"workload": "cdd-3d-%d-loading-window" % n, "grid": [n, n, n], "rk_order": 3,
+                      "checkpoint_every": ck, "fixed_dt": dt0}}
+    del b, lt, J, beta0
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_adjoint(args):
+    """The 2D 512^2 adjoint workload (adjoint_cost_2d) as its own line, with
+    central finite differences on random unit directions (eps = 1e-6, 1e-8)
+    replaying the recorded time steps, reported as data: the scheme is only
+    piecewise smooth, so these do not converge (the exactness checks are the
+    tangent-linear dot products of the GPU tests)."""
+    import numpy as np
+    import torch
+    nsteps = args.steps if args.steps != 20 else 400
+    summary, c = adjoint_cost_2d(nsteps)
+    plan, beta0, log, grad, n, Lam, shear = (c[k] for k in ("plan", "beta0", "log", "grad", "n", "Lam", "shear"))
+    t_fwd, t_adj, t_adj_rc, e_fwd, mob_grad = (c[k] for k in ("t_fwd", "t_adj", "t_adj_rc", "e_fwd", "mob_grad"))
+    from Plasticity.amd import lib as L
     store_bytes = L.Plan(2, n, dynamics=L.GLIDE_CLIMB, rk_order=2, Lambda=Lam, **shear).segment_store_bytes(20) * (
         nsteps // 20)
-    grad = b.grad.detach()
-    mob_grad = -lt.grad.item()  # climb mobility = 1 - Lambda
 
     def replay(b0):
         plan.set_state(b0)
@@ -376,6 +467,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sub", action="store_true",
+                    help="3d workload: skip the adjoint/forward and batched/serial sub-results")
     ap.add_argument("--fd-dirs", type=int, default=32, help="adjoint workload: finite-difference directions")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
                     help="3d workload: arithmetic of the plan (fp32 also reports its error against fp64)")
@@ -535,15 +628,35 @@ def main():
     }
     if fp32_err is not None:
         out["fp32_rel_l2_vs_fp64"] = fp32_err
+    plan.close()
+    del state, beta_t, slog
+    torch.cuda.empty_cache()
+    if not args.no_sub:
+        # the two other metrics of the benchmark, compact: adjoint/forward cost
+        # (2D 512^2 x 400 RK2 energy gradient; 3D 256^3 100-step stress window)
+        # and the batched step against a serial loop of plain plans
+        sub = {}
+        for key, fn in (("2d_512_400_rk2_energy", lambda: adjoint_cost_2d(400)[0]),
+                        ("3d_256_window_100_rk3_stress", lambda: adjoint_cost_3d_window(n))):
+            try:
+                sub[key] = fn()
+            except Exception as e:  # reported, never fatal to the headline line
+                sub[key] = {"error": repr(e)}
+            print("adjoint %s: %s" % (key, sub[key].get("value", sub[key])), file=sys.stderr, flush=True)
+            torch.cuda.empty_cache()
+        out["adjoint_vs_forward"] = sub
+        try:
+            out["batched_vs_serial"] = batched_vs_serial(10, 3)
+        except Exception as e:
+            out["batched_vs_serial"] = {"error": repr(e)}
+        print("batched: %s" % out["batched_vs_serial"].get("value"), file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
-        omp = cpu_baseline_openmp()
-        if omp is not None:
-            out["cpu_baseline_openmp"] = omp
+        c = cpu_baseline(n)
+        if c is not None:
+            out["cpu_baseline"] = c
+        out["cpu_baseline_numpy"] = cpu_baseline_numpy()
     print(json.dumps(out))
 
 
 if __name__ == "__main__":
-    if "OMP_NUM_THREADS" not in os.environ:
-        os.environ["OMP_NUM_THREADS"] = "1"
     main()
