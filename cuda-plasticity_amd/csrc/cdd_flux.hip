@@ -119,6 +119,28 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
       P[a][s] = p;
     }
   }
+  // the cross terms of |rho|^2 per pair of axes and their signs (each
+  // corner's is the sum of three of them, 3D; one, 2D):
+  //   X01[s0][s1] = sum_j u_0^s0[1,j] u_1^s1[0,j],  X12[s1][s2] = sum_j u_1^s1[2,j] u_2^s2[1,j],
+  //   X20[s2][s0] = sum_j u_2^s2[0,j] u_0^s0[2,j]
+  T X01[2][2], X12[2][2], X20[2][2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      T a = T(0), b = T(0), e = T(0);
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) {
+        a += u[0][s][3 + jj] * u[1][r][jj];
+        if constexpr (D == 3) {
+          b += u[1][s][6 + jj] * u[2][r][3 + jj];
+          e += u[2][s][jj] * u[0][r][6 + jj];
+        }
+      }
+      X01[s][r] = a;
+      X12[s][r] = b;
+      X20[s][r] = e;
+    }
   const T third = T(1.0 / 3.0);
   // climb suppression: glide + climb uses Lambda tr(sigma)/3 and q; the woJzz
   // variant (2D) Lambda tr(sigma)/2 and w' (CentralUpwindHJBetaPDynamics_woJzz.py:71-75)
@@ -211,15 +233,8 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
     T r2 = MEU;
 #pragma unroll
     for (int a = 0; a < D; ++a) r2 += P[a][sgn[a]];
-    T cross = T(0);
-#pragma unroll
-    for (int jj = 0; jj < 3; ++jj) {
-      cross += u[0][sgn[0]][3 + jj] * u[1][sgn[1]][jj];
-      if constexpr (D == 3) {
-        cross += u[1][sgn[1]][6 + jj] * u[2][sgn[2]][3 + jj];
-        cross += u[2][sgn[2]][jj] * u[0][sgn[0]][6 + jj];
-      }
-    }
+    T cross = X01[sgn[0]][sgn[1]];
+    if constexpr (D == 3) cross += X12[sgn[1]][sgn[2]] + X20[sgn[2]][sgn[0]];
     r2 -= T(2) * cross;
     const T rinv = rsqrt_nr(r2);
     T vv[3];
