@@ -219,7 +219,7 @@ def header_times(header, step, end):
 
 # ------------------------------------------------------------ the executable
 # Compile-time options of the executable that this runner does not carry out.
-UNSUPPORTED = ("VACANCIES", "DYNAMIC_NUCLEATION", "DEBUG_TIMESTEPS", "SINGLE_STEP_DEBUG")
+UNSUPPORTED = ("VACANCIES", "DEBUG_TIMESTEPS", "SINGLE_STEP_DEBUG")
 
 
 def _value(v):
@@ -278,6 +278,17 @@ def header_options(header):
     return kw, h
 
 
+def nucleation_cap(plan, src):
+    """maxNucleationTimestep (plasticity.cu:190-199): 1 / sqrt(|s|) for the
+    entry s of largest magnitude over all nine components of the stress of
+    src (calculateSigma, no load)."""
+    from Plasticity.amd.lib import Plan
+    p = Plan(plan.ndim, plan.n, mu=MU, nu=NU, device=plan.device)
+    s = p.sigma(src)
+    p.close()
+    return 1.0 / float(s.abs().max()) ** 0.5
+
+
 def header_plan(header, device=0, rk_order=1):
     """The header's stepper (header_options) as a Plan; rk_order 1 is the
     executable's TVDstep (simpleTVD), 3 its TVD3rd."""
@@ -295,11 +306,19 @@ def run_header(header, state=None, seed=0, device=0, rk_order=1, directory=".", 
     interval ends with the step whose dt = t_end - t + ME carries t just past
     it (simpleTVD, plasticity_kernel.cu:1124-1125).  An existing non-empty
     FILE_OUTPUT is resumed from its last record (CONTINUE_RUN, plasticity.h:10,
-    plasticity.cu:105-122).  Returns (t, state tensor on the GPU)."""
+    plasticity.cu:105-122).
+
+    DYNAMIC_NUCLEATION (plasticity.cu:178-201): the loaded field (the input,
+    or on a resume the last record) becomes the source beta0dot, added to
+    every flux as beta0dot / TIME_FINAL, the state starts from zero, and the
+    time step is capped at 1 / sqrt(|sigma(beta0dot)|) of its largest stress
+    entry; the t = 0 record holds the loaded field, as the executable writes
+    its host copy before the first step.  Returns (t, state tensor on the GPU)."""
     plan, h = header_plan(header, device=device, rk_order=rk_order)
     N, dim = plan.n, plan.ndim
     args = (N, seed, float(h["lambda"]))
     out = os.path.join(directory, _cformat(str(h["FILE_OUTPUT"]), args))
+    nucleation = "DYNAMIC_NUCLEATION" in h
     t = 0.0
     if os.path.isfile(out) and os.path.getsize(out) > 0:
         t, _, _ = plan.plas_resume(out)
@@ -313,6 +332,11 @@ def run_header(header, state=None, seed=0, device=0, rk_order=1, directory=".", 
         plan.set_state(state.GetOrderParameterField().data if hasattr(state, "GetOrderParameterField") else state)
         plan.set_time(0.0)
         plan.plas_append(out)  # the t = 0 record (plasticity.cu:232-233)
+    if nucleation:
+        src = plan.get_state()
+        plan.set_source(src, 1.0 / float(h["TIME_FINAL"]), nucleation_cap(plan, src))
+        plan.set_state(plan.torch.zeros_like(src))
+        plan.set_time(t)
     end = float(h["TIME_FINAL"])
     timed = "LOADING" in h or "TIME_STEP" in h
     while t < end:

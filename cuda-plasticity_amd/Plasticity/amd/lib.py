@@ -59,7 +59,7 @@ EXPORTS = [
     "cdd_energy_record", "cdd_status", "cdd_instability", "cdd_segment_store_bytes", "cdd_segment_forward",
     "cdd_segment_vjp_stored", "cdd_set_vacancy", "cdd_vacancy_set", "cdd_vacancy_get",
     "cdd_set_disorder", "cdd_disorder_get", "cdd_disorder_sigma", "cdd_disorder_flux", "cdd_sigma_noalloc",
-    "cdd_flux_frozen", "cdd_flux_tangent",
+    "cdd_flux_frozen", "cdd_flux_tangent", "cdd_set_source",
 ]
 KERNEL_KINDS = ["r2c", "cols_fwd", "contract", "cols_inv", "c2r", "flux_stage1", "cfl_reduce",
                 "flux_stage2", "flux_stage3", "advance"]
@@ -129,6 +129,7 @@ def _load():
         "cdd_vacancy_set": ([P, P, I], I),
         "cdd_vacancy_get": ([P, P, I], I),
         "cdd_set_disorder": ([P, P, I, I], I),
+        "cdd_set_source": ([P, P, I, D, D], I),
         "cdd_disorder_get": ([P, P, I], I),
         "cdd_disorder_sigma": ([P, P, P, D, P], I),
         "cdd_disorder_flux": ([P, P, P, D, P, P, P], I),
@@ -378,6 +379,17 @@ class Plan:
         c starts at zero (vacancy_set)."""
         check(lib.cdd_set_vacancy(self.h, 1 if enable else 0, float(alpha), float(beta), float(gamma),
                                   float(dt_bound) if dt_bound else 0.0))
+
+    def set_source(self, src, rate=1.0, dt_cap=0.0):
+        """Dynamic nucleation: every flux evaluation gains rate * src (nine
+        components, the grid shape) and the adaptive time step is capped at
+        dt_cap (0: no cap).  None switches it off."""
+        if src is None:
+            check(lib.cdd_set_source(self.h, None, 0, 0.0, 0.0))
+            return
+        t = self._as_dev(src)
+        check(lib.cdd_set_source(self.h, _ptr(t), 1, float(rate), float(dt_cap)))
+        self.torch.cuda.current_stream(self.dev).synchronize()
 
     def vacancy_set(self, c):
         """Set the vacancy concentration field (gridShape, float64)."""

@@ -95,6 +95,10 @@ struct cdd_plan {
   double* dis_L;
   double* dis_F1;
   double* dsig;
+  // dynamic nucleation (cdd_set_source): a fixed source added to every flux
+  // evaluation and the cap on the adaptive time step
+  double* nuc_src;
+  double nuc_rate, nuc_cap;
   // HIP-event timing of every kernel launched by cdd_step (optional)
   bool timing;
   std::vector<hipEvent_t>* ev;
@@ -314,6 +318,7 @@ int cdd_plan_destroy(cdd_plan* p) {
   (void)hipFree(p->dis_L);
   (void)hipFree(p->dis_F1);
   (void)hipFree(p->dsig);
+  (void)hipFree(p->nuc_src);
   if (p->own) (void)hipStreamDestroy(p->own);
   if (p->ev) {
     for (hipEvent_t e : *p->ev) (void)hipEventDestroy(e);
@@ -446,6 +451,8 @@ static FluxJob flux_job(cdd_plan* p) {
   f.stream = p->stream;
   f.fp32 = p->fp32;
   f.llf = p->cfg.scheme == CDD_SCHEME_LLF;
+  f.src = p->nuc_src;
+  f.src_rate = p->nuc_rate;
   return f;
 }
 
@@ -535,7 +542,8 @@ static int launch_step(cdd_plan* p, const StepBufs& b) {
   int nb = 0;
   HIPCHK(launch_flux(f, &nb));
   if (p->timing) timing_hook(p, KK_FLUX1);
-  HIPCHK(launch_finalize_dt(p->partial, nb, p->clk, nmax, p->stream, 1, 0, p->vacancy ? p->dt_bound : 0.0));
+  HIPCHK(launch_finalize_dt(p->partial, nb, p->clk, nmax, p->stream, 1, 0,
+                            p->vacancy ? p->dt_bound : (p->nuc_src ? p->nuc_cap : 0.0)));
   if (p->timing) timing_hook(p, KK_FINALIZE);
   if (p->cfg.rk_order == 1) {
     // forward Euler (simpleTVD, plasticity_kernel.cu:1112-1137), in place
@@ -660,6 +668,7 @@ int cdd_set_vacancy(cdd_plan* p, int enable, double alpha, double beta, double g
   if (p->cfg.scheme != CDD_SCHEME_CENTRAL_UPWIND)
     return fail(CDD_ERR_UNSUPPORTED, "the vacancy coupling uses the central-upwind scheme");
   if (p->disorder) return fail(CDD_ERR_UNSUPPORTED, "the vacancy and disorder couplings do not combine");
+  if (p->nuc_src) return fail(CDD_ERR_UNSUPPORTED, "the vacancy coupling excludes the nucleation source");
   if (!(gamma >= 0.0) || !(dt_bound >= 0.0) || !isfinite(alpha) || !isfinite(beta))
     return fail(CDD_ERR_ARG, "bad vacancy parameters");
   HIPCHK(hipSetDevice(p->cfg.device));
@@ -756,6 +765,29 @@ int cdd_disorder_flux(cdd_plan* p, const double* beta, const double* dfield, dou
     f.outd = rhs_d;
   }
   HIPCHK(launch_flux(f, nullptr));
+  return CDD_OK;
+}
+
+int cdd_set_source(cdd_plan* p, const void* src, int on_device, double rate, double dt_cap) {
+  if (!p) return fail(CDD_ERR_ARG, "null plan");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  if (!src) {
+    HIPCHK(hipStreamSynchronize(p->stream));
+    (void)hipFree(p->nuc_src);
+    p->nuc_src = nullptr;
+    p->nuc_rate = p->nuc_cap = 0.0;
+    return CDD_OK;
+  }
+  if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "the nucleation source is float64 only");
+  if (p->vacancy) return fail(CDD_ERR_UNSUPPORTED, "the nucleation source excludes the vacancy coupling");
+  if (!isfinite(rate) || !(dt_cap >= 0.0)) return fail(CDD_ERR_ARG, "bad nucleation rate or time-step cap");
+  const size_t fb = sizeof(double) * 9 * (size_t)p->cells;
+  if (!p->nuc_src && hipMalloc((void**)&p->nuc_src, fb) != hipSuccess)
+    return fail(CDD_ERR_HIP, "hipMalloc failed (nucleation source)");
+  HIPCHK(hipMemcpyAsync(p->nuc_src, src, fb, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, p->stream));
+  if (!on_device) HIPCHK(hipStreamSynchronize(p->stream));
+  p->nuc_rate = rate;
+  p->nuc_cap = dt_cap;
   return CDD_OK;
 }
 
@@ -907,7 +939,7 @@ int cdd_flux_vjp(cdd_plan* p, const double* beta, double t, const double* rbar, 
 int cdd_flux_frozen(cdd_plan* p, const double* beta, double t, unsigned long long* sel, int record, double* rhs) {
   if (!p || !beta || !sel || !rhs) return fail(CDD_ERR_ARG, "null argument");
   if (p->fp32 || p->cfg.scheme != CDD_SCHEME_CENTRAL_UPWIND || p->cfg.dynamics == CDD_GLIDE_CLIMB_WOJZZ ||
-      p->vacancy || p->disorder)
+      p->vacancy || p->disorder || p->nuc_src)
     return fail(CDD_ERR_UNSUPPORTED,
                 "the branch-frozen flux covers float64 central-upwind glide+climb and glide-only plans");
   HIPCHK(hipSetDevice(p->cfg.device));
@@ -934,7 +966,7 @@ int cdd_flux_tangent(cdd_plan* p, const double* beta, const double* dbeta, doubl
                      double* drhs) {
   if (!p || !beta || !dbeta || !rhs || !drhs) return fail(CDD_ERR_ARG, "null argument");
   if (p->fp32 || p->cfg.scheme != CDD_SCHEME_CENTRAL_UPWIND || p->cfg.dynamics == CDD_GLIDE_CLIMB_WOJZZ ||
-      p->vacancy || p->disorder)
+      p->vacancy || p->disorder || p->nuc_src)
     return fail(CDD_ERR_UNSUPPORTED, "the tangent flux covers float64 central-upwind glide+climb and glide-only plans");
   HIPCHK(hipSetDevice(p->cfg.device));
   HIPCHK(launch_spectral(spectral_job(p, 6, beta, nullptr, nullptr, t, p->G)));

@@ -766,6 +766,12 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   T rhs[9], speed, rhs_sc = T(0);
   if constexpr (LLF) cell_flux_llf<T, D, DYN>(u, sg, T(job.Lambda), rhs, speed, &vsh[0][tid], FNT);
   else cell_flux<T, D, DYN, PRE, SC>(u, sg, T(job.Lambda), rhs, speed, &vsh[0][tid], FNT, T(n), usc, &rhs_sc);
+  if constexpr (sizeof(T) == 8) {
+    if (job.src) {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) rhs[c] += T(job.src_rate) * job.src[c * cells + idx];
+    }
+  }
   if constexpr (SC) {
     double* od = job.outd;
     if (job.mode == 0) od[idx] = (double)rhs_sc;
@@ -1231,7 +1237,7 @@ static int march_chunk(int n, int per_cu = 1) {
 }
 
 static bool march_ok(const FluxJob& job) {
-  return job.d == 3 && !job.llf && !job.S9 && !job.Ad && !job.speed && job.count <= 1 &&
+  return job.d == 3 && !job.llf && !job.S9 && !job.Ad && !job.src && !job.speed && job.count <= 1 &&
          job.tile == 0 && job.n >= 32 &&
          job.dyn != DYN_WOJZZ && (job.mode == 0 || job.U) && march_chunk(job.n) > 0;
 }

@@ -106,6 +106,11 @@ struct FluxJob {
   const double* Ad;
   const double* Ud;
   double* outd;
+  // fp64, per-tile kernel: a fixed source added to every cell's rhs,
+  // rhs += src_rate * src (nine planes; the CUDA executable's dynamic
+  // nucleation, plasticity_kernel.cu:1089-1091)
+  const double* src;
+  double src_rate;
 };
 
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out);

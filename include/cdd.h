@@ -179,6 +179,16 @@ int cdd_set_vacancy(cdd_plan* plan, int enable, double alpha, double beta, doubl
 int cdd_vacancy_set(cdd_plan* plan, const void* c, int on_device);
 int cdd_vacancy_get(cdd_plan* plan, void* c, int on_device);
 
+/* The CUDA executable's DYNAMIC_NUCLEATION (plasticity.cu:178-201,
+ * plasticity_kernel.cu:31-33,1089-1091,1122-1125): every flux evaluation
+ * gains rate * src (the executable's rhs += beta0dot / TIME_FINAL, every
+ * stage) and the adaptive time step is capped at dt_cap (its
+ * maxNucleationTimestep) before the end-time clamp.  src: 9 N^d float64
+ * values, host (on_device = 0) or device; NULL switches the source off.
+ * float64 plans; excludes the vacancy coupling.  The source does not
+ * depend on the state, so the adjoint is unchanged. */
+int cdd_set_source(cdd_plan* plan, const void* src, int on_device, double rate, double dt_cap);
+
 /* Disorder (FieldDynamics/DisorderDynamics.py:17-44,
  * PlasticityStates/DisorderState.py:11-30, RunPython.py:196-214):
  * the stress gains sum_{l,t} eps_lit irfftn(k_l rfftn(rho'_tj)) with

@@ -1125,23 +1125,28 @@ def cuda_llf_flux(beta, sig, lam, glide_only=False):
     return deriv, at * d
 
 
-def cuda_euler_step(flux, t, u, t_end, cfl, N):
+def cuda_euler_step(flux, t, u, t_end, cfl, N, dt_cap=None):
     """simpleTVD (plasticity_kernel.cu:1112-1137): dt = CFL / max speed / N,
+    capped at maxNucleationTimestep under DYNAMIC_NUCLEATION (:1122-1125),
     past the end time dt = t_end - t + ME; u += dt rhs."""
     rhs, speed = flux(t, u)
     dt = cfl / speed.max() / N
+    if dt_cap is not None and dt > dt_cap:
+        dt = dt_cap
     if t + dt > t_end:
         dt = t_end - t + ME
     return u + dt * rhs, dt
 
 
-def cuda_tvd3_step(flux, t, u, t_end, cfl, N):
+def cuda_tvd3_step(flux, t, u, t_end, cfl, N, dt_cap=None):
     """TVD3rd (plasticity_kernel.cu:1140-1206): the Shu-Osher stages of the
     Python mover (every stage at the step's start time) with simpleTVD's dt
     rule and end clamp.  Pinned against tvd_rk3_step, whose stages it shares,
     by tests/test_oracle_cuda.py."""
     rhs, speed = flux(t, u)
     dt = cfl / speed.max() / N
+    if dt_cap is not None and dt > dt_cap:
+        dt = dt_cap
     if t + dt > t_end:
         dt = t_end - t + ME
     L0 = dt * rhs
@@ -1152,20 +1157,36 @@ def cuda_tvd3_step(flux, t, u, t_end, cfl, N):
     return 1. / 3. * u + 2. / 3. * F1 + 2. / 3. * (dt * rhs), dt
 
 
-def run_cuda(dyn, beta, t0, nsteps, cfl=0.5, t_end=np.inf, order=1):
+def nucleation_cap(source, mu, nu):
+    """maxNucleationTimestep = 1 / sqrt(|sigma|) of the sigma entry of largest
+    magnitude over all nine components of the source's stress
+    (plasticity.cu:190-199, reduceMax = Iamax, plasticity_kernel.cu:1096-1103)."""
+    s = sigma(source, mu, nu)
+    return 1.0 / np.sqrt(np.abs(s).max())
+
+
+def run_cuda(dyn, beta, t0, nsteps, cfl=0.5, t_end=np.inf, order=1, source=None, rate=0.0, dt_cap=None):
     """The CUDA executable's run loop (plasticity.cu:242-271) for a fixed
     number of steps with the LLF flux and forward Euler (TVDstep = simpleTVD,
     plasticity.h:9) or, order 3, TVD3rd; dyn gives Lambda (its 'lambda'),
-    glide-only (NEWGLIDEONLY), elastic constants and the load.
+    glide-only (NEWGLIDEONLY), elastic constants and the load.  source,
+    rate, dt_cap: DYNAMIC_NUCLEATION, every flux gains rate * source
+    (calculateFlux's updateField(rhs, 1/TIME_FINAL, beta0dot),
+    plasticity_kernel.cu:1089-1091) and dt is capped at dt_cap.
     Returns (u, t, dts)."""
     glide_only = dyn.dynamics == GLIDE_ONLY
     lam = 0.0 if glide_only else dyn.Lambda
     N = beta.shape[-1]
     step = cuda_euler_step if order == 1 else cuda_tvd3_step
     t, u, dts = t0, beta, []
+
+    def flux(tt, uu):
+        rhs, speed = cuda_llf_flux(uu, dyn.get_sigma(uu, tt), lam, glide_only)
+        if source is not None:
+            rhs = rhs + rate * source
+        return rhs, speed
     for _ in range(nsteps):
-        u, dt = step(lambda tt, uu: cuda_llf_flux(uu, dyn.get_sigma(uu, tt), lam, glide_only),
-                     t, u, t_end, cfl, N)
+        u, dt = step(flux, t, u, t_end, cfl, N, dt_cap)
         t += dt
         dts.append(dt)
         if t >= t_end:

@@ -82,6 +82,32 @@ def test_run_header_records_match_oracle(R, tmp_path, dim, N, method):
     assert abs(t - want[-1][0]) <= 1e-13 * t and rel_l2(u.cpu().numpy(), want[-1][1]) < 1e-11
 
 
+@pytest.mark.parametrize("dim,N,method", [(2, 16, "gcd"), (3, 16, "lvp")])
+def test_run_header_dynamic_nucleation(R, tmp_path, dim, N, method):
+    """DYNAMIC_NUCLEATION: the input field is the source beta0dot (every flux
+    gains beta0dot / TIME_FINAL), the state starts at zero, the time step is
+    capped at 1 / sqrt(max |sigma(beta0dot)|); the t = 0 record holds the
+    input field (plasticity.cu:178-201,232-233)."""
+    say("run_header nucleation", dim, N, method)
+    src = SpectralGaussianField((N,) * dim, 2.0, 0.01, seed=43)
+    od = O.Dynamics(O.GLIDE_ONLY if method == "lvp" else O.GLIDE_CLIMB, 0.0, MU, NU)
+    cap = O.nucleation_cap(src, MU, NU)
+    inc, end = 0.7 * cap, 2.0 * cap
+    np.stack([_reverse_axes(src[c]) for c in range(9)]).tofile(str(tmp_path / "run.ics"))
+    h = _relax_header(R, dim, N, method, inc, end)
+    R.header_dynamic_nucleation(h)
+    t, u = R.run_header(h, directory=str(tmp_path))
+    got = read_plas(str(tmp_path / "run.plas"), N, dim)
+    assert got[0][0] == 0.0 and np.array_equal(got[0][1], src)
+    want, tw, uw = [(0.0, src)], 0.0, np.zeros_like(src)
+    while tw < end:
+        uw, tw, dts = O.run_cuda(od, uw, tw, 10 ** 6, CFL, t_end=tw + inc, source=src, rate=1.0 / end, dt_cap=cap)
+        want.append((tw, uw))
+    assert any(d == cap for d in dts) or len(want) > 2
+    _compare(got, want)
+    assert rel_l2(u.cpu().numpy(), want[-1][1]) < 1e-11
+
+
 def test_run_header_continues_an_existing_record_file(R, tmp_path):
     """CONTINUE_RUN: a second run with a later TIME_FINAL resumes from the last
     record and appends; the file equals one uninterrupted run's."""

@@ -140,3 +140,19 @@ def test_tvd3_step_is_the_python_rk3_with_the_executables_dt():
     u5, dt5 = O.cuda_tvd3_step(lin, 0.0, b, np.inf, 0.5, 16)
     z = k * dt5
     assert relmax(u5, b * (1 - z + z * z / 2 - z ** 3 / 6)) < 1e-14
+
+
+def test_nucleation_step_from_zero_is_the_capped_source():
+    """DYNAMIC_NUCLEATION from the zero state: the flux of zero vanishes (all
+    speeds are zero), so one Euler step is exactly dt * rate * source with
+    dt = maxNucleationTimestep = 1 / sqrt(max |sigma(source)|)."""
+    from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
+    src = SpectralGaussianField((16, 16), 2.0, 0.01, seed=44)
+    dyn = O.Dynamics(O.GLIDE_CLIMB, 0.0, 0.5, 0.3)
+    cap = O.nucleation_cap(src, 0.5, 0.3)
+    s = O.sigma(src, 0.5, 0.3)
+    assert cap == 1.0 / np.sqrt(np.abs(s).max()) and 1.0 < cap < 1e3
+    rate = 1.0 / 7.0
+    u, t, dts = O.run_cuda(dyn, np.zeros_like(src), 0.0, 1, 0.5, source=src, rate=rate, dt_cap=cap)
+    assert list(dts) == [cap] and t == cap
+    np.testing.assert_array_equal(u, cap * (rate * src))
