@@ -40,7 +40,12 @@ class BetaP_VacancyDynamics(GpuBetaPDynamics):
             p = L.Plan(len(state.gridShape), n, device=dev.index or 0, mu=state.mu, nu=state.nu, cfl=cfl,
                        rk_order=rk_order, **kw)
             self._plans[key] = p
-        p.set_vacancy(self.alpha, self.beta, self.gamma, dt_bound)
+        import numpy as np
+        import torch
+        a = self.alpha
+        field = torch.is_tensor(a) or (isinstance(a, np.ndarray) and a.ndim > 0)
+        p.set_vacancy(1.0 if field else float(a), self.beta, self.gamma, dt_bound)
+        p.vacancy_alpha(a if field else None)  # alpha may be a field, as alpha * cfield allows
         p.bind_stream()
         return p
 
@@ -48,8 +53,12 @@ class BetaP_VacancyDynamics(GpuBetaPDynamics):
         """The state's stress with sigma_ii -= alpha c (:24-28)."""
         p = self._vacancy_plan(state)
         sig = p.sigma(state.betaP.data, time)
+        a = self.alpha
+        if not isinstance(a, (int, float)):
+            import torch
+            a = torch.as_tensor(a, dtype=torch.float64, device=sig.device)
         for k in (0, 4, 8):
-            sig[k] -= self.alpha * cfield
+            sig[k] -= a * cfield
         return TensorField(state.gridShape, data=sig)
 
     def CalculateFlux(self, time, betaPstate, cfield, CFLCondition=False):

@@ -50,11 +50,20 @@ class VacancyState(PlasticityState):
         """sigma of the state with -alpha c on the diagonal (:186-195)."""
         sig = self.CalculateSigma(type=type, source=source)
         c = self.betaP_V[S] if cfield is None else cfield
+        a = self._alpha_like(sig.data)
         for k in (0, 4, 8):
-            sig.data[k] -= self.alpha * c
+            sig.data[k] -= a * c
         return sig
+
+    def _alpha_like(self, t):
+        """alpha as a number, or an alpha field as a tensor beside t."""
+        a = self.alpha
+        if isinstance(a, (int, float)):
+            return a
+        import torch
+        return torch.as_tensor(a, dtype=torch.float64, device=t.device)
 
     def CalculateVacancyEnergy(self):
         """Energy density 1/2 alpha c^2 (:198-199)."""
         c = self.betaP_V[S]
-        return 0.5 * self.alpha * c * c
+        return 0.5 * self._alpha_like(c) * c * c

@@ -60,6 +60,7 @@ EXPORTS = [
     "cdd_segment_vjp_stored", "cdd_set_vacancy", "cdd_vacancy_set", "cdd_vacancy_get",
     "cdd_set_disorder", "cdd_disorder_get", "cdd_disorder_sigma", "cdd_disorder_flux", "cdd_sigma_noalloc",
     "cdd_flux_frozen", "cdd_flux_tangent", "cdd_set_source",
+    "cdd_vacancy_alpha",
 ]
 KERNEL_KINDS = ["r2c", "cols_fwd", "contract", "cols_inv", "c2r", "flux_stage1", "cfl_reduce",
                 "flux_stage2", "flux_stage3", "advance"]
@@ -127,6 +128,7 @@ def _load():
         "cdd_lambda_bar": ([P, dp, I], I),
         "cdd_set_vacancy": ([P, I, D, D, D, D], I),
         "cdd_vacancy_set": ([P, P, I], I),
+        "cdd_vacancy_alpha": ([P, P, I], I),
         "cdd_vacancy_get": ([P, P, I], I),
         "cdd_set_disorder": ([P, P, I, I], I),
         "cdd_set_source": ([P, P, I, D, D], I),
@@ -398,6 +400,19 @@ class Plan:
             raise CDDError("c must have the grid shape %s" % (self.shape,))
         t = t.to(self.dev).contiguous()
         check(lib.cdd_vacancy_set(self.h, ctypes.c_void_p(t.data_ptr()), 1))
+
+    def vacancy_alpha(self, alpha):
+        """alpha as a field (the grid shape) in the vacancies' stress
+        -alpha c; None returns to the scalar of set_vacancy."""
+        if alpha is None:
+            check(lib.cdd_vacancy_alpha(self.h, None, 0))
+            return
+        t = self.torch.as_tensor(alpha, dtype=self.torch.float64)
+        if tuple(t.shape) != self.shape:
+            raise CDDError("alpha must have the grid shape %s" % (self.shape,))
+        t = t.to(self.dev).contiguous()
+        check(lib.cdd_vacancy_alpha(self.h, ctypes.c_void_p(t.data_ptr()), 1))
+        self.torch.cuda.current_stream(self.dev).synchronize()
 
     def vacancy_get(self):
         """The vacancy concentration field as a device tensor."""

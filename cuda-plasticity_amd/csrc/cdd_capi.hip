@@ -85,6 +85,7 @@ struct cdd_plan {
   // vacancy coupling (cdd_set_vacancy): the concentration field and the
   // operator-splitting mover's parameters
   double* vac;
+  double* valpha_f;  // optional alpha field (cdd_vacancy_alpha)
   int vacancy;
   double valpha, vbeta, vgamma, dt_bound;
   // disorder (cdd_set_disorder): the field d, advected by the flux
@@ -319,6 +320,7 @@ int cdd_plan_destroy(cdd_plan* p) {
   (void)hipFree(p->dis_F1);
   (void)hipFree(p->dsig);
   (void)hipFree(p->nuc_src);
+  (void)hipFree(p->valpha_f);
   if (p->own) (void)hipStreamDestroy(p->own);
   if (p->ev) {
     for (hipEvent_t e : *p->ev) (void)hipEventDestroy(e);
@@ -420,6 +422,7 @@ static void vac_stress(const cdd_plan* p, SpectralJob& j) {
   if (p->vacancy) {
     j.vac = p->vac;
     j.valpha = p->valpha;
+    j.valpha_f = p->valpha_f;
   }
 }
 
@@ -788,6 +791,24 @@ int cdd_set_source(cdd_plan* p, const void* src, int on_device, double rate, dou
   if (!on_device) HIPCHK(hipStreamSynchronize(p->stream));
   p->nuc_rate = rate;
   p->nuc_cap = dt_cap;
+  return CDD_OK;
+}
+
+int cdd_vacancy_alpha(cdd_plan* p, const void* alpha, int on_device) {
+  if (!p) return fail(CDD_ERR_ARG, "null plan");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  if (!alpha) {
+    HIPCHK(hipStreamSynchronize(p->stream));
+    (void)hipFree(p->valpha_f);
+    p->valpha_f = nullptr;
+    return CDD_OK;
+  }
+  if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "the vacancy coupling is float64 only");
+  const size_t b = sizeof(double) * (size_t)p->cells;
+  if (!p->valpha_f && hipMalloc((void**)&p->valpha_f, b) != hipSuccess)
+    return fail(CDD_ERR_HIP, "hipMalloc failed (alpha field)");
+  HIPCHK(hipMemcpyAsync(p->valpha_f, alpha, b, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, p->stream));
+  if (!on_device) HIPCHK(hipStreamSynchronize(p->stream));
   return CDD_OK;
 }
 

@@ -61,6 +61,7 @@ struct SpectralJob {
   // -valpha * vac (the vacancies' chemical stress)
   const double* vac;
   double valpha;
+  const double* valpha_f;  // optional: alpha as a field (one plane), in place of valpha
 };
 
 hipError_t launch_spectral(const SpectralJob& job);

@@ -176,7 +176,7 @@ template <typename T, int N>
 __global__ void __launch_bounds__(Tiles<N>::TY * Tiles<N>::TT)
 k_c2r_last(const cx<T>* __restrict__ spec, T* __restrict__ out, long long cs, const cx<T>* __restrict__ tw,
            int NO, int NC, const Clock* clk, long long sstride, long long ostride, const T* __restrict__ vac,
-           T valpha) {
+           T valpha, const T* __restrict__ valpha_f) {
   using TL = Tiles<N>;
   constexpr int TT = TL::TT, TY = TL::TY, P = TL::P, V = FFTPlan<N>::V, H = N / 2 + 1;
   const int inst = (NO == 1) ? (int)blockIdx.y : 0;
@@ -225,10 +225,14 @@ k_c2r_last(const cx<T>* __restrict__ spec, T* __restrict__ out, long long cs, co
   const long long rowbase = ((long long)o * N + (r0 + r)) * N;
   if (vac && (ca < 3 || (hasb && cb < 3))) {
     // vacancy coupling of the stepper's stress solves: the diagonal stress
-    // planes (xx, yy, zz) get -alpha c (VacancyDynamics.py:24-28)
+    // planes (xx, yy, zz) get -alpha c (VacancyDynamics.py:24-28), alpha a
+    // constant or a field
     T cv[V];
 #pragma unroll
-    for (int i = 0; i < V; ++i) cv[i] = valpha * vac[rowbase + fft_out_index<N>(j, i)];
+    for (int i = 0; i < V; ++i) {
+      const long long idx = rowbase + fft_out_index<N>(j, i);
+      cv[i] = (valpha_f ? valpha_f[idx] : valpha) * vac[idx];
+    }
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const long long idx = rowbase + fft_out_index<N>(j, i);
@@ -604,7 +608,8 @@ static hipError_t launch_spectral_t(const SpectralJob& job) {
   }
   if (job.vac && (NC != 6 || sizeof(T) != 8)) return hipErrorInvalidValue;
   hipLaunchKernelGGL((k_c2r_last<T, N>), g1, b1, 0, st, spec, out, cs, tw, NO, NC, job.gp.clk, job.sstride,
-                     job.ostride, reinterpret_cast<const T*>(job.vac), T(job.valpha));
+                     job.ostride, reinterpret_cast<const T*>(job.vac), T(job.valpha),
+                     reinterpret_cast<const T*>(job.valpha_f));
   hook(KK_C2R);
   return hipGetLastError();
 }

@@ -177,6 +177,11 @@ int cdd_step(cdd_plan* plan, int nsteps);
  *   doubles, the VacancyState's ('s','s') component), host or device. */
 int cdd_set_vacancy(cdd_plan* plan, int enable, double alpha, double beta, double gamma, double dt_bound);
 int cdd_vacancy_set(cdd_plan* plan, const void* c, int on_device);
+/* alpha as a field (N^d float64 values, host or device) in place of the
+ * scalar of cdd_set_vacancy: sigma_ii -= alpha(x) c(x) (the dynamics'
+ * alpha * cfield, VacancyDynamics.py:24-28, with an array alpha).  NULL
+ * returns to the scalar. */
+int cdd_vacancy_alpha(cdd_plan* plan, const void* alpha, int on_device);
 int cdd_vacancy_get(cdd_plan* plan, void* c, int on_device);
 
 /* The CUDA executable's DYNAMIC_NUCLEATION (plasticity.cu:178-201,

@@ -152,3 +152,39 @@ def test_reference_api_vacancy_mover(L):
     rhs, speed = dyn.CalculateFlux(0.0, bstate, cf, CFLCondition=True)
     ref = O.central_upwind_flux(beta, O.vacancy_sigma(odyn, vac, beta, c, 0.0), O.GLIDE_CLIMB, LAM)
     assert rel(rhs.numpy(), ref[0]) < 1e-12
+
+
+@pytest.mark.parametrize("shape,order", [((32, 32), 3), ((32, 32, 32), 2)])
+def test_vacancy_alpha_field_matches_oracle(L, shape, order):
+    """alpha as a field (the dynamics' alpha * cfield with an array alpha,
+    VacancyDynamics.py:24-28): the GPU run and the reference-API mover
+    against the oracle with the same field."""
+    from Plasticity.PlasticityStates.VacancyState import VacancyState
+    from Plasticity.FieldDynamics.VacancyDynamics import BetaP_VacancyDynamics
+    from Plasticity.FieldMovers.FieldMover import OperatorSplittingTVDRK_FieldMover
+    from Plasticity.PlasticitySystem import PlasticitySystem
+    from Plasticity.Observers.Observer import RecallStateObserver
+    say("vacancy alpha field", shape, order)
+    beta, c = _fields(shape, 73)
+    alpha = ALPHA * (1.0 + 0.5 * SpectralGaussianField(shape, 3.0, 1.0, seed=75)[4])
+    dyn = O.Dynamics(O.GLIDE_CLIMB, LAM, MU, NU)
+    ru, rc, rt, rdts = O.run_vacancy(dyn, O.Vacancy(alpha, VBETA, GAMMA), beta, c, 0.0, 5, 0.25, order=order)
+    p = _plan(L, shape, order)
+    p.vacancy_alpha(alpha)
+    p.set_state(beta)
+    p.vacancy_set(c)
+    p.set_time(0.0)
+    p.step(5)
+    assert rel(p.get_state().cpu().numpy(), ru) < 1e-11 and rel(p.vacancy_get().cpu().numpy(), rc) < 1e-11
+    # the scalar run differs (the field matters)
+    su, _, _, _ = O.run_vacancy(dyn, O.Vacancy(ALPHA, VBETA, GAMMA), beta, c, 0.0, 5, 0.25, order=order)
+    assert rel(su, ru) > 1e-8
+    # through the reference API: the dynamics' alpha field
+    state = VacancyState(shape, field=np.concatenate([c[None], beta]), mu=MU, nu=NU)
+    obs = RecallStateObserver()
+    mover = OperatorSplittingTVDRK_FieldMover(CFLsafeFactor=0.25, dtBound=None, order=order)
+    PlasticitySystem(shape, state, mover, BetaP_VacancyDynamics(Lambda=LAM, alpha=alpha, beta=VBETA, gamma=GAMMA),
+                     [obs]).Run(startTime=0.0, endTime=rt * (1 - 1e-12))
+    assert mover.numberOfSteps == 5
+    got = obs.state.GetOrderParameterField().numpy()
+    assert rel(got[1:], ru) < 1e-11 and rel(got[0], rc) < 1e-11

@@ -98,3 +98,14 @@ def test_zero_coupling_is_the_plain_run():
     ur, tr, dtr = O.run(dyn, beta, 0.0, 3, 0.25)
     assert np.array_equal(u, ur) and t == tr
     assert np.max(np.abs(c1 - c)) < 1e-16
+
+
+def test_constant_alpha_field_equals_scalar_alpha():
+    """An alpha field that is constant is the scalar alpha, bit for bit (the
+    reference's alpha * cfield broadcasts either)."""
+    beta = SpectralGaussianField((16, 16), 2.0, 0.01, seed=81)
+    c = 0.01 * SpectralGaussianField((16, 16), 2.0, 1.0, seed=82)[0]
+    dyn = O.Dynamics(O.GLIDE_CLIMB, 0.5, 1.0, 0.3)
+    a = O.run_vacancy(dyn, O.Vacancy(0.7, 1.0, 0.05), beta, c, 0.0, 3, 0.25)
+    b = O.run_vacancy(dyn, O.Vacancy(np.full((16, 16), 0.7), 1.0, 0.05), beta, c, 0.0, 3, 0.25)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[3], b[3])
