@@ -34,6 +34,15 @@ def convertStateToCUDA(state, filename):
     np.stack([_reverse_axes(np.asarray(data[c], dtype=np.float64)) for c in range(9)]).tofile(filename)
 
 
+def read_raw_fields(filename, N, dim, ncomp):
+    """The ncomp components of a raw .ics file (no time, every component with
+    its grid axes reversed) in the NumPy layout."""
+    raw = np.fromfile(filename, dtype=np.float64)
+    if raw.size != ncomp * N ** dim:
+        raise ValueError("%s holds %d values, not %d components of %d^%d" % (filename, raw.size, ncomp, N, dim))
+    return np.stack([_reverse_axes(c) for c in raw.reshape((ncomp,) + (N,) * dim)])
+
+
 def convertCUDAToState(filename, N, dim, time=None):
     """(t, state) of a .plas record (the last, or the first with t >= time)."""
     return LoadStateRaw(filename, N, dim, time=time)
@@ -65,6 +74,8 @@ HEADER_KEYS = {
     "FILE_OUTPUT": "string",
     "TIME_STEP": "number",        # record spacing
     "TIME_FINAL": "number",
+    "vacancycost": "number",      # VACANCIES: sigma_ii -= vacancycost c
+    "vacancydiffusion": "number",  # VACANCIES: c diffuses by exp(-vacancydiffusion dt k^2)
 }
 # the CFL factor RunCUDA compiles every run with (RunCUDA.py:54-58)
 EXECUTABLE_CFL = 0.5
@@ -219,7 +230,12 @@ def header_times(header, step, end):
 
 # ------------------------------------------------------------ the executable
 # Compile-time options of the executable that this runner does not carry out.
-UNSUPPORTED = ("VACANCIES", "DEBUG_TIMESTEPS", "SINGLE_STEP_DEBUG")
+UNSUPPORTED = ("DEBUG_TIMESTEPS", "SINGLE_STEP_DEBUG")
+# The executable's VACANCIES constants; the reference's headers leave them
+# undefined, so they are read from the header when present and otherwise
+# take the defaults of the Python vacancy dynamics (alpha = gamma = 1,
+# FieldDynamics/VacancyDynamics.py:16-17).
+VACANCY_DEFAULTS = {"vacancycost": 1.0, "vacancydiffusion": 1.0}
 
 
 def _value(v):
@@ -264,6 +280,9 @@ def header_options(header):
         if k in h:
             raise NotImplementedError("the in-process executable does not implement %s" % k)
     glide = "NEWGLIDEONLY" in h
+    if "lambda" not in h:
+        # the reference's 'vac' method sets VACANCIES alone (RunCUDA.py:84-85)
+        raise ValueError("the header defines no lambda: combine VACANCIES with a dynamics method (gcd, mdp, lvp)")
     lam = float(h["lambda"])
     kw = dict(ndim=3 if "DIMENSION3" in h else 2, n=int(h["N"]), dynamics=GLIDE_ONLY if glide else GLIDE_CLIMB,
               Lambda=0.0 if glide else lam, mu=MU, nu=NU, cfl=float(h["CFLsafeFactor"]), scheme="llf")
@@ -313,23 +332,46 @@ def run_header(header, state=None, seed=0, device=0, rk_order=1, directory=".", 
     every flux as beta0dot / TIME_FINAL, the state starts from zero, and the
     time step is capped at 1 / sqrt(|sigma(beta0dot)|) of its largest stress
     entry; the t = 0 record holds the loaded field, as the executable writes
-    its host copy before the first step.  Returns (t, state tensor on the GPU)."""
+    its host copy before the first step.
+
+    VACANCIES (plasticity_kernel.cu:176-196,284-290,676-775): ten
+    components, the tensor and the vacancy concentration c (in the .ics
+    input and every .plas record); the stress loses vacancycost c on the
+    diagonal, c's rhs is the trace of the tensor rhs, and c diffuses by
+    exp(-vacancydiffusion dt k^2) after every forward Euler step.  Returns
+    (t, state tensor on the GPU; VACANCIES: the ten components)."""
     plan, h = header_plan(header, device=device, rk_order=rk_order)
     N, dim = plan.n, plan.ndim
     args = (N, seed, float(h["lambda"]))
     out = os.path.join(directory, _cformat(str(h["FILE_OUTPUT"]), args))
     nucleation = "DYNAMIC_NUCLEATION" in h
+    vacancies = "VACANCIES" in h
+    if vacancies:
+        # ten components: the tensor and the vacancy concentration, whose rhs
+        # is the tensor rhs's trace (the executable's forward Euler step)
+        if nucleation:
+            raise NotImplementedError("VACANCIES with DYNAMIC_NUCLEATION")
+        vc = float(h.get("vacancycost", VACANCY_DEFAULTS["vacancycost"]))
+        vd = float(h.get("vacancydiffusion", VACANCY_DEFAULTS["vacancydiffusion"]))
+        plan.set_vacancy(alpha=vc, beta=1.0, gamma=vd, dt_bound=None)
     t = 0.0
     if os.path.isfile(out) and os.path.getsize(out) > 0:
         t, _, _ = plan.plas_resume(out)
         if verbose:
             print(" Restarting from t=%f" % t)
     else:
-        if state is None:
+        if state is None and vacancies:
+            state = read_raw_fields(os.path.join(directory, _cformat(str(h["FILE_INPUT"]), args)), N, dim, 10)
+        elif state is None:
             _, state = LoadStateRaw(os.path.join(directory, _cformat(str(h["FILE_INPUT"]), args)), N, dim,
                                     hastimes=False)
-        # a PlasticityState, or the nine components as an array or tensor
-        plan.set_state(state.GetOrderParameterField().data if hasattr(state, "GetOrderParameterField") else state)
+        # a PlasticityState, or the nine (VACANCIES: ten) components as an array or tensor
+        data = state.GetOrderParameterField().data if hasattr(state, "GetOrderParameterField") else state
+        if vacancies:
+            plan.set_state(data[:9])
+            plan.vacancy_set(data[9])
+        else:
+            plan.set_state(data)
         plan.set_time(0.0)
         plan.plas_append(out)  # the t = 0 record (plasticity.cu:232-233)
     if nucleation:
@@ -354,6 +396,8 @@ def run_header(header, state=None, seed=0, device=0, rk_order=1, directory=".", 
         if verbose:
             print("%le +%le" % (t, dt))
         plan.plas_append(out)
+    if vacancies:
+        return t, plan.torch.cat([plan.get_state(), plan.vacancy_get()[None]])
     return t, plan.get_state()
 
 

@@ -551,6 +551,13 @@ static int launch_step(cdd_plan* p, const StepBufs& b) {
   if (p->cfg.rk_order == 1) {
     // forward Euler (simpleTVD, plasticity_kernel.cu:1112-1137), in place
     HIPCHK(launch_euler(b.out, p->L0, 9 * p->cells, p->clk, p->stream));
+    if (p->vacancy) {
+      // the executable's VACANCIES: c += dt tr(rhs), then diffusion over the step
+      HIPCHK(launch_vacancy_euler(p->vac, p->L0, p->cells, p->vbeta, p->clk, p->stream));
+      SpectralJob jd = spectral_job(p, 1, p->vac, nullptr, p->clk, 0.0, p->vac);
+      jd.gp.vgamma = p->vgamma;
+      HIPCHK(launch_spectral(jd));
+    }
     HIPCHK(launch_advance(p->clk, p->stream));
     if (p->timing) timing_hook(p, KK_ADVANCE);
     return CDD_OK;
@@ -668,8 +675,9 @@ int cdd_set_vacancy(cdd_plan* p, int enable, double alpha, double beta, double g
     return CDD_OK;
   }
   if (p->fp32) return fail(CDD_ERR_UNSUPPORTED, "the vacancy coupling is float64 only");
-  if (p->cfg.scheme != CDD_SCHEME_CENTRAL_UPWIND)
-    return fail(CDD_ERR_UNSUPPORTED, "the vacancy coupling uses the central-upwind scheme");
+  if (p->cfg.scheme == CDD_SCHEME_LLF && p->cfg.rk_order != 1)
+    return fail(CDD_ERR_UNSUPPORTED,
+                "with the LLF scheme the vacancy coupling follows the executable's forward Euler step (rk_order 1)");
   if (p->disorder) return fail(CDD_ERR_UNSUPPORTED, "the vacancy and disorder couplings do not combine");
   if (p->nuc_src) return fail(CDD_ERR_UNSUPPORTED, "the vacancy coupling excludes the nucleation source");
   if (!(gamma >= 0.0) || !(dt_bound >= 0.0) || !isfinite(alpha) || !isfinite(beta))
@@ -1317,9 +1325,13 @@ int cdd_lambda_bar(cdd_plan* p, double* value, int reset) {
 }
 
 // ------------------------------------------------------------- .plas files
+// .plas records of a plan with the vacancy coupling on carry c (float64 plans)
+static bool vac_record(const cdd_plan* p) { return p->vacancy && p->vac && !p->fp32; }
+
 static int host_alloc(cdd_plan* p) {
   if (p->host) return CDD_OK;
-  const size_t bytes = p->es * (1 + 9 * (size_t)p->cells);
+  // one record: t and ten planes (the vacancy component of VACANCIES records)
+  const size_t bytes = p->es * (1 + 10 * (size_t)p->cells);
   if (hipHostMalloc(&p->host, bytes) != hipSuccess) return fail(CDD_ERR_HIP, "hipHostMalloc failed (record staging)");
   return CDD_OK;
 }
@@ -1334,6 +1346,13 @@ int cdd_plas_append(cdd_plan* p, const char* path) {
   HIPCHK(launch_reverse_axes(p->state[cur_state(p)], p->work9, p->d, p->n, 9, p->fp32, p->stream));
   char* h = static_cast<char*>(p->host);
   HIPCHK(hipMemcpyAsync(h + p->es, p->work9, fb, hipMemcpyDeviceToHost, p->stream));
+  // with the vacancy coupling on, the record carries c as its tenth
+  // component (the executable's VACANCIES records, VACANCY_COMP = 9)
+  const size_t cb = vac_record(p) ? p->es * (size_t)p->cells : 0;
+  if (cb) {
+    HIPCHK(launch_reverse_axes(p->vac, p->L0, p->d, p->n, 1, 0, p->stream));
+    HIPCHK(hipMemcpyAsync(h + p->es + fb, p->L0, cb, hipMemcpyDeviceToHost, p->stream));
+  }
   HIPCHK(hipStreamSynchronize(p->stream));
   if (p->fp32) {
     const float tf = (float)c.t;
@@ -1343,7 +1362,7 @@ int cdd_plas_append(cdd_plan* p, const char* path) {
   }
   FILE* f = fopen(path, "ab");
   if (!f) return fail(CDD_ERR_ARG, (std::string("cannot open ") + path).c_str());
-  const size_t want = p->es + fb;
+  const size_t want = p->es + fb + cb;
   const size_t got = fwrite(h, 1, want, f);
   const int fl = fflush(f);
   const int cl = fclose(f);
@@ -1358,7 +1377,8 @@ int cdd_plas_resume(cdd_plan* p, const char* path, double* t, long long* records
   FILE* f = fopen(path, "rb");
   if (!f) return fail(CDD_ERR_ARG, (std::string("cannot open ") + path).c_str());
   const size_t fb = p->es * 9 * (size_t)p->cells;
-  const long long rec = (long long)(p->es + fb);
+  const size_t cb = vac_record(p) ? p->es * (size_t)p->cells : 0;
+  const long long rec = (long long)(p->es + fb + cb);
   if (fseeko(f, 0, SEEK_END) != 0) {
     fclose(f);
     return fail(CDD_ERR_ARG, "cannot seek in the .plas file");
@@ -1392,6 +1412,10 @@ int cdd_plas_resume(cdd_plan* p, const char* path, double* t, long long* records
   if (sync_clock(p, &c)) return CDD_ERR_HIP;
   HIPCHK(hipMemcpyAsync(p->work9, h + p->es, fb, hipMemcpyHostToDevice, p->stream));
   HIPCHK(launch_reverse_axes(p->work9, p->state[cur_state(p)], p->d, p->n, 9, p->fp32, p->stream));
+  if (cb) {
+    HIPCHK(hipMemcpyAsync(p->L0, h + p->es + fb, cb, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(launch_reverse_axes(p->L0, p->vac, p->d, p->n, 1, 0, p->stream));
+  }
   HIPCHK(hipStreamSynchronize(p->stream));
   c.t = tr;
   c.done = (tr >= c.t_end) ? 1 : 0;

@@ -1564,6 +1564,26 @@ hipError_t launch_euler(double* u, const double* L, long long n, const Clock* cl
   return hipGetLastError();
 }
 
+// The executable's vacancy component under forward Euler (VACANCIES,
+// plasticity_kernel.cu:692-697 and the axpy of simpleTVD): its rhs is the
+// trace of the tensor rhs, c += vbeta dt (L_xx + L_yy + L_zz).
+__global__ void __launch_bounds__(256) k_vacancy_euler(double* __restrict__ c, const double* __restrict__ L,
+                                                       long long cells, double vbeta, const Clock* clk) {
+  if (clk->done) return;
+  const double dt = clk->dt;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < cells;
+       i += (long long)gridDim.x * blockDim.x)
+    c[i] += vbeta * (dt * ((L[i] + L[4 * cells + i]) + L[8 * cells + i]));
+}
+
+hipError_t launch_vacancy_euler(double* c, const double* L, long long cells, double vbeta, const Clock* clk,
+                                hipStream_t st) {
+  long long blocks = (cells + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_vacancy_euler, dim3((unsigned)blocks), dim3(256), 0, st, c, L, cells, vbeta, clk);
+  return hipGetLastError();
+}
+
 hipError_t launch_advance(Clock* clk, hipStream_t st, int count) {
   if (count < 1 || count > 1024) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(count), 0, st, clk, count);

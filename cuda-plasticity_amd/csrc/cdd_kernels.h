@@ -123,6 +123,9 @@ hipError_t launch_finalize_dt(const double* partial, int nblocks, Clock* clk, do
 hipError_t launch_advance(Clock* clk, hipStream_t st, int count = 1);
 // forward Euler update u += dt L with the clock's dt (skipped once done)
 hipError_t launch_euler(double* u, const double* L, long long n, const Clock* clk, hipStream_t st);
+// the vacancy component's Euler step: c += vbeta dt tr(L)
+hipError_t launch_vacancy_euler(double* c, const double* L, long long cells, double vbeta, const Clock* clk,
+                                hipStream_t st);
 // out = u + dt L with the clock's dt (out may alias L; skipped once done)
 hipError_t launch_axpy_dt(double* out, const double* u, const double* L, long long n, const Clock* clk,
                           hipStream_t st);

@@ -268,7 +268,9 @@ int cdd_instability(cdd_plan* plan, const void* beta_dev, double* value);
  * (plasticity.cu:107-125): it loads the last complete record into the plan's
  * state and clock; unlike the reference's seek-from-end it checks that the
  * file holds whole records and skips a torn tail left by an interrupted
- * append (*torn = 1; *records = complete records; both nullable). */
+ * append (*torn = 1; *records = complete records; both nullable).  With the
+ * vacancy coupling on, each record carries the concentration c as a tenth
+ * component (the executable's VACANCIES records, [t, 10 N^d]). */
 int cdd_plas_append(cdd_plan* plan, const char* path);
 int cdd_plas_resume(cdd_plan* plan, const char* path, double* t, long long* records, int* torn);
 

@@ -1192,3 +1192,34 @@ def run_cuda(dyn, beta, t0, nsteps, cfl=0.5, t_end=np.inf, order=1, source=None,
         if t >= t_end:
             break
     return u, t, np.array(dts)
+
+
+def run_cuda_vacancy(dyn, vac, beta, c, t0, nsteps, cfl=0.5, t_end=np.inf):
+    """The CUDA executable's VACANCIES variant under its simpleTVD step
+    (plasticity_kernel.cu:176-196,284-290,676-699,700-775,1112-1137): the
+    stress at every cell loses vacancycost c on the diagonal (after the
+    load), the tensor rhs is the LLF flux, the vacancy component's rhs is
+    the trace of the tensor rhs, u and c advance by dt times their rhs, and
+    c then diffuses by exp(-vacancydiffusion dt k^2) (calculateKDiffusion).
+    vac.alpha is vacancycost, vac.gamma vacancydiffusion.  Returns
+    (u, c, t, dts)."""
+    glide_only = dyn.dynamics == GLIDE_ONLY
+    lam = 0.0 if glide_only else dyn.Lambda
+    N = beta.shape[-1]
+    t, u, dts = t0, beta, []
+    for _ in range(nsteps):
+        sig = dyn.get_sigma(u, t).copy()
+        for i in range(3):
+            sig[cidx(i, i)] = sig[cidx(i, i)] - vac.alpha * c
+        rhs, speed = cuda_llf_flux(u, sig, lam, glide_only)
+        dt = cfl / speed.max() / N
+        if t + dt > t_end:
+            dt = t_end - t + ME
+        u = u + dt * rhs
+        c = c + dt * ((rhs[0] + rhs[4]) + rhs[8])
+        c = vacancy_diffuse(c, dt, vac.gamma)
+        t += dt
+        dts.append(dt)
+        if t >= t_end:
+            break
+    return u, c, t, np.array(dts)

@@ -108,6 +108,40 @@ def test_run_header_dynamic_nucleation(R, tmp_path, dim, N, method):
     assert rel_l2(u.cpu().numpy(), want[-1][1]) < 1e-11
 
 
+@pytest.mark.parametrize("dim,N", [(2, 16), (3, 16)])
+def test_run_header_vacancies(R, tmp_path, dim, N):
+    """VACANCIES: ten-component .ics input and .plas records (the tensor and
+    c); the stress loses vacancycost c on the diagonal, c's rhs is the trace
+    of the tensor rhs, c diffuses after every forward Euler step; a resumed
+    run continues from the last ten-component record."""
+    say("run_header vacancies", dim, N)
+    beta = SpectralGaussianField((N,) * dim, 2.0, 0.01, seed=47)
+    c = 0.01 * SpectralGaussianField((N,) * dim, 2.0, 1.0, seed=48)[0]
+    od = O.Dynamics(O.GLIDE_CLIMB, 1.0, MU, NU)
+    vac = O.Vacancy(0.6, 1.0, 0.05)
+    _, _, _, dts = O.run_cuda_vacancy(od, vac, beta, c, 0.0, 1, CFL)
+    inc = 2.5 * dts[0]
+    fields = np.concatenate([beta, c[None]])
+    np.stack([_reverse_axes(f) for f in fields]).tofile(str(tmp_path / "run.ics"))
+    h = _relax_header(R, dim, N, "mdp", inc, 1.5 * inc)
+    R.header_dynamics(h, "vac")
+    h.update(vacancycost=0.6, vacancydiffusion=0.05)
+    R.run_header(h, directory=str(tmp_path))
+    h["TIME_FINAL"] = 3.5 * inc
+    t, u = R.run_header(h, directory=str(tmp_path))
+    raw = np.fromfile(str(tmp_path / "run.plas"), dtype=np.float64).reshape(-1, 1 + 10 * N ** dim)
+    got = [(r[0], np.stack([_reverse_axes(f) for f in r[1:].reshape((10,) + (N,) * dim)])) for r in raw]
+    want, tw, uw, cw = [(0.0, fields)], 0.0, beta, c
+    while tw < 3.5 * inc:
+        uw, cw, tw, _ = O.run_cuda_vacancy(od, vac, uw, cw, tw, 10 ** 6, CFL, t_end=tw + inc)
+        want.append((tw, np.concatenate([uw, cw[None]])))
+    assert len(got) == len(want) and np.array_equal(got[0][1], fields)
+    for (tg, ug), (tw_, uw_) in zip(got[1:], want[1:]):
+        assert abs(tg - tw_) <= 1e-13 * tw_
+        assert rel_l2(ug[:9], uw_[:9]) < 1e-11 and rel_l2(ug[9], uw_[9]) < 1e-11
+    assert rel_l2(u.cpu().numpy(), want[-1][1]) < 1e-11
+
+
 def test_run_header_continues_an_existing_record_file(R, tmp_path):
     """CONTINUE_RUN: a second run with a later TIME_FINAL resumes from the last
     record and appends; the file equals one uninterrupted run's."""

@@ -156,3 +156,24 @@ def test_nucleation_step_from_zero_is_the_capped_source():
     u, t, dts = O.run_cuda(dyn, np.zeros_like(src), 0.0, 1, 0.5, source=src, rate=rate, dt_cap=cap)
     assert list(dts) == [cap] and t == cap
     np.testing.assert_array_equal(u, cap * (rate * src))
+
+
+def test_vacancy_variant_closed_forms():
+    """The executable's VACANCIES step: with no vacancy cost it advances the
+    tensor exactly as run_cuda, the concentration gains dt tr(rhs) before
+    diffusing (diffusion keeps the mean: the total changes by dt sum tr(rhs));
+    a uniform concentration acts as the uniform stress -vacancycost c."""
+    from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
+    beta = SpectralGaussianField((16, 16), 2.0, 0.01, seed=45)
+    c = 0.01 * SpectralGaussianField((16, 16), 2.0, 1.0, seed=46)[0]
+    dyn = O.Dynamics(O.GLIDE_CLIMB, 0.3, 0.5, 0.3)
+    u0, t0, d0 = O.run_cuda(dyn, beta, 0.0, 1, 0.5)
+    u, c1, t, dts = O.run_cuda_vacancy(dyn, O.Vacancy(0.0, 1.0, 0.2), beta, c, 0.0, 1, 0.5)
+    assert np.array_equal(u, u0) and list(dts) == list(d0)
+    rhs = (u0 - beta) / d0[0]
+    assert abs(c1.sum() - (c.sum() + d0[0] * (rhs[0] + rhs[4] + rhs[8]).sum())) < 1e-12 * abs(c).sum()
+    cu = np.full((16, 16), 0.004)
+    a = O.run_cuda_vacancy(dyn, O.Vacancy(0.7, 1.0, 0.2), beta, cu, 0.0, 1, 0.5)
+    b = O.run_cuda(O.Dynamics(O.GLIDE_CLIMB, 0.3, 0.5, 0.3, O.Load("stress", initial=(-0.7 * 0.004,) * 3)), beta,
+                   0.0, 1, 0.5)
+    assert np.array_equal(a[0], b[0]) and list(a[3]) == list(b[2])

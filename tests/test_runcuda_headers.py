@@ -80,8 +80,11 @@ def test_unsupported_options_raise():
     kw, hh = R.header_options(h)  # dynamic nucleation is carried out (run_header)
     assert "DYNAMIC_NUCLEATION" in hh
     h = _header(method="vac")
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(ValueError, match="lambda"):
         R.header_options(h)
+    h = _header(method="mdp")
+    R.header_dynamics(h, "vac")
+    assert "VACANCIES" in R.header_options(h)[1]
     h = _header()
     h["LOAD_DEF"] = [[0, 1, 0], [0, 0, 0], [0, 0, 0]]
     with pytest.raises(NotImplementedError):
