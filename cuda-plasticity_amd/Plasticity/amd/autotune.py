@@ -1,24 +1,42 @@
-"""Autotuner for the batched stepper: picks the flux tile shape (and hence
-the halo-to-interior ratio of the LDS tile) per grid-size class.
+"""Autotuner for the batched stepper.
 
-Candidates are the tile variants compiled into the library (0: 32x8,
-1: 64x4, 2: 16x16 cells of the two fastest axes).  Each candidate is timed
-on the batch's actual instances of that size with HIP events
-(cdd_batch_time_flux); the fastest is installed with cdd_batch_set_tile.
-Choices are cached per (device name, side, dynamics) in a JSON file so a
+Two choices are timed with HIP events on the batch's own instances and
+states (no state change) and installed:
+  * per size class, the flux tile of the per-size launch (0: 32x8, 1: 64x4,
+    2: 16x16 cells of the two fastest axes; cdd_batch_time_flux,
+    cdd_batch_set_tile) -- the flux of sizes outside the multi-instance
+    launch, and of the per-size mode;
+  * the persistent multi-instance flux launch (one launch per stage over
+    every instance of every size): its tile variant and persistent blocks per
+    CU (1 or 2), against the per-size launches as a whole
+    (cdd_batch_time_flux_multi, cdd_batch_set_flux; per_cu 0 selects the
+    per-size launches).
+Choices are cached per (device name, sizes, dynamics) in a JSON file so a
 later run can reuse them without timing.
 """
 import json
 import os
 
 VARIANTS = {0: "32x8", 1: "64x4", 2: "16x16"}
+TILE_SHAPE = {0: (32, 8), 1: (64, 4), 2: (16, 16)}
+PER_CU = (1, 2)
 
 
 def _valid(n, v):
     return {0: n >= 32, 1: n >= 64, 2: n >= 16}[v]
 
 
+def _multi_ok(n, v):
+    """Whether a group of side n joins the multi-instance launch of tile v."""
+    tc, tr = TILE_SHAPE[v]
+    return n >= tc and n >= tr and n % tc == 0 and n % tr == 0
+
+
 def autotune_batch(batch, reps=5, cache_path=None, dynamics=None, verbose=False):
+    """Returns (choices, timings): choices[n] is size n's per-size tile and
+    choices["multi"] = (tile, per_cu) the multi-instance launch's
+    configuration (per_cu 0: per-size launches); timings holds the measured
+    ms per launch of every candidate."""
     import torch
     name = torch.cuda.get_device_name(batch.dev)
     cache = {}
@@ -28,7 +46,9 @@ def autotune_batch(batch, reps=5, cache_path=None, dynamics=None, verbose=False)
         except ValueError:
             cache = {}
     choices, timings = {}, {}
-    for n in sorted(set(batch.sizes)):
+    sides = sorted(set(batch.sizes))
+    best_ms = {}
+    for n in sides:
         key = "%s|%d|%s" % (name, n, dynamics)
         if key in cache:
             best = int(cache[key])
@@ -36,11 +56,34 @@ def autotune_batch(batch, reps=5, cache_path=None, dynamics=None, verbose=False)
             t = {v: batch.time_flux(n, v, reps) for v in VARIANTS if _valid(n, v)}
             timings[n] = t
             best = min(t, key=t.get)
+            best_ms[n] = t[best]
             cache[key] = best
         batch.set_tile(n, best)
         choices[n] = best
         if verbose:
             print("autotune n=%d -> %s %s" % (n, VARIANTS[best], timings.get(n, "(cached)")))
+    counts = {n: batch.sizes.count(n) for n in sides}
+    mkey = "%s|multi|%s|%s" % (name, ",".join("%dx%d" % (n, counts[n]) for n in sides), dynamics)
+    if mkey in cache:
+        tile, per_cu = (int(x) for x in cache[mkey])
+    else:
+        cand = {}
+        for v in VARIANTS:
+            if not any(_multi_ok(n, v) for n in sides):
+                continue
+            # sizes outside this tile's multi launch keep their per-size launch
+            rest = sum(best_ms.get(n, 0.0) for n in sides if not _multi_ok(n, v))
+            for p in PER_CU:
+                cand[(v, p)] = batch.time_flux_multi(v, p, reps) + rest
+        if len(best_ms) == len(sides):
+            cand[(0, 0)] = sum(best_ms.values())  # the per-size launches one after another
+        timings["multi"] = {"%d/%d" % k: ms for k, ms in cand.items()}
+        tile, per_cu = min(cand, key=cand.get) if cand else (0, 0)
+        cache[mkey] = [tile, per_cu]
+    batch.set_flux(tile, per_cu)
+    choices["multi"] = (tile, per_cu)
+    if verbose:
+        print("autotune multi -> tile %s, %d blocks/CU %s" % (VARIANTS[tile], per_cu, timings.get("multi", "(cached)")))
     if cache_path:
         with open(cache_path, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)

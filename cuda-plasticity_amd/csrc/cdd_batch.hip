@@ -1,17 +1,24 @@
 // Batched stepping of many independent 2D realisations of uneven sizes
 // (include/cdd.h, cdd_batch_*).  The reference runs one realisation per
 // process per GPU (Plasticity/exec_batch.py:26-58); here all of them advance
-// together: instances are grouped by grid side n, every group's buffers have
-// uniform per-instance strides so each kernel launch covers all instances of
-// a size (instance = blockIdx.y), each instance keeps its own device clock
-// (own CFL dt and end time), and one full step of every group is captured
-// once into a hipGraph -- each size group on its own forked capture stream,
-// so the groups are independent branches that run concurrently -- and
-// replayed for every step.
+// together.  Instances are grouped by grid side n; every group's buffers
+// have uniform per-instance strides and each instance keeps its own device
+// clock (own CFL dt and end time).  One step is captured once into a
+// hipGraph and replayed for every step:
+//   per stage: the spectral stress passes of every size group, each group on
+//   its own forked capture stream (independent branches that run
+//   concurrently; launch = all instances of the size, instance =
+//   blockIdx.y), joined into ONE persistent multi-instance flux launch that
+//   covers every instance of every size (k_flux_multi: a fixed grid walking
+//   the flat (size, instance, tile) list); after the first stage the CFL
+//   reductions of every instance, after the last the clock advance.
+// Groups whose side the flux tile does not divide, and the per-size mode
+// (cdd_batch_set_flux with per_cu = 0), launch the per-size k_flux instead.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -54,6 +61,8 @@ struct Group {
 struct cdd_batch {
   cdd_config cfg;
   GreenParams gp0;  // mu/nu/lam/A/C (norm per group)
+  int flux_tile = 0;    // tile variant of the multi-instance flux
+  int flux_per_cu = 2;  // its persistent blocks per CU; 0: per-size k_flux launches
   std::vector<Group> groups;
   std::vector<int> inst_group, inst_slot;
   std::vector<long long> set_global, set_nsteps;  // ping-pong bookkeeping (RK2)
@@ -73,90 +82,191 @@ static void free_graphs(cdd_batch* b) {
   }
 }
 
-// One full RK step of group g reading state[phase] (RK2 writes state[phase^1]).
-static hipError_t group_step(cdd_batch* b, Group& g, int phase, hipStream_t st) {
-  hipError_t e;
+// The buffers of stage s of one RK step of group g reading state[phase]
+// (RK2 writes state[phase^1]): the spectral pass transforms SA (+ dt SB,
+// written to SW when SB is set); the flux reads FA (and FU) and writes Fout.
+struct StageIO {
+  const double *SA, *SB;
+  double* SW;
+  const double *FA, *FU;
+  double* Fout;
+  int mode;
+  double cu, cf;
+  bool partial;
+};
+
+static int num_stages(const cdd_batch* b) { return b->cfg.rk_order == 3 ? 3 : 2; }
+
+static StageIO stage_io(const cdd_batch* b, const Group& g, int phase, int s) {
   const int rk = b->cfg.rk_order;
   double* u = g.state[rk == 3 ? 0 : phase];
-  auto spectral = [&](const double* A, const double* B, double* W) -> hipError_t {
-    SpectralJob j;
-    memset(&j, 0, sizeof j);
-    j.d = 2;
-    j.n = g.n;
-    j.nc = 6;
-    j.cells = g.cells;
-    j.src = SrcField<double>{A, B, g.clk, g.cells, W, g.fstride};
-    j.spec = g.spec;
-    j.tw = g.tw;
-    j.out = g.G;
-    j.gp = b->gp0;
-    j.gp.norm = 1.0 / (double)g.cells;
-    j.gp.clk = g.clk;
-    j.stream = st;
-    j.count = g.count;
-    j.sstride = g.sstride;
-    j.ostride = g.gstride;
-    return launch_spectral(j);
-  };
-  auto flux = [&](const double* A, const double* U, double* out, int mode, double cu, double cf,
-                  double* partial) -> hipError_t {
-    FluxJob f;
-    memset(&f, 0, sizeof f);
-    f.d = 2;
-    f.n = g.n;
-    f.dyn = b->cfg.dynamics;
-    f.cells = g.cells;
-    f.A = A;
-    f.G = g.G;
-    f.U = U;
-    f.out = out;
-    f.partial = partial;
-    f.clk = g.clk;
-    f.mode = mode;
-    f.cu = cu;
-    f.cf = cf;
-    f.Lambda = b->cfg.dynamics == CDD_GLIDE_ONLY ? 0.0 : b->cfg.Lambda;
-    f.mu = b->gp0.mu;
-    f.lam = b->gp0.lam;
-    f.stream = st;
-    f.count = g.count;
-    f.fstride = g.fstride;
-    f.gstride = g.gstride;
-    f.pstride = g.nblocks;
-    f.tile = g.tile;
-    return launch_flux(f, nullptr);
-  };
-  if ((e = spectral(u, nullptr, nullptr)) != hipSuccess) return e;
-  if ((e = flux(u, nullptr, g.L0, 0, 0.0, 0.0, g.partial)) != hipSuccess) return e;
-  if ((e = launch_finalize_dt(g.partial, g.nblocks, g.clk, (double)g.n, st, g.count, g.nblocks)) != hipSuccess)
-    return e;
-  if ((e = spectral(u, g.L0, g.L0)) != hipSuccess) return e;  // forms F0 = u + dt L0 over L0
-  if (rk == 3) {
-    if ((e = flux(g.L0, u, g.buf, 1, 0.75, 0.25, nullptr)) != hipSuccess) return e;
-    if ((e = spectral(g.buf, nullptr, nullptr)) != hipSuccess) return e;
-    if ((e = flux(g.buf, u, u, 1, 1.0 / 3.0, 2.0 / 3.0, nullptr)) != hipSuccess) return e;
-  } else {
-    if ((e = flux(g.L0, u, g.state[phase ^ 1], 1, 0.5, 0.5, nullptr)) != hipSuccess) return e;
+  StageIO io;
+  memset(&io, 0, sizeof io);
+  if (s == 0) {  // L0 = L(u), CFL partials
+    io.SA = u;
+    io.FA = u;
+    io.Fout = g.L0;
+    io.partial = true;
+  } else if (s == 1) {  // F0 = u + dt L0 formed over L0
+    io.SA = u;
+    io.SB = g.L0;
+    io.SW = g.L0;
+    io.FA = g.L0;
+    io.FU = u;
+    io.mode = 1;
+    if (rk == 3) {
+      io.Fout = g.buf;
+      io.cu = 0.75;
+      io.cf = 0.25;
+    } else {
+      io.Fout = g.state[phase ^ 1];
+      io.cu = 0.5;
+      io.cf = 0.5;
+    }
+  } else {  // RK3: u' = u/3 + 2/3 (F1 + dt L(F1))
+    io.SA = g.buf;
+    io.FA = g.buf;
+    io.FU = u;
+    io.Fout = u;
+    io.mode = 1;
+    io.cu = 1.0 / 3.0;
+    io.cf = 2.0 / 3.0;
   }
-  return launch_advance(g.clk, st, g.count);
+  return io;
 }
 
-// Capture one step of every group into graph[phase]: fork one capture
-// stream per group from the batch stream, join them back.
+static hipError_t group_spectral(cdd_batch* b, Group& g, const StageIO& io, hipStream_t st) {
+  SpectralJob j;
+  memset(&j, 0, sizeof j);
+  j.d = 2;
+  j.n = g.n;
+  j.nc = 6;
+  j.cells = g.cells;
+  j.src = SrcField<double>{io.SA, io.SB, g.clk, g.cells, io.SW, g.fstride};
+  j.spec = g.spec;
+  j.tw = g.tw;
+  j.out = g.G;
+  j.gp = b->gp0;
+  j.gp.norm = 1.0 / (double)g.cells;
+  j.gp.clk = g.clk;
+  j.stream = st;
+  j.count = g.count;
+  j.sstride = g.sstride;
+  j.ostride = g.gstride;
+  return launch_spectral(j);
+}
+
+// the per-size flux launch (tile variant g.tile)
+static hipError_t group_flux(cdd_batch* b, Group& g, const StageIO& io, hipStream_t st) {
+  FluxJob f;
+  memset(&f, 0, sizeof f);
+  f.d = 2;
+  f.n = g.n;
+  f.dyn = b->cfg.dynamics;
+  f.cells = g.cells;
+  f.A = io.FA;
+  f.G = g.G;
+  f.U = io.FU;
+  f.out = io.Fout;
+  f.partial = io.partial ? g.partial : nullptr;
+  f.clk = g.clk;
+  f.mode = io.mode;
+  f.cu = io.cu;
+  f.cf = io.cf;
+  f.Lambda = b->cfg.dynamics == CDD_GLIDE_ONLY ? 0.0 : b->cfg.Lambda;
+  f.mu = b->gp0.mu;
+  f.lam = b->gp0.lam;
+  f.stream = st;
+  f.count = g.count;
+  f.fstride = g.fstride;
+  f.gstride = g.gstride;
+  f.pstride = g.nblocks;
+  f.tile = g.tile;
+  return launch_flux(f, nullptr);
+}
+
+static bool in_multi(const cdd_batch* b, const Group& g) {
+  return b->flux_per_cu > 0 && flux_multi_ok(g.n, b->flux_tile);
+}
+
+// One multi-instance flux launch of stage s over every group in_multi.
+static hipError_t multi_flux(cdd_batch* b, int phase, int s, hipStream_t st) {
+  FluxMultiJob m;
+  memset(&m, 0, sizeof m);
+  m.dyn = b->cfg.dynamics;
+  m.tile = b->flux_tile;
+  m.per_cu = b->flux_per_cu;
+  m.Lambda = b->cfg.dynamics == CDD_GLIDE_ONLY ? 0.0 : b->cfg.Lambda;
+  m.mu = b->gp0.mu;
+  m.lam = b->gp0.lam;
+  int k = 0;
+  // largest sides first
+  std::vector<int> order;
+  for (size_t gi = 0; gi < b->groups.size(); ++gi)
+    if (in_multi(b, b->groups[gi])) order.push_back((int)gi);
+  if (order.empty()) return hipSuccess;
+  std::sort(order.begin(), order.end(), [&](int x, int y) { return b->groups[x].n > b->groups[y].n; });
+  for (int gi : order) {
+    Group& g = b->groups[gi];
+    const StageIO io = stage_io(b, g, phase, s);
+    m.mode = io.mode;
+    m.cu = io.cu;
+    m.cf = io.cf;
+    FluxGroup& G = m.g[k++];
+    G.n = g.n;
+    G.count = g.count;
+    G.cells = g.cells;
+    G.fstride = g.fstride;
+    G.gstride = g.gstride;
+    G.pstride = g.nblocks;
+    G.A = io.FA;
+    G.G = g.G;
+    G.U = io.FU;
+    G.out = io.Fout;
+    G.partial = io.partial ? g.partial : nullptr;
+    G.clk = g.clk;
+  }
+  return launch_flux_multi(m, k, st);
+}
+
+// Capture one step of every group into graph[phase]: per stage, fork one
+// capture stream per group from the batch stream for the spectral passes
+// (and the per-size flux of groups outside the multi-instance launch), join
+// them back, then the multi-instance flux on the batch stream.
 static int capture(cdd_batch* b, int phase) {
   // capture on the private stream (the caller's stream may be the null
   // stream, which cannot be captured); the graph is launched on the caller's
   hipStream_t main = b->own;
   BCHK(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
-  hipEvent_t fork = b->ev[0];
-  hipError_t err = hipEventRecord(fork, main);
-  for (size_t gi = 0; gi < b->groups.size() && err == hipSuccess; ++gi) {
-    Group& g = b->groups[gi];
-    err = hipStreamWaitEvent(g.st, fork, 0);
-    if (err == hipSuccess) err = group_step(b, g, phase, g.st);
-    if (err == hipSuccess) err = hipEventRecord(b->ev[1 + gi], g.st);
-    if (err == hipSuccess) err = hipStreamWaitEvent(main, b->ev[1 + gi], 0);
+  const int ng = (int)b->groups.size();
+  hipError_t err = hipSuccess;
+  for (int s = 0; s < num_stages(b) && err == hipSuccess; ++s) {
+    hipEvent_t fork = b->ev[s * (1 + ng)];
+    err = hipEventRecord(fork, main);
+    for (int gi = 0; gi < ng && err == hipSuccess; ++gi) {
+      Group& g = b->groups[gi];
+      const StageIO io = stage_io(b, g, phase, s);
+      err = hipStreamWaitEvent(g.st, fork, 0);
+      if (err == hipSuccess) err = group_spectral(b, g, io, g.st);
+      if (err == hipSuccess && !in_multi(b, g)) {
+        err = group_flux(b, g, io, g.st);
+        if (err == hipSuccess && s == 0)
+          err = launch_finalize_dt(g.partial, g.nblocks, g.clk, (double)g.n, g.st, g.count, g.nblocks);
+      }
+      hipEvent_t join = b->ev[s * (1 + ng) + 1 + gi];
+      if (err == hipSuccess) err = hipEventRecord(join, g.st);
+      if (err == hipSuccess) err = hipStreamWaitEvent(main, join, 0);
+    }
+    if (err == hipSuccess) err = multi_flux(b, phase, s, main);
+    if (s == 0) {
+      for (int gi = 0; gi < ng && err == hipSuccess; ++gi) {
+        Group& g = b->groups[gi];
+        if (in_multi(b, g))
+          err = launch_finalize_dt(g.partial, g.nblocks, g.clk, (double)g.n, main, g.count, g.nblocks);
+      }
+    }
   }
+  for (int gi = 0; gi < ng && err == hipSuccess; ++gi) err = launch_advance(b->groups[gi].clk, main, b->groups[gi].count);
   hipGraph_t graph = nullptr;
   hipError_t e2 = hipStreamEndCapture(main, &graph);
   if (err != hipSuccess || e2 != hipSuccess) {
@@ -277,7 +387,7 @@ int cdd_batch_create(const cdd_config* cfg, const int* sizes, int count, cdd_bat
   }
   if (hipStreamCreateWithFlags(&b->own, hipStreamNonBlocking) != hipSuccess) return bail("stream create failed");
   b->stream = b->own;
-  b->ev.resize(1 + b->groups.size());
+  b->ev.resize(3 * (1 + b->groups.size()));
   for (auto& e : b->ev)
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return bail("event create failed");
   *out = b;
@@ -383,6 +493,14 @@ int cdd_batch_set_tile(cdd_batch* b, int n, int variant) {
   return CDD_OK;
 }
 
+int cdd_batch_set_flux(cdd_batch* b, int tile, int per_cu) {
+  if (!b || tile < 0 || tile > 2 || per_cu < 0 || per_cu > 2) return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
+  b->flux_tile = tile;
+  b->flux_per_cu = per_cu;
+  free_graphs(b);
+  return CDD_OK;
+}
+
 int cdd_batch_step(cdd_batch* b, int nsteps) {
   if (!b || nsteps < 0) return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
   BCHK(hipSetDevice(b->cfg.device));
@@ -440,6 +558,35 @@ int cdd_batch_time_flux(cdd_batch* b, int n, int variant, int reps, double* ms) 
   BCHK(hipEventElapsedTime(&t, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  *ms = t / reps;
+  return CDD_OK;
+}
+
+// Autotuner hook: mean time (ms) of `reps` first-stage multi-instance flux
+// launches over every eligible group with tile variant `tile` and `per_cu`
+// persistent blocks per CU, on the current states (no state change).
+int cdd_batch_time_flux_multi(cdd_batch* b, int tile, int per_cu, int reps, double* ms) {
+  if (!b || !ms || reps < 1 || tile < 0 || tile > 2 || per_cu < 1 || per_cu > 2)
+    return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
+  BCHK(hipSetDevice(b->cfg.device));
+  const int t0 = b->flux_tile, p0 = b->flux_per_cu;
+  b->flux_tile = tile;
+  b->flux_per_cu = per_cu;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = multi_flux(b, 0, 0, b->stream);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = multi_flux(b, 0, 0, b->stream);
+  if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  b->flux_tile = t0;
+  b->flux_per_cu = p0;
+  if (e != hipSuccess) return cdd_internal_fail(CDD_ERR_HIP, hipGetErrorString(e));
   *ms = t / reps;
   return CDD_OK;
 }

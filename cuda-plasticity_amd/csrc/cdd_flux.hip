@@ -63,6 +63,15 @@ __device__ __forceinline__ void limited_derivs(const T f[5], T& up, T& um) {
   um = fma(T(0.5), m_im1, Dm1);
 }
 
+// The RK stage combination cu U + cf (f + dt rhs) with its roundings fixed
+// by explicit fused multiply-adds: left to contraction, the compiler may
+// fuse either product into the addition depending on the surrounding code,
+// and every kernel that forms a stage must give the same bits.
+template <typename T>
+__device__ __forceinline__ T rk_combine(T cu, T U, T cf, T f, T dt, T rhs) {
+  return fma(cu, U, cf * fma(dt, rhs, f));
+}
+
 // Per-cell flux.  u[a][s][c]: derivative along axis a, s = 0 ('+') or 1
 // ('-'), component c = 3 i + j, in units of 1/N (limited_derivs); rhs comes
 // out in the reference's units.  sg: full (symmetric) stress.  vbuf: this
@@ -775,7 +784,7 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
   if constexpr (SC) {
     double* od = job.outd;
     if (job.mode == 0) od[idx] = (double)rhs_sc;
-    else od[idx] = job.cu * job.Ud[idx] + job.cf * ((double)fsc + (double)dt * (double)rhs_sc);
+    else od[idx] = rk_combine<double>(job.cu, job.Ud[idx], job.cf, (double)fsc, (double)dt, (double)rhs_sc);
   }
   if (job.mode == 0) {
 #pragma unroll
@@ -786,7 +795,7 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
       uv[c] = Uv(c);
-      ov[c] = cu * uv[c] + cf * (fc[c] + dt * rhs[c]);
+      ov[c] = rk_combine<T>(cu, uv[c], cf, fc[c], dt, rhs[c]);
       outp[c * cells + idx] = ov[c];
     }
     if constexpr (sizeof(T) == 8) {
@@ -806,6 +815,227 @@ __global__ void __launch_bounds__(FTC* FTR) k_flux(FluxJob job) {
     T m = block_max<T>(fin ? speed : T(INFINITY), red);
     if (tid == 0) partialp[blockIdx.x] = (double)m;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent multi-instance 2D flux (k_flux_multi): the per-cell arithmetic
+// of k_flux<double, 2, DYN, TC, TR, PRE = true> (same operands, same
+// operations, so every output is bit-identical to it), with the work of
+// every instance of every size group in one launch.  A fixed grid (per_cu
+// blocks per CU) walks the flat item list (group, instance, tile) with a
+// stride of the grid size; an instance whose clock is done is skipped.
+
+// A work item's group entry (copied field by field with constant indices:
+// a computed index into the kernel-argument table would move the whole
+// table to scratch memory), instance and tile origin -- uniform values.
+struct MultiItem {
+  FluxGroup G;
+  int tl, c0, r0;
+  long long inst;
+};
+
+template <int TC, int TR>
+__device__ __forceinline__ MultiItem multi_locate(const FluxMultiJob& job, int it, bool xperm) {
+  // XCD-aware order: hardware block b runs on XCD b % 8 and, with a grid
+  // that is a multiple of 8, takes the items it = b (mod 8), so logical
+  // item (it % 8) (nitems / 8) + it / 8 gives every XCD one contiguous range
+  // of tiles (neighbouring halos served from its L2)
+  const int L = xperm ? (it % 8) * (job.nitems / 8) + it / 8 : it;
+  MultiItem w;
+  w.G = job.g[0];
+#pragma unroll
+  for (int k = 1; k < kFluxGroups; ++k) {
+    if (k < job.ngroups && L >= job.g[k].item0) {
+      w.G.n = job.g[k].n;
+      w.G.ntiles = job.g[k].ntiles;
+      w.G.item0 = job.g[k].item0;
+      w.G.cells = job.g[k].cells;
+      w.G.fstride = job.g[k].fstride;
+      w.G.gstride = job.g[k].gstride;
+      w.G.pstride = job.g[k].pstride;
+      w.G.A = job.g[k].A;
+      w.G.G = job.g[k].G;
+      w.G.U = job.g[k].U;
+      w.G.out = job.g[k].out;
+      w.G.partial = job.g[k].partial;
+      w.G.clk = job.g[k].clk;
+    }
+  }
+  const int loc = L - w.G.item0;
+  w.inst = loc / w.G.ntiles;
+  w.tl = loc % w.G.ntiles;
+  const int nct = w.G.n / TC;
+  w.c0 = (w.tl % nct) * TC;
+  w.r0 = (w.tl / nct) * TR;
+  return w;
+}
+
+template <int DYN, int TC, int TR, bool COMB>
+__global__ void __launch_bounds__(TC* TR, 2) k_flux_multi(FluxMultiJob job) {
+  using FT = FluxTile<TC, TR>;
+  constexpr int PW = FT::PW, TILE = FT::TILE, NT = FT::NT, FILL = FT::FILL;
+  __shared__ double tile[9][TILE];
+  __shared__ double vsh[16][NT];
+  __shared__ double red[(NT + 63) / 64];
+  const int tc = threadIdx.x, trr = threadIdx.y;
+  const int tid = trr * TC + tc;
+  const int lr = trr + 2, lc = tc + 2;
+  const int nitems = job.nitems;
+  const bool xperm = (nitems % 8) == 0 && (gridDim.x % 8) == 0;
+  const double mu = job.mu, lam = job.lam;
+  for (int it = blockIdx.x; it < nitems; it += (int)gridDim.x) {
+    const MultiItem w = multi_locate<TC, TR>(job, it, xperm);
+    const FluxGroup& G = w.G;
+    const Clock* ck = G.clk + w.inst;
+    if (ck->done) continue;  // uniform over the block
+    const double dt = ck->dt;
+    const int n = G.n;
+    const long long cells = G.cells;
+    const double* A = G.A + w.inst * G.fstride;
+    const long long idx = (long long)(w.r0 + trr) * n + w.c0 + tc;
+    // one batch of loads: the halo tile of all nine components, then this
+    // cell's stress and U values
+    double tf[9][FILL];
+#pragma unroll
+    for (int k = 0; k < FILL; ++k) {
+      const int e = tid + k * NT;
+      const bool ok = e < TILE;
+      const int er = e / PW, ec = e % PW;
+      int gr = w.r0 + er - 2, gc = w.c0 + ec - 2;
+      gr = gr < 0 ? gr + n : (gr >= n ? gr - n : gr);
+      gc = gc < 0 ? gc + n : (gc >= n ? gc - n : gc);
+      const long long off = (long long)gr * n + gc;
+#pragma unroll
+      for (int c = 0; c < 9; ++c) tf[c][k] = ok ? A[c * cells + off] : 0.0;
+    }
+    const double* Gp = G.G + w.inst * G.gstride + idx;
+    double gv[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) gv[c] = Gp[c * cells];
+    // the previous item's reads of the tile are complete
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+#pragma unroll
+      for (int k = 0; k < FILL; ++k)
+        if (tid + k * NT < TILE) tile[c][tid + k * NT] = tf[c][k];
+    __syncthreads();
+    double u[2][2][9], fc[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      const double* tb = tile[c];
+      const double ctr = tb[lr * PW + lc];
+      fc[c] = ctr;
+      {  // contiguous axis
+        double f[5] = {tb[lr * PW + lc - 2], tb[lr * PW + lc - 1], ctr, tb[lr * PW + lc + 1], tb[lr * PW + lc + 2]};
+        limited_derivs<double>(f, u[1][0][c], u[1][1][c]);
+      }
+      {  // row axis
+        double f[5] = {tb[(lr - 2) * PW + lc], tb[(lr - 1) * PW + lc], ctr, tb[(lr + 1) * PW + lc],
+                       tb[(lr + 2) * PW + lc]};
+        limited_derivs<double>(f, u[0][0][c], u[0][1][c]);
+      }
+    }
+    double sg[9];
+    {
+      const double tr = fc[0] + fc[4] + fc[8];
+      sg[0] = gv[0] - lam * tr - mu * (fc[0] + fc[0]);
+      sg[4] = gv[1] - lam * tr - mu * (fc[4] + fc[4]);
+      sg[8] = gv[2] - lam * tr - mu * (fc[8] + fc[8]);
+      sg[1] = sg[3] = gv[3] - mu * (fc[1] + fc[3]);
+      sg[2] = sg[6] = gv[4] - mu * (fc[2] + fc[6]);
+      sg[5] = sg[7] = gv[5] - mu * (fc[5] + fc[7]);
+    }
+    double rhs[9], speed;
+    cell_flux<double, 2, DYN, true>(u, sg, job.Lambda, rhs, speed, &vsh[0][tid], NT, double(n));
+    double* o = G.out + w.inst * G.fstride + idx;
+    if constexpr (!COMB) {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) o[c * cells] = rhs[c];
+    } else {
+      const double* Up = G.U + w.inst * G.fstride + idx;
+      const double cu = job.cu, cf = job.cf;
+#pragma unroll
+      for (int c = 0; c < 9; ++c) o[c * cells] = rk_combine<double>(cu, Up[c * cells], cf, fc[c], dt, rhs[c]);
+    }
+    if (G.partial) {
+      // a non-finite speed or rhs must reach the CFL reduction
+      bool fin = isfinite(speed);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) fin = fin && isfinite(rhs[c]);
+      const double m = block_max<double>(fin ? speed : INFINITY, red);
+      if (tid == 0) G.partial[w.inst * G.pstride + w.tl] = m;
+    }
+  }
+}
+
+static void flux_multi_tile(int variant, int& tc, int& tr) {
+  if (variant == 1) {
+    tc = 64;
+    tr = 4;
+  } else if (variant == 2) {
+    tc = 16;
+    tr = 16;
+  } else {
+    tc = 32;
+    tr = 8;
+  }
+}
+
+bool flux_multi_ok(int n, int tile) {
+  int tc, tr;
+  flux_multi_tile(tile, tc, tr);
+  return n >= tc && n >= tr && n % tc == 0 && n % tr == 0;
+}
+
+template <int TC, int TR>
+static void launch_flux_multi_t(const FluxMultiJob& job, dim3 g, hipStream_t st) {
+  const dim3 b(TC, TR);
+#define CDD_MULTI(DYN_)                                                               \
+  do {                                                                                \
+    if (job.mode) hipLaunchKernelGGL((k_flux_multi<DYN_, TC, TR, true>), g, b, 0, st, job); \
+    else hipLaunchKernelGGL((k_flux_multi<DYN_, TC, TR, false>), g, b, 0, st, job);         \
+  } while (0)
+  if (job.dyn == DYN_GLIDE_ONLY) CDD_MULTI(DYN_GLIDE_ONLY);
+  else CDD_MULTI(DYN_GLIDE_CLIMB);
+#undef CDD_MULTI
+}
+
+hipError_t launch_flux_multi(FluxMultiJob& job, int ngroups, hipStream_t st) {
+  if (ngroups < 1 || ngroups > kFluxGroups) return hipErrorInvalidValue;
+  if (job.dyn != DYN_GLIDE_ONLY && job.dyn != DYN_GLIDE_CLIMB) return hipErrorInvalidValue;
+  int tc, tr;
+  flux_multi_tile(job.tile, tc, tr);
+  long long items = 0;
+  for (int k = 0; k < ngroups; ++k) {
+    FluxGroup& G = job.g[k];
+    if (!flux_multi_ok(G.n, job.tile) || G.count < 1 || !G.clk || !G.A || !G.G || !G.out ||
+        (job.mode && !G.U))
+      return hipErrorInvalidValue;
+    G.ntiles = (G.n / tc) * (G.n / tr);
+    if (G.partial && G.pstride < G.ntiles) return hipErrorInvalidValue;
+    G.item0 = (int)items;
+    items += (long long)G.ntiles * G.count;
+  }
+  if (items > (1LL << 30)) return hipErrorInvalidValue;
+  job.ngroups = ngroups;
+  job.nitems = (int)items;
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if (cus[dev] <= 0) {
+    int c = 0;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+    cus[dev] = c;
+  }
+  const int per_cu = job.per_cu == 1 ? 1 : 2;
+  long long grid = (long long)per_cu * cus[dev];
+  if (grid > items) grid = items;
+  const dim3 g((unsigned)grid);
+  if (job.tile == 1) launch_flux_multi_t<64, 4>(job, g, st);
+  else if (job.tile == 2) launch_flux_multi_t<16, 16>(job, g, st);
+  else launch_flux_multi_t<32, 8>(job, g, st);
+  return hipGetLastError();
 }
 
 // Plane-marching variant of k_flux for the 3D fp64 stepper (no per-cell
@@ -1169,7 +1399,7 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
       lds_read9(a_ub, uv);
 #pragma unroll
       for (int c = 0; c < 9; ++c) {
-        ov[c] = cu * uv[c] + cf * (fc[c] + dt * rhs[c]);
+        ov[c] = rk_combine<T>(cu, uv[c], cf, fc[c], dt, rhs[c]);
         o[c * cells] = ov[c];
       }
       // vacancy coupling after the last stage: c += beta (tr F2 - tr u)

@@ -115,6 +115,37 @@ struct FluxJob {
 };
 
 hipError_t launch_flux(const FluxJob& job, int* nblocks_out);
+
+// Persistent multi-instance 2D flux: one launch covers every instance of up
+// to kFluxGroups size groups.  The work items are (group, instance, tile) in
+// group order; a fixed grid of blocks walks them with a stride of the grid
+// size, each block fetching its next tile while it computes the current one.
+// Per group: instance i's fields are A + i fstride (likewise U and out),
+// G + i gstride, its clock clk + i and its per-tile CFL partials
+// partial + i pstride (pstride >= the instance's tile count).
+constexpr int kFluxGroups = 10;
+struct FluxGroup {
+  int n, count, ntiles, item0;  // side, instances, tiles per instance, first item
+  long long cells, fstride, gstride, pstride;
+  const double* A;
+  const double* G;
+  const double* U;
+  double* out;
+  double* partial;
+  Clock* clk;
+};
+struct FluxMultiJob {
+  int ngroups, nitems;  // filled by launch_flux_multi
+  int dyn, mode;        // mode as FluxJob (0: out = rhs; 1: out = cu U + cf (f + dt rhs))
+  int tile;             // tile variant (0: 32x8, 1: 64x4, 2: 16x16)
+  int per_cu;           // persistent blocks per CU (1 or 2; 0: 2)
+  double cu, cf, Lambda, mu, lam;
+  FluxGroup g[kFluxGroups];
+};
+// whether a group of side n can join a multi-instance launch with this tile
+bool flux_multi_ok(int n, int tile);
+// fills ngroups' item ranges and nitems from g[0..ngroups) and launches
+hipError_t launch_flux_multi(FluxMultiJob& job, int ngroups, hipStream_t st);
 // the Green's-tensor, elastic and load constants of a configuration
 GreenParams green_params(const cdd_config& cfg);
 int flux_nblocks(int d, int n);

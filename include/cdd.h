@@ -375,10 +375,19 @@ int cdd_batch_set_time(cdd_batch* batch, int i, double t, double t_end, double f
 int cdd_batch_get_clock(cdd_batch* batch, int i, double* t, double* last_dt, int* done, long long* nsteps);
 /* Replays the captured step nsteps times (no host synchronisation). */
 int cdd_batch_step(cdd_batch* batch, int nsteps);
-/* Autotuner hooks: flux tile variant per size class (0: 32x8, 1: 64x4,
- * 2: 16x16 cells) and the timing of one variant on the current states. */
+/* The flux of every stage is one persistent multi-instance launch over all
+ * instances of all sizes (a fixed grid of per_cu blocks per CU walking the
+ * (size, instance, tile) list), with tile variant `tile` (0: 32x8,
+ * 1: 64x4, 2: 16x16 cells); per_cu = 0 launches one flux per size instead.
+ * Default: tile 0, per_cu 2. */
+int cdd_batch_set_flux(cdd_batch* batch, int tile, int per_cu);
+/* Autotuner hooks: the per-size flux tile variant (sizes outside the
+ * multi-instance launch, or per_cu = 0), the timing (ms per launch, on the
+ * current states, no state change) of one per-size variant and of one
+ * multi-instance configuration. */
 int cdd_batch_set_tile(cdd_batch* batch, int n, int variant);
 int cdd_batch_time_flux(cdd_batch* batch, int n, int variant, int reps, double* ms);
+int cdd_batch_time_flux_multi(cdd_batch* batch, int tile, int per_cu, int reps, double* ms);
 
 const char* cdd_last_error(void);
 

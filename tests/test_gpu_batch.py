@@ -92,9 +92,11 @@ def test_autotuned_tiles_same_result(amd):
             b.set_state(i, betas[i])
         b.set_time(-1, 0.0)
         if variant == "auto":
-            choices, _ = autotune_batch(b, reps=2)
-            assert set(choices) == {64, 128}
+            choices, timings = autotune_batch(b, reps=2)
+            assert {64, 128} <= set(choices) and choices["multi"][1] in (0, 1, 2)
+            say("  auto", choices, timings.get("multi"))
         else:
+            b.set_flux(0, 0)  # per-size launches with this tile
             for n in (64, 128):
                 b.set_tile(n, variant)
         b.step(3)
@@ -103,3 +105,40 @@ def test_autotuned_tiles_same_result(amd):
     for o in outs[1:]:
         for a, c in zip(outs[0], o):
             assert np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("rk", [2, 3])
+def test_multi_instance_flux_configurations(amd, rk):
+    """Every multi-instance flux configuration (tile variant x persistent
+    blocks per CU, with the sides a tile does not divide left to per-size
+    launches) gives the per-size launches' states bit for bit; one instance
+    ends early."""
+    L = amd.lib
+    sizes = [16, 32, 64, 256, 16, 128]
+    say("multi-instance configurations", rk, sizes)
+    betas = [SpectralGaussianField((n, n), 2.5, 0.01, seed=400 + i) for i, n in enumerate(sizes)]
+    p = L.Plan(2, 256, dynamics=0, rk_order=rk, Lambda=0.9, cfl=0.25)
+    p.set_state(betas[3])
+    p.set_time(0.0)
+    p.step(1)
+    t_end = 1.5 * p.clock()[1]  # the 256 instance stops after its second step
+    p.close()
+    ref = None
+    for tile, per_cu in [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 1), (2, 2)]:
+        b = L.Batch(sizes, dynamics=0, rk_order=rk, Lambda=0.9, cfl=0.25)
+        for i in range(len(sizes)):
+            b.set_state(i, betas[i])
+        b.set_time(-1, 0.0)
+        b.set_time(3, 0.0, t_end)
+        b.set_flux(tile, per_cu)
+        b.step(4)
+        out = [b.get_state(i).cpu().numpy() for i in range(len(sizes))]
+        clk = [b.clock(i) for i in range(len(sizes))]
+        b.close()
+        if ref is None:
+            ref = (out, clk)
+            assert clk[3][2] and clk[3][3] == 2 and clk[0][3] == 4
+            continue
+        for i in range(len(sizes)):
+            assert np.array_equal(out[i], ref[0][i]), (tile, per_cu, sizes[i])
+            assert clk[i] == ref[1][i], (tile, per_cu, sizes[i])
