@@ -1190,7 +1190,9 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
   // into the other; 161824 bytes of LDS in all for fp64, 80 KB for fp32
   __shared__ T tile0[DM::TBUF];
   __shared__ T tile1[DM::TBUF];
-  __shared__ T vsh[3 * 8][NT];
+  // fp32: the corner velocities are parked in LDS (two blocks per CU need
+  // the registers); fp64 (one block per CU) keeps them in registers
+  __shared__ T vsh[sizeof(T) == 8 ? 1 : 3 * 8][NT];
   __shared__ T gb[NT / 64][6][64];    // stress planes
   __shared__ T nbuf[NT / 64][9][64];  // the slow axis's next plane
   __shared__ T ub[NT / 64][9][64];    // U cells
@@ -1332,6 +1334,15 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
     T nv[9], gv[6];
     lds_read9(a_nb, nv);
     lds_read6(a_gb, gv);
+    // this wave's G and next-plane cells are read (the reads complete inside
+    // lds_read*): refill them for the next plane (per-wave buffers, no
+    // barrier).  Without the combination the refill goes out at once, under
+    // the plane's whole arithmetic; with it (U in flight too) after the
+    // derivatives, which measured faster there.
+    if constexpr (!COMB) {
+      dma_g(p + 1);
+      dma_9(A, p + 3, nbuf[wv]);
+    }
     T u[3][2][9], fc[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
@@ -1358,10 +1369,10 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
         xd[c] = D1;
       }
     }
-    // this wave's G and next-plane cells are read: refill them for the next
-    // plane (per-wave buffers, so no barrier is needed)
-    dma_g(p + 1);
-    dma_9(A, p + 3, nbuf[wv]);
+    if constexpr (COMB) {
+      dma_g(p + 1);
+      dma_9(A, p + 3, nbuf[wv]);
+    }
     T sg[9];
     {
       const T tr = fc[0] + fc[4] + fc[8];
@@ -1387,7 +1398,12 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
     lds_barrier();
     dma_tile(p + 2, tcur);
     T rhs[9], speed;
-    cell_flux<T, 3, DYN, true, false, CLIMB ? 1 : 0, true>(u, sg, Lambda, rhs, speed, &vsh[0][tid], NT, T(n));
+    if constexpr (sizeof(T) == 8) {
+      T vreg[3 * 8];
+      cell_flux<T, 3, DYN, true, false, CLIMB ? 1 : 0, true>(u, sg, Lambda, rhs, speed, vreg, 1, T(n));
+    } else {
+      cell_flux<T, 3, DYN, true, false, CLIMB ? 1 : 0, true>(u, sg, Lambda, rhs, speed, &vsh[0][tid], NT, T(n));
+    }
     bool fin = isfinite(speed);
 #pragma unroll
     for (int c = 0; c < 9; ++c) fin = fin && isfinite(rhs[c]);
