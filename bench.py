@@ -226,6 +226,11 @@ def batched_vs_serial(K, W, count=48):
     torch.cuda.synchronize()
     tb = time.perf_counter() - t0
     steps_b = [b.clock(i)[3] for i in range(count)]
+    # the dominant kernel: the persistent multi-instance flux launch (first
+    # stage: beta and G in, rhs out over every instance), HIP events on the
+    # batch's stream
+    tile, per_cu = choices["multi"]
+    flux_ms = b.time_flux_multi(tile, per_cu if per_cu else 2, reps=10)
     b.close()
     plans = []
     for i, n in enumerate(sizes):
@@ -247,11 +252,17 @@ def batched_vs_serial(K, W, count=48):
     torch.cuda.empty_cache()
     # the 2D RK2 step's canonical bytes, summed over the instances
     step_bytes = sum(n * n * canonical_bytes_per_cell_update(n, d=2, order=2) for n in sizes)
+    flux_bytes = cells * spectral_bytes(512, d=2)["flux_stage1"]
+    flux_gbs = flux_bytes / (flux_ms * 1e-3) / 1e9
     return {"value": ts / tb, "unit": "x (serial time / batched time)", "steps": K, "warmup": W,
             "ms_per_step": 1e3 * tb / K, "serial_ms_per_step": 1e3 * ts / K,
             "cell_updates_per_s_batched": cells * K / tb, "cell_updates_per_s_serial": cells * K / ts,
             "step_roofline": {"bytes_per_step": step_bytes, "frac_batched": step_bytes * K / tb / (PEAK_HBM_GBS * 1e9),
                               "frac_serial": step_bytes * K / ts / (PEAK_HBM_GBS * 1e9)},
+            "roofline": {"bound": "hbm", "kernel": "%s (first stage, all 48 instances in one launch)" % (
+                "k_flux_rows" if tile == 3 else "k_flux_multi"),
+                         "achieved": flux_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": flux_gbs / PEAK_HBM_GBS,
+                         "traffic": None, "launch_ms": flux_ms, "algorithmic_bytes": flux_bytes},
             "sizes": sizes, "cells": cells, "flux_tiles": {str(k): v for k, v in choices.items()},
             "all_steps_ran": all(s == K + W for s in steps_b)}
 
@@ -267,6 +278,7 @@ def bench_batched(args):
         "cell_updates_per_s_batched": r["cell_updates_per_s_batched"],
         "cell_updates_per_s_serial": r["cell_updates_per_s_serial"],
         "step_roofline": r["step_roofline"],
+        "roofline": r["roofline"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: seeded Gaussian random beta^P per instance (L ~ U[3,8] cells, RMS 0.01)",
         "config": {"workload": "cdd-2d-batched-48", "sizes": r["sizes"], "cells": r["cells"], "dynamics": "lvp",
@@ -460,6 +472,45 @@ def bench_adjoint(args):
     print(json.dumps(out))
 
 
+def bench_flux_vjp(args):
+    """The hand-written flux adjoint on its own: cdd_flux_vjp (stress solve
+    of beta, cell VJP, transposed stress solve, gather) on a seeded n^3
+    field, `--steps` calls each for glide + climb (Lambda = 0.9) and glide
+    only, then `--steps` forward glide-only fluxes; HIP events around each
+    group.  One JSON line (ms per call)."""
+    import torch
+    from Plasticity.amd import lib as L
+    from Plasticity.FieldInitializers.FieldInitializer import SpectralGaussianField
+    n = args.n
+    beta = torch.from_numpy(SpectralGaussianField((n,) * 3, 6.0, 0.01, seed=0)).cuda()
+    rbar = torch.from_numpy(SpectralGaussianField((n,) * 3, 4.0, 1.0, seed=1)).cuda()
+    res = {}
+
+    def timed(name, fn, reps):
+        fn()  # warm-up
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name] = e0.elapsed_time(e1) / reps
+        print("%s: %.3f ms per call" % (name, res[name]), file=sys.stderr, flush=True)
+
+    for label, dyn, lam in (("vjp_glide_climb", L.GLIDE_CLIMB, 0.9), ("vjp_glide_only", L.GLIDE_ONLY, 0.0)):
+        p = L.Plan(3, n, dynamics=dyn, rk_order=3, Lambda=lam)
+        timed(label, lambda: p.flux_vjp(beta, 0.0, rbar), args.steps)
+        if dyn == L.GLIDE_ONLY:
+            timed("flux_glide_only", lambda: p.flux(beta, 0.0), args.steps)
+        p.close()
+    out = {"metric": "flux adjoint (cdd_flux_vjp) time per call", "value": res["vjp_glide_climb"], "unit": "ms",
+           "n_gpus": 1, "steps": args.steps, "warmup": 1, "higher_is_better": False, "dtype": "f64",
+           "data": "synthetic: seeded Gaussian random beta^P (L=6 cells, RMS 0.01) and cotangent (L=4, RMS 1)",
+           "config": {"workload": "cdd-3d-flux-vjp", "grid": [n] * 3}, "per_call_ms": res}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -472,7 +523,7 @@ def main():
     ap.add_argument("--fd-dirs", type=int, default=32, help="adjoint workload: finite-difference directions")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
                     help="3d workload: arithmetic of the plan (fp32 also reports its error against fp64)")
-    ap.add_argument("--workload", default="3d", choices=["3d", "batched", "adjoint", "loading"],
+    ap.add_argument("--workload", default="3d", choices=["3d", "batched", "adjoint", "loading", "flux_vjp"],
                     help="3d: the headline 256^3 stepper (default); batched: 48 uneven 2D realisations, "
                          "batched hipGraph step vs a serial loop of plain plans")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -495,6 +546,8 @@ def main():
         return bench_batched(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "flux_vjp":
+        return bench_flux_vjp(args)
     if args.workload == "loading":
         return bench_loading(args)
 

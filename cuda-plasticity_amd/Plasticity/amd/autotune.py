@@ -7,8 +7,9 @@ states (no state change) and installed:
     cdd_batch_set_tile) -- the flux of sizes outside the multi-instance
     launch, and of the per-size mode;
   * the persistent multi-instance flux launch (one launch per stage over
-    every instance of every size): its tile variant and persistent blocks per
-    CU (1 or 2), against the per-size launches as a whole
+    every instance of every size): the tile walk (tiles 0-2, 1 or 2 blocks
+    per CU) or the row march (tile 3, sides that are multiples of 128, 1-3
+    blocks per CU), against the per-size launches as a whole
     (cdd_batch_time_flux_multi, cdd_batch_set_flux; per_cu 0 selects the
     per-size launches).
 Choices are cached per (device name, sizes, dynamics) in a JSON file so a
@@ -19,7 +20,8 @@ import os
 
 VARIANTS = {0: "32x8", 1: "64x4", 2: "16x16"}
 TILE_SHAPE = {0: (32, 8), 1: (64, 4), 2: (16, 16)}
-PER_CU = (1, 2)
+MULTI = {0: "32x8 tiles", 1: "64x4 tiles", 2: "16x16 tiles", 3: "row march"}
+PER_CU = {0: (1, 2), 1: (1, 2), 2: (1, 2), 3: (1, 2, 3)}
 
 
 def _valid(n, v):
@@ -28,6 +30,8 @@ def _valid(n, v):
 
 def _multi_ok(n, v):
     """Whether a group of side n joins the multi-instance launch of tile v."""
+    if v == 3:
+        return n >= 128 and n % 128 == 0
     tc, tr = TILE_SHAPE[v]
     return n >= tc and n >= tr and n % tc == 0 and n % tr == 0
 
@@ -68,12 +72,12 @@ def autotune_batch(batch, reps=5, cache_path=None, dynamics=None, verbose=False)
         tile, per_cu = (int(x) for x in cache[mkey])
     else:
         cand = {}
-        for v in VARIANTS:
+        for v in MULTI:
             if not any(_multi_ok(n, v) for n in sides):
                 continue
-            # sizes outside this tile's multi launch keep their per-size launch
+            # sizes outside this variant's multi launch keep their per-size launch
             rest = sum(best_ms.get(n, 0.0) for n in sides if not _multi_ok(n, v))
-            for p in PER_CU:
+            for p in PER_CU[v]:
                 cand[(v, p)] = batch.time_flux_multi(v, p, reps) + rest
         if len(best_ms) == len(sides):
             cand[(0, 0)] = sum(best_ms.values())  # the per-size launches one after another
@@ -83,7 +87,7 @@ def autotune_batch(batch, reps=5, cache_path=None, dynamics=None, verbose=False)
     batch.set_flux(tile, per_cu)
     choices["multi"] = (tile, per_cu)
     if verbose:
-        print("autotune multi -> tile %s, %d blocks/CU %s" % (VARIANTS[tile], per_cu, timings.get("multi", "(cached)")))
+        print("autotune multi -> %s, %d blocks/CU %s" % (MULTI[tile], per_cu, timings.get("multi", "(cached)")))
     if cache_path:
         with open(cache_path, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)

@@ -61,7 +61,7 @@ struct Group {
 struct cdd_batch {
   cdd_config cfg;
   GreenParams gp0;  // mu/nu/lam/A/C (norm per group)
-  int flux_tile = 0;    // tile variant of the multi-instance flux
+  int flux_tile = 3;    // multi-instance flux variant (0-2: tile walk, 3: row march)
   int flux_per_cu = 2;  // its persistent blocks per CU; 0: per-size k_flux launches
   std::vector<Group> groups;
   std::vector<int> inst_group, inst_slot;
@@ -186,7 +186,15 @@ static hipError_t group_flux(cdd_batch* b, Group& g, const StageIO& io, hipStrea
 }
 
 static bool in_multi(const cdd_batch* b, const Group& g) {
-  return b->flux_per_cu > 0 && flux_multi_ok(g.n, b->flux_tile);
+  if (b->flux_per_cu <= 0) return false;
+  return b->flux_tile == 3 ? flux_rows_ok(g.n) : flux_multi_ok(g.n, b->flux_tile);
+}
+
+// The CFL reduction of a group in the multi-instance launch: the row-marching
+// kernel leaves one atomically maximised slot per instance (read and reset).
+static hipError_t multi_finalize(cdd_batch* b, Group& g, hipStream_t st) {
+  if (b->flux_tile == 3) return launch_finalize_dt(g.partial, 1, g.clk, (double)g.n, st, g.count, g.nblocks, 0.0, 1);
+  return launch_finalize_dt(g.partial, g.nblocks, g.clk, (double)g.n, st, g.count, g.nblocks);
 }
 
 // One multi-instance flux launch of stage s over every group in_multi.
@@ -226,7 +234,7 @@ static hipError_t multi_flux(cdd_batch* b, int phase, int s, hipStream_t st) {
     G.partial = io.partial ? g.partial : nullptr;
     G.clk = g.clk;
   }
-  return launch_flux_multi(m, k, st);
+  return b->flux_tile == 3 ? launch_flux_rows(m, k, st) : launch_flux_multi(m, k, st);
 }
 
 // Capture one step of every group into graph[phase]: per stage, fork one
@@ -261,8 +269,7 @@ static int capture(cdd_batch* b, int phase) {
     if (s == 0) {
       for (int gi = 0; gi < ng && err == hipSuccess; ++gi) {
         Group& g = b->groups[gi];
-        if (in_multi(b, g))
-          err = launch_finalize_dt(g.partial, g.nblocks, g.clk, (double)g.n, main, g.count, g.nblocks);
+        if (in_multi(b, g)) err = multi_finalize(b, g, main);
       }
     }
   }
@@ -277,6 +284,16 @@ static int capture(cdd_batch* b, int phase) {
   }
   b->graph[phase] = graph;
   BCHK(hipGraphInstantiate(&b->exec[phase], graph, nullptr, nullptr, 0));
+  return CDD_OK;
+}
+
+// Zero every CFL partial (outside any capture): the row-marching flux
+// maximises into slot 0 of each instance atomically, so a slot left by a
+// timing launch or by another flux variant must not reach a later step.
+static int zero_partials(cdd_batch* b) {
+  BCHK(hipStreamSynchronize(b->stream));
+  for (Group& g : b->groups) BCHK(hipMemset(g.partial, 0, sizeof(double) * (size_t)g.nblocks * g.count));
+  BCHK(hipDeviceSynchronize());
   return CDD_OK;
 }
 
@@ -363,6 +380,9 @@ int cdd_batch_create(const cdd_config* cfg, const int* sizes, int count, cdd_bat
     if (hipMalloc(&g.spec, sizeof(double) * 2 * (size_t)g.sstride * g.count) != hipSuccess) return bail("hipMalloc failed");
     if (hipMalloc((void**)&g.partial, sizeof(double) * (size_t)g.nblocks * g.count) != hipSuccess)
       return bail("hipMalloc failed");
+    // the row-marching flux maximises into slot 0 of each instance atomically
+    if (hipMemset(g.partial, 0, sizeof(double) * (size_t)g.nblocks * g.count) != hipSuccess)
+      return bail("memset failed");
     if (hipMalloc((void**)&g.clk, sizeof(Clock) * g.count) != hipSuccess) return bail("hipMalloc failed");
     if (hipMalloc(&g.tw, sizeof(double) * 2 * n) != hipSuccess) return bail("hipMalloc failed");
     std::vector<double> h(2 * n);
@@ -494,7 +514,10 @@ int cdd_batch_set_tile(cdd_batch* b, int n, int variant) {
 }
 
 int cdd_batch_set_flux(cdd_batch* b, int tile, int per_cu) {
-  if (!b || tile < 0 || tile > 2 || per_cu < 0 || per_cu > 2) return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
+  if (!b || tile < 0 || tile > 3 || per_cu < 0 || per_cu > (tile == 3 ? 4 : 2))
+    return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
+  BCHK(hipSetDevice(b->cfg.device));
+  if (zero_partials(b)) return CDD_ERR_HIP;
   b->flux_tile = tile;
   b->flux_per_cu = per_cu;
   free_graphs(b);
@@ -558,6 +581,7 @@ int cdd_batch_time_flux(cdd_batch* b, int n, int variant, int reps, double* ms) 
   BCHK(hipEventElapsedTime(&t, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (zero_partials(b)) return CDD_ERR_HIP;
   *ms = t / reps;
   return CDD_OK;
 }
@@ -566,7 +590,7 @@ int cdd_batch_time_flux(cdd_batch* b, int n, int variant, int reps, double* ms) 
 // launches over every eligible group with tile variant `tile` and `per_cu`
 // persistent blocks per CU, on the current states (no state change).
 int cdd_batch_time_flux_multi(cdd_batch* b, int tile, int per_cu, int reps, double* ms) {
-  if (!b || !ms || reps < 1 || tile < 0 || tile > 2 || per_cu < 1 || per_cu > 2)
+  if (!b || !ms || reps < 1 || tile < 0 || tile > 3 || per_cu < 1 || per_cu > (tile == 3 ? 4 : 2))
     return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
   BCHK(hipSetDevice(b->cfg.device));
   const int t0 = b->flux_tile, p0 = b->flux_per_cu;
@@ -587,6 +611,7 @@ int cdd_batch_time_flux_multi(cdd_batch* b, int tile, int per_cu, int reps, doub
   b->flux_tile = t0;
   b->flux_per_cu = p0;
   if (e != hipSuccess) return cdd_internal_fail(CDD_ERR_HIP, hipGetErrorString(e));
+  if (zero_partials(b)) return CDD_ERR_HIP;
   *ms = t / reps;
   return CDD_OK;
 }

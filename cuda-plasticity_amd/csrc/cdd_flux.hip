@@ -26,6 +26,13 @@
 // The rounding differs from the reference at the 1e-16 level only.
 #include "cdd_kernels.h"
 
+// Multiply-adds fuse only within one expression in this file (the build's
+// default lets the compiler fuse across statements, choosing by the
+// surrounding code): every kernel here that states the same arithmetic --
+// the per-tile, multi-instance and row-marching fluxes share cell_flux --
+// then rounds it the same way, so their states agree bit for bit.
+#pragma clang fp contract(on)
+
 namespace cdd {
 
 // Two-argument minmod: min if both > 0, max if both <= 0, else 0.  That is
@@ -1454,6 +1461,302 @@ __global__ void __launch_bounds__(march::NT, 2) k_flux_march32(FluxJob job, int 
   flux_march_body<float, DYN, CLIMB, COMB>(job, chunk);
 }
 
+// ---------------------------------------------------------------------------
+// Row-marching persistent multi-instance 2D flux (k_flux_rows), fp64: the
+// batched stepper's flux as one launch over every instance of every size
+// group whose side is a multiple of 128.  Every block takes one contiguous,
+// equal share of the flat (group, instance, 128-column strip, row) space and
+// marches down its rows: per row, the 128-cell segment with its 2-cell halo
+// arrives in LDS by asynchronous DMA two rows ahead, the stress cells and
+// the next row's own cells one row ahead (per-wave buffers), and the
+// row-axis derivatives are carried from row to row in registers exactly as
+// the 3D marching kernel carries its slow axis (bit-identical to
+// limited_derivs).  The per-cell arithmetic is k_flux's (cell_flux with
+// PRE, the same sigma and stage combination), so the states equal the
+// per-tile kernels' bit for bit.  The CFL maximum of each instance goes to
+// its first partial slot by an atomic maximum on the bit pattern (speeds
+// are positive, a non-finite speed or rhs enters as +inf); the CFL
+// reduction reads and clears that slot.
+namespace rows {
+constexpr int TC = 128, NT = TC;                                  // one row segment, one cell per thread
+constexpr int PW = TC + 4;                                        // with its 2-cell halo
+constexpr int EPL = 2;                                            // values per 16-byte DMA unit
+constexpr int NUNIT = 9 * PW / EPL;                               // DMA units of a row tile
+constexpr int NDMA = (NUNIT + NT - 1) / NT;                       // tile DMA instructions per thread
+constexpr int WLAST = (NUNIT - 1 - (NDMA - 1) * NT) / 64;         // waves the last round needs
+constexpr int TBUF = EPL * ((NDMA - 1) * NT + (WLAST + 1) * 64);  // tile buffer (values)
+constexpr int N_T = NDMA, N_9 = 4 + 2, N_G = 3, N_GN = N_G + N_9, N_ST = 9;
+static_assert(PW % EPL == 0 && NDMA >= 2 && WLAST >= 0 && WLAST < NT / 64, "row tile DMA rounds");
+}  // namespace rows
+
+bool flux_rows_ok(int n) { return n >= rows::TC && n % rows::TC == 0; }
+
+template <int DYN, bool COMB>
+__global__ void __launch_bounds__(rows::NT) k_flux_rows(FluxMultiJob job) {
+  using namespace rows;
+  typedef __attribute__((address_space(3))) void lds_void;
+  __shared__ double tile0[TBUF];
+  __shared__ double tile1[TBUF];
+  __shared__ double gb[NT / 64][6][64];    // stress cells
+  __shared__ double nbuf[NT / 64][9][64];  // the row axis's next row
+  __shared__ double ub[NT / 64][9][64];    // U cells
+  __shared__ double red[NT / 64];
+  const int tid = threadIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int half = lane >> 5;
+  const unsigned a_gb = lds_addr(&gb[wv][0][lane]), a_nb = lds_addr(&nbuf[wv][0][lane]),
+                 a_ub = lds_addr(&ub[wv][0][lane]);
+  const double mu = job.mu, lam = job.lam, Lambda = job.Lambda, cu = job.cu, cf = job.cf;
+  const long long total = job.nitems;
+  const long long r_beg = total * blockIdx.x / gridDim.x, r_end = total * (blockIdx.x + 1) / gridDim.x;
+  long long r = r_beg;
+  while (r < r_end) {
+    // this row-step's group (constant indices into the kernel-argument
+    // table), instance, strip and row
+    FluxGroup G = job.g[0];
+#pragma unroll
+    for (int k = 1; k < kFluxGroups; ++k) {
+      if (k < job.ngroups && r >= job.g[k].item0) {
+        G.n = job.g[k].n;
+        G.ntiles = job.g[k].ntiles;
+        G.item0 = job.g[k].item0;
+        G.cells = job.g[k].cells;
+        G.fstride = job.g[k].fstride;
+        G.gstride = job.g[k].gstride;
+        G.pstride = job.g[k].pstride;
+        G.A = job.g[k].A;
+        G.G = job.g[k].G;
+        G.U = job.g[k].U;
+        G.out = job.g[k].out;
+        G.partial = job.g[k].partial;
+        G.clk = job.g[k].clk;
+      }
+    }
+    const int n = G.n;
+    const long long loc = r - G.item0;
+    const long long inst = loc / G.ntiles;
+    const int rem = (int)(loc % G.ntiles);
+    const int c0 = (rem / n) * TC, p0 = rem % n;
+    const int len = (int)((r_end - r) < (long long)(n - p0) ? (r_end - r) : (long long)(n - p0));
+    r += len;
+    const Clock* ck = G.clk + inst;
+    if (ck->done) continue;  // uniform over the block
+    const double dt = ck->dt;
+    const long long cells = G.cells;
+    const double* __restrict__ A = G.A + inst * G.fstride;
+    const double* __restrict__ Gp = G.G + inst * G.gstride;
+    const double* __restrict__ Up = COMB ? G.U + inst * G.fstride : nullptr;
+    double* __restrict__ outp = G.out + inst * G.fstride;
+    auto wrapr = [n](int p) { return p < 0 ? p + n : (p >= n ? p - n : p); };
+    // per-lane element offsets inside one row (row base added per step):
+    // halo-segment units (component, even column pair; n is even, so a pair
+    // never straddles the periodic wrap), the cells of this lane in the
+    // per-wave buffers (its pair, and the cell of its 4-byte word of
+    // component 8), and this thread's own cell
+    unsigned toff[NDMA];
+#pragma unroll
+    for (int k = 0; k < NDMA; ++k) {
+      const int wk = (k == NDMA - 1 && wv > WLAST) ? WLAST : wv;
+      const int e = k * NT + wk * 64 + lane;
+      const int ee = e < NUNIT ? e : 0;
+      const int comp = ee / (PW / EPL), pr = ee % (PW / EPL);
+      int gc = c0 - 2 + EPL * pr;
+      gc = gc < 0 ? gc + n : (gc >= n ? gc - n : gc);
+      toff[k] = (unsigned)(comp * cells + gc);
+    }
+    const unsigned poff = (unsigned)(c0 + 64 * wv + 2 * (lane & 31));
+    const unsigned woff = (unsigned)(c0 + 64 * wv + (lane >> 1));  // + 32: the wave's second half
+    const unsigned xoff = (unsigned)(c0 + tid);
+    auto dma_tile = [&](int p, double* dst) {
+      const double* base = A + (long long)wrapr(p) * n;
+#pragma unroll
+      for (int k = 0; k < NDMA; ++k) {
+        const int wk = (k == NDMA - 1 && wv > WLAST) ? WLAST : wv;
+        __builtin_amdgcn_global_load_lds((const void*)(base + toff[k]), (lds_void*)&dst[EPL * (k * NT + wk * 64)], 16,
+                                         0, 0);
+      }
+    };
+    auto dma_g = [&](int p) {
+      const double* base = Gp + (long long)wrapr(p) * n + poff;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        __builtin_amdgcn_global_load_lds((const void*)(base + (2 * k + half) * cells), (lds_void*)&gb[wv][2 * k][0], 16,
+                                         0, 0);
+    };
+    auto dma_9 = [&](const double* src, int p, double (*dst)[64]) {
+      const double* row = src + (long long)wrapr(p) * n;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        __builtin_amdgcn_global_load_lds((const void*)(row + poff + (2 * k + half) * cells), (lds_void*)&dst[2 * k][0],
+                                         16, 0, 0);
+      const char* w8 = (const char*)(row + 8 * cells + woff) + 4 * (lane & 1);
+      __builtin_amdgcn_global_load_lds((const void*)w8, (lds_void*)&dst[8][0], 4, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(w8 + 8 * 32), (lds_void*)&dst[8][32], 4, 0, 0);
+    };
+    auto center = [&](int p, double (&v)[9]) {
+      const double* base = A + (long long)wrapr(p) * n + xoff;
+#pragma unroll
+      for (int c = 0; c < 9; ++c) v[c] = base[c * cells];
+    };
+    // row-axis derivatives carried from row to row (as in the 3D marching
+    // kernel's slow axis): f(p+2) arrives in nbuf one row ahead
+    dma_tile(p0, tile0);
+    dma_tile(p0 + 1, tile1);
+    dma_g(p0);
+    dma_9(A, p0 + 2, nbuf[wv]);
+    if constexpr (COMB) dma_9(Up, p0, ub[wv]);
+    double xf[9], xd[9], xs[9], xum[9];
+    {
+      double fm2[9], fm1[9], f0[9];
+      center(p0 - 2, fm2);
+      center(p0 - 1, fm1);
+      center(p0, f0);
+      center(p0 + 1, xf);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) {
+        const double Dm2 = fm1[c] - fm2[c], Dm1 = f0[c] - fm1[c];
+        xd[c] = xf[c] - f0[c];
+        xs[c] = xd[c] - Dm1;
+        xum[c] = fma(0.5, minmod2(xs[c], Dm1 - Dm2), Dm1);
+      }
+    }
+    wait_vmcnt<0>();
+    double run_max = -INFINITY;
+    const int lc = tid + 2;
+    // One row.  Per wave, the vector-memory issue order of a step is
+    //   [G(p+1), next(p+3)] [tile(p+2)] [stores(p)] [U(p+1)]
+    // so U(p) (issued last on the step before) is complete once at most
+    // N_GN + N_T younger operations are outstanding, and G(p+1), next(p+3)
+    // and tile(p+1) (issued on the step before) once at most N_T + N_ST
+    // (+ N_9 for U(p+1)) are.
+    auto step = [&](int p, double* tcur) __attribute__((always_inline)) {
+      // tile(p), G(p) and next(p+2) (this wave's part) landed at the wait
+      // that closed the previous step; the barrier publishes every wave's part
+      lds_barrier();
+      double nv[9], gv[6];
+      lds_read9(a_nb, nv);
+      lds_read6(a_gb, gv);
+      if constexpr (!COMB) {  // the refill goes out under the whole row's arithmetic
+        dma_g(p + 1);
+        dma_9(A, p + 3, nbuf[wv]);
+      }
+      double u[2][2][9], fc[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) {
+        const double* tb = tcur + c * PW;
+        const double ctr = tb[lc];
+        fc[c] = ctr;
+        {  // contiguous axis
+          double f[5] = {tb[lc - 2], tb[lc - 1], ctr, tb[lc + 1], tb[lc + 2]};
+          limited_derivs<double>(f, u[1][0][c], u[1][1][c]);
+        }
+        {  // row axis, from the carried differences
+          const double D1 = nv[c] - xf[c];
+          const double s1 = D1 - xd[c];
+          const double m = minmod2(s1, xs[c]);
+          u[0][0][c] = fma(-0.5, m, xd[c]);
+          u[0][1][c] = xum[c];
+          xum[c] = fma(0.5, m, xd[c]);
+          xf[c] = nv[c];
+          xs[c] = s1;
+          xd[c] = D1;
+        }
+      }
+      if constexpr (COMB) {
+        dma_g(p + 1);
+        dma_9(A, p + 3, nbuf[wv]);
+      }
+      double sg[9];
+      {
+        const double tr = fc[0] + fc[4] + fc[8];
+        sg[0] = gv[0] - lam * tr - mu * (fc[0] + fc[0]);
+        sg[4] = gv[1] - lam * tr - mu * (fc[4] + fc[4]);
+        sg[8] = gv[2] - lam * tr - mu * (fc[8] + fc[8]);
+        sg[1] = sg[3] = gv[3] - mu * (fc[1] + fc[3]);
+        sg[2] = sg[6] = gv[4] - mu * (fc[2] + fc[6]);
+        sg[5] = sg[7] = gv[5] - mu * (fc[5] + fc[7]);
+      }
+      // every wave is done with tile(p) before tile(p + 2) refills its buffer
+      lds_barrier();
+      dma_tile(p + 2, tcur);
+      double rhs[9], speed, vreg[16];
+      cell_flux<double, 2, DYN, true>(u, sg, Lambda, rhs, speed, vreg, 1, double(n));
+      bool fin = isfinite(speed);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) fin = fin && isfinite(rhs[c]);
+      run_max = fmax(run_max, fin ? speed : INFINITY);
+      double* o = outp + (long long)p * n + xoff;
+      if constexpr (COMB) {
+        wait_vmcnt<N_GN + N_T>();  // U(p)
+        double uv[9];
+        lds_read9(a_ub, uv);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) o[c * cells] = rk_combine<double>(cu, uv[c], cf, fc[c], dt, rhs[c]);
+        dma_9(Up, p + 1, ub[wv]);
+        wait_vmcnt<N_T + N_ST + N_9>();
+      } else {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) o[c * cells] = rhs[c];
+        wait_vmcnt<N_T + N_ST>();
+      }
+    };
+    const int pe = p0 + len;
+    int p = p0;
+    for (; p + 1 < pe; p += 2) {
+      step(p, tile0);
+      step(p + 1, tile1);
+    }
+    if (p < pe) step(p, tile0);
+    // nothing may still write this block's LDS when the next segment starts
+    wait_vmcnt<0>();
+    if (G.partial) {
+      const double m = block_max<double>(run_max, red);
+      if (tid == 0)
+        atomicMax(reinterpret_cast<unsigned long long*>(G.partial + inst * G.pstride),
+                  (unsigned long long)__double_as_longlong(m));
+    }
+  }
+}
+
+hipError_t launch_flux_rows(FluxMultiJob& job, int ngroups, hipStream_t st) {
+  if (ngroups < 1 || ngroups > kFluxGroups) return hipErrorInvalidValue;
+  if (job.dyn != DYN_GLIDE_ONLY && job.dyn != DYN_GLIDE_CLIMB) return hipErrorInvalidValue;
+  long long steps = 0;
+  for (int k = 0; k < ngroups; ++k) {
+    FluxGroup& G = job.g[k];
+    if (!flux_rows_ok(G.n) || G.count < 1 || !G.clk || !G.A || !G.G || !G.out || (job.mode && !G.U) ||
+        (G.partial && G.pstride < 1))
+      return hipErrorInvalidValue;
+    G.ntiles = (G.n / rows::TC) * G.n;  // row-steps per instance
+    G.item0 = (int)steps;
+    steps += (long long)G.ntiles * G.count;
+  }
+  if (steps > (1LL << 30)) return hipErrorInvalidValue;
+  job.ngroups = ngroups;
+  job.nitems = (int)steps;
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if (cus[dev] <= 0) {
+    int c = 0;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+    cus[dev] = c;
+  }
+  const int per_cu = job.per_cu >= 1 && job.per_cu <= 4 ? job.per_cu : 2;
+  long long grid = (long long)per_cu * cus[dev];
+  // at least 16 rows per block
+  if (grid > steps / 16) grid = steps / 16 > 0 ? steps / 16 : 1;
+  const dim3 g((unsigned)grid), b(rows::NT);
+  if (job.dyn == DYN_GLIDE_ONLY) {
+    if (job.mode) hipLaunchKernelGGL((k_flux_rows<DYN_GLIDE_ONLY, true>), g, b, 0, st, job);
+    else hipLaunchKernelGGL((k_flux_rows<DYN_GLIDE_ONLY, false>), g, b, 0, st, job);
+  } else {
+    if (job.mode) hipLaunchKernelGGL((k_flux_rows<DYN_GLIDE_CLIMB, true>), g, b, 0, st, job);
+    else hipLaunchKernelGGL((k_flux_rows<DYN_GLIDE_CLIMB, false>), g, b, 0, st, job);
+  }
+  return hipGetLastError();
+}
+
 // Planes per marching block: the longest chunk (a power of two, at least 16
 // planes) that still gives every CU of the current device per_cu blocks (the
 // fp64 kernel's LDS admits one block per CU, the fp32 kernel's two), so
@@ -1639,8 +1942,8 @@ hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
 
 // dt = CFL / (max(gridShape) max speed), clamped to the end time
 // (FieldMover.py:94-111).  One block.
-__global__ void __launch_bounds__(1024) k_finalize_dt(const double* partial, int nb, Clock* clk, double nmax,
-                                                     long long pstride, double dt_bound) {
+__global__ void __launch_bounds__(1024) k_finalize_dt(double* partial, int nb, Clock* clk, double nmax,
+                                                     long long pstride, double dt_bound, int clear) {
   // one block per instance
   partial += blockIdx.x * pstride;
   clk += blockIdx.x;
@@ -1655,6 +1958,7 @@ __global__ void __launch_bounds__(1024) k_finalize_dt(const double* partial, int
     double mm = -INFINITY;
     for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) mm = fmax(mm, red[k]);
     clk->last_max = mm;
+    if (clear) partial[0] = 0.0;  // an atomically maximised slot starts the next step at zero
     if (!(mm < INFINITY)) {  // a non-finite speed: stop before the state is updated
       clk->status = 1;
       clk->done = 1;
@@ -1754,10 +2058,12 @@ __global__ void k_advance(Clock* clk, int count) {
 }
 
 hipError_t launch_finalize_dt(const double* partial, int nb, Clock* clk, double nmax, hipStream_t st, int count,
-                              long long pstride, double dt_bound) {
+                              long long pstride, double dt_bound, int clear) {
   if (count < 1 || count > 1024) return hipErrorInvalidValue;
+  if (clear && nb != 1) return hipErrorInvalidValue;
   if (partial)
-    hipLaunchKernelGGL(k_finalize_dt, dim3(count), dim3(1024), 0, st, partial, nb, clk, nmax, pstride, dt_bound);
+    hipLaunchKernelGGL(k_finalize_dt, dim3(count), dim3(1024), 0, st, const_cast<double*>(partial), nb, clk, nmax,
+                       pstride, dt_bound, clear);
   else hipLaunchKernelGGL(k_fixed_dt, dim3(1), dim3(count), 0, st, clk, count);
   return hipGetLastError();
 }

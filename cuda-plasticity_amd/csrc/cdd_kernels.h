@@ -119,7 +119,7 @@ hipError_t launch_flux(const FluxJob& job, int* nblocks_out);
 // Persistent multi-instance 2D flux: one launch covers every instance of up
 // to kFluxGroups size groups.  The work items are (group, instance, tile) in
 // group order; a fixed grid of blocks walks them with a stride of the grid
-// size, each block fetching its next tile while it computes the current one.
+// size.
 // Per group: instance i's fields are A + i fstride (likewise U and out),
 // G + i gstride, its clock clk + i and its per-tile CFL partials
 // partial + i pstride (pstride >= the instance's tile count).
@@ -137,8 +137,8 @@ struct FluxGroup {
 struct FluxMultiJob {
   int ngroups, nitems;  // filled by launch_flux_multi
   int dyn, mode;        // mode as FluxJob (0: out = rhs; 1: out = cu U + cf (f + dt rhs))
-  int tile;             // tile variant (0: 32x8, 1: 64x4, 2: 16x16)
-  int per_cu;           // persistent blocks per CU (1 or 2; 0: 2)
+  int tile;             // tile variant (0: 32x8, 1: 64x4, 2: 16x16; 3: row marching)
+  int per_cu;           // persistent blocks per CU (tiles: 1 or 2, 0: 2; rows: 1..4, 0: 2)
   double cu, cf, Lambda, mu, lam;
   FluxGroup g[kFluxGroups];
 };
@@ -146,11 +146,21 @@ struct FluxMultiJob {
 bool flux_multi_ok(int n, int tile);
 // fills ngroups' item ranges and nitems from g[0..ngroups) and launches
 hipError_t launch_flux_multi(FluxMultiJob& job, int ngroups, hipStream_t st);
+// The row-marching variant (sides that are multiples of 128): every block
+// marches an equal share of the (group, instance, strip, row) space; each
+// instance's CFL maximum is maximised atomically into partial[0] of the
+// instance (pstride apart), which must be zero before the launch (batch
+// creation zeroes it; launch_finalize_dt with clear resets it).  per_cu:
+// blocks per CU (1..4, 0: 2).  ntiles/item0 become row-step counts.
+bool flux_rows_ok(int n);
+hipError_t launch_flux_rows(FluxMultiJob& job, int ngroups, hipStream_t st);
 // the Green's-tensor, elastic and load constants of a configuration
 GreenParams green_params(const cdd_config& cfg);
 int flux_nblocks(int d, int n);
+// clear (with nblocks = 1): the partial is one slot that k_flux_rows
+// maximises atomically; it is reset to zero once read
 hipError_t launch_finalize_dt(const double* partial, int nblocks, Clock* clk, double nmax, hipStream_t st,
-                              int count = 1, long long pstride = 0, double dt_bound = 0.0);
+                              int count = 1, long long pstride = 0, double dt_bound = 0.0, int clear = 0);
 hipError_t launch_advance(Clock* clk, hipStream_t st, int count = 1);
 // forward Euler update u += dt L with the clock's dt (skipped once done)
 hipError_t launch_euler(double* u, const double* L, long long n, const Clock* clk, hipStream_t st);

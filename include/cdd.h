@@ -376,10 +376,13 @@ int cdd_batch_get_clock(cdd_batch* batch, int i, double* t, double* last_dt, int
 /* Replays the captured step nsteps times (no host synchronisation). */
 int cdd_batch_step(cdd_batch* batch, int nsteps);
 /* The flux of every stage is one persistent multi-instance launch over all
- * instances of all sizes (a fixed grid of per_cu blocks per CU walking the
- * (size, instance, tile) list), with tile variant `tile` (0: 32x8,
- * 1: 64x4, 2: 16x16 cells); per_cu = 0 launches one flux per size instead.
- * Default: tile 0, per_cu 2. */
+ * instances of all sizes: tile = 0, 1, 2 (32x8, 64x4, 16x16 cells) walks the
+ * (size, instance, tile) list with a fixed grid of per_cu (1 or 2) blocks per
+ * CU; tile = 3 marches rows (sides that are multiples of 128): per_cu (1..4)
+ * blocks per CU, each an equal share of the (size, instance, 128-column
+ * strip, row) space.  per_cu = 0 launches one flux per size instead.  Sides
+ * the chosen tile does not divide keep one launch per size.  Default: tile
+ * 3, per_cu 2. */
 int cdd_batch_set_flux(cdd_batch* batch, int tile, int per_cu);
 /* Autotuner hooks: the per-size flux tile variant (sizes outside the
  * multi-instance launch, or per_cu = 0), the timing (ms per launch, on the

@@ -109,10 +109,11 @@ def test_autotuned_tiles_same_result(amd):
 
 @pytest.mark.parametrize("rk", [2, 3])
 def test_multi_instance_flux_configurations(amd, rk):
-    """Every multi-instance flux configuration (tile variant x persistent
-    blocks per CU, with the sides a tile does not divide left to per-size
-    launches) gives the per-size launches' states bit for bit; one instance
-    ends early."""
+    """Every multi-instance flux configuration (tile walk variant x
+    persistent blocks per CU, and the row march at 1-3 blocks per CU, with
+    the sides a variant does not take left to per-size launches) gives the
+    per-size launches' states and clocks bit for bit; one instance ends
+    early."""
     L = amd.lib
     sizes = [16, 32, 64, 256, 16, 128]
     say("multi-instance configurations", rk, sizes)
@@ -124,7 +125,7 @@ def test_multi_instance_flux_configurations(amd, rk):
     t_end = 1.5 * p.clock()[1]  # the 256 instance stops after its second step
     p.close()
     ref = None
-    for tile, per_cu in [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 1), (2, 2)]:
+    for tile, per_cu in [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 1), (2, 2), (3, 1), (3, 2), (3, 3)]:
         b = L.Batch(sizes, dynamics=0, rk_order=rk, Lambda=0.9, cfl=0.25)
         for i in range(len(sizes)):
             b.set_state(i, betas[i])

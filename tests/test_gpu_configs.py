@@ -103,23 +103,31 @@ def test_batch_of_benchmark_sides_vs_single_plans_and_oracles(env):
     say("batch of sides", sizes)
     betas = [SpectralGaussianField((s, s), float(lens[i]), 0.01, seed=i) for i, s in enumerate(sizes)]
     kw = dict(dynamics=L.GLIDE_ONLY, rk_order=2, cfl=0.25)
-    b = L.Batch(sizes, **kw)
-    plans = []
-    for i, s in enumerate(sizes):
-        b.set_state(i, betas[i])
-        p = L.Plan(2, s, **kw)
-        p.set_state(betas[i])
-        p.set_time(0.0)
-        plans.append(p)
-    b.set_time(-1, 0.0)
     nsteps = 5
-    for step in range(nsteps):
-        b.step(1)
-        for i, p in enumerate(plans):
-            p.step(1)
-            assert torch.equal(b.get_state(i), p.get_state()), (step, sizes[i])
-            assert b.clock(i)[3] == step + 1 and b.clock(i)[0] == p.clock()[0]
-    say("  batch = single plans for %d steps; oracles ..." % nsteps)
+    # the multi-instance flux as the tile walk and as the row march
+    for tile, per_cu in ((0, 2), (3, 3)):
+        b = L.Batch(sizes, **kw)
+        b.set_flux(tile, per_cu)
+        plans = []
+        for i, s in enumerate(sizes):
+            b.set_state(i, betas[i])
+            p = L.Plan(2, s, **kw)
+            p.set_state(betas[i])
+            p.set_time(0.0)
+            plans.append(p)
+        b.set_time(-1, 0.0)
+        for step in range(nsteps):
+            b.step(1)
+            for i, p in enumerate(plans):
+                p.step(1)
+                assert torch.equal(b.get_state(i), p.get_state()), (tile, step, sizes[i])
+                assert b.clock(i)[3] == step + 1 and b.clock(i)[0] == p.clock()[0]
+        say("  batch (flux variant %d, %d blocks/CU) = single plans for %d steps" % (tile, per_cu, nsteps))
+        if tile == 0:
+            b.close()
+            for p in plans:
+                p.close()
+    say("  oracles ...")
     for i, s in enumerate(sizes):
         if s not in (384, 1024) or (s == 384 and i != sizes.index(384)):
             continue
