@@ -1,7 +1,10 @@
 """Autotuner for the batched stepper.
 
-Two choices are timed with HIP events on the batch's own instances and
+Three choices are timed with HIP events on the batch's own instances and
 states (no state change) and installed:
+  * per size class, the stress solve's R2C / C2R row tile (0: the default
+    rows per block, 1: half of them; sides >= 256; cdd_batch_time_spectral,
+    cdd_batch_set_spectral);
   * per size class, the flux tile of the per-size launch (0: 32x8, 1: 64x4,
     2: 16x16 cells of the two fastest axes; cdd_batch_time_flux,
     cdd_batch_set_tile) -- the flux of sizes outside the multi-instance
@@ -37,10 +40,11 @@ def _multi_ok(n, v):
 
 
 def autotune_batch(batch, reps=5, cache_path=None, dynamics=None, verbose=False):
-    """Returns (choices, timings): choices[n] is size n's per-size tile and
-    choices["multi"] = (tile, per_cu) the multi-instance launch's
-    configuration (per_cu 0: per-size launches); timings holds the measured
-    ms per launch of every candidate."""
+    """Returns (choices, timings): choices[n] is size n's per-size flux tile,
+    choices["spectral"][n] its stress-solve row tile and choices["multi"] =
+    (tile, per_cu) the multi-instance launch's configuration (per_cu 0:
+    per-size launches); timings holds the measured ms per launch of every
+    candidate."""
     import torch
     name = torch.cuda.get_device_name(batch.dev)
     cache = {}
@@ -66,6 +70,20 @@ def autotune_batch(batch, reps=5, cache_path=None, dynamics=None, verbose=False)
         choices[n] = best
         if verbose:
             print("autotune n=%d -> %s %s" % (n, VARIANTS[best], timings.get(n, "(cached)")))
+    choices["spectral"] = {}
+    for n in sides:
+        if n < 256:
+            continue
+        key = "%s|%d|%s|spectral" % (name, n, dynamics)
+        if key in cache:
+            best = int(cache[key])
+        else:
+            t = {v: batch.time_spectral(n, v, reps) for v in (0, 1)}
+            timings["spectral %d" % n] = t
+            best = min(t, key=t.get)
+            cache[key] = best
+        batch.set_spectral(n, best)
+        choices["spectral"][n] = best
     counts = {n: batch.sizes.count(n) for n in sides}
     mkey = "%s|multi|%s|%s" % (name, ",".join("%dx%d" % (n, counts[n]) for n in sides), dynamics)
     if mkey in cache:

@@ -54,7 +54,8 @@ EXPORTS = [
     "cdd_state_ptr", "cdd_last_error", "cdd_timing_enable", "cdd_timing_read",
     "cdd_batch_create", "cdd_batch_destroy", "cdd_batch_set_stream", "cdd_batch_size", "cdd_batch_set_state",
     "cdd_batch_get_state", "cdd_batch_set_time", "cdd_batch_get_clock", "cdd_batch_step", "cdd_batch_set_tile",
-    "cdd_batch_time_flux", "cdd_batch_set_flux", "cdd_batch_time_flux_multi",
+    "cdd_batch_time_flux", "cdd_batch_set_flux", "cdd_batch_time_flux_multi", "cdd_batch_set_spectral",
+    "cdd_batch_time_spectral",
     "cdd_flux_vjp", "cdd_step_vjp", "cdd_energy_grad", "cdd_dt_record", "cdd_dt_replay",
     "cdd_segment_vjp", "cdd_lambda_bar", "cdd_stress_record", "cdd_plas_append", "cdd_plas_resume",
     "cdd_energy_record", "cdd_status", "cdd_instability", "cdd_segment_store_bytes", "cdd_segment_forward",
@@ -155,6 +156,8 @@ def _load():
         "cdd_batch_time_flux": ([P, I, I, I, dp], I),
         "cdd_batch_set_flux": ([P, I, I], I),
         "cdd_batch_time_flux_multi": ([P, I, I, I, dp], I),
+        "cdd_batch_set_spectral": ([P, I, I], I),
+        "cdd_batch_time_spectral": ([P, I, I, I, dp], I),
         "cdd_timing_read": ([P, dp, ctypes.POINTER(I), I], I),
     }
     for name, (args, res) in sig.items():
@@ -680,6 +683,15 @@ class Batch:
         """The multi-instance flux launch's tile variant and persistent blocks
         per CU (per_cu = 0: one flux launch per size instead)."""
         check(lib.cdd_batch_set_flux(self.h, int(tile), int(per_cu)))
+
+    def set_spectral(self, n, variant):
+        """Size n's stress-solve row tile (0: default rows per block, 1: half)."""
+        check(lib.cdd_batch_set_spectral(self.h, int(n), int(variant)))
+
+    def time_spectral(self, n, variant, reps=5):
+        ms = ctypes.c_double()
+        check(lib.cdd_batch_time_spectral(self.h, int(n), int(variant), int(reps), ctypes.byref(ms)))
+        return ms.value
 
     def time_flux_multi(self, tile, per_cu, reps=5):
         ms = ctypes.c_double()

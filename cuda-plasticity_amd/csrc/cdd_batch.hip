@@ -43,7 +43,7 @@ extern int cdd_internal_fail(int code, const char* msg);
 namespace {
 
 struct Group {
-  int n = 0, count = 0, tile = 0, nblocks = 0;
+  int n = 0, count = 0, tile = 0, stile = 0, nblocks = 0;
   long long cells = 0, fstride = 0, gstride = 0, sstride = 0;
   double* state[2] = {nullptr, nullptr};
   double* L0 = nullptr;
@@ -153,6 +153,7 @@ static hipError_t group_spectral(cdd_batch* b, Group& g, const StageIO& io, hipS
   j.count = g.count;
   j.sstride = g.sstride;
   j.ostride = g.gstride;
+  j.stile = g.stile;
   return launch_spectral(j);
 }
 
@@ -582,6 +583,46 @@ int cdd_batch_time_flux(cdd_batch* b, int n, int variant, int reps, double* ms) 
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (zero_partials(b)) return CDD_ERR_HIP;
+  *ms = t / reps;
+  return CDD_OK;
+}
+
+int cdd_batch_set_spectral(cdd_batch* b, int n, int variant) {
+  if (!b || variant < 0 || variant > 1) return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
+  for (Group& g : b->groups)
+    if (g.n == n) g.stile = variant;
+  free_graphs(b);
+  return CDD_OK;
+}
+
+// Autotuner hook: mean time (ms) of `reps` first-stage stress solves of size
+// group n (every instance) with R2C / C2R row tile `variant`, on the current
+// states (writes only the group's stress planes and spectrum workspace).
+int cdd_batch_time_spectral(cdd_batch* b, int n, int variant, int reps, double* ms) {
+  if (!b || !ms || reps < 1 || variant < 0 || variant > 1) return cdd_internal_fail(CDD_ERR_ARG, "bad argument");
+  BCHK(hipSetDevice(b->cfg.device));
+  Group* gp = nullptr;
+  for (Group& g : b->groups)
+    if (g.n == n) gp = &g;
+  if (!gp) return cdd_internal_fail(CDD_ERR_ARG, "no instances of this size");
+  Group& g = *gp;
+  const int s0 = g.stile;
+  g.stile = variant;
+  const StageIO io = stage_io(b, g, 0, 0);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = group_spectral(b, g, io, b->stream);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = group_spectral(b, g, io, b->stream);
+  if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  g.stile = s0;
+  if (e != hipSuccess) return cdd_internal_fail(CDD_ERR_HIP, hipGetErrorString(e));
   *ms = t / reps;
   return CDD_OK;
 }

@@ -62,6 +62,9 @@ struct SpectralJob {
   const double* vac;
   double valpha;
   const double* valpha_f;  // optional: alpha as a field (one plane), in place of valpha
+  // R2C / C2R row tile: 0 the default rows per block, 1 half of them (2D
+  // fp64 stress solves of sides >= 256; otherwise the default)
+  int stile;
 };
 
 hipError_t launch_spectral(const SpectralJob& job);

@@ -67,12 +67,14 @@ __device__ __forceinline__ void src_comps(int c, int& k0, int& k1) {
 // grid: (N / TY, NO, npairs); block: TY * TT.  NO = N in 3D (outer x), 1 in 2D.
 // (two waves per SIMD: the LDS tile admits two blocks per CU, and the
 // register budget is held to match)
-template <typename T, int N, bool SYM>
-__global__ void __launch_bounds__(Tiles<N>::TY * Tiles<N>::TT, 2)
+// TY: rows per block (Tiles<N>::TY, or a smaller power of two for more,
+// smaller blocks).
+template <typename T, int N, bool SYM, int TY = Tiles<N>::TY>
+__global__ void __launch_bounds__(TY * Tiles<N>::TT, 2)
 k_r2c_last(SrcField<T> src, cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, int NO, int NC,
            long long sstride) {
   using TL = Tiles<N>;
-  constexpr int TT = TL::TT, TY = TL::TY, P = TL::P, V = FFTPlan<N>::V, H = N / 2 + 1;
+  constexpr int TT = TL::TT, P = TL::P, V = FFTPlan<N>::V, H = N / 2 + 1;
   // 3D: blockIdx.y is the outer x index; batched 2D: it is the instance
   const int inst = (NO == 1) ? (int)blockIdx.y : 0;
   const int o = (NO == 1) ? 0 : (int)blockIdx.y;
@@ -172,13 +174,13 @@ k_r2c_last(SrcField<T> src, cx<T>* __restrict__ spec, const cx<T>* __restrict__ 
 
 // --------------------------------------------------------------- c2r_last
 // Inverse of k_r2c_last: reads [c][k][o][r], writes real out[c][o][r][e].
-template <typename T, int N>
-__global__ void __launch_bounds__(Tiles<N>::TY * Tiles<N>::TT)
+template <typename T, int N, int TY = Tiles<N>::TY>
+__global__ void __launch_bounds__(TY * Tiles<N>::TT)
 k_c2r_last(const cx<T>* __restrict__ spec, T* __restrict__ out, long long cs, const cx<T>* __restrict__ tw,
            int NO, int NC, const Clock* clk, long long sstride, long long ostride, const T* __restrict__ vac,
            T valpha, const T* __restrict__ valpha_f) {
   using TL = Tiles<N>;
-  constexpr int TT = TL::TT, TY = TL::TY, P = TL::P, V = FFTPlan<N>::V, H = N / 2 + 1;
+  constexpr int TT = TL::TT, P = TL::P, V = FFTPlan<N>::V, H = N / 2 + 1;
   const int inst = (NO == 1) ? (int)blockIdx.y : 0;
   const int o = (NO == 1) ? 0 : (int)blockIdx.y;
   spec += inst * sstride;
@@ -565,10 +567,20 @@ static hipError_t launch_spectral_t(const SpectralJob& job) {
   auto hook = [&](int kind) {
     if (job.hook) job.hook(job.hook_ctx, kind);
   };
-  if (NC == 6)
+  // row-tile variant 1: half the rows per R2C / C2R block (2D fp64 stress
+  // solves of sides >= 256; twice the blocks, half the LDS each)
+  constexpr int TYH = TL::TY >= 2 ? TL::TY / 2 : 1;
+  constexpr bool HALF_OK = N >= 256 && sizeof(T) == 8 && TL::TY >= 2;
+  const bool half = HALF_OK && job.stile == 1 && d == 2 && NC == 6;
+  dim3 g1h(N / TYH, count, npairs), b1h(TYH * TL::TT);
+  if (half) {
+    if constexpr (HALF_OK)
+      hipLaunchKernelGGL((k_r2c_last<T, N, true, TYH>), g1h, b1h, 0, st, src, spec, tw, NO, NC, job.sstride);
+  } else if (NC == 6) {
     hipLaunchKernelGGL((k_r2c_last<T, N, true>), g1, b1, 0, st, src, spec, tw, NO, NC, job.sstride);
-  else
+  } else {
     hipLaunchKernelGGL((k_r2c_last<T, N, false>), g1, b1, 0, st, src, spec, tw, NO, NC, job.sstride);
+  }
   hook(KK_R2C);
   if (d == 3) {
     dim3 g2(N / TL::TW, H, NC), b2(TL::TW * TL::TT);
@@ -607,9 +619,16 @@ static hipError_t launch_spectral_t(const SpectralJob& job) {
     hook(KK_COLS_INV);
   }
   if (job.vac && (NC != 6 || sizeof(T) != 8)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_c2r_last<T, N>), g1, b1, 0, st, spec, out, cs, tw, NO, NC, job.gp.clk, job.sstride,
-                     job.ostride, reinterpret_cast<const T*>(job.vac), T(job.valpha),
-                     reinterpret_cast<const T*>(job.valpha_f));
+  if (half) {
+    if constexpr (HALF_OK)
+      hipLaunchKernelGGL((k_c2r_last<T, N, TYH>), g1h, b1h, 0, st, spec, out, cs, tw, NO, NC, job.gp.clk,
+                         job.sstride, job.ostride, reinterpret_cast<const T*>(job.vac), T(job.valpha),
+                         reinterpret_cast<const T*>(job.valpha_f));
+  } else {
+    hipLaunchKernelGGL((k_c2r_last<T, N>), g1, b1, 0, st, spec, out, cs, tw, NO, NC, job.gp.clk, job.sstride,
+                       job.ostride, reinterpret_cast<const T*>(job.vac), T(job.valpha),
+                       reinterpret_cast<const T*>(job.valpha_f));
+  }
   hook(KK_C2R);
   return hipGetLastError();
 }

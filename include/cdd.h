@@ -391,6 +391,11 @@ int cdd_batch_set_flux(cdd_batch* batch, int tile, int per_cu);
 int cdd_batch_set_tile(cdd_batch* batch, int n, int variant);
 int cdd_batch_time_flux(cdd_batch* batch, int n, int variant, int reps, double* ms);
 int cdd_batch_time_flux_multi(cdd_batch* batch, int tile, int per_cu, int reps, double* ms);
+/* The stress solve's R2C / C2R row tile per size (0: the default rows per
+ * block, 1: half of them; sides >= 256) and the timing of one stress solve
+ * of a size group with a variant. */
+int cdd_batch_set_spectral(cdd_batch* batch, int n, int variant);
+int cdd_batch_time_spectral(cdd_batch* batch, int n, int variant, int reps, double* ms);
 
 const char* cdd_last_error(void);
 
