@@ -108,6 +108,22 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
   constexpr int VR = JY ? 3 : 4;  // vbuf rows per corner
   const T ME = T(kME);
   const T MEU = ME / (nscale * nscale);  // ME in the units of u^2
+  const T third = T(1.0 / 3.0);
+  // climb suppression: glide + climb uses Lambda tr(sigma)/3 and q; the woJzz
+  // variant (2D) Lambda tr(sigma)/2 and w' (CentralUpwindHJBetaPDynamics_woJzz.py:71-75)
+  const T st = (DYN == DYN_GLIDE_CLIMB) ? Lambda * (sg[0] + sg[4] + sg[8]) * third
+               : (DYN == DYN_WOJZZ)    ? (sg[0] + sg[4] + sg[8]) * T(0.5) * Lambda
+                                       : T(0);
+  const T jfac = (DYN == DYN_WOJZZ) ? -Lambda * T(0.5) : -Lambda * third;
+  const bool climb_sup =
+      (DYN == DYN_GLIDE_CLIMB || DYN == DYN_WOJZZ) && (CS < 0 ? (Lambda != T(0)) : (CS == 1));
+  // FOLD (glide + climb with climb suppression compiled in, J from the Y
+  // sums): the corner velocity numerator
+  //   v0_l = sum_a w_a[l] - [l<d] z_l + st ([l<d] tr_l - sum_a u_a[l,a])
+  // is linear in per-axis quantities, so each axis / sign carries
+  //   w''_a[l] = w_a[l] - st u_a[l,a] - [l==a] (z_a - st tr_a)
+  // and a corner only adds three of them.
+  constexpr bool FOLD = PRE && JY && !SC && DYN == DYN_GLIDE_CLIMB && CS == 1;
   // per axis / sign quantities
   T w[D][2][3], zz[D][2], tr[D][2], P[D][2];
 #pragma unroll
@@ -125,6 +141,11 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
         zz[a][s] = z;
       }
       tr[a][s] = ua[0] + ua[4] + ua[8];
+      if constexpr (FOLD) {
+#pragma unroll
+        for (int l = 0; l < 3; ++l) w[a][s][l] = fma(-st, ua[3 * l + a], w[a][s][l]);
+        w[a][s][a] -= fma(-st, tr[a][s], zz[a][s]);
+      }
       T p = T(0);
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
@@ -157,16 +178,6 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
       X12[s][r] = b;
       X20[s][r] = e;
     }
-  const T third = T(1.0 / 3.0);
-  // climb suppression: glide + climb uses Lambda tr(sigma)/3 and q; the woJzz
-  // variant (2D) Lambda tr(sigma)/2 and w' (CentralUpwindHJBetaPDynamics_woJzz.py:71-75)
-  const T st = (DYN == DYN_GLIDE_CLIMB) ? Lambda * (sg[0] + sg[4] + sg[8]) * third
-               : (DYN == DYN_WOJZZ)    ? (sg[0] + sg[4] + sg[8]) * T(0.5) * Lambda
-                                       : T(0);
-  const T jfac = (DYN == DYN_WOJZZ) ? -Lambda * T(0.5) : -Lambda * third;
-  const bool climb_sup =
-      (DYN == DYN_GLIDE_CLIMB || DYN == DYN_WOJZZ) && (CS < 0 ? (Lambda != T(0)) : (CS == 1));
-
   // running max / min of the corner speeds, started at 0: the one-sided
   // speeds are max(0, max v) and -min(0, min v) (Max5/Max9 with 0)
   T amax[3], amin[3];
@@ -181,8 +192,12 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
 #pragma unroll
     for (int a = 0; a < D; ++a) sgn[a] = (c >> (D - 1 - a)) & 1;
     T R[3], v0[3], q[3];
+    if constexpr (FOLD) {
 #pragma unroll
-    for (int l = 0; l < 3; ++l) {
+      for (int l = 0; l < 3; ++l) v0[l] = (w[0][sgn[0]][l] + w[1][sgn[1]][l]) + w[2][sgn[2]][l];
+    }
+#pragma unroll
+    for (int l = 0; l < 3 && !FOLD; ++l) {
       T r = T(0), vv = T(0);
 #pragma unroll
       for (int a = 0; a < D; ++a) {
@@ -198,7 +213,7 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
       v0[l] = vv;
     }
 #pragma unroll
-    for (int a = 0; a < D; ++a) {
+    for (int a = 0; a < D && !FOLD; ++a) {
       if constexpr (PRE) {
         v0[a] -= zz[a][sgn[a]];
       } else {
@@ -209,7 +224,7 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
       }
     }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < 3 && !FOLD; ++a) {
       // q_a = tr(u_a) - sum_b u_b[a, b]; tr(u_a) = 0 for a >= d
       const T tra = (a < D) ? tr[a % D][sgn[a]] : T(0);
       q[a] = tra - R[a];
@@ -222,7 +237,7 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
       wz[2] = -(u[0][sgn[0]][6] + u[D - 1][sgn[D - 1]][7]);
     }
     if (DYN == DYN_GLIDE_CLIMB) {
-      if (climb_sup) {
+      if (climb_sup && !FOLD) {
 #pragma unroll
         for (int a = 0; a < D; ++a) v0[a] += st * tr[a][sgn[a]];
 #pragma unroll
@@ -256,7 +271,7 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
     T vv[3];
 #pragma unroll
     for (int l = 0; l < 3; ++l) vv[l] = v0[l] * rinv;
-    const T J = !climb_sup              ? T(0)
+    const T J = (JY || !climb_sup)      ? T(0)
                 : (DYN == DYN_WOJZZ) ? jfac * (vv[0] * wz[0] + vv[1] * wz[1] + vv[2] * wz[2])
                                      : jfac * (vv[0] * q[0] + vv[1] * q[1] + vv[2] * q[2]);
 #pragma unroll
@@ -340,6 +355,41 @@ __device__ __forceinline__ void cell_flux(const T (&u)[D][2][9], const T (&sg)[9
   T tw[D];
 #pragma unroll
   for (int a = 0; a < D; ++a) tw[a] = am[a] * ap[a] * iat[a];
+  if constexpr (FOLD) {
+    // rhs = -inv h + sum_a (u_a^+ - u_a^-) tw_a with h linear in the same
+    // derivatives: one coefficient per derivative of the cell's component,
+    //   rhs = sum_{a,s} u_a^s[cc] k_a^s + sum_{s,l} u_i^s[l,jj] inv Y_i^s[l] - [i==jj] inv J,
+    //   k_a^+ = tw_a - inv Y_a^+[a],  k_a^- = -tw_a - inv Y_a^-[a]
+    T k[D][2], Yi[D][2][3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      k[a][0] = fma(-inv, Y[a][0][a], tw[a]);
+      k[a][1] = fma(-inv, Y[a][1][a], -tw[a]);
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) Yi[a][s][l] = inv * Y[a][s][l];
+    }
+    const T Jn = -inv * Jsum;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) {
+        const int cc = 3 * i + jj;
+        T r = (i == jj) ? Jn : T(0);
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+          for (int s = 0; s < 2; ++s) r = fma(u[a][s][cc], k[a][s], r);
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+          for (int l = 0; l < 3; ++l) r = fma(u[i][s][3 * l + jj], Yi[i][s][l], r);
+        rhs[cc] = r * nscale;
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
 #pragma unroll
@@ -1131,6 +1181,31 @@ __device__ __forceinline__ void lds_read9(unsigned a, double (&v)[9]) {
       : "v"(a)
       : "memory");
 }
+// The next-plane cells (9 values from a9) and the stress planes (6 from a6)
+// together, under one wait.
+__device__ __forceinline__ void lds_read9_6(unsigned a9, unsigned a6, double (&v)[9], double (&g)[6]) {
+  asm volatile(
+      "ds_read_b64 %0, %15\n\t"
+      "ds_read_b64 %1, %15 offset:512\n\t"
+      "ds_read_b64 %2, %15 offset:1024\n\t"
+      "ds_read_b64 %3, %15 offset:1536\n\t"
+      "ds_read_b64 %4, %15 offset:2048\n\t"
+      "ds_read_b64 %5, %15 offset:2560\n\t"
+      "ds_read_b64 %6, %15 offset:3072\n\t"
+      "ds_read_b64 %7, %15 offset:3584\n\t"
+      "ds_read_b64 %8, %15 offset:4096\n\t"
+      "ds_read_b64 %9, %16\n\t"
+      "ds_read_b64 %10, %16 offset:512\n\t"
+      "ds_read_b64 %11, %16 offset:1024\n\t"
+      "ds_read_b64 %12, %16 offset:1536\n\t"
+      "ds_read_b64 %13, %16 offset:2048\n\t"
+      "ds_read_b64 %14, %16 offset:2560\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]), "=&v"(v[6]), "=&v"(v[7]),
+        "=&v"(v[8]), "=&v"(g[0]), "=&v"(g[1]), "=&v"(g[2]), "=&v"(g[3]), "=&v"(g[4]), "=&v"(g[5])
+      : "v"(a9), "v"(a6)
+      : "memory");
+}
 // The same for fp32 buffers (N floats, 256 bytes apart).
 __device__ __forceinline__ void lds_read6(unsigned a, float (&v)[6]) {
   asm volatile(
@@ -1168,13 +1243,23 @@ __device__ __forceinline__ void lds_read9(unsigned a, float (&v)[9]) {
 // below).  A block owns a TC x TR column of cells (fast axes) and
 // marches a chunk of planes along the slow axis, one cell per thread.  Per
 // plane it DMAs (global_load_lds) into LDS the halo tile of the stage field,
-// the six non-local stress planes G, the slow axis's next plane and (stage
-// combinations) the step-start state U, and carries the slow-axis
-// differences in registers.
+// the six non-local stress planes G and (stage combinations) the step-start
+// state U, and carries the slow-axis differences in registers.
 // The fetches are pipelined so that the memory pipe never drains between
-// planes: the tile of plane p + 2 is fetched into the other of two tile
-// buffers while plane p is computed; G(p + 1) and f(p + 3) as soon as plane
-// p has read theirs, U(p + 1) as soon as U(p) is read.
+// planes, in one of three schemes (NB tile buffers):
+//  - 4 (fp64 stage combinations): a ring of four tile buffers.  Plane p reads
+//    its stencil from tile(p), the slow axis's next cells f(p + 2) from the
+//    centre of tile(p + 2), and fetches tile(p + 3) into the buffer plane
+//    p - 1 used, so every plane of the stage field is fetched once.
+//  - 3 (fp64 first stage, whose shorter plane leaves too little time for a
+//    tile fetched mid-plane to land by the plane's end): three tile
+//    buffers, tile(p + 2) fetched into plane p - 1's, and the next cells
+//    f(p + 3) into a per-wave buffer of their own.
+//  - 2 (fp32, two blocks per CU at 80 KB of LDS each): two tile buffers,
+//    tile(p + 2) fetched into plane p's after a second barrier, and the next
+//    cells as for 3.
+// One barrier per plane for 4 and 3.  G(p + 1) goes out as soon as plane p has
+// read G(p), U(p + 1) as soon as U(p) is read.
 // CLIMB: Lambda != 0 (climb suppression terms compiled in); COMB: a stage
 // combination (mode != 0, reads U) rather than the bare rhs.  The host picks
 // the instance from the job; both are compile-time so that the compiler's
@@ -1192,17 +1277,22 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
   const T* __restrict__ Up = reinterpret_cast<const T*>(job.U);
   T* __restrict__ outp = reinterpret_cast<T*>(job.out);
   if (job.clk && job.clk->done) return;
-  // two tile buffers as separate arrays (planes of even / odd offset in the
-  // chunk), so the compiler sees that reads of one never wait for the DMA
-  // into the other; 161824 bytes of LDS in all for fp64, 80 KB for fp32
-  __shared__ T tile0[DM::TBUF];
-  __shared__ T tile1[DM::TBUF];
+  // NB 4: the ring as one array (155 KB of LDS in all); no tile DMA is in
+  // flight while any of its buffers is read.  NB 3 and 2: the tile buffers
+  // as separate arrays (plane k of the chunk in buffer k % NB), so the
+  // compiler sees that reads of one never wait for the DMA into another.
+  constexpr int NB = sizeof(T) == 8 ? (COMB ? 4 : 3) : 2;
+  constexpr bool R4 = NB == 4;
+  __shared__ T tring[R4 ? 4 : 1][R4 ? DM::TBUF : 1];
+  __shared__ T tile0[R4 ? 1 : DM::TBUF];
+  __shared__ T tile1[R4 ? 1 : DM::TBUF];
+  __shared__ T tile2[NB == 3 ? DM::TBUF : 1];
   // fp32: the corner velocities are parked in LDS (two blocks per CU need
   // the registers); fp64 (one block per CU) keeps them in registers
-  __shared__ T vsh[sizeof(T) == 8 ? 1 : 3 * 8][NT];
-  __shared__ T gb[NT / 64][6][64];    // stress planes
-  __shared__ T nbuf[NT / 64][9][64];  // the slow axis's next plane
-  __shared__ T ub[NT / 64][9][64];    // U cells
+  __shared__ T vsh[sizeof(T) == 8 ? 1 : 3 * 8][sizeof(T) == 8 ? 1 : NT];
+  __shared__ T gb[NT / 64][6][64];                       // stress planes
+  __shared__ T nbuf[R4 ? 1 : NT / 64][R4 ? 1 : 9][64];  // NB 3, 2: the slow axis's next plane
+  __shared__ T ub[NT / 64][9][64];                       // U cells
   __shared__ T red[NT / 64];
   const int n = job.n;
   const long long cells = job.cells, splane = (long long)n * n;
@@ -1243,7 +1333,7 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
   const unsigned poff = (unsigned)((r0 + e2 / TC) * n + c0 + e2 % TC);
   const unsigned woff = (unsigned)((r0 + 2 * wv) * n + c0 + (lane >> 1));  // + n: the wave's second row
   const unsigned xoff = (unsigned)((r0 + trr) * n + c0 + tc);
-  const unsigned a_gb = lds_addr(&gb[wv][0][lane]), a_nb = lds_addr(&nbuf[wv][0][lane]),
+  const unsigned a_gb = lds_addr(&gb[wv][0][lane]), a_nb = lds_addr(&nbuf[R4 ? 0 : wv][0][lane]),
                  a_ub = lds_addr(&ub[wv][0][lane]);
 
   // Planes wrap periodically, so a fetch past the chunk reads valid cells
@@ -1303,11 +1393,17 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
   // it forms m_p = minmod(D_{p+1} - D_p, s_p) and u+(p) = D_p - m_p/2 and
   // hands on D_{p+1} - D_p and D_p + m_p/2.  Every value comes from the same
   // operands by the same operation as in limited_derivs, so the result is
-  // bit-identical to it.  f(p+2) arrives in nbuf one plane ahead.
-  dma_tile(x0, tile0);
-  dma_tile(x0 + 1, tile1);
+  // bit-identical to it.
+  if constexpr (R4) {
+    dma_tile(x0, tring[0]);
+    dma_tile(x0 + 1, tring[1]);
+    dma_tile(x0 + 2, tring[2]);
+  } else {
+    dma_tile(x0, tile0);
+    dma_tile(x0 + 1, tile1);
+    dma_9(A, x0 + 2, nbuf[wv]);
+  }
   dma_g(x0);
-  dma_9(A, x0 + 2, nbuf[wv]);
   if constexpr (COMB) dma_9(Up, x0, ub[wv]);
   T xf[9], xd[9], xs[9], xum[9];
   {
@@ -1328,55 +1424,98 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
   T run_max = -INFINITY;
   const int lr = trr + 2, lc = tc + 2;
 
-  // One plane.  Per wave, the vector-memory issue order of a step is
-  //   [G(p+1), next(p+3)] [tile(p+2)] [stores(p)] [U(p+1)]
+  // One plane, its stencil tile in tcur; tpre receives the tile of p + 3
+  // (NB 4; tnx holds tile(p + 2)) or of p + 2 (NB 3, 2).  Per wave, the
+  // vector-memory issue order of a step is
+  //   NB 4:    [G(p+1)] [tile(p+3)] [stores(p)] [U(p+1)]
+  //   NB 3, 2: [G(p+1), next(p+3)] [tile(p+2)] [stores(p)] [U(p+1)]
   // so U(p) (issued last on the step before) is complete once at most
-  // N_GN + N_T younger operations are outstanding, and G(p+1), next(p+3)
-  // and tile(p+1) (issued on the step before) once at most N_T + N_ST
-  // (+ N_9 for U(p+1)) are.
-  auto step = [&](int p, T* tcur) __attribute__((always_inline)) {
-    // tile(p), G(p) and next(p+2) (this wave's part) landed at the wait that
-    // closed the previous step; the barrier publishes every wave's part
+  // N_G + N_T (N_GN + N_T) younger operations are outstanding.  A step ends
+  // once G(p+1) and tile(p+3) (NB 4), or G(p+1), next(p+3) and tile(p+1)
+  // (NB 3, 2; issued on the step before), are complete: at most N_ST
+  // (N_T + N_ST) outstanding, + N_9 for U(p+1).
+  constexpr int N_G = EPL == 2 ? 3 : 6;
+  auto step = [&](int p, T* tcur, T* tpre, const T* tnx) __attribute__((always_inline)) {
+    // tile(p), G(p) and (fp32) next(p+2), this wave's part, landed at the
+    // wait that closed the previous step; the barrier publishes every wave's
+    // part
     lds_barrier();
     T nv[9], gv[6];
-    lds_read9(a_nb, nv);
-    lds_read6(a_gb, gv);
+    if constexpr (R4) {
+      lds_read6(a_gb, gv);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) nv[c] = tnx[c * TILE + lr * PW + lc];
+    } else if constexpr (sizeof(T) == 8) {
+      lds_read9_6(a_nb, a_gb, nv, gv);
+    } else {
+      lds_read9(a_nb, nv);
+      lds_read6(a_gb, gv);
+    }
     // this wave's G and next-plane cells are read (the reads complete inside
     // lds_read*): refill them for the next plane (per-wave buffers, no
-    // barrier).  Without the combination the refill goes out at once, under
-    // the plane's whole arithmetic; with it (U in flight too) after the
-    // derivatives, which measured faster there.
-    if constexpr (!COMB) {
+    // barrier).  Without the combination, or with three tile buffers, the
+    // refill goes out at once, under the plane's whole arithmetic; with the
+    // combination and two buffers (U in flight too) after the derivatives,
+    // which measured faster there.
+    if constexpr (R4) {
+      dma_g(p + 1);
+    } else if constexpr (!COMB) {
       dma_g(p + 1);
       dma_9(A, p + 3, nbuf[wv]);
     }
+    // The in-plane stencil values are read in three groups of three
+    // components, each group's LDS reads issued before the arithmetic of the
+    // group before it, so that one wave per SIMD does not sit out every read's
+    // latency (the scheduling barriers keep the compiler from sinking the
+    // reads to their first use).
     T u[3][2][9], fc[9];
+    T sv[2][3][9];  // per component: z - 2 .. z + 2, then y - 2, y - 1, y + 1, y + 2
+    auto read_group = [&](int g, T (&s)[3][9]) __attribute__((always_inline)) {
 #pragma unroll
-    for (int c = 0; c < 9; ++c) {
-      const T* tb = tcur + c * TILE;
-      const T ctr = tb[lr * PW + lc];
-      fc[c] = ctr;
-      {
-        T f[5] = {tb[lr * PW + lc - 2], tb[lr * PW + lc - 1], ctr, tb[lr * PW + lc + 1], tb[lr * PW + lc + 2]};
-        limited_derivs<T>(f, u[2][0][c], u[2][1][c]);
+      for (int k = 0; k < 3; ++k) {
+        const T* tb = tcur + (3 * g + k) * TILE;
+#pragma unroll
+        for (int o = 0; o < 5; ++o) s[k][o] = tb[lr * PW + lc - 2 + o];
+        s[k][5] = tb[(lr - 2) * PW + lc];
+        s[k][6] = tb[(lr - 1) * PW + lc];
+        s[k][7] = tb[(lr + 1) * PW + lc];
+        s[k][8] = tb[(lr + 2) * PW + lc];
       }
-      {
-        T f[5] = {tb[(lr - 2) * PW + lc], tb[(lr - 1) * PW + lc], ctr, tb[(lr + 1) * PW + lc], tb[(lr + 2) * PW + lc]};
-        limited_derivs<T>(f, u[1][0][c], u[1][1][c]);
+    };
+    read_group(0, sv[0]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+      if (g < 2) read_group(g + 1, sv[(g + 1) & 1]);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int c = 3 * g + k;
+        const T* s = sv[g & 1][k];
+        fc[c] = s[2];
+        {
+          T f[5] = {s[0], s[1], s[2], s[3], s[4]};
+          limited_derivs<T>(f, u[2][0][c], u[2][1][c]);
+        }
+        {
+          T f[5] = {s[5], s[6], s[2], s[7], s[8]};
+          limited_derivs<T>(f, u[1][0][c], u[1][1][c]);
+        }
+        {  // slow axis, from the carried differences
+          const T D1 = nv[c] - xf[c];
+          const T s1 = D1 - xd[c];
+          const T m = minmod2(s1, xs[c]);
+          u[0][0][c] = fma(T(-0.5), m, xd[c]);
+          u[0][1][c] = xum[c];
+          xum[c] = fma(T(0.5), m, xd[c]);
+          xf[c] = nv[c];
+          xs[c] = s1;
+          xd[c] = D1;
+        }
       }
-      {  // slow axis, from the carried differences
-        const T D1 = nv[c] - xf[c];
-        const T s1 = D1 - xd[c];
-        const T m = minmod2(s1, xs[c]);
-        u[0][0][c] = fma(T(-0.5), m, xd[c]);
-        u[0][1][c] = xum[c];
-        xum[c] = fma(T(0.5), m, xd[c]);
-        xf[c] = nv[c];
-        xs[c] = s1;
-        xd[c] = D1;
-      }
+      __builtin_amdgcn_sched_barrier(0);
     }
-    if constexpr (COMB) {
+    if constexpr (COMB && !R4) {
       dma_g(p + 1);
       dma_9(A, p + 3, nbuf[wv]);
     }
@@ -1401,9 +1540,17 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
         for (int c = 0; c < 9; ++c) asm volatile("" : "+v"(u[a][s][c]));
 #pragma unroll
     for (int c = 0; c < 9; ++c) asm volatile("" : "+v"(sg[c]), "+v"(fc[c]));
-    // every wave is done with tile(p) before tile(p + 2) refills its buffer
-    lds_barrier();
-    dma_tile(p + 2, tcur);
+    // NB 4, 3: every wave has finished plane p - 1 (the barrier that opened
+    // this plane), so its buffer takes the tile of p + 3 (p + 2); NB 2: every
+    // wave is done with tile(p) before tile(p + 2) refills its buffer.
+    // Issued after the stencil reads, so that the DMA does not compete with
+    // them for the LDS.
+    if constexpr (R4) {
+      dma_tile(p + 3, tpre);
+    } else {
+      if constexpr (NB == 2) lds_barrier();
+      dma_tile(p + 2, tpre);
+    }
     T rhs[9], speed;
     if constexpr (sizeof(T) == 8) {
       T vreg[3 * 8];
@@ -1417,7 +1564,7 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
     run_max = fmax(run_max, fin ? speed : T(INFINITY));
     T* o = outp + (long long)p * splane + xoff;
     if constexpr (COMB) {
-      wait_vmcnt<N_GN + N_T>();  // U(p)
+      wait_vmcnt<(R4 ? N_G : N_GN) + N_T>();  // U(p)
       T uv[9], ov[9];
       lds_read9(a_ub, uv);
 #pragma unroll
@@ -1431,16 +1578,32 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
         wait_vmcnt<0>();
       }
       dma_9(Up, p + 1, ub[wv]);
-      wait_vmcnt<N_T + N_ST + N_9>();
+      wait_vmcnt<(R4 ? 0 : N_T) + N_ST + N_9>();
     } else {
 #pragma unroll
       for (int c = 0; c < 9; ++c) o[c * cells] = rhs[c];
-      wait_vmcnt<N_T + N_ST>();
+      wait_vmcnt<(R4 ? 0 : N_T) + N_ST>();
     }
   };
-  for (int p = x0; p < x0 + chunk; p += 2) {
-    step(p, tile0);
-    step(p + 1, tile1);
+  if constexpr (R4) {
+    for (int p = x0; p < x0 + chunk; ++p) {
+      const int k = p - x0;
+      step(p, tring[k & 3], tring[(k + 3) & 3], tring[(k + 2) & 3]);
+    }
+  } else if constexpr (NB == 3) {
+    int p = x0;
+    for (; p + 3 <= x0 + chunk; p += 3) {
+      step(p, tile0, tile2, nullptr);
+      step(p + 1, tile1, tile0, nullptr);
+      step(p + 2, tile2, tile1, nullptr);
+    }
+    if (p < x0 + chunk) step(p, tile0, tile2, nullptr);
+    if (p + 1 < x0 + chunk) step(p + 1, tile1, tile0, nullptr);
+  } else {
+    for (int p = x0; p < x0 + chunk; p += 2) {
+      step(p, tile0, tile0, nullptr);
+      step(p + 1, tile1, tile1, nullptr);
+    }
   }
   // nothing may still write this block's LDS when it exits
   wait_vmcnt<0>();
