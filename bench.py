@@ -100,11 +100,17 @@ def cpu_baseline(n=256):
 
 def bench_loading(args):
     """3D n^3 (default 256) glide+climb, climb mobility 0.1, under the uniaxial
-    strain-rate control ramped to a strain of 0.01 over 1000 TVD-RK3 steps:
-    the forward run with its per-step stress-strain record, then the
-    checkpointed adjoint of the final mean stress sigma_zz with respect to the
-    beta^P that starts the last 100 steps (back through those 100 steps,
-    checkpoints every 10)."""
+    strain-rate control, 1000 TVD-RK3 steps: the forward run with its per-step
+    stress-strain record, then the checkpointed adjoint of the final mean
+    stress sigma_zz with respect to the beta^P that starts the last 100 steps
+    (back through those 100 steps, checkpoints every 10).  The steps follow
+    the CFL rule (CalculateOneAdaptiveTimeStep, as RunLoadingMPI's mover
+    does), so the strain 1000 steps reach at a given rate is only known after
+    the run: two pilot runs of 1000 steps, at the rate r0 = 0.01 / (1000 dt0)
+    (dt0 the initial CFL step) and at r1 = 0.01 / T0 (T0 the time the first
+    covered), fix the rate by the secant of log strain against log rate, and
+    the measured run ramps at that rate; the strain it reaches and its CFL
+    numbers are reported."""
     import numpy as np
     import torch
     from Plasticity.amd import lib as L
@@ -122,16 +128,35 @@ def bench_loading(args):
     dt0 = 0.25 / (n * float(speed.max()))
     probe.close()
     del speed
-    rate = 0.01 / (total * dt0)
+    def pilot(r):
+        pl = L.Plan(3, n, dynamics=L.GLIDE_CLIMB, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25,
+                    load_kind=L.LOAD_UNIAXIAL, load_init_strain=init_strain, load_uniaxial_rate=r)
+        pl.set_state(beta_t)
+        pl.set_time(0.0)
+        pl.step(total)
+        tt = pl.clock()[0]
+        pl.close()
+        print("pilot: rate %.4g, %d CFL steps cover t = %.6g, strain %.4g" % (r, total, tt, r * tt),
+              file=sys.stderr, flush=True)
+        return r * tt
+    r0 = 0.01 / (total * dt0)
+    s0 = pilot(r0)
+    r1 = r0 * 0.01 / s0
+    s1 = pilot(r1)
+    rate = r1 * (0.01 / s1) ** (np.log(r0 / r1) / np.log(s0 / s1))
+    pilots = [[r0, s0], [r1, s1]]
     load = dict(load_kind=L.LOAD_UNIAXIAL, load_init_strain=init_strain, load_uniaxial_rate=rate)
     plan = L.Plan(3, n, dynamics=L.GLIDE_CLIMB, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25, **load)
     plan.set_state(beta_t)
-    # fixed steps of the initial CFL step (CalculateOneFixedTimeStep), so the
-    # ramp reaches strain 0.01 exactly at step 1000; the speeds only fall
-    # during the run, so every step stays within the CFL bound
-    plan.set_time(0.0, float("inf"), dt0)
+    # CFL steps; every step's CFL speed is logged on the device and the run's
+    # CFL number checked afterwards
+    plan.set_time(0.0)
     slog = torch.zeros((total, 8), dtype=torch.float64, device=dev)
     plan.stress_record(slog)
+    vlog = torch.zeros(total, dtype=torch.float64, device=dev)
+    plan.speed_record(vlog)
+    dtl = torch.zeros(total, dtype=torch.float64, device=dev)
+    plan.dt_record(dtl)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     plan.step(total - window)
@@ -144,8 +169,11 @@ def bench_loading(args):
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     plan.stress_record(None)
+    plan.speed_record(None)
+    plan.dt_record(None)
     t_end = plan.clock()[0]
     fwd_s = (t1 - t0) + (t2 - t1)
+    cfl_num = n * (vlog * dtl).cpu().numpy()  # N max speed dt per step
     rec = slog.cpu().numpy()
     print("forward %d steps: %.2f s" % (total, fwd_s), file=sys.stderr, flush=True)
     curve = [[float(rate * rec[k, 0]), float(rec[k, 3])] for k in range(0, total, 100)]
@@ -156,8 +184,7 @@ def bench_loading(args):
     lt = torch.tensor(0.9, dtype=torch.float64, device=dev, requires_grad=True)
     torch.cuda.synchronize()
     ta = time.perf_counter()
-    J = mean_stress_window(b, lt, window, 1, "zz", t0=t_mid, checkpoint_every=ck, rk_order=3, load=load,
-                           fixed_dt=dt0)
+    J = mean_stress_window(b, lt, window, 1, "zz", t0=t_mid, checkpoint_every=ck, rk_order=3, load=load)
     torch.cuda.synchronize()
     tb = time.perf_counter()
     J.backward()
@@ -176,11 +203,15 @@ def bench_loading(args):
         "data": "synthetic: seeded Gaussian random beta^P (L=6 cells, RMS 0.01)",
         "config": {"workload": "cdd-3d-%d-loading" % n, "grid": [n, n, n], "dynamics": "glide+climb",
                    "Lambda": 0.9, "rk_order": 3, "cfl": 0.25,
-                   "load": "uniaxial strain-rate control, rate %.4g: strain 0.01 at step %d (fixed dt %.4g)"
-                           % (rate, total, dt0),
+                   "load": "uniaxial strain-rate control at rate %.4g, set so that %d CFL steps reach strain 0.01 "
+                           "(two pilot runs; dt0 = %.4g the initial CFL step)" % (rate, total, dt0),
+                   "pilot_runs_rate_strain": pilots,
                    "adjoint_window": window, "checkpoint_every": ck},
         "stress_strain_every_100_steps": curve,
         "final_strain": rate * t_end,
+        "cfl_check": {"cfl": 0.25, "max_cfl_number": float(cfl_num.max()), "at_step": int(cfl_num.argmax()),
+                      "first": float(cfl_num[0]), "last": float(cfl_num[-1]),
+                      "within_cfl": bool(cfl_num.max() <= 0.25 * (1 + 1e-12))},
         "final_mean_sigma_zz": float(J.item()),
         "adjoint": {
             "objective": "mean sigma_zz (load included) of the final state",
@@ -356,10 +387,10 @@ def adjoint_cost_2d(nsteps=400):
 
 def adjoint_cost_3d_window(n=256, window=100, ck=10):
     """3D n^3 glide + climb (Lambda = 0.9) under the uniaxial strain-rate
-    control of the loading ramp (fixed steps of the initial CFL step): the
-    adjoint of the final mean sigma_zz over a window of `window` RK3 steps
-    (checkpoints every ck, stored while memory allows) against the plain
-    forward run of the same steps, from the ramp's start state."""
+    control of the loading ramp (CFL steps): the adjoint of the final mean
+    sigma_zz over a window of `window` RK3 steps (checkpoints every ck, stored
+    while memory allows) against the plain forward run of the same steps,
+    from the ramp's start state."""
     import torch
     from Plasticity.amd import lib as L
     from Plasticity.amd.autograd import mean_stress_window
@@ -377,7 +408,7 @@ def adjoint_cost_3d_window(n=256, window=100, ck=10):
     plan = L.Plan(3, n, dynamics=L.GLIDE_CLIMB, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25, **load)
     for rep in range(2):  # the first run warms up
         plan.set_state(beta0)
-        plan.set_time(0.0, float("inf"), dt0)
+        plan.set_time(0.0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         plan.step(window)
@@ -388,8 +419,7 @@ def adjoint_cost_3d_window(n=256, window=100, ck=10):
     lt = torch.tensor(0.9, dtype=torch.float64, device="cuda", requires_grad=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    J = mean_stress_window(b, lt, window, 1, "zz", t0=0.0, checkpoint_every=ck, rk_order=3, load=load,
-                           fixed_dt=dt0)
+    J = mean_stress_window(b, lt, window, 1, "zz", t0=0.0, checkpoint_every=ck, rk_order=3, load=load)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     J.backward()
@@ -399,7 +429,7 @@ def adjoint_cost_3d_window(n=256, window=100, ck=10):
            "forward_s": t_fwd, "adjoint_forward_s": t1 - t0, "adjoint_backward_s": t2 - t1, "steps": window,
            "objective": "final mean sigma_zz (load included)", "J": J.item(), "dJ_dLambda": lt.grad.item(),
            "config": {"workload": "cdd-3d-%d-loading-window" % n, "grid": [n, n, n], "rk_order": 3,
-                      "checkpoint_every": ck, "fixed_dt": dt0}}
+                      "checkpoint_every": ck, "steps": "CFL", "initial_dt": dt0}}
     del b, lt, J, beta0
     torch.cuda.empty_cache()
     return out

@@ -81,7 +81,8 @@ HEADER_KEYS = {
 EXECUTABLE_CFL = 0.5
 # the switches each dynamics method of RunCUDA compiles to; 'slip' also
 # takes the slip system's own keys (the reference marks 'slip' and 'vac' as
-# incomplete, and the executable here refuses VACANCIES)
+# incomplete; 'vac' sets VACANCIES alone, which the executable here runs
+# only beside a dynamics method that defines lambda)
 METHOD_DEFINES = {
     "gcd": {"lambda": 0},
     "mdp": {"lambda": 1},

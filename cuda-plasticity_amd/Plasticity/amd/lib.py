@@ -58,7 +58,7 @@ EXPORTS = [
     "cdd_batch_time_spectral",
     "cdd_flux_vjp", "cdd_step_vjp", "cdd_energy_grad", "cdd_dt_record", "cdd_dt_replay",
     "cdd_segment_vjp", "cdd_lambda_bar", "cdd_stress_record", "cdd_plas_append", "cdd_plas_resume",
-    "cdd_energy_record", "cdd_status", "cdd_instability", "cdd_segment_store_bytes", "cdd_segment_forward",
+    "cdd_energy_record", "cdd_speed_record", "cdd_status", "cdd_instability", "cdd_segment_store_bytes", "cdd_segment_forward",
     "cdd_segment_vjp_stored", "cdd_set_vacancy", "cdd_vacancy_set", "cdd_vacancy_get",
     "cdd_set_disorder", "cdd_disorder_get", "cdd_disorder_sigma", "cdd_disorder_flux", "cdd_sigma_noalloc",
     "cdd_flux_frozen", "cdd_flux_tangent", "cdd_set_source",
@@ -139,6 +139,7 @@ def _load():
         "cdd_disorder_flux": ([P, P, P, D, P, P, P], I),
         "cdd_stress_record": ([P, P, ctypes.c_longlong], I),
         "cdd_energy_record": ([P, P, ctypes.c_longlong], I),
+        "cdd_speed_record": ([P, P, ctypes.c_longlong], I),
         "cdd_status": ([P, ctypes.POINTER(I)], I),
         "cdd_instability": ([P, P, dp], I),
         "cdd_plas_append": ([P, ctypes.c_char_p], I),
@@ -557,6 +558,15 @@ class Plan:
             check(lib.cdd_energy_record(self.h, None, 0))
         else:
             check(lib.cdd_energy_record(self.h, self._f64_log(log, "energy log"), log.numel()))
+
+    def speed_record(self, log):
+        """Append every step's CFL speed (the largest one-sided corner speed
+        of its first stage) to the float64 device tensor `log` (None stops);
+        logged under fixed time steps too."""
+        if log is None:
+            check(lib.cdd_speed_record(self.h, None, 0))
+        else:
+            check(lib.cdd_speed_record(self.h, self._f64_log(log, "speed log"), log.numel()))
 
     def status(self):
         """0, or 1 if a non-finite speed stopped the run on the device."""

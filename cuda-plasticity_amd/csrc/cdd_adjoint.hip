@@ -1473,6 +1473,9 @@ __global__ void k_clock_segment(Clock* clk, int mode, double t0, const double* d
     c.elog = nullptr;
     c.elog_base = 0;
     c.elog_cap = 0;
+    c.vlog = nullptr;
+    c.vlog_base = 0;
+    c.vlog_cap = 0;
     c.status = 0;
     clk[0] = c;
   } else {

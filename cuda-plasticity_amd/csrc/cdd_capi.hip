@@ -1436,6 +1436,18 @@ int cdd_dt_record(cdd_plan* p, double* log_dev, long long cap) {
   return CDD_OK;
 }
 
+int cdd_speed_record(cdd_plan* p, double* log_dev, long long cap) {
+  if (!p || cap < 0) return fail(CDD_ERR_ARG, "bad argument");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  Clock c;
+  if (sync_clock(p, &c)) return CDD_ERR_HIP;
+  c.vlog = cap > 0 ? log_dev : nullptr;
+  c.vlog_base = c.nsteps;
+  c.vlog_cap = cap;
+  HIPCHK(hipMemcpy(p->clk, &c, sizeof c, hipMemcpyHostToDevice));
+  return CDD_OK;
+}
+
 int cdd_stress_record(cdd_plan* p, double* log_dev, long long cap) {
   if (!p || cap < 0) return fail(CDD_ERR_ARG, "bad argument");
   HIPCHK(hipSetDevice(p->cfg.device));

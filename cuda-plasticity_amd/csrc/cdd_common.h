@@ -70,6 +70,9 @@ struct Clock {
   // optional energy log: total elastic energy of each step's start state
   double* elog;
   long long elog_base, elog_cap;
+  // optional speed log: the step's CFL speed (last_max) at vlog[nsteps - vlog_base]
+  double* vlog;
+  long long vlog_base, vlog_cap;
   // 0: dt = CFL / (N max speed), the end step lands on t_end
   //    (FieldMover.py:94-111); 1: dt = CFL / max speed / N, the end step is
   //    t_end - t + ME (plasticity_kernel.cu simpleTVD :1121-1128)

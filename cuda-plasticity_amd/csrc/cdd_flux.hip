@@ -2215,6 +2215,8 @@ __global__ void k_advance(Clock* clk, int count) {
   if (clk->done) return;
   const long long li = clk->nsteps - clk->log_base;
   if (clk->dt_log && li >= 0 && li < clk->log_cap) clk->dt_log[li] = clk->dt;
+  const long long vi = clk->nsteps - clk->vlog_base;
+  if (clk->vlog && vi >= 0 && vi < clk->vlog_cap) clk->vlog[vi] = clk->last_max;
   clk->t += clk->dt;
   clk->nsteps += 1;
   if (clk->t >= clk->t_end) clk->done = 1;

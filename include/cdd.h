@@ -256,6 +256,13 @@ int cdd_stress_record(cdd_plan* plan, double* log_dev, long long cap);
  *     (Plasticity/FieldMovers/LargeKInstabilityChecking_FieldMover.py:73-89),
  *     computed as sum over cells of (-1)^(i+j[+k]) |rho|. */
 int cdd_energy_record(cdd_plan* plan, double* log_dev, long long cap);
+/* Speed log: from the next step on, every step appends its CFL speed (the
+ * largest one-sided corner speed of its first stage, the max that
+ * FieldMover.CalculateOneAdaptiveTimeStep turns into dt, FieldMover.py:94-96)
+ * to log_dev (cap doubles; 0 stops).  Logged also under a fixed time step
+ * (CalculateOneFixedTimeStep), so a fixed-step run can check its CFL number
+ * N max speed dt afterwards. */
+int cdd_speed_record(cdd_plan* plan, double* log_dev, long long cap);
 int cdd_status(cdd_plan* plan, int* status);
 int cdd_instability(cdd_plan* plan, const void* beta_dev, double* value);
 

@@ -196,8 +196,8 @@ def test_adjoint_is_the_transposed_tangent_512_400_steps(L):
 
 @pytest.mark.timeout(900)
 def test_stress_window_adjoint_is_the_transposed_tangent_3d_256(L):
-    """The 3D 256^3 uniaxial ramp (glide + climb, Lambda = 0.9, fixed steps
-    of the initial CFL step, strain 0.01 after 1000 RK3 steps): the adjoint
+    """The 3D 256^3 uniaxial ramp (glide + climb, Lambda = 0.9, CFL steps,
+    strain rate 0.01 / (1000 dt0), 1000 RK3 steps): the adjoint
     of the final mean sigma_zz (load included) with respect to the beta^P
     that starts the last 100 steps, and to Lambda, against the tangent-linear
     derivative carried on the stepper's own stage states, along 3 random unit
@@ -219,15 +219,14 @@ def test_stress_window_adjoint_is_the_transposed_tangent_3d_256(L):
     load = dict(load_kind=L.LOAD_UNIAXIAL, load_init_strain=init_strain, load_uniaxial_rate=0.01 / (total * dt0))
     plan = L.Plan(3, n, dynamics=L.GLIDE_CLIMB, rk_order=3, Lambda=0.9, mu=1.0, nu=0.3, cfl=0.25, **load)
     plan.set_state(beta0)
-    plan.set_time(0.0, float("inf"), dt0)
+    plan.set_time(0.0)
     plan.step(total - window)
     t_mid = plan.clock()[0]
     beta_mid = plan.get_state()
     del beta0
     b = beta_mid.clone().requires_grad_(True)
     lt = torch.tensor(0.9, dtype=torch.float64, device="cuda", requires_grad=True)
-    J = mean_stress_window(b, lt, window, 1, "zz", t0=t_mid, checkpoint_every=ck, rk_order=3, load=load,
-                           fixed_dt=dt0)
+    J = mean_stress_window(b, lt, window, 1, "zz", t0=t_mid, checkpoint_every=ck, rk_order=3, load=load)
     J.backward()
     grad, lam_grad = b.grad.detach(), lt.grad.item()
     del b
@@ -235,7 +234,7 @@ def test_stress_window_adjoint_is_the_transposed_tangent_3d_256(L):
     gvec = torch.tensor(_mean_stress_grad("zz", load, 1.0, 0.3, n ** 3), dtype=torch.float64, device="cuda")
 
     def dJ(dvec, dlam):
-        u, du, t = tangent_stored(plan, beta_mid, dvec, window, t0=t_mid, dLambda=dlam, fixed_dt=dt0, segment=ck)
+        u, du, t = tangent_stored(plan, beta_mid, dvec, window, t0=t_mid, dLambda=dlam, segment=ck)
         return float((gvec * du.reshape(9, -1).sum(dim=1)).sum())
     rng = np.random.default_rng(256)
     worst = 0.0

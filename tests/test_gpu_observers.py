@@ -48,6 +48,47 @@ def test_energy_record_matches_energy_of_each_step_start(L, d, n):
         assert abs(got[j] - want) <= 1e-12 * abs(want)
 
 
+@pytest.mark.parametrize("d,n,fixed", [(2, 32, False), (3, 32, True)])
+def test_speed_record_is_each_steps_cfl_speed(L, d, n, fixed):
+    """Every step's logged speed is the oracle's largest one-sided corner
+    speed of that step's start state (the max CalculateOneAdaptiveTimeStep
+    turns into dt), under the CFL rule and under a fixed step with the
+    uniaxial strain-rate control (3D 32^3: the plane-marching kernel)."""
+    import torch
+    say("speed record", d, n, fixed)
+    beta = SpectralGaussianField((n,) * d, 2.0, 0.01, seed=24)
+    init = float(beta[8].mean())
+    kw = dict(load_kind=L.LOAD_UNIAXIAL, load_init_strain=init, load_uniaxial_rate=0.05) if d == 3 else {}
+    p = L.Plan(d, n, dynamics=0, rk_order=3, Lambda=0.9, cfl=0.25, **kw)
+    p.set_state(beta)
+    if fixed:
+        p.set_time(0.0, float("inf"), 2e-3)
+    else:
+        p.set_time(0.0)
+    nsteps = 4
+    log = torch.zeros(nsteps, dtype=torch.float64, device="cuda")
+    dts = torch.zeros(nsteps, dtype=torch.float64, device="cuda")
+    p.speed_record(log)
+    p.dt_record(dts)
+    load = O.Load("uniaxial", init_strain=init, uniaxial_rate=0.05) if d == 3 else None
+    dyn = O.Dynamics(O.GLIDE_CLIMB, 0.9, 1.0, 0.3, load=load)
+    want = []
+    for _ in range(nsteps):
+        t = p.clock()[0]
+        _, speed = dyn.flux(t, p.get_state().cpu().numpy())
+        want.append(speed.max())
+        p.step(1)
+    p.speed_record(None)
+    p.dt_record(None)
+    got = log.cpu().numpy()
+    assert np.all(np.abs(got - np.array(want)) <= 1e-11 * np.abs(want))
+    dt = dts.cpu().numpy()
+    if fixed:
+        assert np.all(dt == 2e-3)
+    else:  # the adaptive step is CFL / (N max speed)
+        assert np.all(np.abs(dt * n * got - 0.25) <= 1e-14)
+
+
 @pytest.mark.parametrize("d,n", [(2, 64), (3, 16)])
 def test_instability_matches_oracle(L, d, n):
     say("instability", d, n)

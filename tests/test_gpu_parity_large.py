@@ -106,3 +106,31 @@ def test_config2_3d_128_200_steps(env):
     of host CPU for the C oracle on 16 threads)."""
     say("config 2: 3D 128^3 x 200 RK3 steps vs C oracle")
     _compare(env, 128, 200)
+
+
+def test_3d_256_fp32_vs_fp64(env):
+    """The fp32 plan at the bench grid, where its plane-marching kernel marches
+    128-plane chunks at two blocks per CU: 5 RK3 steps against the fp64 plan
+    on the same field, within a few hundred float32 ulps (relative L2; float32
+    eps = 1.19e-7), both dynamics."""
+    import torch
+    A, _ = env
+    n = 256
+    beta = torch.from_numpy(_field(n, 2)).cuda()
+    for dynamics, Lambda in ((0, 0.9), (1, 0.0)):
+        say("3D 256^3 fp32 vs fp64, dynamics %d" % dynamics)
+        out = []
+        for prec in ("fp64", "fp32"):
+            p = A.Plan(3, n, dynamics=dynamics, rk_order=3, Lambda=Lambda, mu=1.0, nu=0.3, cfl=0.25, precision=prec)
+            p.set_state(beta)
+            p.set_time(0.0)
+            p.step(5)
+            out.append((p.get_state().double(), p.clock()[0]))
+            p.close()
+        (s64, t64), (s32, t32) = out
+        e = float(torch.linalg.norm(s32 - s64) / torch.linalg.norm(s64))
+        say("  rel L2 %.2e, t %.9g vs %.9g" % (e, t32, t64))
+        assert e < 1e-5
+        assert abs(t32 - t64) < 1e-5 * t64
+        del out, s64, s32
+        torch.cuda.empty_cache()
