@@ -55,17 +55,29 @@ class PlasticityState:
 
     # -- physics (PlasticityState.py:85-330)
     def CalculateSigma(self, type='k', source='betaE', time=0.0):
-        """sigma of CalculateSigma(type='k', source='betaE') (:85-127)."""
-        if type != 'k' or source not in ('betaE', 'betaP'):
-            raise NotImplementedError("only the k-space betaE/betaP stress (identical results) is on the GPU path")
+        """CalculateSigma(type, source) (:85-230): source 'betaE' (the
+        stepper's stress; the reference ignores type here), 'betaP' (the
+        M-tensor form) or 'rho' (free boundary, from the Nye tensor), with the
+        ktools ('k') or sinktools ('sinK') k vectors."""
+        if type not in ('k', 'sinK') or source not in ('betaE', 'betaP', 'rho'):
+            raise ValueError("type must be 'k' or 'sinK', source 'betaE', 'betaP' or 'rho'")
         p = self._plan()
+        if source in ('rho', 'betaP'):
+            return TensorField(self.gridShape, data=p.sigma_variant(self.betaP.data, type, source))
         return TensorField(self.gridShape, data=p.sigma(self.betaP.data, time))
 
     def CalculateRhoFourier(self, type='k'):
-        """Nye tensor rho_ij = -eps_ilm d_l beta_mj (:264-304)."""
-        if type != 'k':
-            raise NotImplementedError("only type='k'")
+        """Nye tensor rho_ij = -eps_ilm d_l beta_mj (:264-304), spectral
+        derivatives with the ktools ('k') or sinktools ('sinK')."""
+        if type not in ('k', 'sinK'):
+            raise ValueError("type must be 'k' or 'sinK'")
+        if type == 'sinK':
+            return TensorField(self.gridShape, data=self._plan().rho_variant(self.betaP.data, type))
         return TensorField(self.gridShape, data=self._plan().rho(self.betaP.data))
+
+    def CalculateRhoSymmetric(self):
+        """Nye tensor from central differences (:232-262)."""
+        return TensorField(self.gridShape, data=self._plan().rho_symmetric(self.betaP.data))
 
     def CalculateElasticEnergy(self, type='k', source='betaE'):
         """Energy density 1/2 S_ijkm sigma_ij sigma_km (:317-330), a device array."""

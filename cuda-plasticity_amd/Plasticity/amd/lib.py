@@ -51,6 +51,7 @@ class Config(ctypes.Structure):
 EXPORTS = [
     "cdd_plan_create", "cdd_plan_destroy", "cdd_set_stream", "cdd_set_state", "cdd_get_state",
     "cdd_set_time", "cdd_get_clock", "cdd_sigma", "cdd_rho", "cdd_flux", "cdd_step", "cdd_energy",
+    "cdd_sigma_variant", "cdd_rho_variant", "cdd_rho_symmetric",
     "cdd_state_ptr", "cdd_last_error", "cdd_timing_enable", "cdd_timing_read",
     "cdd_batch_create", "cdd_batch_destroy", "cdd_batch_set_stream", "cdd_batch_size", "cdd_batch_set_state",
     "cdd_batch_get_state", "cdd_batch_set_time", "cdd_batch_get_clock", "cdd_batch_step", "cdd_batch_set_tile",
@@ -109,6 +110,9 @@ def _load():
         "cdd_get_clock": ([P, dp, dp, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_longlong)], I),
         "cdd_sigma": ([P, P, D, P], I),
         "cdd_rho": ([P, P, P], I),
+        "cdd_sigma_variant": ([P, P, I, I, P], I),
+        "cdd_rho_variant": ([P, P, I, P], I),
+        "cdd_rho_symmetric": ([P, P, P], I),
         "cdd_sigma_noalloc": ([ctypes.POINTER(Config), P, D, P, P], I),
         "cdd_flux": ([P, P, D, P, P], I),
         "cdd_step": ([P, I], I),
@@ -295,6 +299,35 @@ class Plan:
         b = self._as_dev(beta)
         out = self.empty()
         check(lib.cdd_rho(self.h, _ptr(b), _ptr(out)))
+        return out
+
+    _KTYPES = {"k": 0, "sinK": 1}
+    _SOURCES = {"betaE": 0, "betaP": 1, "rho": 2}
+
+    def sigma_variant(self, beta, type="k", source="betaE"):
+        """PlasticityState.CalculateSigma(type, source) of beta (no external
+        load): type 'k' or 'sinK', source 'betaE', 'betaP' or 'rho'."""
+        if type not in self._KTYPES or source not in self._SOURCES:
+            raise CDDError("type must be 'k' or 'sinK', source 'betaE', 'betaP' or 'rho'")
+        b = self._as_dev(beta)
+        out = self.empty()
+        check(lib.cdd_sigma_variant(self.h, _ptr(b), self._KTYPES[type], self._SOURCES[source], _ptr(out)))
+        return out
+
+    def rho_variant(self, beta, type="k"):
+        """PlasticityState.CalculateRhoFourier(type) of beta."""
+        if type not in self._KTYPES:
+            raise CDDError("type must be 'k' or 'sinK'")
+        b = self._as_dev(beta)
+        out = self.empty()
+        check(lib.cdd_rho_variant(self.h, _ptr(b), self._KTYPES[type], _ptr(out)))
+        return out
+
+    def rho_symmetric(self, beta):
+        """PlasticityState.CalculateRhoSymmetric of beta (central differences)."""
+        b = self._as_dev(beta)
+        out = self.empty()
+        check(lib.cdd_rho_symmetric(self.h, _ptr(b), _ptr(out)))
         return out
 
     def flux(self, beta, t=0.0):

@@ -480,6 +480,69 @@ int cdd_rho(cdd_plan* p, const void* beta_v, void* rho_v) {
   return CDD_OK;
 }
 
+int cdd_sigma_variant(cdd_plan* p, const void* beta_v, int ktype, int source, void* sigma_v) {
+  const double* beta = static_cast<const double*>(beta_v);
+  double* sigma = static_cast<double*>(sigma_v);
+  if (!p || !beta || !sigma) return fail(CDD_ERR_ARG, "null argument");
+  if (ktype != CDD_KTYPE_K && ktype != CDD_KTYPE_SINK) return fail(CDD_ERR_ARG, "ktype must be CDD_KTYPE_K or CDD_KTYPE_SINK");
+  if (source < CDD_SOURCE_BETAE || source > CDD_SOURCE_RHO) return fail(CDD_ERR_ARG, "bad stress source");
+  if (p->fp32) return fail(CDD_ERR_ARG, "cdd_sigma_variant takes float64 plans");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  if (source == CDD_SOURCE_RHO) {
+    // the reference forms rho in real space (CalculateRhoFourier) and
+    // transforms it again: two spectral chains
+    SpectralJob jr = spectral_job(p, 9, beta, nullptr, nullptr, 0.0, p->work9);
+    jr.gp.sink = ktype;
+    HIPCHK(launch_spectral(jr));
+    SpectralJob js = spectral_job(p, 9, p->work9, nullptr, nullptr, 0.0, sigma);
+    js.gp.sink = ktype;
+    js.gp.rho_sigma = 1;
+    HIPCHK(launch_spectral(js));
+    HIPCHK(launch_sym6_to_9(sigma, p->cells, p->stream));
+    return CDD_OK;
+  }
+  if (source == CDD_SOURCE_BETAP) {
+    // the M-tensor form, local terms and mean correction included
+    SpectralJob jm = spectral_job(p, 6, beta, nullptr, nullptr, 0.0, sigma);
+    memset(&jm.gp.load, 0, sizeof jm.gp.load);
+    jm.gp.sink = ktype;
+    jm.gp.mform = 1;
+    HIPCHK(launch_spectral(jm));
+    HIPCHK(launch_sym6_to_9(sigma, p->cells, p->stream));
+    return CDD_OK;
+  }
+  // the betaE source ignores type (CalculateBetaElastic() is called without it)
+  SpectralJob j = spectral_job(p, 6, beta, nullptr, nullptr, 0.0, p->G);
+  memset(&j.gp.load, 0, sizeof j.gp.load);  // CalculateSigma carries no external load
+  HIPCHK(launch_spectral(j));
+  HIPCHK(launch_sigma_finalize(p->G, beta, sigma, p->cells, p->gp.mu, p->gp.lam, p->stream, 0));
+  return CDD_OK;
+}
+
+int cdd_rho_variant(cdd_plan* p, const void* beta_v, int ktype, void* rho_v) {
+  const double* beta = static_cast<const double*>(beta_v);
+  double* rho = static_cast<double*>(rho_v);
+  if (!p || !beta || !rho) return fail(CDD_ERR_ARG, "null argument");
+  if (ktype != CDD_KTYPE_K && ktype != CDD_KTYPE_SINK) return fail(CDD_ERR_ARG, "ktype must be CDD_KTYPE_K or CDD_KTYPE_SINK");
+  if (p->fp32) return fail(CDD_ERR_ARG, "cdd_rho_variant takes float64 plans");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  SpectralJob j = spectral_job(p, 9, beta, nullptr, nullptr, 0.0, rho);
+  j.gp.sink = ktype;
+  HIPCHK(launch_spectral(j));
+  return CDD_OK;
+}
+
+int cdd_rho_symmetric(cdd_plan* p, const void* beta_v, void* rho_v) {
+  const double* beta = static_cast<const double*>(beta_v);
+  double* rho = static_cast<double*>(rho_v);
+  if (!p || !beta || !rho) return fail(CDD_ERR_ARG, "null argument");
+  if (p->fp32) return fail(CDD_ERR_ARG, "cdd_rho_symmetric takes float64 plans");
+  if (beta == rho) return fail(CDD_ERR_ARG, "cdd_rho_symmetric cannot work in place");
+  HIPCHK(hipSetDevice(p->cfg.device));
+  HIPCHK(launch_rho_symmetric(beta, rho, p->d, p->n, p->cells, p->stream));
+  return CDD_OK;
+}
+
 int cdd_flux(cdd_plan* p, const void* beta_v, double t, void* rhs_v, void* speed_v) {
   const double* beta = static_cast<const double*>(beta_v);
   double* rhs = static_cast<double*>(rhs_v);

@@ -30,6 +30,11 @@ struct GreenParams {
   double vgamma;    // nc = 1: diffusion filter exp(-vgamma dt k^2) (dt from clk)
   int curl;         // nc = 9: the disorder stress sum_lt eps_lit k_l F_tj (real k_l) in
                     // place of the Nye tensor's i k_l
+  int rho_sigma;    // nc = 9: the free-boundary stress K_ijmn rho_mn of a Nye tensor (six
+                    // symmetric outputs, the last three planes zero)
+  int sink;         // 1: the sinktools' k vectors N sin(2 pi n / N) in place of 2 pi n_wrapped
+  int mform;        // nc = 6: the whole stress by the M-tensor form (source 'betaP'), its
+                    // k = 0 mode the mean correction; no load
 };
 
 // Called after each kernel launch with the kernel kind (HIP-event timing).
@@ -68,6 +73,8 @@ struct SpectralJob {
 };
 
 hipError_t launch_spectral(const SpectralJob& job);
+hipError_t launch_sym6_to_9(double* s, long long cells, hipStream_t st);
+hipError_t launch_rho_symmetric(const double* b, double* rho, int d, int n, long long cells, hipStream_t st);
 // the twiddle table W_n^e = exp(-2 pi i e / n), e < n, as 2 n doubles (host)
 void fill_twiddles(double* h, int n);
 hipError_t launch_sigma_finalize(const double* G, const double* beta, double* sig, long long cells, double mu,

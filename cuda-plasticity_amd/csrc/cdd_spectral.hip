@@ -318,6 +318,39 @@ __device__ __forceinline__ void green_sigma(const cx<T>* e, cx<T>* out, T kx, T 
   }
 }
 
+// The whole stress of CalculateSigma(source='betaP') (PlasticityState.py:
+// 151-182) at one k != 0, times 1/N^d: the M tensor summed over the
+// symmetric part e of beta,
+//   sigma_ij = a1 ((d_ij q + k_i k_j t) / K2 - d_ij t) - 2 mu e_ij
+//              - a2 k_i k_j q / K4 + 2 mu (k_i p_j + k_j p_i) / K2,
+// t = tr e, p = e k, q = k.e.k, a1 = 2 mu nu / (1 - nu) = 2 mu A,
+// a2 = 2 mu / (1 - nu) = 2 mu C, K2 = kSq + ME, K4 = kSqSq + ME.  Unlike the
+// fixed-volume form it keeps the local terms, so where the ME
+// regularisation dominates (the sinktools' Nyquist wavenumbers, N sin(pi))
+// the two differ as the reference's sources do.
+template <typename T>
+__device__ __forceinline__ void green_sigma_M(const cx<T>* e, cx<T>* out, T kx, T ky, T kz, const GreenParams& gp) {
+  const T kSq = kx * kx + ky * ky + kz * kz;
+  const T iK2 = T(1) / (kSq + T(kME)), iK4 = T(1) / (kSq * kSq + T(kME));
+  const cx<T> t = e[0] + e[1] + e[2];
+  cx<T> p[3];
+  p[0] = e[0] * kx + e[3] * ky + e[4] * kz;
+  p[1] = e[3] * kx + e[1] * ky + e[5] * kz;
+  p[2] = e[4] * kx + e[5] * ky + e[2] * kz;
+  const cx<T> q = p[0] * kx + p[1] * ky + p[2] * kz;
+  const T mu = T(gp.mu), a1 = T(2) * mu * T(gp.A), a2 = T(2) * mu * T(gp.C);
+  const T k3[3] = {kx, ky, kz};
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const int i = sym_i(c), j = sym_j(c);
+    const T kk = k3[i] * k3[j];
+    cx<T> o = (t * kk) * (a1 * iK2) - e[c] * (T(2) * mu) - q * (a2 * kk * iK4) +
+              (p[j] * k3[i] + p[i] * k3[j]) * (T(2) * mu * iK2);
+    if (i == j) o = o + (q * iK2 - t) * a1;
+    out[c] = o * T(gp.norm);
+  }
+}
+
 // Transpose of green_sigma w.r.t. the Frobenius product of symmetric
 // matrices: with Gb (the 6 components of a symmetric cotangent), tg = tr Gb,
 // gk = Gb k, qg = k.Gb.k,
@@ -348,6 +381,46 @@ __device__ __forceinline__ void green_sigma_T(const cx<T>* g, cx<T>* out, T kx, 
     if (i == j) o = o + dg;
     out[c] = o * T(gp.norm);
   }
+}
+
+// The sinktools' wavenumber of index i (FourierSpaceTools(gridShape,
+// func=sin), PlasticityState.py:35-36): N sin(2 pi i / N), the argument
+// rounded as numpy forms it.
+__device__ __forceinline__ double ksin(int i, int n) {
+  return (double)n * sin((6.283185307179586 * (double)i) / (double)n);
+}
+
+// Free-boundary stress of a Nye tensor (CalculateSigma(source='rho'),
+// PlasticityState.py:128-150), times 1/N^d, six symmetric outputs:
+//   sigma_ij = (-i mu / K2) [X_ij + X_ji + 2C (k_i k_j / K2 - d_ij) tr X],
+//   X_ab = sum_ns eps_bns rho_an k_s (row a of rho crossed with k),
+// K2 = kSq + ME, 2C = 2 / (1 - nu): the reference's
+// sum_mn sum_s (-i mu k_s / K2)(eps_ins d_jm + eps_jns d_im
+//   + 2 eps_mns / (1 - nu)(k_i k_j / K2 - d_ij)) rho_mn, summed per term.
+template <typename T>
+__device__ __forceinline__ void green_sigma_rho(const cx<T>* r, cx<T>* out, T kx, T ky, T kz, const GreenParams& gp) {
+  const T kSq = kx * kx + ky * ky + kz * kz;
+  const T iK2 = T(1) / (kSq + T(kME));
+  const T k3[3] = {kx, ky, kz};
+  cx<T> X[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    X[a][0] = r[3 * a + 1] * kz - r[3 * a + 2] * ky;
+    X[a][1] = r[3 * a + 2] * kx - r[3 * a + 0] * kz;
+    X[a][2] = r[3 * a + 0] * ky - r[3 * a + 1] * kx;
+  }
+  const cx<T> trX = X[0][0] + X[1][1] + X[2][2];
+  const T c2 = T(2) * T(gp.C);
+  const T f = T(gp.mu) * iK2 * T(gp.norm);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const int i = sym_i(c), j = sym_j(c);
+    const T g = k3[i] * k3[j] * iK2 - (i == j ? T(1) : T(0));
+    const cx<T> s = X[i][j] + X[j][i] + trX * (c2 * g);
+    out[c] = mul_pi(s) * (-f);  // -i mu / K2
+  }
+#pragma unroll
+  for (int c = 6; c < 9; ++c) out[c] = cx<T>{T(0), T(0)};
 }
 
 // Nye tensor rho_ij = -eps_ilm d_l beta_mj in k space (times 1/N^d), all
@@ -442,20 +515,44 @@ k_contract(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, GreenParams g
     if (rowk >= NR) continue;
     T kx, ky, kz;
     if constexpr (D == 3) {
-      kz = T(6.283185307179586 * (double)(rowk / N));
-      kx = T(kwrap((int)(rowk % N), N));
-      ky = T(kwrap(e, N));
+      if (gp.sink) {
+        kz = T(ksin((int)(rowk / N), N));
+        kx = T(ksin((int)(rowk % N), N));
+        ky = T(ksin(e, N));
+      } else {
+        kz = T(6.283185307179586 * (double)(rowk / N));
+        kx = T(kwrap((int)(rowk % N), N));
+        ky = T(kwrap(e, N));
+      }
     } else {
-      ky = T(6.283185307179586 * (double)rowk);
+      if (gp.sink) {
+        ky = T(ksin((int)rowk, N));
+        kx = T(ksin(e, N));
+      } else {
+        ky = T(6.283185307179586 * (double)rowk);
+        kx = T(kwrap(e, N));
+      }
       kz = T(0);
-      kx = T(kwrap(e, N));
     }
     cx<T> in[NC], o[NC];
 #pragma unroll
     for (int cc = 0; cc < NC; ++cc) in[cc] = lds[(cc * TR + rr) * P + lds_pad(e)];
     if constexpr (NC == 6) {
       const bool k0 = (rowk == 0) && (e == 0);
-      if (TRANS) {
+      if (gp.mform) {
+        if (k0) {
+          // the reference subtracts lam tr<beta> d_ij + mu (<beta_ij> + <beta_ji>)
+          const cx<T> t = in[0] + in[1] + in[2];
+#pragma unroll
+          for (int cc = 0; cc < 6; ++cc) {
+            cx<T> v = in[cc] * T(-2.0 * gp.mu);
+            if (cc < 3) v = v - t * T(gp.lam);
+            o[cc] = v * T(gp.norm);
+          }
+        } else {
+          green_sigma_M<T>(in, o, kx, ky, kz, gp);
+        }
+      } else if (TRANS) {
         if (k0) {
 #pragma unroll
           for (int cc = 0; cc < 6; ++cc) o[cc] = cx<T>{T(0), T(0)};
@@ -495,8 +592,18 @@ k_contract(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, GreenParams g
         for (int cc = 0; cc < 6; ++cc) o[cc] = cx<T>{T(s[CS9[cc]]), T(0)};
       }
     } else if constexpr (NC == 9) {
-      if (gp.curl) green_curl<T>(in, o, kx, ky, kz, T(gp.norm));
-      else green_rho<T>(in, o, kx, ky, kz, T(gp.norm));
+      if (gp.rho_sigma) {
+        if ((rowk == 0) && (e == 0)) {  // the ktools' kmask
+#pragma unroll
+          for (int cc = 0; cc < 9; ++cc) o[cc] = cx<T>{T(0), T(0)};
+        } else {
+          green_sigma_rho<T>(in, o, kx, ky, kz, gp);
+        }
+      } else if (gp.curl) {
+        green_curl<T>(in, o, kx, ky, kz, T(gp.norm));
+      } else {
+        green_rho<T>(in, o, kx, ky, kz, T(gp.norm));
+      }
     } else {
       // diffusion filter of the vacancy concentration (VacancyDynamics.py:
       // 40-70): kc exp(kSq * -gamma * dt), normalised for the inverse pass
@@ -535,6 +642,90 @@ __global__ void k_sigma_finalize(const T* __restrict__ G, const T* __restrict__ 
     if (i == jj) s -= lam * tr;
     sig[c * cells + idx] = s;
   }
+}
+
+// The six symmetric planes (xx yy zz xy xz yz) in planes 0-5 of s,
+// expanded in place into the nine components c = 3 i + j.
+__global__ void __launch_bounds__(256) k_sym6_to_9(double* __restrict__ s, long long cells) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= cells) return;
+  double v[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) v[c] = s[c * cells + idx];
+  constexpr int SC[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+#pragma unroll
+  for (int c = 0; c < 9; ++c) s[c * cells + idx] = v[SC[c]];
+}
+
+hipError_t launch_sym6_to_9(double* s, long long cells, hipStream_t st) {
+  hipLaunchKernelGGL(k_sym6_to_9, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, s, cells);
+  return hipGetLastError();
+}
+
+// CalculateRhoSymmetric (PlasticityState.py:232-262): the Nye tensor from
+// SymmetricDerivative (NumericalMethods.py:6-13, order 1),
+// D_a f = 0.5 N (f(i+1) - f(i-1)) along axis a, N = the last grid side;
+// rho_xj = D_y b_zj - D_z b_yj, rho_yj = D_z b_xj - D_x b_zj,
+// rho_zj = D_x b_yj - D_y b_xj (2D: D_z = 0).
+__global__ void __launch_bounds__(256) k_rho_symmetric(const double* __restrict__ b, double* __restrict__ rho, int d,
+                                                      int n, long long cells) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= cells) return;
+  long long st[3];
+  int co[3];
+  if (d == 3) {
+    st[0] = (long long)n * n;
+    st[1] = n;
+    st[2] = 1;
+    co[0] = (int)(idx / st[0]);
+    co[1] = (int)((idx / n) % n);
+    co[2] = (int)(idx % n);
+  } else {
+    st[0] = n;
+    st[1] = 1;
+    st[2] = 0;
+    co[0] = (int)(idx / n);
+    co[1] = (int)(idx % n);
+    co[2] = 0;
+  }
+  const double h = 0.5 * (double)n;
+  // offsets of the +1 and -1 neighbours along each axis (periodic)
+  long long op[3], om[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (a >= d) {
+      op[a] = om[a] = 0;
+      continue;
+    }
+    op[a] = (co[a] == n - 1 ? -(long long)(n - 1) : 1) * st[a];
+    om[a] = (co[a] == 0 ? (long long)(n - 1) : -1) * st[a];
+  }
+  auto D = [&](int a, int c) -> double {
+    if (a >= d) return 0.0;
+    const double* f = b + (long long)c * cells + idx;
+    return h * (f[op[a]] - f[om[a]]);
+  };
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    double rx, ry, rz;
+    if (d == 3) {
+      rx = D(2, 3 + j) - D(1, 6 + j);
+      ry = D(0, 6 + j) - D(2, j);
+      rz = D(1, j) - D(0, 3 + j);
+    } else {
+      rx = -D(1, 6 + j);
+      ry = D(0, 6 + j);
+      rz = D(1, j) - D(0, 3 + j);
+    }
+    rho[(long long)j * cells + idx] = rx;
+    rho[(long long)(3 + j) * cells + idx] = ry;
+    rho[(long long)(6 + j) * cells + idx] = rz;
+  }
+}
+
+hipError_t launch_rho_symmetric(const double* b, double* rho, int d, int n, long long cells, hipStream_t st) {
+  hipLaunchKernelGGL(k_rho_symmetric, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, b, rho, d, n, cells);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------- launchers

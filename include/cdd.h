@@ -146,6 +146,30 @@ int cdd_sigma_noalloc(const cdd_config* cfg, const void* beta_dev, double t, voi
 /* rho = PlasticityState.CalculateRhoFourier(type='k') (PlasticityState.py:264-304). */
 int cdd_rho(cdd_plan* plan, const void* beta_dev, void* rho_dev);
 
+/* The state's other stress and Nye-tensor forms, off the stepper's path
+ * (float64 plans; no external load):
+ *   cdd_sigma_variant: PlasticityState.CalculateSigma(type, source)
+ *     (PlasticityState.py:85-230).  ktype CDD_KTYPE_K (the state's ktools)
+ *     or CDD_KTYPE_SINK (its sinktools, k = N sin(2 pi n / N),
+ *     FourierSpaceTools.py:13-61, PlasticityState.py:35-36); source
+ *     CDD_SOURCE_BETAE (fixed volume; like the reference it ignores ktype),
+ *     CDD_SOURCE_BETAP (the M-tensor form, same result for 'k') or
+ *     CDD_SOURCE_RHO (free boundary: K_ijmn rho_mn of the Nye tensor of
+ *     that ktype, zero mean).
+ *   cdd_rho_variant: CalculateRhoFourier(type) (PlasticityState.py:264-304).
+ *   cdd_rho_symmetric: CalculateRhoSymmetric (PlasticityState.py:232-262):
+ *     the Nye tensor from central differences 0.5 N (f(i+1) - f(i-1))
+ *     (NumericalMethods.py:6-13); rho_dev must not alias beta_dev.
+ * beta_dev, sigma_dev, rho_dev: (9, N^d) float64 device arrays. */
+#define CDD_KTYPE_K 0
+#define CDD_KTYPE_SINK 1
+#define CDD_SOURCE_BETAE 0
+#define CDD_SOURCE_BETAP 1
+#define CDD_SOURCE_RHO 2
+int cdd_sigma_variant(cdd_plan* plan, const void* beta_dev, int ktype, int source, void* sigma_dev);
+int cdd_rho_variant(cdd_plan* plan, const void* beta_dev, int ktype, void* rho_dev);
+int cdd_rho_symmetric(cdd_plan* plan, const void* beta_dev, void* rho_dev);
+
 /* rhs (9, N^d) and the CFL speed field a_tot + b_tot (+ c_tot) (N^d,
  * nullable) of BetaPDynamics.CalculateFlux(t, state, CFLCondition=True)
  * (CentralUpwindHJBetaPDynamics3D.py:214-221 / GlideOnly :255-262 ->

@@ -60,10 +60,14 @@ class KTools:
 
     2D uses ``gridShape[1]`` as the scale of both axes (:29-30); 3D scales
     each axis by its own size (:47-49).  ``kz`` is zero in 2D (:31).
-    Pinned by: closed form k = 2 pi n_wrapped (tests/test_oracle_fft.py).
+    ``func``: the wrap ``_identity`` (the state's ``ktools``) or ``np.sin``
+    (its ``sinktools``, PlasticityState.py:35-36).  ``kmask`` does not depend
+    on it.  Pinned by: closed forms k = 2 pi n_wrapped and N sin(2 pi n / N)
+    (tests/test_oracle_fft_sigma.py, tests/test_oracle_sink.py).
     """
 
-    def __init__(self, shape):
+    def __init__(self, shape, func=None):
+        func = _identity if func is None else func
         shape = tuple(shape)
         self.shape = shape
         d = len(shape)
@@ -71,8 +75,8 @@ class KTools:
             n0, n1 = shape
             ks = (n0, n1 // 2 + 1)
             ix, iy = np.meshgrid(np.arange(ks[0]), np.arange(ks[1]), indexing="ij")
-            self.kx = n1 * _identity(2.0 * np.pi * ix / float(n0))
-            self.ky = n1 * _identity(2.0 * np.pi * iy / float(n1))
+            self.kx = n1 * func(2.0 * np.pi * ix / float(n0))
+            self.ky = n1 * func(2.0 * np.pi * iy / float(n1))
             self.kz = np.zeros(ks)
             self.kSq = self.kx * self.kx + self.ky * self.ky
             self.kmask = 1.0 - ((ix == 0) * (iy == 0)).astype(float)
@@ -81,9 +85,9 @@ class KTools:
             ks = (n0, n1, n2 // 2 + 1)
             ix, iy, iz = np.meshgrid(np.arange(ks[0]), np.arange(ks[1]),
                                      np.arange(ks[2]), indexing="ij")
-            self.kx = n0 * _identity(2.0 * np.pi * ix / float(n0))
-            self.ky = n1 * _identity(2.0 * np.pi * iy / float(n1))
-            self.kz = n2 * _identity(2.0 * np.pi * iz / float(n2))
+            self.kx = n0 * func(2.0 * np.pi * ix / float(n0))
+            self.ky = n1 * func(2.0 * np.pi * iy / float(n1))
+            self.kz = n2 * func(2.0 * np.pi * iz / float(n2))
             self.kSq = self.kx * self.kx + self.ky * self.ky + self.kz * self.kz
             self.kmask = 1.0 - ((ix == 0) * (iy == 0) * (iz == 0)).astype(float)
         else:
@@ -95,12 +99,16 @@ class KTools:
 _KT = {}
 
 
-def ktools(shape):
-    """Per-shape cache (PlasticityState.py:15-16,35-39)."""
+def ktools(shape, type="k"):
+    """Per-shape cache (PlasticityState.py:15-16,35-39): type 'k' the state's
+    ktools, 'sinK' its sinktools (FourierSpaceTools(gridShape, func=sin))."""
     shape = tuple(shape)
-    if shape not in _KT:
-        _KT[shape] = KTools(shape)
-    return _KT[shape]
+    if type not in ("k", "sinK"):
+        raise ValueError("type must be 'k' or 'sinK'")
+    key = (shape, type)
+    if key not in _KT:
+        _KT[key] = KTools(shape, None if type == "k" else np.sin)
+    return _KT[key]
 
 
 # GridArray.py:40-44 (numpy.fft; irfftn with s=gridShape)
@@ -167,14 +175,17 @@ def sigma(beta, mu=1.0, nu=0.3):
     return out
 
 
-def sigma_betaP_source(beta, mu=1.0, nu=0.3):
-    """CalculateSigma(type='k', source='betaP') (PlasticityState.py:151-182).
+def sigma_betaP_source(beta, mu=1.0, nu=0.3, type="k"):
+    """CalculateSigma(type, source='betaP') (PlasticityState.py:151-182).
 
     Independent formula (the M_ijmn tensor, also written out in
-    plasticity_kernel.cu:891-908) used to cross-check ``sigma``.
+    plasticity_kernel.cu:891-908) used to cross-check ``sigma``; type 'sinK'
+    takes the k vectors of the sinktools (the mask stays the ktools' one).
+    Pinned by: agreement with ``sigma`` (type 'k') and with the 'rho' source
+    plus the mean terms it removes (both types).
     """
     shape = beta.shape[1:]
-    K = ktools(shape)
+    K = ktools(shape, type)
     k = K.k
     kb = fft_field(beta)
     kSq = K.kSq + ME
@@ -197,15 +208,18 @@ def sigma_betaP_source(beta, mu=1.0, nu=0.3):
     return s
 
 
-def rho_fourier(beta):
-    """CalculateRhoFourier(type='k') (PlasticityState.py:264-304).
+def rho_fourier(beta, type="k"):
+    """CalculateRhoFourier(type) (PlasticityState.py:264-304).
 
-    rho_ij = -eps_ilm d_l beta_mj via i k.  Pinned by rho(grad u) = 0,
-    div rho = 0 and the single-mode closed form.
+    rho_ij = -eps_ilm d_l beta_mj via i k (type 'k') or i N sin(2 pi n / N)
+    (type 'sinK').  Pinned by rho(grad u) = 0, div rho = 0 and the
+    single-mode closed form (type 'k'); for 'sinK', by its equality with
+    ``rho_symmetric`` (central differences) and the single-mode factor
+    sin(theta) / theta.
     """
     shape = beta.shape[1:]
     d = len(shape)
-    K = ktools(shape)
+    K = ktools(shape, type)
     kb = fft_field(beta)
     D = [kb * 1.0j * K.kx, kb * 1.0j * K.ky]
     if d == 3:
@@ -221,6 +235,62 @@ def rho_fourier(beta):
             kr[cidx(Y, i)] = D[0][cidx(Z, i)] - D[2][cidx(X, i)]
             kr[cidx(Z, i)] = D[1][cidx(X, i)] - D[0][cidx(Y, i)]
     return ifft_field(kr, shape)
+
+
+def sigma_rho_source(beta, mu=1.0, nu=0.3, type="k"):
+    """CalculateSigma(type, source='rho') (PlasticityState.py:128-150): the
+    free-boundary stress from the Nye tensor,
+      sigma_ij(k) = sum_mn K_ijmn(k) rho_mn(k),
+      K_ijmn = sum_s (-i mu k_s / K2) (eps_ins d_jm + eps_jns d_im
+               + 2 eps_mns / (1 - nu) (k_i k_j / K2 - d_ij)),  K2 = kSq + ME,
+    times the ktools' kmask, with rho = CalculateRhoFourier(type) in real
+    space transformed again.  Pinned by: equal to ``sigma`` plus the mean
+    terms lam tr<beta> d_ij + mu (<beta_ij> + <beta_ji>) that the fixed-volume
+    sources keep (type 'k'), and to ``sigma_betaP_source`` without its mean
+    correction (type 'sinK'), on band-limited fields.
+    """
+    shape = beta.shape[1:]
+    K = ktools(shape, type)
+    k = K.k
+    kSq = K.kSq + ME
+    krho = fft_field(rho_fourier(beta, type))
+    ks = np.zeros_like(krho)
+    for i, j in COMPS:
+        if j < i:
+            continue
+        acc = np.zeros(krho.shape[1:], dtype=complex)
+        for m, n in COMPS:
+            for s_ in range(3):
+                Kijmn = ((-1.0j) * mu * k[s_] / kSq) * (PERM[i, n, s_] * (j == m) + PERM[j, n, s_] * (i == m)
+                                                        + (2.0 * PERM[m, n, s_] / (1.0 - nu)) * (k[i] * k[j] / kSq - (i == j)))
+                acc = acc + Kijmn * krho[cidx(m, n)]
+        ks[cidx(i, j)] = acc * ktools(shape).kmask
+        ks[cidx(j, i)] = ks[cidx(i, j)]
+    return ifft_field(ks, shape)
+
+
+def rho_symmetric(beta):
+    """CalculateRhoSymmetric (PlasticityState.py:232-262): the Nye tensor from
+    SymmetricDerivative (NumericalMethods.py:6-13, order 1),
+    D_a f = 0.5 N (f(i+1) - f(i-1)) along axis a with N = gridShape[-1].
+    Pinned by: the Levi-Civita form -eps_ikl D_k beta_lj on the same
+    differences, and equality with rho_fourier(type='sinK') (the central
+    difference of a Fourier mode is i N sin(2 pi n / N) times it)."""
+    shape = beta.shape[1:]
+    d = len(shape)
+    N = shape[-1]
+    D = [0.5 * N * (np.roll(beta, -1, axis=1 + a) - np.roll(beta, 1, axis=1 + a)) for a in range(d)]
+    r = np.zeros_like(beta)
+    for i in range(3):
+        if d == 2:
+            r[cidx(X, i)] = -D[1][cidx(Z, i)]
+            r[cidx(Y, i)] = D[0][cidx(Z, i)]
+            r[cidx(Z, i)] = D[1][cidx(X, i)] - D[0][cidx(Y, i)]
+        else:
+            r[cidx(X, i)] = D[2][cidx(Y, i)] - D[1][cidx(Z, i)]
+            r[cidx(Y, i)] = D[0][cidx(Z, i)] - D[2][cidx(X, i)]
+            r[cidx(Z, i)] = D[1][cidx(X, i)] - D[0][cidx(Y, i)]
+    return r
 
 
 def elastic_energy_density(sig, mu=1.0, nu=0.3):
