@@ -1261,15 +1261,17 @@ __device__ __forceinline__ void lds_read9(unsigned a, float (&v)[9]) {
 // One barrier per plane for 4 and 3.  G(p + 1) goes out as soon as plane p has
 // read G(p), U(p + 1) as soon as U(p) is read.
 // CLIMB: Lambda != 0 (climb suppression terms compiled in); COMB: a stage
-// combination (mode != 0, reads U) rather than the bare rhs.  The host picks
-// the instance from the job; both are compile-time so that the compiler's
-// wait tracking sees one vector-memory sequence per step.
-template <typename T, int DYN, bool CLIMB, bool COMB>
+// combination (mode != 0, reads U) rather than the bare rhs; SPEED (rhs
+// only): also the per-cell CFL speed plane (cdd_flux's speed output).  The
+// host picks the instance from the job; all are compile-time so that the
+// compiler's wait tracking sees one vector-memory sequence per step.
+template <typename T, int DYN, bool CLIMB, bool COMB, bool SPEED = false>
 __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
+  static_assert(!(COMB && SPEED), "the speed plane comes with the bare rhs");
   using namespace march;
   using DM = Dma<T>;
   constexpr int NDMA = DM::NDMA, WLAST = DM::WLAST, EPL = DM::EPL, NUNIT = DM::NUNIT;
-  constexpr int N_T = DM::N_T, N_9 = DM::N_9, N_GN = DM::N_GN, N_ST = DM::N_ST;
+  constexpr int N_T = DM::N_T, N_9 = DM::N_9, N_GN = DM::N_GN, N_ST = DM::N_ST + (SPEED ? 1 : 0);
   typedef __attribute__((address_space(3))) void lds_void;
   // the job's field pointers carry T data (float for fp32 plans)
   const T* __restrict__ A = reinterpret_cast<const T*>(job.A);
@@ -1582,6 +1584,7 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
     } else {
 #pragma unroll
       for (int c = 0; c < 9; ++c) o[c * cells] = rhs[c];
+      if constexpr (SPEED) reinterpret_cast<T*>(job.speed)[(long long)p * splane + xoff] = speed;
       wait_vmcnt<(R4 ? 0 : N_T) + N_ST>();
     }
   };
@@ -1613,15 +1616,15 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
   }
 }
 
-template <int DYN, bool CLIMB, bool COMB>
+template <int DYN, bool CLIMB, bool COMB, bool SPEED = false>
 __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk) {
-  flux_march_body<double, DYN, CLIMB, COMB>(job, chunk);
+  flux_march_body<double, DYN, CLIMB, COMB, SPEED>(job, chunk);
 }
 // fp32 plans: the same kernel on float fields.  Its registers (at most 256
 // per lane) and LDS (80 KB) admit two blocks per CU, two waves per SIMD.
-template <int DYN, bool CLIMB, bool COMB>
+template <int DYN, bool CLIMB, bool COMB, bool SPEED = false>
 __global__ void __launch_bounds__(march::NT, 2) k_flux_march32(FluxJob job, int chunk) {
-  flux_march_body<float, DYN, CLIMB, COMB>(job, chunk);
+  flux_march_body<float, DYN, CLIMB, COMB, SPEED>(job, chunk);
 }
 
 // ---------------------------------------------------------------------------
@@ -1949,8 +1952,8 @@ static int march_chunk(int n, int per_cu = 1) {
 }
 
 static bool march_ok(const FluxJob& job) {
-  return job.d == 3 && !job.llf && !job.S9 && !job.Ad && !job.src && !job.speed && job.count <= 1 &&
-         job.tile == 0 && job.n >= 32 &&
+  return job.d == 3 && !job.llf && !job.S9 && !job.Ad && !job.src && !(job.speed && job.mode) &&
+         job.count <= 1 && job.tile == 0 && job.n >= 32 &&
          job.dyn != DYN_WOJZZ && (job.mode == 0 || job.U) && march_chunk(job.n) > 0;
 }
 
@@ -2073,12 +2076,14 @@ hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
     const int nb = (job.n / march::TC) * (job.n / march::TR) * (job.n / chunk);
     if (nblocks_out) *nblocks_out = nb;
     const bool comb = job.mode != 0;
-#define CDD_MARCH(K_, DYN_, CLIMB_)                                                                        \
-  do {                                                                                                     \
-    if (comb)                                                                                              \
-      hipLaunchKernelGGL((K_<DYN_, CLIMB_, true>), dim3(nb), dim3(march::NT), 0, job.stream, job, chunk);  \
-    else                                                                                                   \
-      hipLaunchKernelGGL((K_<DYN_, CLIMB_, false>), dim3(nb), dim3(march::NT), 0, job.stream, job, chunk); \
+#define CDD_MARCH(K_, DYN_, CLIMB_)                                                                              \
+  do {                                                                                                           \
+    if (comb)                                                                                                    \
+      hipLaunchKernelGGL((K_<DYN_, CLIMB_, true>), dim3(nb), dim3(march::NT), 0, job.stream, job, chunk);        \
+    else if (job.speed)                                                                                          \
+      hipLaunchKernelGGL((K_<DYN_, CLIMB_, false, true>), dim3(nb), dim3(march::NT), 0, job.stream, job, chunk); \
+    else                                                                                                         \
+      hipLaunchKernelGGL((K_<DYN_, CLIMB_, false>), dim3(nb), dim3(march::NT), 0, job.stream, job, chunk);       \
   } while (0)
 #define CDD_MARCH_T(K_)                                           \
   do {                                                            \
