@@ -1262,12 +1262,15 @@ __device__ __forceinline__ void lds_read9(unsigned a, float (&v)[9]) {
 // read G(p), U(p + 1) as soon as U(p) is read.
 // CLIMB: Lambda != 0 (climb suppression terms compiled in); COMB: a stage
 // combination (mode != 0, reads U) rather than the bare rhs; SPEED (rhs
-// only): also the per-cell CFL speed plane (cdd_flux's speed output).  The
-// host picks the instance from the job; all are compile-time so that the
+// only): also the per-cell CFL speed plane (cdd_flux's speed output); LLF
+// (fp64): the CUDA executable's flux (llf_derivs, cell_flux_llf), its
+// slow-axis stencil carried as the four values around the plane.  The host
+// picks the instance from the job; all are compile-time so that the
 // compiler's wait tracking sees one vector-memory sequence per step.
-template <typename T, int DYN, bool CLIMB, bool COMB, bool SPEED = false>
+template <typename T, int DYN, bool CLIMB, bool COMB, bool SPEED = false, bool LLF = false>
 __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
   static_assert(!(COMB && SPEED), "the speed plane comes with the bare rhs");
+  static_assert(!LLF || sizeof(T) == 8, "the LLF flux is float64");
   using namespace march;
   using DM = Dma<T>;
   constexpr int NDMA = DM::NDMA, WLAST = DM::WLAST, EPL = DM::EPL, NUNIT = DM::NUNIT;
@@ -1407,8 +1410,13 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
   }
   dma_g(x0);
   if constexpr (COMB) dma_9(Up, x0, ub[wv]);
+  // LLF: f(p-2), f(p-1) and f(p+1) carried in xs, xd and xf
   T xf[9], xd[9], xs[9], xum[9];
-  {
+  if constexpr (LLF) {
+    center(x0 - 2, xs);
+    center(x0 - 1, xd);
+    center(x0 + 1, xf);
+  } else {
     T fm2[9], fm1[9], f0[9];
     center(x0 - 2, fm2);
     center(x0 - 1, fm1);
@@ -1495,6 +1503,18 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
         const int c = 3 * g + k;
         const T* s = sv[g & 1][k];
         fc[c] = s[2];
+        if constexpr (LLF) {
+          T fz[5] = {s[0], s[1], s[2], s[3], s[4]};
+          llf_derivs<T>(fz, T(n), u[2][0][c], u[2][1][c]);
+          T fy[5] = {s[5], s[6], s[2], s[7], s[8]};
+          llf_derivs<T>(fy, T(n), u[1][0][c], u[1][1][c]);
+          T fx[5] = {xs[c], xd[c], s[2], xf[c], nv[c]};
+          llf_derivs<T>(fx, T(n), u[0][0][c], u[0][1][c]);
+          xs[c] = xd[c];
+          xd[c] = s[2];
+          xf[c] = nv[c];
+          continue;
+        }
         {
           T f[5] = {s[0], s[1], s[2], s[3], s[4]};
           limited_derivs<T>(f, u[2][0][c], u[2][1][c]);
@@ -1556,7 +1576,8 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
     T rhs[9], speed;
     if constexpr (sizeof(T) == 8) {
       T vreg[3 * 8];
-      cell_flux<T, 3, DYN, true, false, CLIMB ? 1 : 0, true>(u, sg, Lambda, rhs, speed, vreg, 1, T(n));
+      if constexpr (LLF) cell_flux_llf<T, 3, DYN>(u, sg, Lambda, rhs, speed, vreg, 1);
+      else cell_flux<T, 3, DYN, true, false, CLIMB ? 1 : 0, true>(u, sg, Lambda, rhs, speed, vreg, 1, T(n));
     } else {
       cell_flux<T, 3, DYN, true, false, CLIMB ? 1 : 0, true>(u, sg, Lambda, rhs, speed, &vsh[0][tid], NT, T(n));
     }
@@ -1616,9 +1637,9 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
   }
 }
 
-template <int DYN, bool CLIMB, bool COMB, bool SPEED = false>
+template <int DYN, bool CLIMB, bool COMB, bool SPEED = false, bool LLF = false>
 __global__ void __launch_bounds__(march::NT) k_flux_march(FluxJob job, int chunk) {
-  flux_march_body<double, DYN, CLIMB, COMB, SPEED>(job, chunk);
+  flux_march_body<double, DYN, CLIMB, COMB, SPEED, LLF>(job, chunk);
 }
 // fp32 plans: the same kernel on float fields.  Its registers (at most 256
 // per lane) and LDS (80 KB) admit two blocks per CU, two waves per SIMD.
@@ -1951,8 +1972,9 @@ static int march_chunk(int n, int per_cu = 1) {
   return chunk;
 }
 
+// (job.llf selects the LLF instances of the same kernel)
 static bool march_ok(const FluxJob& job) {
-  return job.d == 3 && !job.llf && !job.S9 && !job.Ad && !job.src && !(job.speed && job.mode) &&
+  return job.d == 3 && !(job.llf && job.fp32) && !job.S9 && !job.Ad && !job.src && !(job.speed && job.mode) &&
          job.count <= 1 && job.tile == 0 && job.n >= 32 &&
          job.dyn != DYN_WOJZZ && (job.mode == 0 || job.U) && march_chunk(job.n) > 0;
 }
@@ -2062,6 +2084,29 @@ hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
   if (job.count > 1 && job.d != 2) return hipErrorInvalidValue;
   if (job.llf) {
     if (job.fp32 || job.count > 1 || job.dyn == DYN_WOJZZ) return hipErrorInvalidValue;
+    if (march_ok(job)) {
+      // 3D: the plane-marching kernel with the executable's flux
+      const int chunk = march_chunk(job.n, 1);
+      const int nb = (job.n / march::TC) * (job.n / march::TR) * (job.n / chunk);
+      if (nblocks_out) *nblocks_out = nb;
+      const bool comb = job.mode != 0;
+#define CDD_MARCH_LLF(DYN_)                                                                                   \
+  do {                                                                                                        \
+    if (comb)                                                                                                 \
+      hipLaunchKernelGGL((k_flux_march<DYN_, false, true, false, true>), dim3(nb), dim3(march::NT), 0,        \
+                         job.stream, job, chunk);                                                             \
+    else if (job.speed)                                                                                       \
+      hipLaunchKernelGGL((k_flux_march<DYN_, false, false, true, true>), dim3(nb), dim3(march::NT), 0,        \
+                         job.stream, job, chunk);                                                             \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_flux_march<DYN_, false, false, false, true>), dim3(nb), dim3(march::NT), 0,       \
+                         job.stream, job, chunk);                                                             \
+  } while (0)
+      if (job.dyn == DYN_GLIDE_ONLY) CDD_MARCH_LLF(DYN_GLIDE_ONLY);
+      else CDD_MARCH_LLF(DYN_GLIDE_CLIMB);
+#undef CDD_MARCH_LLF
+      return hipGetLastError();
+    }
     dim3 g, b;
     flux_geometry(job.d, job.n, 0, 1, g, b);
     if (nblocks_out) *nblocks_out = (int)g.x;

@@ -48,8 +48,9 @@ DYNS = [(0, 0.0), (0, 1.0), (0, 0.4), (1, 0.0)]
 
 
 @pytest.mark.parametrize("dyn,Lambda", DYNS)
-@pytest.mark.parametrize("d,n", [(2, 8), (2, 16), (2, 64), (3, 8), (3, 16), (3, 32)])
+@pytest.mark.parametrize("d,n", [(2, 8), (2, 16), (2, 64), (3, 8), (3, 16), (3, 32), (3, 64)])
 def test_llf_flux_matches_oracle(amd, d, n, dyn, Lambda):
+    """3D from 32^3 up: the plane-marching kernel's LLF instances."""
     say("llf flux", d, n, dyn, Lambda)
     beta = SpectralGaussianField((n,) * d, 2.0, 0.01, seed=31)
     p = _plan(amd, d, n, dyn, Lambda)
@@ -61,7 +62,7 @@ def test_llf_flux_matches_oracle(amd, d, n, dyn, Lambda):
 
 @pytest.mark.parametrize("rk", [1, 3])
 @pytest.mark.parametrize("d,n,dyn,Lambda", [(2, 32, 1, 0.0), (2, 32, 0, 1.0), (3, 16, 0, 0.0), (3, 16, 1, 0.0),
-                                            (3, 16, 0, 1.0)])
+                                            (3, 16, 0, 1.0), (3, 32, 0, 0.4), (3, 32, 1, 0.0)])
 def test_llf_steps_match_oracle(amd, d, n, dyn, Lambda, rk):
     nsteps = 10
     say("llf steps", d, n, dyn, Lambda, rk)
