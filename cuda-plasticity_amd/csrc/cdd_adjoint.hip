@@ -898,28 +898,69 @@ __global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
   // component: 2 bits at 2 (9 a + c), 3 = the zero case
   unsigned long long code = 0ull;
   const int lr = ty + 2, lc = tx + 2;
-  for (int a = 0; a < D; ++a) {
-    for (int c = 0; c < 9; ++c) {
-      const T* tb = tile[c];
-      T f[5];
-      if (a == D - 1) {
-        for (int k = 0; k < 5; ++k) f[k] = tb[lr * PW + lc + k - 2];
-      } else if (a == D - 2) {
-        for (int k = 0; k < 5; ++k) f[k] = tb[(lr + k - 2) * PW + lc];
-      } else {
-        f[0] = xn[c][0];
-        f[1] = xn[c][1];
-        f[2] = tb[lr * PW + lc];
-        f[3] = xn[c][2];
-        f[4] = xn[c][3 % 4];
+  // Every component's five-point stencils along the in-plane axes, read from
+  // the LDS tile in three groups of three components, each group's reads
+  // issued before the arithmetic of the group before it (at one wave per
+  // SIMD every exposed LDS latency is lost time; the scheduling barriers
+  // keep the compiler from sinking the reads to their first use).
+  // fn(c, s): s[0..4] along the contiguous axis, s[5, 6, 7, 8] at rows
+  // -2, -1, +1, +2 of the next axis, s[2] the cell.
+  auto stencils = [&](auto&& fn) __attribute__((always_inline)) {
+    T sv[2][3][9];
+    auto rd = [&](int g, T (&sq)[3][9]) __attribute__((always_inline)) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const T* tb = tile[3 * g + k];
+#pragma unroll
+        for (int o = 0; o < 5; ++o) sq[k][o] = tb[lr * PW + lc - 2 + o];
+        sq[k][5] = tb[(lr - 2) * PW + lc];
+        sq[k][6] = tb[(lr - 1) * PW + lc];
+        sq[k][7] = tb[(lr + 1) * PW + lc];
+        sq[k][8] = tb[(lr + 2) * PW + lc];
       }
-      if (a == 0) fc[c] = f[2];
+    };
+    rd(0, sv[0]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+      if (g < 2) rd(g + 1, sv[(g + 1) & 1]);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) fn(3 * g + k, sv[g & 1][k]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+  // the five-point stencil of component c along axis a
+  auto stencil5 = [&](int a, int c, const T* sq, const T (*xv)[4], T (&f)[5]) __attribute__((always_inline)) {
+    if (a == D - 1) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) f[k] = sq[k];
+    } else if (a == D - 2) {
+      f[0] = sq[5];
+      f[1] = sq[6];
+      f[2] = sq[2];
+      f[3] = sq[7];
+      f[4] = sq[8];
+    } else {
+      f[0] = xv[c][0];
+      f[1] = xv[c][1];
+      f[2] = sq[2];
+      f[3] = xv[c][2];
+      f[4] = xv[c][3 % 4];
+    }
+  };
+  stencils([&](int c, const T* sq) __attribute__((always_inline)) {
+    fc[c] = sq[2];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      T f[5];
+      stencil5(a, c, sq, xn, f);
       lderivs<T>(f, N, u[a][0][c], u[a][1][c]);
       const T Dm1 = f[2] - f[1], D0 = f[3] - f[2], D1 = f[4] - f[3];
       const int sel = mm3_sel(D1 - D0, (D1 - Dm1) * T(0.5), D0 - Dm1);
       code |= (unsigned long long)(sel < 0 ? 3 : sel) << (2 * (9 * a + c));
     }
-  }
+  });
   job.codes[ii] = code;
   T sg[9];
   {
@@ -943,23 +984,21 @@ __global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
     auto reload = [&](T (&uu)[D][2][9]) {
       long long xo[4];
       for (int k = 0; k < 4; ++k) xo[k] = ii + (long long)(wrapi(co[0] + (k < 2 ? k - 2 : k - 1), n) - co[0]) * str[0];
-      for (int a = 0; a < D; ++a)
-        for (int c = 0; c < 9; ++c) {
-          const T* tb = tile[c];
+      T xr[D == 3 ? 9 : 1][4];
+      if constexpr (D == 3) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) xr[c][k] = job.beta[c * cells + xo[k]];
+      }
+      stencils([&](int c, const T* sq) __attribute__((always_inline)) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
           T f[5];
-          if (a == D - 1) {
-            for (int k = 0; k < 5; ++k) f[k] = tb[lr * PW + lc + k - 2];
-          } else if (a == D - 2) {
-            for (int k = 0; k < 5; ++k) f[k] = tb[(lr + k - 2) * PW + lc];
-          } else {
-            f[0] = job.beta[c * cells + xo[0]];
-            f[1] = job.beta[c * cells + xo[1]];
-            f[2] = tb[lr * PW + lc];
-            f[3] = job.beta[c * cells + xo[2]];
-            f[4] = job.beta[c * cells + xo[3]];
-          }
+          stencil5(a, c, sq, xr, f);
           lderivs<T>(f, N, uu[a][0][c], uu[a][1][c]);
         }
+      });
     };
     cell_flux_vjp_lean<T, D, DYN>(
         u, sg, T(job.Lambda), rb, &vsh[0][tid], &ysh[0][tid], NT, sgb, lamb,
