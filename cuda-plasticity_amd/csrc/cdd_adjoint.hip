@@ -816,15 +816,28 @@ static __device__ __forceinline__ int wrapi(int p, int n) { return p < 0 ? p + n
 // contiguous range of tiles, so the stencil neighbours of a tile (the tiles
 // above/below it and, in 3D, the neighbouring planes) are mostly in flight
 // on the same XCD and served from its L2.
-template <int D>
+// ROWS (3D): logical tiles ordered (row tile, plane, column tile), column
+// tile fastest: a row of tiles along the contiguous axis and its
+// neighbouring planes are in flight together, so the halo columns and the
+// slow axis's neighbours come from the XCD's L2 (the cell VJP's and the
+// gather's order; the gather read 1.4x its bytes in the (column, plane)
+// order); the default orders them (column, plane), planes fastest.
+template <int D, bool ROWS = false>
 __device__ __forceinline__ long long adj_tile_cell(int n, int co[3], long long str[3]) {
   const int TX = blockDim.x, TY = blockDim.y;
   const int nct = n / TX, nrt = n / TY;
   const int nb = (D == 3 ? n : 1) * nct * nrt;
   const int bl = blockIdx.x;
   const int L = (nb % 8 == 0) ? (bl % 8) * (nb / 8) + bl / 8 : bl;
-  const int px = (D == 3) ? L % n : 0;
-  const int tyz = (D == 3) ? L / n : L;
+  int px, tyz;
+  if (D == 3 && ROWS) {
+    const int rt = L / (n * nct), rem = L % (n * nct);
+    px = rem / nct;
+    tyz = rt * nct + rem % nct;
+  } else {
+    px = (D == 3) ? L % n : 0;
+    tyz = (D == 3) ? L / n : L;
+  }
   const int col = (tyz % nct) * TX + threadIdx.x, row = (tyz / nct) * TY + threadIdx.y;
   if (D == 3) {
     co[0] = px;
@@ -863,7 +876,7 @@ __global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
   const long long cells = job.cells;
   int co[3];
   long long str[3];
-  const long long ii = adj_tile_cell<D>(n, co, str);
+  const long long ii = adj_tile_cell<D, true>(n, co, str);
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
   // tile origin (contiguous axis = last)
   const int col0 = co[D - 1] - tx, row0 = co[D - 2] - ty;
@@ -1356,7 +1369,7 @@ __global__ void __launch_bounds__(256) k_flux_vjp_gather(GatherJob g) {
   const T* __restrict__ ubar = g.ubar;
   int co[3];
   long long str[3];
-  const long long idx = adj_tile_cell<D>(n, co, str);
+  const long long idx = adj_tile_cell<D, true>(n, co, str);
   T acc[9];
   for (int c = 0; c < 9; ++c) acc[c] = T(0);
   for (int a = 0; a < D; ++a) {
