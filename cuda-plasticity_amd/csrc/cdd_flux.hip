@@ -1365,12 +1365,12 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
 #pragma unroll
       for (int k = 0; k < 3; ++k)
         __builtin_amdgcn_global_load_lds((const void*)(base + (2 * k + half) * cells), (lds_void*)&gb[wv][2 * k][0],
-                                         16, 0, CPOL_STREAM);
+                                         16, 0, march::CPOL_STREAM);
     } else {
       const T* base = Gp + (long long)wrapp(p) * splane + xoff;
 #pragma unroll
       for (int c = 0; c < 6; ++c)
-        __builtin_amdgcn_global_load_lds((const void*)(base + c * cells), (lds_void*)&gb[wv][c][0], 4, 0, CPOL_STREAM);
+        __builtin_amdgcn_global_load_lds((const void*)(base + c * cells), (lds_void*)&gb[wv][c][0], 4, 0, march::CPOL_STREAM);
     }
   };
   // nine components; fp64: 0-7 in pairs, 8 as 4-byte words (lane l: word
@@ -1416,7 +1416,7 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
     dma_9(A, x0 + 2, nbuf[wv], std::integral_constant<int, 0>{});
   }
   dma_g(x0);
-  if constexpr (COMB) dma_9(Up, x0, ub[wv], std::integral_constant<int, CPOL_STREAM>{});
+  if constexpr (COMB) dma_9(Up, x0, ub[wv], std::integral_constant<int, march::CPOL_STREAM>{});
   // LLF: f(p-2), f(p-1) and f(p+1) carried in xs, xd and xf
   T xf[9], xd[9], xs[9], xum[9];
   if constexpr (LLF) {
@@ -1607,7 +1607,7 @@ __device__ __forceinline__ void flux_march_body(const FluxJob& job, int chunk) {
         job.vac[(long long)p * splane + xoff] += job.vbeta * ((ov[0] + ov[4] + ov[8]) - (uv[0] + uv[4] + uv[8]));
         wait_vmcnt<0>();
       }
-      dma_9(Up, p + 1, ub[wv], std::integral_constant<int, CPOL_STREAM>{});
+      dma_9(Up, p + 1, ub[wv], std::integral_constant<int, march::CPOL_STREAM>{});
       wait_vmcnt<(R4 ? 0 : N_T) + N_ST + N_9>();
     } else {
 #pragma unroll
@@ -1775,17 +1775,18 @@ __global__ void __launch_bounds__(rows::NT) k_flux_rows(FluxMultiJob job) {
 #pragma unroll
       for (int k = 0; k < 3; ++k)
         __builtin_amdgcn_global_load_lds((const void*)(base + (2 * k + half) * cells), (lds_void*)&gb[wv][2 * k][0], 16,
-                                         0, 0);
+                                         0, march::CPOL_STREAM);
     };
-    auto dma_9 = [&](const double* src, int p, double (*dst)[64]) {
+    auto dma_9 = [&](const double* src, int p, double (*dst)[64], auto cp) {
+      constexpr int CP = decltype(cp)::value;
       const double* row = src + (long long)wrapr(p) * n;
 #pragma unroll
       for (int k = 0; k < 4; ++k)
         __builtin_amdgcn_global_load_lds((const void*)(row + poff + (2 * k + half) * cells), (lds_void*)&dst[2 * k][0],
-                                         16, 0, 0);
+                                         16, 0, CP);
       const char* w8 = (const char*)(row + 8 * cells + woff) + 4 * (lane & 1);
-      __builtin_amdgcn_global_load_lds((const void*)w8, (lds_void*)&dst[8][0], 4, 0, 0);
-      __builtin_amdgcn_global_load_lds((const void*)(w8 + 8 * 32), (lds_void*)&dst[8][32], 4, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)w8, (lds_void*)&dst[8][0], 4, 0, CP);
+      __builtin_amdgcn_global_load_lds((const void*)(w8 + 8 * 32), (lds_void*)&dst[8][32], 4, 0, CP);
     };
     auto center = [&](int p, double (&v)[9]) {
       const double* base = A + (long long)wrapr(p) * n + xoff;
@@ -1797,8 +1798,8 @@ __global__ void __launch_bounds__(rows::NT) k_flux_rows(FluxMultiJob job) {
     dma_tile(p0, tile0);
     dma_tile(p0 + 1, tile1);
     dma_g(p0);
-    dma_9(A, p0 + 2, nbuf[wv]);
-    if constexpr (COMB) dma_9(Up, p0, ub[wv]);
+    dma_9(A, p0 + 2, nbuf[wv], std::integral_constant<int, 0>{});
+    if constexpr (COMB) dma_9(Up, p0, ub[wv], std::integral_constant<int, march::CPOL_STREAM>{});
     double xf[9], xd[9], xs[9], xum[9];
     {
       double fm2[9], fm1[9], f0[9];
@@ -1832,7 +1833,7 @@ __global__ void __launch_bounds__(rows::NT) k_flux_rows(FluxMultiJob job) {
       lds_read6(a_gb, gv);
       if constexpr (!COMB) {  // the refill goes out under the whole row's arithmetic
         dma_g(p + 1);
-        dma_9(A, p + 3, nbuf[wv]);
+        dma_9(A, p + 3, nbuf[wv], std::integral_constant<int, 0>{});
       }
       double u[2][2][9], fc[9];
 #pragma unroll
@@ -1858,7 +1859,7 @@ __global__ void __launch_bounds__(rows::NT) k_flux_rows(FluxMultiJob job) {
       }
       if constexpr (COMB) {
         dma_g(p + 1);
-        dma_9(A, p + 3, nbuf[wv]);
+        dma_9(A, p + 3, nbuf[wv], std::integral_constant<int, 0>{});
       }
       double sg[9];
       {
@@ -1885,12 +1886,13 @@ __global__ void __launch_bounds__(rows::NT) k_flux_rows(FluxMultiJob job) {
         double uv[9];
         lds_read9(a_ub, uv);
 #pragma unroll
-        for (int c = 0; c < 9; ++c) o[c * cells] = rk_combine<double>(cu, uv[c], cf, fc[c], dt, rhs[c]);
-        dma_9(Up, p + 1, ub[wv]);
+        for (int c = 0; c < 9; ++c)
+          __builtin_nontemporal_store(rk_combine<double>(cu, uv[c], cf, fc[c], dt, rhs[c]), &o[c * cells]);
+        dma_9(Up, p + 1, ub[wv], std::integral_constant<int, march::CPOL_STREAM>{});
         wait_vmcnt<N_T + N_ST + N_9>();
       } else {
 #pragma unroll
-        for (int c = 0; c < 9; ++c) o[c * cells] = rhs[c];
+        for (int c = 0; c < 9; ++c) __builtin_nontemporal_store(rhs[c], &o[c * cells]);
         wait_vmcnt<N_T + N_ST>();
       }
     };
