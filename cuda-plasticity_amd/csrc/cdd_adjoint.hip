@@ -982,13 +982,14 @@ __global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
     const int SC[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
     for (int c = 0; c < 9; ++c) {
       const int i = c / 3, j = c % 3;
-      sg[c] = job.G[SC[c] * cells + ii] - mu * (fc[c] + fc[3 * j + i]) - (i == j ? lam * tr : T(0));
+      sg[c] = __builtin_nontemporal_load(&job.G[SC[c] * cells + ii]) - mu * (fc[c] + fc[3 * j + i]) -
+              (i == j ? lam * tr : T(0));
     }
   }
   // rbar scaled by rscale (times the step's dt when dtp is given)
   const T rs = T(job.rscale) * (job.dtp ? T(*job.dtp) : T(1));
   T rb[9];
-  for (int c = 0; c < 9; ++c) rb[c] = rs * job.rbar[c * cells + ii];
+  for (int c = 0; c < 9; ++c) rb[c] = rs * __builtin_nontemporal_load(&job.rbar[c * cells + ii]);
   T sgb[9], lamb;
   if constexpr (LEAN) {
     double* ubar = job.ubar;
@@ -1015,7 +1016,10 @@ __global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
     };
     cell_flux_vjp_lean<T, D, DYN>(
         u, sg, T(job.Lambda), rb, &vsh[0][tid], &ysh[0][tid], NT, sgb, lamb,
-        [&](int a, int s, int c, T val) { ubar[((a * 2 + s) * 9 + c) * cells + ii] = N * val; }, reload);
+        [&](int a, int s, int c, T val) {
+          __builtin_nontemporal_store(N * val, &ubar[((a * 2 + s) * 9 + c) * cells + ii]);
+        },
+        reload);
   } else {
     T ub[D][2][9];
     cell_flux_vjp<T, D, DYN>(u, sg, T(job.Lambda), rb, ub, sgb, lamb);
@@ -1024,7 +1028,8 @@ __global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
         for (int c = 0; c < 9; ++c) job.ubar[((a * 2 + s) * 9 + c) * cells + ii] = N * ub[a][s][c];
   }
   for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) job.sbar[(3 * i + j) * cells + ii] = T(0.5) * (sgb[3 * i + j] + sgb[3 * j + i]);
+    for (int j = 0; j < 3; ++j)
+      __builtin_nontemporal_store(T(0.5) * (sgb[3 * i + j] + sgb[3 * j + i]), &job.sbar[(3 * i + j) * cells + ii]);
   const int nw = (int)(blockDim.x * blockDim.y + 63) / 64;
   for (int off = 32; off > 0; off >>= 1) lamb += __shfl_xor(lamb, off, 64);
   if ((tid & 63) == 0) red[tid >> 6] = lamb;
@@ -1399,19 +1404,19 @@ __global__ void __launch_bounds__(256) k_flux_vjp_gather(GatherJob g) {
     }
   }
   T s[9];
-  for (int c = 0; c < 9; ++c) s[c] = g.sbar[c * cells + idx];
+  for (int c = 0; c < 9; ++c) s[c] = __builtin_nontemporal_load(&g.sbar[c * cells + idx]);
   const T trs = s[0] + s[4] + s[8];
   const int SC[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
   for (int c = 0; c < 9; ++c) {
     const int i = c / 3, j = c % 3;
     T v = acc[c] - T(2) * T(g.mu) * s[c] - (i == j ? T(g.lam) * trs : T(0));
-    if (g.ebar) v += g.ebar[SC[c] * cells + idx];
-    for (int k = 0; k < g.nterm; ++k) v += T(g.coef[k]) * g.term[k][c * cells + idx];
+    if (g.ebar) v += __builtin_nontemporal_load(&g.ebar[SC[c] * cells + idx]);
+    for (int k = 0; k < g.nterm; ++k) v += T(g.coef[k]) * __builtin_nontemporal_load(&g.term[k][c * cells + idx]);
     if (g.csrc) v += g.csrc[c];
     acc[c] = v;
   }
   if (g.uzz) acc[8] += T(g.uzz_scale) * g.uzz[0];
-  for (int c = 0; c < 9; ++c) g.out[c * cells + idx] = acc[c];
+  for (int c = 0; c < 9; ++c) __builtin_nontemporal_store(acc[c], &g.out[c * cells + idx]);
 }
 
 // sbar = S : sigma (the energy density's stress gradient, PlasticityState.py:
