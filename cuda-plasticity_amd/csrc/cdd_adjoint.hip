@@ -868,6 +868,9 @@ __global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
   // Y, Hb and sigma in LDS (the plain form spills in 3D; in 2D it does not
   // and measures slightly faster)
   constexpr bool LEAN = (D == 3);
+  // 3D grids stream the planes touched once (G, rbar, the cotangents out);
+  // 2D ones (512^2 at most here) keep them in the infinity cache
+  constexpr bool STRM = (D == 3);
   __shared__ T red[4];
   __shared__ T tile[9][TILE];
   __shared__ T vsh[LEAN ? 3 * (1 << D) : 1][LEAN ? NT : 1];
@@ -982,14 +985,14 @@ __global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
     const int SC[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
     for (int c = 0; c < 9; ++c) {
       const int i = c / 3, j = c % 3;
-      sg[c] = __builtin_nontemporal_load(&job.G[SC[c] * cells + ii]) - mu * (fc[c] + fc[3 * j + i]) -
+      sg[c] = ld_pol<STRM>(&job.G[SC[c] * cells + ii]) - mu * (fc[c] + fc[3 * j + i]) -
               (i == j ? lam * tr : T(0));
     }
   }
   // rbar scaled by rscale (times the step's dt when dtp is given)
   const T rs = T(job.rscale) * (job.dtp ? T(*job.dtp) : T(1));
   T rb[9];
-  for (int c = 0; c < 9; ++c) rb[c] = rs * __builtin_nontemporal_load(&job.rbar[c * cells + ii]);
+  for (int c = 0; c < 9; ++c) rb[c] = rs * ld_pol<STRM>(&job.rbar[c * cells + ii]);
   T sgb[9], lamb;
   if constexpr (LEAN) {
     double* ubar = job.ubar;
@@ -1017,7 +1020,7 @@ __global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
     cell_flux_vjp_lean<T, D, DYN>(
         u, sg, T(job.Lambda), rb, &vsh[0][tid], &ysh[0][tid], NT, sgb, lamb,
         [&](int a, int s, int c, T val) {
-          __builtin_nontemporal_store(N * val, &ubar[((a * 2 + s) * 9 + c) * cells + ii]);
+          st_pol<STRM>(&ubar[((a * 2 + s) * 9 + c) * cells + ii], N * val);
         },
         reload);
   } else {
@@ -1029,7 +1032,7 @@ __global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
   }
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j)
-      __builtin_nontemporal_store(T(0.5) * (sgb[3 * i + j] + sgb[3 * j + i]), &job.sbar[(3 * i + j) * cells + ii]);
+      st_pol<STRM>(&job.sbar[(3 * i + j) * cells + ii], T(0.5) * (sgb[3 * i + j] + sgb[3 * j + i]));
   const int nw = (int)(blockDim.x * blockDim.y + 63) / 64;
   for (int off = 32; off > 0; off >>= 1) lamb += __shfl_xor(lamb, off, 64);
   if ((tid & 63) == 0) red[tid >> 6] = lamb;
@@ -1369,6 +1372,7 @@ __device__ __forceinline__ void site_coef(unsigned sel, T& up, T& mid, T& dn) {
 
 template <typename T, int D>
 __global__ void __launch_bounds__(256) k_flux_vjp_gather(GatherJob g) {
+  constexpr bool STRM = (D == 3);  // as in k_flux_vjp_cell
   const int n = g.n;
   const long long cells = g.cells;
   const T* __restrict__ ubar = g.ubar;
@@ -1404,19 +1408,19 @@ __global__ void __launch_bounds__(256) k_flux_vjp_gather(GatherJob g) {
     }
   }
   T s[9];
-  for (int c = 0; c < 9; ++c) s[c] = __builtin_nontemporal_load(&g.sbar[c * cells + idx]);
+  for (int c = 0; c < 9; ++c) s[c] = ld_pol<STRM>(&g.sbar[c * cells + idx]);
   const T trs = s[0] + s[4] + s[8];
   const int SC[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
   for (int c = 0; c < 9; ++c) {
     const int i = c / 3, j = c % 3;
     T v = acc[c] - T(2) * T(g.mu) * s[c] - (i == j ? T(g.lam) * trs : T(0));
-    if (g.ebar) v += __builtin_nontemporal_load(&g.ebar[SC[c] * cells + idx]);
-    for (int k = 0; k < g.nterm; ++k) v += T(g.coef[k]) * __builtin_nontemporal_load(&g.term[k][c * cells + idx]);
+    if (g.ebar) v += ld_pol<STRM>(&g.ebar[SC[c] * cells + idx]);
+    for (int k = 0; k < g.nterm; ++k) v += T(g.coef[k]) * ld_pol<STRM>(&g.term[k][c * cells + idx]);
     if (g.csrc) v += g.csrc[c];
     acc[c] = v;
   }
   if (g.uzz) acc[8] += T(g.uzz_scale) * g.uzz[0];
-  for (int c = 0; c < 9; ++c) __builtin_nontemporal_store(acc[c], &g.out[c * cells + idx]);
+  for (int c = 0; c < 9; ++c) st_pol<STRM>(&g.out[c * cells + idx], acc[c]);
 }
 
 // sbar = S : sigma (the energy density's stress gradient, PlasticityState.py:

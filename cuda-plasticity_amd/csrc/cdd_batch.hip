@@ -154,6 +154,8 @@ static hipError_t group_spectral(cdd_batch* b, Group& g, const StageIO& io, hipS
   j.sstride = g.sstride;
   j.ostride = g.gstride;
   j.stile = g.stile;
+  // the size groups' passes run concurrently: the policy follows all of them
+  for (const Group& o : b->groups) j.ws_bytes += 6ll * (o.n / 2 + 1) * o.n * 16 * o.count;
   return launch_spectral(j);
 }
 

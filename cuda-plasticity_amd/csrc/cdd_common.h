@@ -25,6 +25,42 @@ __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
 }
 template <typename T>
 __device__ __forceinline__ cx<T> conj(cx<T> a) { return {a.x, -a.y}; }
+// Loads and stores with a cache policy chosen at compile time: NT = the
+// non-temporal (streaming) policy, for data a pass touches once and that
+// would otherwise displace lines other accesses re-read.  Complex values go
+// as one 2-T vector (2-T-aligned addresses).
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_pol(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_pol(T* p, T v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ cx<T> ld_pol(const cx<T>* p) {
+  if constexpr (NT) {
+    typedef T V2 __attribute__((ext_vector_type(2)));
+    const V2 v = __builtin_nontemporal_load(reinterpret_cast<const V2*>(p));
+    return {v.x, v.y};
+  } else {
+    return *p;
+  }
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_pol(cx<T>* p, cx<T> a) {
+  if constexpr (NT) {
+    typedef T V2 __attribute__((ext_vector_type(2)));
+    V2 v;
+    v.x = a.x;
+    v.y = a.y;
+    __builtin_nontemporal_store(v, reinterpret_cast<V2*>(p));
+  } else {
+    *p = a;
+  }
+}
 // -i * a
 template <typename T>
 __device__ __forceinline__ cx<T> mul_ni(cx<T> a) { return {a.y, -a.x}; }

@@ -70,7 +70,14 @@ struct SpectralJob {
   // R2C / C2R row tile: 0 the default rows per block, 1 half of them (2D
   // fp64 stress solves of sides >= 256; otherwise the default)
   int stile;
+  // working set that picks the passes' cache policy (streaming above
+  // kStreamBytes); 0: this launch's spectrum.  Launches that run beside
+  // others (the batch's size groups) give the spectrum of all of them.
+  long long ws_bytes;
 };
+// Working sets above the 256 MB infinity cache stream (non-temporal loads
+// and stores) through the spectral passes.
+constexpr long long kStreamBytes = 256ll << 20;
 
 hipError_t launch_spectral(const SpectralJob& job);
 hipError_t launch_sym6_to_9(double* s, long long cells, hipStream_t st);

@@ -22,6 +22,11 @@
 // sigma; the local part -lam tr(beta) d_ij - mu (beta_ij + beta_ji) is added
 // by whoever reads beta at the cell anyway (the flux kernel, or
 // k_sigma_finalize for the standalone CalculateSigma).
+//
+// Every pass touches each of its input and output values once, so their
+// global loads and stores take the non-temporal cache policy: the lines
+// stream through instead of displacing others in L2 (3-4% of a 256^3 RK3
+// step: the passes themselves and the flux kernel that follows run faster).
 #include "cdd_kernels.h"
 
 namespace cdd {
@@ -69,7 +74,7 @@ __device__ __forceinline__ void src_comps(int c, int& k0, int& k1) {
 // register budget is held to match)
 // TY: rows per block (Tiles<N>::TY, or a smaller power of two for more,
 // smaller blocks).
-template <typename T, int N, bool SYM, int TY = Tiles<N>::TY>
+template <typename T, int N, bool SYM, int TY = Tiles<N>::TY, bool STRM = false>
 __global__ void __launch_bounds__(TY * Tiles<N>::TT, 2)
 k_r2c_last(SrcField<T> src, cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, int NO, int NC,
            long long sstride) {
@@ -122,12 +127,12 @@ k_r2c_last(SrcField<T> src, cx<T>* __restrict__ spec, const cx<T>* __restrict__ 
     if (ks[q] < 0) continue;
     const T* Ak = src.A + ks[q] * src.cs + rowbase;
 #pragma unroll
-    for (int i = 0; i < V; ++i) a[q][i] = Ak[fft_in_index<N>(j, i)];
+    for (int i = 0; i < V; ++i) a[q][i] = ld_pol<STRM>(&Ak[fft_in_index<N>(j, i)]);
     if (src.B) {
       const T* Bk = src.B + ks[q] * src.cs + rowbase;
       T b[V];
 #pragma unroll
-      for (int i = 0; i < V; ++i) b[i] = Bk[fft_in_index<N>(j, i)];
+      for (int i = 0; i < V; ++i) b[i] = ld_pol<STRM>(&Bk[fft_in_index<N>(j, i)]);
 #pragma unroll
       for (int i = 0; i < V; ++i) a[q][i] += dt * b[i];
     }
@@ -139,7 +144,7 @@ k_r2c_last(SrcField<T> src, cx<T>* __restrict__ spec, const cx<T>* __restrict__ 
       if (ks[q] < 0) continue;
       T* Wk = src.W + ks[q] * src.cs + rowbase;
 #pragma unroll
-      for (int i = 0; i < V; ++i) Wk[fft_in_index<N>(j, i)] = a[q][i];
+      for (int i = 0; i < V; ++i) st_pol<STRM>(&Wk[fft_in_index<N>(j, i)], a[q][i]);
     }
   }
   cx<T> v[V];
@@ -164,17 +169,17 @@ k_r2c_last(SrcField<T> src, cx<T>* __restrict__ spec, const cx<T>* __restrict__ 
     cx<T> z1 = lds[rr * P + lds_pad(k)];
     cx<T> z2 = conj(lds[rr * P + lds_pad((N - k) % N)]);
     const long long oa = (((long long)ca * H + k) * NO + o) * N + r0 + rr;
-    spec[oa] = (z1 + z2) * T(0.5);
+    st_pol<STRM>(&spec[oa], (z1 + z2) * T(0.5));
     if (hasb) {
       const long long ob = (((long long)cb * H + k) * NO + o) * N + r0 + rr;
-      spec[ob] = mul_ni(z1 - z2) * T(0.5);
+      st_pol<STRM>(&spec[ob], mul_ni(z1 - z2) * T(0.5));
     }
   }
 }
 
 // --------------------------------------------------------------- c2r_last
 // Inverse of k_r2c_last: reads [c][k][o][r], writes real out[c][o][r][e].
-template <typename T, int N, int TY = Tiles<N>::TY>
+template <typename T, int N, int TY = Tiles<N>::TY, bool STRM = false>
 __global__ void __launch_bounds__(TY * Tiles<N>::TT)
 k_c2r_last(const cx<T>* __restrict__ spec, T* __restrict__ out, long long cs, const cx<T>* __restrict__ tw,
            int NO, int NC, const Clock* clk, long long sstride, long long ostride, const T* __restrict__ vac,
@@ -201,8 +206,8 @@ k_c2r_last(const cx<T>* __restrict__ spec, T* __restrict__ out, long long cs, co
     for (int m = 0; m < NK; ++m) {
       const int k = g + m * TT;
       const bool in = k < H;
-      Av[m] = in ? spec[(((long long)ca * H + k) * NO + o) * N + r0 + rr] : cx<T>{T(0), T(0)};
-      Bv[m] = (in && hasb) ? spec[(((long long)cb * H + k) * NO + o) * N + r0 + rr] : cx<T>{T(0), T(0)};
+      Av[m] = in ? ld_pol<STRM>(&spec[(((long long)ca * H + k) * NO + o) * N + r0 + rr]) : cx<T>{T(0), T(0)};
+      Bv[m] = (in && hasb) ? ld_pol<STRM>(&spec[(((long long)cb * H + k) * NO + o) * N + r0 + rr]) : cx<T>{T(0), T(0)};
     }
 #pragma unroll
     for (int m = 0; m < NK; ++m) {
@@ -246,14 +251,14 @@ k_c2r_last(const cx<T>* __restrict__ spec, T* __restrict__ out, long long cs, co
 #pragma unroll
   for (int i = 0; i < V; ++i) {
     const long long idx = rowbase + fft_out_index<N>(j, i);
-    out[ca * cs + idx] = v[i].x;   // conj(FFT(conj z)).x
-    if (hasb) out[cb * cs + idx] = -v[i].y;
+    st_pol<STRM>(&out[ca * cs + idx], v[i].x);  // conj(FFT(conj z)).x
+    if (hasb) st_pol<STRM>(&out[cb * cs + idx], -v[i].y);
   }
 }
 
 // --------------------------------------------------------------- fft_cols
 // In-place complex FFT along x of spec[c][kz][x][y]; grid (N / TW, H, NC).
-template <typename T, int N, bool INV>
+template <typename T, int N, bool INV, bool STRM = false>
 __global__ void __launch_bounds__(Tiles<N>::TW * Tiles<N>::TT)
 k_fft_cols(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, const Clock* clk) {
   using TL = Tiles<N>;
@@ -267,7 +272,7 @@ k_fft_cols(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, const Clock* 
   cx<T> v[V];
 #pragma unroll
   for (int i = 0; i < V; ++i) {
-    cx<T> a = spec[base + (long long)fft_in_index<N>(j, i) * N];
+    cx<T> a = ld_pol<STRM>(&spec[base + (long long)fft_in_index<N>(j, i) * N]);
     v[i] = INV ? conj(a) : a;
   }
   ColExchange<T, TW> ex{lds + t};
@@ -275,7 +280,7 @@ k_fft_cols(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, const Clock* 
 #pragma unroll
   for (int i = 0; i < V; ++i) {
     cx<T> a = v[i];
-    spec[base + (long long)fft_out_index<N>(j, i) * N] = INV ? conj(a) : a;
+    st_pol<STRM>(&spec[base + (long long)fft_out_index<N>(j, i) * N], INV ? conj(a) : a);
   }
 }
 
@@ -478,7 +483,7 @@ hipError_t launch_disorder_weight(double* rho, const double* d, long long cells,
 // the contraction with each thread owning whole k points (the k-dependent
 // terms once per k, all outputs at once, in place in LDS), then the
 // inverse FFT of every output row.
-template <typename T, int N, int NC, int D, bool TRANS = false>
+template <typename T, int N, int NC, int D, bool TRANS = false, bool STRM = false>
 __global__ void __launch_bounds__((NC * ContractTiles<N, NC>::TR * ContractTiles<N, NC>::TT))
 k_contract(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, GreenParams gp, long long sstride) {
   using CT = ContractTiles<N, NC>;
@@ -499,7 +504,7 @@ k_contract(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, GreenParams g
 
   cx<T> v[V];
 #pragma unroll
-  for (int i = 0; i < V; ++i) v[i] = valid ? spec[rbase + fft_in_index<N>(j, i)] : cx<T>{T(0), T(0)};
+  for (int i = 0; i < V; ++i) v[i] = valid ? ld_pol<STRM>(&spec[rbase + fft_in_index<N>(j, i)]) : cx<T>{T(0), T(0)};
   cx<T>* myrow = lds + (c * TR + r) * P;
   RowExchange<T> ex{myrow};
   fft_forward<T, N>(v, j, tw, ex);
@@ -620,7 +625,7 @@ k_contract(cx<T>* __restrict__ spec, const cx<T>* __restrict__ tw, GreenParams g
   fft_forward<T, N>(v, j, tw, ex);
   if (valid) {
 #pragma unroll
-    for (int i = 0; i < V; ++i) spec[rbase + fft_out_index<N>(j, i)] = conj(v[i]);
+    for (int i = 0; i < V; ++i) st_pol<STRM>(&spec[rbase + fft_out_index<N>(j, i)], conj(v[i]));
   }
 }
 
@@ -737,8 +742,8 @@ static SrcField<T> src_as(const SrcField<double>& s) {
                      reinterpret_cast<T*>(s.W), s.istride};
 }
 
-template <typename T, int N>
-static hipError_t launch_spectral_t(const SpectralJob& job) {
+template <typename T, int N, bool STRM>
+static hipError_t launch_spectral_tp(const SpectralJob& job) {
   using TL = Tiles<N>;
   const SrcField<T> src = src_as<T>(job.src);
   T* out = reinterpret_cast<T*>(job.out);
@@ -766,16 +771,16 @@ static hipError_t launch_spectral_t(const SpectralJob& job) {
   dim3 g1h(N / TYH, count, npairs), b1h(TYH * TL::TT);
   if (half) {
     if constexpr (HALF_OK)
-      hipLaunchKernelGGL((k_r2c_last<T, N, true, TYH>), g1h, b1h, 0, st, src, spec, tw, NO, NC, job.sstride);
+      hipLaunchKernelGGL((k_r2c_last<T, N, true, TYH, STRM>), g1h, b1h, 0, st, src, spec, tw, NO, NC, job.sstride);
   } else if (NC == 6) {
-    hipLaunchKernelGGL((k_r2c_last<T, N, true>), g1, b1, 0, st, src, spec, tw, NO, NC, job.sstride);
+    hipLaunchKernelGGL((k_r2c_last<T, N, true, TL::TY, STRM>), g1, b1, 0, st, src, spec, tw, NO, NC, job.sstride);
   } else {
-    hipLaunchKernelGGL((k_r2c_last<T, N, false>), g1, b1, 0, st, src, spec, tw, NO, NC, job.sstride);
+    hipLaunchKernelGGL((k_r2c_last<T, N, false, TL::TY, STRM>), g1, b1, 0, st, src, spec, tw, NO, NC, job.sstride);
   }
   hook(KK_R2C);
   if (d == 3) {
     dim3 g2(N / TL::TW, H, NC), b2(TL::TW * TL::TT);
-    hipLaunchKernelGGL((k_fft_cols<T, N, false>), g2, b2, 0, st, spec, tw, job.gp.clk);
+    hipLaunchKernelGGL((k_fft_cols<T, N, false, STRM>), g2, b2, 0, st, spec, tw, job.gp.clk);
     hook(KK_COLS_FWD);
   }
   const long long NR = d == 3 ? (long long)H * N : H;
@@ -783,45 +788,61 @@ static hipError_t launch_spectral_t(const SpectralJob& job) {
     using CT = ContractTiles<N, 6>;
     dim3 g3((unsigned)((NR + CT::TR - 1) / CT::TR), count), b3(6 * CT::TR * CT::TT);
     if (job.trans) {
-      if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 6, 3, true>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
-      else hipLaunchKernelGGL((k_contract<T, N, 6, 2, true>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+      if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 6, 3, true, STRM>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+      else hipLaunchKernelGGL((k_contract<T, N, 6, 2, true, STRM>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
     } else if (d == 3) {
-      hipLaunchKernelGGL((k_contract<T, N, 6, 3>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+      hipLaunchKernelGGL((k_contract<T, N, 6, 3, false, STRM>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
     } else {
-      hipLaunchKernelGGL((k_contract<T, N, 6, 2>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+      hipLaunchKernelGGL((k_contract<T, N, 6, 2, false, STRM>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
     }
   } else if (NC == 9) {
     using CT = ContractTiles<N, 9>;
     dim3 g3((unsigned)((NR + CT::TR - 1) / CT::TR), count), b3(9 * CT::TR * CT::TT);
-    if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 9, 3>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
-    else hipLaunchKernelGGL((k_contract<T, N, 9, 2>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+    if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 9, 3, false, STRM>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+    else hipLaunchKernelGGL((k_contract<T, N, 9, 2, false, STRM>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
   } else if (NC == 1) {
     using CT = ContractTiles<N, 1>;
     dim3 g3((unsigned)((NR + CT::TR - 1) / CT::TR), count), b3(CT::TR * CT::TT);
-    if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 1, 3>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
-    else hipLaunchKernelGGL((k_contract<T, N, 1, 2>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+    if (d == 3) hipLaunchKernelGGL((k_contract<T, N, 1, 3, false, STRM>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
+    else hipLaunchKernelGGL((k_contract<T, N, 1, 2, false, STRM>), g3, b3, 0, st, spec, tw, job.gp, job.sstride);
   } else {
     return hipErrorInvalidValue;
   }
   hook(KK_CONTRACT);
   if (d == 3) {
     dim3 g2(N / TL::TW, H, NC), b2(TL::TW * TL::TT);
-    hipLaunchKernelGGL((k_fft_cols<T, N, true>), g2, b2, 0, st, spec, tw, job.gp.clk);
+    hipLaunchKernelGGL((k_fft_cols<T, N, true, STRM>), g2, b2, 0, st, spec, tw, job.gp.clk);
     hook(KK_COLS_INV);
   }
   if (job.vac && (NC != 6 || sizeof(T) != 8)) return hipErrorInvalidValue;
   if (half) {
     if constexpr (HALF_OK)
-      hipLaunchKernelGGL((k_c2r_last<T, N, TYH>), g1h, b1h, 0, st, spec, out, cs, tw, NO, NC, job.gp.clk,
+      hipLaunchKernelGGL((k_c2r_last<T, N, TYH, STRM>), g1h, b1h, 0, st, spec, out, cs, tw, NO, NC, job.gp.clk,
                          job.sstride, job.ostride, reinterpret_cast<const T*>(job.vac), T(job.valpha),
                          reinterpret_cast<const T*>(job.valpha_f));
   } else {
-    hipLaunchKernelGGL((k_c2r_last<T, N>), g1, b1, 0, st, spec, out, cs, tw, NO, NC, job.gp.clk, job.sstride,
+    hipLaunchKernelGGL((k_c2r_last<T, N, TL::TY, STRM>), g1, b1, 0, st, spec, out, cs, tw, NO, NC, job.gp.clk, job.sstride,
                        job.ostride, reinterpret_cast<const T*>(job.vac), T(job.valpha),
                        reinterpret_cast<const T*>(job.valpha_f));
   }
   hook(KK_C2R);
   return hipGetLastError();
+}
+
+// A launch whose working set (its spectrum over all components and
+// instances, or job.ws_bytes) exceeds the 256 MB infinity cache streams its
+// passes (non-temporal loads and stores: 3-4% of a 256^3 RK3 step); smaller
+// ones keep the default policy, under which a pass re-reads its
+// predecessor's output from that cache (streaming cost 2D 512^2 steps 8%).
+template <typename T, int N>
+static hipError_t launch_spectral_t(const SpectralJob& job) {
+  if constexpr (N >= 128) {
+    long long sb = (long long)job.nc * (N / 2 + 1) * (long long)sizeof(cx<T>) * (job.count > 0 ? job.count : 1);
+    for (int a = 1; a < job.d; ++a) sb *= N;
+    if (job.ws_bytes > 0) sb = job.ws_bytes;
+    if (sb > kStreamBytes) return launch_spectral_tp<T, N, true>(job);
+  }
+  return launch_spectral_tp<T, N, false>(job);
 }
 
 template <int N>
