@@ -193,11 +193,28 @@ static bool in_multi(const cdd_batch* b, const Group& g) {
   return b->flux_tile == 3 ? flux_rows_ok(g.n) : flux_multi_ok(g.n, b->flux_tile);
 }
 
-// The CFL reduction of a group in the multi-instance launch: the row-marching
-// kernel leaves one atomically maximised slot per instance (read and reset).
-static hipError_t multi_finalize(cdd_batch* b, Group& g, hipStream_t st) {
-  if (b->flux_tile == 3) return launch_finalize_dt(g.partial, 1, g.clk, (double)g.n, st, g.count, g.nblocks, 0.0, 1);
-  return launch_finalize_dt(g.partial, g.nblocks, g.clk, (double)g.n, st, g.count, g.nblocks);
+// The CFL reductions of the groups in the multi-instance launch, in one
+// launch: the row-marching kernel leaves one atomically maximised slot per
+// instance (read and reset), the tile walk one partial per tile.
+static hipError_t multi_finalize(cdd_batch* b, hipStream_t st) {
+  std::vector<double*> partial;
+  std::vector<int> nb, count, clear;
+  std::vector<long long> pstride;
+  std::vector<Clock*> clk;
+  std::vector<double> nmax;
+  for (Group& g : b->groups) {
+    if (!in_multi(b, g)) continue;
+    partial.push_back(g.partial);
+    nb.push_back(b->flux_tile == 3 ? 1 : g.nblocks);
+    clear.push_back(b->flux_tile == 3 ? 1 : 0);
+    pstride.push_back(g.nblocks);
+    clk.push_back(g.clk);
+    nmax.push_back((double)g.n);
+    count.push_back(g.count);
+  }
+  if (partial.empty()) return hipSuccess;
+  return launch_finalize_dt_groups((int)partial.size(), partial.data(), nb.data(), pstride.data(), clk.data(),
+                                   nmax.data(), count.data(), clear.data(), st);
 }
 
 // One multi-instance flux launch of stage s over every group in_multi.
@@ -269,14 +286,17 @@ static int capture(cdd_batch* b, int phase) {
       if (err == hipSuccess) err = hipStreamWaitEvent(main, join, 0);
     }
     if (err == hipSuccess) err = multi_flux(b, phase, s, main);
-    if (s == 0) {
-      for (int gi = 0; gi < ng && err == hipSuccess; ++gi) {
-        Group& g = b->groups[gi];
-        if (in_multi(b, g)) err = multi_finalize(b, g, main);
-      }
-    }
+    if (s == 0 && err == hipSuccess) err = multi_finalize(b, main);
   }
-  for (int gi = 0; gi < ng && err == hipSuccess; ++gi) err = launch_advance(b->groups[gi].clk, main, b->groups[gi].count);
+  if (err == hipSuccess) {
+    std::vector<Clock*> clk;
+    std::vector<int> count;
+    for (Group& g : b->groups) {
+      clk.push_back(g.clk);
+      count.push_back(g.count);
+    }
+    err = launch_advance_groups(ng, clk.data(), count.data(), main);
+  }
   hipGraph_t graph = nullptr;
   hipError_t e2 = hipStreamEndCapture(main, &graph);
   if (err != hipSuccess || e2 != hipSuccess) {

@@ -2163,14 +2163,11 @@ hipError_t launch_flux(const FluxJob& job, int* nblocks_out) {
 }
 
 // dt = CFL / (max(gridShape) max speed), clamped to the end time
-// (FieldMover.py:94-111).  One block.
-__global__ void __launch_bounds__(1024) k_finalize_dt(double* partial, int nb, Clock* clk, double nmax,
-                                                     long long pstride, double dt_bound, int clear) {
-  // one block per instance
-  partial += blockIdx.x * pstride;
-  clk += blockIdx.x;
-  if (clk->done) return;
-  __shared__ double red[16];
+// (FieldMover.py:94-111), for one instance: a whole block of up to 1024
+// threads reduces its nb speed partials (red: 16 doubles of LDS).
+__device__ __forceinline__ void finalize_block(double* partial, int nb, Clock* clk, double nmax, double dt_bound,
+                                               int clear, double* red) {
+  if (clk->done) return;  // uniform over the block
   double m = -INFINITY;
   for (int k = threadIdx.x; k < nb; k += blockDim.x) m = fmax(m, partial[k]);
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
@@ -2196,6 +2193,33 @@ __global__ void __launch_bounds__(1024) k_finalize_dt(double* partial, int nb, C
     if (clk->t + dt > clk->t_end) dt = clk->t_end - clk->t + (clk->dt_rule ? kME : 0.0);
     clk->dt = dt;
   }
+}
+
+// one block per instance
+__global__ void __launch_bounds__(1024) k_finalize_dt(double* partial, int nb, Clock* clk, double nmax,
+                                                     long long pstride, double dt_bound, int clear) {
+  __shared__ double red[16];
+  finalize_block(partial + blockIdx.x * pstride, nb, clk + blockIdx.x, nmax, dt_bound, clear, red);
+}
+
+// Several instance groups in one launch (a batch's size groups): block b is
+// instance b - first[g] of the group g whose range holds it.
+struct FinalizeGroups {
+  int ng;
+  int first[kFluxGroups + 1];
+  double* partial[kFluxGroups];
+  int nb[kFluxGroups], clear[kFluxGroups];
+  long long pstride[kFluxGroups];
+  Clock* clk[kFluxGroups];
+  double nmax[kFluxGroups];
+};
+__global__ void __launch_bounds__(1024) k_finalize_dt_groups(FinalizeGroups f) {
+  __shared__ double red[16];
+  const int b = (int)blockIdx.x;
+  int g = 0;
+  while (g + 1 < f.ng && b >= f.first[g + 1]) ++g;
+  const int i = b - f.first[g];
+  finalize_block(f.partial[g] + i * f.pstride[g], f.nb[g], f.clk[g] + i, f.nmax[g], 0.0, f.clear[g], red);
 }
 
 // Energy record of the step-start state: the per-block energy partials of
@@ -2268,9 +2292,7 @@ __global__ void k_fixed_dt(Clock* clk, int count) {
 }
 
 // t += dt after the step; stop once t >= endTime (FieldMover.py:30-36)
-__global__ void k_advance(Clock* clk, int count) {
-  if ((int)threadIdx.x >= count) return;
-  clk += threadIdx.x;
+__device__ __forceinline__ void advance_one(Clock* clk) {
   if (clk->done) return;
   const long long li = clk->nsteps - clk->log_base;
   if (clk->dt_log && li >= 0 && li < clk->log_cap) clk->dt_log[li] = clk->dt;
@@ -2279,6 +2301,23 @@ __global__ void k_advance(Clock* clk, int count) {
   clk->t += clk->dt;
   clk->nsteps += 1;
   if (clk->t >= clk->t_end) clk->done = 1;
+}
+
+__global__ void k_advance(Clock* clk, int count) {
+  if ((int)threadIdx.x < count) advance_one(clk + threadIdx.x);
+}
+
+struct ClockGroups {
+  int ng;
+  int first[kFluxGroups + 1];
+  Clock* clk[kFluxGroups];
+};
+__global__ void k_advance_groups(ClockGroups c) {
+  const int t = (int)threadIdx.x;
+  if (t >= c.first[c.ng]) return;
+  int g = 0;
+  while (g + 1 < c.ng && t >= c.first[g + 1]) ++g;
+  advance_one(c.clk[g] + (t - c.first[g]));
 }
 
 hipError_t launch_finalize_dt(const double* partial, int nb, Clock* clk, double nmax, hipStream_t st, int count,
@@ -2364,6 +2403,47 @@ hipError_t launch_advance(Clock* clk, hipStream_t st, int count) {
   if (count < 1 || count > 1024) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(count), 0, st, clk, count);
   return hipGetLastError();
+}
+
+hipError_t launch_finalize_dt_groups(int ng, double* const* partial, const int* nb, const long long* pstride,
+                                     Clock* const* clk, const double* nmax, const int* count, const int* clear,
+                                     hipStream_t st) {
+  for (int g0 = 0; g0 < ng; g0 += kFluxGroups) {
+    FinalizeGroups f{};
+    f.ng = ng - g0 < kFluxGroups ? ng - g0 : kFluxGroups;
+    for (int k = 0; k < f.ng; ++k) {
+      const int g = g0 + k;
+      if (count[g] < 1 || !partial[g] || !clk[g] || (clear[g] && nb[g] != 1)) return hipErrorInvalidValue;
+      f.first[k + 1] = f.first[k] + count[g];
+      f.partial[k] = partial[g];
+      f.nb[k] = nb[g];
+      f.clear[k] = clear[g];
+      f.pstride[k] = pstride[g];
+      f.clk[k] = clk[g];
+      f.nmax[k] = nmax[g];
+    }
+    hipLaunchKernelGGL(k_finalize_dt_groups, dim3(f.first[f.ng]), dim3(1024), 0, st, f);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_advance_groups(int ng, Clock* const* clk, const int* count, hipStream_t st) {
+  for (int g0 = 0; g0 < ng; g0 += kFluxGroups) {
+    ClockGroups c{};
+    c.ng = ng - g0 < kFluxGroups ? ng - g0 : kFluxGroups;
+    for (int k = 0; k < c.ng; ++k) {
+      if (count[g0 + k] < 1 || !clk[g0 + k]) return hipErrorInvalidValue;
+      c.first[k + 1] = c.first[k] + count[g0 + k];
+      c.clk[k] = clk[g0 + k];
+    }
+    if (c.first[c.ng] > 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_advance_groups, dim3(1), dim3(c.first[c.ng]), 0, st, c);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 // Elastic energy density e = 1/2 S_ijkm sigma_ij sigma_km

@@ -179,6 +179,12 @@ int flux_nblocks(int d, int n);
 hipError_t launch_finalize_dt(const double* partial, int nblocks, Clock* clk, double nmax, hipStream_t st,
                               int count = 1, long long pstride = 0, double dt_bound = 0.0, int clear = 0);
 hipError_t launch_advance(Clock* clk, hipStream_t st, int count = 1);
+// the same for ng instance groups in one launch per ten groups (group g:
+// count[g] clocks at clk[g], speed partials at partial[g] + i pstride[g])
+hipError_t launch_finalize_dt_groups(int ng, double* const* partial, const int* nb, const long long* pstride,
+                                     Clock* const* clk, const double* nmax, const int* count, const int* clear,
+                                     hipStream_t st);
+hipError_t launch_advance_groups(int ng, Clock* const* clk, const int* count, hipStream_t st);
 // forward Euler update u += dt L with the clock's dt (skipped once done)
 hipError_t launch_euler(double* u, const double* L, long long n, const Clock* clk, hipStream_t st);
 // the vacancy component's Euler step: c += vbeta dt tr(L)
