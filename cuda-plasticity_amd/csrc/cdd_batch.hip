@@ -217,8 +217,10 @@ static hipError_t multi_finalize(cdd_batch* b, hipStream_t st) {
                                    nmax.data(), count.data(), clear.data(), st);
 }
 
-// One multi-instance flux launch of stage s over every group in_multi.
-static hipError_t multi_flux(cdd_batch* b, int phase, int s, hipStream_t st) {
+// One multi-instance flux launch of stage s over every group in_multi
+// (only == -1), or over the groups other than group `only` (skip_only), or
+// over group `only` alone.
+static hipError_t multi_flux(cdd_batch* b, int phase, int s, hipStream_t st, int only = -1, bool skip_only = false) {
   FluxMultiJob m;
   memset(&m, 0, sizeof m);
   m.dyn = b->cfg.dynamics;
@@ -230,8 +232,11 @@ static hipError_t multi_flux(cdd_batch* b, int phase, int s, hipStream_t st) {
   int k = 0;
   // largest sides first
   std::vector<int> order;
-  for (size_t gi = 0; gi < b->groups.size(); ++gi)
-    if (in_multi(b, b->groups[gi])) order.push_back((int)gi);
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    if (!in_multi(b, b->groups[gi])) continue;
+    if (only >= 0 && (((int)gi == only) == skip_only)) continue;
+    order.push_back((int)gi);
+  }
   if (order.empty()) return hipSuccess;
   std::sort(order.begin(), order.end(), [&](int x, int y) { return b->groups[x].n > b->groups[y].n; });
   for (int gi : order) {
@@ -260,13 +265,28 @@ static hipError_t multi_flux(cdd_batch* b, int phase, int s, hipStream_t st) {
 // Capture one step of every group into graph[phase]: per stage, fork one
 // capture stream per group from the batch stream for the spectral passes
 // (and the per-size flux of groups outside the multi-instance launch), join
-// them back, then the multi-instance flux on the batch stream.
+// them back, then the multi-instance flux on the batch stream.  The group of
+// the largest side, whose stress solve ends last (its 2D contraction holds
+// one block per CU), gets its own multi-instance flux launch on its stream
+// right after that solve, and the other groups' launch runs as soon as
+// theirs are done, beside the large group's last spectral passes (1.5-2%
+// per batched step at sides 256-1024; one launch per group measured the same).
 static int capture(cdd_batch* b, int phase) {
   // capture on the private stream (the caller's stream may be the null
   // stream, which cannot be captured); the graph is launched on the caller's
   hipStream_t main = b->own;
   BCHK(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
   const int ng = (int)b->groups.size();
+  int late = -1, nmulti = 0;
+  for (int gi = 0; gi < ng; ++gi) {
+    const Group& g = b->groups[gi];
+    if (!in_multi(b, g)) continue;
+    ++nmulti;
+    if (late < 0 || g.n > b->groups[late].n ||
+        (g.n == b->groups[late].n && g.count > b->groups[late].count))
+      late = gi;
+  }
+  if (nmulti < 2) late = -1;
   hipError_t err = hipSuccess;
   for (int s = 0; s < num_stages(b) && err == hipSuccess; ++s) {
     hipEvent_t fork = b->ev[s * (1 + ng)];
@@ -281,11 +301,19 @@ static int capture(cdd_batch* b, int phase) {
         if (err == hipSuccess && s == 0)
           err = launch_finalize_dt(g.partial, g.nblocks, g.clk, (double)g.n, g.st, g.count, g.nblocks);
       }
+      if (err == hipSuccess && gi == late) continue;  // joined after its own flux below
       hipEvent_t join = b->ev[s * (1 + ng) + 1 + gi];
       if (err == hipSuccess) err = hipEventRecord(join, g.st);
       if (err == hipSuccess) err = hipStreamWaitEvent(main, join, 0);
     }
-    if (err == hipSuccess) err = multi_flux(b, phase, s, main);
+    if (err == hipSuccess) err = multi_flux(b, phase, s, main, late, true);
+    if (err == hipSuccess && late >= 0) {
+      Group& g = b->groups[late];
+      err = multi_flux(b, phase, s, g.st, late, false);
+      hipEvent_t join = b->ev[s * (1 + ng) + 1 + late];
+      if (err == hipSuccess) err = hipEventRecord(join, g.st);
+      if (err == hipSuccess) err = hipStreamWaitEvent(main, join, 0);
+    }
     if (s == 0 && err == hipSuccess) err = multi_finalize(b, main);
   }
   if (err == hipSuccess) {
