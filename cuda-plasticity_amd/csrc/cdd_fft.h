@@ -51,10 +51,15 @@ struct FFTPlan<256> {
   static constexpr int NS = 2, T = 16, V = 16;
   static constexpr int R[2] = {16, 16};
 };
+// 384 and 768 keep 12 values per thread (radix 3, then radix 4 and 2): the
+// 2D contraction then fits 256 registers, two waves per SIMD (24 values per
+// thread held one 3-wave block per CU; single 384^2 / 768^2 plans step 3%
+// faster).  1024 stays at 16 values: 8 values per thread on 128 threads
+// spilled in the contraction and measured slower.
 template <>
 struct FFTPlan<384> {
-  static constexpr int NS = 4, T = 16, V = 24;
-  static constexpr int R[4] = {3, 8, 4, 4};
+  static constexpr int NS = 5, T = 32, V = 12;
+  static constexpr int R[5] = {3, 4, 4, 4, 2};
 };
 template <>
 struct FFTPlan<512> {
@@ -63,8 +68,8 @@ struct FFTPlan<512> {
 };
 template <>
 struct FFTPlan<768> {
-  static constexpr int NS = 4, T = 32, V = 24;
-  static constexpr int R[4] = {3, 4, 8, 8};
+  static constexpr int NS = 5, T = 64, V = 12;
+  static constexpr int R[5] = {3, 4, 4, 4, 4};
 };
 template <>
 struct FFTPlan<1024> {
