@@ -37,7 +37,8 @@ def _plan(amd, d, n, dyn=0, Lambda=0.0, rk=3, **kw):
     return amd.Plan(d, n, dynamics=dyn, rk_order=rk, Lambda=Lambda, mu=MU, nu=NU, cfl=kw.pop("cfl", 0.25), **kw)
 
 
-@pytest.mark.parametrize("d,n", [(2, 8), (2, 16), (2, 64), (2, 384), (3, 8), (3, 16), (3, 32)])
+@pytest.mark.parametrize("d,n", [(2, 8), (2, 16), (2, 64), (2, 384), (2, 512), (2, 768), (2, 1024), (3, 8), (3, 16),
+                                 (3, 32)])
 def test_sigma_matches_oracle(amd, d, n):
     say("sigma", d, n)
     beta = SpectralGaussianField((n,) * d, 2.0, 0.01, seed=11)
@@ -47,7 +48,7 @@ def test_sigma_matches_oracle(amd, d, n):
     assert rel_l2(got, ref) < 1e-13
 
 
-@pytest.mark.parametrize("d,n", [(2, 16), (2, 128), (3, 16)])
+@pytest.mark.parametrize("d,n", [(2, 16), (2, 128), (2, 384), (2, 768), (3, 16)])
 def test_rho_matches_oracle(amd, d, n):
     say("rho", d, n)
     beta = SpectralGaussianField((n,) * d, 2.0, 0.01, seed=12)
