@@ -1030,9 +1030,13 @@ __global__ void __launch_bounds__(TX* TY) k_flux_vjp_cell(VjpJob job) {
       for (int s = 0; s < 2; ++s)
         for (int c = 0; c < 9; ++c) job.ubar[((a * 2 + s) * 9 + c) * cells + ii] = N * ub[a][s][c];
   }
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j)
-      st_pol<STRM>(&job.sbar[(3 * i + j) * cells + ii], T(0.5) * (sgb[3 * i + j] + sgb[3 * j + i]));
+  // the symmetrised stress cotangent as six planes, xx yy zz xy xz yz
+  {
+    constexpr int SI[6] = {0, 1, 2, 0, 0, 1}, SJ[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+      st_pol<STRM>(&job.sbar[q * cells + ii], T(0.5) * (sgb[3 * SI[q] + SJ[q]] + sgb[3 * SJ[q] + SI[q]]));
+  }
   const int nw = (int)(blockDim.x * blockDim.y + 63) / 64;
   for (int off = 32; off > 0; off >>= 1) lamb += __shfl_xor(lamb, off, 64);
   if ((tid & 63) == 0) red[tid >> 6] = lamb;
@@ -1407,13 +1411,13 @@ __global__ void __launch_bounds__(256) k_flux_vjp_gather(GatherJob g) {
       acc[c] += Dbm1 - Db0;
     }
   }
-  T s[9];
-  for (int c = 0; c < 9; ++c) s[c] = ld_pol<STRM>(&g.sbar[c * cells + idx]);
-  const T trs = s[0] + s[4] + s[8];
+  T s[6];  // sbar: six planes, xx yy zz xy xz yz
+  for (int q = 0; q < 6; ++q) s[q] = ld_pol<STRM>(&g.sbar[q * cells + idx]);
+  const T trs = s[0] + s[1] + s[2];
   const int SC[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
   for (int c = 0; c < 9; ++c) {
     const int i = c / 3, j = c % 3;
-    T v = acc[c] - T(2) * T(g.mu) * s[c] - (i == j ? T(g.lam) * trs : T(0));
+    T v = acc[c] - T(2) * T(g.mu) * s[SC[c]] - (i == j ? T(g.lam) * trs : T(0));
     if (g.ebar) v += ld_pol<STRM>(&g.ebar[SC[c] * cells + idx]);
     for (int k = 0; k < g.nterm; ++k) v += T(g.coef[k]) * ld_pol<STRM>(&g.term[k][c * cells + idx]);
     if (g.csrc) v += g.csrc[c];

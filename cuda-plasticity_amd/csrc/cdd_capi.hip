@@ -978,6 +978,7 @@ static int flux_vjp_impl(cdd_plan* p, const double* beta, double t, const double
   if (j.upart) HIPCHK(launch_reduce_sum(p->upart, nb, p->scalar + 4, p->stream, 0));
   SpectralJob jt = spectral_job(p, 6, p->sbar, nullptr, nullptr, t, p->G);
   jt.trans = 1;
+  jt.src_sym6 = 1;
   HIPCHK(launch_spectral(jt));
   GatherJob g;
   memset(&g, 0, sizeof g);
@@ -1193,6 +1194,7 @@ static int stage_vjp(cdd_plan* p, const double* beta, const double* G, const dou
   if (j.upart) HIPCHK(launch_reduce_sum(p->upart, nb, p->scalar + 4, p->stream, 0));
   SpectralJob jt = spectral_job(p, 6, p->sbar, nullptr, nullptr, 0.0, p->G);
   jt.trans = 1;
+  jt.src_sym6 = 1;
   HIPCHK(launch_spectral(jt));
   GatherJob g;
   memset(&g, 0, sizeof g);

@@ -70,6 +70,10 @@ struct SpectralJob {
   // R2C / C2R row tile: 0 the default rows per block, 1 half of them (2D
   // fp64 stress solves of sides >= 256; otherwise the default)
   int stile;
+  // nc = 6: the source holds the six symmetric components (xx yy zz xy xz
+  // yz) as planes, so the R2C reads them as they are (the flux adjoint's
+  // stress cotangent); 0: it symmetrises a nine-component source
+  int src_sym6;
   // working set that picks the passes' cache policy (streaming above
   // kStreamBytes); 0: this launch's spectrum.  Launches that run beside
   // others (the batch's size groups) give the spectrum of all of them.
@@ -218,7 +222,7 @@ struct VjpJob {
   const double* dtp;   // optional device dt (the factor is 1 when null)
   double* ubar;        // out: 2 d 9 derivative-cotangent planes, scaled by n
   unsigned long long* codes;  // out: per-cell minmod selections (2 bits per axis and component)
-  double* sbar;        // out: symmetrised stress cotangent (9 planes)
+  double* sbar;        // out: symmetrised stress cotangent (6 planes, xx yy zz xy xz yz)
   double* lpart;       // out: per-block Lambda cotangent
   double* upart;       // out (optional): per-block sum of lam tr(sbar) + 2 mu sbar_zz,
                        // the cotangent of the uniaxial control's betaE_zz
@@ -231,7 +235,7 @@ struct GatherJob {
   long long cells;
   const unsigned long long* codes;
   const double* ubar;
-  const double* sbar;
+  const double* sbar;  // 6 planes, xx yy zz xy xz yz
   const double* ebar;
   double mu, lam;
   int nterm;

@@ -767,12 +767,12 @@ static hipError_t launch_spectral_tp(const SpectralJob& job) {
   // solves of sides >= 256; twice the blocks, half the LDS each)
   constexpr int TYH = TL::TY >= 2 ? TL::TY / 2 : 1;
   constexpr bool HALF_OK = N >= 256 && sizeof(T) == 8 && TL::TY >= 2;
-  const bool half = HALF_OK && job.stile == 1 && d == 2 && NC == 6;
+  const bool half = HALF_OK && job.stile == 1 && d == 2 && NC == 6 && !job.src_sym6;
   dim3 g1h(N / TYH, count, npairs), b1h(TYH * TL::TT);
   if (half) {
     if constexpr (HALF_OK)
       hipLaunchKernelGGL((k_r2c_last<T, N, true, TYH, STRM>), g1h, b1h, 0, st, src, spec, tw, NO, NC, job.sstride);
-  } else if (NC == 6) {
+  } else if (NC == 6 && !job.src_sym6) {
     hipLaunchKernelGGL((k_r2c_last<T, N, true, TL::TY, STRM>), g1, b1, 0, st, src, spec, tw, NO, NC, job.sstride);
   } else {
     hipLaunchKernelGGL((k_r2c_last<T, N, false, TL::TY, STRM>), g1, b1, 0, st, src, spec, tw, NO, NC, job.sstride);
