@@ -285,6 +285,15 @@ def batched_vs_serial(K, W, count=48):
     step_bytes = sum(n * n * canonical_bytes_per_cell_update(n, d=2, order=2) for n in sizes)
     flux_bytes = cells * spectral_bytes(512, d=2)["flux_stage1"]
     flux_gbs = flux_bytes / (flux_ms * 1e-3) / 1e9
+    # calibrated counter bytes of the same launch (rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE of this workload), when the row march is the chosen form
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_batched.json")
+    if tile == 3 and count == 48 and os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("k_flux_rows_stage1")
+        except Exception:
+            traffic = None
     return {"value": ts / tb, "unit": "x (serial time / batched time)", "steps": K, "warmup": W,
             "ms_per_step": 1e3 * tb / K, "serial_ms_per_step": 1e3 * ts / K,
             "cell_updates_per_s_batched": cells * K / tb, "cell_updates_per_s_serial": cells * K / ts,
@@ -293,7 +302,7 @@ def batched_vs_serial(K, W, count=48):
             "roofline": {"bound": "hbm", "kernel": "%s (first stage, all 48 instances in one launch)" % (
                 "k_flux_rows" if tile == 3 else "k_flux_multi"),
                          "achieved": flux_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": flux_gbs / PEAK_HBM_GBS,
-                         "traffic": None, "launch_ms": flux_ms, "algorithmic_bytes": flux_bytes},
+                         "traffic": traffic, "launch_ms": flux_ms, "algorithmic_bytes": flux_bytes},
             "sizes": sizes, "cells": cells, "flux_tiles": {str(k): v for k, v in choices.items()},
             "all_steps_ran": all(s == K + W for s in steps_b)}
 
